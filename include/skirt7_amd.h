@@ -1,0 +1,160 @@
+/* skirt7_amd.h -- C-ABI of the MI355X photon-packet transport library.
+ *
+ * Drop-in boundary for SKIRT7's Monte Carlo photon-packet transport loop. Every entry point
+ * replaces one reference interface (file:line relative to the SKIRT7 source tree) with a
+ * batch-oriented call: a per-packet round trip to the GPU would cost more than the work.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative status on failure; sg_last_error()
+ *    returns a thread-local message. No C++ exception crosses this boundary (the reference
+ *    throws FatalError, e.g. DustSystem.cpp:953-955; the host wrapper rethrows its own).
+ *  - Pointers are host pointers; sizes are element counts. Positions in the caller's length
+ *    unit, luminosities in the caller's power unit, angles in radians.
+ *  - Objects are immutable after creation except for their tallies; one host thread drives
+ *    one device stream (the reference's Parallel threads become device lanes).
+ *  - precision: SG_FP64 reproduces the reference arithmetic (cell-index sequences bit-exact);
+ *    SG_MIXED runs positions, directions, optical depths and weights in FP32, with FP64
+ *    wall-distance recompute near ties and FP64 tallies.
+ */
+#ifndef SKIRT7_AMD_H
+#define SKIRT7_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sg_grid sg_grid;            /* DustGrid                        */
+typedef struct sg_dust sg_dust;            /* DustSystem (one dust component) */
+typedef struct sg_stars sg_stars;          /* StellarSystem                   */
+typedef struct sg_instruments sg_instruments;  /* InstrumentSystem of FrameInstruments */
+typedef struct sg_sim sg_sim;              /* MonteCarloSimulation            */
+
+enum { SG_FP64 = 0, SG_MIXED = 1 };
+enum { SG_TOPDOWN = 0, SG_NEIGHBOR = 1 };   /* TreeDustGrid::SearchMethod (TreeDustGrid.hpp:50) */
+
+/* Number of doubles in the parameter records below. */
+#define SG_STAR_NPARAM 16
+#define SG_INSTR_NPARAM 10
+#define SG_MC_NPARAM 8
+#define SG_NCOUNTERS 12
+
+/* ---- library ---------------------------------------------------------------------------- */
+const char* sg_last_error(void);
+/* Selects the device and checks it is a gfx950; every other call implies sg_init(0). */
+int sg_init(int device);
+int sg_device_name(char* buf, int len);
+
+/* ---- DustGrid ----------------------------------------------------------------------------- */
+/* CartesianDustGrid (CartesianDustGrid.cpp:28-43): box extent {xmin,ymin,zmin,xmax,ymax,zmax}
+ * and the Nx+1, Ny+1, Nz+1 cell borders; cell index m = k + Nz*j + Nz*Ny*i (:326-329). */
+int sg_grid_create_cartesian(const double ext[6], const double* xv, int nxv, const double* yv, int nyv,
+                             const double* zv, int nzv, sg_grid** out);
+/* OctTreeDustGrid (arity 8) or BinTreeDustGrid with alternating directions (arity 2)
+ * (TreeDustGrid.cpp:49-162): one subdivision flag per node in the reference's breadth-first
+ * node numbering; cell numbers are the leaves' ranks in that order (:110-121). */
+int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivided, int64_t nnodes, int search,
+                        sg_grid** out);
+/* VoronoiDustGrid / VoronoiMesh (VoronoiMesh.cpp:297-405): generator sites (x,y,z per site)
+ * and per-site neighbour lists in CSR form; walls are the ids -1..-6 (xmin,xmax,ymin,ymax,zmin,zmax). */
+int sg_grid_create_voronoi(const double ext[6], const double* sites, int64_t nsites, const int64_t* nb_offsets,
+                           const int32_t* nb_ids, sg_grid** out);
+int sg_grid_destroy(sg_grid* g);
+int sg_grid_num_cells(const sg_grid* g, int64_t* ncells);
+/* DustGrid::whichcell (DustGrid.hpp:89) for n positions (x,y,z); -1 outside. */
+int sg_grid_whichcell(sg_grid* g, int64_t n, const double* r, int32_t* m);
+/* DustGrid::path (DustGrid.hpp:106) for n rays (rx,ry,rz,kx,ky,kz): each ray's (m, ds)
+ * segment list exactly as DustGridPath holds it, including the -1 entry segments. Ray i writes
+ * its segments at [i*max_seg, ...); counts[i] = number of segments, or -1 if more than max_seg. */
+int sg_grid_paths(sg_grid* g, int64_t n, const double* rk, int max_seg, int precision, int32_t* counts,
+                  int32_t* m, double* ds);
+
+/* ---- DustSystem ----------------------------------------------------------------------------- */
+/* Dust density per cell (DustSystem::_rhovv, DustSystem.hpp:412) and the dust mix per
+ * wavelength: kappaext, albedo and asymmetry parameter as DustMix derives them
+ * (DustMix.cpp:44-91, SimpleOligoDustMix.cpp:22-51). store_absorption mirrors
+ * OligoDustSystem::storeabsorptionrates (OligoDustSystem.cpp:61-64). */
+int sg_dust_create(sg_grid* g, const double* rho, int64_t ncells, int nlambda, const double* kappaext,
+                   const double* albedo, const double* asymmpar, int store_absorption, sg_dust** out);
+int sg_dust_destroy(sg_dust* d);
+/* DustSystem::opticaldepth (DustSystem.cpp:960-976) along n rays at wavelength ell. */
+int sg_dust_optical_depths(sg_dust* d, int ell, int64_t n, const double* rk, int precision, double* tau);
+/* DustGridPath::pathlength after fillOpticalDepth (DustGridPath.cpp:162-173): for ray i and
+ * optical depth taus[i], the path length s[i]. */
+int sg_dust_path_lengths(sg_dust* d, int ell, int64_t n, const double* rk, const double* taus, int precision,
+                         double* s);
+/* OligoDustSystem::Labs (OligoDustSystem.cpp:82-85): the absorbed-luminosity table,
+ * ncells x nlambda doubles in the reference's (m, ell) row-major order. */
+int sg_dust_read_absorption(sg_dust* d, double* labs);
+int sg_dust_clear_absorption(sg_dust* d);
+
+/* ---- StellarSystem ------------------------------------------------------------------------ */
+/* Stellar components (StellarSystem.cpp:116-158, GeometricStellarComp.cpp:75-81): params is
+ * ncomp x SG_STAR_NPARAM doubles
+ *   [0] geometry: 1 exponential disc (ExpDiskGeometry.cpp:117-146), 2 Sersic (SersicGeometry.cpp:85-90)
+ *   [1..5] disc: hR, hz, Rmax, zmax, Rmin | Sersic: n, reff
+ *   [6] spiral decorator flag, [7..12] arms, pitch, R0, phi0, weight, index
+ *       (SpiralStructureGeometryDecorator.cpp:23-40,176-229)
+ * lum is ncomp x nlambda luminosities; sersic tables (101 radii and normalised masses per
+ * Sersic component, SersicFunction.cpp:15-68) are concatenated in component order. */
+int sg_stars_create(int ncomp, const double* params, int nlambda, const double* lum, const double* sersic_s,
+                    const double* sersic_M, int ntable, double emission_bias, sg_stars** out);
+int sg_stars_destroy(sg_stars* s);
+/* SersicFunction table helper (SersicFunction.cpp:15-68): 101 radii s and masses M. */
+int sg_sersic_table(double n, double* s101, double* M101);
+
+/* ---- InstrumentSystem (FrameInstrument) ------------------------------------------------------ */
+/* n x SG_INSTR_NPARAM: distance, inclination, azimuth, position angle, Nx, Ny, fovx, fovy, xc, yc
+ * (DistantInstrument.cpp:27-50, SingleFrameInstrument.cpp:26-42). */
+int sg_instruments_create(int n, const double* params, int nlambda, sg_instruments** out);
+int sg_instruments_destroy(sg_instruments* in);
+/* FrameInstrument::_distftotv (FrameInstrument.cpp:33-47): instrument i's raw detected
+ * luminosities, nlambda x Ny x Nx doubles ([ell][j][i] = reference's ell*Nframep + i + Nx*j). */
+int sg_instruments_read(sg_instruments* in, int i, double* frame);
+int sg_instruments_clear(sg_instruments* in);
+
+/* ---- MonteCarloSimulation ----------------------------------------------------------------- */
+/* mc is SG_MC_NPARAM doubles: packages, minWeightReduction, minScattEvents, scattBias, seed,
+ * reserved, member, tau_cut (MonteCarloSimulation.hpp:37-66). tau_cut > 0 ends a peel-off walk
+ * once its optical depth exceeds tau_cut (its contribution is below exp(-tau_cut) of L); 0 keeps
+ * the reference's full walk. dust may be null (no dust system). */
+int sg_sim_create(sg_dust* dust, sg_stars* stars, sg_instruments* instr, const double* mc, sg_sim** out);
+int sg_sim_destroy(sg_sim* s);
+/* MonteCarloSimulation::dostellaremissionchunk (MonteCarloSimulation.cpp:259-297) for packets
+ * [first, first+count) at wavelength ell, each with luminosity L(ell)/npp. Packet p draws its
+ * random numbers from its own Philox4x32-10 stream keyed (seed, member) and counted by
+ * (p, draw, ell), so results do not depend on scheduling. Accumulates into the instrument and
+ * absorption tallies; counters (may be null) receive SG_NCOUNTERS values:
+ *   [0] packets [1] own-path crossings [2] peel-off crossings [3] scattering events
+ *   [4] peel-offs detected [5] stuck-path escapes [6] npp [7] walks [8] re-walk crossings
+ *   [9] FP64 wall recomputes [10] kernel microseconds [11] launches */
+int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, int precision,
+               uint64_t* counters);
+/* Batched population (one persistent launch over many members, each its own simulation on a
+ * grid of the same kind): member i runs packets [0, counts[i]) at wavelength ells[i]. */
+int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const uint64_t* counts, int precision,
+                     uint64_t* counters);
+/* Launch tuning used by sg_sim_run (0 = keep current): block size, waves per CU, compaction
+ * threshold (lanes that must finish their walk before the wave re-arms them). */
+int sg_set_tuning(int block, int waves_per_cu, int compact_lanes);
+
+/* ---- setup helpers (the reference's own setup phase, made reproducible) ---------------------- */
+/* Normalised density of an exponential-disc geometry record laid out like a stellar component
+ * (ExpDiskGeometry.cpp:25-40,103-114; spiral: SpiralStructureGeometryDecorator.cpp:165-172). */
+int sg_geometry_density(const double* geom16, int64_t n, const double* xyz, double* out);
+/* TreeDustGrid::setupSelfBefore/subdivide (TreeDustGrid.cpp:49-230), mass-fraction criterion,
+ * with node masses from nsamples counter-based density samples: one flag per node in
+ * breadth-first id order (at most cap), the node count in *nnodes. */
+int sg_tree_subdivide(int arity, const double ext[6], const double* geom16, double scale, double total_mass,
+                      int minlevel, int maxlevel, double maxmassfraction, int nsamples, uint32_t seed,
+                      uint8_t* flags, int64_t cap, int64_t* nnodes);
+/* DustSystem::setSampleDensityBody (DustSystem.cpp:145-172): mean of nsamples densities per cell
+ * (Cartesian and tree grids), times scale. */
+int sg_grid_sample_density(sg_grid* g, const double* geom16, double scale, int nsamples, uint32_t seed,
+                           double* rho_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

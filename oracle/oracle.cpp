@@ -1,0 +1,1581 @@
+// oracle.cpp -- CPU restatement of SKIRT7's Monte Carlo photon-packet transport loop.
+//
+// TEST INFRASTRUCTURE ONLY: this is the checker for the HIP implementation in skirt7_amd/.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the product
+// never does. Every routine cites the SKIRT7 file:line it restates (paths relative to the
+// SKIRT7 source tree). Parity status of each routine is listed at the end of this header.
+//
+// Deliberate scope: one dust component (the Ncomp==1 branches of MonteCarloSimulation.cpp),
+// unpolarised Henyey-Greenstein scattering, distant frame instruments, isotropic emission
+// from exponential-disc / Sersic geometries with the optional spiral perturbation.
+//
+// Parity pins (tests/test_oracle_*.py):
+//   - NR locate/locate_clip/locate_fail/lingrid/interpolate: against the reference's own
+//     Fundamentals/NR.hpp compiled into oracle/_ref (recipe oracle/ref/Makefile);
+//   - MT19937 with 69069 seeding: against numpy's MT19937 with the injected state;
+//   - Philox4x32-10: against the Random123 known-answer vectors;
+//   - grid walks: closed-form rays, brute-force midpoint classification, chord-length sums;
+//   - HG sampling/value, exponcutoff, LambertW_{-1}, Sersic table: closed forms;
+//   - escape/absorption: energy conservation identity.
+//   Parity unpinned: none of the reference's golden outputs exist (the tree ships no tests).
+
+#include "oracle.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+// ===================================================================== NR (Fundamentals/NR.hpp)
+
+// NR.hpp:103-116 (locate_basic_impl)
+inline int nr_basic(const double* xv, double x, int n)
+{
+    int jl = -1, ju = n;
+    while (ju - jl > 1) {
+        int jm = (ju + jl) >> 1;
+        if (x < xv[jm]) ju = jm;
+        else jl = jm;
+    }
+    return jl;
+}
+// NR.hpp:139-144
+inline int nr_locate_clip(const double* xv, int size, double x)
+{
+    if (x < xv[0]) return 0;
+    return nr_basic(xv, x, size - 1);
+}
+// NR.hpp:148-153
+inline int nr_locate_fail(const double* xv, int size, double x)
+{
+    if (x > xv[size - 1]) return -1;
+    return nr_basic(xv, x, size - 1);
+}
+// NR.hpp:296-299
+inline double nr_linlin(double x, double x1, double x2, double f1, double f2)
+{
+    return f1 + ((x - x1) / (x2 - x1)) * (f2 - f1);
+}
+// NR.hpp:321-345
+inline double nr_loglog(double x, double x1, double x2, double f1, double f2)
+{
+    x = std::log10(x);
+    x1 = std::log10(x1);
+    x2 = std::log10(x2);
+    bool logf = f1 > 0 && f2 > 0;
+    if (logf) {
+        f1 = std::log10(f1);
+        f2 = std::log10(f2);
+    }
+    double fx = f1 + ((x - x1) / (x2 - x1)) * (f2 - f1);
+    if (logf) fx = std::pow(10, fx);
+    return fx;
+}
+
+// ===================================================================== DustGridPath
+
+struct Segment { int m; double ds, s, dtau, tau; };
+
+// DustGridPath.{hpp,cpp}: initial position/direction plus the segment list.
+struct Path {
+    double r[3] = {0, 0, 0};
+    double k[3] = {0, 0, 1};
+    double s = 0;
+    std::vector<Segment> v;
+
+    Path() { v.reserve(1000); }                               // DustGridPath.cpp:16-33
+    void clear() { s = 0; v.clear(); }                        // DustGridPath.cpp:38-42
+    void addSegment(int m, double ds)                         // DustGridPath.cpp:46-53
+    {
+        if (ds > 0) {
+            s += ds;
+            v.push_back(Segment{m, ds, s, 0, 0});
+        }
+    }
+    // DustGridPath.cpp:57-150; returns false for the OUTSIDE sentinel (infinite position)
+    void moveInside(const double b[6], double eps, double out[3])
+    {
+        const double inf = std::numeric_limits<double>::infinity();
+        double kx = k[0], ky = k[1], kz = k[2];
+        double rx = r[0], ry = r[1], rz = r[2];
+        auto outside = [&]() { out[0] = out[1] = out[2] = inf; };
+        if (rx <= b[0]) {
+            if (kx <= 0.0) return outside();
+            double ds = (b[0] - rx) / kx;
+            addSegment(-1, ds);
+            rx = b[0] + eps; ry += ky * ds; rz += kz * ds;
+        } else if (rx >= b[3]) {
+            if (kx >= 0.0) return outside();
+            double ds = (b[3] - rx) / kx;
+            addSegment(-1, ds);
+            rx = b[3] - eps; ry += ky * ds; rz += kz * ds;
+        }
+        if (ry <= b[1]) {
+            if (ky <= 0.0) return outside();
+            double ds = (b[1] - ry) / ky;
+            addSegment(-1, ds);
+            rx += kx * ds; ry = b[1] + eps; rz += kz * ds;
+        } else if (ry >= b[4]) {
+            if (ky >= 0.0) return outside();
+            double ds = (b[4] - ry) / ky;
+            addSegment(-1, ds);
+            rx += kx * ds; ry = b[4] - eps; rz += kz * ds;
+        }
+        if (rz <= b[2]) {
+            if (kz <= 0.0) return outside();
+            double ds = (b[2] - rz) / kz;
+            addSegment(-1, ds);
+            rx += kx * ds; ry += ky * ds; rz = b[2] + eps;
+        } else if (rz >= b[5]) {
+            if (kz >= 0.0) return outside();
+            double ds = (b[5] - rz) / kz;
+            addSegment(-1, ds);
+            rx += kx * ds; ry += ky * ds; rz = b[5] - eps;
+        }
+        out[0] = rx; out[1] = ry; out[2] = rz;
+    }
+    template <class F> void fillOpticalDepth(F kr)            // DustGridPath.hpp:117-129
+    {
+        double tau = 0;
+        for (auto& sg : v) {
+            double dtau = kr(sg.m) * sg.ds;
+            tau += dtau;
+            sg.dtau = dtau;
+            sg.tau = tau;
+        }
+    }
+    template <class F> double opticalDepth(F kr, double distance = DBL_MAX) const   // DustGridPath.hpp:97-108
+    {
+        double tau = 0;
+        for (const auto& sg : v) {
+            tau += kr(sg.m) * sg.ds;
+            if (sg.s > distance) break;
+        }
+        return tau;
+    }
+    double tau() const { return v.empty() ? 0 : v.back().tau; }   // DustGridPath.cpp:154-158
+    // NR.hpp:76-95 (locate over the segment vector, ordered by tau)
+    int locateTau(double x) const
+    {
+        int n = (int)v.size();
+        if (x < v[0].tau) return -1;
+        if (v[n - 1].tau < x) return n - 1;
+        int jl = -1, ju = n;
+        while (ju - jl > 1) {
+            int jm = (ju + jl) >> 1;
+            if (x < v[jm].tau) ju = jm;
+            else jl = jm;
+        }
+        if (jl <= 0) return 0;
+        if (jl >= n - 2) return n - 2;
+        return jl;
+    }
+    double pathlength(double tau) const                        // DustGridPath.cpp:162-173
+    {
+        int N = (int)v.size();
+        if (N > 0 && tau > 0) {
+            int i = locateTau(tau);
+            if (i < 0) return nr_linlin(tau, 0, v[0].tau, 0, v[0].s);
+            if (i < N - 1) return nr_linlin(tau, v[i].tau, v[i + 1].tau, v[i].s, v[i + 1].s);
+            return v[N - 1].s;
+        }
+        return 0;
+    }
+};
+
+struct WalkStats { uint64_t stuck = 0; };
+
+struct Grid {
+    virtual ~Grid() {}
+    virtual void path(Path& p, WalkStats& st) const = 0;
+    virtual int whichcell(const double r[3]) const = 0;
+    virtual int64_t ncells() const = 0;
+};
+
+// ===================================================================== Cartesian grid
+
+// CartesianDustGrid.cpp:28-43,109-118,136-283,326-329
+struct CartesianGrid : Grid {
+    double b[6];
+    std::vector<double> xv, yv, zv;
+    int Nx, Ny, Nz;
+    int index(int i, int j, int k) const { return k + Nz * j + Nz * Ny * i; }
+    int64_t ncells() const override { return (int64_t)Nx * Ny * Nz; }
+    int whichcell(const double r[3]) const override
+    {
+        int i = nr_locate_fail(xv.data(), Nx + 1, r[0]);
+        int j = nr_locate_fail(yv.data(), Ny + 1, r[1]);
+        int k = nr_locate_fail(zv.data(), Nz + 1, r[2]);
+        if (i < 0 || j < 0 || k < 0) return -1;
+        return index(i, j, k);
+    }
+    void path(Path& p, WalkStats&) const override
+    {
+        p.clear();
+        double kx = p.k[0], ky = p.k[1], kz = p.k[2];
+        double x = p.r[0], y = p.r[1], z = p.r[2];
+        double ds, dsx, dsy, dsz;
+        const double xmin = b[0], ymin = b[1], zmin = b[2], xmax = b[3], ymax = b[4], zmax = b[5];
+        if (x < xmin) {
+            if (kx <= 0.0) return p.clear();
+            ds = (xmin - x) / kx;
+            p.addSegment(-1, ds);
+            x = xmin + 1e-8 * (xv[1] - xv[0]);
+            y += ky * ds; z += kz * ds;
+        } else if (x > xmax) {
+            if (kx >= 0.0) return p.clear();
+            ds = (xmax - x) / kx;
+            p.addSegment(-1, ds);
+            x = xmax - 1e-8 * (xv[Nx] - xv[Nx - 1]);
+            y += ky * ds; z += kz * ds;
+        }
+        if (y < ymin) {
+            if (ky <= 0.0) return p.clear();
+            ds = (ymin - y) / ky;
+            p.addSegment(-1, ds);
+            x += kx * ds; y = ymin + 1e-8 * (yv[1] - yv[0]); z += kz * ds;
+        } else if (y > ymax) {
+            if (ky >= 0.0) return p.clear();
+            ds = (ymax - y) / ky;
+            p.addSegment(-1, ds);
+            x += kx * ds; y = ymax - 1e-8 * (yv[Ny] - yv[Ny - 1]); z += kz * ds;
+        }
+        if (z < zmin) {
+            if (kz <= 0.0) return p.clear();
+            ds = (zmin - z) / kz;
+            p.addSegment(-1, ds);
+            x += kx * ds; y += ky * ds; z = zmin + 1e-8 * (zv[1] - zv[0]);
+        } else if (z > zmax) {
+            if (kz >= 0.0) return p.clear();
+            ds = (zmax - z) / kz;
+            p.addSegment(-1, ds);
+            x += kx * ds; y += ky * ds; z = zmax - 1e-8 * (zv[Nz] - zv[Nz - 1]);
+        }
+        if (x < xmin || x > xmax || y < ymin || y > ymax || z < zmin || z > zmax) return p.clear();
+
+        int i = nr_locate_clip(xv.data(), Nx + 1, x);
+        int j = nr_locate_clip(yv.data(), Ny + 1, y);
+        int k = nr_locate_clip(zv.data(), Nz + 1, z);
+        while (true) {
+            int m = index(i, j, k);
+            double xE = (kx < 0.0) ? xv[i] : xv[i + 1];
+            double yE = (ky < 0.0) ? yv[j] : yv[j + 1];
+            double zE = (kz < 0.0) ? zv[k] : zv[k + 1];
+            dsx = (std::fabs(kx) > 1e-15) ? (xE - x) / kx : DBL_MAX;
+            dsy = (std::fabs(ky) > 1e-15) ? (yE - y) / ky : DBL_MAX;
+            dsz = (std::fabs(kz) > 1e-15) ? (zE - z) / kz : DBL_MAX;
+            if (dsx <= dsy && dsx <= dsz) {
+                ds = dsx;
+                p.addSegment(m, ds);
+                i += (kx < 0.0) ? -1 : 1;
+                if (i >= Nx || i < 0) return;
+                x = xE; y += ky * ds; z += kz * ds;
+            } else if (dsy < dsx && dsy <= dsz) {
+                ds = dsy;
+                p.addSegment(m, ds);
+                j += (ky < 0.0) ? -1 : 1;
+                if (j >= Ny || j < 0) return;
+                x += kx * ds; y = yE; z += kz * ds;
+            } else if (dsz < dsx && dsz < dsy) {
+                ds = dsz;
+                p.addSegment(m, ds);
+                k += (kz < 0.0) ? -1 : 1;
+                if (k >= Nz || k < 0) return;
+                x += kx * ds; y += ky * ds; z = zE;
+            }
+        }
+    }
+};
+
+// ===================================================================== Tree grids
+
+enum Wall { BACK = 0, FRONT, LEFT, RIGHT, BOTTOM, TOP };      // TreeNode.hpp:100
+const int kComplement[6] = {FRONT, BACK, RIGHT, LEFT, TOP, BOTTOM};   // TreeNode.cpp:193
+
+struct TNode {
+    double b[6];       // xmin ymin zmin xmax ymax zmax
+    int id, level, father, child0 = -1, dir = -1;
+    std::vector<int> nb[6];
+};
+
+// TreeDustGrid.cpp:49-162 (topology from a frozen subdivision flag per node, BFS id order),
+// TreeNode.cpp, OctTreeNode.cpp:38-189, BinTreeNode.cpp:40-335
+struct TreeGrid : Grid {
+    int arity = 8;
+    int search = 1;
+    double b[6];
+    double eps;
+    std::vector<TNode> nodes;
+    std::vector<int> cellnumber, idv;
+
+    int64_t ncells() const override { return (int64_t)idv.size(); }
+
+    static bool contains(const double* bb, const double* r)   // Box.hpp:108-109
+    {
+        return r[0] >= bb[0] && r[0] <= bb[3] && r[1] >= bb[1] && r[1] <= bb[4] && r[2] >= bb[2] && r[2] <= bb[5];
+    }
+
+    void createChildren(int l)
+    {
+        TNode& nd = nodes[l];
+        int id = (int)nodes.size();
+        nd.child0 = id;
+        double bb[6];
+        std::memcpy(bb, nd.b, sizeof bb);
+        int level = nd.level;
+        if (arity == 8) {
+            // OctTreeNode.cpp:38-49 with the split point at Box::center() (Box.hpp:123-124)
+            double c[3] = {0.5 * (bb[0] + bb[3]), 0.5 * (bb[1] + bb[4]), 0.5 * (bb[2] + bb[5])};
+            for (int q = 0; q < 8; q++) {
+                TNode ch;
+                for (int a = 0; a < 3; a++) {
+                    bool hi = (q >> a) & 1;
+                    ch.b[a] = hi ? c[a] : bb[a];
+                    ch.b[a + 3] = hi ? bb[a + 3] : c[a];
+                }
+                ch.id = id + q; ch.level = level + 1; ch.father = l;
+                nodes.push_back(ch);
+            }
+        } else {
+            // BinTreeNode.cpp:40-77 (alternating direction: level % 3)
+            int dir = level % 3;
+            nodes[l].dir = dir;
+            double c = 0.5 * (bb[dir] + bb[dir + 3]);
+            for (int q = 0; q < 2; q++) {
+                TNode ch;
+                std::memcpy(ch.b, bb, sizeof bb);
+                if (q == 0) ch.b[dir + 3] = c;
+                else ch.b[dir] = c;
+                ch.id = id + q; ch.level = level + 1; ch.father = l;
+                nodes.push_back(ch);
+            }
+        }
+    }
+
+    // TreeNode.cpp:70-80 (whichnode from this node), OctTreeNode.cpp:184-189, BinTreeNode.cpp:326-335
+    int whichnode(int from, const double r[3]) const
+    {
+        if (!contains(nodes[from].b, r)) return -1;
+        int l = from;
+        while (nodes[l].child0 >= 0) {
+            const TNode& nd = nodes[l];
+            const TNode& c0 = nodes[nd.child0];
+            if (arity == 8) {
+                int q = (r[0] < c0.b[3] ? 0 : 1) + (r[1] < c0.b[4] ? 0 : 2) + (r[2] < c0.b[5] ? 0 : 4);
+                l = nd.child0 + q;
+            } else {
+                int d = nd.dir;
+                l = nd.child0 + (r[d] < c0.b[d + 3] ? 0 : 1);
+            }
+        }
+        return l;
+    }
+    int whichcell(const double r[3]) const override
+    {
+        int l = whichnode(0, r);
+        return l < 0 ? -1 : cellnumber[l];
+    }
+    // TreeNode.cpp:84-93
+    int whichnodeWall(int l, int wall, const double r[3]) const
+    {
+        for (int n : nodes[l].nb[wall])
+            if (contains(nodes[n].b, r)) return n;
+        return -1;
+    }
+
+    // ---- neighbour lists (TreeNode.cpp:151-197, OctTreeNode.cpp:53-180, BinTreeNode.cpp:88-322)
+    void makeneighbors(int wall1, int n1, int n2)
+    {
+        nodes[n1].nb[wall1].push_back(n2);
+        nodes[n2].nb[kComplement[wall1]].push_back(n1);
+    }
+    void deleteneighbor(int l, int wall, int n)
+    {
+        auto& v = nodes[l].nb[wall];
+        for (size_t i = 0; i < v.size(); i++)
+            if (v[i] == n) { v.erase(v.begin() + i); break; }
+    }
+    void addneighbors(int l)
+    {
+        if (nodes[l].child0 < 0) return;
+        int c0 = nodes[l].child0;
+        if (arity == 8) {
+            static const int internal[12][3] = {
+                {FRONT, 0, 1}, {RIGHT, 0, 2}, {TOP, 0, 4}, {RIGHT, 1, 3}, {TOP, 1, 5}, {FRONT, 2, 3},
+                {TOP, 2, 6}, {TOP, 3, 7}, {FRONT, 4, 5}, {RIGHT, 4, 6}, {RIGHT, 5, 7}, {FRONT, 6, 7}};
+            for (auto& t : internal) makeneighbors(t[0], c0 + t[1], c0 + t[2]);
+            double c[3] = {nodes[c0].b[3], nodes[c0].b[4], nodes[c0].b[5]};
+            for (int w = 0; w < 6; w++) {
+                int a = w / 2, side = w % 2;
+                int ax[2], n2 = 0;
+                for (int d = 0; d < 3; d++) if (d != a) ax[n2++] = d;
+                const std::vector<int> list = nodes[l].nb[w];
+                for (int nbr : list) {
+                    deleteneighbor(nbr, kComplement[w], l);
+                    for (int kk = 0; kk < 2; kk++)
+                        for (int j = 0; j < 2; j++) {
+                            const double* nbb = nodes[nbr].b;
+                            bool okj = j ? (nbb[ax[0] + 3] >= c[ax[0]]) : (nbb[ax[0]] <= c[ax[0]]);
+                            bool okk = kk ? (nbb[ax[1] + 3] >= c[ax[1]]) : (nbb[ax[1]] <= c[ax[1]]);
+                            if (okj && okk) {
+                                int q = (side << a) | (j << ax[0]) | (kk << ax[1]);
+                                makeneighbors(kComplement[w], nbr, c0 + q);
+                            }
+                        }
+                }
+            }
+        } else {
+            int a = nodes[l].dir;
+            double c = nodes[c0].b[a + 3];
+            makeneighbors(2 * a + 1, c0, c0 + 1);
+            for (int w = 0; w < 6; w++) {
+                const std::vector<int> list = nodes[l].nb[w];
+                for (int nbr : list) {
+                    deleteneighbor(nbr, kComplement[w], l);
+                    if (w / 2 == a) {
+                        makeneighbors(kComplement[w], nbr, c0 + (w % 2));
+                    } else {
+                        if (nodes[nbr].b[a] <= c) makeneighbors(kComplement[w], nbr, c0);
+                        if (nodes[nbr].b[a + 3] >= c) makeneighbors(kComplement[w], nbr, c0 + 1);
+                    }
+                }
+            }
+        }
+    }
+    // TreeNode.cpp:97-148 (LargerOverlap) and :151-158 (sortneighbors)
+    void sortneighbors(int l)
+    {
+        const double* base = nodes[l].b;
+        for (int w = 0; w < 6; w++) {
+            int a = w / 2;
+            int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+            auto overlap = [&](int n) {
+                const double* nb = nodes[n].b;
+                return std::max(std::min(base[u + 3], nb[u + 3]) - std::max(base[u], nb[u]), 0.) *
+                       std::max(std::min(base[v + 3], nb[v + 3]) - std::max(base[v], nb[v]), 0.);
+            };
+            std::sort(nodes[l].nb[w].begin(), nodes[l].nb[w].end(),
+                      [&](int n1, int n2) { return overlap(n1) > overlap(n2); });
+        }
+    }
+
+    bool build(int ar, const double ext[6], const uint8_t* flags, int64_t nnodes, int srch)
+    {
+        arity = ar; search = srch;
+        std::memcpy(b, ext, sizeof b);
+        // TreeDustGrid.cpp:74: eps = 1e-12 * extent().widths().norm()  (Vec.hpp:61-64)
+        double wx = b[3] - b[0], wy = b[4] - b[1], wz = b[5] - b[2];
+        eps = 1e-12 * std::sqrt(wx * wx + wy * wy + wz * wz);
+        nodes.clear();
+        TNode root;
+        std::memcpy(root.b, b, sizeof b);
+        root.id = 0; root.level = 0; root.father = -1;
+        nodes.push_back(root);
+        for (int64_t l = 0; l < (int64_t)nodes.size(); l++) {
+            if (l >= nnodes) { g_err = "subdivision flags shorter than the tree"; return false; }
+            if (flags[l]) createChildren((int)l);
+        }
+        if ((int64_t)nodes.size() != nnodes) { g_err = "subdivision flags longer than the tree"; return false; }
+        // TreeDustGrid.cpp:110-121
+        cellnumber.assign(nodes.size(), -1);
+        idv.clear();
+        for (size_t l = 0; l < nodes.size(); l++)
+            if (nodes[l].child0 < 0) { cellnumber[l] = (int)idv.size(); idv.push_back((int)l); }
+        // TreeDustGrid.cpp:156-161
+        if (search == 1) {
+            for (size_t l = 0; l < nodes.size(); l++) addneighbors((int)l);
+            for (size_t l = 0; l < nodes.size(); l++) sortneighbors((int)l);
+        }
+        if (search == 2 && arity != 8) { g_err = "Bookkeeping requires an octree"; return false; }
+        return true;
+    }
+
+    // TreeDustGrid.cpp:371-643
+    void path(Path& p, WalkStats& st) const override
+    {
+        p.clear();
+        double r[3];
+        p.moveInside(b, eps, r);
+        int node = whichnode(0, r);
+        if (node < 0) return p.clear();
+        double x = r[0], y = r[1], z = r[2];
+        double kx = p.k[0], ky = p.k[1], kz = p.k[2];
+
+        if (search == 0 || search == 1) {
+            while (node >= 0) {
+                const double* nb = nodes[node].b;
+                double xnext = (kx < 0.0) ? nb[0] : nb[3];
+                double ynext = (ky < 0.0) ? nb[1] : nb[4];
+                double znext = (kz < 0.0) ? nb[2] : nb[5];
+                double dsx = (std::fabs(kx) > 1e-15) ? (xnext - x) / kx : DBL_MAX;
+                double dsy = (std::fabs(ky) > 1e-15) ? (ynext - y) / ky : DBL_MAX;
+                double dsz = (std::fabs(kz) > 1e-15) ? (znext - z) / kz : DBL_MAX;
+                double ds;
+                int wall;
+                if (dsx <= dsy && dsx <= dsz) { ds = dsx; wall = (kx < 0.0) ? BACK : FRONT; }
+                else if (dsy <= dsx && dsy <= dsz) { ds = dsy; wall = (ky < 0.0) ? LEFT : RIGHT; }
+                else { ds = dsz; wall = (kz < 0.0) ? BOTTOM : TOP; }
+                p.addSegment(cellnumber[node], ds);
+                x += (ds + eps) * kx;
+                y += (ds + eps) * ky;
+                z += (ds + eps) * kz;
+                int oldnode = node;
+                double q[3] = {x, y, z};
+                if (search == 0) node = whichnode(0, q);
+                else {
+                    node = whichnodeWall(node, wall, q);
+                    if (node < 0) node = whichnode(0, q);
+                }
+                if (node == oldnode) {
+                    st.stuck++;
+                    x = std::nextafter(x, (kx < 0.0) ? -DBL_MAX : DBL_MAX);
+                    y = std::nextafter(y, (ky < 0.0) ? -DBL_MAX : DBL_MAX);
+                    z = std::nextafter(z, (kz < 0.0) ? -DBL_MAX : DBL_MAX);
+                    double q2[3] = {x, y, z};
+                    node = whichnode(0, q2);
+                    if (node == oldnode) break;
+                }
+            }
+            return;
+        }
+
+        // Bookkeeping (octree only), TreeDustGrid.cpp:508-640
+        int l = node;
+        auto child = [&](int n, int q) { return nodes[n].child0 + q; };
+        while (true) {
+            const double* nb = nodes[l].b;
+            double xnext = (kx < 0.0) ? nb[0] : nb[3];
+            double ynext = (ky < 0.0) ? nb[1] : nb[4];
+            double znext = (kz < 0.0) ? nb[2] : nb[5];
+            double dsx = (std::fabs(kx) > 1e-15) ? (xnext - x) / kx : DBL_MAX;
+            double dsy = (std::fabs(ky) > 1e-15) ? (ynext - y) / ky : DBL_MAX;
+            double dsz = (std::fabs(kz) > 1e-15) ? (znext - z) / kz : DBL_MAX;
+            if (dsx <= dsy && dsx <= dsz) {
+                p.addSegment(cellnumber[l], dsx);
+                x = xnext; y += ky * dsx; z += kz * dsx;
+                while (true) {
+                    int oct = ((l - 1) % 8) + 1;
+                    bool place = (kx < 0.0) ? (oct % 2 == 1) : (oct % 2 == 0);
+                    if (!place) break;
+                    l = nodes[l].father;
+                    if (l == 0) return;
+                }
+                l += (kx < 0.0) ? -1 : 1;
+                while (cellnumber[l] == -1) {
+                    double yM = nodes[child(l, 0)].b[4], zM = nodes[child(l, 0)].b[5];
+                    if (kx < 0.0) l = (y <= yM) ? ((z <= zM) ? child(l, 1) : child(l, 5)) : ((z <= zM) ? child(l, 3) : child(l, 7));
+                    else l = (y <= yM) ? ((z <= zM) ? child(l, 0) : child(l, 4)) : ((z <= zM) ? child(l, 2) : child(l, 6));
+                }
+            } else if (dsy < dsx && dsy <= dsz) {
+                p.addSegment(cellnumber[l], dsy);
+                x += kx * dsy; y = ynext; z += kz * dsy;
+                while (true) {
+                    bool place = (ky < 0.0) ? ((l - 1) % 4 < 2) : ((l - 1) % 4 > 1);
+                    if (!place) break;
+                    l = nodes[l].father;
+                    if (l == 0) return;
+                }
+                l += (ky < 0.0) ? -2 : 2;
+                while (cellnumber[l] == -1) {
+                    double xM = nodes[child(l, 0)].b[3], zM = nodes[child(l, 0)].b[5];
+                    if (ky < 0.0) l = (x <= xM) ? ((z <= zM) ? child(l, 2) : child(l, 6)) : ((z <= zM) ? child(l, 3) : child(l, 7));
+                    else l = (x <= xM) ? ((z <= zM) ? child(l, 0) : child(l, 4)) : ((z <= zM) ? child(l, 1) : child(l, 5));
+                }
+            } else if (dsz < dsx && dsz < dsy) {
+                p.addSegment(cellnumber[l], dsz);
+                x += kx * dsz; y += ky * dsz; z = znext;
+                while (true) {
+                    int oct = ((l - 1) % 8) + 1;
+                    bool place = (kz < 0.0) ? (oct < 5) : (oct > 4);
+                    if (!place) break;
+                    l = nodes[l].father;
+                    if (l == 0) return;
+                }
+                l += (kz < 0.0) ? -4 : 4;
+                while (cellnumber[l] == -1) {
+                    double xM = nodes[child(l, 0)].b[3], yM = nodes[child(l, 0)].b[4];
+                    if (kz < 0.0) l = (x <= xM) ? ((y <= yM) ? child(l, 4) : child(l, 6)) : ((y <= yM) ? child(l, 5) : child(l, 7));
+                    else l = (x <= xM) ? ((y <= yM) ? child(l, 0) : child(l, 2)) : ((y <= yM) ? child(l, 1) : child(l, 3));
+                }
+            }
+        }
+    }
+};
+
+// ===================================================================== Voronoi grid
+
+// VoronoiMesh.cpp:541-570 (cellIndex), :778-873 (path). The neighbour lists are an input
+// (frozen); the nearest-site search is exact with (distance, index) ordering.
+struct VoronoiGrid : Grid {
+    double b[6];
+    double eps;
+    std::vector<double> px, py, pz;
+    std::vector<int64_t> off;
+    std::vector<int32_t> ids;
+    // uniform bucket grid for the exact nearest-site search
+    int nb = 1;
+    std::vector<int64_t> boff;
+    std::vector<int32_t> bids;
+
+    int64_t ncells() const override { return (int64_t)px.size(); }
+
+    void build(const double ext[6], const double* sites, int64_t n, const int64_t* o, const int32_t* nid)
+    {
+        std::memcpy(b, ext, sizeof b);
+        double wx = b[3] - b[0], wy = b[4] - b[1], wz = b[5] - b[2];
+        eps = 1e-12 * std::sqrt(wx * wx + wy * wy + wz * wz);   // VoronoiMesh.cpp:220
+        px.resize(n); py.resize(n); pz.resize(n);
+        for (int64_t i = 0; i < n; i++) { px[i] = sites[3 * i]; py[i] = sites[3 * i + 1]; pz[i] = sites[3 * i + 2]; }
+        off.assign(o, o + n + 1);
+        ids.assign(nid, nid + o[n]);
+        nb = std::max(1, (int)std::cbrt((double)n / 4.0));
+        std::vector<std::vector<int32_t>> tmp((size_t)nb * nb * nb);
+        for (int64_t i = 0; i < n; i++) tmp[bucket(px[i], py[i], pz[i])].push_back((int32_t)i);
+        boff.assign(tmp.size() + 1, 0);
+        for (size_t q = 0; q < tmp.size(); q++) boff[q + 1] = boff[q] + tmp[q].size();
+        bids.clear();
+        for (auto& t : tmp) bids.insert(bids.end(), t.begin(), t.end());
+    }
+    int bidx(double x, int a) const
+    {
+        int i = (int)(nb * (x - b[a]) / (b[a + 3] - b[a]));
+        return std::max(0, std::min(nb - 1, i));
+    }
+    size_t bucket(double x, double y, double z) const
+    {
+        return ((size_t)bidx(x, 0) * nb + bidx(y, 1)) * nb + bidx(z, 2);
+    }
+    static bool better(double d, int i, double bd, int bi) { return d < bd || (d == bd && i < bi); }
+    int nearest(const double r[3]) const
+    {
+        int ci = bidx(r[0], 0), cj = bidx(r[1], 1), ck = bidx(r[2], 2);
+        double bd = DBL_MAX;
+        int best = -1;
+        double w[3] = {(b[3] - b[0]) / nb, (b[4] - b[1]) / nb, (b[5] - b[2]) / nb};
+        double wmin = std::min(w[0], std::min(w[1], w[2]));
+        for (int ring = 0; ring <= nb; ring++) {
+            for (int i = ci - ring; i <= ci + ring; i++)
+                for (int j = cj - ring; j <= cj + ring; j++)
+                    for (int k = ck - ring; k <= ck + ring; k++) {
+                        if (std::max(std::abs(i - ci), std::max(std::abs(j - cj), std::abs(k - ck))) != ring) continue;
+                        if (i < 0 || j < 0 || k < 0 || i >= nb || j >= nb || k >= nb) continue;
+                        size_t q = ((size_t)i * nb + j) * nb + k;
+                        for (int64_t t = boff[q]; t < boff[q + 1]; t++) {
+                            int s = bids[t];
+                            double dx = r[0] - px[s], dy = r[1] - py[s], dz = r[2] - pz[s];
+                            double d = dx * dx + dy * dy + dz * dz;
+                            if (better(d, s, bd, best)) { bd = d; best = s; }
+                        }
+                    }
+            // every site outside the rings scanned so far is at least ring*wmin away
+            double reach = ring * wmin;
+            if (best >= 0 && reach * reach > bd) break;
+        }
+        return best;
+    }
+    int whichcell(const double r[3]) const override
+    {
+        if (!(r[0] >= b[0] && r[0] <= b[3] && r[1] >= b[1] && r[1] <= b[4] && r[2] >= b[2] && r[2] <= b[5])) return -1;
+        return nearest(r);
+    }
+    void path(Path& p, WalkStats&) const override
+    {
+        p.clear();
+        double k[3] = {p.k[0], p.k[1], p.k[2]};
+        double r[3];
+        p.moveInside(b, eps, r);
+        int mr = whichcell(r);
+        if (mr < 0) return p.clear();
+        while (mr >= 0) {
+            double prx = px[mr], pry = py[mr], prz = pz[mr];
+            double sq = DBL_MAX;
+            const int NO_INDEX = -99;
+            int mq = NO_INDEX;
+            for (int64_t t = off[mr]; t < off[mr + 1]; t++) {
+                int mi = ids[t];
+                double si = 0;
+                if (mi >= 0) {
+                    double nx = px[mi] - prx, ny = py[mi] - pry, nz = pz[mi] - prz;
+                    double ndotk = nx * k[0] + ny * k[1] + nz * k[2];
+                    if (ndotk > 0) {
+                        double qx = 0.5 * (px[mi] + prx), qy = 0.5 * (py[mi] + pry), qz = 0.5 * (pz[mi] + prz);
+                        si = (nx * (qx - r[0]) + ny * (qy - r[1]) + nz * (qz - r[2])) / ndotk;
+                    }
+                } else {
+                    switch (mi) {
+                    case -1: si = (b[0] - r[0]) / k[0]; break;
+                    case -2: si = (b[3] - r[0]) / k[0]; break;
+                    case -3: si = (b[1] - r[1]) / k[1]; break;
+                    case -4: si = (b[4] - r[1]) / k[1]; break;
+                    case -5: si = (b[2] - r[2]) / k[2]; break;
+                    case -6: si = (b[5] - r[2]) / k[2]; break;
+                    default: si = 0;
+                    }
+                }
+                if (si > 0 && si < sq) { sq = si; mq = mi; }
+            }
+            if (mq == NO_INDEX) {
+                r[0] += k[0] * eps; r[1] += k[1] * eps; r[2] += k[2] * eps;
+                mr = whichcell(r);
+            } else {
+                p.addSegment(mr, sq);
+                r[0] += (sq + eps) * k[0]; r[1] += (sq + eps) * k[1]; r[2] += (sq + eps) * k[2];
+                mr = mq;
+            }
+        }
+    }
+};
+
+// ===================================================================== random numbers
+
+// Random.cpp:41-56 (69069 seeding) and :90-127 (MT19937 + uniform in (0,1))
+struct MT {
+    uint32_t mt[624];
+    int mti = 624;
+    void seed(uint32_t s)
+    {
+        mt[0] = s;
+        for (mti = 1; mti < 624; mti++) mt[mti] = 69069u * mt[mti - 1];
+    }
+    uint32_t next()
+    {
+        static const uint32_t mag01[2] = {0x0u, 0x9908b0dfu};
+        uint32_t y;
+        if (mti >= 624) {
+            int kk;
+            for (kk = 0; kk < 227; kk++) {
+                y = (mt[kk] & 0x80000000u) | (mt[kk + 1] & 0x7fffffffu);
+                mt[kk] = mt[kk + 397] ^ (y >> 1) ^ mag01[y & 0x1];
+            }
+            for (; kk < 623; kk++) {
+                y = (mt[kk] & 0x80000000u) | (mt[kk + 1] & 0x7fffffffu);
+                mt[kk] = mt[kk - 227] ^ (y >> 1) ^ mag01[y & 0x1];
+            }
+            y = (mt[623] & 0x80000000u) | (mt[0] & 0x7fffffffu);
+            mt[623] = mt[396] ^ (y >> 1) ^ mag01[y & 0x1];
+            mti = 0;
+        }
+        y = mt[mti++];
+        y ^= (y >> 11);
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= (y >> 18);
+        return y;
+    }
+    double uniform()
+    {
+        double ans;
+        do { ans = (double)next() / 4294967295.0; } while (ans <= 0.0 || ans >= 1.0);
+        return ans;
+    }
+};
+
+// Philox4x32-10 (Salmon et al., SC'11), written from the published round function.
+void philox(const uint32_t c_in[4], const uint32_t k_in[2], uint32_t out[4])
+{
+    uint32_t c0 = c_in[0], c1 = c_in[1], c2 = c_in[2], c3 = c_in[3];
+    uint32_t k0 = k_in[0], k1 = k_in[1];
+    for (int r = 0; r < 10; r++) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0; c1 = lo1; c2 = hi0 ^ c3 ^ k1; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Uniform deviate source: MT19937 (reference) or the counter-based packet stream
+// key = (seed, member), counter = (packet lo, packet hi, draw/4, ell), word = draw%4,
+// u = (w + 0.5) / 2^32.
+struct Rng {
+    bool use_mt = true;
+    MT* mt = nullptr;
+    uint32_t key[2] = {0, 0};
+    uint64_t pkt = 0;
+    uint32_t ell = 0;
+    uint32_t draw = 0;
+    uint32_t blk[4];
+    double uniform()
+    {
+        if (use_mt) return mt->uniform();
+        if ((draw & 3) == 0) {
+            uint32_t c[4] = {(uint32_t)pkt, (uint32_t)(pkt >> 32), draw >> 2, ell};
+            philox(c, key, blk);
+        }
+        uint32_t w = blk[draw & 3];
+        draw++;
+        return ((double)w + 0.5) * (1.0 / 4294967296.0);
+    }
+    void start_packet(uint64_t p) { pkt = p; draw = 0; }
+};
+
+// Random.cpp:163-176
+double exponcutoff(Rng& rng, double xmax)
+{
+    if (xmax == 0.0) return 0.0;
+    else if (xmax < 1e-10) return rng.uniform() * xmax;
+    double x = -std::log(1.0 - rng.uniform() * (1.0 - std::exp(-xmax)));
+    while (x > xmax) x = -std::log(1.0 - rng.uniform() * (1.0 - std::exp(-xmax)));
+    return x;
+}
+
+// Direction.cpp:12-36 (Direction(theta, phi))
+void direction_from_angles(double theta, double phi, double k[3])
+{
+    const double eps = 1e-8;
+    if (theta <= eps) { k[0] = 0; k[1] = 0; k[2] = 1; }
+    else if (theta >= M_PI - eps) { k[0] = 0; k[1] = 0; k[2] = -1; }
+    else {
+        double st = std::sin(theta);
+        k[0] = st * std::cos(phi); k[1] = st * std::sin(phi); k[2] = std::cos(theta);
+    }
+}
+// Random.cpp:180-185
+void random_direction(Rng& rng, double k[3])
+{
+    double theta = std::acos(2.0 * rng.uniform() - 1.0);
+    double phi = 2.0 * M_PI * rng.uniform();
+    direction_from_angles(theta, phi, k);
+}
+// Random.cpp:189-223
+void random_direction_about(Rng& rng, const double kin[3], double costheta, double k[3])
+{
+    double phi = 2.0 * M_PI * rng.uniform();
+    double cosphi = std::cos(phi), sinphi = std::sin(phi);
+    double sintheta = std::sqrt(std::fabs((1.0 - costheta) * (1.0 + costheta)));
+    double kx = kin[0], ky = kin[1], kz = kin[2];
+    if (kz > 0.99999) { k[0] = cosphi * sintheta; k[1] = sinphi * sintheta; k[2] = costheta; }
+    else if (kz < -0.99999) { k[0] = cosphi * sintheta; k[1] = sinphi * sintheta; k[2] = -costheta; }
+    else {
+        double root = std::sqrt((1.0 - kz) * (1.0 + kz));
+        k[0] = sintheta / root * (-kx * kz * cosphi + ky * sinphi) + kx * costheta;
+        k[1] = -sintheta / root * (ky * kz * cosphi + kx * sinphi) + ky * costheta;
+        k[2] = root * sintheta * cosphi + kz * costheta;
+    }
+}
+
+// ===================================================================== special functions
+
+// SpecialFunctions.cpp:579-620; the branch-point series coefficients are evaluated from the
+// published rational series of W_{-1} in p = sqrt(2(1+e z)) (Corless et al. 1996, eq. 4.22).
+double lambertw1(double z)
+{
+    static double c[12];
+    static bool init = false;
+    if (!init) {
+        static const double a[12] = {-1.0, 1.0, -1.0 / 3.0, 11.0 / 72.0, -43.0 / 540.0, 769.0 / 17280.0,
+                                     -221.0 / 8505.0, 680863.0 / 43545600.0, -1963.0 / 204120.0,
+                                     226287557.0 / 37623398400.0, -5776369.0 / 1515591000.0,
+                                     169709463197.0 / 69528040243200.0};
+        double s = std::sqrt(2.0 * M_E), f = 1.0;
+        for (int i = 0; i < 12; i++) { c[i] = a[i] * f; f *= s; }
+        init = true;
+    }
+    const double eps = 1.0e-12;
+    const double em1 = 0.3678794411714423215955237701614608;
+    if (z < -em1 || z > 0.0 || std::isinf(z) || std::isnan(z)) return NAN;
+    if (z == 0.0) return -DBL_MAX;
+    double q = z + em1;
+    double r = -std::sqrt(q);
+    double t8 = c[8] + r * (c[9] + r * (c[10] + r * c[11]));
+    double t5 = c[5] + r * (c[6] + r * (c[7] + r * t8));
+    double t1 = c[1] + r * (c[2] + r * (c[3] + r * (c[4] + r * t5)));
+    double w0 = c[0] + r * t1;
+    if (q < 3.0e-3) return w0;
+    double w, e, p, t;
+    if (z < -1e-6) w = w0;
+    else {
+        double l1 = std::log(-z), l2 = std::log(-l1);
+        w = l1 - l2 + l2 / l1;
+    }
+    for (int i = 0; i < 10; i++) {
+        e = std::exp(w);
+        t = w * e - z;
+        p = w + 1.0;
+        t /= e * p - 0.5 * (p + 1.0) * t / p;
+        w -= t;
+        if (std::fabs(t) < eps * (1.0 + std::fabs(w))) return w;
+    }
+    return NAN;
+}
+
+// SpecialFunctions.cpp:16-40 (Lanczos approximation of ln Gamma, Numerical Recipes gammln)
+double lngamma(double a)
+{
+    static const double cof[6] = {76.18009172947146, -86.50532032941677, 24.01409824083091,
+                                  -1.231739572450155, 0.1208650973866179e-2, -0.5395239384953e-5};
+    double y = a, xx = a;
+    double tmp = xx + 5.5;
+    tmp -= (xx + 0.5) * std::log(tmp);
+    double ser = 1.000000000190015;
+    for (int j = 0; j < 6; j++) ser += cof[j] / ++y;
+    return -tmp + std::log(2.5066282746310005 * ser / xx);
+}
+double gammafn(double a) { return std::exp(lngamma(a)); }
+
+// SersicFunction.cpp:15-68 (deprojected Sersic profile and normalised cumulative mass on 101 radii)
+void sersic_table(double n, std::vector<double>& sv, std::vector<double>& Mv)
+{
+    double b = 2.0 * n - 1.0 / 3.0 + 4.0 / 405.0 / n + 46.0 / 25515.0 / (n * n) + 131.0 / 1148175.0 / (n * n * n);
+    double I0 = std::pow(b, 2.0 * n) / (M_PI * gammafn(2.0 * n + 1));
+    const int Ns = 101;
+    sv.assign(Ns, 0); Mv.assign(Ns, 0);
+    std::vector<double> Sv(Ns);
+    double logsmin = -6.0, logsmax = 4.0, dlogs = (logsmax - logsmin) / (Ns - 1.0);
+    for (int i = 0; i < Ns; i++) {
+        double logs = logsmin + i * dlogs;
+        double s = std::pow(10.0, logs);
+        sv[i] = s;
+        double alpha = b * std::pow(s, 1.0 / n);
+        double sum = 0.0;
+        int Nu = 10000;
+        double tmax = 100.0;
+        double umax = std::sqrt((tmax + 1.0) * (tmax - 1.0));
+        double du = umax / Nu;
+        for (int j = 0; j <= Nu; j++) {
+            double weight = (j == 0 || j == Nu) ? 0.5 : 1.0;
+            double u = j * du, u2 = u * u, w;
+            if (u > 1e-3) w = (std::pow(1.0 + u2, 2.0 * n) - 1.0) / u2;
+            else w = 2.0 * n + n * (2.0 * n - 1.0) * u2 + 2.0 / 3.0 * n * (2.0 * n - 1.0) * (n - 1.0) * u2 * u2;
+            sum += weight * 2.0 * std::exp(-alpha * (1.0 + u2)) / std::sqrt(w);
+        }
+        Sv[i] = I0 * std::pow(b, n) * std::pow(alpha, 1.0 - n) / M_PI * du * sum;
+    }
+    auto S = [&](double s) {
+        if (s <= sv[0]) return Sv[0];
+        if (s >= sv[Ns - 1]) return Sv[Ns - 1];
+        int i = nr_locate_clip(sv.data(), Ns, s);
+        return nr_loglog(s, sv[i], sv[i + 1], Sv[i], Sv[i + 1]);
+    };
+    for (int i = 1; i < Ns; i++) {
+        double sum = 0.0;
+        for (int j = 0; j <= 32; j++) {
+            double weight = (j == 0 || j == 32) ? 0.5 : 1.0;
+            double ds = (sv[i] - sv[i - 1]) / 32.0;
+            double s = sv[i - 1] + j * ds;
+            sum += weight * S(s) * s * s * ds;
+        }
+        Mv[i] = Mv[i - 1] + 4.0 * M_PI * sum;
+    }
+    double tot = Mv[Ns - 1];
+    for (int i = 0; i < Ns; i++) Mv[i] /= tot;
+}
+
+// ===================================================================== scene
+
+struct Star {
+    double p[16];
+    std::vector<double> lum;
+    std::vector<double> ts, tM;
+    double CN = 1, c = 1, tanp = 1;
+};
+
+struct Instrument {
+    double distance, incl, azim, pa;
+    int Nx, Ny;
+    double fovx, fovy, xc, yc;
+    double costheta, sintheta, cosphi, sinphi, cospa, sinpa;
+    double kobs[3];
+    double xpmin, xpsiz, ypmin, ypsiz;
+};
+
+struct Scene {
+    std::unique_ptr<Grid> grid;
+    std::vector<double> rho;
+    int nlambda = 1;
+    std::vector<double> kext, alb, g;
+    bool store_abs = false;
+    bool has_dust = false;
+    std::vector<Star> stars;
+    std::vector<Instrument> instr;
+};
+
+// ---------------- sources: StellarSystem.cpp:116-158, GeometricStellarComp.cpp:75-81
+
+// ExpDiskGeometry.cpp:117-146, SepAxGeometry.cpp:21-30, Position.cpp:23-31
+void expdisk_position(const Star& st, Rng& rng, double r[3])
+{
+    double hR = st.p[1], hz = st.p[2], Rmax = st.p[3], zmax = st.p[4], Rmin = st.p[5];
+    double R, X;
+    do {
+        X = rng.uniform();
+        R = hR * (-1.0 - lambertw1((X - 1.0) / M_E));
+    } while ((Rmax > 0.0 && R >= Rmax) || R <= Rmin);
+    double phi = 2.0 * M_PI * rng.uniform();
+    double z;
+    do {
+        X = rng.uniform();
+        z = (X <= 0.5) ? hz * std::log(2.0 * X) : -hz * std::log(2.0 * (1.0 - X));
+    } while (zmax > 0.0 && std::fabs(z) >= zmax);
+    r[0] = R * std::cos(phi); r[1] = R * std::sin(phi); r[2] = z;
+}
+// SersicFunction.cpp:112-124
+double sersic_inversemass(const Star& st, double M)
+{
+    int Ns = (int)st.ts.size();
+    if (M <= st.tM[0]) return st.ts[0];
+    if (M >= st.tM[Ns - 1]) return st.ts[Ns - 1];
+    int i = nr_locate_clip(st.tM.data(), Ns, M);
+    return nr_loglog(M, st.tM[i], st.tM[i + 1], st.ts[i], st.ts[i + 1]);
+}
+// SersicGeometry.cpp:85-90, SpheGeometry.cpp:36-44, Position.cpp:51-54
+void sersic_position(const Star& st, Rng& rng, double r[3])
+{
+    double X = rng.uniform();
+    double rad = st.p[2] * sersic_inversemass(st, X);
+    double k[3];
+    random_direction(rng, k);
+    r[0] = rad * k[0]; r[1] = rad * k[1]; r[2] = rad * k[2];
+}
+// SpiralStructureGeometryDecorator.cpp:223-229
+double spiral_perturbation(const Star& st, double R, double phi)
+{
+    double m = st.p[7], R0 = st.p[9], phi0 = st.p[10], w = st.p[11];
+    int N = (int)st.p[12];
+    double gamma = std::log(R / R0) / st.tanp + phi0 + 0.5 * M_PI / m;
+    return (1.0 - w) + w * st.CN * std::pow(std::sin(0.5 * m * (gamma - phi)), 2 * N);
+}
+// SpiralStructureGeometryDecorator.cpp:176-191
+void star_position(const Star& st, Rng& rng, double r[3])
+{
+    if ((int)st.p[0] == 1) expdisk_position(st, rng, r);
+    else sersic_position(st, rng, r);
+    if (st.p[6] != 0) {
+        double R = std::sqrt(r[0] * r[0] + r[1] * r[1]);
+        double z = r[2];
+        double c = 1.0 + (st.CN - 1.0) * st.p[11];
+        double phi, t;
+        do {
+            phi = 2.0 * M_PI * rng.uniform();
+            t = rng.uniform() * c / spiral_perturbation(st, R, phi);
+        } while (t > 1);
+        r[0] = R * std::cos(phi); r[1] = R * std::sin(phi); r[2] = z;
+    }
+}
+
+struct PP {
+    double L;
+    int ell;
+    double r[3], k[3];
+    int nscatt;
+};
+
+// StellarSystem.cpp:116-158
+void launch(const Scene& sc, PP& pp, int ell, double L, Rng& rng, double emissionBias, const std::vector<double>& Ltot)
+{
+    int N = (int)sc.stars.size();
+    int h = 0;
+    double Lw = L;
+    if (N > 1) {
+        double X = rng.uniform();
+        if (X < emissionBias) h = std::max(0, std::min(N - 1, (int)(N * X / emissionBias)));
+        else {
+            // StellarSystem.cpp setup: normalised cumulative luminosity per wavelength
+            std::vector<double> Xv(N + 1, 0.0);
+            for (int i = 0; i < N; i++) Xv[i + 1] = Xv[i] + sc.stars[i].lum[ell];
+            for (int i = 0; i <= N; i++) Xv[i] /= Xv[N];
+            h = nr_locate_clip(Xv.data(), N + 1, (X - emissionBias) / (1.0 - emissionBias));
+        }
+        double Lh = sc.stars[h].lum[ell];
+        if (!(Lh > 0)) { pp.L = 0; pp.ell = ell; return; }
+        double Lmean = Ltot[ell] / N;
+        Lw = L * (1.0 / (1.0 - emissionBias + emissionBias * Lmean / Lh));
+    }
+    star_position(sc.stars[h], rng, pp.r);
+    random_direction(rng, pp.k);
+    pp.L = Lw; pp.ell = ell; pp.nscatt = 0;
+}
+
+// ---------------- instruments: DistantInstrument.cpp:27-50, SingleFrameInstrument.cpp:26-42,130-147
+
+void setup_instrument(Instrument& in)
+{
+    in.costheta = std::cos(in.incl); in.sintheta = std::sin(in.incl);
+    in.cosphi = std::cos(in.azim); in.sinphi = std::sin(in.azim);
+    in.cospa = std::cos(in.pa); in.sinpa = std::sin(in.pa);
+    direction_from_angles(in.incl, in.azim, in.kobs);
+    in.xpmin = in.xc - 0.5 * in.fovx; in.xpsiz = in.fovx / in.Nx;
+    in.ypmin = in.yc - 0.5 * in.fovy; in.ypsiz = in.fovy / in.Ny;
+}
+int pixelondetector(const Instrument& in, const double r[3])
+{
+    double x = r[0], y = r[1], z = r[2];
+    double xpp = -in.sinphi * x + in.cosphi * y;
+    double ypp = -in.cosphi * in.costheta * x - in.sinphi * in.costheta * y + in.sintheta * z;
+    double xp = in.cospa * xpp - in.sinpa * ypp;
+    double yp = in.sinpa * xpp + in.cospa * ypp;
+    int i = (int)std::floor((xp - in.xpmin) / in.xpsiz);
+    int j = (int)std::floor((yp - in.ypmin) / in.ypsiz);
+    if (i < 0 || i >= in.Nx || j < 0 || j >= in.Ny) return -1;
+    return i + in.Nx * j;
+}
+
+struct Tally {
+    std::vector<double*> frames;   // per instrument, [ell*Npix + pixel]
+    double* labs = nullptr;        // [m*nlambda + ell]
+};
+
+struct Counters { uint64_t pk = 0, own = 0, peel = 0, scat = 0, npeel = 0, walks = 0; WalkStats ws; };
+
+// LockFree.hpp:22-34 (compare-and-swap add of a double)
+inline void atomic_add(double* t, double v)
+{
+    uint64_t* p = reinterpret_cast<uint64_t*>(t);
+    uint64_t oldb = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (true) {
+        double oldv;
+        std::memcpy(&oldv, &oldb, 8);
+        double nv = oldv + v;
+        uint64_t nb;
+        std::memcpy(&nb, &nv, 8);
+        if (__atomic_compare_exchange_n(p, &oldb, nb, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return;
+    }
+}
+
+inline int64_t count_cells(const Path& p)
+{
+    int64_t n = 0;
+    for (auto& s : p.v) n += (s.m >= 0);
+    return n;
+}
+
+// FrameInstrument.cpp:33-47 with Instrument.cpp:69-72 and DustSystem.cpp:960-976
+void detect(const Scene& sc, size_t i, const double r[3], double L, int ell, Path& path, Tally& t, Counters& c)
+{
+    const Instrument& in = sc.instr[i];
+    int l = pixelondetector(in, r);
+    if (l < 0) return;
+    double tau = 0;
+    if (sc.has_dust) {
+        std::memcpy(path.r, r, sizeof(double) * 3);
+        std::memcpy(path.k, in.kobs, sizeof(double) * 3);
+        sc.grid->path(path, c.ws);
+        c.peel += count_cells(path);
+        c.walks++;
+        double ke = sc.kext[ell];
+        const double* rho = sc.rho.data();
+        tau = path.opticalDepth([&](int m) { double res = 0; if (m >= 0) res += ke * rho[m]; return res; });
+    }
+    c.npeel++;
+    atomic_add(&t.frames[i][(size_t)ell * in.Nx * in.Ny + l], L * std::exp(-tau));
+}
+
+struct MCParams {
+    double packages, minWeightReduction;
+    int minfs;
+    double xi;
+    uint32_t seed;
+    double emissionBias;
+    uint32_t member;
+};
+
+// MonteCarloSimulation.cpp:259-297 body for one packet, with :299-357, :432-543
+int one_packet(const Scene& sc, const MCParams& mp, int ell, double L, double Lthreshold, Rng& rng,
+               const std::vector<double>& Ltot, Path& path, Path& ppath, Tally& t, Counters& c)
+{
+    PP pp;
+    launch(sc, pp, ell, L, rng, mp.emissionBias, Ltot);
+    c.pk++;
+    if (!(pp.L > 0)) return 0;
+    // peeloffemission :299-309 (isotropic geometry: probabilityForDirection == 1)
+    for (size_t i = 0; i < sc.instr.size(); i++) detect(sc, i, pp.r, pp.L, ell, ppath, t, c);
+    if (!sc.has_dust) return 0;
+    const double ke = sc.kext[ell], albedo = sc.alb[ell], g = sc.g[ell];
+    const double* rho = sc.rho.data();
+    auto kr = [&](int m) { double res = 0; if (m >= 0) res += ke * rho[m]; return res; };
+    while (true) {
+        // DustSystem::fillOpticalDepth, DustSystem.cpp:935-956
+        std::memcpy(path.r, pp.r, sizeof pp.r);
+        std::memcpy(path.k, pp.k, sizeof pp.k);
+        sc.grid->path(path, c.ws);
+        c.own += count_cells(path);
+        c.walks++;
+        path.fillOpticalDepth(kr);
+        double taupath = path.tau();
+        if (taupath < 0.0 || std::isnan(taupath) || std::isinf(taupath)) {
+            g_err = "The optical depth along the path is not a positive number";
+            return -1;
+        }
+        // simulateescapeandabsorption :432-461 (single component)
+        double L0 = pp.L;
+        double expfactor = -std::expm1(-taupath);
+        if (sc.store_abs && t.labs) {
+            int N = (int)path.v.size();
+            for (int n = 0; n < N; n++) {
+                int m = path.v[n].m;
+                if (m != -1) {
+                    double taustart = (n == 0) ? 0.0 : path.v[n - 1].tau;
+                    double expfactorm = -std::expm1(-path.v[n].dtau);
+                    double Lintm = L0 * std::exp(-taustart) * expfactorm;
+                    atomic_add(&t.labs[(size_t)m * sc.nlambda + ell], (1.0 - albedo) * Lintm);
+                }
+            }
+        }
+        pp.L = L0 * albedo * expfactor;
+        if (pp.L <= 0 || (pp.L <= Lthreshold && pp.nscatt >= mp.minfs)) break;
+        // simulatepropagation :513-531
+        if (taupath != 0.0) {
+            double tau;
+            if (mp.xi == 0.0) tau = exponcutoff(rng, taupath);
+            else {
+                double X = rng.uniform();
+                tau = (X < mp.xi) ? rng.uniform() * taupath : exponcutoff(rng, taupath);
+                double p = -std::exp(-tau) / std::expm1(-taupath);
+                double q = (1.0 - mp.xi) * p + mp.xi / taupath;
+                pp.L = pp.L * (p / q);
+            }
+            double s = path.pathlength(tau);
+            pp.r[0] += s * pp.k[0]; pp.r[1] += s * pp.k[1]; pp.r[2] += s * pp.k[2];
+        }
+        // peeloffscattering :313-357 (Ncomp == 1: w = phase function value)
+        for (size_t i = 0; i < sc.instr.size(); i++) {
+            const double* ko = sc.instr[i].kobs;
+            double cosalpha = pp.k[0] * ko[0] + pp.k[1] * ko[1] + pp.k[2] * ko[2];
+            double tt = 1.0 + g * g - 2 * g * cosalpha;
+            double w = (1.0 - g) * (1.0 + g) / std::sqrt(tt * tt * tt);   // DustMix.cpp:650-653
+            detect(sc, i, pp.r, pp.L * w, ell, ppath, t, c);
+        }
+        // simulatescattering :535-543, DustMix.cpp:597-601
+        double knew[3];
+        if (std::fabs(g) < 1e-6) random_direction(rng, knew);
+        else {
+            double f = ((1.0 - g) * (1.0 + g)) / (1.0 - g + 2.0 * g * rng.uniform());
+            double costheta = (1.0 + g * g - f * f) / (2.0 * g);
+            random_direction_about(rng, pp.k, costheta, knew);
+        }
+        std::memcpy(pp.k, knew, sizeof knew);
+        pp.nscatt++;
+        c.scat++;
+    }
+    return 0;
+}
+
+bool parse_mc(const double* mc8, MCParams& mp)
+{
+    mp.packages = mc8[0];
+    mp.minWeightReduction = mc8[1];
+    mp.minfs = (int)mc8[2];
+    mp.xi = mc8[3];
+    mp.seed = (uint32_t)mc8[4];
+    mp.emissionBias = mc8[5];
+    mp.member = (uint32_t)mc8[6];
+    return true;
+}
+
+std::vector<double> total_lum(const Scene& sc)
+{
+    std::vector<double> L(sc.nlambda, 0.0);
+    for (int ell = 0; ell < sc.nlambda; ell++)
+        for (auto& s : sc.stars) L[ell] += s.lum[ell];
+    return L;
+}
+
+Tally make_tally(const Scene& sc, double* frames, double* labs)
+{
+    Tally t;
+    size_t off = 0;
+    for (auto& in : sc.instr) {
+        t.frames.push_back(frames + off);
+        off += (size_t)sc.nlambda * in.Nx * in.Ny;
+    }
+    t.labs = labs;
+    return t;
+}
+
+void fold(uint64_t* out, const Counters& c, uint64_t npp)
+{
+    out[0] += c.pk; out[1] += c.own; out[2] += c.peel; out[3] += c.scat;
+    out[4] += c.npeel; out[5] += c.ws.stuck; out[6] = npp; out[7] += c.walks;
+}
+
+} // namespace
+
+// ===================================================================== C API
+
+extern "C" {
+
+const char* or_last_error(void) { return g_err.c_str(); }
+
+void* or_scene_create(void) { return new Scene(); }
+void or_scene_destroy(void* s) { delete (Scene*)s; }
+
+int or_scene_set_cartesian(void* s, const double ext[6], const double* xv, int nxv, const double* yv, int nyv,
+                           const double* zv, int nzv)
+{
+    auto g = std::make_unique<CartesianGrid>();
+    std::memcpy(g->b, ext, sizeof g->b);
+    g->xv.assign(xv, xv + nxv); g->yv.assign(yv, yv + nyv); g->zv.assign(zv, zv + nzv);
+    g->Nx = nxv - 1; g->Ny = nyv - 1; g->Nz = nzv - 1;
+    ((Scene*)s)->grid = std::move(g);
+    return 0;
+}
+
+int or_scene_set_tree(void* s, int arity, const double ext[6], const uint8_t* flags, int64_t nnodes, int search)
+{
+    auto g = std::make_unique<TreeGrid>();
+    if (!g->build(arity, ext, flags, nnodes, search)) return -1;
+    ((Scene*)s)->grid = std::move(g);
+    return 0;
+}
+
+int or_scene_set_voronoi(void* s, const double ext[6], const double* sites, int64_t n, const int64_t* off,
+                         const int32_t* ids)
+{
+    auto g = std::make_unique<VoronoiGrid>();
+    g->build(ext, sites, n, off, ids);
+    ((Scene*)s)->grid = std::move(g);
+    return 0;
+}
+
+int64_t or_scene_num_cells(void* s) { return ((Scene*)s)->grid ? ((Scene*)s)->grid->ncells() : 0; }
+
+int or_scene_set_dust(void* s, const double* rho, int64_t ncells, int nlambda, const double* kext,
+                      const double* albedo, const double* g, int store_abs)
+{
+    Scene* sc = (Scene*)s;
+    sc->rho.assign(rho, rho + ncells);
+    sc->nlambda = nlambda;
+    sc->kext.assign(kext, kext + nlambda);
+    sc->alb.assign(albedo, albedo + nlambda);
+    sc->g.assign(g, g + nlambda);
+    sc->store_abs = store_abs != 0;
+    sc->has_dust = true;
+    return 0;
+}
+
+int or_scene_add_star(void* s, const double* p16, const double* lum, const double* ts, const double* tM, int nt)
+{
+    Scene* sc = (Scene*)s;
+    Star st;
+    std::memcpy(st.p, p16, sizeof st.p);
+    st.lum.assign(lum, lum + sc->nlambda);
+    if (nt > 0) { st.ts.assign(ts, ts + nt); st.tM.assign(tM, tM + nt); }
+    if (st.p[6] != 0) {
+        // SpiralStructureGeometryDecorator.cpp:23-40
+        st.tanp = std::tan(st.p[8]);
+        double N = st.p[12];
+        st.CN = std::sqrt(M_PI) * gammafn(N + 1.0) / gammafn(N + 0.5);
+        st.c = 1.0 + (st.CN - 1.0) * st.p[11];
+    }
+    sc->stars.push_back(st);
+    return 0;
+}
+
+int or_scene_add_instrument(void* s, const double* p)
+{
+    Instrument in;
+    in.distance = p[0]; in.incl = p[1]; in.azim = p[2]; in.pa = p[3];
+    in.Nx = (int)p[4]; in.Ny = (int)p[5]; in.fovx = p[6]; in.fovy = p[7]; in.xc = p[8]; in.yc = p[9];
+    setup_instrument(in);
+    ((Scene*)s)->instr.push_back(in);
+    return 0;
+}
+
+int or_scene_path(void* s, const double rk[6], int maxseg, int32_t* m, double* ds)
+{
+    Scene* sc = (Scene*)s;
+    Path p;
+    std::memcpy(p.r, rk, 3 * sizeof(double));
+    std::memcpy(p.k, rk + 3, 3 * sizeof(double));
+    WalkStats st;
+    sc->grid->path(p, st);
+    int n = (int)p.v.size();
+    if (n > maxseg) return -1;
+    for (int i = 0; i < n; i++) { m[i] = p.v[i].m; ds[i] = p.v[i].ds; }
+    return n;
+}
+
+int or_scene_whichcell(void* s, const double r[3]) { return ((Scene*)s)->grid->whichcell(r); }
+
+double or_scene_opticaldepth(void* s, const double rk[6], int ell)
+{
+    Scene* sc = (Scene*)s;
+    Path p;
+    std::memcpy(p.r, rk, 3 * sizeof(double));
+    std::memcpy(p.k, rk + 3, 3 * sizeof(double));
+    WalkStats st;
+    sc->grid->path(p, st);
+    double ke = sc->kext[ell];
+    const double* rho = sc->rho.data();
+    return p.opticalDepth([&](int m) { double res = 0; if (m >= 0) res += ke * rho[m]; return res; });
+}
+
+void or_scene_pathlength(void* s, const double rk[6], int ell, int ntau, const double* taus, double* out)
+{
+    Scene* sc = (Scene*)s;
+    Path p;
+    std::memcpy(p.r, rk, 3 * sizeof(double));
+    std::memcpy(p.k, rk + 3, 3 * sizeof(double));
+    WalkStats st;
+    sc->grid->path(p, st);
+    double ke = sc->kext[ell];
+    const double* rho = sc->rho.data();
+    p.fillOpticalDepth([&](int m) { double res = 0; if (m >= 0) res += ke * rho[m]; return res; });
+    for (int i = 0; i < ntau; i++) out[i] = p.pathlength(taus[i]);
+}
+
+// MonteCarloSimulation.cpp:68-114 (chunking), :241-255 (run), Parallel.cpp:169-199 (claiming)
+int or_run_reference(void* s, const double* mc8, int threads, double* frames, double* labs, uint64_t* counters)
+{
+    Scene* sc = (Scene*)s;
+    MCParams mp;
+    parse_mc(mc8, mp);
+    int T = std::max(1, threads);
+    int Nl = sc->nlambda;
+    int64_t totalChunks = (T == 1) ? 1 : (int64_t)std::ceil(std::max(10. * T / Nl, mp.packages / 1e7));
+    uint64_t chunksize = (uint64_t)std::ceil(mp.packages / totalChunks);
+    uint64_t Npp = totalChunks * chunksize;
+    std::vector<double> Ltot = total_lum(*sc);
+    Tally t = make_tally(*sc, frames, labs);
+    std::vector<MT> mts(T);
+    for (int i = 0; i < T; i++) mts[i].seed(mp.seed + i);
+    std::atomic<int64_t> next(0);
+    std::atomic<int> failed(0);
+    std::vector<Counters> cs(T);
+    int64_t limit = totalChunks * Nl;
+    auto worker = [&](int tid) {
+        Rng rng;
+        rng.use_mt = true;
+        rng.mt = &mts[tid];
+        Path path, ppath;
+        while (!failed) {
+            int64_t index = next++;
+            if (index >= limit) break;
+            int ell = (int)(index % Nl);
+            double L = Ltot[ell] / Npp;
+            if (!(L > 0)) continue;
+            double Lthreshold = L / mp.minWeightReduction;
+            for (uint64_t i = 0; i < chunksize; i++)
+                if (one_packet(*sc, mp, ell, L, Lthreshold, rng, Ltot, path, ppath, t, cs[tid]) != 0) { failed = 1; break; }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int i = 1; i < T; i++) th.emplace_back(worker, i);
+    worker(0);
+    for (auto& x : th) x.join();
+    for (int i = 0; i < 8; i++) counters[i] = 0;
+    for (auto& c : cs) fold(counters, c, Npp);
+    return failed ? -1 : 0;
+}
+
+int or_run_packets(void* s, const double* mc8, int ell, uint64_t first, uint64_t count, uint64_t npp, int threads,
+                   double* frames, double* labs, uint64_t* counters)
+{
+    Scene* sc = (Scene*)s;
+    MCParams mp;
+    parse_mc(mc8, mp);
+    int T = std::max(1, threads);
+    std::vector<double> Ltot = total_lum(*sc);
+    Tally t = make_tally(*sc, frames, labs);
+    double L = Ltot[ell] / (double)npp;
+    double Lthreshold = L / mp.minWeightReduction;
+    std::atomic<uint64_t> next(0);
+    std::atomic<int> failed(0);
+    std::vector<Counters> cs(T);
+    auto worker = [&](int tid) {
+        Rng rng;
+        rng.use_mt = false;
+        rng.key[0] = mp.seed;
+        rng.key[1] = mp.member;
+        rng.ell = (uint32_t)ell;
+        Path path, ppath;
+        while (!failed) {
+            uint64_t i0 = next.fetch_add(256);
+            if (i0 >= count) break;
+            uint64_t i1 = std::min(count, i0 + 256);
+            for (uint64_t i = i0; i < i1; i++) {
+                rng.start_packet(first + i);
+                if (one_packet(*sc, mp, ell, L, Lthreshold, rng, Ltot, path, ppath, t, cs[tid]) != 0) { failed = 1; break; }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int i = 1; i < T; i++) th.emplace_back(worker, i);
+    worker(0);
+    for (auto& x : th) x.join();
+    for (int i = 0; i < 8; i++) counters[i] = 0;
+    for (auto& c : cs) fold(counters, c, npp);
+    return failed ? -1 : 0;
+}
+
+double or_mt_first_uniforms(uint32_t seed, int n, double* out)
+{
+    MT mt;
+    mt.seed(seed);
+    for (int i = 0; i < n; i++) out[i] = mt.uniform();
+    return n ? out[0] : 0;
+}
+
+void or_mt_first_words(uint32_t seed, int n, uint32_t* out)
+{
+    MT mt;
+    mt.seed(seed);
+    for (int i = 0; i < n; i++) out[i] = mt.next();
+}
+
+void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
+
+double or_lambertw1(double z) { return lambertw1(z); }
+
+int or_nr_locate_clip(const double* xv, int n, double x) { return nr_locate_clip(xv, n, x); }
+int or_nr_locate_fail(const double* xv, int n, double x) { return nr_locate_fail(xv, n, x); }
+int or_nr_locate_tau(const double* taus, int n, double x)
+{
+    Path p;
+    for (int i = 0; i < n; i++) p.v.push_back(Segment{0, 0, 0, 0, taus[i]});
+    return p.locateTau(x);
+}
+double or_nr_interp_loglog(double x, double x1, double x2, double f1, double f2)
+{
+    return nr_loglog(x, x1, x2, f1, f2);
+}
+
+void or_sersic_table(double n, double* s101, double* M101)
+{
+    std::vector<double> sv, Mv;
+    sersic_table(n, sv, Mv);
+    std::memcpy(s101, sv.data(), 101 * sizeof(double));
+    std::memcpy(M101, Mv.data(), 101 * sizeof(double));
+}
+
+void or_hg_sample(double g, const double k[3], double u1, double u2, double out[3])
+{
+    // DustMix.cpp:597-601 then Random.cpp:189-223 with the two deviates given explicitly
+    struct Fixed { double u[2]; int i = 0; } f{{u1, u2}};
+    double fv = ((1.0 - g) * (1.0 + g)) / (1.0 - g + 2.0 * g * f.u[0]);
+    double costheta = (1.0 + g * g - fv * fv) / (2.0 * g);
+    double phi = 2.0 * M_PI * f.u[1];
+    double cosphi = std::cos(phi), sinphi = std::sin(phi);
+    double sintheta = std::sqrt(std::fabs((1.0 - costheta) * (1.0 + costheta)));
+    double kx = k[0], ky = k[1], kz = k[2];
+    if (kz > 0.99999) { out[0] = cosphi * sintheta; out[1] = sinphi * sintheta; out[2] = costheta; }
+    else if (kz < -0.99999) { out[0] = cosphi * sintheta; out[1] = sinphi * sintheta; out[2] = -costheta; }
+    else {
+        double root = std::sqrt((1.0 - kz) * (1.0 + kz));
+        out[0] = sintheta / root * (-kx * kz * cosphi + ky * sinphi) + kx * costheta;
+        out[1] = -sintheta / root * (ky * kz * cosphi + kx * sinphi) + ky * costheta;
+        out[2] = root * sintheta * cosphi + kz * costheta;
+    }
+}
+
+double or_hg_value(double g, const double k[3], const double kobs[3])
+{
+    double cosalpha = k[0] * kobs[0] + k[1] * kobs[1] + k[2] * kobs[2];
+    double t = 1.0 + g * g - 2 * g * cosalpha;
+    return (1.0 - g) * (1.0 + g) / std::sqrt(t * t * t);
+}
+
+} // extern "C"

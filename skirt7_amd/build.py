@@ -1,0 +1,35 @@
+"""Builds libskirt7_amd.so for gfx950 in-tree (skirt7_amd/lib/).
+
+All HIP code is compiled with -ffp-contract=off, so the FP64 walks repeat the reference's
+arithmetic without fused multiply-adds, and with packed FP32 instructions disabled.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = [os.path.join(HERE, "csrc", f) for f in ("sg_kernels.hip", "sg_build.cpp")]
+OUT = os.path.join(HERE, "lib", "libskirt7_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
+         "-Wno-unused-command-line-argument"]
+
+
+def build(verbose=False):
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    tmp = OUT + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-o", tmp] + SRC + ["-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed for libskirt7_amd.so")
+    if verbose:
+        sys.stderr.write("\n".join(l for l in (r.stdout + r.stderr).splitlines() if "error" in l or "warning: " in l
+                                   and "packed-fp32" not in l))
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,192 @@
+// sg_common.hpp -- device-side data views and small numerics shared by all kernels.
+//
+// Layouts in HBM (all read-only during transport except the tallies):
+//   Cartesian : three edge arrays (FP64 and an FP32 copy) and rho[m], m = k + Nz*j + Nz*Ny*i.
+//   Tree      : node words in breadth-first id order (>= 0 first child id, < 0 leaf: -(m+1));
+//               one 64-byte record per cell in each precision (bounds, rho, first neighbour
+//               per wall); the full neighbour lists per (cell, wall) in CSR form.
+//   Voronoi   : one 32-byte record per site (x, y, z, rho); neighbour lists in CSR form
+//               with walls -1..-6; a uniform bucket grid over the sites for cellIndex.
+//   Tallies   : FP64, instrument frames [ell][pixel], absorption [ell][m].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+
+#define SG_MAX_INSTR 8
+#define SG_MAX_STARS 4
+#define SG_LDS_NODES 4096
+#define SG_MAX_STEPS 4000000
+
+namespace sg {
+
+enum { WALL_BACK = 0, WALL_FRONT, WALL_LEFT, WALL_RIGHT, WALL_BOTTOM, WALL_TOP };
+
+// ------------------------------------------------------------------------- grid views
+
+struct CartView {
+    const double* xv; const double* yv; const double* zv;
+    const float* xvf; const float* yvf; const float* zvf;
+    const double* rho; const float* rhof;
+    int Nx, Ny, Nz;
+    double xmin, ymin, zmin, xmax, ymax, zmax;
+};
+
+struct alignas(16) TreeRec32 {       // 64 bytes
+    float lo[3], hi[3];
+    float rho;
+    int32_t nb[6];                    // first neighbour cell across each wall (-1: none)
+    int32_t node;                     // node id of this leaf
+    int32_t pad[2];
+};
+struct alignas(16) TreeRec64 {       // 64 bytes
+    double lo[3], hi[3];
+    double rho;
+    int32_t node;
+    int32_t pad;
+};
+
+struct TreeView {
+    const int32_t* node;             // breadth-first node words
+    const TreeRec64* rec64;
+    const TreeRec32* rec32;
+    const int32_t* nboff;            // CSR offsets, index cell*6 + wall
+    const int32_t* nbids;            // neighbour cell numbers
+    int32_t nnodes, ncells;
+    int arity;                       // 8 octree, 2 alternating binary tree
+    int search;                      // 0 TopDown, 1 Neighbor
+    double ext[6];
+    float extf[6];
+    double eps;
+};
+
+struct alignas(16) VoroSite { double x, y, z, rho; };
+
+struct VoroView {
+    const VoroSite* site;
+    const int32_t* off;              // CSR offsets, nsites + 1
+    const int32_t* nb;               // neighbour ids, walls -1..-6
+    const int32_t* boff;             // bucket offsets, nbk^3 + 1
+    const int32_t* bids;             // site ids per bucket
+    int32_t nsites;
+    int nbk;
+    double ext[6];
+    double eps;
+};
+
+// ------------------------------------------------------------------------- scene records
+
+struct InstrDev {
+    double kobs[3];
+    double costheta, sintheta, cosphi, sinphi, cospa, sinpa;
+    double xpmin, xpsiz, ypmin, ypsiz;
+    int Nx, Ny;
+    double* frame;                   // [nlambda][Ny*Nx]
+};
+
+struct StarDev {
+    int kind;                        // 1 exponential disc, 2 Sersic
+    int spiral;
+    double p[6];                     // disc: hR hz Rmax zmax Rmin | Sersic: n reff
+    double sp[6];                    // arms pitch R0 phi0 weight index
+    double tanp, CN;
+    const double* ts;                // Sersic table radii (101)
+    const double* tM;                // Sersic table masses (101)
+    int nt;
+    double lum;                      // luminosity at the run's wavelength
+};
+
+struct McArgs {
+    uint64_t first, count;
+    double Lpkt, Lthreshold;
+    int ell, nlambda;
+    double kext, albedo, g;
+    int minfs;
+    double xi;
+    uint32_t seed, member;
+    double tau_cut;
+    int store_abs;
+    double* labs;                    // [ell][m] when stored
+    int64_t ncells;
+    int has_dust;
+    int ni;
+    InstrDev instr[SG_MAX_INSTR];
+    int ns;
+    StarDev star[SG_MAX_STARS];
+    double emission_bias;
+    double Xcum[SG_MAX_STARS + 1];   // normalised cumulative luminosity
+    double Lmean;
+    unsigned long long* counters;    // SG_NCOUNTERS
+    unsigned long long* next_packet; // work counter
+    int* error;
+    int compact;
+};
+
+// ------------------------------------------------------------------------- Philox4x32-10
+
+__device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
+                                         uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+}
+
+// Per-packet stream: key (seed, member), counter (packet lo, packet hi, draw/4, ell); each
+// Philox block yields four consecutive draws.
+struct PacketRng {
+    uint32_t k0, k1, plo, phi, ell, draw;
+    uint32_t w0, w1, w2, w3;
+    __device__ void start(uint32_t seed, uint32_t member, uint64_t pkt, int l)
+    {
+        k0 = seed; k1 = member; plo = (uint32_t)pkt; phi = (uint32_t)(pkt >> 32); ell = (uint32_t)l; draw = 0;
+    }
+    __device__ __forceinline__ uint32_t word()
+    {
+        uint32_t q = draw & 3u;
+        if (q == 0) {
+            w0 = plo; w1 = phi; w2 = draw >> 2; w3 = ell;
+            philox10(w0, w1, w2, w3, k0, k1);
+        }
+        draw++;
+        return q == 0 ? w0 : q == 1 ? w1 : q == 2 ? w2 : w3;
+    }
+    __device__ __forceinline__ double ud() { return ((double)word() + 0.5) * 2.3283064365386962890625e-10; }
+    __device__ __forceinline__ float uf() { return ((float)(word() >> 9) + 0.5f) * 1.1920928955078125e-07f; }
+};
+
+// ------------------------------------------------------------------------- small numerics
+
+__device__ __forceinline__ bool box_contains(const double* b, double x, double y, double z)
+{
+    return x >= b[0] && x <= b[3] && y >= b[1] && y <= b[4] && z >= b[2] && z <= b[5];
+}
+
+__device__ __forceinline__ double interp_linlin(double x, double x1, double x2, double f1, double f2)
+{
+    return f1 + ((x - x1) / (x2 - x1)) * (f2 - f1);
+}
+__device__ __forceinline__ float interp_linlin(float x, float x1, float x2, float f1, float f2)
+{
+    return f1 + ((x - x1) / (x2 - x1)) * (f2 - f1);
+}
+
+// Binary search of NR::locate_clip semantics (x below the first border -> bin 0).
+template <typename T> __device__ __forceinline__ int locate_clip(const T* xv, int size, T x)
+{
+    if (x < xv[0]) return 0;
+    int jl = -1, ju = size - 1;
+    while (ju - jl > 1) {
+        int jm = (ju + jl) >> 1;
+        if (x < xv[jm]) ju = jm;
+        else jl = jm;
+    }
+    return jl;
+}
+
+} // namespace sg

@@ -1,0 +1,809 @@
+// sg_kernels.hip -- kernels and the C-ABI of include/skirt7_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/skirt7_amd.h"
+#include "sg_host.hpp"
+#include "sg_mc.hpp"
+
+namespace sg {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+static int g_device = -1;
+static int g_cus = 256;
+static int g_tune_block = 256, g_tune_waves = 0, g_tune_compact = 16;
+
+static void ensure_init()
+{
+    if (g_device >= 0) return;
+    if (sg_init(0) != 0) throw Fail{-3};
+}
+
+// ------------------------------------------------------------------------- probe kernels
+
+// DustGrid::path for a batch of rays: the segment list of each ray.
+template <class W>
+__global__ void __launch_bounds__(256) k_paths(const typename W::View g, int64_t n, const double* rk, int maxseg,
+                                                int32_t* counts, int32_t* mo, double* dso)
+{
+    __shared__ int32_t s_nodes[SG_LDS_NODES];
+    int nlds;
+    stage_tree_nodes(g, s_nodes, nlds);
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* q = rk + 6 * i;
+    W w;
+    w.nrecompute = 0; w.nstuck = 0;
+    w.init(g, s_nodes, nlds, q[0], q[1], q[2], q[3], q[4], q[5]);
+    int c = 0, m;
+    typename W::Real ds, rho;
+    while (w.next(g, s_nodes, nlds, m, ds, rho)) {
+        if (c < maxseg) { mo[i * maxseg + c] = m; dso[i * maxseg + c] = (double)ds; }
+        c++;
+    }
+    counts[i] = c <= maxseg ? c : -1;
+}
+
+// DustSystem::opticaldepth (mode 0) or DustGridPath::pathlength (mode 1) for a batch of rays.
+template <class W>
+__global__ void __launch_bounds__(256) k_tau(const typename W::View g, int64_t n, const double* rk, double kext,
+                                              const double* taus, int mode, double* out)
+{
+    using R = typename W::Real;
+    __shared__ int32_t s_nodes[SG_LDS_NODES];
+    int nlds;
+    stage_tree_nodes(g, s_nodes, nlds);
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* q = rk + 6 * i;
+    W w;
+    w.nrecompute = 0; w.nstuck = 0;
+    w.init(g, s_nodes, nlds, q[0], q[1], q[2], q[3], q[4], q[5]);
+    int m;
+    R ds, rho;
+    R tau = 0, s = 0, tprev = 0, sprev = 0;
+    int nseg = 0;
+    if (mode == 0) {
+        while (w.next(g, s_nodes, nlds, m, ds, rho)) tau += ((R)kext * rho) * ds;
+        out[i] = (double)tau;
+        return;
+    }
+    R target = (R)taus[i];
+    bool found = false;
+    R res = 0;
+    while (w.next(g, s_nodes, nlds, m, ds, rho)) {
+        tprev = tau; sprev = s;
+        s += ds;
+        tau += ((R)kext * rho) * ds;
+        nseg++;
+        if (target > (R)0 && tau > target) {
+            res = nseg == 1 ? interp_linlin(target, (R)0, tau, (R)0, s) : interp_linlin(target, tprev, tau, sprev, s);
+            found = true;
+            break;
+        }
+    }
+    if (!found) {
+        if (nseg == 0 || !(target > (R)0)) res = 0;
+        else if (target > tau) res = s;
+        else if (nseg == 1) res = s;
+        else res = interp_linlin(target, tprev, tau, sprev, s);
+    }
+    out[i] = (double)res;
+}
+
+__global__ void k_whichcell_cart(const CartView g, int64_t n, const double* r, int32_t* out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = r[3 * i], y = r[3 * i + 1], z = r[3 * i + 2];
+    // CartesianDustGrid::whichcell (CartesianDustGrid.cpp:109-118) with NR::locate_fail
+    auto fail = [](const double* v, int size, double q) {
+        if (q > v[size - 1]) return -1;
+        int jl = -1, ju = size - 1;
+        while (ju - jl > 1) {
+            int jm = (ju + jl) >> 1;
+            if (q < v[jm]) ju = jm;
+            else jl = jm;
+        }
+        return jl;
+    };
+    int a = fail(g.xv, g.Nx + 1, x), b = fail(g.yv, g.Ny + 1, y), c = fail(g.zv, g.Nz + 1, z);
+    out[i] = (a < 0 || b < 0 || c < 0) ? -1 : c + g.Nz * b + g.Nz * g.Ny * a;
+}
+__global__ void k_whichcell_tree(const TreeView t, int64_t n, const double* r, int32_t* out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double lo[3], hi[3];
+    out[i] = descend64(t, t.node, 0, r[3 * i], r[3 * i + 1], r[3 * i + 2], lo, hi);
+}
+__global__ void k_whichcell_voro(const VoroView g, int64_t n, const double* r, int32_t* out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = voro_nearest(g, r[3 * i], r[3 * i + 1], r[3 * i + 2]);
+}
+
+} // namespace sg
+
+using namespace sg;
+
+// ========================================================================= C-ABI
+
+#define SG_TRY try {
+#define SG_CATCH                                                                                     \
+    }                                                                                                \
+    catch (const sg::Fail& f) { return f.code; }                                                     \
+    catch (const std::exception& e) { sg::set_error(e.what()); return -4; }                          \
+    return 0;
+
+extern "C" {
+
+const char* sg_last_error(void) { return sg::g_err.c_str(); }
+
+int sg_init(int device)
+{
+    SG_TRY
+    int n = 0;
+    SG_HIP(hipGetDeviceCount(&n));
+    SG_REQUIRE(n > device && device >= 0, "no such HIP device");
+    SG_HIP(hipSetDevice(device));
+    hipDeviceProp_t p;
+    SG_HIP(hipGetDeviceProperties(&p, device));
+    SG_REQUIRE(std::string(p.gcnArchName).find("gfx950") != std::string::npos,
+               std::string("this library is built for gfx950, found ") + p.gcnArchName);
+    g_cus = p.multiProcessorCount;
+    g_device = device;
+    SG_CATCH
+}
+
+int sg_device_name(char* buf, int len)
+{
+    SG_TRY
+    ensure_init();
+    hipDeviceProp_t p;
+    SG_HIP(hipGetDeviceProperties(&p, g_device));
+    std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    SG_CATCH
+}
+
+int sg_set_tuning(int block, int waves_per_cu, int compact_lanes)
+{
+    if (block == 64 || block == 128 || block == 256) g_tune_block = block;
+    if (waves_per_cu > 0) g_tune_waves = waves_per_cu;
+    if (compact_lanes > 0 && compact_lanes <= 64) g_tune_compact = compact_lanes;
+    return 0;
+}
+
+// ---------------------------------------------------------------- grids
+
+int sg_grid_create_cartesian(const double ext[6], const double* xv, int nxv, const double* yv, int nyv,
+                             const double* zv, int nzv, sg_grid** out)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(nxv >= 2 && nyv >= 2 && nzv >= 2, "a Cartesian grid needs at least one bin per axis");
+    SG_REQUIRE((int64_t)(nxv - 1) * (nyv - 1) * (nzv - 1) < (int64_t)1 << 31, "too many cells");
+    auto* g = new sg_grid();
+    g->kind = GK_CART;
+    g->ncells = (int64_t)(nxv - 1) * (nyv - 1) * (nzv - 1);
+    std::vector<float> f;
+    g->xv.upload(xv, nxv); g->yv.upload(yv, nyv); g->zv.upload(zv, nzv);
+    f.assign(xv, xv + nxv); g->xvf.upload(f);
+    f.assign(yv, yv + nyv); g->yvf.upload(f);
+    f.assign(zv, zv + nzv); g->zvf.upload(f);
+    g->rho.alloc(g->ncells * sizeof(double)); g->rho.zero();
+    g->rhof.alloc(g->ncells * sizeof(float)); g->rhof.zero();
+    CartView& c = g->cart;
+    c.xv = g->xv.as<double>(); c.yv = g->yv.as<double>(); c.zv = g->zv.as<double>();
+    c.xvf = g->xvf.as<float>(); c.yvf = g->yvf.as<float>(); c.zvf = g->zvf.as<float>();
+    c.rho = g->rho.as<double>(); c.rhof = g->rhof.as<float>();
+    c.Nx = nxv - 1; c.Ny = nyv - 1; c.Nz = nzv - 1;
+    c.xmin = ext[0]; c.ymin = ext[1]; c.zmin = ext[2]; c.xmax = ext[3]; c.ymax = ext[4]; c.zmax = ext[5];
+    *out = g;
+    SG_CATCH
+}
+
+int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivided, int64_t nnodes, int search,
+                        sg_grid** out)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(arity == 8 || arity == 2, "arity must be 8 (octree) or 2 (binary tree)");
+    SG_REQUIRE(search == SG_TOPDOWN || search == SG_NEIGHBOR, "search method must be TopDown or Neighbor");
+    SG_REQUIRE(nnodes >= 1 && nnodes < (int64_t)1 << 31, "bad node count");
+    auto* g = new sg_grid();
+    g->kind = GK_TREE;
+    HostTree& h = g->ht;
+    if (!h.build(arity, ext, subdivided, nnodes)) { delete g; throw Fail{-1}; }
+    if (search == SG_NEIGHBOR) h.build_neighbors();
+    else h.nb.assign(6 * h.child0.size(), {});
+    int64_t nn = (int64_t)h.child0.size(), nc = (int64_t)h.idv.size();
+    g->ncells = nc;
+    std::vector<int32_t> words(nn);
+    for (int64_t l = 0; l < nn; l++) words[l] = h.child0[l] >= 0 ? h.child0[l] : -(h.cellnumber[l] + 1);
+    g->node.upload(words);
+    g->hrec64.assign(nc, TreeRec64{});
+    g->hrec32.assign(nc, TreeRec32{});
+    std::vector<int32_t> off(nc * 6 + 1, 0), ids;
+    for (int64_t c = 0; c < nc; c++) {
+        int l = h.idv[c];
+        TreeRec64& r = g->hrec64[c];
+        TreeRec32& f = g->hrec32[c];
+        for (int a = 0; a < 3; a++) {
+            r.lo[a] = h.bounds[6 * l + a]; r.hi[a] = h.bounds[6 * l + 3 + a];
+            f.lo[a] = (float)r.lo[a]; f.hi[a] = (float)r.hi[a];
+        }
+        r.rho = 0; r.node = l; f.rho = 0; f.node = l;
+        for (int w = 0; w < 6; w++) {
+            const auto& lst = h.nb[6 * (size_t)l + w];
+            off[c * 6 + w] = (int32_t)ids.size();
+            for (int32_t nbn : lst) ids.push_back(h.cellnumber[nbn]);
+            f.nb[w] = lst.empty() ? -1 : h.cellnumber[lst[0]];
+        }
+    }
+    off[nc * 6] = (int32_t)ids.size();
+    SG_REQUIRE(ids.size() < ((size_t)1 << 31), "neighbour lists too long");
+    g->nboff.upload(off);
+    if (ids.empty()) ids.push_back(-1);
+    g->nbids.upload(ids);
+    g->rec64.upload(g->hrec64);
+    g->rec32.upload(g->hrec32);
+    TreeView& t = g->tree;
+    t.node = g->node.as<int32_t>();
+    t.rec64 = g->rec64.as<TreeRec64>();
+    t.rec32 = g->rec32.as<TreeRec32>();
+    t.nboff = g->nboff.as<int32_t>();
+    t.nbids = g->nbids.as<int32_t>();
+    t.nnodes = (int32_t)nn; t.ncells = (int32_t)nc;
+    t.arity = arity; t.search = search;
+    for (int i = 0; i < 6; i++) { t.ext[i] = ext[i]; t.extf[i] = (float)ext[i]; }
+    t.eps = h.eps;
+    // the neighbour lists are only needed on the host while building records
+    std::vector<std::vector<int32_t>>().swap(h.nb);
+    *out = g;
+    SG_CATCH
+}
+
+int sg_grid_create_voronoi(const double ext[6], const double* sites, int64_t n, const int64_t* nb_offsets,
+                           const int32_t* nb_ids, sg_grid** out)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(n >= 1 && n < (int64_t)1 << 31, "bad site count");
+    SG_REQUIRE(nb_offsets[n] < (int64_t)1 << 31, "neighbour lists too long");
+    auto* g = new sg_grid();
+    g->kind = GK_VORO;
+    g->ncells = n;
+    g->hsite.resize(n);
+    for (int64_t i = 0; i < n; i++) g->hsite[i] = VoroSite{sites[3 * i], sites[3 * i + 1], sites[3 * i + 2], 0.0};
+    g->site.upload(g->hsite);
+    std::vector<int32_t> off(n + 1);
+    for (int64_t i = 0; i <= n; i++) off[i] = (int32_t)nb_offsets[i];
+    g->voff.upload(off);
+    std::vector<int32_t> ids(nb_ids, nb_ids + nb_offsets[n]);
+    for (int32_t v : ids) SG_REQUIRE(v >= -6 && v < n && v != -99, "neighbour id out of range");
+    if (ids.empty()) ids.push_back(-1);
+    g->vnb.upload(ids);
+    // bucket grid for the nearest-site search: about 2 sites per bucket
+    int nbk = std::max(1, std::min(256, (int)std::cbrt((double)n / 2.0)));
+    std::vector<int32_t> cnt((size_t)nbk * nbk * nbk + 1, 0), bucket(n);
+    auto bidx = [&](double x, int a) {
+        int i = (int)(nbk * (x - ext[a]) / (ext[a + 3] - ext[a]));
+        return std::max(0, std::min(nbk - 1, i));
+    };
+    for (int64_t i = 0; i < n; i++) {
+        int q = (bidx(g->hsite[i].x, 0) * nbk + bidx(g->hsite[i].y, 1)) * nbk + bidx(g->hsite[i].z, 2);
+        bucket[i] = q;
+        cnt[q + 1]++;
+    }
+    for (size_t q = 1; q < cnt.size(); q++) cnt[q] += cnt[q - 1];
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1), bid(n);
+    for (int64_t i = 0; i < n; i++) bid[fill[bucket[i]]++] = (int32_t)i;
+    g->boff.upload(cnt);
+    g->bids.upload(bid);
+    VoroView& v = g->voro;
+    v.site = g->site.as<VoroSite>();
+    v.off = g->voff.as<int32_t>();
+    v.nb = g->vnb.as<int32_t>();
+    v.boff = g->boff.as<int32_t>();
+    v.bids = g->bids.as<int32_t>();
+    v.nsites = (int32_t)n;
+    v.nbk = nbk;
+    for (int i = 0; i < 6; i++) v.ext[i] = ext[i];
+    double wx = ext[3] - ext[0], wy = ext[4] - ext[1], wz = ext[5] - ext[2];
+    v.eps = 1e-12 * std::sqrt(wx * wx + wy * wy + wz * wz);
+    *out = g;
+    SG_CATCH
+}
+
+int sg_grid_destroy(sg_grid* g)
+{
+    delete g;
+    return 0;
+}
+
+int sg_grid_num_cells(const sg_grid* g, int64_t* n)
+{
+    *n = g->ncells;
+    return 0;
+}
+
+void sg_grid::set_rho(const double* r)
+{
+    if (kind == GK_CART) {
+        SG_HIP(hipMemcpy(rho.p, r, ncells * sizeof(double), hipMemcpyHostToDevice));
+        std::vector<float> f(r, r + ncells);
+        SG_HIP(hipMemcpy(rhof.p, f.data(), ncells * sizeof(float), hipMemcpyHostToDevice));
+    } else if (kind == GK_TREE) {
+        for (int64_t c = 0; c < ncells; c++) { hrec64[c].rho = r[c]; hrec32[c].rho = (float)r[c]; }
+        SG_HIP(hipMemcpy(rec64.p, hrec64.data(), ncells * sizeof(TreeRec64), hipMemcpyHostToDevice));
+        SG_HIP(hipMemcpy(rec32.p, hrec32.data(), ncells * sizeof(TreeRec32), hipMemcpyHostToDevice));
+    } else {
+        for (int64_t c = 0; c < ncells; c++) hsite[c].rho = r[c];
+        SG_HIP(hipMemcpy(site.p, hsite.data(), ncells * sizeof(VoroSite), hipMemcpyHostToDevice));
+    }
+}
+
+// Host helper: copies a batch of n records to the device and back.
+struct Batch {
+    DevBuf in, in2, o1, o2, o3;
+};
+
+static int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+int sg_grid_whichcell(sg_grid* g, int64_t n, const double* r, int32_t* m)
+{
+    SG_TRY
+    ensure_init();
+    if (n <= 0) return 0;
+    Batch b;
+    b.in.upload(r, 3 * n);
+    b.o1.alloc(n * sizeof(int32_t));
+    int nb = blocks_for(n, 256);
+    if (g->kind == GK_CART) hipLaunchKernelGGL(k_whichcell_cart, dim3(nb), dim3(256), 0, 0, g->cart, n, b.in.as<double>(), b.o1.as<int32_t>());
+    else if (g->kind == GK_TREE) hipLaunchKernelGGL(k_whichcell_tree, dim3(nb), dim3(256), 0, 0, g->tree, n, b.in.as<double>(), b.o1.as<int32_t>());
+    else hipLaunchKernelGGL(k_whichcell_voro, dim3(nb), dim3(256), 0, 0, g->voro, n, b.in.as<double>(), b.o1.as<int32_t>());
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipMemcpy(m, b.o1.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    SG_CATCH
+}
+
+} // extern "C"
+template <class W>
+static void launch_paths(const typename W::View& v, int64_t n, const double* rk, int maxseg, int32_t* c, int32_t* m,
+                         double* ds)
+{
+    hipLaunchKernelGGL(k_paths<W>, dim3(blocks_for(n, 256)), dim3(256), 0, 0, v, n, rk, maxseg, c, m, ds);
+    SG_HIP(hipGetLastError());
+}
+extern "C" {
+
+int sg_grid_paths(sg_grid* g, int64_t n, const double* rk, int max_seg, int precision, int32_t* counts, int32_t* m,
+                  double* ds)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(max_seg > 0, "max_seg must be positive");
+    if (n <= 0) return 0;
+    SG_REQUIRE((double)n * max_seg * 12.0 < 64e9, "output too large");
+    Batch b;
+    b.in.upload(rk, 6 * n);
+    b.o1.alloc(n * sizeof(int32_t));
+    b.o2.alloc((size_t)n * max_seg * sizeof(int32_t));
+    b.o3.alloc((size_t)n * max_seg * sizeof(double));
+    const double* d = b.in.as<double>();
+    int32_t* oc = b.o1.as<int32_t>();
+    int32_t* om = b.o2.as<int32_t>();
+    double* od = b.o3.as<double>();
+    bool mixed = precision == SG_MIXED;
+    if (g->kind == GK_CART) {
+        if (mixed) launch_paths<CartW32>(g->cart, n, d, max_seg, oc, om, od);
+        else launch_paths<CartW64>(g->cart, n, d, max_seg, oc, om, od);
+    } else if (g->kind == GK_TREE) {
+        if (!mixed) launch_paths<TreeW64>(g->tree, n, d, max_seg, oc, om, od);
+        else if (g->tree.search == SG_NEIGHBOR) launch_paths<TreeW32>(g->tree, n, d, max_seg, oc, om, od);
+        else launch_paths<TreeW32TopDown>(g->tree, n, d, max_seg, oc, om, od);
+    } else {
+        if (mixed) launch_paths<VoroW<float>>(g->voro, n, d, max_seg, oc, om, od);
+        else launch_paths<VoroW<double>>(g->voro, n, d, max_seg, oc, om, od);
+    }
+    SG_HIP(hipDeviceSynchronize());
+    SG_HIP(hipMemcpy(counts, oc, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    SG_HIP(hipMemcpy(m, om, (size_t)n * max_seg * sizeof(int32_t), hipMemcpyDeviceToHost));
+    SG_HIP(hipMemcpy(ds, od, (size_t)n * max_seg * sizeof(double), hipMemcpyDeviceToHost));
+    SG_CATCH
+}
+
+// ---------------------------------------------------------------- dust
+
+int sg_dust_create(sg_grid* g, const double* rho, int64_t ncells, int nlambda, const double* kappaext,
+                   const double* albedo, const double* asymmpar, int store_absorption, sg_dust** out)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(ncells == g->ncells, "density array size differs from the number of cells");
+    SG_REQUIRE(nlambda >= 1, "need at least one wavelength");
+    for (int64_t i = 0; i < ncells; i++)
+        SG_REQUIRE(rho[i] >= 0 && std::isfinite(rho[i]), "dust density must be finite and non-negative");
+    auto* d = new sg_dust();
+    d->grid = g;
+    d->nlambda = nlambda;
+    d->kext.assign(kappaext, kappaext + nlambda);
+    d->albedo.assign(albedo, albedo + nlambda);
+    d->g.assign(asymmpar, asymmpar + nlambda);
+    d->store_abs = store_absorption != 0;
+    g->set_rho(rho);
+    if (d->store_abs) {
+        d->labs.alloc((size_t)nlambda * ncells * sizeof(double));
+        d->labs.zero();
+    }
+    *out = d;
+    SG_CATCH
+}
+
+int sg_dust_destroy(sg_dust* d)
+{
+    delete d;
+    return 0;
+}
+
+} // extern "C"
+template <class W>
+static void launch_tau(const typename W::View& v, int64_t n, const double* rk, double kext, const double* taus,
+                       int mode, double* out)
+{
+    hipLaunchKernelGGL(k_tau<W>, dim3(blocks_for(n, 256)), dim3(256), 0, 0, v, n, rk, kext, taus, mode, out);
+    SG_HIP(hipGetLastError());
+}
+
+static int tau_common(sg_dust* d, int ell, int64_t n, const double* rk, const double* taus, int precision, int mode,
+                      double* out)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(ell >= 0 && ell < d->nlambda, "wavelength index out of range");
+    if (n <= 0) return 0;
+    Batch b;
+    b.in.upload(rk, 6 * n);
+    if (taus) b.in2.upload(taus, n);
+    b.o1.alloc(n * sizeof(double));
+    const double* ri = b.in.as<double>();
+    const double* ti = taus ? b.in2.as<double>() : nullptr;
+    double* o = b.o1.as<double>();
+    sg_grid* g = d->grid;
+    bool mixed = precision == SG_MIXED;
+    double ke = d->kext[ell];
+    if (g->kind == GK_CART) {
+        if (mixed) launch_tau<CartW32>(g->cart, n, ri, ke, ti, mode, o);
+        else launch_tau<CartW64>(g->cart, n, ri, ke, ti, mode, o);
+    } else if (g->kind == GK_TREE) {
+        if (!mixed) launch_tau<TreeW64>(g->tree, n, ri, ke, ti, mode, o);
+        else if (g->tree.search == SG_NEIGHBOR) launch_tau<TreeW32>(g->tree, n, ri, ke, ti, mode, o);
+        else launch_tau<TreeW32TopDown>(g->tree, n, ri, ke, ti, mode, o);
+    } else {
+        if (mixed) launch_tau<VoroW<float>>(g->voro, n, ri, ke, ti, mode, o);
+        else launch_tau<VoroW<double>>(g->voro, n, ri, ke, ti, mode, o);
+    }
+    SG_HIP(hipDeviceSynchronize());
+    SG_HIP(hipMemcpy(out, o, n * sizeof(double), hipMemcpyDeviceToHost));
+    SG_CATCH
+}
+extern "C" {
+
+int sg_dust_optical_depths(sg_dust* d, int ell, int64_t n, const double* rk, int precision, double* tau)
+{
+    return tau_common(d, ell, n, rk, nullptr, precision, 0, tau);
+}
+
+int sg_dust_path_lengths(sg_dust* d, int ell, int64_t n, const double* rk, const double* taus, int precision,
+                         double* s)
+{
+    return tau_common(d, ell, n, rk, taus, precision, 1, s);
+}
+
+int sg_dust_read_absorption(sg_dust* d, double* labs)
+{
+    SG_TRY
+    SG_REQUIRE(d->store_abs, "absorption rates are not stored for this dust system");
+    int64_t nc = d->grid->ncells;
+    std::vector<double> t((size_t)d->nlambda * nc);
+    SG_HIP(hipMemcpy(t.data(), d->labs.p, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t m = 0; m < nc; m++)
+        for (int ell = 0; ell < d->nlambda; ell++) labs[m * d->nlambda + ell] = t[(size_t)ell * nc + m];
+    SG_CATCH
+}
+
+int sg_dust_clear_absorption(sg_dust* d)
+{
+    SG_TRY
+    if (d->store_abs) d->labs.zero();
+    SG_CATCH
+}
+
+// ---------------------------------------------------------------- stars
+
+int sg_stars_create(int ncomp, const double* params, int nlambda, const double* lum, const double* sersic_s,
+                    const double* sersic_M, int ntable, double emission_bias, sg_stars** out)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(ncomp >= 1 && ncomp <= SG_MAX_STARS, "between 1 and 4 stellar components are supported");
+    SG_REQUIRE(emission_bias >= 0 && emission_bias <= 1, "emission bias must be in [0,1]");
+    auto* s = new sg_stars();
+    s->nlambda = nlambda;
+    s->params.assign(params, params + ncomp * SG_STAR_NPARAM);
+    s->lum.assign(lum, lum + (size_t)ncomp * nlambda);
+    s->bias = emission_bias;
+    int nser = 0;
+    for (int c = 0; c < ncomp; c++) {
+        int kind = (int)params[c * SG_STAR_NPARAM];
+        SG_REQUIRE(kind == 1 || kind == 2, "stellar geometry must be 1 (exponential disc) or 2 (Sersic)");
+        auto* tb = new DevBuf();
+        s->tables.push_back(tb);
+        if (kind == 2) {
+            SG_REQUIRE(sersic_s && sersic_M && ntable >= 2, "Sersic component needs its mass table");
+            std::vector<double> both;
+            both.insert(both.end(), sersic_s + nser * ntable, sersic_s + (nser + 1) * ntable);
+            both.insert(both.end(), sersic_M + nser * ntable, sersic_M + (nser + 1) * ntable);
+            tb->upload(both);
+            nser++;
+        }
+    }
+    s->ts.resize(ncomp);
+    *out = s;
+    (void)ntable;
+    SG_CATCH
+}
+
+int sg_stars_destroy(sg_stars* s)
+{
+    delete s;
+    return 0;
+}
+
+// ---------------------------------------------------------------- instruments
+
+int sg_instruments_create(int n, const double* params, int nlambda, sg_instruments** out)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(n >= 0 && n <= SG_MAX_INSTR, "at most 8 instruments per simulation");
+    auto* in = new sg_instruments();
+    in->n = n;
+    in->nlambda = nlambda;
+    for (int i = 0; i < n; i++) {
+        const double* p = params + i * SG_INSTR_NPARAM;
+        InstrDev d{};
+        double incl = p[1], azim = p[2], pa = p[3];
+        // DistantInstrument::setupSelfBefore (DistantInstrument.cpp:27-50)
+        d.costheta = std::cos(incl); d.sintheta = std::sin(incl);
+        d.cosphi = std::cos(azim); d.sinphi = std::sin(azim);
+        d.cospa = std::cos(pa); d.sinpa = std::sin(pa);
+        const double eps = 1e-8;
+        if (incl <= eps) { d.kobs[0] = 0; d.kobs[1] = 0; d.kobs[2] = 1; }
+        else if (incl >= M_PI - eps) { d.kobs[0] = 0; d.kobs[1] = 0; d.kobs[2] = -1; }
+        else {
+            double st = std::sin(incl);
+            d.kobs[0] = st * std::cos(azim); d.kobs[1] = st * std::sin(azim); d.kobs[2] = std::cos(incl);
+        }
+        // SingleFrameInstrument::setupSelfBefore (SingleFrameInstrument.cpp:26-42)
+        d.Nx = (int)p[4]; d.Ny = (int)p[5];
+        SG_REQUIRE(d.Nx > 0 && d.Ny > 0 && p[6] > 0 && p[7] > 0, "instrument needs pixels and a field of view");
+        d.xpmin = p[8] - 0.5 * p[6]; d.xpsiz = p[6] / d.Nx;
+        d.ypmin = p[9] - 0.5 * p[7]; d.ypsiz = p[7] / d.Ny;
+        auto* f = new DevBuf();
+        f->alloc((size_t)nlambda * d.Nx * d.Ny * sizeof(double));
+        f->zero();
+        d.frame = f->as<double>();
+        in->frames.push_back(f);
+        in->dev.push_back(d);
+    }
+    *out = in;
+    SG_CATCH
+}
+
+int sg_instruments_destroy(sg_instruments* in)
+{
+    delete in;
+    return 0;
+}
+
+int sg_instruments_read(sg_instruments* in, int i, double* frame)
+{
+    SG_TRY
+    SG_REQUIRE(i >= 0 && i < in->n, "instrument index out of range");
+    SG_HIP(hipMemcpy(frame, in->frames[i]->p, in->frames[i]->bytes, hipMemcpyDeviceToHost));
+    SG_CATCH
+}
+
+int sg_instruments_clear(sg_instruments* in)
+{
+    SG_TRY
+    for (auto* f : in->frames) f->zero();
+    SG_CATCH
+}
+
+// ---------------------------------------------------------------- simulation
+
+int sg_sim_create(sg_dust* dust, sg_stars* stars, sg_instruments* instr, const double* mc, sg_sim** out)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(stars && instr, "a simulation needs a stellar system and an instrument system");
+    SG_REQUIRE(mc[1] > 0, "minWeightReduction must be positive");
+    SG_REQUIRE(mc[3] >= 0 && mc[3] <= 1, "scattBias must be in [0,1]");
+    if (dust) SG_REQUIRE(dust->nlambda == stars->nlambda && dust->nlambda == instr->nlambda,
+                         "dust, stars and instruments disagree on the number of wavelengths");
+    auto* s = new sg_sim();
+    s->dust = dust; s->stars = stars; s->instr = instr;
+    std::memcpy(s->mc, mc, sizeof s->mc);
+    s->counters.alloc(SG_NCOUNTERS * sizeof(unsigned long long));
+    s->next.alloc(sizeof(unsigned long long));
+    s->err.alloc(sizeof(int));
+    *out = s;
+    SG_CATCH
+}
+
+int sg_sim_destroy(sg_sim* s)
+{
+    delete s;
+    return 0;
+}
+} // extern "C"
+
+static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, McArgs& a)
+{
+    std::memset(&a, 0, sizeof a);
+    sg_stars* st = s->stars;
+    int ns = (int)(st->params.size() / SG_STAR_NPARAM);
+    SG_REQUIRE(ell >= 0 && ell < st->nlambda, "wavelength index out of range");
+    double Ltot = 0;
+    for (int c = 0; c < ns; c++) Ltot += st->lum[(size_t)c * st->nlambda + ell];
+    a.first = first; a.count = count;
+    a.Lpkt = Ltot / (double)npp;
+    a.Lthreshold = a.Lpkt / s->mc[1];
+    a.ell = ell; a.nlambda = st->nlambda;
+    a.minfs = (int)s->mc[2];
+    a.xi = s->mc[3];
+    a.seed = (uint32_t)s->mc[4];
+    a.member = (uint32_t)s->mc[6];
+    a.tau_cut = s->mc[7];
+    a.has_dust = s->dust != nullptr;
+    if (s->dust) {
+        a.kext = s->dust->kext[ell]; a.albedo = s->dust->albedo[ell]; a.g = s->dust->g[ell];
+        a.store_abs = s->dust->store_abs;
+        a.labs = s->dust->store_abs ? s->dust->labs.as<double>() : nullptr;
+        a.ncells = s->dust->grid->ncells;
+    }
+    a.ni = s->instr->n;
+    for (int i = 0; i < a.ni; i++) a.instr[i] = s->instr->dev[i];
+    a.ns = ns;
+    a.emission_bias = st->bias;
+    // StellarSystem setup: normalised cumulative luminosity (NR::cdf)
+    a.Xcum[0] = 0;
+    for (int c = 0; c < ns; c++) a.Xcum[c + 1] = a.Xcum[c] + st->lum[(size_t)c * st->nlambda + ell];
+    for (int c = 0; c <= ns; c++) a.Xcum[c] /= a.Xcum[ns];
+    a.Lmean = Ltot / ns;
+    for (int c = 0; c < ns; c++) {
+        const double* p = &st->params[c * SG_STAR_NPARAM];
+        StarDev& d = a.star[c];
+        d.kind = (int)p[0];
+        for (int i = 0; i < 5; i++) d.p[i] = p[1 + i];
+        if (d.kind == 2) { d.p[0] = p[1]; d.p[1] = p[2]; }
+        d.spiral = p[6] != 0;
+        for (int i = 0; i < 6; i++) d.sp[i] = p[7 + i];
+        if (d.spiral) {
+            // SpiralStructureGeometryDecorator::setupSelfBefore (SpiralStructureGeometryDecorator.cpp:23-40)
+            d.tanp = std::tan(p[8]);
+            double N = p[12];
+            d.CN = std::sqrt(M_PI) * std::exp(lngamma(N + 1.0)) / std::exp(lngamma(N + 0.5));
+        }
+        if (d.kind == 2) {
+            DevBuf* tb = st->tables[c];
+            int nt = (int)(tb->bytes / sizeof(double) / 2);
+            d.ts = tb->as<double>(); d.tM = tb->as<double>() + nt; d.nt = nt;
+        }
+        d.lum = st->lum[(size_t)c * st->nlambda + ell];
+    }
+    a.counters = s->counters.as<unsigned long long>();
+    a.next_packet = s->next.as<unsigned long long>();
+    a.error = s->err.as<int>();
+    a.compact = g_tune_compact;
+}
+
+template <class W, bool ABS>
+static void launch_mc(const typename W::View& v, const McArgs& a, int& blocks)
+{
+    int bs = 256;
+    int per = 0;
+    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc<W, ABS>, bs, 0));
+    if (per < 1) per = 1;
+    if (g_tune_waves > 0) per = std::max(1, std::min(per, g_tune_waves * 64 / bs));
+    blocks = per * g_cus;
+    hipLaunchKernelGGL((k_mc<W, ABS>), dim3(blocks), dim3(bs), 0, 0, v, a);
+    SG_HIP(hipGetLastError());
+}
+
+template <bool ABS> static void dispatch_mc(sg_sim* s, const McArgs& a, int precision, int& blocks)
+{
+    bool mixed = precision == SG_MIXED;
+    sg_grid* g = s->dust ? s->dust->grid : nullptr;
+    if (!g) {
+        // no dust system: peel-offs need no walk; any grid kind serves as a placeholder
+        CartView v{};
+        launch_mc<CartW64, false>(v, a, blocks);
+        return;
+    }
+    if (g->kind == GK_CART) {
+        if (mixed) launch_mc<CartW32, ABS>(g->cart, a, blocks);
+        else launch_mc<CartW64, ABS>(g->cart, a, blocks);
+    } else if (g->kind == GK_TREE) {
+        if (!mixed) launch_mc<TreeW64, ABS>(g->tree, a, blocks);
+        else if (g->tree.search == SG_NEIGHBOR) launch_mc<TreeW32, ABS>(g->tree, a, blocks);
+        else launch_mc<TreeW32TopDown, ABS>(g->tree, a, blocks);
+    } else {
+        if (mixed) launch_mc<VoroW<float>, ABS>(g->voro, a, blocks);
+        else launch_mc<VoroW<double>, ABS>(g->voro, a, blocks);
+    }
+}
+extern "C" {
+
+int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, int precision, uint64_t* counters)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(npp > 0, "npp must be positive");
+    McArgs a;
+    fill_args(s, ell, first, count, npp, a);
+    s->counters.zero();
+    s->next.zero();
+    s->err.zero();
+    hipEvent_t e0, e1;
+    SG_HIP(hipEventCreate(&e0));
+    SG_HIP(hipEventCreate(&e1));
+    SG_HIP(hipEventRecord(e0, 0));
+    int blocks = 0;
+    if (count > 0) {
+        if (a.store_abs) dispatch_mc<true>(s, a, precision, blocks);
+        else dispatch_mc<false>(s, a, precision, blocks);
+    }
+    SG_HIP(hipEventRecord(e1, 0));
+    SG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    unsigned long long c[SG_NCOUNTERS];
+    SG_HIP(hipMemcpy(c, s->counters.p, sizeof c, hipMemcpyDeviceToHost));
+    int err = 0;
+    SG_HIP(hipMemcpy(&err, s->err.p, sizeof err, hipMemcpyDeviceToHost));
+    if (counters) {
+        for (int i = 0; i < SG_NCOUNTERS; i++) counters[i] = c[i];
+        counters[6] = npp;
+        counters[10] = (uint64_t)(ms * 1000.0);
+        counters[11] = blocks;
+    }
+    SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
+    SG_CATCH
+}
+
+int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const uint64_t* counts, int precision,
+                     uint64_t* counters)
+{
+    SG_TRY
+    (void)nmembers; (void)sims; (void)ells; (void)counts; (void)precision; (void)counters;
+    SG_REQUIRE(false, "batched runs are not available in this build");
+    SG_CATCH
+}
+
+} // extern "C"

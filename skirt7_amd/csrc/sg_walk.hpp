@@ -1,0 +1,777 @@
+// sg_walk.hpp -- DustGrid::path as per-lane segment generators.
+//
+// Every walker yields the (m, ds) segments of DustGridPath one at a time, including the m = -1
+// segments that move an outside start point onto the grid, and drops ds <= 0 segments like
+// DustGridPath::addSegment (DustGridPath.cpp:46-53). No segment list is stored: callers fuse
+// the optical-depth sums, absorption and the tau -> s inversion into the stream.
+//
+// The FP64 walkers repeat the reference's arithmetic operation for operation (compile with
+// -ffp-contract=off) so the cell sequences are bit-exact; the FP32 walkers are the mixed-precision
+// variants: FP32 positions, reciprocal directions, snapped exit coordinates, and an FP64
+// recompute of the wall distances when the smallest two are within 1e-5 of each other or the
+// smallest is within 1e-5 of zero (relative to the second).
+#pragma once
+#include "sg_common.hpp"
+
+namespace sg {
+
+// Stages the first breadth-first tree node words (the top levels) in LDS.
+__device__ __forceinline__ void stage_tree_nodes(const TreeView& t, int32_t* lds, int& n)
+{
+    n = min((int)t.nnodes, SG_LDS_NODES);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = t.node[i];
+}
+__device__ __forceinline__ void stage_tree_nodes(const CartView&, int32_t*, int& n) { n = 0; }
+__device__ __forceinline__ void stage_tree_nodes(const VoroView&, int32_t*, int& n) { n = 0; }
+
+// ====================================================================== Cartesian
+
+// CartesianDustGrid.cpp:136-283, exact FP64.
+struct CartW64 {
+    using Real = double;
+    using View = CartView;
+    double x, y, z, kx, ky, kz;
+    int i, j, k;
+    int ne;
+    double e0, e1, e2;
+    bool done;
+    uint32_t nrecompute, nstuck;
+
+    __device__ __forceinline__ void push(double ds)
+    {
+        if (ds > 0) {
+            // value selects, not an indexed store: keeps the walker out of scratch memory
+            e0 = ne == 0 ? ds : e0;
+            e1 = ne == 1 ? ds : e1;
+            e2 = ne >= 2 ? ds : e2;
+            ne++;
+        }
+    }
+    __device__ __forceinline__ void init(const View& g, const int32_t*, int, double rx, double ry, double rz, double dx, double dy,
+                         double dz)
+    {
+        x = rx; y = ry; z = rz; kx = dx; ky = dy; kz = dz;
+        ne = 0; done = false;
+        double ds;
+        if (x < g.xmin) {
+            if (kx <= 0.0) { ne = 0; done = true; return; }
+            ds = (g.xmin - x) / kx; push(ds);
+            x = g.xmin + 1e-8 * (g.xv[1] - g.xv[0]); y += ky * ds; z += kz * ds;
+        } else if (x > g.xmax) {
+            if (kx >= 0.0) { ne = 0; done = true; return; }
+            ds = (g.xmax - x) / kx; push(ds);
+            x = g.xmax - 1e-8 * (g.xv[g.Nx] - g.xv[g.Nx - 1]); y += ky * ds; z += kz * ds;
+        }
+        if (y < g.ymin) {
+            if (ky <= 0.0) { ne = 0; done = true; return; }
+            ds = (g.ymin - y) / ky; push(ds);
+            x += kx * ds; y = g.ymin + 1e-8 * (g.yv[1] - g.yv[0]); z += kz * ds;
+        } else if (y > g.ymax) {
+            if (ky >= 0.0) { ne = 0; done = true; return; }
+            ds = (g.ymax - y) / ky; push(ds);
+            x += kx * ds; y = g.ymax - 1e-8 * (g.yv[g.Ny] - g.yv[g.Ny - 1]); z += kz * ds;
+        }
+        if (z < g.zmin) {
+            if (kz <= 0.0) { ne = 0; done = true; return; }
+            ds = (g.zmin - z) / kz; push(ds);
+            x += kx * ds; y += ky * ds; z = g.zmin + 1e-8 * (g.zv[1] - g.zv[0]);
+        } else if (z > g.zmax) {
+            if (kz >= 0.0) { ne = 0; done = true; return; }
+            ds = (g.zmax - z) / kz; push(ds);
+            x += kx * ds; y += ky * ds; z = g.zmax - 1e-8 * (g.zv[g.Nz] - g.zv[g.Nz - 1]);
+        }
+        if (x < g.xmin || x > g.xmax || y < g.ymin || y > g.ymax || z < g.zmin || z > g.zmax) {
+            ne = 0; done = true; return;
+        }
+        i = locate_clip(g.xv, g.Nx + 1, x);
+        j = locate_clip(g.yv, g.Ny + 1, y);
+        k = locate_clip(g.zv, g.Nz + 1, z);
+    }
+    __device__ __forceinline__ bool next(const View& g, const int32_t*, int, int& m, double& ds, double& rho)
+    {
+        if (ne) {
+            ds = e0; e0 = e1; e1 = e2; ne--; m = -1; rho = 0.0;
+            return true;
+        }
+        while (!done) {
+            int mm = k + g.Nz * j + g.Nz * g.Ny * i;
+            double xE = (kx < 0.0) ? g.xv[i] : g.xv[i + 1];
+            double yE = (ky < 0.0) ? g.yv[j] : g.yv[j + 1];
+            double zE = (kz < 0.0) ? g.zv[k] : g.zv[k + 1];
+            double dsx = (fabs(kx) > 1e-15) ? (xE - x) / kx : DBL_MAX;
+            double dsy = (fabs(ky) > 1e-15) ? (yE - y) / ky : DBL_MAX;
+            double dsz = (fabs(kz) > 1e-15) ? (zE - z) / kz : DBL_MAX;
+            double d;
+            if (dsx <= dsy && dsx <= dsz) {
+                d = dsx; i += (kx < 0.0) ? -1 : 1;
+                if (i >= g.Nx || i < 0) done = true;
+                else { x = xE; y += ky * d; z += kz * d; }
+            } else if (dsy < dsx && dsy <= dsz) {
+                d = dsy; j += (ky < 0.0) ? -1 : 1;
+                if (j >= g.Ny || j < 0) done = true;
+                else { x += kx * d; y = yE; z += kz * d; }
+            } else if (dsz < dsx && dsz < dsy) {
+                d = dsz; k += (kz < 0.0) ? -1 : 1;
+                if (k >= g.Nz || k < 0) done = true;
+                else { x += kx * d; y += ky * d; z = zE; }
+            } else {
+                // only reachable with NaN input; the reference would not terminate
+                done = true;
+                return false;
+            }
+            if (d > 0) { m = mm; ds = d; rho = g.rho[mm]; return true; }
+        }
+        return false;
+    }
+};
+
+// Chooses the exit wall among three distances with the reference tie rules
+// (CartesianDustGrid.cpp:243-269: x on <=, y on < x and <= z, z strictly smallest).
+__device__ __forceinline__ int choose_axis_cart(double dx, double dy, double dz)
+{
+    if (dx <= dy && dx <= dz) return 0;
+    if (dy < dx && dy <= dz) return 1;
+    return 2;
+}
+__device__ __forceinline__ int choose_axis_cart(float dx, float dy, float dz)
+{
+    if (dx <= dy && dx <= dz) return 0;
+    if (dy < dx && dy <= dz) return 1;
+    return 2;
+}
+// Tree tie rules (TreeDustGrid.cpp:451-466: x, then y, then z, all on <=).
+template <typename T> __device__ __forceinline__ int choose_axis_tree(T dx, T dy, T dz)
+{
+    if (dx <= dy && dx <= dz) return 0;
+    if (dy <= dx && dy <= dz) return 1;
+    return 2;
+}
+
+// True when the FP32 exit decision is ambiguous: the two smallest distances agree to 1e-5
+// relative, or the smallest is within 1e-5 of zero relative to the second.
+__device__ __forceinline__ bool ambiguous(float dx, float dy, float dz)
+{
+    float a = fminf(dx, fminf(dy, dz));
+    float c = fmaxf(dx, fmaxf(dy, dz));
+    float b = dx + dy + dz - a - c;
+    if (!(b < FLT_MAX)) b = (c < FLT_MAX) ? c : FLT_MAX;
+    return (b - a) <= 1e-5f * b || a <= 1e-5f * b;
+}
+
+// Mixed-precision Cartesian walk.
+struct CartW32 {
+    using Real = float;
+    using View = CartView;
+    float x, y, z, kx, ky, kz, ix, iy, iz;
+    int i, j, k;
+    int ne;
+    float e0, e1, e2;
+    bool done;
+    uint32_t nrecompute, nstuck;
+
+    __device__ __forceinline__ void push(float ds)
+    {
+        if (ds > 0) {
+            // value selects, not an indexed store: keeps the walker out of scratch memory
+            e0 = ne == 0 ? ds : e0;
+            e1 = ne == 1 ? ds : e1;
+            e2 = ne >= 2 ? ds : e2;
+            ne++;
+        }
+    }
+    __device__ __forceinline__ void init(const View& g, const int32_t*, int, double rxd, double ryd, double rzd, double dxd,
+                         double dyd, double dzd)
+    {
+        // entry in FP64, state in FP32
+        double x0 = rxd, y0 = ryd, z0 = rzd;
+        ne = 0; done = false;
+        double ds;
+        if (x0 < g.xmin) {
+            if (dxd <= 0.0) { done = true; return; }
+            ds = (g.xmin - x0) / dxd; push((float)ds); x0 = g.xmin; y0 += dyd * ds; z0 += dzd * ds;
+        } else if (x0 > g.xmax) {
+            if (dxd >= 0.0) { done = true; return; }
+            ds = (g.xmax - x0) / dxd; push((float)ds); x0 = g.xmax; y0 += dyd * ds; z0 += dzd * ds;
+        }
+        if (y0 < g.ymin) {
+            if (dyd <= 0.0) { ne = 0; done = true; return; }
+            ds = (g.ymin - y0) / dyd; push((float)ds); x0 += dxd * ds; y0 = g.ymin; z0 += dzd * ds;
+        } else if (y0 > g.ymax) {
+            if (dyd >= 0.0) { ne = 0; done = true; return; }
+            ds = (g.ymax - y0) / dyd; push((float)ds); x0 += dxd * ds; y0 = g.ymax; z0 += dzd * ds;
+        }
+        if (z0 < g.zmin) {
+            if (dzd <= 0.0) { ne = 0; done = true; return; }
+            ds = (g.zmin - z0) / dzd; push((float)ds); x0 += dxd * ds; y0 += dyd * ds; z0 = g.zmin;
+        } else if (z0 > g.zmax) {
+            if (dzd >= 0.0) { ne = 0; done = true; return; }
+            ds = (g.zmax - z0) / dzd; push((float)ds); x0 += dxd * ds; y0 += dyd * ds; z0 = g.zmax;
+        }
+        if (x0 < g.xmin || x0 > g.xmax || y0 < g.ymin || y0 > g.ymax || z0 < g.zmin || z0 > g.zmax) {
+            ne = 0; done = true; return;
+        }
+        x = (float)x0; y = (float)y0; z = (float)z0;
+        kx = (float)dxd; ky = (float)dyd; kz = (float)dzd;
+        ix = fabsf(kx) > 1e-15f ? 1.0f / kx : 0.0f;
+        iy = fabsf(ky) > 1e-15f ? 1.0f / ky : 0.0f;
+        iz = fabsf(kz) > 1e-15f ? 1.0f / kz : 0.0f;
+        i = locate_clip(g.xvf, g.Nx + 1, x);
+        j = locate_clip(g.yvf, g.Ny + 1, y);
+        k = locate_clip(g.zvf, g.Nz + 1, z);
+    }
+    __device__ __forceinline__ bool next(const View& g, const int32_t*, int, int& m, float& ds, float& rho)
+    {
+        if (ne) {
+            ds = e0; e0 = e1; e1 = e2; ne--; m = -1; rho = 0.0f;
+            return true;
+        }
+        while (!done) {
+            int mm = k + g.Nz * j + g.Nz * g.Ny * i;
+            int ie = (kx < 0.0f) ? i : i + 1, je = (ky < 0.0f) ? j : j + 1, ke = (kz < 0.0f) ? k : k + 1;
+            float xE = g.xvf[ie], yE = g.yvf[je], zE = g.zvf[ke];
+            float dsx = ix != 0.0f ? (xE - x) * ix : FLT_MAX;
+            float dsy = iy != 0.0f ? (yE - y) * iy : FLT_MAX;
+            float dsz = iz != 0.0f ? (zE - z) * iz : FLT_MAX;
+            int a = choose_axis_cart(dsx, dsy, dsz);
+            float d = a == 0 ? dsx : a == 1 ? dsy : dsz;
+            if (ambiguous(dsx, dsy, dsz)) {
+                nrecompute++;
+                double X = x, Y = y, Z = z;
+                double ddx = ix != 0.0f ? (g.xv[ie] - X) / (double)kx : DBL_MAX;
+                double ddy = iy != 0.0f ? (g.yv[je] - Y) / (double)ky : DBL_MAX;
+                double ddz = iz != 0.0f ? (g.zv[ke] - Z) / (double)kz : DBL_MAX;
+                a = choose_axis_cart(ddx, ddy, ddz);
+                d = (float)(a == 0 ? ddx : a == 1 ? ddy : ddz);
+            }
+            if (a == 0) {
+                i += (kx < 0.0f) ? -1 : 1;
+                if (i >= g.Nx || i < 0) done = true;
+                else { x = xE; y = fmaf(ky, d, y); z = fmaf(kz, d, z); }
+            } else if (a == 1) {
+                j += (ky < 0.0f) ? -1 : 1;
+                if (j >= g.Ny || j < 0) done = true;
+                else { x = fmaf(kx, d, x); y = yE; z = fmaf(kz, d, z); }
+            } else {
+                k += (kz < 0.0f) ? -1 : 1;
+                if (k >= g.Nz || k < 0) done = true;
+                else { x = fmaf(kx, d, x); y = fmaf(ky, d, y); z = zE; }
+            }
+            if (d > 0) { m = mm; ds = d; rho = g.rhof[mm]; return true; }
+        }
+        return false;
+    }
+};
+
+// ====================================================================== trees
+
+__device__ __forceinline__ int32_t node_word(const TreeView& t, const int32_t* lds, int nlds, int id)
+{
+    return id < nlds ? lds[id] : t.node[id];
+}
+
+// TreeNode::whichnode from the root (TreeNode.cpp:70-80) with the child choice of
+// OctTreeNode.cpp:184-189 / BinTreeNode.cpp:326-335. Node bounds are not stored: each child box
+// is recomputed from its parent's with the split at 0.5*(min+max) exactly as the reference
+// creates it (OctTreeNode.cpp:38-49, BinTreeNode.cpp:40-70), so only one node word per level is
+// read. Returns the leaf's cell number or -1 when the point is outside the root box.
+__device__ __forceinline__ int descend64(const TreeView& t, const int32_t* lds, int nlds, double x, double y,
+                                         double z, double* lo, double* hi)
+{
+    if (!box_contains(t.ext, x, y, z)) return -1;
+    lo[0] = t.ext[0]; lo[1] = t.ext[1]; lo[2] = t.ext[2];
+    hi[0] = t.ext[3]; hi[1] = t.ext[4]; hi[2] = t.ext[5];
+    int32_t w = node_word(t, lds, nlds, 0);
+    int level = 0;
+    while (w >= 0) {
+        if (t.arity == 8) {
+            double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+            int q = 0;
+            if (x < cx) hi[0] = cx; else { lo[0] = cx; q += 1; }
+            if (y < cy) hi[1] = cy; else { lo[1] = cy; q += 2; }
+            if (z < cz) hi[2] = cz; else { lo[2] = cz; q += 4; }
+            w = node_word(t, lds, nlds, w + q);
+        } else {
+            int a = level % 3;
+            double p = a == 0 ? x : a == 1 ? y : z;
+            double l = a == 0 ? lo[0] : a == 1 ? lo[1] : lo[2];
+            double h = a == 0 ? hi[0] : a == 1 ? hi[1] : hi[2];
+            double c = 0.5 * (l + h);
+            int q = (p < c) ? 0 : 1;
+            if (q == 0) { if (a == 0) hi[0] = c; else if (a == 1) hi[1] = c; else hi[2] = c; }
+            else { if (a == 0) lo[0] = c; else if (a == 1) lo[1] = c; else lo[2] = c; }
+            w = node_word(t, lds, nlds, w + q);
+        }
+        level++;
+    }
+    return -w - 1;
+}
+
+// DustGridPath::moveInside (DustGridPath.cpp:57-150). Returns false for the OUTSIDE result.
+struct Entry {
+    int ne;
+    double e0, e1, e2;
+    __device__ __forceinline__ void push(double ds)
+    {
+        if (ds > 0) {
+            // value selects, not an indexed store: keeps the walker out of scratch memory
+            e0 = ne == 0 ? ds : e0;
+            e1 = ne == 1 ? ds : e1;
+            e2 = ne >= 2 ? ds : e2;
+            ne++;
+        }
+    }
+    __device__ __forceinline__ double pop()
+    {
+        double d = e0; e0 = e1; e1 = e2; ne--;
+        return d;
+    }
+    __device__ __forceinline__ bool move_inside(const double* b, double eps, double& rx, double& ry, double& rz, double kx,
+                                double ky, double kz)
+    {
+        ne = 0;
+        if (rx <= b[0]) {
+            if (kx <= 0.0) return false;
+            double ds = (b[0] - rx) / kx; push(ds);
+            rx = b[0] + eps; ry += ky * ds; rz += kz * ds;
+        } else if (rx >= b[3]) {
+            if (kx >= 0.0) return false;
+            double ds = (b[3] - rx) / kx; push(ds);
+            rx = b[3] - eps; ry += ky * ds; rz += kz * ds;
+        }
+        if (ry <= b[1]) {
+            if (ky <= 0.0) return false;
+            double ds = (b[1] - ry) / ky; push(ds);
+            rx += kx * ds; ry = b[1] + eps; rz += kz * ds;
+        } else if (ry >= b[4]) {
+            if (ky >= 0.0) return false;
+            double ds = (b[4] - ry) / ky; push(ds);
+            rx += kx * ds; ry = b[4] - eps; rz += kz * ds;
+        }
+        if (rz <= b[2]) {
+            if (kz <= 0.0) return false;
+            double ds = (b[2] - rz) / kz; push(ds);
+            rx += kx * ds; ry += ky * ds; rz = b[2] + eps;
+        } else if (rz >= b[5]) {
+            if (kz >= 0.0) return false;
+            double ds = (b[5] - rz) / kz; push(ds);
+            rx += kx * ds; ry += ky * ds; rz = b[5] - eps;
+        }
+        return true;
+    }
+};
+
+// TreeDustGrid::path, TopDown (:393-437) and Neighbor (:441-502) methods, exact FP64.
+struct TreeW64 {
+    using Real = double;
+    using View = TreeView;
+    double x, y, z, kx, ky, kz;
+    double lo[3], hi[3];
+    double rho;
+    int m;
+    Entry en;
+    bool done;
+    uint32_t steps, nrecompute, nstuck;
+
+    __device__ __forceinline__ void load_cell(const View& t, int c)
+    {
+        const TreeRec64& r = t.rec64[c];
+        lo[0] = r.lo[0]; lo[1] = r.lo[1]; lo[2] = r.lo[2];
+        hi[0] = r.hi[0]; hi[1] = r.hi[1]; hi[2] = r.hi[2];
+        rho = r.rho;
+        m = c;
+    }
+    __device__ __forceinline__ bool descend(const View& t, const int32_t* lds, int nlds)
+    {
+        double l[3], h[3];
+        int c = descend64(t, lds, nlds, x, y, z, l, h);
+        if (c < 0) return false;
+        lo[0] = l[0]; lo[1] = l[1]; lo[2] = l[2];
+        hi[0] = h[0]; hi[1] = h[1]; hi[2] = h[2];
+        m = c;
+        rho = t.rec64[c].rho;
+        return true;
+    }
+    __device__ __forceinline__ void init(const View& t, const int32_t* lds, int nlds, double rx, double ry, double rz, double dx,
+                         double dy, double dz)
+    {
+        kx = dx; ky = dy; kz = dz;
+        steps = 0; done = false;
+        if (!en.move_inside(t.ext, t.eps, rx, ry, rz, kx, ky, kz)) {
+            // OUTSIDE sentinel: no node contains it, path cleared
+            en.ne = 0; done = true; return;
+        }
+        x = rx; y = ry; z = rz;
+        if (!descend(t, lds, nlds)) { en.ne = 0; done = true; }
+    }
+    __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, double& ds,
+                                         double& rh)
+    {
+        if (en.ne) { ds = en.pop(); mo = -1; rh = 0.0; return true; }
+        while (!done) {
+            double xnext = (kx < 0.0) ? lo[0] : hi[0];
+            double ynext = (ky < 0.0) ? lo[1] : hi[1];
+            double znext = (kz < 0.0) ? lo[2] : hi[2];
+            double dsx = (fabs(kx) > 1e-15) ? (xnext - x) / kx : DBL_MAX;
+            double dsy = (fabs(ky) > 1e-15) ? (ynext - y) / ky : DBL_MAX;
+            double dsz = (fabs(kz) > 1e-15) ? (znext - z) / kz : DBL_MAX;
+            int a = choose_axis_tree(dsx, dsy, dsz);
+            double d = a == 0 ? dsx : a == 1 ? dsy : dsz;
+            int wall = 2 * a + ((a == 0 ? kx : a == 1 ? ky : kz) < 0.0 ? 0 : 1);
+            int segm = m;
+            double segrho = rho;
+            x += (d + t.eps) * kx;
+            y += (d + t.eps) * ky;
+            z += (d + t.eps) * kz;
+            int oldm = m;
+            bool found = false;
+            if (t.search == 1) {
+                int i0 = t.nboff[oldm * 6 + wall], i1 = t.nboff[oldm * 6 + wall + 1];
+                for (int q = i0; q < i1; q++) {
+                    int c = t.nbids[q];
+                    const TreeRec64& r = t.rec64[c];
+                    if (x >= r.lo[0] && x <= r.hi[0] && y >= r.lo[1] && y <= r.hi[1] && z >= r.lo[2] &&
+                        z <= r.hi[2]) {
+                        load_cell(t, c);
+                        found = true;
+                        break;
+                    }
+                }
+            }
+            if (!found) found = descend(t, lds, nlds);
+            if (!found) done = true;
+            else if (m == oldm) {
+                nstuck++;
+                x = nextafter(x, (kx < 0.0) ? -DBL_MAX : DBL_MAX);
+                y = nextafter(y, (ky < 0.0) ? -DBL_MAX : DBL_MAX);
+                z = nextafter(z, (kz < 0.0) ? -DBL_MAX : DBL_MAX);
+                if (!descend(t, lds, nlds)) done = true;
+                else if (m == oldm) done = true;
+            }
+            if (++steps > SG_MAX_STEPS) done = true;
+            if (d > 0) { mo = segm; ds = d; rh = segrho; return true; }
+        }
+        return false;
+    }
+};
+
+// Mixed-precision tree walk (Neighbor semantics): FP32 position snapped onto the exit wall,
+// the neighbour across that wall taken from the current cell's record, the CSR list and a
+// directed root descent as fallbacks.
+struct TreeW32 {
+    using Real = float;
+    using View = TreeView;
+    float x, y, z, kx, ky, kz, ix, iy, iz;
+    float lo[3], hi[3];
+    float rho;
+    int32_t nb[6];
+    int m;
+    int ne;
+    float e0, e1, e2;
+    bool done;
+    uint32_t steps, nrecompute, nstuck;
+
+    __device__ __forceinline__ void load_cell(const View& t, int c)
+    {
+        const float4* p = reinterpret_cast<const float4*>(t.rec32 + c);
+        float4 a = p[0], b = p[1], cc = p[2], d = p[3];
+        lo[0] = a.x; lo[1] = a.y; lo[2] = a.z; hi[0] = a.w; hi[1] = b.x; hi[2] = b.y;
+        rho = b.z;
+        nb[0] = __float_as_int(b.w); nb[1] = __float_as_int(cc.x); nb[2] = __float_as_int(cc.y);
+        nb[3] = __float_as_int(cc.z); nb[4] = __float_as_int(cc.w); nb[5] = __float_as_int(d.x);
+        m = c;
+        (void)d;
+    }
+    // Root descent in FP32 with the exit axis compared in the travel direction, so a point
+    // lying exactly on the exit wall goes to the cell beyond it.
+    __device__ __forceinline__ int descend(const View& t, const int32_t* lds, int nlds, int xa, int xdir)
+    {
+        const float* e = t.extf;
+        float p[3] = {x, y, z};
+        for (int a = 0; a < 3; a++) {
+            bool in = p[a] >= e[a] && p[a] <= e[a + 3];
+            if (a == xa) in = xdir > 0 ? (p[a] >= e[a] && p[a] < e[a + 3]) : (p[a] > e[a] && p[a] <= e[a + 3]);
+            if (!in) return -1;
+        }
+        float l[3] = {e[0], e[1], e[2]}, h[3] = {e[3], e[4], e[5]};
+        int32_t w = node_word(t, lds, nlds, 0);
+        int level = 0;
+        while (w >= 0) {
+            int q = 0;
+            if (t.arity == 8) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    float c = 0.5f * (l[a] + h[a]);
+                    bool low = (a == xa && xdir < 0) ? (p[a] <= c) : (p[a] < c);
+                    if (low) h[a] = c; else { l[a] = c; q += 1 << a; }
+                }
+            } else {
+                int a = level % 3;
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    if (b != a) continue;
+                    float c = 0.5f * (l[b] + h[b]);
+                    bool low = (b == xa && xdir < 0) ? (p[b] <= c) : (p[b] < c);
+                    if (low) h[b] = c; else { l[b] = c; q = 1; }
+                }
+            }
+            w = node_word(t, lds, nlds, w + q);
+            level++;
+        }
+        return -w - 1;
+    }
+    __device__ __forceinline__ void push(float ds)
+    {
+        if (ds > 0) {
+            // value selects, not an indexed store: keeps the walker out of scratch memory
+            e0 = ne == 0 ? ds : e0;
+            e1 = ne == 1 ? ds : e1;
+            e2 = ne >= 2 ? ds : e2;
+            ne++;
+        }
+    }
+    __device__ __forceinline__ void init(const View& t, const int32_t* lds, int nlds, double rx, double ry, double rz, double dx,
+                         double dy, double dz)
+    {
+        steps = 0; done = false;
+        Entry en;
+        if (!en.move_inside(t.ext, t.eps, rx, ry, rz, dx, dy, dz)) { ne = 0; done = true; return; }
+        ne = 0;
+        for (int q = 0; q < en.ne; q++) push((float)(q == 0 ? en.e0 : q == 1 ? en.e1 : en.e2));
+        x = (float)rx; y = (float)ry; z = (float)rz;
+        kx = (float)dx; ky = (float)dy; kz = (float)dz;
+        ix = fabsf(kx) > 1e-15f ? 1.0f / kx : 0.0f;
+        iy = fabsf(ky) > 1e-15f ? 1.0f / ky : 0.0f;
+        iz = fabsf(kz) > 1e-15f ? 1.0f / kz : 0.0f;
+        int c = descend(t, lds, nlds, -1, 0);
+        if (c < 0) { ne = 0; done = true; return; }
+        load_cell(t, c);
+    }
+    __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds,
+                                         float& rh)
+    {
+        if (ne) {
+            ds = e0; e0 = e1; e1 = e2; ne--; mo = -1; rh = 0.0f;
+            return true;
+        }
+        while (!done) {
+            float xn = (kx < 0.0f) ? lo[0] : hi[0];
+            float yn = (ky < 0.0f) ? lo[1] : hi[1];
+            float zn = (kz < 0.0f) ? lo[2] : hi[2];
+            float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
+            float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
+            float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
+            int a = choose_axis_tree(dsx, dsy, dsz);
+            float d = a == 0 ? dsx : a == 1 ? dsy : dsz;
+            if (ambiguous(dsx, dsy, dsz)) {
+                nrecompute++;
+                double X = x, Y = y, Z = z;
+                double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
+                double ddy = iy != 0.0f ? ((double)yn - Y) / (double)ky : DBL_MAX;
+                double ddz = iz != 0.0f ? ((double)zn - Z) / (double)kz : DBL_MAX;
+                a = choose_axis_tree(ddx, ddy, ddz);
+                d = (float)(a == 0 ? ddx : a == 1 ? ddy : ddz);
+            }
+            float dir = a == 0 ? kx : a == 1 ? ky : kz;
+            int wall = 2 * a + (dir < 0.0f ? 0 : 1);
+            int segm = m;
+            float segrho = rho;
+            float dd = fmaxf(d, 0.0f);
+            if (a == 0) { x = xn; y = fmaf(ky, dd, y); z = fmaf(kz, dd, z); }
+            else if (a == 1) { x = fmaf(kx, dd, x); y = yn; z = fmaf(kz, dd, z); }
+            else { x = fmaf(kx, dd, x); y = fmaf(ky, dd, y); z = zn; }
+            int oldm = m;
+            int cand = wall == 0 ? nb[0] : wall == 1 ? nb[1] : wall == 2 ? nb[2] : wall == 3 ? nb[3]
+                     : wall == 4 ? nb[4] : nb[5];
+            if (cand < 0) {
+                done = true;   // boundary wall of the grid: no neighbours, the path leaves the grid
+            } else {
+                const TreeRec32& r = t.rec32[cand];
+                bool in = x >= r.lo[0] && x <= r.hi[0] && y >= r.lo[1] && y <= r.hi[1] && z >= r.lo[2] && z <= r.hi[2];
+                if (in) load_cell(t, cand);
+                else {
+                    int i0 = t.nboff[oldm * 6 + wall], i1 = t.nboff[oldm * 6 + wall + 1];
+                    bool found = false;
+                    for (int q = i0 + 1; q < i1; q++) {
+                        int c = t.nbids[q];
+                        const TreeRec32& rr = t.rec32[c];
+                        if (x >= rr.lo[0] && x <= rr.hi[0] && y >= rr.lo[1] && y <= rr.hi[1] && z >= rr.lo[2] &&
+                            z <= rr.hi[2]) {
+                            load_cell(t, c);
+                            found = true;
+                            break;
+                        }
+                    }
+                    if (!found) {
+                        int c = descend(t, lds, nlds, a, dir < 0.0f ? -1 : 1);
+                        if (c < 0) done = true;
+                        else load_cell(t, c);
+                    }
+                }
+                if (!done && m == oldm) { nstuck++; done = true; }
+            }
+            if (++steps > SG_MAX_STEPS) done = true;
+            if (d > 0) { mo = segm; ds = d; rh = segrho; return true; }
+        }
+        return false;
+    }
+};
+
+// Mixed-precision tree walk with TopDown semantics (every next cell by directed root descent).
+struct TreeW32TopDown : TreeW32 {
+    __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds,
+                                         float& rh)
+    {
+        if (ne) {
+            ds = e0; e0 = e1; e1 = e2; ne--; mo = -1; rh = 0.0f;
+            return true;
+        }
+        while (!done) {
+            float xn = (kx < 0.0f) ? lo[0] : hi[0];
+            float yn = (ky < 0.0f) ? lo[1] : hi[1];
+            float zn = (kz < 0.0f) ? lo[2] : hi[2];
+            float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
+            float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
+            float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
+            int a = choose_axis_tree(dsx, dsy, dsz);
+            float d = a == 0 ? dsx : a == 1 ? dsy : dsz;
+            if (ambiguous(dsx, dsy, dsz)) {
+                nrecompute++;
+                double X = x, Y = y, Z = z;
+                double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
+                double ddy = iy != 0.0f ? ((double)yn - Y) / (double)ky : DBL_MAX;
+                double ddz = iz != 0.0f ? ((double)zn - Z) / (double)kz : DBL_MAX;
+                a = choose_axis_tree(ddx, ddy, ddz);
+                d = (float)(a == 0 ? ddx : a == 1 ? ddy : ddz);
+            }
+            float dir = a == 0 ? kx : a == 1 ? ky : kz;
+            int segm = m;
+            float segrho = rho;
+            float dd = fmaxf(d, 0.0f);
+            if (a == 0) { x = xn; y = fmaf(ky, dd, y); z = fmaf(kz, dd, z); }
+            else if (a == 1) { x = fmaf(kx, dd, x); y = yn; z = fmaf(kz, dd, z); }
+            else { x = fmaf(kx, dd, x); y = fmaf(ky, dd, y); z = zn; }
+            int oldm = m;
+            int c = descend(t, lds, nlds, a, dir < 0.0f ? -1 : 1);
+            if (c < 0) done = true;
+            else {
+                load_cell(t, c);
+                if (m == oldm) { nstuck++; done = true; }
+            }
+            if (++steps > SG_MAX_STEPS) done = true;
+            if (d > 0) { mo = segm; ds = d; rh = segrho; return true; }
+        }
+        return false;
+    }
+};
+
+// ====================================================================== Voronoi
+
+// VoronoiMesh::cellIndex (VoronoiMesh.cpp:541-570): the nearest site, ties to the lower index,
+// found over a uniform bucket grid by growing shells until no closer site can remain.
+__device__ int voro_nearest(const VoroView& g, double x, double y, double z)
+{
+    if (!box_contains(g.ext, x, y, z)) return -1;
+    int n = g.nbk;
+    double w[3] = {(g.ext[3] - g.ext[0]) / n, (g.ext[4] - g.ext[1]) / n, (g.ext[5] - g.ext[2]) / n};
+    int ci = min(n - 1, max(0, (int)(n * (x - g.ext[0]) / (g.ext[3] - g.ext[0]))));
+    int cj = min(n - 1, max(0, (int)(n * (y - g.ext[1]) / (g.ext[4] - g.ext[1]))));
+    int ck = min(n - 1, max(0, (int)(n * (z - g.ext[2]) / (g.ext[5] - g.ext[2]))));
+    double wmin = fmin(w[0], fmin(w[1], w[2]));
+    double bd = DBL_MAX;
+    int best = -1;
+    for (int ring = 0; ring <= n; ring++) {
+        for (int i = max(0, ci - ring); i <= min(n - 1, ci + ring); i++)
+            for (int j = max(0, cj - ring); j <= min(n - 1, cj + ring); j++)
+                for (int k = max(0, ck - ring); k <= min(n - 1, ck + ring); k++) {
+                    int cheb = max(abs(i - ci), max(abs(j - cj), abs(k - ck)));
+                    if (cheb != ring) continue;
+                    int q = (i * n + j) * n + k;
+                    for (int t = g.boff[q]; t < g.boff[q + 1]; t++) {
+                        int s = g.bids[t];
+                        VoroSite p = g.site[s];
+                        double dx = x - p.x, dy = y - p.y, dz = z - p.z;
+                        double d = dx * dx + dy * dy + dz * dz;
+                        if (d < bd || (d == bd && s < best)) { bd = d; best = s; }
+                    }
+                }
+        double reach = ring * wmin * (1.0 - 1e-12);
+        if (best >= 0 && reach * reach > bd) break;
+    }
+    return best;
+}
+
+// VoronoiMesh::path (VoronoiMesh.cpp:778-873): the bisector-plane tests stay FP64 in both
+// precision modes; Real only sets the type of the emitted segment lengths.
+template <typename R> struct VoroW {
+    using Real = R;
+    using View = VoroView;
+    double x, y, z, kx, ky, kz;
+    int mr;
+    Entry en;
+    bool done;
+    uint32_t steps, nrecompute, nstuck;
+
+    __device__ __forceinline__ void init(const View& g, const int32_t*, int, double rx, double ry, double rz, double dx, double dy,
+                         double dz)
+    {
+        kx = dx; ky = dy; kz = dz;
+        steps = 0; done = false;
+        if (!en.move_inside(g.ext, g.eps, rx, ry, rz, kx, ky, kz)) { en.ne = 0; done = true; return; }
+        x = rx; y = ry; z = rz;
+        mr = voro_nearest(g, x, y, z);
+        if (mr < 0) { en.ne = 0; done = true; }
+    }
+    __device__ __forceinline__ bool next(const View& g, const int32_t*, int, int& mo, R& ds, R& rh)
+    {
+        if (en.ne) { ds = (R)en.pop(); mo = -1; rh = (R)0; return true; }
+        while (!done) {
+            VoroSite pr = g.site[mr];
+            double sq = DBL_MAX;
+            const int NO_INDEX = -99;
+            int mq = NO_INDEX;
+            int t0 = g.off[mr], t1 = g.off[mr + 1];
+            for (int t = t0; t < t1; t++) {
+                int mi = g.nb[t];
+                double si = 0;
+                if (mi >= 0) {
+                    VoroSite pi = g.site[mi];
+                    double nx = pi.x - pr.x, ny = pi.y - pr.y, nz = pi.z - pr.z;
+                    double ndotk = nx * kx + ny * ky + nz * kz;
+                    if (ndotk > 0) {
+                        double px = 0.5 * (pi.x + pr.x), py = 0.5 * (pi.y + pr.y), pz = 0.5 * (pi.z + pr.z);
+                        si = (nx * (px - x) + ny * (py - y) + nz * (pz - z)) / ndotk;
+                    }
+                } else {
+                    switch (mi) {
+                    case -1: si = (g.ext[0] - x) / kx; break;
+                    case -2: si = (g.ext[3] - x) / kx; break;
+                    case -3: si = (g.ext[1] - y) / ky; break;
+                    case -4: si = (g.ext[4] - y) / ky; break;
+                    case -5: si = (g.ext[2] - z) / kz; break;
+                    case -6: si = (g.ext[5] - z) / kz; break;
+                    default: si = 0;
+                    }
+                }
+                if (si > 0 && si < sq) { sq = si; mq = mi; }
+            }
+            if (++steps > SG_MAX_STEPS) { done = true; return false; }
+            if (mq == NO_INDEX) {
+                nstuck++;
+                x += kx * g.eps; y += ky * g.eps; z += kz * g.eps;
+                mr = voro_nearest(g, x, y, z);
+                if (mr < 0) done = true;
+                continue;
+            }
+            int segm = mr;
+            double segrho = pr.rho;
+            x += (sq + g.eps) * kx; y += (sq + g.eps) * ky; z += (sq + g.eps) * kz;
+            mr = mq;
+            if (mr < 0) done = true;
+            mo = segm; ds = (R)sq; rh = (R)segrho;
+            return true;
+        }
+        return false;
+    }
+};
+
+} // namespace sg

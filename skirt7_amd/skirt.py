@@ -1,0 +1,471 @@
+"""Host-side mirror of SKIRT7's simulation items on the photon-packet transport path.
+
+The classes keep the reference's names and argument meanings (file:line in each docstring) and
+forward all work to the HIP library through the C-ABI (skirt7_amd.capi). Errors from the library
+raise FatalError, as the reference's FATALERROR does.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import capi
+from .capi import SG_FP64, SG_MIXED, SG_NEIGHBOR, SG_TOPDOWN, as_f64, as_i32, as_i64, as_u8, check, dptr, lib
+
+FatalError = capi.SkirtError
+
+
+def _handle():
+    return ctypes.c_void_p()
+
+
+# ----------------------------------------------------------------------------------- grids
+
+
+class DustGridPath:
+    """Segments of one path (DustGridPath.hpp:27-168): cell numbers m, lengths ds, cumulative s."""
+
+    def __init__(self, m, ds):
+        self.m = np.asarray(m, dtype=np.int32)
+        self.ds = np.asarray(ds, dtype=np.float64)
+        self.s = np.cumsum(self.ds) if len(self.ds) else np.zeros(0)
+
+    def size(self):
+        return len(self.m)
+
+
+class DustGrid:
+    """Common interface: path (DustGrid.hpp:106) and whichcell (DustGrid.hpp:89)."""
+
+    _h = None
+
+    def numCells(self):
+        n = ctypes.c_int64()
+        check(lib().sg_grid_num_cells(self._h, ctypes.byref(n)))
+        return n.value
+
+    def paths(self, rk, max_seg=4096, precision=SG_FP64):
+        """Segment lists for n rays rk[n,6] = (rx,ry,rz,kx,ky,kz): (counts, m[n,max_seg], ds[n,max_seg])."""
+        rk = as_f64(rk).reshape(-1, 6)
+        n = rk.shape[0]
+        counts = np.zeros(n, np.int32)
+        m = np.zeros((n, max_seg), np.int32)
+        ds = np.zeros((n, max_seg), np.float64)
+        check(lib().sg_grid_paths(self._h, n, dptr(rk), max_seg, precision, counts.ctypes.data_as(capi._I32),
+                                  m.ctypes.data_as(capi._I32), dptr(ds)))
+        return counts, m, ds
+
+    def path(self, r, k, precision=SG_FP64, max_seg=65536):
+        counts, m, ds = self.paths(np.concatenate([r, k])[None, :], max_seg, precision)
+        if counts[0] < 0:
+            raise FatalError("path longer than %d segments" % max_seg)
+        return DustGridPath(m[0, :counts[0]], ds[0, :counts[0]])
+
+    def whichcell(self, r):
+        r = as_f64(r).reshape(-1, 3)
+        out = np.zeros(r.shape[0], np.int32)
+        check(lib().sg_grid_whichcell(self._h, r.shape[0], dptr(r), out.ctypes.data_as(capi._I32)))
+        return out
+
+    def __del__(self):
+        if self._h is not None and capi._lib is not None:
+            capi._lib.sg_grid_destroy(self._h)
+            self._h = None
+
+
+def linmesh(vmin, vmax, n):
+    """Cell borders of a LinMesh scaled to [vmin, vmax] (LinMesh.cpp:19-24, NR.hpp lingrid,
+    CartesianDustGrid.cpp:33-35): (0 + i*(1/n)) * (vmax - vmin) + vmin."""
+    dx = (1.0 - 0.0) / n
+    t = np.array([0.0 + i * dx for i in range(n + 1)])
+    return t * (vmax - vmin) + vmin
+
+
+class CartesianDustGrid(DustGrid):
+    """CartesianDustGrid (CartesianDustGrid.cpp:28-43,136-283)."""
+
+    def __init__(self, extent, xv, yv, zv):
+        self.extent = as_f64(extent)
+        self.xv, self.yv, self.zv = as_f64(xv), as_f64(yv), as_f64(zv)
+        h = _handle()
+        check(lib().sg_grid_create_cartesian(dptr(self.extent), dptr(self.xv), len(self.xv), dptr(self.yv),
+                                             len(self.yv), dptr(self.zv), len(self.zv), ctypes.byref(h)))
+        self._h = h
+
+    @classmethod
+    def uniform(cls, extent, nx, ny, nz):
+        e = as_f64(extent)
+        return cls(e, linmesh(e[0], e[3], nx), linmesh(e[1], e[4], ny), linmesh(e[2], e[5], nz))
+
+    def cell_boxes(self):
+        Nx, Ny, Nz = len(self.xv) - 1, len(self.yv) - 1, len(self.zv) - 1
+        i, j, k = np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")
+        return np.stack([self.xv[i], self.yv[j], self.zv[k], self.xv[i + 1], self.yv[j + 1], self.zv[k + 1]],
+                        -1).reshape(-1, 6)
+
+
+class TreeDustGrid(DustGrid):
+    """TreeDustGrid (TreeDustGrid.cpp:49-162,371-643) from a frozen subdivision: one flag per node
+    in the reference's breadth-first node numbering."""
+
+    arity = 8
+
+    def __init__(self, extent, subdivided, search=SG_NEIGHBOR):
+        self.extent = as_f64(extent)
+        self.subdivided = as_u8(subdivided)
+        self.search = search
+        h = _handle()
+        check(lib().sg_grid_create_tree(self.arity, dptr(self.extent), self.subdivided.ctypes.data_as(capi._U8),
+                                        len(self.subdivided), search, ctypes.byref(h)))
+        self._h = h
+
+    @classmethod
+    def subdivide(cls, extent, geometry, scale, total_mass, minlevel=2, maxlevel=6, maxmassfraction=1e-6,
+                  nsamples=100, seed=4357, search=SG_NEIGHBOR):
+        """Builds the subdivision from a dust geometry density (TreeDustGrid.cpp:166-230)."""
+        e = as_f64(extent)
+        cap = 1 << 27
+        flags = np.zeros(cap, np.uint8)
+        n = ctypes.c_int64()
+        g = as_f64(geometry.params())
+        check(lib().sg_tree_subdivide(cls.arity, dptr(e), dptr(g), scale, total_mass, minlevel, maxlevel,
+                                      maxmassfraction, nsamples, seed, flags.ctypes.data_as(capi._U8), cap,
+                                      ctypes.byref(n)))
+        return cls(e, flags[:n.value].copy(), search)
+
+
+class OctTreeDustGrid(TreeDustGrid):
+    """OctTreeDustGrid (OctTreeDustGrid.cpp:32-43, OctTreeNode.cpp)."""
+    arity = 8
+
+
+class BinTreeDustGrid(TreeDustGrid):
+    """BinTreeDustGrid with the Alternating direction method (BinTreeDustGrid.cpp:19-53)."""
+    arity = 2
+
+
+class VoronoiDustGrid(DustGrid):
+    """VoronoiDustGrid / VoronoiMesh (VoronoiMesh.cpp:297-405,541-570,778-873) from generator sites and
+    neighbour lists (walls -1..-6)."""
+
+    def __init__(self, extent, sites, nb_offsets, nb_ids):
+        self.extent = as_f64(extent)
+        self.sites = as_f64(sites).reshape(-1, 3)
+        self.nb_offsets = as_i64(nb_offsets)
+        self.nb_ids = as_i32(nb_ids)
+        h = _handle()
+        check(lib().sg_grid_create_voronoi(dptr(self.extent), dptr(self.sites), self.sites.shape[0],
+                                           self.nb_offsets.ctypes.data_as(capi._I64),
+                                           self.nb_ids.ctypes.data_as(capi._I32), ctypes.byref(h)))
+        self._h = h
+
+
+# ----------------------------------------------------------------------------------- geometry
+
+
+class ExpDiskGeometry:
+    """ExpDiskGeometry (ExpDiskGeometry.cpp:23-43,117-161): scale length hR, scale height hz, optional
+    truncations."""
+
+    def __init__(self, hR, hz, Rmax=0.0, zmax=0.0, Rmin=0.0):
+        self.hR, self.hz, self.Rmax, self.zmax, self.Rmin = hR, hz, Rmax, zmax, Rmin
+
+    def params(self):
+        p = np.zeros(capi.SG_STAR_NPARAM)
+        p[0] = 1
+        p[1:6] = [self.hR, self.hz, self.Rmax, self.zmax, self.Rmin]
+        return p
+
+    def SigmaZ(self):
+        """Face-on surface density of the normalised geometry (ExpDiskGeometry.cpp:175-187)."""
+        intphi = 2.0 * math.pi
+        intz = -2.0 * self.hz * math.expm1(-self.zmax / self.hz) if self.zmax > 0 else 2.0 * self.hz
+        tmin = math.exp(-self.Rmin / self.hR) * (1.0 + self.Rmin / self.hR) if self.Rmin > 0 else 1.0
+        tmax = math.exp(-self.Rmax / self.hR) * (1.0 + self.Rmax / self.hR) if self.Rmax > 0 else 0.0
+        rho0 = 1.0 / (self.hR * self.hR * (tmin - tmax) * intphi * intz)
+        if self.Rmin > 0:
+            return 0.0
+        if self.zmax > 0:
+            return -2.0 * rho0 * self.hz * math.expm1(-self.zmax / self.hz)
+        return 2.0 * rho0 * self.hz
+
+
+class SersicGeometry:
+    """SersicGeometry (SersicGeometry.cpp:30-42,85-90) with its SersicFunction mass table."""
+
+    def __init__(self, n, reff):
+        self.n, self.reff = n, reff
+        self.s = np.zeros(101)
+        self.M = np.zeros(101)
+        check(lib().sg_sersic_table(n, dptr(self.s), dptr(self.M)))
+
+    def params(self):
+        p = np.zeros(capi.SG_STAR_NPARAM)
+        p[0] = 2
+        p[1:3] = [self.n, self.reff]
+        return p
+
+
+class SpiralStructureGeometryDecorator:
+    """SpiralStructureGeometryDecorator (SpiralStructureGeometryDecorator.cpp:23-40,165-229)."""
+
+    def __init__(self, geometry, arms=1, pitch=0.1745329252, radius=1.0, phase=0.0, perturbWeight=0.3, index=1):
+        self.geometry = geometry
+        self.arms, self.pitch, self.radius, self.phase, self.perturbWeight, self.index = (
+            arms, pitch, radius, phase, perturbWeight, index)
+
+    def params(self):
+        p = self.geometry.params()
+        p[6] = 1
+        p[7:13] = [self.arms, self.pitch, self.radius, self.phase, self.perturbWeight, self.index]
+        return p
+
+    def SigmaZ(self):
+        return self.geometry.SigmaZ()
+
+    @property
+    def s(self):
+        return getattr(self.geometry, "s", None)
+
+    @property
+    def M(self):
+        return getattr(self.geometry, "M", None)
+
+
+def geometry_density(geometry, xyz):
+    xyz = as_f64(xyz).reshape(-1, 3)
+    out = np.zeros(xyz.shape[0])
+    check(lib().sg_geometry_density(dptr(as_f64(geometry.params())), xyz.shape[0], dptr(xyz), dptr(out)))
+    return out
+
+
+# ----------------------------------------------------------------------------------- dust
+
+
+class SimpleOligoDustMix:
+    """SimpleOligoDustMix (SimpleOligoDustMix.cpp:22-51) with the derived per-wavelength
+    properties of DustMix::setupSelfAfter (DustMix.cpp:44-91)."""
+
+    def __init__(self, opacities, albedos, asymmetryParameters):
+        kext = np.atleast_1d(as_f64(opacities))
+        alb = np.atleast_1d(as_f64(albedos))
+        g = np.atleast_1d(as_f64(asymmetryParameters))
+        if not (len(kext) == len(alb) == len(g)):
+            raise FatalError("The number of opacities, albedos and asymmetry parameters differ")
+        kabs = kext * (1.0 - alb)
+        ksca = kext * alb
+        sumabs = 0.0 + kabs
+        sumsca = 0.0 + ksca
+        sumgsca = 0.0 + g * ksca
+        sumext = sumabs + sumsca
+        self.albedo = np.where(sumext != 0, sumsca / np.where(sumext != 0, sumext, 1.0), 0.0)
+        self.asymmpar = np.where(sumsca != 0, sumgsca / np.where(sumsca != 0, sumsca, 1.0), 0.0)
+        mu = 0.0 + 1.0
+        self.kappaext = sumext / mu
+        self.kappaabs = sumabs / mu
+        self.kappasca = sumsca / mu
+
+    def Nlambda(self):
+        return len(self.kappaext)
+
+
+class DustSystem:
+    """OligoDustSystem with one dust component (DustSystem.cpp:935-976, OligoDustSystem.cpp:53-87):
+    the cell densities rho[m] of the component on a grid, and its dust mix."""
+
+    def __init__(self, grid, rho, mix, writeMeanIntensity=False):
+        self.grid = grid
+        self.mix = mix
+        self.rho = as_f64(rho)
+        self.nlambda = mix.Nlambda()
+        h = _handle()
+        check(lib().sg_dust_create(grid._h, dptr(self.rho), len(self.rho), self.nlambda, dptr(mix.kappaext),
+                                   dptr(mix.albedo), dptr(mix.asymmpar), int(bool(writeMeanIntensity)),
+                                   ctypes.byref(h)))
+        self._h = h
+        self.writeMeanIntensity = bool(writeMeanIntensity)
+
+    def storeabsorptionrates(self):
+        return self.writeMeanIntensity
+
+    def opticaldepths(self, rk, ell=0, precision=SG_FP64):
+        """DustSystem::opticaldepth (DustSystem.cpp:960-976) for rays rk[n,6]."""
+        rk = as_f64(rk).reshape(-1, 6)
+        out = np.zeros(rk.shape[0])
+        check(lib().sg_dust_optical_depths(self._h, ell, rk.shape[0], dptr(rk), precision, dptr(out)))
+        return out
+
+    def pathlengths(self, rk, taus, ell=0, precision=SG_FP64):
+        """DustGridPath::pathlength after fillOpticalDepth (DustGridPath.cpp:162-173)."""
+        rk = as_f64(rk).reshape(-1, 6)
+        taus = as_f64(taus)
+        out = np.zeros(rk.shape[0])
+        check(lib().sg_dust_path_lengths(self._h, ell, rk.shape[0], dptr(rk), dptr(taus), precision, dptr(out)))
+        return out
+
+    def Labs(self):
+        """OligoDustSystem::Labs table, (ncells, nlambda) (OligoDustSystem.cpp:82-85)."""
+        out = np.zeros((len(self.rho), self.nlambda))
+        check(lib().sg_dust_read_absorption(self._h, dptr(out)))
+        return out
+
+    def clear(self):
+        check(lib().sg_dust_clear_absorption(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and capi._lib is not None:
+            capi._lib.sg_dust_destroy(self._h)
+            self._h = None
+
+
+def sample_density(grid, geometry, scale, nsamples=100, seed=4357):
+    """Cell densities by DustSystem::setSampleDensityBody (DustSystem.cpp:145-172)."""
+    out = np.zeros(grid.numCells())
+    check(lib().sg_grid_sample_density(grid._h, dptr(as_f64(geometry.params())), scale, nsamples, seed,
+                                       dptr(out)))
+    return out
+
+
+def faceon_normalization(geometry, tau, kappaext):
+    """FaceOnDustCompNormalization::normalizationFactor (FaceOnDustCompNormalization.cpp:67-74)."""
+    return tau / (geometry.SigmaZ() * kappaext)
+
+
+# ----------------------------------------------------------------------------------- stars
+
+
+class OligoStellarComp:
+    """OligoStellarComp (OligoStellarComp.cpp:26-76): a geometry with one luminosity per wavelength
+    (here directly in the caller's power unit)."""
+
+    def __init__(self, geometry, luminosities):
+        self.geometry = geometry
+        self.luminosities = np.atleast_1d(as_f64(luminosities))
+
+
+class StellarSystem:
+    """StellarSystem (StellarSystem.cpp:116-158) with its emission bias (default 0.5)."""
+
+    def __init__(self, components, emissionBias=0.5):
+        self.components = list(components)
+        self.emissionBias = emissionBias
+        nl = len(self.components[0].luminosities)
+        params = np.concatenate([c.geometry.params() for c in self.components])
+        lum = np.concatenate([c.luminosities for c in self.components])
+        ts = [c.geometry.s for c in self.components if getattr(c.geometry, "s", None) is not None]
+        tM = [c.geometry.M for c in self.components if getattr(c.geometry, "M", None) is not None]
+        s_arr = as_f64(np.concatenate(ts)) if ts else np.zeros(1)
+        M_arr = as_f64(np.concatenate(tM)) if tM else np.zeros(1)
+        self._keep = (params, lum, s_arr, M_arr)
+        h = _handle()
+        check(lib().sg_stars_create(len(self.components), dptr(as_f64(params)), nl, dptr(as_f64(lum)),
+                                    dptr(s_arr), dptr(M_arr), 101, emissionBias, ctypes.byref(h)))
+        self._h = h
+        self.nlambda = nl
+
+    def luminosity(self, ell):
+        return sum(c.luminosities[ell] for c in self.components)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and capi._lib is not None:
+            capi._lib.sg_stars_destroy(self._h)
+            self._h = None
+
+
+# ----------------------------------------------------------------------------------- instruments
+
+
+class FrameInstrument:
+    """FrameInstrument (FrameInstrument.cpp:33-47; DistantInstrument.cpp:27-50;
+    SingleFrameInstrument.cpp:26-42,130-147). Angles in radians."""
+
+    def __init__(self, distance, inclination, azimuth, positionangle, pixelsX, pixelsY, fieldOfViewX,
+                 fieldOfViewY, centerX=0.0, centerY=0.0):
+        self.p = np.array([distance, inclination, azimuth, positionangle, pixelsX, pixelsY, fieldOfViewX,
+                           fieldOfViewY, centerX, centerY], dtype=np.float64)
+        self.Nx, self.Ny = int(pixelsX), int(pixelsY)
+
+
+class InstrumentSystem:
+    def __init__(self, instruments, nlambda=1):
+        self.instruments = list(instruments)
+        self.nlambda = nlambda
+        params = as_f64(np.concatenate([i.p for i in self.instruments])) if self.instruments else np.zeros(1)
+        h = _handle()
+        check(lib().sg_instruments_create(len(self.instruments), dptr(params), nlambda, ctypes.byref(h)))
+        self._h = h
+
+    def frame(self, i):
+        """Raw detected luminosities of instrument i, shape (nlambda, Ny, Nx)."""
+        ins = self.instruments[i]
+        out = np.zeros((self.nlambda, ins.Ny, ins.Nx))
+        check(lib().sg_instruments_read(self._h, i, dptr(out)))
+        return out
+
+    def clear(self):
+        check(lib().sg_instruments_clear(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and capi._lib is not None:
+            capi._lib.sg_instruments_destroy(self._h)
+            self._h = None
+
+
+# ----------------------------------------------------------------------------------- simulation
+
+
+class OligoMonteCarloSimulation:
+    """OligoMonteCarloSimulation / MonteCarloSimulation (MonteCarloSimulation.cpp:68-114,241-297) on one
+    GPU. Each packet draws from its own Philox stream keyed by (seed, member)."""
+
+    def __init__(self, dustSystem, stellarSystem, instrumentSystem, packages=1e6, minWeightReduction=1e4,
+                 minScattEvents=0, scattBias=0.5, seed=4357, member=0, tauCut=0.0):
+        self.ds, self.ss, self.instr = dustSystem, stellarSystem, instrumentSystem
+        self.packages = packages
+        self.mc = np.array([packages, minWeightReduction, minScattEvents, scattBias, seed, 0, member, tauCut],
+                           dtype=np.float64)
+        h = _handle()
+        check(lib().sg_sim_create(dustSystem._h if dustSystem is not None else None, stellarSystem._h,
+                                  instrumentSystem._h, dptr(self.mc), ctypes.byref(h)))
+        self._h = h
+        self.counters = None
+
+    def setChunkParams(self, threads=1):
+        """Npp of MonteCarloSimulation::setChunkParams (MonteCarloSimulation.cpp:68-114) for one process."""
+        nl = self.ss.nlambda
+        total = 1 if threads == 1 else math.ceil(max(10.0 * threads / nl, self.packages / 1e7))
+        chunksize = math.ceil(self.packages / total)
+        return total * chunksize
+
+    def run_packets(self, ell, first, count, npp, precision=SG_FP64):
+        """Packets [first, first+count) of wavelength ell with normalisation L(ell)/npp."""
+        c = np.zeros(capi.SG_NCOUNTERS, np.uint64)
+        check(lib().sg_sim_run(self._h, ell, int(first), int(count), int(npp), precision,
+                               c.ctypes.data_as(capi._U64)))
+        self.counters = dict(zip(capi.COUNTER_NAMES, (int(x) for x in c)))
+        return self.counters
+
+    def runstellaremission(self, precision=SG_FP64):
+        """The stellar emission phase (MonteCarloSimulation.cpp:241-255): all wavelengths, Npp packets each."""
+        npp = self.setChunkParams(1)
+        total = None
+        for ell in range(self.ss.nlambda):
+            c = self.run_packets(ell, 0, npp, npp, precision)
+            total = c if total is None else {k: total[k] + c[k] for k in c}
+        self.counters = total
+        return total
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and capi._lib is not None:
+            capi._lib.sg_sim_destroy(self._h)
+            self._h = None
+
+
+def set_tuning(block=0, waves_per_cu=0, compact_lanes=0):
+    check(lib().sg_set_tuning(block, waves_per_cu, compact_lanes))
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    check(lib().sg_device_name(buf, 256))
+    return buf.value.decode()

@@ -1,0 +1,71 @@
+"""Synthetic simulation setups of the benchmark configurations (all lengths in kpc, angles in radians).
+
+config0: plain FP64 reference case -- stellar exponential disc (hR 4, hz 0.4) plus Sersic n=4 bulge
+         (reff 1, B/T 0.2), dust exponential disc (hR 6, hz 0.25, face-on tau_V 1), HG g=0.6 albedo 0.6,
+         Cartesian 50^3 over +-20, three 256x256 frames over 40 kpc at inclinations 0/60/88 deg.
+config1: the same galaxy with a two-arm logarithmic spiral (pitch 20 deg, weight 0.3) on stars and dust,
+         face-on tau in {0.5, 2, 5}, octree to level 8 by dust mass fraction 1e-6, four 512x512 frames at
+         0/45/75/89 deg.
+Sizes can be scaled down for tests; every array a run depends on is kept in the returned dict so that
+the CPU oracle can be given the same inputs.
+"""
+import math
+
+import numpy as np
+
+from . import skirt as S
+from .capi import SG_NEIGHBOR
+
+EXTENT = np.array([-20.0, -20.0, -20.0, 20.0, 20.0, 20.0])
+DISTANCE = 1.0e4   # 10 Mpc in kpc
+
+
+def _galaxy(spiral=False, dust_tau=1.0, bt=0.2, ltot=1.0e10, dust_hR=6.0, dust_hz=0.25):
+    disk = S.ExpDiskGeometry(4.0, 0.4)
+    dgeom = S.ExpDiskGeometry(dust_hR, dust_hz)
+    if spiral:
+        disk = S.SpiralStructureGeometryDecorator(disk, arms=2, pitch=math.radians(20.0), radius=4.0, phase=0.0,
+                                                  perturbWeight=0.3, index=1)
+        dgeom = S.SpiralStructureGeometryDecorator(dgeom, arms=2, pitch=math.radians(20.0), radius=4.0, phase=0.0,
+                                                   perturbWeight=0.3, index=1)
+    comps = [S.OligoStellarComp(disk, [ltot * (1.0 - bt)])]
+    if bt > 0:
+        comps.append(S.OligoStellarComp(S.SersicGeometry(4.0, 1.0), [ltot * bt]))
+    mix = S.SimpleOligoDustMix([1.0], [0.6], [0.6])
+    scale = S.faceon_normalization(dgeom, dust_tau, mix.kappaext[0])
+    return comps, dgeom, mix, scale
+
+
+def _frames(incl_deg, pixels, nlambda=1):
+    ins = [S.FrameInstrument(DISTANCE, math.radians(a), 0.0, 0.0, pixels, pixels, 40.0, 40.0) for a in incl_deg]
+    return S.InstrumentSystem(ins, nlambda)
+
+
+def _finish(d, packages, seed, tau_cut=0.0):
+    d["stars"] = S.StellarSystem(d["comps"], emissionBias=0.5)
+    d["dust"] = S.DustSystem(d["grid"], d["rho"], d["mix"], writeMeanIntensity=d.get("store_abs", False))
+    d["sim"] = S.OligoMonteCarloSimulation(d["dust"], d["stars"], d["instr"], packages=packages,
+                                           minWeightReduction=1e4, minScattEvents=0, scattBias=0.5, seed=seed,
+                                           tauCut=tau_cut)
+    d["packages"] = packages
+    return d
+
+
+def config0(grid_n=50, pixels=256, packages=1e6, seed=4357, nsamples=100, tau=1.0):
+    comps, dgeom, mix, scale = _galaxy(False, tau)
+    grid = S.CartesianDustGrid.uniform(EXTENT, grid_n, grid_n, grid_n)
+    rho = S.sample_density(grid, dgeom, scale, nsamples, seed)
+    d = dict(name="config0", kind="cartesian", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
+             instr=_frames((0.0, 60.0, 88.0), pixels))
+    return _finish(d, packages, seed)
+
+
+def config1(tau=2.0, maxlevel=8, pixels=512, packages=1e8, seed=4357, nsamples=100, search=SG_NEIGHBOR,
+            maxmassfraction=1e-6, tau_cut=0.0):
+    comps, dgeom, mix, scale = _galaxy(True, tau)
+    grid = S.OctTreeDustGrid.subdivide(EXTENT, dgeom, scale, total_mass=scale, minlevel=2, maxlevel=maxlevel,
+                                       maxmassfraction=maxmassfraction, nsamples=nsamples, seed=seed, search=search)
+    rho = S.sample_density(grid, dgeom, scale, nsamples, seed)
+    d = dict(name="config1", kind="octree", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
+             instr=_frames((0.0, 45.0, 75.0, 89.0), pixels))
+    return _finish(d, packages, seed, tau_cut)

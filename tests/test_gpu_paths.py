@@ -1,0 +1,144 @@
+"""DustGrid::path on the GPU against the CPU oracle on identical rays.
+
+FP64: cell-index sequences and segment lengths are compared bit for bit (the walks use only
++ - * / and comparisons, compiled without contraction). Mixed precision: the cell sequences agree
+for the large majority of rays and optical depths to 1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+from skirt7_amd import skirt as S
+from skirt7_amd.capi import SG_FP64, SG_MIXED, SG_NEIGHBOR, SG_TOPDOWN
+from tests import oracle_lib as O
+from tests.helpers import EXT, random_rays, random_tree_flags, voronoi_superset
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_paths_equal(grid, scene, rays, maxseg=4096):
+    counts, m, ds = grid.paths(rays, maxseg, SG_FP64)
+    assert (counts >= 0).all()
+    for i, rk in enumerate(rays):
+        om, ods = scene.path(rk[:3], rk[3:])
+        n = counts[i]
+        assert n == len(om), ("segment count", i, rk.tolist(), n, len(om))
+        assert np.array_equal(m[i, :n], om), ("cells", i, rk.tolist())
+        assert np.array_equal(ds[i, :n], ods), ("lengths", i, rk.tolist(), np.max(np.abs(ds[i, :n] - ods)))
+    return counts
+
+
+def test_cartesian_uniform_bitexact():
+    g = S.CartesianDustGrid.uniform(EXT, 50, 50, 50)
+    sc = O.Scene().cartesian(g.extent, g.xv, g.yv, g.zv)
+    counts = assert_paths_equal(g, sc, random_rays(3000, 1))
+    assert counts.max() > 50
+
+
+def test_cartesian_nonuniform_bitexact():
+    rng = np.random.default_rng(7)
+    ext = np.array([-3.0, -2.0, -5.0, 4.0, 2.5, 1.0])
+    xv = np.concatenate([[ext[0]], np.sort(rng.uniform(ext[0], ext[3], 6)), [ext[3]]])
+    yv = np.concatenate([[ext[1]], np.sort(rng.uniform(ext[1], ext[4], 3)), [ext[4]]])
+    zv = np.concatenate([[ext[2]], np.sort(rng.uniform(ext[2], ext[5], 9)), [ext[5]]])
+    g = S.CartesianDustGrid(ext, xv, yv, zv)
+    sc = O.Scene().cartesian(ext, xv, yv, zv)
+    assert_paths_equal(g, sc, random_rays(2000, 2, ext))
+
+
+@pytest.mark.parametrize("search", [SG_TOPDOWN, SG_NEIGHBOR])
+def test_octree_bitexact(search):
+    flags = random_tree_flags(8, 5, 0.35, seed=11, minlevel=1)
+    g = S.OctTreeDustGrid(EXT, flags, search)
+    sc = O.Scene().tree(8, EXT, flags, search)
+    assert g.numCells() == O.lib().or_scene_num_cells(sc.h)
+    assert_paths_equal(g, sc, random_rays(3000, 3))
+
+
+@pytest.mark.parametrize("search", [SG_TOPDOWN, SG_NEIGHBOR])
+def test_bintree_bitexact(search):
+    flags = random_tree_flags(2, 12, 0.55, seed=12, minlevel=3)
+    g = S.BinTreeDustGrid(EXT, flags, search)
+    sc = O.Scene().tree(2, EXT, flags, search)
+    assert_paths_equal(g, sc, random_rays(3000, 4))
+
+
+def test_voronoi_bitexact():
+    rng = np.random.default_rng(5)
+    sites = rng.uniform(-19.0, 19.0, (120, 3))
+    off, ids = voronoi_superset(sites)
+    g = S.VoronoiDustGrid(EXT, sites, off, ids)
+    sc = O.Scene().voronoi(EXT, sites, off, ids)
+    assert_paths_equal(g, sc, random_rays(1500, 6))
+
+
+def test_whichcell_all_grids():
+    rng = np.random.default_rng(9)
+    pts = rng.uniform(-25, 25, (4000, 3))
+    pts[:50] = np.round(pts[:50] / 5.0) * 5.0
+    cart = S.CartesianDustGrid.uniform(EXT, 20, 30, 10)
+    flags = random_tree_flags(8, 4, 0.4, seed=13)
+    tree = S.OctTreeDustGrid(EXT, flags, SG_TOPDOWN)
+    sites = rng.uniform(-19, 19, (80, 3))
+    off, ids = voronoi_superset(sites)
+    vor = S.VoronoiDustGrid(EXT, sites, off, ids)
+    scenes = [O.Scene().cartesian(cart.extent, cart.xv, cart.yv, cart.zv), O.Scene().tree(8, EXT, flags, 0),
+              O.Scene().voronoi(EXT, sites, off, ids)]
+    for grid, sc in zip([cart, tree, vor], scenes):
+        got = grid.whichcell(pts)
+        ref = np.array([sc.whichcell(p) for p in pts])
+        assert np.array_equal(got, ref)
+
+
+def _dust_pair(grid, scene, seed):
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.0, 0.4, grid.numCells())
+    rho[rng.random(len(rho)) < 0.1] = 0.0
+    mix = S.SimpleOligoDustMix([1.3], [0.6], [0.6])
+    ds = S.DustSystem(grid, rho, mix)
+    scene.dust(rho, mix.kappaext, mix.albedo, mix.asymmpar)
+    return ds
+
+
+@pytest.mark.parametrize("kind", ["cartesian", "octree"])
+def test_opticaldepth_and_pathlength_bitexact(kind):
+    if kind == "cartesian":
+        g = S.CartesianDustGrid.uniform(EXT, 24, 24, 24)
+        sc = O.Scene().cartesian(g.extent, g.xv, g.yv, g.zv)
+    else:
+        flags = random_tree_flags(8, 4, 0.4, seed=21)
+        g = S.OctTreeDustGrid(EXT, flags, SG_NEIGHBOR)
+        sc = O.Scene().tree(8, EXT, flags, SG_NEIGHBOR)
+    ds = _dust_pair(g, sc, 3)
+    rays = random_rays(1500, 8)
+    tau = ds.opticaldepths(rays)
+    ref = np.array([sc.opticaldepth(rk[:3], rk[3:]) for rk in rays])
+    assert np.array_equal(tau, ref)
+    rng = np.random.default_rng(4)
+    targets = ref * rng.random(len(ref))
+    targets[::7] = ref[::7]          # exactly the full optical depth
+    targets[::11] = ref[::11] * 1.5  # beyond the path
+    s = ds.pathlengths(rays, targets)
+    sref = np.array([sc.pathlength(rk[:3], rk[3:], [t])[0] for rk, t in zip(rays, targets)])
+    assert np.array_equal(np.nan_to_num(s, nan=-1.0), np.nan_to_num(sref, nan=-1.0))
+
+
+@pytest.mark.parametrize("kind", ["cartesian", "octree_neighbor", "octree_topdown"])
+def test_mixed_precision_walks_close(kind):
+    if kind == "cartesian":
+        g = S.CartesianDustGrid.uniform(EXT, 50, 50, 50)
+        sc = O.Scene().cartesian(g.extent, g.xv, g.yv, g.zv)
+    else:
+        search = SG_NEIGHBOR if kind.endswith("neighbor") else SG_TOPDOWN
+        flags = random_tree_flags(8, 5, 0.35, seed=11)
+        g = S.OctTreeDustGrid(EXT, flags, search)
+        sc = O.Scene().tree(8, EXT, flags, search)
+    ds = _dust_pair(g, sc, 5)
+    rays = random_rays(3000, 10, special=False)
+    c64, m64, d64 = g.paths(rays, 4096, SG_FP64)
+    c32, m32, d32 = g.paths(rays, 4096, SG_MIXED)
+    same = np.array([c64[i] == c32[i] and np.array_equal(m64[i, :c64[i]], m32[i, :c32[i]]) for i in range(len(rays))])
+    assert same.mean() > 0.98, same.mean()
+    t64 = ds.opticaldepths(rays, 0, SG_FP64)
+    t32 = ds.opticaldepths(rays, 0, SG_MIXED)
+    rel = np.abs(t32 - t64) / np.maximum(t64, 1e-3)
+    assert np.quantile(rel, 0.99) < 1e-4, np.quantile(rel, 0.99)
