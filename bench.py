@@ -1,0 +1,169 @@
+"""Benchmark: photon packets per second on one MI355X for the octree configuration.
+
+Workload (config1): stellar exponential disc + Sersic bulge (B/T 0.2), both discs with a two-arm
+spiral (pitch 20 deg, weight 0.3); dust disc face-on tau (default 2); octree over +-20 kpc refined to
+level 8 by dust mass fraction 1e-6; four 512x512 frames at 0/45/75/89 deg; HG g=0.6, albedo 0.6.
+A step is one pass of the transport loop over a batch of packets (default 2e6) with all inputs
+resident in HBM. The timed region is the mixed-precision path; the FP64 path is timed on one step
+of the same packets, and the image error of mixed vs FP64 is measured on identical packet streams.
+
+Prints one JSON line (see the project notes for the field meanings).
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+# algorithmic bytes per cell crossing (octree leaf walk: 6 bounds + kappa*rho + one neighbour link)
+BYTES_PER_CROSSING = {"mixed": 6 * 4 + 4 + 4, "fp64": 6 * 8 + 8 + 4}
+BYTES_PER_PEELOFF = 16   # FP64 read-modify-write of one pixel
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--packets", type=float, default=2e6, help="packets per step")
+    ap.add_argument("--tau", type=float, default=2.0)
+    ap.add_argument("--maxlevel", type=int, default=8)
+    ap.add_argument("--pixels", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--tau-cut", type=float, default=0.0)
+    return ap.parse_args()
+
+
+def frames(w):
+    ins = w["instr"]
+    return [ins.frame(i)[0].copy() for i in range(len(ins.instruments))]
+
+
+def run_steps(sim, first, nsteps, per, npp, precision):
+    tot = None
+    kernel_s = 0.0
+    for s in range(nsteps):
+        c = sim.run_packets(0, first + s * per, per, npp, precision)
+        kernel_s += c["kernel_us"] * 1e-6
+        tot = c if tot is None else {k: tot[k] + c[k] for k in c}
+    return tot, kernel_s
+
+
+def cpu_baseline(w, seconds):
+    from tests import oracle_lib as O
+    sc = O.scene_from_workload(w)
+    threads = min(os.cpu_count() or 1, 16)
+    mc = np.array(w["sim"].mc)
+    n = 2000.0
+    while True:
+        mc[0] = n
+        t0 = time.perf_counter()
+        _, c = sc.run_reference(mc, threads)
+        dt = time.perf_counter() - t0
+        if dt * 3 > seconds or n >= 1e7:
+            break
+        n = min(1e7, n * max(2.0, 0.9 * seconds / max(dt, 1e-3)))
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": float(c[0]) / dt, "unit": "packets/s", "cores": threads, "kind": "port",
+            "sample": "%d packets of the same workload, MT19937 streams, %d threads on %s, %.1f s" % (
+                int(c[0]), threads, cpu, dt),
+            "crossings_per_s": float(c[1] + c[2]) / dt}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only; run it with --gpus 1 (multi-GPU runs are not part of this build)",
+              file=sys.stderr)
+        sys.exit(2)
+    from skirt7_amd import skirt as S
+    from skirt7_amd import workloads
+    from skirt7_amd.capi import SG_FP64, SG_MIXED
+
+    per = int(a.packets)
+    total = per * (a.steps + a.warmup)
+    t0 = time.perf_counter()
+    w = workloads.config1(tau=a.tau, maxlevel=a.maxlevel, pixels=a.pixels, packages=total, tau_cut=a.tau_cut)
+    ncell = w["grid"].numCells()
+    log("setup: octree with %d leaves built in %.1f s on %s" % (ncell, time.perf_counter() - t0, S.device_name()))
+    sim = w["sim"]
+    npp = total
+
+    # warmup (mixed), then the timed mixed steps
+    run_steps(sim, 0, a.warmup, per, npp, SG_MIXED)
+    w["instr"].clear()
+    t0 = time.perf_counter()
+    cm, kmix = run_steps(sim, a.warmup * per, a.steps, per, npp, SG_MIXED)
+    wall = time.perf_counter() - t0
+    img_mixed = frames(w)
+    pk = cm["packets"]
+    cross = cm["own_crossings"] + cm["peel_crossings"] + cm["rewalk_crossings"]
+    log("mixed: %d packets in %.3f s (kernel %.3f s): %.3e packets/s, %.3e crossings/s" % (
+        pk, wall, kmix, pk / wall, cross / kmix))
+
+    out = {"metric": "packets/s on 1 MI355X, mixed vs FP64 GPU at equal image error; % HBM peak",
+           "value": pk / wall, "unit": "packets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 (walk, weights) + f64 (wall fallback, tallies)", "data": "synthetic",
+           "config": {"workload": "config1 octree spiral galaxy, face-on tau %.3g, %d leaves, %d packets per step, "
+                                  "4x%d^2 frames" % (a.tau, ncell, per, a.pixels),
+                      "packets_per_step": per, "leaves": ncell, "tau": a.tau, "instruments": 4,
+                      "pixels": a.pixels}}
+    bytes_launch = cross * BYTES_PER_CROSSING["mixed"] + cm["peeloffs"] * BYTES_PER_PEELOFF
+    achieved = bytes_launch / kmix
+    out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK, "traffic": None}
+    out["crossings_per_s"] = cross / kmix
+    out["counters"] = cm
+
+    if not a.no_fp64:
+        # FP64 on the first timed step's packets, and the mixed image of the same packets
+        w["instr"].clear()
+        c64, k64 = run_steps(sim, a.warmup * per, 1, per, npp, SG_FP64)
+        img64 = frames(w)
+        w["instr"].clear()
+        run_steps(sim, a.warmup * per, 1, per, npp, SG_MIXED)
+        img32 = frames(w)
+        errs = []
+        for x, y in zip(img64, img32):
+            errs.append(float(np.sqrt(np.mean((x - y) ** 2)) / np.sqrt(np.mean(x ** 2))))
+        fp64_rate = c64["packets"] / k64
+        mix_rate_kernel = pk / kmix
+        out["fp64"] = {"packets_per_s": fp64_rate, "crossings_per_s": (c64["own_crossings"] + c64["peel_crossings"]
+                                                                         + c64["rewalk_crossings"]) / k64,
+                       "speedup_mixed_over_fp64": mix_rate_kernel / fp64_rate,
+                       "image_rms_error_mixed_vs_fp64": errs}
+        log("fp64: %.3e packets/s; mixed/fp64 = %.2fx; image rms error %s" % (
+            fp64_rate, mix_rate_kernel / fp64_rate, ["%.2e" % e for e in errs]))
+
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
+        log("cpu: %.3e packets/s on %d threads" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"]))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

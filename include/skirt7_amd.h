@@ -116,7 +116,8 @@ int sg_instruments_clear(sg_instruments* in);
 
 /* ---- MonteCarloSimulation ----------------------------------------------------------------- */
 /* mc is SG_MC_NPARAM doubles: packages, minWeightReduction, minScattEvents, scattBias, seed,
- * reserved, member, tau_cut (MonteCarloSimulation.hpp:37-66). tau_cut > 0 ends a peel-off walk
+ * emissionBias (informational: the stellar system's own value is used), member, tau_cut
+ * (MonteCarloSimulation.hpp:37-66). tau_cut > 0 ends a peel-off walk
  * once its optical depth exceeds tau_cut (its contribution is below exp(-tau_cut) of L); 0 keeps
  * the reference's full walk. dust may be null (no dust system). */
 int sg_sim_create(sg_dust* dust, sg_stars* stars, sg_instruments* instr, const double* mc, sg_sim** out);

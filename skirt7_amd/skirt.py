@@ -422,8 +422,8 @@ class OligoMonteCarloSimulation:
                  minScattEvents=0, scattBias=0.5, seed=4357, member=0, tauCut=0.0):
         self.ds, self.ss, self.instr = dustSystem, stellarSystem, instrumentSystem
         self.packages = packages
-        self.mc = np.array([packages, minWeightReduction, minScattEvents, scattBias, seed, 0, member, tauCut],
-                           dtype=np.float64)
+        self.mc = np.array([packages, minWeightReduction, minScattEvents, scattBias, seed,
+                            stellarSystem.emissionBias, member, tauCut], dtype=np.float64)
         h = _handle()
         check(lib().sg_sim_create(dustSystem._h if dustSystem is not None else None, stellarSystem._h,
                                   instrumentSystem._h, dptr(self.mc), ctypes.byref(h)))
