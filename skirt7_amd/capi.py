@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libskirt7_amd.so")
+LIB_PATH = os.environ.get("SKIRT7_AMD_LIB", os.path.join(_HERE, "lib", "libskirt7_amd.so"))
 
 SG_FP64 = 0
 SG_MIXED = 1

@@ -30,6 +30,7 @@ struct CartView {
     const double* rho; const float* rhof;
     int Nx, Ny, Nz;
     double xmin, ymin, zmin, xmax, ymax, zmax;
+    float wminf;                     // smallest cell width over the three axes
 };
 
 struct alignas(16) TreeRec32 {       // 64 bytes
@@ -96,6 +97,14 @@ struct StarDev {
     double lum;                      // luminosity at the run's wavelength
 };
 
+// Per-run tables indexed per lane (component, instrument): kept in device memory rather than in
+// the kernel arguments, where per-lane indexing would pull whole arrays into registers.
+struct SceneDev {
+    InstrDev instr[SG_MAX_INSTR];
+    StarDev star[SG_MAX_STARS];
+    double Xcum[SG_MAX_STARS + 1];   // normalised cumulative luminosity
+};
+
 struct McArgs {
     uint64_t first, count;
     double Lpkt, Lthreshold;
@@ -110,16 +119,19 @@ struct McArgs {
     int64_t ncells;
     int has_dust;
     int ni;
-    InstrDev instr[SG_MAX_INSTR];
     int ns;
-    StarDev star[SG_MAX_STARS];
+    const struct SceneDev* scene;    // instruments and stellar components (device memory)
     double emission_bias;
-    double Xcum[SG_MAX_STARS + 1];   // normalised cumulative luminosity
     double Lmean;
     unsigned long long* counters;    // SG_NCOUNTERS
     unsigned long long* next_packet; // work counter
     int* error;
     int compact;
+    // launch states written by the emission kernel for packets [first, first+count):
+    // position, direction and luminosity as 7 arrays of `count` values of the run's real type,
+    // and the number of random draws the emission consumed
+    const void* emitted;
+    const uint32_t* emit_draw;
 };
 
 // ------------------------------------------------------------------------- Philox4x32-10

@@ -127,5 +127,5 @@ struct sg_sim {
     sg_stars* stars;
     sg_instruments* instr;
     double mc[8];
-    sg::DevBuf counters, next, err;
+    sg::DevBuf counters, next, err, scene, cold, args, emitted, emitdraw;
 };

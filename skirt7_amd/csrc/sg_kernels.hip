@@ -209,6 +209,11 @@ int sg_grid_create_cartesian(const double ext[6], const double* xv, int nxv, con
     c.rho = g->rho.as<double>(); c.rhof = g->rhof.as<float>();
     c.Nx = nxv - 1; c.Ny = nyv - 1; c.Nz = nzv - 1;
     c.xmin = ext[0]; c.ymin = ext[1]; c.zmin = ext[2]; c.xmax = ext[3]; c.ymax = ext[4]; c.zmax = ext[5];
+    double wmin = DBL_MAX;
+    for (int i = 0; i + 1 < nxv; i++) wmin = std::min(wmin, xv[i + 1] - xv[i]);
+    for (int i = 0; i + 1 < nyv; i++) wmin = std::min(wmin, yv[i + 1] - yv[i]);
+    for (int i = 0; i + 1 < nzv; i++) wmin = std::min(wmin, zv[i + 1] - zv[i]);
+    c.wminf = (float)wmin;
     *out = g;
     SG_CATCH
 }
@@ -664,6 +669,8 @@ int sg_sim_destroy(sg_sim* s)
 static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, McArgs& a)
 {
     std::memset(&a, 0, sizeof a);
+    SceneDev sd;
+    std::memset(&sd, 0, sizeof sd);
     sg_stars* st = s->stars;
     int ns = (int)(st->params.size() / SG_STAR_NPARAM);
     SG_REQUIRE(ell >= 0 && ell < st->nlambda, "wavelength index out of range");
@@ -686,17 +693,17 @@ static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64
         a.ncells = s->dust->grid->ncells;
     }
     a.ni = s->instr->n;
-    for (int i = 0; i < a.ni; i++) a.instr[i] = s->instr->dev[i];
+    for (int i = 0; i < a.ni; i++) sd.instr[i] = s->instr->dev[i];
     a.ns = ns;
     a.emission_bias = st->bias;
     // StellarSystem setup: normalised cumulative luminosity (NR::cdf)
-    a.Xcum[0] = 0;
-    for (int c = 0; c < ns; c++) a.Xcum[c + 1] = a.Xcum[c] + st->lum[(size_t)c * st->nlambda + ell];
-    for (int c = 0; c <= ns; c++) a.Xcum[c] /= a.Xcum[ns];
+    sd.Xcum[0] = 0;
+    for (int c = 0; c < ns; c++) sd.Xcum[c + 1] = sd.Xcum[c] + st->lum[(size_t)c * st->nlambda + ell];
+    for (int c = 0; c <= ns; c++) sd.Xcum[c] /= sd.Xcum[ns];
     a.Lmean = Ltot / ns;
     for (int c = 0; c < ns; c++) {
         const double* p = &st->params[c * SG_STAR_NPARAM];
-        StarDev& d = a.star[c];
+        StarDev& d = sd.star[c];
         d.kind = (int)p[0];
         for (int i = 0; i < 5; i++) d.p[i] = p[1 + i];
         if (d.kind == 2) { d.p[0] = p[1]; d.p[1] = p[2]; }
@@ -719,18 +726,36 @@ static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64
     a.next_packet = s->next.as<unsigned long long>();
     a.error = s->err.as<int>();
     a.compact = g_tune_compact;
+    if (!s->scene.p) s->scene.alloc(sizeof(SceneDev));
+    SG_HIP(hipMemcpy(s->scene.p, &sd, sizeof sd, hipMemcpyHostToDevice));
+    a.scene = s->scene.as<SceneDev>();
 }
 
 template <class W, bool ABS>
-static void launch_mc(const typename W::View& v, const McArgs& a, int& blocks)
+static void launch_mc(const typename W::View& v, const McArgs& a, int& blocks, DevBuf& cold, DevBuf& args,
+                      int nlds = 0)
 {
-    int bs = 256;
+    using R = typename W::Real;
+    int bs = g_tune_block;
+    size_t shm = hot_words<W>() * 4 * (size_t)bs + (size_t)nlds * sizeof(int32_t);
     int per = 0;
-    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc<W, ABS>, bs, 0));
+    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc<W, ABS>, bs, shm));
     if (per < 1) per = 1;
     if (g_tune_waves > 0) per = std::max(1, std::min(per, g_tune_waves * 64 / bs));
     blocks = per * g_cus;
-    hipLaunchKernelGGL((k_mc<W, ABS>), dim3(blocks), dim3(bs), 0, 0, v, a);
+    int nslots = blocks * bs;
+    size_t need = cold_bytes<R>(nslots);
+    if (cold.bytes < need) cold.alloc(need);
+    R* cr = cold.as<R>();
+    uint32_t* cu = reinterpret_cast<uint32_t*>(cold.as<unsigned char>() + (size_t)nslots * CR_NR * sizeof(R));
+    // device copies of the view and arguments for the re-arming function
+    const size_t off = (sizeof(v) + 255) / 256 * 256;
+    if (args.bytes < off + sizeof(McArgs)) args.alloc(off + sizeof(McArgs));
+    SG_HIP(hipMemcpy(args.p, &v, sizeof(v), hipMemcpyHostToDevice));
+    SG_HIP(hipMemcpy(args.as<unsigned char>() + off, &a, sizeof(McArgs), hipMemcpyHostToDevice));
+    const auto* vd = reinterpret_cast<const typename W::View*>(args.p);
+    const auto* ad = reinterpret_cast<const McArgs*>(args.as<unsigned char>() + off);
+    hipLaunchKernelGGL((k_mc<W, ABS>), dim3(blocks), dim3(bs), shm, 0, v, a, nlds, cr, cu, nslots, vd, ad);
     SG_HIP(hipGetLastError());
 }
 
@@ -741,19 +766,19 @@ template <bool ABS> static void dispatch_mc(sg_sim* s, const McArgs& a, int prec
     if (!g) {
         // no dust system: peel-offs need no walk; any grid kind serves as a placeholder
         CartView v{};
-        launch_mc<CartW64, false>(v, a, blocks);
+        launch_mc<CartW64, false>(v, a, blocks, s->cold, s->args);
         return;
     }
     if (g->kind == GK_CART) {
-        if (mixed) launch_mc<CartW32, ABS>(g->cart, a, blocks);
-        else launch_mc<CartW64, ABS>(g->cart, a, blocks);
+        if (mixed) launch_mc<CartW32, ABS>(g->cart, a, blocks, s->cold, s->args);
+        else launch_mc<CartW64, ABS>(g->cart, a, blocks, s->cold, s->args);
     } else if (g->kind == GK_TREE) {
-        if (!mixed) launch_mc<TreeW64, ABS>(g->tree, a, blocks);
-        else if (g->tree.search == SG_NEIGHBOR) launch_mc<TreeW32, ABS>(g->tree, a, blocks);
-        else launch_mc<TreeW32TopDown, ABS>(g->tree, a, blocks);
+        if (!mixed) launch_mc<TreeW64, ABS>(g->tree, a, blocks, s->cold, s->args, g->tree.search == SG_TOPDOWN ? SG_LDS_NODES : 0);
+        else if (g->tree.search == SG_NEIGHBOR) launch_mc<TreeW32, ABS>(g->tree, a, blocks, s->cold, s->args);
+        else launch_mc<TreeW32TopDown, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
     } else {
-        if (mixed) launch_mc<VoroW<float>, ABS>(g->voro, a, blocks);
-        else launch_mc<VoroW<double>, ABS>(g->voro, a, blocks);
+        if (mixed) launch_mc<VoroW<float>, ABS>(g->voro, a, blocks, s->cold, s->args);
+        else launch_mc<VoroW<double>, ABS>(g->voro, a, blocks, s->cold, s->args);
     }
 }
 extern "C" {
@@ -766,16 +791,31 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
     McArgs a;
     fill_args(s, ell, first, count, npp, a);
     s->counters.zero();
-    s->next.zero();
     s->err.zero();
+    const bool mixed = precision == SG_MIXED;
+    const size_t rsz = mixed ? sizeof(float) : sizeof(double);
+    const uint64_t batch = std::min<uint64_t>(count, (uint64_t)1 << 22);
+    if (s->emitted.bytes < 7 * batch * rsz) s->emitted.alloc(7 * batch * rsz);
+    if (s->emitdraw.bytes < batch * sizeof(uint32_t)) s->emitdraw.alloc(batch * sizeof(uint32_t));
     hipEvent_t e0, e1;
     SG_HIP(hipEventCreate(&e0));
     SG_HIP(hipEventCreate(&e1));
     SG_HIP(hipEventRecord(e0, 0));
     int blocks = 0;
-    if (count > 0) {
-        if (a.store_abs) dispatch_mc<true>(s, a, precision, blocks);
-        else dispatch_mc<false>(s, a, precision, blocks);
+    // the pipeline runs in batches: emission kernel, then the persistent transport kernel
+    for (uint64_t b = 0; b < count; b += batch) {
+        McArgs ab = a;
+        ab.first = first + b;
+        ab.count = std::min(batch, count - b);
+        ab.emitted = s->emitted.p;
+        ab.emit_draw = s->emitdraw.as<uint32_t>();
+        s->next.zero();
+        int eb = (int)std::min<uint64_t>((ab.count + 255) / 256, (uint64_t)g_cus * 16);
+        if (mixed) hipLaunchKernelGGL(k_emit<float>, dim3(eb), dim3(256), 0, 0, ab, s->emitted.as<float>(), s->emitdraw.as<uint32_t>());
+        else hipLaunchKernelGGL(k_emit<double>, dim3(eb), dim3(256), 0, 0, ab, s->emitted.as<double>(), s->emitdraw.as<uint32_t>());
+        SG_HIP(hipGetLastError());
+        if (ab.store_abs) dispatch_mc<true>(s, ab, precision, blocks);
+        else dispatch_mc<false>(s, ab, precision, blocks);
     }
     SG_HIP(hipEventRecord(e1, 0));
     SG_HIP(hipEventSynchronize(e1));

@@ -140,14 +140,34 @@ __device__ __forceinline__ double emit(const McArgs& a, PacketRng& rng, double& 
     if (N > 1) {
         double X = rng.ud();
         if (X < a.emission_bias) h = max(0, min(N - 1, (int)(N * X / a.emission_bias)));
-        else h = locate_clip(a.Xcum, N + 1, (X - a.emission_bias) / (1.0 - a.emission_bias));
-        double Lh = a.star[h].lum;
+        else h = locate_clip(a.scene->Xcum, N + 1, (X - a.emission_bias) / (1.0 - a.emission_bias));
+        double Lh = a.scene->star[h].lum;
         if (!(Lh > 0)) return 0.0;
         L = a.Lpkt * (1.0 / (1.0 - a.emission_bias + a.emission_bias * a.Lmean / Lh));
     }
-    star_position(a.star[h], rng, x, y, z);
+    star_position(a.scene->star[h], rng, x, y, z);
     iso_direction(rng, kx, ky, kz);
     return L;
+}
+
+// Emission stage of the pipeline: StellarSystem::launch for packets [first, first+count), one lane
+// per packet, writing the launch state (structure of arrays in the run's real type) and the
+// number of random draws consumed, so the transport kernel continues each packet's stream.
+template <typename R>
+__global__ void __launch_bounds__(256) k_emit(const McArgs a, R* out, uint32_t* draws)
+{
+    const size_t n = a.count;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        PacketRng rng;
+        rng.start(a.seed, a.member, a.first + i, a.ell);
+        double x = 0, y = 0, z = 0, kx = 0, ky = 0, kz = 0;
+        double L = emit(a, rng, x, y, z, kx, ky, kz);
+        if (!(L > 0)) { L = 0; x = y = z = kx = ky = kz = 0; }
+        out[i] = (R)x; out[n + i] = (R)y; out[2 * n + i] = (R)z;
+        out[3 * n + i] = (R)kx; out[4 * n + i] = (R)ky; out[5 * n + i] = (R)kz;
+        out[6 * n + i] = (R)L;
+        draws[i] = rng.draw;
+    }
 }
 
 // SingleFrameInstrument::pixelondetector (SingleFrameInstrument.cpp:130-147)
@@ -164,17 +184,59 @@ template <typename R> __device__ __forceinline__ int pixel_on_detector(const Ins
 }
 
 // ------------------------------------------------------------------------- lane state
+//
+// Hot state (touched on every crossing) stays in registers: the walker, the stage and the running
+// optical depth / path length. Cold per-packet state (position, direction, luminosity, random
+// stream, peel-off bookkeeping) lives in LDS as structure-of-arrays, one slot per lane, and is only
+// read and written when a lane is re-armed. This keeps the crossing loop's register footprint low
+// enough for several waves per SIMD.
 
-template <class W> struct Lane {
+template <class W> struct Hot {
     using R = typename W::Real;
     W w;
-    PacketRng rng;
-    R L, L0, rx, ry, rz, kx, ky, kz;
-    R tau, s, tprev, sprev, target, Lpeel;
-    int nseg;
-    int stage, peel, peelkind, pixel, nscatt;
-    uint32_t c_pk, c_own, c_peel, c_scat, c_npeel, c_walks, c_re;
+    R tau, s, tprev, sprev, target, L0;
+    int nseg, stage;
+    template <class P> __device__ __forceinline__ void park(P& p)
+    {
+        w.park(p);
+        p.io(tau); p.io(s); p.io(tprev); p.io(sprev); p.io(target); p.io(L0);
+        p.io(nseg); p.io(stage);
+    }
 };
+template <class W> __host__ __device__ constexpr int hot_words()
+{
+    return W::kParkWords + (int)(6 * sizeof(typename W::Real) / 4) + 2;
+}
+
+// Hot state is parked in LDS (structure-of-arrays, one slot per lane) while a wave re-arms lanes,
+// so that none of it is live in registers across the state-machine code.
+template <class W> __device__ __forceinline__ void lds_store(Hot<W>& v, uint32_t* base, int nl, int lane)
+{
+    Park<false> p{base, nl, lane, 0};
+    v.park(p);
+}
+template <class W> __device__ __forceinline__ void lds_load(Hot<W>& v, uint32_t* base, int nl, int lane)
+{
+    Park<true> p{base, nl, lane, 0};
+    v.park(p);
+}
+
+enum ColdR { CR_RX = 0, CR_RY, CR_RZ, CR_KX, CR_KY, CR_KZ, CR_L, CR_LPEEL, CR_NR };
+enum ColdU { CU_PLO = 0, CU_PHI, CU_DRAW, CU_W0, CU_W1, CU_W2, CU_W3, CU_NSCATT, CU_PEEL, CU_PEELKIND, CU_PIXEL,
+             CU_PK, CU_SCAT, CU_NPEEL, CU_WALKS, CU_NU };
+
+template <typename R> struct ColdSlot {
+    R* r;          // CR_NR arrays of nl slots (device memory, one slot per launched lane)
+    uint32_t* u;   // CU_NU arrays of nl slots
+    int nl, lane;
+    __device__ __forceinline__ R& fr(int f) const { return r[f * nl + lane]; }
+    __device__ __forceinline__ uint32_t& fu(int f) const { return u[f * nl + lane]; }
+};
+
+template <typename R> __host__ __device__ constexpr size_t cold_bytes(int nl)
+{
+    return (size_t)nl * (CR_NR * sizeof(R) + CU_NU * sizeof(uint32_t));
+}
 
 __device__ __forceinline__ bool wave_claim(unsigned long long* ctr, uint64_t count, uint64_t& p)
 {
@@ -223,106 +285,132 @@ template <> __device__ __forceinline__ float rexpm1<float>(float x) { return exp
 
 // ------------------------------------------------------------------------- stages
 
+// Re-arms one lane: runs the packet state machine from its current stage until the lane has a
+// walk to do (PEELW, OWN or PROP) or no packets are left (DEAD).
 template <class W, bool ABS>
-__device__ __forceinline__ void service(Lane<W>& s, const typename W::View& g, const int32_t* lds, int nlds, const McArgs& a,
-                        const InstrDev* instr)
+__device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Real>& c, const typename W::View& g,
+                                        const int32_t* lds, int nlds, const McArgs& a, const InstrDev* instr)
 {
     using R = typename W::Real;
     const R albedo = (R)a.albedo, gg = (R)a.g;
-    for (;;) {
-        switch (s.stage) {
+    R rx = c.fr(CR_RX), ry = c.fr(CR_RY), rz = c.fr(CR_RZ);
+    R kx = c.fr(CR_KX), ky = c.fr(CR_KY), kz = c.fr(CR_KZ);
+    R L = c.fr(CR_L), Lpeel = c.fr(CR_LPEEL);
+    PacketRng rng;
+    rng.k0 = a.seed; rng.k1 = a.member; rng.ell = (uint32_t)a.ell;
+    rng.plo = c.fu(CU_PLO); rng.phi = c.fu(CU_PHI); rng.draw = c.fu(CU_DRAW);
+    rng.w0 = c.fu(CU_W0); rng.w1 = c.fu(CU_W1); rng.w2 = c.fu(CU_W2); rng.w3 = c.fu(CU_W3);
+    int nscatt = (int)c.fu(CU_NSCATT), peel = (int)c.fu(CU_PEEL), peelkind = (int)c.fu(CU_PEELKIND);
+    int pixel = (int)c.fu(CU_PIXEL);
+    uint32_t c_pk = 0, c_scat = 0, c_npeel = 0, c_walks = 0;
+    bool armed = false;
+    while (!armed) {
+        switch (h.stage) {
         case ST_NEWPKT: {
             uint64_t p;
-            if (!wave_claim(a.next_packet, a.count, p)) { s.stage = ST_DEAD; return; }
-            s.rng.start(a.seed, a.member, a.first + p, a.ell);
-            double x, y, z, kx, ky, kz;
-            double L = emit(a, s.rng, x, y, z, kx, ky, kz);
-            s.c_pk++;
-            if (!(L > 0)) continue;
-            s.L = (R)L; s.rx = (R)x; s.ry = (R)y; s.rz = (R)z; s.kx = (R)kx; s.ky = (R)ky; s.kz = (R)kz;
-            s.nscatt = 0; s.peel = 0; s.peelkind = 0;
-            s.stage = ST_PEEL;
-            continue;
+            if (!wave_claim(a.next_packet, a.count, p)) { h.stage = ST_DEAD; armed = true; break; }
+            // launch state from the emission kernel; continue the packet's random stream where
+            // the emission left it
+            const R* e = reinterpret_cast<const R*>(a.emitted);
+            const size_t n = a.count;
+            c_pk++;
+            R Le = e[6 * n + p];
+            if (!(Le > (R)0)) break;
+            rx = e[p]; ry = e[n + p]; rz = e[2 * n + p];
+            kx = e[3 * n + p]; ky = e[4 * n + p]; kz = e[5 * n + p];
+            L = Le;
+            rng.start(a.seed, a.member, a.first + p, a.ell);
+            rng.draw = a.emit_draw[p];
+            if (rng.draw & 3u) {
+                rng.w0 = rng.plo; rng.w1 = rng.phi; rng.w2 = rng.draw >> 2; rng.w3 = rng.ell;
+                philox10(rng.w0, rng.w1, rng.w2, rng.w3, rng.k0, rng.k1);
+            }
+            nscatt = 0; peel = 0; peelkind = 0;
+            h.stage = ST_PEEL;
+            break;
         }
         case ST_PEEL: {
-            if (s.peel >= a.ni) {
-                if (s.peelkind == 0) s.stage = a.has_dust ? ST_OWN_START : ST_NEWPKT;
-                else s.stage = ST_SCATTER;
-                continue;
+            if (peel >= a.ni) {
+                if (peelkind == 0) h.stage = a.has_dust ? ST_OWN_START : ST_NEWPKT;
+                else h.stage = ST_SCATTER;
+                break;
             }
-            const InstrDev& in = instr[s.peel];
-            int pix = pixel_on_detector<R>(in, s.rx, s.ry, s.rz);
-            if (pix < 0) { s.peel++; continue; }
-            R Lp = s.L;
-            if (s.peelkind) {
+            const InstrDev& in = instr[peel];
+            int pix = pixel_on_detector<R>(in, rx, ry, rz);
+            if (pix < 0) { peel++; break; }
+            R Lp = L;
+            if (peelkind) {
                 // DustMix::phaseFunctionValue, Henyey-Greenstein (DustMix.cpp:648-653)
-                R cosalpha = s.kx * (R)in.kobs[0] + s.ky * (R)in.kobs[1] + s.kz * (R)in.kobs[2];
+                R cosalpha = kx * (R)in.kobs[0] + ky * (R)in.kobs[1] + kz * (R)in.kobs[2];
                 R t = (R)1.0 + gg * gg - (R)2 * gg * cosalpha;
-                Lp = s.L * (((R)1.0 - gg) * ((R)1.0 + gg) / sqrt(t * t * t));
+                Lp = L * (((R)1.0 - gg) * ((R)1.0 + gg) / sqrt(t * t * t));
             }
-            s.c_npeel++;
+            c_npeel++;
             if (!a.has_dust) {
                 unsafeAtomicAdd(in.frame + (size_t)a.ell * in.Nx * in.Ny + pix, (double)Lp);
-                s.peel++;
-                continue;
+                peel++;
+                break;
             }
-            s.pixel = pix; s.Lpeel = Lp; s.tau = 0;
-            s.w.init(g, lds, nlds, s.rx, s.ry, s.rz, in.kobs[0], in.kobs[1], in.kobs[2]);
-            s.c_walks++;
-            s.stage = ST_PEELW;
-            return;
+            pixel = pix; Lpeel = Lp; h.tau = 0;
+            h.w.init(g, lds, nlds, rx, ry, rz, in.kobs[0], in.kobs[1], in.kobs[2]);
+            c_walks++;
+            h.stage = ST_PEELW;
+            armed = true;
+            break;
         }
         case ST_PEELW_DONE: {
-            const InstrDev& in = instr[s.peel];
+            const InstrDev& in = instr[peel];
             // FrameInstrument::detect (FrameInstrument.cpp:33-47)
-            unsafeAtomicAdd(in.frame + (size_t)a.ell * in.Nx * in.Ny + s.pixel, (double)(s.Lpeel * rexp<R>(-s.tau)));
-            s.peel++;
-            s.stage = ST_PEEL;
-            continue;
+            unsafeAtomicAdd(in.frame + (size_t)a.ell * in.Nx * in.Ny + pixel, (double)(Lpeel * rexp<R>(-h.tau)));
+            peel++;
+            h.stage = ST_PEEL;
+            break;
         }
         case ST_OWN_START: {
-            s.tau = 0; s.L0 = s.L;
-            s.w.init(g, lds, nlds, s.rx, s.ry, s.rz, s.kx, s.ky, s.kz);
-            s.c_walks++;
-            s.stage = ST_OWN;
-            return;
+            h.tau = 0; h.L0 = L;
+            h.w.init(g, lds, nlds, rx, ry, rz, kx, ky, kz);
+            c_walks++;
+            h.stage = ST_OWN;
+            armed = true;
+            break;
         }
         case ST_OWN_DONE: {
             // DustSystem::fillOpticalDepth check (DustSystem.cpp:952-955)
-            R taupath = s.tau;
+            R taupath = h.tau;
             if (taupath < (R)0 || isnan(taupath) || isinf(taupath)) {
                 atomicOr(a.error, 1);
-                s.stage = ST_NEWPKT;
-                continue;
+                h.stage = ST_NEWPKT;
+                break;
             }
             // simulateescapeandabsorption, single component (MonteCarloSimulation.cpp:432-461)
             R expfactor = -rexpm1<R>(-taupath);
-            s.L = s.L0 * albedo * expfactor;
-            if (s.L <= (R)0 || (s.L <= (R)a.Lthreshold && s.nscatt >= a.minfs)) { s.stage = ST_NEWPKT; continue; }
+            L = h.L0 * albedo * expfactor;
+            if (L <= (R)0 || (L <= (R)a.Lthreshold && nscatt >= a.minfs)) { h.stage = ST_NEWPKT; break; }
             // simulatepropagation (MonteCarloSimulation.cpp:513-531)
-            if (taupath == (R)0) { s.peel = 0; s.peelkind = 1; s.stage = ST_PEEL; continue; }
+            if (taupath == (R)0) { peel = 0; peelkind = 1; h.stage = ST_PEEL; break; }
             R tt;
-            if (a.xi == 0.0) tt = exponcutoff(s.rng, taupath);
+            if (a.xi == 0.0) tt = exponcutoff(rng, taupath);
             else {
-                R X = uni<R>(s.rng);
-                tt = (X < (R)a.xi) ? uni<R>(s.rng) * taupath : exponcutoff(s.rng, taupath);
+                R X = uni<R>(rng);
+                tt = (X < (R)a.xi) ? uni<R>(rng) * taupath : exponcutoff(rng, taupath);
                 R p = -rexp<R>(-tt) / rexpm1<R>(-taupath);
                 R q = ((R)1.0 - (R)a.xi) * p + (R)a.xi / taupath;
-                s.L = s.L * (p / q);
+                L = L * (p / q);
             }
-            s.target = tt;
-            s.tau = 0; s.s = 0; s.tprev = 0; s.sprev = 0; s.nseg = 0;
-            s.w.init(g, lds, nlds, s.rx, s.ry, s.rz, s.kx, s.ky, s.kz);
-            s.stage = ST_PROP;
-            return;
+            h.target = tt;
+            h.tau = 0; h.s = 0; h.tprev = 0; h.sprev = 0; h.nseg = 0;
+            h.w.init(g, lds, nlds, rx, ry, rz, kx, ky, kz);
+            h.stage = ST_PROP;
+            armed = true;
+            break;
         }
         case ST_PROP_DONE: {
-            // PhotonPackage::propagate (PhotonPackage.cpp:93-96): s.s holds pathlength(tau)
-            R d = s.s;
-            s.rx = s.rx + d * s.kx; s.ry = s.ry + d * s.ky; s.rz = s.rz + d * s.kz;
-            s.peel = 0; s.peelkind = 1;
-            s.stage = ST_PEEL;
-            continue;
+            // PhotonPackage::propagate (PhotonPackage.cpp:93-96): h.s holds pathlength(tau)
+            R d = h.s;
+            rx = rx + d * kx; ry = ry + d * ky; rz = rz + d * kz;
+            peel = 0; peelkind = 1;
+            h.stage = ST_PEEL;
+            break;
         }
         case ST_SCATTER: {
             // simulatescattering, Henyey-Greenstein (DustMix.cpp:596-601) and Random::direction(k, cos)
@@ -330,15 +418,14 @@ __device__ __forceinline__ void service(Lane<W>& s, const typename W::View& g, c
             R nkx, nky, nkz;
             if (fabs((double)gg) < 1e-6) {
                 double a0, a1, a2;
-                iso_direction(s.rng, a0, a1, a2);
+                iso_direction(rng, a0, a1, a2);
                 nkx = (R)a0; nky = (R)a1; nkz = (R)a2;
             } else {
-                R f = (((R)1.0 - gg) * ((R)1.0 + gg)) / ((R)1.0 - gg + (R)2.0 * gg * uni<R>(s.rng));
+                R f = (((R)1.0 - gg) * ((R)1.0 + gg)) / ((R)1.0 - gg + (R)2.0 * gg * uni<R>(rng));
                 R costheta = ((R)1.0 + gg * gg - f * f) / ((R)2.0 * gg);
-                R phi = (R)(2.0 * M_PI) * uni<R>(s.rng);
+                R phi = (R)(2.0 * M_PI) * uni<R>(rng);
                 R cosphi = cos(phi), sinphi = sin(phi);
                 R sintheta = sqrt(fabs(((R)1.0 - costheta) * ((R)1.0 + costheta)));
-                R kx = s.kx, ky = s.ky, kz = s.kz;
                 if (kz > (R)0.99999) { nkx = cosphi * sintheta; nky = sinphi * sintheta; nkz = costheta; }
                 else if (kz < (R)-0.99999) { nkx = cosphi * sintheta; nky = sinphi * sintheta; nkz = -costheta; }
                 else {
@@ -348,114 +435,186 @@ __device__ __forceinline__ void service(Lane<W>& s, const typename W::View& g, c
                     nkz = root * sintheta * cosphi + kz * costheta;
                 }
             }
-            s.kx = nkx; s.ky = nky; s.kz = nkz;
-            s.nscatt++;
-            s.c_scat++;
-            s.stage = ST_OWN_START;
-            continue;
+            kx = nkx; ky = nky; kz = nkz;
+            nscatt++;
+            c_scat++;
+            h.stage = ST_OWN_START;
+            break;
         }
         default:
-            return;   // walking or dead
+            armed = true;   // walking or dead
+            break;
         }
     }
+    c.fr(CR_RX) = rx; c.fr(CR_RY) = ry; c.fr(CR_RZ) = rz;
+    c.fr(CR_KX) = kx; c.fr(CR_KY) = ky; c.fr(CR_KZ) = kz;
+    c.fr(CR_L) = L; c.fr(CR_LPEEL) = Lpeel;
+    c.fu(CU_PLO) = rng.plo; c.fu(CU_PHI) = rng.phi; c.fu(CU_DRAW) = rng.draw;
+    c.fu(CU_W0) = rng.w0; c.fu(CU_W1) = rng.w1; c.fu(CU_W2) = rng.w2; c.fu(CU_W3) = rng.w3;
+    c.fu(CU_NSCATT) = (uint32_t)nscatt; c.fu(CU_PEEL) = (uint32_t)peel; c.fu(CU_PEELKIND) = (uint32_t)peelkind;
+    c.fu(CU_PIXEL) = (uint32_t)pixel;
+    c.fu(CU_PK) += c_pk; c.fu(CU_SCAT) += c_scat; c.fu(CU_NPEEL) += c_npeel; c.fu(CU_WALKS) += c_walks;
+}
+
+// The re-arming step as a separate (not inlined) function: it reads the lane's parked hot state,
+// runs the state machine and parks the result. Its register demand (FP64 emission and scattering
+// math) then stays out of the crossing loop's allocation; any spilling it needs happens here only.
+template <class W, bool ABS>
+__device__ __noinline__ void service_call(uint32_t* park, int nl, int lane, ColdSlot<typename W::Real> c,
+                                          const typename W::View* g, const int32_t* lds, int nlds, const McArgs* a,
+                                          const InstrDev* instr)
+{
+    Hot<W> q;
+    lds_load(q, park, nl, lane);
+    service<W, ABS>(q, c, *g, lds, nlds, *a, instr);
+    lds_store(q, park, nl, lane);
 }
 
 // One crossing of the current walk; marks the walk finished when the path ends (or, for the
-// propagation re-walk, when the target optical depth is reached).
+// propagation re-walk, when the target optical depth is reached). Returns which kind of cell
+// crossing was made: 0 none (entry segment or end of path), 1 own path, 2 peel-off, 3 re-walk.
 template <class W, bool ABS>
-__device__ __forceinline__ void crossing(Lane<W>& s, const typename W::View& g, const int32_t* lds, int nlds,
-                                         const McArgs& a)
+__device__ __forceinline__ int crossing(Hot<W>& h, const typename W::View& g, const int32_t* lds, int nlds,
+                                        const McArgs& a)
 {
     using R = typename W::Real;
     int m;
     R ds, rho;
-    bool got = s.w.next(g, lds, nlds, m, ds, rho);
+    bool got = h.w.next(g, lds, nlds, m, ds, rho);
     if (!got) {
-        if (s.stage == ST_PROP) {
+        if (h.stage == ST_PROP) {
             // DustGridPath::pathlength end cases (DustGridPath.cpp:162-173, NR.hpp:76-95)
             R res;
-            if (s.nseg == 0 || !(s.target > (R)0)) res = 0;
-            else if (s.target > s.tau) res = s.s;
-            else if (s.nseg == 1) res = s.s;
-            else res = interp_linlin(s.target, s.tprev, s.tau, s.sprev, s.s);
-            s.s = res;
-            s.stage = ST_PROP_DONE;
-        } else if (s.stage == ST_OWN) s.stage = ST_OWN_DONE;
-        else s.stage = ST_PEELW_DONE;
-        return;
+            if (h.nseg == 0 || !(h.target > (R)0)) res = 0;
+            else if (h.target > h.tau) res = h.s;
+            else if (h.nseg == 1) res = h.s;
+            else res = interp_linlin(h.target, h.tprev, h.tau, h.sprev, h.s);
+            h.s = res;
+            h.stage = ST_PROP_DONE;
+        } else if (h.stage == ST_OWN) h.stage = ST_OWN_DONE;
+        else h.stage = ST_PEELW_DONE;
+        return 0;
     }
     R kr = (R)a.kext * rho;        // KappaRho (DustSystem.cpp:459-485): 0 + kappaext*rho, 0 outside
     R dtau = kr * ds;
-    if (s.stage == ST_PEELW) {
-        s.tau += dtau;
-        s.c_peel += (m >= 0);
-        if (a.tau_cut > 0.0 && s.tau > (R)a.tau_cut) s.stage = ST_PEELW_DONE;
-    } else if (s.stage == ST_OWN) {
+    int kind = 0;
+    if (h.stage == ST_PEELW) {
+        h.tau += dtau;
+        kind = 2;
+        if (a.tau_cut > 0.0 && h.tau > (R)a.tau_cut) h.stage = ST_PEELW_DONE;
+    } else if (h.stage == ST_OWN) {
         if (ABS && m >= 0) {
             // OligoDustSystem::absorb via simulateescapeandabsorption (MonteCarloSimulation.cpp:445-458)
             R expfactorm = -rexpm1<R>(-dtau);
-            R Lintm = s.L0 * rexp<R>(-s.tau) * expfactorm;
+            R Lintm = h.L0 * rexp<R>(-h.tau) * expfactorm;
             R Labsm = ((R)1.0 - (R)a.albedo) * Lintm;
             unsafeAtomicAdd(a.labs + (size_t)a.ell * a.ncells + m, (double)Labsm);
         }
-        s.tau += dtau;
-        s.c_own += (m >= 0);
+        h.tau += dtau;
+        kind = 1;
     } else {
         // ST_PROP: cumulative s and tau per segment; stop at the first segment past the target
-        s.tprev = s.tau; s.sprev = s.s;
-        s.s += ds;
-        s.tau += dtau;
-        s.nseg++;
-        s.c_re += (m >= 0);
-        if (s.target > (R)0 && s.tau > s.target) {
-            s.s = (s.nseg == 1) ? interp_linlin(s.target, (R)0, s.tau, (R)0, s.s)
-                                : interp_linlin(s.target, s.tprev, s.tau, s.sprev, s.s);
-            s.stage = ST_PROP_DONE;
+        h.tprev = h.tau; h.sprev = h.s;
+        h.s += ds;
+        h.tau += dtau;
+        h.nseg++;
+        kind = 3;
+        if (h.target > (R)0 && h.tau > h.target) {
+            h.s = (h.nseg == 1) ? interp_linlin(h.target, (R)0, h.tau, (R)0, h.s)
+                                : interp_linlin(h.target, h.tprev, h.tau, h.sprev, h.s);
+            h.stage = ST_PROP_DONE;
         }
     }
+    return m >= 0 ? kind : 0;
 }
 
 __device__ __forceinline__ bool walking_stage(int st) { return st == ST_PEELW || st == ST_OWN || st == ST_PROP; }
 
-template <class W, bool ABS>
-__global__ void __launch_bounds__(256) k_mc(const typename W::View g, const McArgs a)
+__device__ __forceinline__ void add_wave(unsigned long long* dst, unsigned long long x)
 {
-    __shared__ int32_t s_nodes[SG_LDS_NODES];
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t lo = __shfl_down((uint32_t)x, off), hi = __shfl_down((uint32_t)(x >> 32), off);
+        x += ((unsigned long long)hi << 32) | lo;
+    }
+    if (__lane_id() == 0 && x) atomicAdd(dst, x);
+}
+
+
+// Occupancy target of the transport kernel (waves per SIMD) and whether lanes are re-armed by an
+// out-of-line function with the hot state parked in LDS (1) or inline (0).
+#ifndef SG_MC_WAVES_PER_EU
+#define SG_MC_WAVES_PER_EU 2
+#endif
+#ifndef SG_PARK
+#define SG_PARK 0
+#endif
+
+// Persistent transport kernel. Dynamic LDS: the hot-state parking area (hot_words<W>() per lane),
+// then (optionally) the top tree nodes. Cold lane state lives in device memory at cold_r/cold_u.
+template <class W, bool ABS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_MC_WAVES_PER_EU, 8)))
+k_mc(const typename W::View g, const McArgs a, int nlds_req, typename W::Real* cold_r, uint32_t* cold_u,
+     int nslots, const typename W::View* gdev, const McArgs* adev)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
     __shared__ InstrDev s_instr[SG_MAX_INSTR];
+    const int nl = blockDim.x, lane = threadIdx.x;
+    uint32_t* park = reinterpret_cast<uint32_t*>(smem);
+    int32_t* s_nodes = reinterpret_cast<int32_t*>(smem + hot_words<W>() * 4 * nl);
     int nlds = 0;
-    stage_tree_nodes(g, s_nodes, nlds);
-    if (threadIdx.x < SG_MAX_INSTR) s_instr[threadIdx.x] = a.instr[threadIdx.x];
+    if (nlds_req > 0) {
+        stage_tree_nodes(g, s_nodes, nlds);
+        nlds = min(nlds, nlds_req);
+    }
+    if (threadIdx.x < SG_MAX_INSTR) s_instr[threadIdx.x] = a.scene->instr[threadIdx.x];
+    ColdSlot<typename W::Real> c;
+    c.r = cold_r; c.u = cold_u; c.nl = nslots;
+    c.lane = blockIdx.x * blockDim.x + threadIdx.x;
+    c.fu(CU_PK) = 0; c.fu(CU_SCAT) = 0; c.fu(CU_NPEEL) = 0; c.fu(CU_WALKS) = 0;
     __syncthreads();
 
-    Lane<W> s;
-    s.stage = ST_NEWPKT;
-    s.c_pk = s.c_own = s.c_peel = s.c_scat = s.c_npeel = s.c_walks = s.c_re = 0;
-    s.w.nrecompute = 0; s.w.nstuck = 0;
+    Hot<W> h;
+    h.stage = ST_NEWPKT;
+    h.w.nrecompute = 0; h.w.nstuck = 0;
+    unsigned long long n_own = 0, n_peel = 0, n_re = 0;   // wave-uniform counts
     const int thr = a.compact;
     for (;;) {
-        if (!walking_stage(s.stage) && s.stage != ST_DEAD) service<W, ABS>(s, g, s_nodes, nlds, a, s_instr);
-        uint64_t alive = __ballot(s.stage != ST_DEAD);
+        bool need = !walking_stage(h.stage) && h.stage != ST_DEAD;
+#if SG_PARK
+        if (__ballot(need)) {
+            // park every lane's hot state, re-arm the idle lanes, reload
+            lds_store(h, park, nl, lane);
+            if (need) service_call<W, ABS>(park, nl, lane, c, gdev, s_nodes, nlds, adev, s_instr);
+            lds_load(h, park, nl, lane);
+        }
+#else
+        if (need) service<W, ABS>(h, c, g, s_nodes, nlds, a, s_instr);
+#endif
+        uint64_t alive = __ballot(h.stage != ST_DEAD);
         if (!alive) break;
-        int need = max(1, min(thr, __popcll(alive) >> 1));
+        int need_n = max(1, min(thr, __popcll(alive) >> 1));
         for (;;) {
-            if (walking_stage(s.stage)) crossing<W, ABS>(s, g, s_nodes, nlds, a);
-            bool walk = walking_stage(s.stage);
-            uint64_t idle = __ballot(!walk && s.stage != ST_DEAD);
-            if (__popcll(idle) >= need || !__ballot(walk)) break;
+            int kind = 0;
+            if (walking_stage(h.stage)) kind = crossing<W, ABS>(h, g, s_nodes, nlds, a);
+            n_own += __popcll(__ballot(kind == 1));
+            n_peel += __popcll(__ballot(kind == 2));
+            n_re += __popcll(__ballot(kind == 3));
+            bool walk = walking_stage(h.stage);
+            uint64_t idle = __ballot(!walk && h.stage != ST_DEAD);
+            if (__popcll(idle) >= need_n || !__ballot(walk)) break;
         }
     }
-    // per-lane counters -> device totals
-    unsigned long long v[10] = {s.c_pk, s.c_own, s.c_peel, s.c_scat, s.c_npeel, s.w.nstuck, 0, s.c_walks, s.c_re,
-                                s.w.nrecompute};
-#pragma unroll
-    for (int i = 0; i < 10; i++) {
-        unsigned long long x = v[i];
-        for (int off = 32; off > 0; off >>= 1) {
-            uint32_t lo = __shfl_down((uint32_t)x, off), hi = __shfl_down((uint32_t)(x >> 32), off);
-            x += ((unsigned long long)hi << 32) | lo;
-        }
-        if (__lane_id() == 0 && i != 6 && x) atomicAdd(a.counters + i, x);
+    if (__lane_id() == 0) {
+        if (n_own) atomicAdd(a.counters + 1, n_own);
+        if (n_peel) atomicAdd(a.counters + 2, n_peel);
+        if (n_re) atomicAdd(a.counters + 8, n_re);
     }
+    add_wave(a.counters + 0, c.fu(CU_PK));
+    add_wave(a.counters + 3, c.fu(CU_SCAT));
+    add_wave(a.counters + 4, c.fu(CU_NPEEL));
+    add_wave(a.counters + 5, h.w.nstuck);
+    add_wave(a.counters + 7, c.fu(CU_WALKS));
+    add_wave(a.counters + 9, h.w.nrecompute);
 }
 
 } // namespace sg

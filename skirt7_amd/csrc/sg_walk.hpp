@@ -15,6 +15,47 @@
 
 namespace sg {
 
+// Field-by-field parking of lane state in LDS (structure-of-arrays, one slot per lane). LOAD
+// selects the direction. Walkers list their fields in park(); no address of the state is taken,
+// so it stays in registers outside the parking points.
+template <bool LOAD> struct Park {
+    uint32_t* base;
+    int nl, lane, i;
+    __device__ __forceinline__ void w(uint32_t& v)
+    {
+        if (LOAD) v = base[i * nl + lane];
+        else base[i * nl + lane] = v;
+        i++;
+    }
+    __device__ __forceinline__ void io(uint32_t& v) { w(v); }
+    __device__ __forceinline__ void io(int& v)
+    {
+        uint32_t t = (uint32_t)v;
+        w(t);
+        v = (int)t;
+    }
+    __device__ __forceinline__ void io(bool& v)
+    {
+        uint32_t t = v ? 1u : 0u;
+        w(t);
+        v = t != 0;
+    }
+    __device__ __forceinline__ void io(float& v)
+    {
+        uint32_t t = __float_as_uint(v);
+        w(t);
+        v = __uint_as_float(t);
+    }
+    __device__ __forceinline__ void io(double& v)
+    {
+        unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+        w(lo);
+        w(hi);
+        v = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    }
+};
+
 // Stages the first breadth-first tree node words (the top levels) in LDS.
 __device__ __forceinline__ void stage_tree_nodes(const TreeView& t, int32_t* lds, int& n)
 {
@@ -36,6 +77,13 @@ struct CartW64 {
     double e0, e1, e2;
     bool done;
     uint32_t nrecompute, nstuck;
+    static constexpr int kParkWords = 25;
+    template <class P> __device__ __forceinline__ void park(P& p)
+    {
+        p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
+        p.io(i); p.io(j); p.io(k); p.io(ne); p.io(e0); p.io(e1); p.io(e2);
+        p.io(done); p.io(nrecompute); p.io(nstuck);
+    }
 
     __device__ __forceinline__ void push(double ds)
     {
@@ -147,15 +195,14 @@ template <typename T> __device__ __forceinline__ int choose_axis_tree(T dx, T dy
     return 2;
 }
 
-// True when the FP32 exit decision is ambiguous: the two smallest distances agree to 1e-5
-// relative, or the smallest is within 1e-5 of zero relative to the second.
-__device__ __forceinline__ bool ambiguous(float dx, float dy, float dz)
+// True when the FP32 exit decision is ambiguous: the two smallest wall distances agree to 1e-5
+// relative (a corner or edge crossing), or the chosen distance d is within 1e-5 of zero relative
+// to the cell, measured along the exit axis (ka: direction component, wa: cell width on that axis).
+// Distances of directions parallel to a wall are FLT_MAX and never count as ambiguous.
+__device__ __forceinline__ bool ambiguous(float dx, float dy, float dz, float d, float ka, float wa)
 {
-    float a = fminf(dx, fminf(dy, dz));
-    float c = fmaxf(dx, fmaxf(dy, dz));
-    float b = dx + dy + dz - a - c;
-    if (!(b < FLT_MAX)) b = (c < FLT_MAX) ? c : FLT_MAX;
-    return (b - a) <= 1e-5f * b || a <= 1e-5f * b;
+    float mid = fmaxf(fminf(dx, dy), fminf(fmaxf(dx, dy), dz));
+    return (mid < FLT_MAX && (mid - d) <= 1e-5f * mid) || (d * fabsf(ka) <= 1e-5f * wa);
 }
 
 // Mixed-precision Cartesian walk.
@@ -168,6 +215,13 @@ struct CartW32 {
     float e0, e1, e2;
     bool done;
     uint32_t nrecompute, nstuck;
+    static constexpr int kParkWords = 19;
+    template <class P> __device__ __forceinline__ void park(P& p)
+    {
+        p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz); p.io(ix); p.io(iy); p.io(iz);
+        p.io(i); p.io(j); p.io(k); p.io(ne); p.io(e0); p.io(e1); p.io(e2);
+        p.io(done); p.io(nrecompute); p.io(nstuck);
+    }
 
     __device__ __forceinline__ void push(float ds)
     {
@@ -234,7 +288,7 @@ struct CartW32 {
             float dsz = iz != 0.0f ? (zE - z) * iz : FLT_MAX;
             int a = choose_axis_cart(dsx, dsy, dsz);
             float d = a == 0 ? dsx : a == 1 ? dsy : dsz;
-            if (ambiguous(dsx, dsy, dsz)) {
+            if (ambiguous(dsx, dsy, dsz, d, a == 0 ? kx : a == 1 ? ky : kz, g.wminf)) {
                 nrecompute++;
                 double X = x, Y = y, Z = z;
                 double ddx = ix != 0.0f ? (g.xv[ie] - X) / (double)kx : DBL_MAX;
@@ -371,6 +425,14 @@ struct TreeW64 {
     Entry en;
     bool done;
     uint32_t steps, nrecompute, nstuck;
+    static constexpr int kParkWords = 38;
+    template <class P> __device__ __forceinline__ void park(P& p)
+    {
+        p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
+        p.io(lo[0]); p.io(lo[1]); p.io(lo[2]); p.io(hi[0]); p.io(hi[1]); p.io(hi[2]);
+        p.io(rho); p.io(m); p.io(en.ne); p.io(en.e0); p.io(en.e1); p.io(en.e2);
+        p.io(done); p.io(steps); p.io(nrecompute); p.io(nstuck);
+    }
 
     __device__ __forceinline__ void load_cell(const View& t, int c)
     {
@@ -469,6 +531,15 @@ struct TreeW32 {
     float e0, e1, e2;
     bool done;
     uint32_t steps, nrecompute, nstuck;
+    static constexpr int kParkWords = 31;
+    template <class P> __device__ __forceinline__ void park(P& p)
+    {
+        p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz); p.io(ix); p.io(iy); p.io(iz);
+        p.io(lo[0]); p.io(lo[1]); p.io(lo[2]); p.io(hi[0]); p.io(hi[1]); p.io(hi[2]);
+        p.io(rho); p.io(nb[0]); p.io(nb[1]); p.io(nb[2]); p.io(nb[3]); p.io(nb[4]); p.io(nb[5]);
+        p.io(m); p.io(ne); p.io(e0); p.io(e1); p.io(e2);
+        p.io(done); p.io(steps); p.io(nrecompute); p.io(nstuck);
+    }
 
     __device__ __forceinline__ void load_cell(const View& t, int c)
     {
@@ -562,7 +633,8 @@ struct TreeW32 {
             float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
             int a = choose_axis_tree(dsx, dsy, dsz);
             float d = a == 0 ? dsx : a == 1 ? dsy : dsz;
-            if (ambiguous(dsx, dsy, dsz)) {
+            if (ambiguous(dsx, dsy, dsz, d, a == 0 ? kx : a == 1 ? ky : kz,
+                          a == 0 ? hi[0] - lo[0] : a == 1 ? hi[1] - lo[1] : hi[2] - lo[2])) {
                 nrecompute++;
                 double X = x, Y = y, Z = z;
                 double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
@@ -634,7 +706,8 @@ struct TreeW32TopDown : TreeW32 {
             float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
             int a = choose_axis_tree(dsx, dsy, dsz);
             float d = a == 0 ? dsx : a == 1 ? dsy : dsz;
-            if (ambiguous(dsx, dsy, dsz)) {
+            if (ambiguous(dsx, dsy, dsz, d, a == 0 ? kx : a == 1 ? ky : kz,
+                          a == 0 ? hi[0] - lo[0] : a == 1 ? hi[1] - lo[1] : hi[2] - lo[2])) {
                 nrecompute++;
                 double X = x, Y = y, Z = z;
                 double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
@@ -710,6 +783,13 @@ template <typename R> struct VoroW {
     Entry en;
     bool done;
     uint32_t steps, nrecompute, nstuck;
+    static constexpr int kParkWords = 24;
+    template <class P> __device__ __forceinline__ void park(P& p)
+    {
+        p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
+        p.io(mr); p.io(en.ne); p.io(en.e0); p.io(en.e1); p.io(en.e2);
+        p.io(done); p.io(steps); p.io(nrecompute); p.io(nstuck);
+    }
 
     __device__ __forceinline__ void init(const View& g, const int32_t*, int, double rx, double ry, double rz, double dx, double dy,
                          double dz)
