@@ -31,7 +31,7 @@ bool HostTree::build(int ar, const double e[6], const uint8_t* flags, int64_t nn
     dir.assign(1, -1);
     father.assign(1, -1);
     bounds.assign(e, e + 6);
-    std::vector<int16_t> level(1, 0);
+    level.assign(1, 0);
     level.reserve(nnodes);
     child0.reserve(nnodes); dir.reserve(nnodes); father.reserve(nnodes); bounds.reserve(6 * nnodes);
     for (int64_t l = 0; l < (int64_t)child0.size(); l++) {

@@ -33,12 +33,16 @@ struct CartView {
     float wminf;                     // smallest cell width over the three axes
 };
 
+// Mixed-precision walk record, one per tree NODE (breadth-first id). For a leaf, rope[w] is the
+// node across wall w that covers the whole wall (same level or coarser; -1 on the grid boundary):
+// the next cell is found by descending from it, one record load per level.
 struct alignas(16) TreeRec32 {       // 64 bytes
     float lo[3], hi[3];
     float rho;
-    int32_t nb[6];                    // first neighbour cell across each wall (-1: none)
-    int32_t node;                     // node id of this leaf
-    int32_t pad[2];
+    int32_t rope[6];
+    int32_t cell;                     // cell number of a leaf, -1 for an internal node
+    int32_t child0;                   // first child of an internal node, -1 for a leaf
+    int32_t dir;                      // split axis of an internal binary-tree node
 };
 struct alignas(16) TreeRec64 {       // 64 bytes
     double lo[3], hi[3];

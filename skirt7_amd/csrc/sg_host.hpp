@@ -68,6 +68,7 @@ struct HostTree {
     std::vector<int32_t> child0;       // first child id, -1 for leaves
     std::vector<int8_t> dir;           // binary tree split axis
     std::vector<int32_t> father;
+    std::vector<int16_t> level;
     std::vector<double> bounds;        // 6 per node
     std::vector<int32_t> cellnumber;   // per node, -1 for internal nodes
     std::vector<int32_t> idv;          // cell -> node

@@ -238,22 +238,32 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     for (int64_t l = 0; l < nn; l++) words[l] = h.child0[l] >= 0 ? h.child0[l] : -(h.cellnumber[l] + 1);
     g->node.upload(words);
     g->hrec64.assign(nc, TreeRec64{});
-    g->hrec32.assign(nc, TreeRec32{});
+    g->hrec32.assign(nn, TreeRec32{});
     std::vector<int32_t> off(nc * 6 + 1, 0), ids;
     for (int64_t c = 0; c < nc; c++) {
         int l = h.idv[c];
         TreeRec64& r = g->hrec64[c];
-        TreeRec32& f = g->hrec32[c];
-        for (int a = 0; a < 3; a++) {
-            r.lo[a] = h.bounds[6 * l + a]; r.hi[a] = h.bounds[6 * l + 3 + a];
-            f.lo[a] = (float)r.lo[a]; f.hi[a] = (float)r.hi[a];
-        }
-        r.rho = 0; r.node = l; f.rho = 0; f.node = l;
+        for (int a = 0; a < 3; a++) { r.lo[a] = h.bounds[6 * l + a]; r.hi[a] = h.bounds[6 * l + 3 + a]; }
+        r.rho = 0; r.node = l;
         for (int w = 0; w < 6; w++) {
             const auto& lst = h.nb[6 * (size_t)l + w];
             off[c * 6 + w] = (int32_t)ids.size();
             for (int32_t nbn : lst) ids.push_back(h.cellnumber[nbn]);
-            f.nb[w] = lst.empty() ? -1 : h.cellnumber[lst[0]];
+        }
+    }
+    for (int64_t l = 0; l < nn; l++) {
+        TreeRec32& f = g->hrec32[l];
+        for (int a = 0; a < 3; a++) { f.lo[a] = (float)h.bounds[6 * l + a]; f.hi[a] = (float)h.bounds[6 * l + 3 + a]; }
+        f.rho = 0;
+        f.cell = h.cellnumber[l];
+        f.child0 = h.child0[l];
+        f.dir = h.dir[l];
+        for (int w = 0; w < 6; w++) {
+            // rope: the first (largest-overlap) neighbour, raised to this node's level when finer
+            const auto& lst = h.nb[6 * (size_t)l + w];
+            int32_t r = lst.empty() ? -1 : lst[0];
+            while (r >= 0 && h.level[r] > h.level[l]) r = h.father[r];
+            f.rope[w] = (f.cell >= 0) ? r : -1;
         }
     }
     off[nc * 6] = (int32_t)ids.size();
@@ -350,9 +360,9 @@ void sg_grid::set_rho(const double* r)
         std::vector<float> f(r, r + ncells);
         SG_HIP(hipMemcpy(rhof.p, f.data(), ncells * sizeof(float), hipMemcpyHostToDevice));
     } else if (kind == GK_TREE) {
-        for (int64_t c = 0; c < ncells; c++) { hrec64[c].rho = r[c]; hrec32[c].rho = (float)r[c]; }
+        for (int64_t c = 0; c < ncells; c++) { hrec64[c].rho = r[c]; hrec32[hrec64[c].node].rho = (float)r[c]; }
         SG_HIP(hipMemcpy(rec64.p, hrec64.data(), ncells * sizeof(TreeRec64), hipMemcpyHostToDevice));
-        SG_HIP(hipMemcpy(rec32.p, hrec32.data(), ncells * sizeof(TreeRec32), hipMemcpyHostToDevice));
+        SG_HIP(hipMemcpy(rec32.p, hrec32.data(), hrec32.size() * sizeof(TreeRec32), hipMemcpyHostToDevice));
     } else {
         for (int64_t c = 0; c < ncells; c++) hsite[c].rho = r[c];
         SG_HIP(hipMemcpy(site.p, hsite.data(), ncells * sizeof(VoroSite), hipMemcpyHostToDevice));

@@ -516,16 +516,17 @@ struct TreeW64 {
     }
 };
 
-// Mixed-precision tree walk (Neighbor semantics): FP32 position snapped onto the exit wall,
-// the neighbour across that wall taken from the current cell's record, the CSR list and a
-// directed root descent as fallbacks.
+// Mixed-precision tree walk (Neighbor semantics): FP32 position snapped onto the exit wall; the
+// next cell is found by descending from the rope of the exit wall (the node across it that covers
+// the whole wall), one node record per level; a directed root descent covers the rare case of a
+// point that left the rope node's box through an edge or corner.
 struct TreeW32 {
     using Real = float;
     using View = TreeView;
     float x, y, z, kx, ky, kz, ix, iy, iz;
     float lo[3], hi[3];
     float rho;
-    int32_t nb[6];
+    int32_t rope[6];
     int m;
     int ne;
     float e0, e1, e2;
@@ -536,25 +537,53 @@ struct TreeW32 {
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz); p.io(ix); p.io(iy); p.io(iz);
         p.io(lo[0]); p.io(lo[1]); p.io(lo[2]); p.io(hi[0]); p.io(hi[1]); p.io(hi[2]);
-        p.io(rho); p.io(nb[0]); p.io(nb[1]); p.io(nb[2]); p.io(nb[3]); p.io(nb[4]); p.io(nb[5]);
+        p.io(rho); p.io(rope[0]); p.io(rope[1]); p.io(rope[2]); p.io(rope[3]); p.io(rope[4]); p.io(rope[5]);
         p.io(m); p.io(ne); p.io(e0); p.io(e1); p.io(e2);
         p.io(done); p.io(steps); p.io(nrecompute); p.io(nstuck);
     }
 
-    __device__ __forceinline__ void load_cell(const View& t, int c)
+    // Loads node record n (4 x 16 bytes). Returns the first child (>= 0) of an internal node, or -1
+    // after taking over the bounds, density and ropes of a leaf.
+    __device__ __forceinline__ int load_node(const View& t, int n, float* l, float* h)
     {
-        const float4* p = reinterpret_cast<const float4*>(t.rec32 + c);
-        float4 a = p[0], b = p[1], cc = p[2], d = p[3];
-        lo[0] = a.x; lo[1] = a.y; lo[2] = a.z; hi[0] = a.w; hi[1] = b.x; hi[2] = b.y;
-        rho = b.z;
-        nb[0] = __float_as_int(b.w); nb[1] = __float_as_int(cc.x); nb[2] = __float_as_int(cc.y);
-        nb[3] = __float_as_int(cc.z); nb[4] = __float_as_int(cc.w); nb[5] = __float_as_int(d.x);
-        m = c;
-        (void)d;
+        const float4* p = reinterpret_cast<const float4*>(t.rec32 + n);
+        float4 a = p[0], b = p[1], c = p[2], d = p[3];
+        l[0] = a.x; l[1] = a.y; l[2] = a.z; h[0] = a.w; h[1] = b.x; h[2] = b.y;
+        int child0 = __float_as_int(d.z);
+        if (child0 < 0) {
+            lo[0] = l[0]; lo[1] = l[1]; lo[2] = l[2]; hi[0] = h[0]; hi[1] = h[1]; hi[2] = h[2];
+            rho = b.z;
+            rope[0] = __float_as_int(b.w); rope[1] = __float_as_int(c.x); rope[2] = __float_as_int(c.y);
+            rope[3] = __float_as_int(c.z); rope[4] = __float_as_int(c.w); rope[5] = __float_as_int(d.x);
+            m = __float_as_int(d.y);
+        }
+        return child0;
     }
-    // Root descent in FP32 with the exit axis compared in the travel direction, so a point
-    // lying exactly on the exit wall goes to the cell beyond it.
-    __device__ __forceinline__ int descend(const View& t, const int32_t* lds, int nlds, int xa, int xdir)
+    // Descends from node n to the leaf containing the current position; child choice with the
+    // reference's strict "<" against the split point (OctTreeNode.cpp:184-189, BinTreeNode.cpp:326-335).
+    __device__ __forceinline__ void descend_from(const View& t, int n)
+    {
+        float l[3], h[3];
+        int level = 0;
+        int c0;
+        while ((c0 = load_node(t, n, l, h)) >= 0) {
+            if (t.arity == 8) {
+                int q = (x < 0.5f * (l[0] + h[0]) ? 0 : 1) + (y < 0.5f * (l[1] + h[1]) ? 0 : 2) +
+                        (z < 0.5f * (l[2] + h[2]) ? 0 : 4);
+                n = c0 + q;
+            } else {
+                // binary node: split at the midpoint of its recorded axis
+                int a = t.rec32[n].dir;
+                bool low = a == 0 ? x < 0.5f * (l[0] + h[0]) : a == 1 ? y < 0.5f * (l[1] + h[1])
+                                                                      : z < 0.5f * (l[2] + h[2]);
+                n = low ? c0 : c0 + 1;
+            }
+            if (++level > 64) break;
+        }
+    }
+    // Root descent in FP32 with the exit axis compared in the travel direction, so a point lying
+    // exactly on the exit wall goes to the cell beyond it. Returns the leaf's node id or -1.
+    __device__ __forceinline__ int descend_root(const View& t, const int32_t* lds, int nlds, int xa, int xdir)
     {
         const float* e = t.extf;
         float p[3] = {x, y, z};
@@ -564,6 +593,7 @@ struct TreeW32 {
             if (!in) return -1;
         }
         float l[3] = {e[0], e[1], e[2]}, h[3] = {e[3], e[4], e[5]};
+        int id = 0;
         int32_t w = node_word(t, lds, nlds, 0);
         int level = 0;
         while (w >= 0) {
@@ -585,10 +615,11 @@ struct TreeW32 {
                     if (low) h[b] = c; else { l[b] = c; q = 1; }
                 }
             }
-            w = node_word(t, lds, nlds, w + q);
+            id = w + q;
+            w = node_word(t, lds, nlds, id);
             level++;
         }
-        return -w - 1;
+        return id;
     }
     __device__ __forceinline__ void push(float ds)
     {
@@ -600,8 +631,8 @@ struct TreeW32 {
             ne++;
         }
     }
-    __device__ __forceinline__ void init(const View& t, const int32_t* lds, int nlds, double rx, double ry, double rz, double dx,
-                         double dy, double dz)
+    __device__ __forceinline__ void init(const View& t, const int32_t* lds, int nlds, double rx, double ry, double rz,
+                                         double dx, double dy, double dz)
     {
         steps = 0; done = false;
         Entry en;
@@ -613,71 +644,64 @@ struct TreeW32 {
         ix = fabsf(kx) > 1e-15f ? 1.0f / kx : 0.0f;
         iy = fabsf(ky) > 1e-15f ? 1.0f / ky : 0.0f;
         iz = fabsf(kz) > 1e-15f ? 1.0f / kz : 0.0f;
-        int c = descend(t, lds, nlds, -1, 0);
-        if (c < 0) { ne = 0; done = true; return; }
-        load_cell(t, c);
+        int n = descend_root(t, lds, nlds, -1, 0);
+        if (n < 0) { ne = 0; done = true; return; }
+        float l[3], h[3];
+        load_node(t, n, l, h);
     }
-    __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds,
-                                         float& rh)
+    // Chooses the exit wall and advances onto it; returns the wall (0..5) and the segment length.
+    __device__ __forceinline__ int exit_wall(float& d)
+    {
+        float xn = (kx < 0.0f) ? lo[0] : hi[0];
+        float yn = (ky < 0.0f) ? lo[1] : hi[1];
+        float zn = (kz < 0.0f) ? lo[2] : hi[2];
+        float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
+        float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
+        float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
+        int a = choose_axis_tree(dsx, dsy, dsz);
+        d = a == 0 ? dsx : a == 1 ? dsy : dsz;
+        if (ambiguous(dsx, dsy, dsz, d, a == 0 ? kx : a == 1 ? ky : kz,
+                      a == 0 ? hi[0] - lo[0] : a == 1 ? hi[1] - lo[1] : hi[2] - lo[2])) {
+            nrecompute++;
+            double X = x, Y = y, Z = z;
+            double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
+            double ddy = iy != 0.0f ? ((double)yn - Y) / (double)ky : DBL_MAX;
+            double ddz = iz != 0.0f ? ((double)zn - Z) / (double)kz : DBL_MAX;
+            a = choose_axis_tree(ddx, ddy, ddz);
+            d = (float)(a == 0 ? ddx : a == 1 ? ddy : ddz);
+        }
+        float dd = fmaxf(d, 0.0f);
+        if (a == 0) { x = xn; y = fmaf(ky, dd, y); z = fmaf(kz, dd, z); }
+        else if (a == 1) { x = fmaf(kx, dd, x); y = yn; z = fmaf(kz, dd, z); }
+        else { x = fmaf(kx, dd, x); y = fmaf(ky, dd, y); z = zn; }
+        float dir = a == 0 ? kx : a == 1 ? ky : kz;
+        return 2 * a + (dir < 0.0f ? 0 : 1);
+    }
+    __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds, float& rh)
     {
         if (ne) {
             ds = e0; e0 = e1; e1 = e2; ne--; mo = -1; rh = 0.0f;
             return true;
         }
         while (!done) {
-            float xn = (kx < 0.0f) ? lo[0] : hi[0];
-            float yn = (ky < 0.0f) ? lo[1] : hi[1];
-            float zn = (kz < 0.0f) ? lo[2] : hi[2];
-            float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
-            float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
-            float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
-            int a = choose_axis_tree(dsx, dsy, dsz);
-            float d = a == 0 ? dsx : a == 1 ? dsy : dsz;
-            if (ambiguous(dsx, dsy, dsz, d, a == 0 ? kx : a == 1 ? ky : kz,
-                          a == 0 ? hi[0] - lo[0] : a == 1 ? hi[1] - lo[1] : hi[2] - lo[2])) {
-                nrecompute++;
-                double X = x, Y = y, Z = z;
-                double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
-                double ddy = iy != 0.0f ? ((double)yn - Y) / (double)ky : DBL_MAX;
-                double ddz = iz != 0.0f ? ((double)zn - Z) / (double)kz : DBL_MAX;
-                a = choose_axis_tree(ddx, ddy, ddz);
-                d = (float)(a == 0 ? ddx : a == 1 ? ddy : ddz);
-            }
-            float dir = a == 0 ? kx : a == 1 ? ky : kz;
-            int wall = 2 * a + (dir < 0.0f ? 0 : 1);
+            float d;
             int segm = m;
             float segrho = rho;
-            float dd = fmaxf(d, 0.0f);
-            if (a == 0) { x = xn; y = fmaf(ky, dd, y); z = fmaf(kz, dd, z); }
-            else if (a == 1) { x = fmaf(kx, dd, x); y = yn; z = fmaf(kz, dd, z); }
-            else { x = fmaf(kx, dd, x); y = fmaf(ky, dd, y); z = zn; }
-            int oldm = m;
-            int cand = wall == 0 ? nb[0] : wall == 1 ? nb[1] : wall == 2 ? nb[2] : wall == 3 ? nb[3]
-                     : wall == 4 ? nb[4] : nb[5];
-            if (cand < 0) {
-                done = true;   // boundary wall of the grid: no neighbours, the path leaves the grid
-            } else {
-                const TreeRec32& r = t.rec32[cand];
-                bool in = x >= r.lo[0] && x <= r.hi[0] && y >= r.lo[1] && y <= r.hi[1] && z >= r.lo[2] && z <= r.hi[2];
-                if (in) load_cell(t, cand);
+            int wall = exit_wall(d);
+            int r = wall == 0 ? rope[0] : wall == 1 ? rope[1] : wall == 2 ? rope[2] : wall == 3 ? rope[3]
+                  : wall == 4 ? rope[4] : rope[5];
+            if (r < 0) done = true;   // a wall of the grid: the path leaves the grid
+            else {
+                int oldm = m;
+                // the position must lie in the rope node's box (it may have slipped over an edge)
+                const TreeRec32& rr = t.rec32[r];
+                bool in = x >= rr.lo[0] && x <= rr.hi[0] && y >= rr.lo[1] && y <= rr.hi[1] && z >= rr.lo[2] &&
+                          z <= rr.hi[2];
+                if (in) descend_from(t, r);
                 else {
-                    int i0 = t.nboff[oldm * 6 + wall], i1 = t.nboff[oldm * 6 + wall + 1];
-                    bool found = false;
-                    for (int q = i0 + 1; q < i1; q++) {
-                        int c = t.nbids[q];
-                        const TreeRec32& rr = t.rec32[c];
-                        if (x >= rr.lo[0] && x <= rr.hi[0] && y >= rr.lo[1] && y <= rr.hi[1] && z >= rr.lo[2] &&
-                            z <= rr.hi[2]) {
-                            load_cell(t, c);
-                            found = true;
-                            break;
-                        }
-                    }
-                    if (!found) {
-                        int c = descend(t, lds, nlds, a, dir < 0.0f ? -1 : 1);
-                        if (c < 0) done = true;
-                        else load_cell(t, c);
-                    }
+                    int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1);
+                    if (n < 0) done = true;
+                    else descend_from(t, n);
                 }
                 if (!done && m == oldm) { nstuck++; done = true; }
             }
@@ -698,36 +722,16 @@ struct TreeW32TopDown : TreeW32 {
             return true;
         }
         while (!done) {
-            float xn = (kx < 0.0f) ? lo[0] : hi[0];
-            float yn = (ky < 0.0f) ? lo[1] : hi[1];
-            float zn = (kz < 0.0f) ? lo[2] : hi[2];
-            float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
-            float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
-            float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
-            int a = choose_axis_tree(dsx, dsy, dsz);
-            float d = a == 0 ? dsx : a == 1 ? dsy : dsz;
-            if (ambiguous(dsx, dsy, dsz, d, a == 0 ? kx : a == 1 ? ky : kz,
-                          a == 0 ? hi[0] - lo[0] : a == 1 ? hi[1] - lo[1] : hi[2] - lo[2])) {
-                nrecompute++;
-                double X = x, Y = y, Z = z;
-                double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
-                double ddy = iy != 0.0f ? ((double)yn - Y) / (double)ky : DBL_MAX;
-                double ddz = iz != 0.0f ? ((double)zn - Z) / (double)kz : DBL_MAX;
-                a = choose_axis_tree(ddx, ddy, ddz);
-                d = (float)(a == 0 ? ddx : a == 1 ? ddy : ddz);
-            }
-            float dir = a == 0 ? kx : a == 1 ? ky : kz;
+            float d;
             int segm = m;
             float segrho = rho;
-            float dd = fmaxf(d, 0.0f);
-            if (a == 0) { x = xn; y = fmaf(ky, dd, y); z = fmaf(kz, dd, z); }
-            else if (a == 1) { x = fmaf(kx, dd, x); y = yn; z = fmaf(kz, dd, z); }
-            else { x = fmaf(kx, dd, x); y = fmaf(ky, dd, y); z = zn; }
+            int wall = exit_wall(d);
             int oldm = m;
-            int c = descend(t, lds, nlds, a, dir < 0.0f ? -1 : 1);
-            if (c < 0) done = true;
+            int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1);
+            if (n < 0) done = true;
             else {
-                load_cell(t, c);
+                float l[3], h[3];
+                load_node(t, n, l, h);
                 if (m == oldm) { nstuck++; done = true; }
             }
             if (++steps > SG_MAX_STEPS) done = true;
