@@ -154,6 +154,10 @@ int sg_tree_subdivide(int arity, const double ext[6], const double* geom16, doub
  * (Cartesian and tree grids), times scale. */
 int sg_grid_sample_density(sg_grid* g, const double* geom16, double scale, int nsamples, uint32_t seed,
                            double* rho_out);
+/* Voronoi neighbour lists of n sites in the box ext (VoronoiMesh::buildMesh, VoronoiMesh.cpp:297-405):
+ * offsets[n+1] and at most cap ids (walls -1..-6); *nids receives the total; -5 if cap is too small. */
+int sg_voronoi_neighbors(const double ext[6], const double* sites, int64_t n, int64_t* offsets, int32_t* ids,
+                         int64_t cap, int64_t* nids);
 
 #ifdef __cplusplus
 }

@@ -144,13 +144,34 @@ class BinTreeDustGrid(TreeDustGrid):
     arity = 2
 
 
+def voronoi_neighbors(extent, sites):
+    """Neighbour lists of the Voronoi cells of `sites` in the box (VoronoiMesh.cpp:297-405): (offsets, ids)."""
+    e = as_f64(extent)
+    s = as_f64(sites).reshape(-1, 3)
+    n = s.shape[0]
+    off = np.zeros(n + 1, np.int64)
+    cap = 24 * n + 64
+    nids = ctypes.c_int64()
+    while True:
+        ids = np.zeros(cap, np.int32)
+        st = lib().sg_voronoi_neighbors(dptr(e), dptr(s), n, off.ctypes.data_as(capi._I64),
+                                        ids.ctypes.data_as(capi._I32), cap, ctypes.byref(nids))
+        if st == -5:
+            cap = nids.value
+            continue
+        check(st)
+        return off, ids[:nids.value].copy()
+
+
 class VoronoiDustGrid(DustGrid):
     """VoronoiDustGrid / VoronoiMesh (VoronoiMesh.cpp:297-405,541-570,778-873) from generator sites and
-    neighbour lists (walls -1..-6)."""
+    neighbour lists (walls -1..-6); the lists are built from the sites when not given."""
 
-    def __init__(self, extent, sites, nb_offsets, nb_ids):
+    def __init__(self, extent, sites, nb_offsets=None, nb_ids=None):
         self.extent = as_f64(extent)
         self.sites = as_f64(sites).reshape(-1, 3)
+        if nb_offsets is None:
+            nb_offsets, nb_ids = voronoi_neighbors(self.extent, self.sites)
         self.nb_offsets = as_i64(nb_offsets)
         self.nb_ids = as_i32(nb_ids)
         h = _handle()

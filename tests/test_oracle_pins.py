@@ -265,3 +265,32 @@ def test_transport_without_dust_counts_on_frame_emission():
     frames, c = sc.run_packets({"sim": S}, 0, 0, 10000, 10000, threads=2)
     assert abs(frames.sum() - 1.0) < 1e-12
     assert c[3] == 0   # no scattering without dust
+
+
+def test_voronoi_neighbour_builder_gives_exact_paths():
+    """Lists built by cell clipping give the same walks as lists naming every site (bit-exact), and
+    each cell's neighbour relation is symmetric."""
+    from skirt7_amd.skirt import voronoi_neighbors
+    rng = np.random.default_rng(21)
+    for n in (1, 7, 150):
+        sites = rng.uniform(-19.5, 19.5, (n, 3))
+        off, ids = voronoi_neighbors(EXT, sites)
+        assert off[-1] == len(ids)
+        sets = [set(ids[off[i]:off[i + 1]]) for i in range(n)]
+        for i in range(n):
+            for j in sets[i]:
+                if j >= 0:
+                    assert i in sets[j]
+        soff, sids = voronoi_superset(sites)
+        built = O.Scene().voronoi(EXT, sites, off, ids)
+        full = O.Scene().voronoi(EXT, sites, soff, sids)
+        for rk in random_rays(300, 36 + n):
+            a = built.path(rk[:3], rk[3:])
+            b = full.path(rk[:3], rk[3:])
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # mean neighbour count of a Poisson-Voronoi tessellation is about 15.5 (boundary cells have fewer)
+    sites = rng.uniform(-20, 20, (3000, 3))
+    off, ids = voronoi_neighbors(EXT, sites)
+    inner = np.all(np.abs(sites) < 12, axis=1)
+    counts = np.diff(off)[inner]
+    assert 14.5 < counts.mean() < 16.5
