@@ -136,6 +136,21 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
  * grid of the same kind): member i runs packets [0, counts[i]) at wavelength ells[i]. */
 int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const uint64_t* counts, int precision,
                      uint64_t* counters);
+/* ---- FitSKIRT evaluation of a population ------------------------------------------------------ */
+/* GaussianKernel (GaussianKernel.cpp:19-46) normalised to unit sum (ConvolutionKernel.cpp:23-28):
+ * dimension x dimension doubles, pixel (x, y) at x + dimension*y. */
+int sg_gaussian_kernel(double fwhm, int dimension, double* out);
+/* ReferenceImage::chi2 for one stellar component (ReferenceImage.cpp:95-114) over n members: frame
+ * ell of instrument `instrument` of each member's instrument system (raw detected luminosities, as
+ * sg_instruments_read returns them) is convolved with the kx x ky kernel (Convolution::nested_loop,
+ * Convolution.cpp:34-66) and its luminosity scale in [minlum, maxlum] is fitted to the reference
+ * image ref (Ny x Nx) by golden section (GoldenSection.cpp:57-124). lum[i] and chi2[i] receive the
+ * fit; convolved (may be null) receives the n convolved frames with the pixels where ref is zero set
+ * to zero, as GoldenSection::optimize returns them. */
+int sg_fit_frames(int n, sg_instruments* const* instr, int instrument, int ell, const double* kernel, int kx,
+                  int ky, const double* ref, double minlum, double maxlum, double* lum, double* chi2,
+                  double* convolved);
+
 /* Launch tuning used by sg_sim_run (0 = keep current): block size, waves per CU, compaction
  * threshold (lanes that must finish their walk before the wave re-arms them). */
 int sg_set_tuning(int block, int waves_per_cu, int compact_lanes);

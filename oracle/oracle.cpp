@@ -1578,4 +1578,75 @@ double or_hg_value(double g, const double k[3], const double kobs[3])
     return (1.0 - g) * (1.0 + g) / std::sqrt(t * t * t);
 }
 
+// ------------------------------------------------------------------ FitSKIRT evaluation
+
+// GaussianKernel.cpp:19-46, then _data /= _data.sum() (ConvolutionKernel.cpp:23-28)
+void or_gaussian_kernel(double fwhm, int dimension, double* out)
+{
+    double sigma = fwhm / 2.3548;
+    for (int yk = 1; yk < dimension + 1; yk++)
+        for (int xk = 1; xk < dimension + 1; xk++) {
+            int Xi = (dimension + 1) / 2 - xk;
+            int Yi = (dimension + 1) / 2 - yk;
+            out[(xk - 1) + dimension * (yk - 1)] = std::exp(-(Xi * Xi + Yi * Yi) / (2 * sigma * sigma));
+        }
+    double sum = 0;
+    for (int i = 0; i < dimension * dimension; i++) sum += out[i];
+    for (int i = 0; i < dimension * dimension; i++) out[i] /= sum;
+}
+
+// Convolution::nested_loop (Convolution.cpp:34-66): every input pixel scattered through the kernel
+void or_convolve_nested(const double* image, int Nx, int Ny, const double* kernel, int kx, int ky, double* out)
+{
+    std::fill(out, out + (size_t)Nx * Ny, 0.0);
+    for (int l = 0; l < Nx * Ny; l++) {
+        int yi = l / Nx;
+        int xi = l - Nx * yi;
+        for (int xk = 0; xk < kx; xk++)
+            for (int yk = 0; yk < ky; yk++) {
+                int startX = xi - (kx - 1) / 2;
+                int startY = yi - (ky - 1) / 2;
+                if (startX + xk >= 0 && startY + yk >= 0 && startX + xk < Nx && startY + yk < Ny)
+                    out[(startX + xk) + Nx * (startY + yk)] += image[xi + Nx * yi] * kernel[xk + kx * yk];
+            }
+    }
+}
+
+// GoldenSection::function (GoldenSection.cpp:57-79)
+static double gs_function(const double* ref, double* frame, int npix, double x)
+{
+    double chi = 0;
+    for (int m = 0; m < npix; m++) {
+        double total_sim = x * frame[m];
+        double sigma = std::sqrt(std::fabs(ref[m]) + total_sim);
+        if (ref[m] == 0) frame[m] = 0;
+        else chi += std::pow((ref[m] - total_sim) / sigma, 2);
+    }
+    return chi;
+}
+
+// GoldenSection::optimize (GoldenSection.cpp:83-124)
+void or_golden_section(const double* ref, double* frame, int npix, double minlum, double maxlum, double* lum,
+                       double* chi2)
+{
+    std::vector<double> sim(frame, frame + npix);
+    const double GOLD = 0.3819660113;
+    double chi = 1, lumvalue = 1, prev_lumvalue = 1;
+    double a0 = minlum, b0 = maxlum;
+    for (int j = 0; j < 300; j++) {
+        double d = (b0 - a0) * GOLD;
+        double a1 = a0 + d;
+        double b1 = b0 - d;
+        double chia1 = gs_function(ref, sim.data(), npix, a1);
+        double chib1 = gs_function(ref, sim.data(), npix, b1);
+        if (chia1 < chib1) { b0 = b1; lumvalue = a1; chi = chia1; }
+        else { a0 = a1; lumvalue = b1; chi = chib1; }
+        if (std::fabs(prev_lumvalue - lumvalue) / lumvalue <= 1e-8 && j >= 20) j = 300;
+        else prev_lumvalue = lumvalue;
+    }
+    *chi2 = chi;
+    *lum = lumvalue;
+    std::copy(sim.begin(), sim.end(), frame);
+}
+
 } // extern "C"

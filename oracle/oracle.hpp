@@ -74,6 +74,16 @@ void   or_hg_sample(double g, const double k[3], double u1, double u2, double ou
 double or_hg_value(double g, const double k[3], const double kobs[3]);
 const char* or_last_error(void);
 
+// FitSKIRT evaluation (one stellar component)
+//   GaussianKernel::setupSelfBefore (GaussianKernel.cpp:19-46) + ConvolutionKernel::setupSelfAfter
+//   (ConvolutionKernel.cpp:23-28): dimension^2 doubles, pixel (x, y) at x + dimension*y
+void   or_gaussian_kernel(double fwhm, int dimension, double* out);
+//   Convolution::nested_loop (Convolution.cpp:34-66): image Nx x Ny -> out, kernel kx x ky
+void   or_convolve_nested(const double* image, int Nx, int Ny, const double* kernel, int kx, int ky, double* out);
+//   GoldenSection::optimize (GoldenSection.cpp:57-124): frame is updated as the reference returns it
+void   or_golden_section(const double* ref, double* frame, int npix, double minlum, double maxlum, double* lum,
+                         double* chi2);
+
 #ifdef __cplusplus
 }
 #endif

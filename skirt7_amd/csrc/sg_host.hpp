@@ -129,4 +129,7 @@ struct sg_sim {
     sg_instruments* instr;
     double mc[8];
     sg::DevBuf counters, next, err, scene, cold, args, emitted, emitdraw;
+    // work buffers of a batched run, held by the batch's first member: range counters and the
+    // tables of grid views and run arguments
+    sg::DevBuf bnext, btab;
 };

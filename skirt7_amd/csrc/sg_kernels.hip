@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/skirt7_amd.h"
@@ -847,12 +848,170 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
     SG_CATCH
 }
 
+} // extern "C"
+
+// One round of a batched run: emission for every range, then the batched transport kernel. The
+// tables (run arguments, range offsets, grid views) go to the first member's work buffer.
+template <class W, bool ABS>
+static void launch_batch(sg_sim* s0, const std::vector<McArgs>& rg, const std::vector<sg_sim*>& owner, int& blocks)
+{
+    using R = typename W::Real;
+    using V = typename W::View;
+    const size_t nr = rg.size();
+    std::vector<V> views(nr);
+    for (size_t k = 0; k < nr; k++) {
+        if constexpr (std::is_same<V, CartView>::value) views[k] = owner[k]->dust ? owner[k]->dust->grid->cart : CartView{};
+        else if constexpr (std::is_same<V, TreeView>::value) views[k] = owner[k]->dust->grid->tree;
+        else views[k] = owner[k]->dust->grid->voro;
+    }
+    std::vector<uint64_t> roff(nr + 1, 0);
+    for (size_t k = 0; k < nr; k++) roff[k + 1] = roff[k] + rg[k].count;
+    const size_t o_off = (nr * sizeof(McArgs) + 255) / 256 * 256;
+    const size_t o_view = o_off + ((nr + 1) * sizeof(uint64_t) + 255) / 256 * 256;
+    const size_t need = o_view + nr * sizeof(V);
+    if (s0->btab.bytes < need) s0->btab.alloc(need);
+    unsigned char* base = s0->btab.as<unsigned char>();
+    SG_HIP(hipMemcpy(base, rg.data(), nr * sizeof(McArgs), hipMemcpyHostToDevice));
+    SG_HIP(hipMemcpy(base + o_off, roff.data(), (nr + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+    SG_HIP(hipMemcpy(base + o_view, views.data(), nr * sizeof(V), hipMemcpyHostToDevice));
+    const auto* atab = reinterpret_cast<const McArgs*>(base);
+    const auto* otab = reinterpret_cast<const uint64_t*>(base + o_off);
+    const auto* vtab = reinterpret_cast<const V*>(base + o_view);
+
+    int eb = (int)std::min<uint64_t>((roff[nr] + 255) / 256, (uint64_t)g_cus * 16);
+    hipLaunchKernelGGL(k_emit_batch<R>, dim3(eb), dim3(256), 0, 0, atab, otab, (int)nr);
+    SG_HIP(hipGetLastError());
+
+    int bs = g_tune_block;
+    size_t shm = hot_words<W>() * 4 * (size_t)bs;
+    int per = 0;
+    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc_batch<W, ABS>, bs, shm));
+    if (per < 1) per = 1;
+    if (g_tune_waves > 0) per = std::max(1, std::min(per, g_tune_waves * 64 / bs));
+    blocks = per * g_cus;
+    int nslots = blocks * bs;
+    size_t cneed = cold_bytes<R>(nslots);
+    if (s0->cold.bytes < cneed) s0->cold.alloc(cneed);
+    R* cr = s0->cold.as<R>();
+    uint32_t* cu = reinterpret_cast<uint32_t*>(s0->cold.as<unsigned char>() + (size_t)nslots * CR_NR * sizeof(R));
+    hipLaunchKernelGGL((k_mc_batch<W, ABS>), dim3(blocks), dim3(bs), shm, 0, vtab, atab, (int)nr, cr, cu, nslots);
+    SG_HIP(hipGetLastError());
+}
+
+template <bool ABS>
+static void dispatch_batch(sg_sim* s0, const std::vector<McArgs>& rg, const std::vector<sg_sim*>& owner,
+                           int precision, int& blocks)
+{
+    const bool mixed = precision == SG_MIXED;
+    sg_grid* g = s0->dust ? s0->dust->grid : nullptr;
+    if (!g) { launch_batch<CartW64, false>(s0, rg, owner, blocks); return; }
+    if (g->kind == GK_CART) {
+        if (mixed) launch_batch<CartW32, ABS>(s0, rg, owner, blocks);
+        else launch_batch<CartW64, ABS>(s0, rg, owner, blocks);
+    } else if (g->kind == GK_TREE) {
+        if (!mixed) launch_batch<TreeW64, ABS>(s0, rg, owner, blocks);
+        else if (g->tree.search == SG_NEIGHBOR) launch_batch<TreeW32, ABS>(s0, rg, owner, blocks);
+        else launch_batch<TreeW32TopDown, ABS>(s0, rg, owner, blocks);
+    } else {
+        if (mixed) launch_batch<VoroW<float>, ABS>(s0, rg, owner, blocks);
+        else launch_batch<VoroW<double>, ABS>(s0, rg, owner, blocks);
+    }
+}
+
+extern "C" {
+
 int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const uint64_t* counts, int precision,
                      uint64_t* counters)
 {
     SG_TRY
-    (void)nmembers; (void)sims; (void)ells; (void)counts; (void)precision; (void)counters;
-    SG_REQUIRE(false, "batched runs are not available in this build");
+    ensure_init();
+    SG_REQUIRE(nmembers >= 1 && sims && ells && counts, "a batched run needs at least one member");
+    sg_sim* s0 = sims[0];
+    auto kind_of = [](const sg_sim* s) { return s->dust ? (int)s->dust->grid->kind : 0; };
+    auto search_of = [](const sg_sim* s) { return s->dust && s->dust->grid->kind == GK_TREE ? s->dust->grid->tree.search : -1; };
+    auto abs_of = [](const sg_sim* s) { return s->dust ? s->dust->store_abs : 0; };
+    std::vector<const sg_sim*> seen;
+    for (int i = 0; i < nmembers; i++) {
+        SG_REQUIRE(sims[i] != nullptr, "null member simulation");
+        SG_REQUIRE(kind_of(sims[i]) == kind_of(s0) && search_of(sims[i]) == search_of(s0),
+                   "the members of a batched run must have dust grids of one kind (and one tree search method)");
+        SG_REQUIRE(abs_of(sims[i]) == abs_of(s0), "the members of a batched run must agree on storing absorption");
+        SG_REQUIRE(std::find(seen.begin(), seen.end(), sims[i]) == seen.end(),
+                   "a simulation may appear only once in a batched run");
+        seen.push_back(sims[i]);
+    }
+    std::vector<McArgs> base(nmembers);
+    uint64_t total = 0;
+    for (int i = 0; i < nmembers; i++) {
+        fill_args(sims[i], ells[i], 0, counts[i], std::max<uint64_t>(counts[i], 1), base[i]);
+        sims[i]->counters.zero();
+        sims[i]->err.zero();
+        total += counts[i];
+    }
+    const bool mixed = precision == SG_MIXED;
+    const size_t rsz = mixed ? sizeof(float) : sizeof(double);
+    // packets per round: the emission buffer holds one round's launch states
+    const uint64_t B = std::max<uint64_t>(1, std::min<uint64_t>(total, (uint64_t)1 << 23));
+    if (s0->emitted.bytes < 7 * B * rsz) s0->emitted.alloc(7 * B * rsz);
+    if (s0->emitdraw.bytes < B * sizeof(uint32_t)) s0->emitdraw.alloc(B * sizeof(uint32_t));
+    if (s0->bnext.bytes < (size_t)nmembers * sizeof(unsigned long long))
+        s0->bnext.alloc((size_t)nmembers * sizeof(unsigned long long));
+    hipEvent_t e0, e1;
+    SG_HIP(hipEventCreate(&e0));
+    SG_HIP(hipEventCreate(&e1));
+    SG_HIP(hipEventRecord(e0, 0));
+    int blocks = 0;
+    std::vector<uint64_t> done(nmembers, 0);
+    int j = 0;
+    for (;;) {
+        // the next round: the members' remaining packets in order, split where the round is full
+        std::vector<McArgs> rg;
+        std::vector<sg_sim*> owner;
+        uint64_t used = 0;
+        while (j < nmembers && used < B) {
+            uint64_t left = counts[j] - done[j];
+            if (left == 0) { j++; continue; }
+            uint64_t take = std::min(left, B - used);
+            McArgs a = base[j];
+            a.first = done[j];
+            a.count = take;
+            a.emitted = s0->emitted.as<unsigned char>() + 7 * used * rsz;
+            a.emit_draw = s0->emitdraw.as<uint32_t>() + used;
+            a.next_packet = s0->bnext.as<unsigned long long>() + rg.size();
+            rg.push_back(a);
+            owner.push_back(sims[j]);
+            used += take;
+            done[j] += take;
+            if (done[j] == counts[j]) j++;
+        }
+        if (rg.empty()) break;
+        SG_HIP(hipMemset(s0->bnext.p, 0, rg.size() * sizeof(unsigned long long)));
+        if (abs_of(s0)) dispatch_batch<true>(s0, rg, owner, precision, blocks);
+        else dispatch_batch<false>(s0, rg, owner, precision, blocks);
+    }
+    SG_HIP(hipEventRecord(e1, 0));
+    SG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    unsigned long long sum[SG_NCOUNTERS] = {0};
+    int err = 0;
+    for (int i = 0; i < nmembers; i++) {
+        unsigned long long c[SG_NCOUNTERS];
+        SG_HIP(hipMemcpy(c, sims[i]->counters.p, sizeof c, hipMemcpyDeviceToHost));
+        for (int k = 0; k < SG_NCOUNTERS; k++) sum[k] += c[k];
+        int e = 0;
+        SG_HIP(hipMemcpy(&e, sims[i]->err.p, sizeof e, hipMemcpyDeviceToHost));
+        err |= e;
+    }
+    if (counters) {
+        for (int k = 0; k < SG_NCOUNTERS; k++) counters[k] = sum[k];
+        counters[6] = total;
+        counters[10] = (uint64_t)(ms * 1000.0);
+        counters[11] = blocks;
+    }
+    SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH
 }
 

@@ -153,20 +153,47 @@ __device__ __forceinline__ double emit(const McArgs& a, PacketRng& rng, double& 
 // Emission stage of the pipeline: StellarSystem::launch for packets [first, first+count), one lane
 // per packet, writing the launch state (structure of arrays in the run's real type) and the
 // number of random draws consumed, so the transport kernel continues each packet's stream.
+// Emission of packet i of the range of `a`: its launch state goes to `out` (structure of arrays of
+// a.count values) and the number of random draws it consumed to draws[i].
+template <typename R>
+__device__ __forceinline__ void emit_one(const McArgs& a, R* out, uint32_t* draws, size_t i)
+{
+    const size_t n = a.count;
+    PacketRng rng;
+    rng.start(a.seed, a.member, a.first + i, a.ell);
+    double x = 0, y = 0, z = 0, kx = 0, ky = 0, kz = 0;
+    double L = emit(a, rng, x, y, z, kx, ky, kz);
+    if (!(L > 0)) { L = 0; x = y = z = kx = ky = kz = 0; }
+    out[i] = (R)x; out[n + i] = (R)y; out[2 * n + i] = (R)z;
+    out[3 * n + i] = (R)kx; out[4 * n + i] = (R)ky; out[5 * n + i] = (R)kz;
+    out[6 * n + i] = (R)L;
+    draws[i] = rng.draw;
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_emit(const McArgs a, R* out, uint32_t* draws)
 {
-    const size_t n = a.count;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        PacketRng rng;
-        rng.start(a.seed, a.member, a.first + i, a.ell);
-        double x = 0, y = 0, z = 0, kx = 0, ky = 0, kz = 0;
-        double L = emit(a, rng, x, y, z, kx, ky, kz);
-        if (!(L > 0)) { L = 0; x = y = z = kx = ky = kz = 0; }
-        out[i] = (R)x; out[n + i] = (R)y; out[2 * n + i] = (R)z;
-        out[3 * n + i] = (R)kx; out[4 * n + i] = (R)ky; out[5 * n + i] = (R)kz;
-        out[6 * n + i] = (R)L;
-        draws[i] = rng.draw;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.count; i += (size_t)gridDim.x * blockDim.x)
+        emit_one<R>(a, out, draws, i);
+}
+
+// Emission for a table of packet ranges laid end to end (range k owns positions
+// [roff[k], roff[k+1])); each range writes where its own `emitted` and `emit_draw` point.
+template <typename R>
+__global__ void __launch_bounds__(256) k_emit_batch(const McArgs* atab, const uint64_t* roff, int nranges)
+{
+    const uint64_t total = roff[nranges];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nranges - 1;   // the last range that starts at or before i
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (roff[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const McArgs& a = atab[lo];
+        emit_one<R>(a, static_cast<R*>(const_cast<void*>(a.emitted)), const_cast<uint32_t*>(a.emit_draw),
+                    (size_t)(i - roff[lo]));
     }
 }
 
@@ -549,30 +576,17 @@ __device__ __forceinline__ void add_wave(unsigned long long* dst, unsigned long 
 #define SG_PARK 0
 #endif
 
-// Persistent transport kernel. Dynamic LDS: the hot-state parking area (hot_words<W>() per lane),
-// then (optionally) the top tree nodes. Cold lane state lives in device memory at cold_r/cold_u.
+// The lane pipeline of one wave over the packet range of `a`: lanes claim packets until the range
+// is exhausted and every packet the wave claimed has finished; then the wave's event counts go
+// to a.counters.
 template <class W, bool ABS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_MC_WAVES_PER_EU, 8)))
-k_mc(const typename W::View g, const McArgs a, int nlds_req, typename W::Real* cold_r, uint32_t* cold_u,
-     int nslots, const typename W::View* gdev, const McArgs* adev)
+__device__ __forceinline__ void drain(const typename W::View& g, const McArgs& a, uint32_t* park,
+                                      const int32_t* s_nodes, int nlds, const ColdSlot<typename W::Real>& c,
+                                      const InstrDev* s_instr, const typename W::View* gdev, const McArgs* adev)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ InstrDev s_instr[SG_MAX_INSTR];
     const int nl = blockDim.x, lane = threadIdx.x;
-    uint32_t* park = reinterpret_cast<uint32_t*>(smem);
-    int32_t* s_nodes = reinterpret_cast<int32_t*>(smem + hot_words<W>() * 4 * nl);
-    int nlds = 0;
-    if (nlds_req > 0) {
-        stage_tree_nodes(g, s_nodes, nlds);
-        nlds = min(nlds, nlds_req);
-    }
-    if (threadIdx.x < SG_MAX_INSTR) s_instr[threadIdx.x] = a.scene->instr[threadIdx.x];
-    ColdSlot<typename W::Real> c;
-    c.r = cold_r; c.u = cold_u; c.nl = nslots;
-    c.lane = blockIdx.x * blockDim.x + threadIdx.x;
+    (void)park; (void)nl; (void)lane; (void)gdev; (void)adev;
     c.fu(CU_PK) = 0; c.fu(CU_SCAT) = 0; c.fu(CU_NPEEL) = 0; c.fu(CU_WALKS) = 0;
-    __syncthreads();
-
     Hot<W> h;
     h.stage = ST_NEWPKT;
     h.w.nrecompute = 0; h.w.nstuck = 0;
@@ -615,6 +629,71 @@ k_mc(const typename W::View g, const McArgs a, int nlds_req, typename W::Real* c
     add_wave(a.counters + 5, h.w.nstuck);
     add_wave(a.counters + 7, c.fu(CU_WALKS));
     add_wave(a.counters + 9, h.w.nrecompute);
+}
+
+// Persistent transport kernel. Dynamic LDS: the hot-state parking area (hot_words<W>() per lane),
+// then (optionally) the top tree nodes. Cold lane state lives in device memory at cold_r/cold_u.
+template <class W, bool ABS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_MC_WAVES_PER_EU, 8)))
+k_mc(const typename W::View g, const McArgs a, int nlds_req, typename W::Real* cold_r, uint32_t* cold_u,
+     int nslots, const typename W::View* gdev, const McArgs* adev)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ InstrDev s_instr[SG_MAX_INSTR];
+    const int nl = blockDim.x;
+    uint32_t* park = reinterpret_cast<uint32_t*>(smem);
+    int32_t* s_nodes = reinterpret_cast<int32_t*>(smem + hot_words<W>() * 4 * nl);
+    int nlds = 0;
+    if (nlds_req > 0) {
+        stage_tree_nodes(g, s_nodes, nlds);
+        nlds = min(nlds, nlds_req);
+    }
+    if (threadIdx.x < SG_MAX_INSTR) s_instr[threadIdx.x] = a.scene->instr[threadIdx.x];
+    ColdSlot<typename W::Real> c;
+    c.r = cold_r; c.u = cold_u; c.nl = nslots;
+    c.lane = blockIdx.x * blockDim.x + threadIdx.x;
+    __syncthreads();
+    drain<W, ABS>(g, a, park, s_nodes, nlds, c, s_instr, gdev, adev);
+}
+
+// Batched transport over many packet ranges, each with its own grid view and run arguments (one
+// range per member simulation of a population, or per slice of a large member). Each wave takes
+// one range at a time, runs it dry with the lane pipeline above and moves on to the next range that
+// still has packets, so that members too small to fill the device share it. A range whose work
+// counter has reached its count is finished for good, and every visit to an unfinished range
+// advances its counter, so the scan ends.
+template <class W, bool ABS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_MC_WAVES_PER_EU, 8)))
+k_mc_batch(const typename W::View* gtab, const McArgs* atab, int nranges, typename W::Real* cold_r,
+           uint32_t* cold_u, int nslots)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ InstrDev s_instr[4 * SG_MAX_INSTR];   // one instrument table per wave of the block
+    uint32_t* park = reinterpret_cast<uint32_t*>(smem);
+    InstrDev* my_instr = s_instr + (threadIdx.x >> 6) * SG_MAX_INSTR;
+    ColdSlot<typename W::Real> c;
+    c.r = cold_r; c.u = cold_u; c.nl = nslots;
+    c.lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+    int r = (int)(((long long)wave * nranges) / nwaves);
+    for (int scanned = 0; scanned < nranges;) {
+        r = __builtin_amdgcn_readfirstlane(r);
+        const McArgs* ap = atab + r;
+        unsigned long long taken = __hip_atomic_load(ap->next_packet, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (taken >= ap->count) {
+            r = (r + 1 == nranges) ? 0 : r + 1;
+            scanned++;
+            continue;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (__lane_id() < SG_MAX_INSTR) my_instr[__lane_id()] = ap->scene->instr[__lane_id()];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        drain<W, ABS>(gtab[r], *ap, park, nullptr, 0, c, my_instr, gtab + r, ap);
+        scanned = 0;
+    }
 }
 
 } // namespace sg

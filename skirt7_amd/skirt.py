@@ -482,6 +482,58 @@ class OligoMonteCarloSimulation:
             self._h = None
 
 
+def run_batch(simulations, counts, ells=None, precision=SG_FP64):
+    """Batched population: simulation i runs packets [0, counts[i]) at wavelength ells[i] (default 0)
+    with normalisation L/counts[i], all members in one persistent launch per round. Each member's result
+    equals its own run_packets(ell, 0, count, count); the returned counters are summed over members."""
+    sims = list(simulations)
+    n = len(sims)
+    if n == 0:
+        raise ValueError("a batched run needs at least one member")
+    counts = np.ascontiguousarray(counts, dtype=np.uint64)
+    ells = np.zeros(n, np.int32) if ells is None else np.ascontiguousarray(ells, dtype=np.int32)
+    if counts.shape != (n,) or ells.shape != (n,):
+        raise ValueError("counts and ells need one entry per member")
+    handles = (ctypes.c_void_p * n)(*[s._h.value for s in sims])
+    c = np.zeros(capi.SG_NCOUNTERS, np.uint64)
+    check(lib().sg_sim_run_batch(n, handles, ells.ctypes.data_as(capi._I32), counts.ctypes.data_as(capi._U64),
+                                 precision, c.ctypes.data_as(capi._U64)))
+    return dict(zip(capi.COUNTER_NAMES, (int(x) for x in c)))
+
+
+# ----------------------------------------------------------------------- FitSKIRT evaluation
+
+
+class GaussianKernel:
+    """GaussianKernel (FitSKIRTcore/GaussianKernel.cpp:19-46), normalised to unit sum as
+    ConvolutionKernel::setupSelfAfter does (ConvolutionKernel.cpp:23-28). data[y, x]."""
+
+    def __init__(self, fwhm=3.0, dimension=6):
+        self.fwhm, self.dimension = float(fwhm), int(dimension)
+        self.data = np.zeros((self.dimension, self.dimension))
+        check(lib().sg_gaussian_kernel(self.fwhm, self.dimension, dptr(self.data)))
+
+
+def fit_frames(instrument_systems, reference, kernel, minlum, maxlum, instrument=0, ell=0, return_frames=False):
+    """ReferenceImage::chi2 for one stellar component (ReferenceImage.cpp:95-114) over a population: each
+    member's frame is convolved with the kernel and its luminosity scale is fitted to the reference image by
+    golden section. Returns (lum, chi2) arrays, plus the convolved, masked frames when asked."""
+    systems = list(instrument_systems)
+    n = len(systems)
+    if n == 0:
+        raise ValueError("a fit needs at least one member")
+    ref = as_f64(reference)
+    k = as_f64(kernel.data if isinstance(kernel, GaussianKernel) else kernel)
+    if k.ndim != 2 or ref.ndim != 2:
+        raise ValueError("kernel and reference must be 2-D (y, x) arrays")
+    handles = (ctypes.c_void_p * n)(*[s._h.value for s in systems])
+    lum, chi2 = np.zeros(n), np.zeros(n)
+    frames = np.zeros((n,) + ref.shape) if return_frames else None
+    check(lib().sg_fit_frames(n, handles, instrument, ell, dptr(k), k.shape[1], k.shape[0], dptr(ref), float(minlum),
+                              float(maxlum), dptr(lum), dptr(chi2), dptr(frames) if return_frames else None))
+    return (lum, chi2, frames) if return_frames else (lum, chi2)
+
+
 def set_tuning(block=0, waves_per_cu=0, compact_lanes=0):
     check(lib().sg_set_tuning(block, waves_per_cu, compact_lanes))
 

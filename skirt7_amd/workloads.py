@@ -6,6 +6,8 @@ config0: plain FP64 reference case -- stellar exponential disc (hR 4, hz 0.4) pl
 config1: the same galaxy with a two-arm logarithmic spiral (pitch 20 deg, weight 0.3) on stars and dust,
          face-on tau in {0.5, 2, 5}, octree to level 8 by dust mass fraction 1e-6, four 512x512 frames at
          0/45/75/89 deg.
+config3: a population of small Cartesian galaxies run in one batched launch, each fitted to a
+         Poisson-noised target by PSF convolution and golden section (FitSKIRT's evaluation step).
 Sizes can be scaled down for tests; every array a run depends on is kept in the returned dict so that
 the CPU oracle can be given the same inputs.
 """
@@ -69,3 +71,45 @@ def config1(tau=2.0, maxlevel=8, pixels=512, packages=1e8, seed=4357, nsamples=1
     d = dict(name="config1", kind="octree", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
              instr=_frames((0.0, 45.0, 75.0, 89.0), pixels))
     return _finish(d, packages, seed, tau_cut)
+
+
+def config3(members=256, packets=(1.0e5, 1.0e7), grid_n=(32, 128), pixels=(256, 128), seed=4357, nsamples=8,
+            fwhm=3.0, kernel_dim=6, target_counts=1.0e7, store_target=True):
+    """Batched population: `members` galaxies (Cartesian grids with N per axis drawn uniformly from grid_n,
+    dust hR U(2,8), hz U(0.1,0.6), face-on tau U(0.1,5), inclination U(85,90) deg, B/T U(0,0.5), packets
+    log-uniform over `packets`), one pixels[0] x pixels[1] frame each (40 x 20 kpc), plus one extra member
+    drawn the same way that serves as the fit target. Member i runs with seed+1+i, the target with seed; the
+    parameter draws use numpy's default generator seeded with seed. The Gaussian PSF (fwhm in pixels,
+    kernel_dim^2) and the fit interval are included; the target image itself is made by target_image() after
+    the target member has run."""
+    rng = np.random.default_rng(seed)
+    nx_pix, ny_pix = pixels
+    out = []
+    for i in range(members + 1):
+        hR, hz = rng.uniform(2.0, 8.0), rng.uniform(0.1, 0.6)
+        tau, incl, bt = rng.uniform(0.1, 5.0), rng.uniform(85.0, 90.0), rng.uniform(0.0, 0.5)
+        nx, ny, nz = (int(v) for v in rng.integers(grid_n[0], grid_n[1] + 1, size=3))
+        npk = int(round(math.exp(rng.uniform(math.log(packets[0]), math.log(packets[1])))))
+        is_target = i == members
+        mseed = seed if is_target else seed + 1 + i
+        comps, dgeom, mix, scale = _galaxy(False, tau, bt=bt, dust_hR=hR, dust_hz=hz)
+        grid = S.CartesianDustGrid.uniform(EXTENT, nx, ny, nz)
+        rho = S.sample_density(grid, dgeom, scale, nsamples, mseed)
+        instr = S.InstrumentSystem([S.FrameInstrument(DISTANCE, math.radians(incl), 0.0, 0.0, nx_pix, ny_pix, 40.0,
+                                                      40.0 * ny_pix / nx_pix)])
+        d = dict(name="config3-target" if is_target else "config3-member%d" % i, kind="cartesian", grid=grid,
+                 rho=rho, mix=mix, comps=comps, dgeom=dgeom, instr=instr,
+                 params=dict(hR=hR, hz=hz, tau=tau, inclination=incl, bt=bt, grid=(nx, ny, nz)))
+        out.append(_finish(d, npk, mseed))
+    target = out.pop()
+    kernel = S.GaussianKernel(fwhm, kernel_dim)
+    return dict(name="config3", members=out, target=target, kernel=kernel, target_counts=target_counts)
+
+
+def target_image(frame, counts, seed=4357):
+    """Synthetic observation: frame (Ny, Nx) scaled to `counts` in total and Poisson-noised; returns the image
+    and the luminosity-to-counts gain it used."""
+    total = float(np.sum(frame))
+    gain = counts / total if total > 0 else 1.0
+    rng = np.random.default_rng(seed)
+    return rng.poisson(np.clip(frame, 0.0, None) * gain).astype(np.float64), gain

@@ -41,6 +41,9 @@ _SIGS = {
     "or_sersic_table": (None, [ctypes.c_double, _D, _D]),
     "or_hg_sample": (None, [ctypes.c_double, _D, ctypes.c_double, ctypes.c_double, _D]),
     "or_hg_value": (ctypes.c_double, [ctypes.c_double, _D, _D]),
+    "or_gaussian_kernel": (None, [ctypes.c_double, ctypes.c_int, _D]),
+    "or_convolve_nested": (None, [_D, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, _D]),
+    "or_golden_section": (None, [_D, _D, ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D]),
     "or_last_error": (ctypes.c_char_p, []),
     "or_nr_locate_clip": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_double]),
     "or_nr_locate_fail": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_double]),
@@ -178,6 +181,32 @@ class Scene:
         assert rc == 0, lib().or_last_error()
         self.labs = labs
         return frames, c
+
+
+def gaussian_kernel(fwhm, dimension):
+    out = np.zeros((dimension, dimension))
+    lib().or_gaussian_kernel(fwhm, dimension, dp(out))
+    return out
+
+
+def convolve(image, kernel):
+    """Convolution::nested_loop of image (Ny, Nx) with kernel (ky, kx)."""
+    image, kernel = f64(image), f64(kernel)
+    out = np.zeros_like(image)
+    Ny, Nx = image.shape
+    ky, kx = kernel.shape
+    lib().or_convolve_nested(dp(image), Nx, Ny, dp(kernel), kx, ky, dp(out))
+    return out
+
+
+def golden_section(ref, frame, minlum, maxlum):
+    """GoldenSection::optimize: (lum, chi2, frame as returned)."""
+    ref = f64(ref)
+    fr = f64(frame).copy()
+    lum = np.zeros(1)
+    chi2 = np.zeros(1)
+    lib().or_golden_section(dp(ref), dp(fr), ref.size, minlum, maxlum, dp(lum), dp(chi2))
+    return float(lum[0]), float(chi2[0]), fr
 
 
 def scene_from_workload(w):
