@@ -46,7 +46,23 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--tau-cut", type=float, default=0.0)
+    ap.add_argument("--pipeline", type=int, default=1, choices=(0, 1),
+                    help="1: event and walk kernels (default); 0: single lane-pipeline kernel")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (5 times), the reference kernel for scaling "
+                         "the profiler's FETCH_SIZE / WRITE_SIZE counters to HBM bytes")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy calibration: 5 copies of %d MiB (%d bytes read and %d written per copy)" % (mib, 4 * n, 4 * n))
 
 
 def frames(w):
@@ -55,13 +71,12 @@ def frames(w):
 
 
 def run_steps(sim, first, nsteps, per, npp, precision):
+    """Runs nsteps batches; returns the summed counters and the transport kernel's device seconds."""
     tot = None
-    kernel_s = 0.0
     for s in range(nsteps):
         c = sim.run_packets(0, first + s * per, per, npp, precision)
-        kernel_s += c["kernel_us"] * 1e-6
         tot = c if tot is None else {k: tot[k] + c[k] for k in c}
-    return tot, kernel_s
+    return tot, tot["transport_us"] * 1e-6
 
 
 def cpu_baseline(w, seconds):
@@ -103,6 +118,9 @@ def main():
     from skirt7_amd import workloads
     from skirt7_amd.capi import SG_FP64, SG_MIXED
 
+    if a.copy_calibration > 0:
+        copy_calibration(a.copy_calibration)
+    S.set_pipeline(a.pipeline)
     per = int(a.packets)
     total = per * (a.steps + a.warmup)
     t0 = time.perf_counter()
@@ -121,8 +139,9 @@ def main():
     img_mixed = frames(w)
     pk = cm["packets"]
     cross = cm["own_crossings"] + cm["peel_crossings"] + cm["rewalk_crossings"]
-    log("mixed: %d packets in %.3f s (kernel %.3f s): %.3e packets/s, %.3e crossings/s" % (
-        pk, wall, kmix, pk / wall, cross / kmix))
+    log("mixed: %d packets in %.3f s (transport kernel %.3f s in %d launches, emission + transport %.3f s): "
+        "%.3e packets/s, %.3e crossings/s" % (pk, wall, kmix, cm["transport_launches"], cm["kernel_us"] * 1e-6,
+                                               pk / wall, cross / kmix))
 
     out = {"metric": "packets/s on 1 MI355X, mixed vs FP64 GPU at equal image error; % HBM peak",
            "value": pk / wall, "unit": "packets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -132,11 +151,13 @@ def main():
                                   "4x%d^2 frames" % (a.tau, ncell, per, a.pixels),
                       "packets_per_step": per, "leaves": ncell, "tau": a.tau, "instruments": 4,
                       "pixels": a.pixels}}
+    # algorithmic bytes of the transport kernel over its measured device time (all launches)
     bytes_launch = cross * BYTES_PER_CROSSING["mixed"] + cm["peeloffs"] * BYTES_PER_PEELOFF
     achieved = bytes_launch / kmix
     out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK, "traffic": None}
     out["crossings_per_s"] = cross / kmix
+    out["transport_kernel_ms_per_launch"] = 1e3 * kmix / max(1, cm["transport_launches"])
     out["counters"] = cm
 
     if not a.no_fp64:

@@ -38,7 +38,7 @@ enum { SG_TOPDOWN = 0, SG_NEIGHBOR = 1 };   /* TreeDustGrid::SearchMethod (TreeD
 #define SG_STAR_NPARAM 16
 #define SG_INSTR_NPARAM 10
 #define SG_MC_NPARAM 8
-#define SG_NCOUNTERS 12
+#define SG_NCOUNTERS 14
 
 /* ---- library ---------------------------------------------------------------------------- */
 const char* sg_last_error(void);
@@ -129,7 +129,8 @@ int sg_sim_destroy(sg_sim* s);
  * absorption tallies; counters (may be null) receive SG_NCOUNTERS values:
  *   [0] packets [1] own-path crossings [2] peel-off crossings [3] scattering events
  *   [4] peel-offs detected [5] stuck-path escapes [6] npp [7] walks [8] re-walk crossings
- *   [9] FP64 wall recomputes [10] kernel microseconds [11] launches */
+ *   [9] FP64 wall recomputes [10] device microseconds (emission and transport kernels)
+ *   [11] transport workgroups per launch [12] transport-kernel microseconds [13] transport launches */
 int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, int precision,
                uint64_t* counters);
 /* Batched population (one persistent launch over many members, each its own simulation on a
@@ -151,6 +152,11 @@ int sg_fit_frames(int n, sg_instruments* const* instr, int instrument, int ell, 
                   int ky, const double* ref, double minlum, double maxlum, double* lum, double* chi2,
                   double* convolved);
 
+/* Transport organisation used by sg_sim_run (both give the same per-packet histories):
+ * 1 (default) alternating kernels -- an event kernel over in-flight packet slots that runs the
+ *   packet state machine and queues walks, and a persistent walk kernel over that queue;
+ * 0 one persistent kernel whose lanes run the state machine and the walks in turn. */
+int sg_set_pipeline(int mode);
 /* Launch tuning used by sg_sim_run (0 = keep current): block size, waves per CU, compaction
  * threshold (lanes that must finish their walk before the wave re-arms them). */
 int sg_set_tuning(int block, int waves_per_cu, int compact_lanes);

@@ -18,10 +18,11 @@ SG_NEIGHBOR = 1
 SG_STAR_NPARAM = 16
 SG_INSTR_NPARAM = 10
 SG_MC_NPARAM = 8
-SG_NCOUNTERS = 12
+SG_NCOUNTERS = 14
 
 COUNTER_NAMES = ("packets", "own_crossings", "peel_crossings", "scatterings", "peeloffs", "stuck", "npp",
-                 "walks", "rewalk_crossings", "fp64_recomputes", "kernel_us", "blocks")
+                 "walks", "rewalk_crossings", "fp64_recomputes", "kernel_us", "blocks", "transport_us",
+                 "transport_launches")
 
 _lib = None
 
@@ -47,6 +48,7 @@ _SIGS = {
     "sg_init": (ctypes.c_int, [ctypes.c_int]),
     "sg_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "sg_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "sg_set_pipeline": (ctypes.c_int, [ctypes.c_int]),
     "sg_grid_create_cartesian": (ctypes.c_int, [_D, _D, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _VPP]),
     "sg_grid_create_tree": (ctypes.c_int, [ctypes.c_int, _D, _U8, ctypes.c_int64, ctypes.c_int, _VPP]),
     "sg_grid_create_voronoi": (ctypes.c_int, [_D, _D, ctypes.c_int64, _I64, _I32, _VPP]),

@@ -11,6 +11,7 @@
 #include "../../include/skirt7_amd.h"
 #include "sg_host.hpp"
 #include "sg_mc.hpp"
+#include "sg_split.hpp"
 
 namespace sg {
 
@@ -20,6 +21,7 @@ void set_error(const std::string& msg) { g_err = msg; }
 static int g_device = -1;
 static int g_cus = 256;
 static int g_tune_block = 256, g_tune_waves = 0, g_tune_compact = 16;
+static int g_pipeline = 1;   // 0: persistent lane pipeline (k_mc); 1: event and walk kernels (k_events/k_walks)
 
 static void ensure_init()
 {
@@ -175,6 +177,16 @@ int sg_device_name(char* buf, int len)
     SG_HIP(hipGetDeviceProperties(&p, g_device));
     std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
     SG_CATCH
+}
+
+int sg_set_pipeline(int mode)
+{
+    if (mode != 0 && mode != 1) {
+        sg::set_error("pipeline mode must be 0 (lane pipeline) or 1 (event and walk kernels)");
+        return -1;
+    }
+    g_pipeline = mode;
+    return 0;
 }
 
 int sg_set_tuning(int block, int waves_per_cu, int compact_lanes)
@@ -677,6 +689,36 @@ int sg_sim_destroy(sg_sim* s)
 }
 } // extern "C"
 
+// Device time of a series of transport launches: an event pair around each launch on the default
+// stream, summed once the stream has drained.
+struct LaunchTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    void start()
+    {
+        hipEvent_t a, b;
+        SG_HIP(hipEventCreate(&a));
+        SG_HIP(hipEventCreate(&b));
+        ev.push_back({a, b});
+        SG_HIP(hipEventRecord(a, 0));
+    }
+    void stop() { SG_HIP(hipEventRecord(ev.back().second, 0)); }
+    double total_ms()
+    {
+        double t = 0;
+        for (auto& p : ev) {
+            float ms = 0;
+            SG_HIP(hipEventSynchronize(p.second));
+            SG_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+            t += ms;
+        }
+        return t;
+    }
+    ~LaunchTimer()
+    {
+        for (auto& p : ev) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+    }
+};
+
 static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, McArgs& a)
 {
     std::memset(&a, 0, sizeof a);
@@ -792,6 +834,104 @@ template <bool ABS> static void dispatch_mc(sg_sim* s, const McArgs& a, int prec
         else launch_mc<VoroW<double>, ABS>(g->voro, a, blocks, s->cold, s->args);
     }
 }
+// ---------------------------------------------------------------- split pipeline
+
+__global__ void k_revive(uint32_t* state, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && state[i] == PS_DEAD) state[i] = PS_FRESH;
+}
+
+static const int64_t kSplitSlots = (int64_t)1 << 22;
+
+// Packets [first, first+count) through alternating event and walk passes. Emission runs in batches
+// of up to 2^22 packets into the emission buffer; a batch is replaced once all its packets have been
+// claimed (packets in flight keep their own state in their slots).
+template <class W, bool ABS>
+static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uint64_t first, uint64_t count, int nlds,
+                      LaunchTimer& lt, int& blocks)
+{
+    using R = typename W::Real;
+    const int64_t P = (int64_t)std::max<uint64_t>(1, std::min<uint64_t>(count, (uint64_t)kSplitSlots));
+    const int64_t cap = P * (a.ni + 1);
+    if (s->slots.bytes < slot_bytes<R>(P)) s->slots.alloc(slot_bytes<R>(P));
+    if (s->queue.bytes < queue_bytes<R>(cap)) s->queue.alloc(queue_bytes<R>(cap));
+    if (s->prog.bytes < 4 * sizeof(unsigned long long)) s->prog.alloc(4 * sizeof(unsigned long long));
+    Slots<R> sl{s->slots.as<R>(),
+                reinterpret_cast<uint32_t*>(s->slots.as<unsigned char>() + (size_t)P * SR_NR * sizeof(R)), P};
+    // progress: [0] slots holding a packet, [1] queue tail, [2] packets claimed, [3] requests claimed
+    auto* pc = s->prog.as<unsigned long long>();
+    WalkQueue<R> q{s->queue.as<R>(),
+                   reinterpret_cast<uint32_t*>(s->queue.as<unsigned char>() + (size_t)cap * QR_NR * sizeof(R)), cap,
+                   pc + 1};
+    SG_HIP(hipMemset(sl.u + (size_t)SU_STATE * P, 0, (size_t)P * sizeof(uint32_t)));   // all PS_FRESH
+    const int bs = 256;
+    const size_t shm = (size_t)nlds * sizeof(int32_t);
+    int per = 0;
+    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walks<W, ABS>, bs, shm));
+    blocks = std::max(1, per) * g_cus;
+    const int eblocks = (int)((P + 255) / 256);
+    const uint64_t batch = std::min<uint64_t>(count, (uint64_t)1 << 22);
+    for (uint64_t b = 0; b < count; b += batch) {
+        McArgs ab = a;
+        ab.first = first + b;
+        ab.count = std::min(batch, count - b);
+        ab.emitted = s->emitted.p;
+        ab.emit_draw = s->emitdraw.as<uint32_t>();
+        ab.next_packet = pc + 2;
+        SG_HIP(hipMemsetAsync(pc + 2, 0, sizeof(unsigned long long), 0));
+        int emb = (int)std::min<uint64_t>((ab.count + 255) / 256, (uint64_t)g_cus * 16);
+        hipLaunchKernelGGL(k_emit<R>, dim3(emb), dim3(256), 0, 0, ab, s->emitted.as<R>(), s->emitdraw.as<uint32_t>());
+        SG_HIP(hipGetLastError());
+        if (b > 0) hipLaunchKernelGGL(k_revive, dim3(eblocks), dim3(256), 0, 0, sl.u + (size_t)SU_STATE * P, P);
+        const bool last = b + batch >= count;
+        for (;;) {
+            SG_HIP(hipMemsetAsync(pc, 0, 2 * sizeof(unsigned long long), 0));
+            hipLaunchKernelGGL(k_events<R>, dim3(eblocks), dim3(256), 0, 0, ab, sl, q,
+                               reinterpret_cast<SplitCounters*>(pc));
+            SG_HIP(hipGetLastError());
+            unsigned long long h[3];
+            SG_HIP(hipMemcpy(h, pc, sizeof h, hipMemcpyDeviceToHost));
+            SG_REQUIRE(h[1] <= (unsigned long long)cap, "walk queue overflow");
+            if (h[1] > 0) {
+                SG_HIP(hipMemsetAsync(pc + 3, 0, sizeof(unsigned long long), 0));
+                lt.start();
+                hipLaunchKernelGGL((k_walks<W, ABS>), dim3(blocks), dim3(bs), shm, 0, v, ab, q, sl, pc + 3, nlds);
+                SG_HIP(hipGetLastError());
+                lt.stop();
+            }
+            if (h[0] == 0) break;                      // nothing in flight and nothing left to claim
+            if (!last && h[2] >= ab.count) break;      // batch claimed: emit the next one
+        }
+    }
+}
+
+template <bool ABS>
+static void dispatch_split(sg_sim* s, const McArgs& a, int precision, uint64_t first, uint64_t count, LaunchTimer& lt,
+                           int& blocks)
+{
+    const bool mixed = precision == SG_MIXED;
+    sg_grid* g = s->dust ? s->dust->grid : nullptr;
+    if (!g) {
+        CartView v{};
+        if (mixed) run_split<CartW32, false>(s, v, a, first, count, 0, lt, blocks);
+        else run_split<CartW64, false>(s, v, a, first, count, 0, lt, blocks);
+        return;
+    }
+    if (g->kind == GK_CART) {
+        if (mixed) run_split<CartW32, ABS>(s, g->cart, a, first, count, 0, lt, blocks);
+        else run_split<CartW64, ABS>(s, g->cart, a, first, count, 0, lt, blocks);
+    } else if (g->kind == GK_TREE) {
+        const int nl = g->tree.search == SG_TOPDOWN ? SG_LDS_NODES : 0;
+        if (!mixed) run_split<TreeW64, ABS>(s, g->tree, a, first, count, nl, lt, blocks);
+        else if (g->tree.search == SG_NEIGHBOR) run_split<TreeW32, ABS>(s, g->tree, a, first, count, 0, lt, blocks);
+        else run_split<TreeW32TopDown, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
+    } else {
+        if (mixed) run_split<VoroW<float>, ABS>(s, g->voro, a, first, count, 0, lt, blocks);
+        else run_split<VoroW<double>, ABS>(s, g->voro, a, first, count, 0, lt, blocks);
+    }
+}
+
 extern "C" {
 
 int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, int precision, uint64_t* counters)
@@ -813,8 +953,13 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
     SG_HIP(hipEventCreate(&e1));
     SG_HIP(hipEventRecord(e0, 0));
     int blocks = 0;
-    // the pipeline runs in batches: emission kernel, then the persistent transport kernel
-    for (uint64_t b = 0; b < count; b += batch) {
+    LaunchTimer lt;
+    if (g_pipeline == 1 && count > 0) {
+        if (a.store_abs) dispatch_split<true>(s, a, precision, first, count, lt, blocks);
+        else dispatch_split<false>(s, a, precision, first, count, lt, blocks);
+    }
+    // the lane pipeline runs in batches: emission kernel, then the persistent transport kernel
+    for (uint64_t b = 0; g_pipeline == 0 && b < count; b += batch) {
         McArgs ab = a;
         ab.first = first + b;
         ab.count = std::min(batch, count - b);
@@ -825,8 +970,10 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
         if (mixed) hipLaunchKernelGGL(k_emit<float>, dim3(eb), dim3(256), 0, 0, ab, s->emitted.as<float>(), s->emitdraw.as<uint32_t>());
         else hipLaunchKernelGGL(k_emit<double>, dim3(eb), dim3(256), 0, 0, ab, s->emitted.as<double>(), s->emitdraw.as<uint32_t>());
         SG_HIP(hipGetLastError());
+        lt.start();
         if (ab.store_abs) dispatch_mc<true>(s, ab, precision, blocks);
         else dispatch_mc<false>(s, ab, precision, blocks);
+        lt.stop();
     }
     SG_HIP(hipEventRecord(e1, 0));
     SG_HIP(hipEventSynchronize(e1));
@@ -843,6 +990,8 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
         counters[6] = npp;
         counters[10] = (uint64_t)(ms * 1000.0);
         counters[11] = blocks;
+        counters[12] = (uint64_t)(lt.total_ms() * 1000.0);
+        counters[13] = lt.ev.size();
     }
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH
@@ -853,7 +1002,8 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
 // One round of a batched run: emission for every range, then the batched transport kernel. The
 // tables (run arguments, range offsets, grid views) go to the first member's work buffer.
 template <class W, bool ABS>
-static void launch_batch(sg_sim* s0, const std::vector<McArgs>& rg, const std::vector<sg_sim*>& owner, int& blocks)
+static void launch_batch(sg_sim* s0, const std::vector<McArgs>& rg, const std::vector<sg_sim*>& owner, int& blocks,
+                         LaunchTimer& lt)
 {
     using R = typename W::Real;
     using V = typename W::View;
@@ -894,27 +1044,29 @@ static void launch_batch(sg_sim* s0, const std::vector<McArgs>& rg, const std::v
     if (s0->cold.bytes < cneed) s0->cold.alloc(cneed);
     R* cr = s0->cold.as<R>();
     uint32_t* cu = reinterpret_cast<uint32_t*>(s0->cold.as<unsigned char>() + (size_t)nslots * CR_NR * sizeof(R));
+    lt.start();
     hipLaunchKernelGGL((k_mc_batch<W, ABS>), dim3(blocks), dim3(bs), shm, 0, vtab, atab, (int)nr, cr, cu, nslots);
     SG_HIP(hipGetLastError());
+    lt.stop();
 }
 
 template <bool ABS>
 static void dispatch_batch(sg_sim* s0, const std::vector<McArgs>& rg, const std::vector<sg_sim*>& owner,
-                           int precision, int& blocks)
+                           int precision, int& blocks, LaunchTimer& lt)
 {
     const bool mixed = precision == SG_MIXED;
     sg_grid* g = s0->dust ? s0->dust->grid : nullptr;
-    if (!g) { launch_batch<CartW64, false>(s0, rg, owner, blocks); return; }
+    if (!g) { launch_batch<CartW64, false>(s0, rg, owner, blocks, lt); return; }
     if (g->kind == GK_CART) {
-        if (mixed) launch_batch<CartW32, ABS>(s0, rg, owner, blocks);
-        else launch_batch<CartW64, ABS>(s0, rg, owner, blocks);
+        if (mixed) launch_batch<CartW32, ABS>(s0, rg, owner, blocks, lt);
+        else launch_batch<CartW64, ABS>(s0, rg, owner, blocks, lt);
     } else if (g->kind == GK_TREE) {
-        if (!mixed) launch_batch<TreeW64, ABS>(s0, rg, owner, blocks);
-        else if (g->tree.search == SG_NEIGHBOR) launch_batch<TreeW32, ABS>(s0, rg, owner, blocks);
-        else launch_batch<TreeW32TopDown, ABS>(s0, rg, owner, blocks);
+        if (!mixed) launch_batch<TreeW64, ABS>(s0, rg, owner, blocks, lt);
+        else if (g->tree.search == SG_NEIGHBOR) launch_batch<TreeW32, ABS>(s0, rg, owner, blocks, lt);
+        else launch_batch<TreeW32TopDown, ABS>(s0, rg, owner, blocks, lt);
     } else {
-        if (mixed) launch_batch<VoroW<float>, ABS>(s0, rg, owner, blocks);
-        else launch_batch<VoroW<double>, ABS>(s0, rg, owner, blocks);
+        if (mixed) launch_batch<VoroW<float>, ABS>(s0, rg, owner, blocks, lt);
+        else launch_batch<VoroW<double>, ABS>(s0, rg, owner, blocks, lt);
     }
 }
 
@@ -961,6 +1113,7 @@ int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const u
     SG_HIP(hipEventCreate(&e1));
     SG_HIP(hipEventRecord(e0, 0));
     int blocks = 0;
+    LaunchTimer lt;
     std::vector<uint64_t> done(nmembers, 0);
     int j = 0;
     for (;;) {
@@ -986,8 +1139,8 @@ int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const u
         }
         if (rg.empty()) break;
         SG_HIP(hipMemset(s0->bnext.p, 0, rg.size() * sizeof(unsigned long long)));
-        if (abs_of(s0)) dispatch_batch<true>(s0, rg, owner, precision, blocks);
-        else dispatch_batch<false>(s0, rg, owner, precision, blocks);
+        if (abs_of(s0)) dispatch_batch<true>(s0, rg, owner, precision, blocks, lt);
+        else dispatch_batch<false>(s0, rg, owner, precision, blocks, lt);
     }
     SG_HIP(hipEventRecord(e1, 0));
     SG_HIP(hipEventSynchronize(e1));
@@ -1010,6 +1163,8 @@ int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const u
         counters[6] = total;
         counters[10] = (uint64_t)(ms * 1000.0);
         counters[11] = blocks;
+        counters[12] = (uint64_t)(lt.total_ms() * 1000.0);
+        counters[13] = lt.ev.size();
     }
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH

@@ -1,0 +1,345 @@
+// sg_split.hpp -- the photon-packet life cycle (MonteCarloSimulation::dostellaremissionchunk,
+// MonteCarloSimulation.cpp:259-297) as two alternating kernels over device-resident queues:
+//
+//   k_events: one thread per in-flight packet slot runs the packet's state machine up to its next
+//             walk -- launch, emission peel-offs, escape/absorption and the stop test, biased
+//             propagation, scattering peel-offs and the new direction -- and appends the walks it
+//             needs (peel-off, own path, propagation re-walk) to the walk queue;
+//   k_walks:  persistent lanes take walk requests from the queue and run the cell walks, with the
+//             same ballot-compacted refill as the lane pipeline; a peel-off walk ends with its
+//             detection (FrameInstrument::detect), an own-path or propagation walk with its result
+//             written back to the packet slot.
+//
+// The walk kernel carries only the walker and the running sums, so it keeps several times more
+// waves in flight than a kernel that also holds the event logic; the event kernel's FP64 and libm
+// work runs at full width over the slots. Each packet still draws from its own Philox stream in the
+// same order, and every walk repeats the same arithmetic, so results equal the lane pipeline's up
+// to the order of the tally additions.
+#pragma once
+#include "sg_mc.hpp"
+
+namespace sg {
+
+enum SlotR { SR_RX = 0, SR_RY, SR_RZ, SR_KX, SR_KY, SR_KZ, SR_L, SR_RES, SR_NR };
+enum SlotU { SU_PLO = 0, SU_PHI, SU_DRAW, SU_W0, SU_W1, SU_W2, SU_W3, SU_NSCATT, SU_STATE, SU_NU };
+enum PacketState : uint32_t { PS_FRESH = 0, PS_OWN, PS_PROP, PS_DEAD };
+enum WalkKind : uint32_t { WK_PEEL = 0, WK_OWN = 1, WK_PROP = 2 };
+
+// Packet slots: structure of arrays, n slots.
+template <typename R> struct Slots {
+    R* r;
+    uint32_t* u;
+    int64_t n;
+    __device__ __forceinline__ R& fr(int f, int64_t i) const { return r[f * n + i]; }
+    __device__ __forceinline__ uint32_t& fu(int f, int64_t i) const { return u[f * n + i]; }
+};
+template <typename R> __host__ __device__ constexpr size_t slot_bytes(int64_t n)
+{
+    return (size_t)n * (SR_NR * sizeof(R) + SU_NU * sizeof(uint32_t));
+}
+
+// Walk requests: origin (3), direction (3, unused for peel-offs: the instrument's direction is
+// taken from its record) and a value (peel-off luminosity, own-path luminosity, or propagation
+// target optical depth) in the run's real type; a tag (kind | instrument << 2) and an index
+// (packet slot, or pixel for a peel-off).
+enum QueueR { QR_OX = 0, QR_OY, QR_OZ, QR_DX, QR_DY, QR_DZ, QR_VAL, QR_NR };
+enum QueueU { QU_TAG = 0, QU_IDX, QU_NU };
+template <typename R> struct WalkQueue {
+    R* r;
+    uint32_t* u;
+    int64_t cap;
+    unsigned long long* tail;   // requests appended by the event kernel
+    __device__ __forceinline__ R& fr(int f, int64_t i) const { return r[f * cap + i]; }
+    __device__ __forceinline__ uint32_t& fu(int f, int64_t i) const { return u[f * cap + i]; }
+};
+template <typename R> __host__ __device__ constexpr size_t queue_bytes(int64_t cap)
+{
+    return (size_t)cap * (QR_NR * sizeof(R) + QU_NU * sizeof(uint32_t));
+}
+
+// Host-visible progress of one event pass: [0] slots still holding a packet, [1] walks appended.
+struct SplitCounters {
+    unsigned long long alive, pushed;
+};
+
+// Appends one request per active lane (one atomic per wave); returns this lane's queue position.
+__device__ __forceinline__ uint64_t wave_append(unsigned long long* tail)
+{
+    uint64_t mask = __ballot(1);
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(tail, (unsigned long long)__popcll(mask));
+    uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
+    base = ((uint64_t)hi << 32) | lo;
+    uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+    return base + (uint64_t)__popcll(below);
+}
+
+template <typename R>
+__device__ __forceinline__ void push_walk(const WalkQueue<R>& q, uint32_t kind, uint32_t inst, uint32_t idx, R ox,
+                                          R oy, R oz, R dx, R dy, R dz, R val)
+{
+    uint64_t i = wave_append(q.tail);
+    q.fr(QR_OX, i) = ox; q.fr(QR_OY, i) = oy; q.fr(QR_OZ, i) = oz;
+    q.fr(QR_DX, i) = dx; q.fr(QR_DY, i) = dy; q.fr(QR_DZ, i) = dz;
+    q.fr(QR_VAL, i) = val;
+    q.fu(QU_TAG, i) = kind | (inst << 2);
+    q.fu(QU_IDX, i) = idx;
+}
+
+// Peel-offs toward every instrument (MonteCarloSimulation::peeloffemission/peeloffscattering,
+// MonteCarloSimulation.cpp:299-357): pixel, phase-function weight (scattering only), and either the
+// walk request or, without a dust system, the detection itself.
+template <typename R>
+__device__ __forceinline__ uint32_t peel_offs(const McArgs& a, const InstrDev* instr, const WalkQueue<R>& q,
+                                              bool scattering, R rx, R ry, R rz, R kx, R ky, R kz, R L)
+{
+    const R gg = (R)a.g;
+    uint32_t n = 0;
+    for (int i = 0; i < a.ni; i++) {
+        const InstrDev& in = instr[i];
+        int pix = pixel_on_detector<R>(in, rx, ry, rz);
+        if (pix < 0) continue;
+        R Lp = L;
+        if (scattering) {
+            // DustMix::phaseFunctionValue, Henyey-Greenstein (DustMix.cpp:648-653)
+            R cosalpha = kx * (R)in.kobs[0] + ky * (R)in.kobs[1] + kz * (R)in.kobs[2];
+            R t = (R)1.0 + gg * gg - (R)2 * gg * cosalpha;
+            Lp = L * (((R)1.0 - gg) * ((R)1.0 + gg) / sqrt(t * t * t));
+        }
+        n++;
+        if (!a.has_dust) unsafeAtomicAdd(in.frame + (size_t)a.ell * in.Nx * in.Ny + pix, (double)Lp);
+        else push_walk<R>(q, WK_PEEL, (uint32_t)i, (uint32_t)pix, rx, ry, rz, (R)0, (R)0, (R)0, Lp);
+    }
+    return n;
+}
+
+// Henyey-Greenstein scattering direction (DustMix.cpp:596-601) and Random::direction(k, cos)
+// (Random.cpp:189-223).
+template <typename R>
+__device__ __forceinline__ void hg_scatter(PacketRng& rng, R gg, R& kx, R& ky, R& kz)
+{
+    R nkx, nky, nkz;
+    if (fabs((double)gg) < 1e-6) {
+        double a0, a1, a2;
+        iso_direction(rng, a0, a1, a2);
+        nkx = (R)a0; nky = (R)a1; nkz = (R)a2;
+    } else {
+        R f = (((R)1.0 - gg) * ((R)1.0 + gg)) / ((R)1.0 - gg + (R)2.0 * gg * uni<R>(rng));
+        R costheta = ((R)1.0 + gg * gg - f * f) / ((R)2.0 * gg);
+        R phi = (R)(2.0 * M_PI) * uni<R>(rng);
+        R cosphi = cos(phi), sinphi = sin(phi);
+        R sintheta = sqrt(fabs(((R)1.0 - costheta) * ((R)1.0 + costheta)));
+        if (kz > (R)0.99999) { nkx = cosphi * sintheta; nky = sinphi * sintheta; nkz = costheta; }
+        else if (kz < (R)-0.99999) { nkx = cosphi * sintheta; nky = sinphi * sintheta; nkz = -costheta; }
+        else {
+            R root = sqrt(((R)1.0 - kz) * ((R)1.0 + kz));
+            nkx = sintheta / root * (-kx * kz * cosphi + ky * sinphi) + kx * costheta;
+            nky = -sintheta / root * (ky * kz * cosphi + kx * sinphi) + ky * costheta;
+            nkz = root * sintheta * cosphi + kz * costheta;
+        }
+    }
+    kx = nkx; ky = nky; kz = nkz;
+}
+
+// Event pass: every slot advances its packet to its next walk (or takes a new packet).
+template <typename R>
+__global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, WalkQueue<R> q, SplitCounters* prog)
+{
+    __shared__ InstrDev s_instr[SG_MAX_INSTR];
+    if (threadIdx.x < SG_MAX_INSTR) s_instr[threadIdx.x] = a.scene->instr[threadIdx.x];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane stays to the end: the counts below are summed over the whole wave
+    uint32_t state = i < sl.n ? sl.fu(SU_STATE, i) : (uint32_t)PS_DEAD;
+    uint32_t c_pk = 0, c_scat = 0, c_npeel = 0, c_walks = 0;
+    if (state != PS_DEAD) {
+        const R albedo = (R)a.albedo, gg = (R)a.g;
+        R rx = sl.fr(SR_RX, i), ry = sl.fr(SR_RY, i), rz = sl.fr(SR_RZ, i);
+        R kx = sl.fr(SR_KX, i), ky = sl.fr(SR_KY, i), kz = sl.fr(SR_KZ, i);
+        R L = sl.fr(SR_L, i);
+        PacketRng rng;
+        rng.k0 = a.seed; rng.k1 = a.member; rng.ell = (uint32_t)a.ell;
+        rng.plo = sl.fu(SU_PLO, i); rng.phi = sl.fu(SU_PHI, i); rng.draw = sl.fu(SU_DRAW, i);
+        rng.w0 = sl.fu(SU_W0, i); rng.w1 = sl.fu(SU_W1, i); rng.w2 = sl.fu(SU_W2, i); rng.w3 = sl.fu(SU_W3, i);
+        int nscatt = (int)sl.fu(SU_NSCATT, i);
+        // at most a few transitions per pass: a finished packet is replaced by a new one at once; a slot
+        // without a dust system detects its peel-offs directly and takes up to four packets per pass
+        for (int step = 0; step < 4; step++) {
+            if (state == PS_FRESH) {
+                uint64_t p;
+                if (!wave_claim(a.next_packet, a.count, p)) { state = PS_DEAD; break; }
+                // launch state from the emission kernel; continue the packet's random stream
+                const R* e = reinterpret_cast<const R*>(a.emitted);
+                const size_t n = a.count;
+                c_pk++;
+                R Le = e[6 * n + p];
+                if (!(Le > (R)0)) continue;
+                rx = e[p]; ry = e[n + p]; rz = e[2 * n + p];
+                kx = e[3 * n + p]; ky = e[4 * n + p]; kz = e[5 * n + p];
+                L = Le;
+                rng.start(a.seed, a.member, a.first + p, a.ell);
+                rng.draw = a.emit_draw[p];
+                if (rng.draw & 3u) {
+                    rng.w0 = rng.plo; rng.w1 = rng.phi; rng.w2 = rng.draw >> 2; rng.w3 = rng.ell;
+                    philox10(rng.w0, rng.w1, rng.w2, rng.w3, rng.k0, rng.k1);
+                }
+                nscatt = 0;
+                uint32_t np = peel_offs<R>(a, s_instr, q, false, rx, ry, rz, kx, ky, kz, L);
+                c_npeel += np;
+                if (!a.has_dust) continue;
+                c_walks += np + 1;
+                push_walk<R>(q, WK_OWN, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, L);
+                state = PS_OWN;
+                break;
+            }
+            if (state == PS_OWN) {
+                // DustSystem::fillOpticalDepth check (DustSystem.cpp:952-955)
+                R taupath = sl.fr(SR_RES, i);
+                if (taupath < (R)0 || isnan(taupath) || isinf(taupath)) {
+                    atomicOr(a.error, 1);
+                    state = PS_FRESH;
+                    continue;
+                }
+                // simulateescapeandabsorption, single component (MonteCarloSimulation.cpp:432-461)
+                R expfactor = -rexpm1<R>(-taupath);
+                L = L * albedo * expfactor;
+                if (L <= (R)0 || (L <= (R)a.Lthreshold && nscatt >= a.minfs)) { state = PS_FRESH; continue; }
+                if (taupath != (R)0) {
+                    // simulatepropagation (MonteCarloSimulation.cpp:513-531): biased free path, then the
+                    // re-walk to the interaction point
+                    R tt;
+                    if (a.xi == 0.0) tt = exponcutoff(rng, taupath);
+                    else {
+                        R X = uni<R>(rng);
+                        tt = (X < (R)a.xi) ? uni<R>(rng) * taupath : exponcutoff(rng, taupath);
+                        R pp = -rexp<R>(-tt) / rexpm1<R>(-taupath);
+                        R qq = ((R)1.0 - (R)a.xi) * pp + (R)a.xi / taupath;
+                        L = L * (pp / qq);
+                    }
+                    push_walk<R>(q, WK_PROP, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, tt);
+                    state = PS_PROP;
+                    break;
+                }
+                // zero optical depth along the path: scattering in place
+                state = PS_PROP;
+                sl.fr(SR_RES, i) = (R)0;
+            }
+            if (state == PS_PROP) {
+                // PhotonPackage::propagate (PhotonPackage.cpp:93-96) by pathlength(tau)
+                R d = sl.fr(SR_RES, i);
+                rx = rx + d * kx; ry = ry + d * ky; rz = rz + d * kz;
+                uint32_t np = peel_offs<R>(a, s_instr, q, true, rx, ry, rz, kx, ky, kz, L);
+                c_npeel += np;
+                hg_scatter<R>(rng, gg, kx, ky, kz);
+                nscatt++;
+                c_scat++;
+                c_walks += np + 1;
+                push_walk<R>(q, WK_OWN, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, L);
+                state = PS_OWN;
+                break;
+            }
+        }
+        sl.fr(SR_RX, i) = rx; sl.fr(SR_RY, i) = ry; sl.fr(SR_RZ, i) = rz;
+        sl.fr(SR_KX, i) = kx; sl.fr(SR_KY, i) = ky; sl.fr(SR_KZ, i) = kz;
+        sl.fr(SR_L, i) = L;
+        sl.fu(SU_PLO, i) = rng.plo; sl.fu(SU_PHI, i) = rng.phi; sl.fu(SU_DRAW, i) = rng.draw;
+        sl.fu(SU_W0, i) = rng.w0; sl.fu(SU_W1, i) = rng.w1; sl.fu(SU_W2, i) = rng.w2; sl.fu(SU_W3, i) = rng.w3;
+        sl.fu(SU_NSCATT, i) = (uint32_t)nscatt;
+        sl.fu(SU_STATE, i) = state;
+    }
+    add_wave(a.counters + 0, c_pk);
+    add_wave(a.counters + 3, c_scat);
+    add_wave(a.counters + 4, c_npeel);
+    add_wave(a.counters + 7, c_walks);
+    add_wave(&prog->alive, state != PS_DEAD ? 1u : 0u);
+}
+
+// Walk pass: persistent lanes over the requests [0, *q.tail).
+template <class W, bool ABS>
+__global__ void __launch_bounds__(256) k_walks(const typename W::View g, const McArgs a, WalkQueue<typename W::Real> q,
+                                              Slots<typename W::Real> sl, unsigned long long* claim, int nlds_req)
+{
+    using R = typename W::Real;
+    __shared__ InstrDev s_instr[SG_MAX_INSTR];
+    extern __shared__ __align__(16) unsigned char smem[];
+    int32_t* s_nodes = reinterpret_cast<int32_t*>(smem);
+    int nlds = 0;
+    if (nlds_req > 0) {
+        stage_tree_nodes(g, s_nodes, nlds);
+        nlds = min(nlds, nlds_req);
+    }
+    if (threadIdx.x < SG_MAX_INSTR) s_instr[threadIdx.x] = a.scene->instr[threadIdx.x];
+    __syncthreads();
+    const uint64_t nq = *q.tail;
+    Hot<W> h;
+    h.stage = ST_NEWPKT;   // idle: needs a request
+    h.w.nrecompute = 0; h.w.nstuck = 0;
+    uint32_t idx = 0, inst = 0;
+    unsigned long long n_own = 0, n_peel = 0, n_re = 0;   // wave-uniform counts
+    const int thr = a.compact;
+    for (;;) {
+        if (h.stage == ST_PEELW_DONE) {
+            // FrameInstrument::detect (FrameInstrument.cpp:33-47)
+            const InstrDev& in = s_instr[inst];
+            unsafeAtomicAdd(in.frame + (size_t)a.ell * in.Nx * in.Ny + idx, (double)(h.L0 * rexp<R>(-h.tau)));
+            h.stage = ST_NEWPKT;
+        } else if (h.stage == ST_OWN_DONE) {
+            sl.fr(SR_RES, idx) = h.tau;
+            h.stage = ST_NEWPKT;
+        } else if (h.stage == ST_PROP_DONE) {
+            sl.fr(SR_RES, idx) = h.s;
+            h.stage = ST_NEWPKT;
+        }
+        if (h.stage == ST_NEWPKT) {
+            uint64_t r;
+            if (!wave_claim(claim, nq, r)) h.stage = ST_DEAD;
+            else {
+                const uint32_t tag = q.fu(QU_TAG, r);
+                const uint32_t kind = tag & 3u;
+                idx = q.fu(QU_IDX, r);
+                inst = tag >> 2;
+                R ox = q.fr(QR_OX, r), oy = q.fr(QR_OY, r), oz = q.fr(QR_OZ, r);
+                h.L0 = q.fr(QR_VAL, r);
+                h.tau = 0;
+                if (kind == WK_PEEL) {
+                    const InstrDev& in = s_instr[inst];
+                    h.w.init(g, s_nodes, nlds, ox, oy, oz, in.kobs[0], in.kobs[1], in.kobs[2]);
+                    h.stage = ST_PEELW;
+                } else {
+                    R dx = q.fr(QR_DX, r), dy = q.fr(QR_DY, r), dz = q.fr(QR_DZ, r);
+                    h.w.init(g, s_nodes, nlds, ox, oy, oz, dx, dy, dz);
+                    if (kind == WK_OWN) h.stage = ST_OWN;
+                    else {
+                        h.target = h.L0;
+                        h.s = 0; h.tprev = 0; h.sprev = 0; h.nseg = 0;
+                        h.stage = ST_PROP;
+                    }
+                }
+            }
+        }
+        uint64_t alive = __ballot(h.stage != ST_DEAD);
+        if (!alive) break;
+        int need_n = max(1, min(thr, __popcll(alive) >> 1));
+        for (;;) {
+            int kind = 0;
+            if (walking_stage(h.stage)) kind = crossing<W, ABS>(h, g, s_nodes, nlds, a);
+            n_own += __popcll(__ballot(kind == 1));
+            n_peel += __popcll(__ballot(kind == 2));
+            n_re += __popcll(__ballot(kind == 3));
+            bool walk = walking_stage(h.stage);
+            uint64_t idle = __ballot(!walk && h.stage != ST_DEAD);
+            if (__popcll(idle) >= need_n || !__ballot(walk)) break;
+        }
+    }
+    if (__lane_id() == 0) {
+        if (n_own) atomicAdd(a.counters + 1, n_own);
+        if (n_peel) atomicAdd(a.counters + 2, n_peel);
+        if (n_re) atomicAdd(a.counters + 8, n_re);
+    }
+    add_wave(a.counters + 5, h.w.nstuck);
+    add_wave(a.counters + 9, h.w.nrecompute);
+}
+
+} // namespace sg

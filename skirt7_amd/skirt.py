@@ -534,6 +534,11 @@ def fit_frames(instrument_systems, reference, kernel, minlum, maxlum, instrument
     return (lum, chi2, frames) if return_frames else (lum, chi2)
 
 
+def set_pipeline(mode):
+    """1: event and walk kernels over device queues (default); 0: one persistent lane-pipeline kernel."""
+    check(lib().sg_set_pipeline(int(mode)))
+
+
 def set_tuning(block=0, waves_per_cu=0, compact_lanes=0):
     check(lib().sg_set_tuning(block, waves_per_cu, compact_lanes))
 

@@ -81,3 +81,30 @@ def test_scheduling_independent(c0_small):
     b = frames(w)
     for x, y in zip(a, b):
         np.testing.assert_allclose(x, y, rtol=1e-12, atol=1e-300)
+
+
+EXACT = ("packets", "own_crossings", "peel_crossings", "rewalk_crossings", "scatterings", "peeloffs", "walks")
+
+
+@pytest.mark.parametrize("which", ["cartesian", "octree"])
+def test_split_and_lane_pipelines_agree(c0_small, which):
+    """The event/walk kernels and the single lane-pipeline kernel run the same packet histories: identical
+    event and crossing counts, images equal up to the order of the tally additions."""
+    from skirt7_amd import skirt as S
+    w = c0_small if which == "cartesian" else workloads.config1(maxlevel=5, pixels=32, packages=8000, nsamples=4)
+    n = 8000
+    try:
+        for prec in (SG_FP64, SG_MIXED):
+            res = []
+            for mode in (0, 1):
+                S.set_pipeline(mode)
+                w["instr"].clear()
+                c = w["sim"].run_packets(0, 0, n, n, prec)
+                res.append((c, frames(w)))
+            (c0, f0), (c1, f1) = res
+            for k in EXACT:
+                assert c0[k] == c1[k], (prec, k, c0[k], c1[k])
+            for a, b in zip(f0, f1):
+                assert np.allclose(a, b, rtol=1e-10, atol=1e-300)
+    finally:
+        S.set_pipeline(1)
