@@ -158,7 +158,8 @@ int sg_fit_frames(int n, sg_instruments* const* instr, int instrument, int ell, 
  * 0 one persistent kernel whose lanes run the state machine and the walks in turn. */
 int sg_set_pipeline(int mode);
 /* Launch tuning used by sg_sim_run (0 = keep current): block size, waves per CU, compaction
- * threshold (lanes that must finish their walk before the wave re-arms them). */
+ * threshold (lanes that must finish their walk before the wave re-arms them; by default 32 for the
+ * lane pipeline and 16 for the walk kernel). */
 int sg_set_tuning(int block, int waves_per_cu, int compact_lanes);
 
 /* ---- setup helpers (the reference's own setup phase, made reproducible) ---------------------- */

@@ -3,8 +3,9 @@
 // Layouts in HBM (all read-only during transport except the tallies):
 //   Cartesian : three edge arrays (FP64 and an FP32 copy) and rho[m], m = k + Nz*j + Nz*Ny*i.
 //   Tree      : node words in breadth-first id order (>= 0 first child id, < 0 leaf: -(m+1));
-//               one 64-byte record per cell in each precision (bounds, rho, first neighbour
-//               per wall); the full neighbour lists per (cell, wall) in CSR form.
+//               one 64-byte FP64 record per cell (bounds, rho) with the full neighbour lists per
+//               (cell, wall) in CSR form; one 32-byte mixed-precision record per node (subdivision
+//               counts, density or first child, six ropes).
 //   Voronoi   : one 32-byte record per site (x, y, z, rho); neighbour lists in CSR form
 //               with walls -1..-6; a uniform bucket grid over the sites for cellIndex.
 //   Tallies   : FP64, instrument frames [ell][pixel], absorption [ell][m].
@@ -33,16 +34,18 @@ struct CartView {
     float wminf;                     // smallest cell width over the three axes
 };
 
-// Mixed-precision walk record, one per tree NODE (breadth-first id). For a leaf, rope[w] is the
-// node across wall w that covers the whole wall (same level or coarser; -1 on the grid boundary):
-// the next cell is found by descending from it, one record load per level.
-struct alignas(16) TreeRec32 {       // 64 bytes
-    float lo[3], hi[3];
-    float rho;
+// Mixed-precision walk record, one per tree NODE (breadth-first id), 32 bytes:
+//   code : subdivisions of the root along x, y, z (5 bits each: nx | ny << 5 | nz << 10), leaf flag
+//          (bit 15) and the split axis of an internal binary-tree node (bits 16-17);
+//   val  : a leaf's density (float bits) or an internal node's first child;
+//   rope : for a leaf, the node across each wall that covers the whole wall (same level or coarser;
+//          -1 on the grid boundary): the next cell is found by descending from it.
+// Node bounds are not stored: along axis b a node with integer index k_b spans
+// ext_b + [k_b, k_b + 1] * width_b / 2^n_b, and the walker carries the current leaf's k_b.
+struct alignas(32) TreeNode32 {
+    uint32_t code;
+    uint32_t val;
     int32_t rope[6];
-    int32_t cell;                     // cell number of a leaf, -1 for an internal node
-    int32_t child0;                   // first child of an internal node, -1 for a leaf
-    int32_t dir;                      // split axis of an internal binary-tree node
 };
 struct alignas(16) TreeRec64 {       // 64 bytes
     double lo[3], hi[3];
@@ -54,7 +57,8 @@ struct alignas(16) TreeRec64 {       // 64 bytes
 struct TreeView {
     const int32_t* node;             // breadth-first node words
     const TreeRec64* rec64;
-    const TreeRec32* rec32;
+    const TreeNode32* node32;        // mixed-precision walk records
+    const int32_t* cellof;           // cell number per node (-1 for internal nodes)
     const int32_t* nboff;            // CSR offsets, index cell*6 + wall
     const int32_t* nbids;            // neighbour cell numbers
     int32_t nnodes, ncells;
@@ -62,6 +66,7 @@ struct TreeView {
     int search;                      // 0 TopDown, 1 Neighbor
     double ext[6];
     float extf[6];
+    float widthf[3];                 // root box widths
     double eps;
 };
 

@@ -87,9 +87,9 @@ struct sg_grid {
     sg::CartView cart{};
     // tree
     sg::HostTree ht;
-    sg::DevBuf node, rec64, rec32, nboff, nbids;
+    sg::DevBuf node, rec64, node32, cellof, nboff, nbids;
     std::vector<sg::TreeRec64> hrec64;
-    std::vector<sg::TreeRec32> hrec32;
+    std::vector<sg::TreeNode32> hnode32;
     sg::TreeView tree{};
     // Voronoi
     sg::DevBuf site, voff, vnb, boff, bids;

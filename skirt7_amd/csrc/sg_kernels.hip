@@ -20,7 +20,8 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 static int g_device = -1;
 static int g_cus = 256;
-static int g_tune_block = 256, g_tune_waves = 0, g_tune_compact = 16;
+// compaction threshold 0: each pipeline's own default (32 lanes for the lane pipeline, 16 for the walk kernel)
+static int g_tune_block = 256, g_tune_waves = 0, g_tune_compact = 0;
 static int g_pipeline = 1;   // 0: persistent lane pipeline (k_mc); 1: event and walk kernels (k_events/k_walks)
 
 static void ensure_init()
@@ -49,7 +50,7 @@ __global__ void __launch_bounds__(256) k_paths(const typename W::View g, int64_t
     int c = 0, m;
     typename W::Real ds, rho;
     while (w.next(g, s_nodes, nlds, m, ds, rho)) {
-        if (c < maxseg) { mo[i * maxseg + c] = m; dso[i * maxseg + c] = (double)ds; }
+        if (c < maxseg) { mo[i * maxseg + c] = m >= 0 ? W::cell_of(g, m) : m; dso[i * maxseg + c] = (double)ds; }
         c++;
     }
     counts[i] = c <= maxseg ? c : -1;
@@ -251,7 +252,7 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     for (int64_t l = 0; l < nn; l++) words[l] = h.child0[l] >= 0 ? h.child0[l] : -(h.cellnumber[l] + 1);
     g->node.upload(words);
     g->hrec64.assign(nc, TreeRec64{});
-    g->hrec32.assign(nn, TreeRec32{});
+    g->hnode32.assign(nn, TreeNode32{});
     std::vector<int32_t> off(nc * 6 + 1, 0), ids;
     for (int64_t c = 0; c < nc; c++) {
         int l = h.idv[c];
@@ -264,37 +265,52 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
             for (int32_t nbn : lst) ids.push_back(h.cellnumber[nbn]);
         }
     }
+    // mixed-precision records: subdivision counts per axis (a child adds one along each axis of an
+    // octree, or along its parent's split axis of a binary tree), density or first child, ropes
+    std::vector<int32_t> cellof(nn);
+    std::vector<uint32_t> nsub(nn, 0);   // nx | ny << 5 | nz << 10
     for (int64_t l = 0; l < nn; l++) {
-        TreeRec32& f = g->hrec32[l];
-        for (int a = 0; a < 3; a++) { f.lo[a] = (float)h.bounds[6 * l + a]; f.hi[a] = (float)h.bounds[6 * l + 3 + a]; }
-        f.rho = 0;
-        f.cell = h.cellnumber[l];
-        f.child0 = h.child0[l];
-        f.dir = h.dir[l];
+        const int32_t c0 = h.child0[l];
+        if (c0 >= 0) {
+            uint32_t add = arity == 8 ? (1u | 1u << 5 | 1u << 10) : (1u << (5 * h.dir[l]));
+            for (int q = 0; q < arity; q++) nsub[c0 + q] = nsub[l] + add;
+        }
+    }
+    for (int64_t l = 0; l < nn; l++) {
+        uint32_t nx = nsub[l] & 31u, ny = (nsub[l] >> 5) & 31u, nz = (nsub[l] >> 10) & 31u;
+        SG_REQUIRE(nx < 31 && ny < 31 && nz < 31, "tree too deep for the mixed-precision records (30 subdivisions)");
+        TreeNode32& f = g->hnode32[l];
+        const bool leaf = h.child0[l] < 0;
+        f.code = nsub[l] | (leaf ? 1u << 15 : 0u) | ((uint32_t)(leaf ? 0 : h.dir[l]) << 16);
+        f.val = leaf ? 0u : (uint32_t)h.child0[l];
+        cellof[l] = h.cellnumber[l];
         for (int w = 0; w < 6; w++) {
             // rope: the first (largest-overlap) neighbour, raised to this node's level when finer
             const auto& lst = h.nb[6 * (size_t)l + w];
             int32_t r = lst.empty() ? -1 : lst[0];
             while (r >= 0 && h.level[r] > h.level[l]) r = h.father[r];
-            f.rope[w] = (f.cell >= 0) ? r : -1;
+            f.rope[w] = leaf ? r : -1;
         }
     }
+    g->cellof.upload(cellof);
     off[nc * 6] = (int32_t)ids.size();
     SG_REQUIRE(ids.size() < ((size_t)1 << 31), "neighbour lists too long");
     g->nboff.upload(off);
     if (ids.empty()) ids.push_back(-1);
     g->nbids.upload(ids);
     g->rec64.upload(g->hrec64);
-    g->rec32.upload(g->hrec32);
+    g->node32.upload(g->hnode32);
     TreeView& t = g->tree;
     t.node = g->node.as<int32_t>();
     t.rec64 = g->rec64.as<TreeRec64>();
-    t.rec32 = g->rec32.as<TreeRec32>();
+    t.node32 = g->node32.as<TreeNode32>();
+    t.cellof = g->cellof.as<int32_t>();
     t.nboff = g->nboff.as<int32_t>();
     t.nbids = g->nbids.as<int32_t>();
     t.nnodes = (int32_t)nn; t.ncells = (int32_t)nc;
     t.arity = arity; t.search = search;
     for (int i = 0; i < 6; i++) { t.ext[i] = ext[i]; t.extf[i] = (float)ext[i]; }
+    for (int a = 0; a < 3; a++) t.widthf[a] = (float)(ext[a + 3] - ext[a]);
     t.eps = h.eps;
     // the neighbour lists are only needed on the host while building records
     std::vector<std::vector<int32_t>>().swap(h.nb);
@@ -373,9 +389,13 @@ void sg_grid::set_rho(const double* r)
         std::vector<float> f(r, r + ncells);
         SG_HIP(hipMemcpy(rhof.p, f.data(), ncells * sizeof(float), hipMemcpyHostToDevice));
     } else if (kind == GK_TREE) {
-        for (int64_t c = 0; c < ncells; c++) { hrec64[c].rho = r[c]; hrec32[hrec64[c].node].rho = (float)r[c]; }
+        for (int64_t c = 0; c < ncells; c++) {
+            hrec64[c].rho = r[c];
+            float f = (float)r[c];
+            std::memcpy(&hnode32[hrec64[c].node].val, &f, sizeof f);
+        }
         SG_HIP(hipMemcpy(rec64.p, hrec64.data(), ncells * sizeof(TreeRec64), hipMemcpyHostToDevice));
-        SG_HIP(hipMemcpy(rec32.p, hrec32.data(), hrec32.size() * sizeof(TreeRec32), hipMemcpyHostToDevice));
+        SG_HIP(hipMemcpy(node32.p, hnode32.data(), hnode32.size() * sizeof(TreeNode32), hipMemcpyHostToDevice));
     } else {
         for (int64_t c = 0; c < ncells; c++) hsite[c].rho = r[c];
         SG_HIP(hipMemcpy(site.p, hsite.data(), ncells * sizeof(VoroSite), hipMemcpyHostToDevice));
@@ -778,7 +798,7 @@ static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64
     a.counters = s->counters.as<unsigned long long>();
     a.next_packet = s->next.as<unsigned long long>();
     a.error = s->err.as<int>();
-    a.compact = g_tune_compact;
+    a.compact = g_tune_compact > 0 ? g_tune_compact : (g_pipeline == 1 ? 16 : 32);
     if (!s->scene.p) s->scene.alloc(sizeof(SceneDev));
     SG_HIP(hipMemcpy(s->scene.p, &sd, sizeof sd, hipMemcpyHostToDevice));
     a.scene = s->scene.as<SceneDev>();
@@ -790,7 +810,8 @@ static void launch_mc(const typename W::View& v, const McArgs& a, int& blocks, D
 {
     using R = typename W::Real;
     int bs = g_tune_block;
-    size_t shm = hot_words<W>() * 4 * (size_t)bs + (size_t)nlds * sizeof(int32_t);
+    // LDS: the hot-state parking area (only when lanes are re-armed out of line), then the top tree nodes
+    size_t shm = (SG_PARK ? hot_words<W>() * 4 * (size_t)bs : 0) + (size_t)nlds * sizeof(int32_t);
     int per = 0;
     SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc<W, ABS>, bs, shm));
     if (per < 1) per = 1;
@@ -826,8 +847,8 @@ template <bool ABS> static void dispatch_mc(sg_sim* s, const McArgs& a, int prec
         if (mixed) launch_mc<CartW32, ABS>(g->cart, a, blocks, s->cold, s->args);
         else launch_mc<CartW64, ABS>(g->cart, a, blocks, s->cold, s->args);
     } else if (g->kind == GK_TREE) {
-        if (!mixed) launch_mc<TreeW64, ABS>(g->tree, a, blocks, s->cold, s->args, g->tree.search == SG_TOPDOWN ? SG_LDS_NODES : 0);
-        else if (g->tree.search == SG_NEIGHBOR) launch_mc<TreeW32, ABS>(g->tree, a, blocks, s->cold, s->args);
+        if (!mixed) launch_mc<TreeW64, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
+        else if (g->tree.search == SG_NEIGHBOR) launch_mc<TreeW32, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
         else launch_mc<TreeW32TopDown, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
     } else {
         if (mixed) launch_mc<VoroW<float>, ABS>(g->voro, a, blocks, s->cold, s->args);
@@ -859,18 +880,23 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
     if (s->prog.bytes < 4 * sizeof(unsigned long long)) s->prog.alloc(4 * sizeof(unsigned long long));
     Slots<R> sl{s->slots.as<R>(),
                 reinterpret_cast<uint32_t*>(s->slots.as<unsigned char>() + (size_t)P * SR_NR * sizeof(R)), P};
-    // progress: [0] slots holding a packet, [1] queue tail, [2] packets claimed, [3] requests claimed
+    // progress: [1] queue tail, [2] packets claimed, [3] requests claimed
     auto* pc = s->prog.as<unsigned long long>();
     WalkQueue<R> q{s->queue.as<R>(),
                    reinterpret_cast<uint32_t*>(s->queue.as<unsigned char>() + (size_t)cap * QR_NR * sizeof(R)), cap,
                    pc + 1};
-    SG_HIP(hipMemset(sl.u + (size_t)SU_STATE * P, 0, (size_t)P * sizeof(uint32_t)));   // all PS_FRESH
+    // all slots PS_FRESH, slot counts zero (the state and count words are adjacent arrays)
+    SG_HIP(hipMemset(sl.u + (size_t)SU_STATE * P, 0, (size_t)P * (SU_NU - SU_STATE) * sizeof(uint32_t)));
     const int bs = 256;
     const size_t shm = (size_t)nlds * sizeof(int32_t);
     int per = 0;
     SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walks<W, ABS>, bs, shm));
     blocks = std::max(1, per) * g_cus;
-    const int eblocks = (int)((P + 255) / 256);
+    // event workgroups stage up to blockDim * (ni + 1) requests in LDS
+    int ebs = 256;
+    while (ebs > 64 && staged_bytes<R>(ebs * (a.ni + 1)) > 64 * 1024) ebs /= 2;
+    const size_t eshm = staged_bytes<R>(ebs * (a.ni + 1));
+    const int eblocks = (int)((P + ebs - 1) / ebs);
     const uint64_t batch = std::min<uint64_t>(count, (uint64_t)1 << 22);
     for (uint64_t b = 0; b < count; b += batch) {
         McArgs ab = a;
@@ -883,27 +909,30 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
         int emb = (int)std::min<uint64_t>((ab.count + 255) / 256, (uint64_t)g_cus * 16);
         hipLaunchKernelGGL(k_emit<R>, dim3(emb), dim3(256), 0, 0, ab, s->emitted.as<R>(), s->emitdraw.as<uint32_t>());
         SG_HIP(hipGetLastError());
-        if (b > 0) hipLaunchKernelGGL(k_revive, dim3(eblocks), dim3(256), 0, 0, sl.u + (size_t)SU_STATE * P, P);
+        if (b > 0) hipLaunchKernelGGL(k_revive, dim3((P + 255) / 256), dim3(256), 0, 0, sl.u + (size_t)SU_STATE * P, P);
         const bool last = b + batch >= count;
         for (;;) {
-            SG_HIP(hipMemsetAsync(pc, 0, 2 * sizeof(unsigned long long), 0));
-            hipLaunchKernelGGL(k_events<R>, dim3(eblocks), dim3(256), 0, 0, ab, sl, q,
-                               reinterpret_cast<SplitCounters*>(pc));
+            SG_HIP(hipMemsetAsync(pc + 1, 0, sizeof(unsigned long long), 0));
+            hipLaunchKernelGGL(k_events<R>, dim3(eblocks), dim3(ebs), eshm, 0, ab, sl, q);
             SG_HIP(hipGetLastError());
-            unsigned long long h[3];
-            SG_HIP(hipMemcpy(h, pc, sizeof h, hipMemcpyDeviceToHost));
-            SG_REQUIRE(h[1] <= (unsigned long long)cap, "walk queue overflow");
-            if (h[1] > 0) {
+            unsigned long long h[2];   // requests appended, packets claimed
+            SG_HIP(hipMemcpy(h, pc + 1, sizeof h, hipMemcpyDeviceToHost));
+            SG_REQUIRE(h[0] <= (unsigned long long)cap, "walk queue overflow");
+            if (h[0] > 0) {
                 SG_HIP(hipMemsetAsync(pc + 3, 0, sizeof(unsigned long long), 0));
                 lt.start();
                 hipLaunchKernelGGL((k_walks<W, ABS>), dim3(blocks), dim3(bs), shm, 0, v, ab, q, sl, pc + 3, nlds);
                 SG_HIP(hipGetLastError());
                 lt.stop();
             }
-            if (h[0] == 0) break;                      // nothing in flight and nothing left to claim
-            if (!last && h[2] >= ab.count) break;      // batch claimed: emit the next one
+            // all packets of the batch claimed: go on to the next batch, or, after the last one, stop
+            // once a pass has queued no walk (no packet is waiting for one)
+            if (h[1] >= ab.count && (!last || h[0] == 0)) break;
         }
     }
+    hipLaunchKernelGGL(k_slot_counts, dim3(std::min<int64_t>((P + 255) / 256, 1024)), dim3(256), 0, 0, sl.u, P,
+                       a.counters);
+    SG_HIP(hipGetLastError());
 }
 
 template <bool ABS>
@@ -922,9 +951,8 @@ static void dispatch_split(sg_sim* s, const McArgs& a, int precision, uint64_t f
         if (mixed) run_split<CartW32, ABS>(s, g->cart, a, first, count, 0, lt, blocks);
         else run_split<CartW64, ABS>(s, g->cart, a, first, count, 0, lt, blocks);
     } else if (g->kind == GK_TREE) {
-        const int nl = g->tree.search == SG_TOPDOWN ? SG_LDS_NODES : 0;
-        if (!mixed) run_split<TreeW64, ABS>(s, g->tree, a, first, count, nl, lt, blocks);
-        else if (g->tree.search == SG_NEIGHBOR) run_split<TreeW32, ABS>(s, g->tree, a, first, count, 0, lt, blocks);
+        if (!mixed) run_split<TreeW64, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
+        else if (g->tree.search == SG_NEIGHBOR) run_split<TreeW32, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
         else run_split<TreeW32TopDown, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
     } else {
         if (mixed) run_split<VoroW<float>, ABS>(s, g->voro, a, first, count, 0, lt, blocks);
@@ -1033,7 +1061,7 @@ static void launch_batch(sg_sim* s0, const std::vector<McArgs>& rg, const std::v
     SG_HIP(hipGetLastError());
 
     int bs = g_tune_block;
-    size_t shm = hot_words<W>() * 4 * (size_t)bs;
+    size_t shm = SG_PARK ? hot_words<W>() * 4 * (size_t)bs : 0;
     int per = 0;
     SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc_batch<W, ABS>, bs, shm));
     if (per < 1) per = 1;

@@ -535,7 +535,7 @@ __device__ __forceinline__ int crossing(Hot<W>& h, const typename W::View& g, co
             R expfactorm = -rexpm1<R>(-dtau);
             R Lintm = h.L0 * rexp<R>(-h.tau) * expfactorm;
             R Labsm = ((R)1.0 - (R)a.albedo) * Lintm;
-            unsafeAtomicAdd(a.labs + (size_t)a.ell * a.ncells + m, (double)Labsm);
+            unsafeAtomicAdd(a.labs + (size_t)a.ell * a.ncells + W::cell_of(g, m), (double)Labsm);
         }
         h.tau += dtau;
         kind = 1;
@@ -642,7 +642,7 @@ k_mc(const typename W::View g, const McArgs a, int nlds_req, typename W::Real* c
     __shared__ InstrDev s_instr[SG_MAX_INSTR];
     const int nl = blockDim.x;
     uint32_t* park = reinterpret_cast<uint32_t*>(smem);
-    int32_t* s_nodes = reinterpret_cast<int32_t*>(smem + hot_words<W>() * 4 * nl);
+    int32_t* s_nodes = reinterpret_cast<int32_t*>(smem + (SG_PARK ? hot_words<W>() * 4 * nl : 0));
     int nlds = 0;
     if (nlds_req > 0) {
         stage_tree_nodes(g, s_nodes, nlds);

@@ -21,7 +21,10 @@
 namespace sg {
 
 enum SlotR { SR_RX = 0, SR_RY, SR_RZ, SR_KX, SR_KY, SR_KZ, SR_L, SR_RES, SR_NR };
-enum SlotU { SU_PLO = 0, SU_PHI, SU_DRAW, SU_W0, SU_W1, SU_W2, SU_W3, SU_NSCATT, SU_STATE, SU_NU };
+// The last four slot words count the slot's packets, scatterings, detected peel-offs and started
+// walks over a run; they are summed once at the end (no per-pass atomics).
+enum SlotU { SU_PLO = 0, SU_PHI, SU_DRAW, SU_W0, SU_W1, SU_W2, SU_W3, SU_NSCATT, SU_STATE, SU_CPK, SU_CSCAT, SU_CPEEL,
+             SU_CWALK, SU_NU };
 enum PacketState : uint32_t { PS_FRESH = 0, PS_OWN, PS_PROP, PS_DEAD };
 enum WalkKind : uint32_t { WK_PEEL = 0, WK_OWN = 1, WK_PROP = 2 };
 
@@ -57,10 +60,6 @@ template <typename R> __host__ __device__ constexpr size_t queue_bytes(int64_t c
     return (size_t)cap * (QR_NR * sizeof(R) + QU_NU * sizeof(uint32_t));
 }
 
-// Host-visible progress of one event pass: [0] slots still holding a packet, [1] walks appended.
-struct SplitCounters {
-    unsigned long long alive, pushed;
-};
 
 // Appends one request per active lane (one atomic per wave); returns this lane's queue position.
 __device__ __forceinline__ uint64_t wave_append(unsigned long long* tail)
@@ -76,23 +75,41 @@ __device__ __forceinline__ uint64_t wave_append(unsigned long long* tail)
     return base + (uint64_t)__popcll(below);
 }
 
-template <typename R>
-__device__ __forceinline__ void push_walk(const WalkQueue<R>& q, uint32_t kind, uint32_t inst, uint32_t idx, R ox,
-                                          R oy, R oz, R dx, R dy, R dz, R val)
+// Walk requests of one workgroup staged in LDS (structure of arrays, cap entries); the workgroup
+// appends them to the walk queue as one block with a single global atomic.
+template <typename R> struct StagedQueue {
+    R* r;
+    uint32_t* u;
+    int cap;
+    unsigned* cnt;
+    __device__ __forceinline__ void push(uint32_t kind, uint32_t inst, uint32_t idx, R ox, R oy, R oz, R dx, R dy,
+                                         R dz, R val) const
+    {
+        uint64_t mask = __ballot(1);
+        int lane = __lane_id();
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(cnt, (unsigned)__popcll(mask));
+        base = __shfl(base, leader);
+        uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+        const int i = (int)(base + (unsigned)__popcll(below));
+        r[QR_OX * cap + i] = ox; r[QR_OY * cap + i] = oy; r[QR_OZ * cap + i] = oz;
+        r[QR_DX * cap + i] = dx; r[QR_DY * cap + i] = dy; r[QR_DZ * cap + i] = dz;
+        r[QR_VAL * cap + i] = val;
+        u[QU_TAG * cap + i] = kind | (inst << 2);
+        u[QU_IDX * cap + i] = idx;
+    }
+};
+template <typename R> __host__ __device__ constexpr size_t staged_bytes(int cap)
 {
-    uint64_t i = wave_append(q.tail);
-    q.fr(QR_OX, i) = ox; q.fr(QR_OY, i) = oy; q.fr(QR_OZ, i) = oz;
-    q.fr(QR_DX, i) = dx; q.fr(QR_DY, i) = dy; q.fr(QR_DZ, i) = dz;
-    q.fr(QR_VAL, i) = val;
-    q.fu(QU_TAG, i) = kind | (inst << 2);
-    q.fu(QU_IDX, i) = idx;
+    return (size_t)cap * (QR_NR * sizeof(R) + QU_NU * sizeof(uint32_t));
 }
 
 // Peel-offs toward every instrument (MonteCarloSimulation::peeloffemission/peeloffscattering,
 // MonteCarloSimulation.cpp:299-357): pixel, phase-function weight (scattering only), and either the
 // walk request or, without a dust system, the detection itself.
-template <typename R>
-__device__ __forceinline__ uint32_t peel_offs(const McArgs& a, const InstrDev* instr, const WalkQueue<R>& q,
+template <typename R, class Q>
+__device__ __forceinline__ uint32_t peel_offs(const McArgs& a, const InstrDev* instr, const Q& q,
                                               bool scattering, R rx, R ry, R rz, R kx, R ky, R kz, R L)
 {
     const R gg = (R)a.g;
@@ -110,7 +127,7 @@ __device__ __forceinline__ uint32_t peel_offs(const McArgs& a, const InstrDev* i
         }
         n++;
         if (!a.has_dust) unsafeAtomicAdd(in.frame + (size_t)a.ell * in.Nx * in.Ny + pix, (double)Lp);
-        else push_walk<R>(q, WK_PEEL, (uint32_t)i, (uint32_t)pix, rx, ry, rz, (R)0, (R)0, (R)0, Lp);
+        else q.push(WK_PEEL, (uint32_t)i, (uint32_t)pix, rx, ry, rz, (R)0, (R)0, (R)0, Lp);
     }
     return n;
 }
@@ -143,12 +160,20 @@ __device__ __forceinline__ void hg_scatter(PacketRng& rng, R gg, R& kx, R& ky, R
     kx = nkx; ky = nky; kz = nkz;
 }
 
-// Event pass: every slot advances its packet to its next walk (or takes a new packet).
+// Event pass: every slot advances its packet to its next walk (or takes a new packet). Dynamic LDS:
+// the workgroup's staged requests, blockDim.x * (a.ni + 1) entries (the most one pass can append).
 template <typename R>
-__global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, WalkQueue<R> q, SplitCounters* prog)
+__global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, WalkQueue<R> q)
 {
     __shared__ InstrDev s_instr[SG_MAX_INSTR];
+    __shared__ unsigned s_cnt;
+    __shared__ unsigned long long s_base;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int cap = (int)blockDim.x * (a.ni + 1);
+    StagedQueue<R> sq{reinterpret_cast<R*>(smem), reinterpret_cast<uint32_t*>(smem + (size_t)cap * QR_NR * sizeof(R)),
+                      cap, &s_cnt};
     if (threadIdx.x < SG_MAX_INSTR) s_instr[threadIdx.x] = a.scene->instr[threadIdx.x];
+    if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // every lane stays to the end: the counts below are summed over the whole wave
@@ -186,11 +211,11 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
                     philox10(rng.w0, rng.w1, rng.w2, rng.w3, rng.k0, rng.k1);
                 }
                 nscatt = 0;
-                uint32_t np = peel_offs<R>(a, s_instr, q, false, rx, ry, rz, kx, ky, kz, L);
+                uint32_t np = peel_offs<R>(a, s_instr, sq, false, rx, ry, rz, kx, ky, kz, L);
                 c_npeel += np;
                 if (!a.has_dust) continue;
                 c_walks += np + 1;
-                push_walk<R>(q, WK_OWN, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, L);
+                sq.push(WK_OWN, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, L);
                 state = PS_OWN;
                 break;
             }
@@ -218,7 +243,7 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
                         R qq = ((R)1.0 - (R)a.xi) * pp + (R)a.xi / taupath;
                         L = L * (pp / qq);
                     }
-                    push_walk<R>(q, WK_PROP, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, tt);
+                    sq.push(WK_PROP, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, tt);
                     state = PS_PROP;
                     break;
                 }
@@ -230,13 +255,13 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
                 // PhotonPackage::propagate (PhotonPackage.cpp:93-96) by pathlength(tau)
                 R d = sl.fr(SR_RES, i);
                 rx = rx + d * kx; ry = ry + d * ky; rz = rz + d * kz;
-                uint32_t np = peel_offs<R>(a, s_instr, q, true, rx, ry, rz, kx, ky, kz, L);
+                uint32_t np = peel_offs<R>(a, s_instr, sq, true, rx, ry, rz, kx, ky, kz, L);
                 c_npeel += np;
                 hg_scatter<R>(rng, gg, kx, ky, kz);
                 nscatt++;
                 c_scat++;
                 c_walks += np + 1;
-                push_walk<R>(q, WK_OWN, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, L);
+                sq.push(WK_OWN, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, L);
                 state = PS_OWN;
                 break;
             }
@@ -247,18 +272,89 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
         sl.fu(SU_PLO, i) = rng.plo; sl.fu(SU_PHI, i) = rng.phi; sl.fu(SU_DRAW, i) = rng.draw;
         sl.fu(SU_W0, i) = rng.w0; sl.fu(SU_W1, i) = rng.w1; sl.fu(SU_W2, i) = rng.w2; sl.fu(SU_W3, i) = rng.w3;
         sl.fu(SU_NSCATT, i) = (uint32_t)nscatt;
-        sl.fu(SU_STATE, i) = state;
+        sl.fu(SU_CPK, i) += c_pk; sl.fu(SU_CSCAT, i) += c_scat; sl.fu(SU_CPEEL, i) += c_npeel;
+        sl.fu(SU_CWALK, i) += c_walks;
     }
-    add_wave(a.counters + 0, c_pk);
-    add_wave(a.counters + 3, c_scat);
-    add_wave(a.counters + 4, c_npeel);
-    add_wave(a.counters + 7, c_walks);
-    add_wave(&prog->alive, state != PS_DEAD ? 1u : 0u);
+    if (i < sl.n) sl.fu(SU_STATE, i) = state;
+    // append the workgroup's requests to the walk queue
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(q.tail, (unsigned long long)s_cnt) : 0ull;
+    __syncthreads();
+    const unsigned n = s_cnt;
+    const uint64_t base = s_base;
+    for (unsigned j = threadIdx.x; j < n; j += blockDim.x) {
+#pragma unroll
+        for (int f = 0; f < QR_NR; f++) q.fr(f, base + j) = sq.r[f * cap + j];
+#pragma unroll
+        for (int f = 0; f < QU_NU; f++) q.fu(f, base + j) = sq.u[f * cap + j];
+    }
 }
+
+// End of a run: the slots' packet, scattering, peel-off and walk counts into the run counters.
+__global__ void __launch_bounds__(256) k_slot_counts(const uint32_t* u, int64_t n, unsigned long long* counters)
+{
+    __shared__ unsigned long long part[4][4];
+    unsigned long long v[4] = {0, 0, 0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        v[0] += u[SU_CPK * n + i]; v[1] += u[SU_CSCAT * n + i]; v[2] += u[SU_CPEEL * n + i];
+        v[3] += u[SU_CWALK * n + i];
+    }
+    for (int k = 0; k < 4; k++)
+        for (int off = 32; off > 0; off >>= 1) {
+            uint32_t lo = __shfl_down((uint32_t)v[k], off), hi = __shfl_down((uint32_t)(v[k] >> 32), off);
+            v[k] += ((unsigned long long)hi << 32) | lo;
+        }
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 0)
+        for (int k = 0; k < 4; k++) part[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int dst[4] = {0, 3, 4, 7};
+        for (int k = 0; k < 4; k++) {
+            unsigned long long t = 0;
+            for (int j = 0; j < (int)(blockDim.x >> 6); j++) t += part[k][j];
+            if (t) atomicAdd(counters + dst[k], t);
+        }
+    }
+}
+
+// Request claims of one wave: chunks of kClaimChunk requests per global atomic, handed out in order to
+// the lanes that ask. Called by the whole wave; cur/end is the wave's current chunk (equal in every
+// lane). Returns whether this lane got request r.
+constexpr uint64_t kClaimChunk = 128;
+__device__ __forceinline__ bool chunk_claim(unsigned long long* ctr, uint64_t nq, bool want, uint64_t& cur,
+                                            uint64_t& end, uint64_t& r)
+{
+    const uint64_t mask = __ballot(want);
+    const uint64_t k = (uint64_t)__popcll(mask);
+    if (k == 0) return false;
+    const int lane = __lane_id();
+    const uint64_t rank = (uint64_t)__popcll(mask & (lane ? (~0ull >> (64 - lane)) : 0ull));
+    const uint64_t avail = end - cur;
+    uint64_t nb = 0, ne = 0;
+    if (avail < k) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(ctr, (unsigned long long)kClaimChunk);
+        const uint32_t lo = __shfl((uint32_t)b, 0), hi = __shfl((uint32_t)(b >> 32), 0);
+        nb = ((uint64_t)hi << 32) | lo;
+        ne = nb >= nq ? nb : min(nb + kClaimChunk, nq);
+    }
+    bool ok;
+    if (rank < avail) { r = cur + rank; ok = true; }
+    else { r = nb + (rank - avail); ok = r < ne; }
+    if (avail >= k) cur += k;
+    else { cur = min(nb + (k - avail), ne); end = ne; }
+    return ok && want;
+}
+
+// Occupancy target of the walk kernel (waves per SIMD).
+#ifndef SG_WALK_WAVES_PER_EU
+#define SG_WALK_WAVES_PER_EU 4
+#endif
 
 // Walk pass: persistent lanes over the requests [0, *q.tail).
 template <class W, bool ABS>
-__global__ void __launch_bounds__(256) k_walks(const typename W::View g, const McArgs a, WalkQueue<typename W::Real> q,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_WALK_WAVES_PER_EU, 8))) k_walks(const typename W::View g, const McArgs a, WalkQueue<typename W::Real> q,
                                               Slots<typename W::Real> sl, unsigned long long* claim, int nlds_req)
 {
     using R = typename W::Real;
@@ -278,6 +374,7 @@ __global__ void __launch_bounds__(256) k_walks(const typename W::View g, const M
     h.w.nrecompute = 0; h.w.nstuck = 0;
     uint32_t idx = 0, inst = 0;
     unsigned long long n_own = 0, n_peel = 0, n_re = 0;   // wave-uniform counts
+    uint64_t ccur = 0, cend = 0;                          // the wave's chunk of requests
     const int thr = a.compact;
     for (;;) {
         if (h.stage == ST_PEELW_DONE) {
@@ -292,9 +389,11 @@ __global__ void __launch_bounds__(256) k_walks(const typename W::View g, const M
             sl.fr(SR_RES, idx) = h.s;
             h.stage = ST_NEWPKT;
         }
-        if (h.stage == ST_NEWPKT) {
-            uint64_t r;
-            if (!wave_claim(claim, nq, r)) h.stage = ST_DEAD;
+        const bool want = h.stage == ST_NEWPKT;
+        uint64_t r = 0;
+        const bool got = chunk_claim(claim, nq, want, ccur, cend, r);
+        if (want) {
+            if (!got) h.stage = ST_DEAD;
             else {
                 const uint32_t tag = q.fu(QU_TAG, r);
                 const uint32_t kind = tag & 3u;

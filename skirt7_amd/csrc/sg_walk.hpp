@@ -78,6 +78,7 @@ struct CartW64 {
     bool done;
     uint32_t nrecompute, nstuck;
     static constexpr int kParkWords = 25;
+    __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
@@ -216,6 +217,7 @@ struct CartW32 {
     bool done;
     uint32_t nrecompute, nstuck;
     static constexpr int kParkWords = 19;
+    __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz); p.io(ix); p.io(iy); p.io(iz);
@@ -426,6 +428,7 @@ struct TreeW64 {
     bool done;
     uint32_t steps, nrecompute, nstuck;
     static constexpr int kParkWords = 38;
+    __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
@@ -518,81 +521,103 @@ struct TreeW64 {
 
 // Mixed-precision tree walk (Neighbor semantics): FP32 position snapped onto the exit wall; the
 // next cell is found by descending from the rope of the exit wall (the node across it that covers
-// the whole wall), one node record per level; a directed root descent covers the rare case of a
-// point that left the rope node's box through an edge or corner.
+// the whole wall), one 32-byte node record per level; a directed root descent covers the rare case
+// of a point that left the rope node's box through an edge or corner. The walker carries the
+// current leaf's integer indices along x, y, z; every box is ext + [k, k + 1] * width / 2^n.
 struct TreeW32 {
     using Real = float;
     using View = TreeView;
     float x, y, z, kx, ky, kz, ix, iy, iz;
-    float lo[3], hi[3];
+    int32_t ci, cj, ck;      // integer indices of the current leaf
+    uint32_t nsub;           // its subdivision counts, nx | ny << 5 | nz << 10
     float rho;
     int32_t rope[6];
-    int m;
+    int m;                   // node id of the current leaf
     int ne;
     float e0, e1, e2;
     bool done;
     uint32_t steps, nrecompute, nstuck;
-    static constexpr int kParkWords = 31;
+    static constexpr int kParkWords = 29;
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz); p.io(ix); p.io(iy); p.io(iz);
-        p.io(lo[0]); p.io(lo[1]); p.io(lo[2]); p.io(hi[0]); p.io(hi[1]); p.io(hi[2]);
+        p.io(ci); p.io(cj); p.io(ck); p.io(nsub);
         p.io(rho); p.io(rope[0]); p.io(rope[1]); p.io(rope[2]); p.io(rope[3]); p.io(rope[4]); p.io(rope[5]);
         p.io(m); p.io(ne); p.io(e0); p.io(e1); p.io(e2);
         p.io(done); p.io(steps); p.io(nrecompute); p.io(nstuck);
     }
+    // cell number of a walk segment's node
+    __device__ __forceinline__ static int cell_of(const View& t, int n) { return t.cellof[n]; }
 
-    // Loads node record n (4 x 16 bytes). Returns the first child (>= 0) of an internal node, or -1
-    // after taking over the bounds, density and ropes of a leaf.
-    __device__ __forceinline__ int load_node(const View& t, int n, float* l, float* h)
+    __device__ __forceinline__ static float width(const View& t, int a, uint32_t ns)
     {
-        const float4* p = reinterpret_cast<const float4*>(t.rec32 + n);
-        float4 a = p[0], b = p[1], c = p[2], d = p[3];
-        l[0] = a.x; l[1] = a.y; l[2] = a.z; h[0] = a.w; h[1] = b.x; h[2] = b.y;
-        int child0 = __float_as_int(d.z);
-        if (child0 < 0) {
-            lo[0] = l[0]; lo[1] = l[1]; lo[2] = l[2]; hi[0] = h[0]; hi[1] = h[1]; hi[2] = h[2];
-            rho = b.z;
-            rope[0] = __float_as_int(b.w); rope[1] = __float_as_int(c.x); rope[2] = __float_as_int(c.y);
-            rope[3] = __float_as_int(c.z); rope[4] = __float_as_int(c.w); rope[5] = __float_as_int(d.x);
-            m = __float_as_int(d.y);
-        }
-        return child0;
+        return ldexpf(t.widthf[a], -(int)((ns >> (5 * a)) & 31u));
     }
-    // Descends from node n to the leaf containing the current position; child choice with the
-    // reference's strict "<" against the split point (OctTreeNode.cpp:184-189, BinTreeNode.cpp:326-335).
-    __device__ __forceinline__ void descend_from(const View& t, int n)
+    // Node record n as two 16-byte loads issued together: (code, val, rope0, rope1), (rope2..rope5).
+    __device__ __forceinline__ static void load(const View& t, int n, uint4& a, uint4& b)
     {
-        float l[3], h[3];
+        const uint4* p = reinterpret_cast<const uint4*>(t.node32 + n);
+        a = p[0];
+        b = p[1];
+    }
+    // Descends from node n (record a, b already loaded; indices i, j, k at its subdivision) to the
+    // leaf containing the current position and takes it over; child choice with the reference's
+    // strict "<" against the split point (OctTreeNode.cpp:184-189, BinTreeNode.cpp:326-335).
+    __device__ __forceinline__ void descend_loaded(const View& t, uint4 a, uint4 b, int n, int32_t i, int32_t j,
+                                                   int32_t k)
+    {
         int level = 0;
-        int c0;
-        while ((c0 = load_node(t, n, l, h)) >= 0) {
+        while (!(a.x & (1u << 15))) {
+            const uint32_t ns = a.x & 0x7fffu;
+            const int c0 = (int)a.y;
             if (t.arity == 8) {
-                int q = (x < 0.5f * (l[0] + h[0]) ? 0 : 1) + (y < 0.5f * (l[1] + h[1]) ? 0 : 2) +
-                        (z < 0.5f * (l[2] + h[2]) ? 0 : 4);
-                n = c0 + q;
+                // split points ext + (2k + 1) * width / 2^(n + 1)
+                const float cx = t.extf[0] + (float)(2 * i + 1) * width(t, 0, ns + 1u);
+                const float cy = t.extf[1] + (float)(2 * j + 1) * width(t, 1, ns + (1u << 5));
+                const float cz = t.extf[2] + (float)(2 * k + 1) * width(t, 2, ns + (1u << 10));
+                const int qx = x < cx ? 0 : 1, qy = y < cy ? 0 : 1, qz = z < cz ? 0 : 1;
+                n = c0 + qx + 2 * qy + 4 * qz;
+                i = 2 * i + qx; j = 2 * j + qy; k = 2 * k + qz;
             } else {
                 // binary node: split at the midpoint of its recorded axis
-                int a = t.rec32[n].dir;
-                bool low = a == 0 ? x < 0.5f * (l[0] + h[0]) : a == 1 ? y < 0.5f * (l[1] + h[1])
-                                                                      : z < 0.5f * (l[2] + h[2]);
-                n = low ? c0 : c0 + 1;
+                const int d = (int)((a.x >> 16) & 3u);
+                const uint32_t ns1 = ns + (1u << (5 * d));
+                int q;
+                if (d == 0) { q = x < t.extf[0] + (float)(2 * i + 1) * width(t, 0, ns1) ? 0 : 1; i = 2 * i + q; }
+                else if (d == 1) { q = y < t.extf[1] + (float)(2 * j + 1) * width(t, 1, ns1) ? 0 : 1; j = 2 * j + q; }
+                else { q = z < t.extf[2] + (float)(2 * k + 1) * width(t, 2, ns1) ? 0 : 1; k = 2 * k + q; }
+                n = c0 + q;
             }
-            if (++level > 64) break;
+            load(t, n, a, b);
+            if (++level > 64) return;
         }
+        ci = i; cj = j; ck = k;
+        nsub = a.x & 0x7fffu;
+        rho = __uint_as_float(a.y);
+        rope[0] = (int32_t)a.z; rope[1] = (int32_t)a.w;
+        rope[2] = (int32_t)b.x; rope[3] = (int32_t)b.y; rope[4] = (int32_t)b.z; rope[5] = (int32_t)b.w;
+        m = n;
+    }
+    __device__ __forceinline__ void descend_from(const View& t, int n, int32_t i, int32_t j, int32_t k)
+    {
+        uint4 a, b;
+        load(t, n, a, b);
+        descend_loaded(t, a, b, n, i, j, k);
     }
     // Root descent in FP32 with the exit axis compared in the travel direction, so a point lying
-    // exactly on the exit wall goes to the cell beyond it. Returns the leaf's node id or -1.
-    __device__ __forceinline__ int descend_root(const View& t, const int32_t* lds, int nlds, int xa, int xdir)
+    // exactly on the exit wall goes to the cell beyond it. Returns the leaf's node id (and its
+    // indices) or -1.
+    __device__ __forceinline__ int descend_root(const View& t, const int32_t* lds, int nlds, int xa, int xdir,
+                                                int32_t& i, int32_t& j, int32_t& k) const
     {
         const float* e = t.extf;
-        float p[3] = {x, y, z};
+        const float p[3] = {x, y, z};
         for (int a = 0; a < 3; a++) {
             bool in = p[a] >= e[a] && p[a] <= e[a + 3];
             if (a == xa) in = xdir > 0 ? (p[a] >= e[a] && p[a] < e[a + 3]) : (p[a] > e[a] && p[a] <= e[a + 3]);
             if (!in) return -1;
         }
-        float l[3] = {e[0], e[1], e[2]}, h[3] = {e[3], e[4], e[5]};
+        int32_t kk[3] = {0, 0, 0};
         int id = 0;
         int32_t w = node_word(t, lds, nlds, 0);
         int level = 0;
@@ -601,24 +626,27 @@ struct TreeW32 {
             if (t.arity == 8) {
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    float c = 0.5f * (l[a] + h[a]);
-                    bool low = (a == xa && xdir < 0) ? (p[a] <= c) : (p[a] < c);
-                    if (low) h[a] = c; else { l[a] = c; q += 1 << a; }
+                    const float c = e[a] + (float)(2 * kk[a] + 1) * ldexpf(t.widthf[a], -(level + 1));
+                    const bool low = (a == xa && xdir < 0) ? (p[a] <= c) : (p[a] < c);
+                    kk[a] = 2 * kk[a] + (low ? 0 : 1);
+                    q += low ? 0 : 1 << a;
                 }
             } else {
-                int a = level % 3;
+                const int a = level % 3;
 #pragma unroll
                 for (int b = 0; b < 3; b++) {
                     if (b != a) continue;
-                    float c = 0.5f * (l[b] + h[b]);
-                    bool low = (b == xa && xdir < 0) ? (p[b] <= c) : (p[b] < c);
-                    if (low) h[b] = c; else { l[b] = c; q = 1; }
+                    const float c = e[b] + (float)(2 * kk[b] + 1) * ldexpf(t.widthf[b], -(level / 3 + 1));
+                    const bool low = (b == xa && xdir < 0) ? (p[b] <= c) : (p[b] < c);
+                    kk[b] = 2 * kk[b] + (low ? 0 : 1);
+                    q = low ? 0 : 1;
                 }
             }
             id = w + q;
             w = node_word(t, lds, nlds, id);
             level++;
         }
+        i = kk[0]; j = kk[1]; k = kk[2];
         return id;
     }
     __device__ __forceinline__ void push(float ds)
@@ -644,24 +672,24 @@ struct TreeW32 {
         ix = fabsf(kx) > 1e-15f ? 1.0f / kx : 0.0f;
         iy = fabsf(ky) > 1e-15f ? 1.0f / ky : 0.0f;
         iz = fabsf(kz) > 1e-15f ? 1.0f / kz : 0.0f;
-        int n = descend_root(t, lds, nlds, -1, 0);
+        int32_t i, j, k;
+        int n = descend_root(t, lds, nlds, -1, 0, i, j, k);
         if (n < 0) { ne = 0; done = true; return; }
-        float l[3], h[3];
-        load_node(t, n, l, h);
+        descend_from(t, n, i, j, k);
     }
     // Chooses the exit wall and advances onto it; returns the wall (0..5) and the segment length.
-    __device__ __forceinline__ int exit_wall(float& d)
+    __device__ __forceinline__ int exit_wall(const View& t, float& d)
     {
-        float xn = (kx < 0.0f) ? lo[0] : hi[0];
-        float yn = (ky < 0.0f) ? lo[1] : hi[1];
-        float zn = (kz < 0.0f) ? lo[2] : hi[2];
+        const float wx = width(t, 0, nsub), wy = width(t, 1, nsub), wz = width(t, 2, nsub);
+        const float xn = t.extf[0] + (float)(kx < 0.0f ? ci : ci + 1) * wx;
+        const float yn = t.extf[1] + (float)(ky < 0.0f ? cj : cj + 1) * wy;
+        const float zn = t.extf[2] + (float)(kz < 0.0f ? ck : ck + 1) * wz;
         float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
         float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
         float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
         int a = choose_axis_tree(dsx, dsy, dsz);
         d = a == 0 ? dsx : a == 1 ? dsy : dsz;
-        if (ambiguous(dsx, dsy, dsz, d, a == 0 ? kx : a == 1 ? ky : kz,
-                      a == 0 ? hi[0] - lo[0] : a == 1 ? hi[1] - lo[1] : hi[2] - lo[2])) {
+        if (ambiguous(dsx, dsy, dsz, d, a == 0 ? kx : a == 1 ? ky : kz, a == 0 ? wx : a == 1 ? wy : wz)) {
             nrecompute++;
             double X = x, Y = y, Z = z;
             double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
@@ -677,6 +705,37 @@ struct TreeW32 {
         float dir = a == 0 ? kx : a == 1 ? ky : kz;
         return 2 * a + (dir < 0.0f ? 0 : 1);
     }
+    // Moves to the leaf beyond `wall` from the node r across it (or, when the position is not in r's
+    // box, by a directed root descent).
+    __device__ __forceinline__ void step_through(const View& t, const int32_t* lds, int nlds, int wall, int r)
+    {
+        uint4 a, b;
+        load(t, r, a, b);
+        const uint32_t nsr = a.x & 0x7fffu;
+        const int ax = wall >> 1;
+        const int s = (wall & 1) ? 1 : -1;
+        const int dx = (int)(nsub & 31u) - (int)(nsr & 31u);
+        const int dy = (int)((nsub >> 5) & 31u) - (int)((nsr >> 5) & 31u);
+        const int dz = (int)((nsub >> 10) & 31u) - (int)((nsr >> 10) & 31u);
+        bool in = dx >= 0 && dy >= 0 && dz >= 0;
+        int32_t i = 0, j = 0, k = 0;
+        if (in) {
+            i = (ci + (ax == 0 ? s : 0)) >> dx;
+            j = (cj + (ax == 1 ? s : 0)) >> dy;
+            k = (ck + (ax == 2 ? s : 0)) >> dz;
+            const float wx = width(t, 0, nsr), wy = width(t, 1, nsr), wz = width(t, 2, nsr);
+            const float lx = t.extf[0] + (float)i * wx, hx = t.extf[0] + (float)(i + 1) * wx;
+            const float ly = t.extf[1] + (float)j * wy, hy = t.extf[1] + (float)(j + 1) * wy;
+            const float lz = t.extf[2] + (float)k * wz, hz = t.extf[2] + (float)(k + 1) * wz;
+            in = x >= lx && x <= hx && y >= ly && y <= hy && z >= lz && z <= hz;
+        }
+        if (in) descend_loaded(t, a, b, r, i, j, k);
+        else {
+            int n = descend_root(t, lds, nlds, ax, s, i, j, k);
+            if (n < 0) done = true;
+            else descend_from(t, n, i, j, k);
+        }
+    }
     __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds, float& rh)
     {
         if (ne) {
@@ -687,22 +746,13 @@ struct TreeW32 {
             float d;
             int segm = m;
             float segrho = rho;
-            int wall = exit_wall(d);
+            int wall = exit_wall(t, d);
             int r = wall == 0 ? rope[0] : wall == 1 ? rope[1] : wall == 2 ? rope[2] : wall == 3 ? rope[3]
                   : wall == 4 ? rope[4] : rope[5];
             if (r < 0) done = true;   // a wall of the grid: the path leaves the grid
             else {
                 int oldm = m;
-                // the position must lie in the rope node's box (it may have slipped over an edge)
-                const TreeRec32& rr = t.rec32[r];
-                bool in = x >= rr.lo[0] && x <= rr.hi[0] && y >= rr.lo[1] && y <= rr.hi[1] && z >= rr.lo[2] &&
-                          z <= rr.hi[2];
-                if (in) descend_from(t, r);
-                else {
-                    int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1);
-                    if (n < 0) done = true;
-                    else descend_from(t, n);
-                }
+                step_through(t, lds, nlds, wall, r);
                 if (!done && m == oldm) { nstuck++; done = true; }
             }
             if (++steps > SG_MAX_STEPS) done = true;
@@ -725,13 +775,13 @@ struct TreeW32TopDown : TreeW32 {
             float d;
             int segm = m;
             float segrho = rho;
-            int wall = exit_wall(d);
+            int wall = exit_wall(t, d);
             int oldm = m;
-            int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1);
+            int32_t i, j, k;
+            int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1, i, j, k);
             if (n < 0) done = true;
             else {
-                float l[3], h[3];
-                load_node(t, n, l, h);
+                descend_from(t, n, i, j, k);
                 if (m == oldm) { nstuck++; done = true; }
             }
             if (++steps > SG_MAX_STEPS) done = true;
@@ -788,6 +838,7 @@ template <typename R> struct VoroW {
     bool done;
     uint32_t steps, nrecompute, nstuck;
     static constexpr int kParkWords = 24;
+    __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
