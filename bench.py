@@ -29,6 +29,9 @@ HBM_PEAK = 8.0e12
 # algorithmic bytes per cell crossing (octree leaf walk: 6 bounds + kappa*rho + one neighbour link)
 BYTES_PER_CROSSING = {"mixed": 6 * 4 + 4 + 4, "fp64": 6 * 8 + 8 + 4}
 BYTES_PER_PEELOFF = 16   # FP64 read-modify-write of one pixel
+# HBM bytes per cell crossing of the walk kernel, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by
+# the copy-kernel calibration) on this workload
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_walks_config1_octree_traffic.json")
 
 
 def log(msg):
@@ -158,8 +161,19 @@ def main():
     # algorithmic bytes of the transport kernel over its measured device time (all launches)
     bytes_launch = cross * BYTES_PER_CROSSING["mixed"] + cm["peeloffs"] * BYTES_PER_PEELOFF
     achieved = bytes_launch / kmix
+    launches = max(1, cm["transport_launches"])
+    traffic = None
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        if a.pipeline == tr["pipeline"]:
+            # measured HBM bytes per crossing times this run's crossings per launch
+            traffic = (tr["fetch_bytes_per_crossing"] + tr["write_bytes_per_crossing"]) * cross / launches
+    except (OSError, KeyError, ValueError):
+        traffic = None
     out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK, "traffic": None}
+                       "frac": achieved / HBM_PEAK, "traffic": traffic,
+                       "algorithmic_bytes_per_launch": bytes_launch / launches}
     out["crossings_per_s"] = cross / kmix
     out["transport_kernel_ms_per_launch"] = 1e3 * kmix / max(1, cm["transport_launches"])
     out["counters"] = cm
