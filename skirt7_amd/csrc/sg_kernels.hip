@@ -722,9 +722,14 @@ struct LaunchTimer {
         SG_HIP(hipEventRecord(a, 0));
     }
     void stop() { SG_HIP(hipEventRecord(ev.back().second, 0)); }
+    // a span measured elsewhere that covered `launches` launches
+    void add(double ms, int launches) { extra_ms += ms; extra_launches += launches; }
+    double extra_ms = 0;
+    int extra_launches = 0;
+    int launches() const { return (int)ev.size() + extra_launches; }
     double total_ms()
     {
-        double t = 0;
+        double t = extra_ms;
         for (auto& p : ev) {
             float ms = 0;
             SG_HIP(hipEventSynchronize(p.second));
@@ -864,29 +869,52 @@ __global__ void k_revive(uint32_t* state, int64_t n)
 }
 
 static const int64_t kSplitSlots = (int64_t)1 << 22;
+static const uint64_t kSplitEmit = (uint64_t)1 << 25;   // packets emitted at once
+static const int kSplitCheck = 4;                       // passes between progress checks
 
-// Packets [first, first+count) through alternating event and walk passes. Emission runs in batches
-// of up to 2^22 packets into the emission buffer; a batch is replaced once all its packets have been
-// claimed (packets in flight keep their own state in their slots).
+// Two non-blocking streams of default priority, one per slot group, and a pinned progress buffer.
+static hipStream_t g_split_stream[2] = {nullptr, nullptr};
+static unsigned long long* g_split_host = nullptr;
+static void split_resources()
+{
+    if (g_split_stream[0]) return;
+    for (int g = 0; g < 2; g++) SG_HIP(hipStreamCreateWithFlags(&g_split_stream[g], hipStreamNonBlocking));
+    SG_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_split_host), 8 * sizeof(unsigned long long), hipHostMallocDefault));
+}
+
+// Packets [first, first+count) through event and walk passes over two slot groups that alternate on two
+// streams, so that one group's event pass and the drain of its walk pass overlap the other group's
+// walks. Up to 2^25 packets are emitted at once; the host reads the progress every few passes and
+// stops when every packet has been claimed and neither group queued a walk in its last pass.
 template <class W, bool ABS>
 static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uint64_t first, uint64_t count, int nlds,
                       LaunchTimer& lt, int& blocks)
 {
     using R = typename W::Real;
+    split_resources();
+    hipStream_t* st = g_split_stream;
     const int64_t P = (int64_t)std::max<uint64_t>(1, std::min<uint64_t>(count, (uint64_t)kSplitSlots));
-    const int64_t cap = P * (a.ni + 1);
-    if (s->slots.bytes < slot_bytes<R>(P)) s->slots.alloc(slot_bytes<R>(P));
-    if (s->queue.bytes < queue_bytes<R>(cap)) s->queue.alloc(queue_bytes<R>(cap));
-    if (s->prog.bytes < 4 * sizeof(unsigned long long)) s->prog.alloc(4 * sizeof(unsigned long long));
-    Slots<R> sl{s->slots.as<R>(),
-                reinterpret_cast<uint32_t*>(s->slots.as<unsigned char>() + (size_t)P * SR_NR * sizeof(R)), P};
-    // progress: [1] queue tail, [2] packets claimed, [3] requests claimed
-    auto* pc = s->prog.as<unsigned long long>();
-    WalkQueue<R> q{s->queue.as<R>(),
-                   reinterpret_cast<uint32_t*>(s->queue.as<unsigned char>() + (size_t)cap * QR_NR * sizeof(R)), cap,
-                   pc + 1};
-    // all slots PS_FRESH, slot counts zero (the state and count words are adjacent arrays)
-    SG_HIP(hipMemset(sl.u + (size_t)SU_STATE * P, 0, (size_t)P * (SU_NU - SU_STATE) * sizeof(uint32_t)));
+    const int G = P >= 2 ? 2 : 1;
+    const int64_t Pg[2] = {G == 2 ? (P + 1) / 2 : P, G == 2 ? P / 2 : 0};
+    Slots<R> sl[2];
+    WalkQueue<R> q[2];
+    unsigned long long* pc[2];   // per group: [1] queue tail, [3] requests claimed
+    for (int g = 0; g < G; g++) {
+        const int64_t cap = Pg[g] * (a.ni + 1);
+        if (s->slots[g].bytes < slot_bytes<R>(Pg[g])) s->slots[g].alloc(slot_bytes<R>(Pg[g]));
+        if (s->queue[g].bytes < queue_bytes<R>(cap)) s->queue[g].alloc(queue_bytes<R>(cap));
+        if (s->prog[g].bytes < 4 * sizeof(unsigned long long)) s->prog[g].alloc(4 * sizeof(unsigned long long));
+        pc[g] = s->prog[g].as<unsigned long long>();
+        sl[g] = Slots<R>{s->slots[g].as<R>(),
+                         reinterpret_cast<uint32_t*>(s->slots[g].as<unsigned char>() + (size_t)Pg[g] * SR_NR * sizeof(R)),
+                         Pg[g]};
+        q[g] = WalkQueue<R>{s->queue[g].as<R>(),
+                            reinterpret_cast<uint32_t*>(s->queue[g].as<unsigned char>() + (size_t)cap * QR_NR * sizeof(R)),
+                            cap, pc[g] + 1};
+        // slot states PS_FRESH and slot counts zero (adjacent arrays)
+        SG_HIP(hipMemsetAsync(sl[g].u + (size_t)SU_STATE * Pg[g], 0, (size_t)Pg[g] * (SU_NU - SU_STATE) * sizeof(uint32_t),
+                              st[g]));
+    }
     const int bs = 256;
     const size_t shm = (size_t)nlds * sizeof(int32_t);
     int per = 0;
@@ -896,43 +924,84 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
     int ebs = 256;
     while (ebs > 64 && staged_bytes<R>(ebs * (a.ni + 1)) > 64 * 1024) ebs /= 2;
     const size_t eshm = staged_bytes<R>(ebs * (a.ni + 1));
-    const int eblocks = (int)((P + ebs - 1) / ebs);
-    const uint64_t batch = std::min<uint64_t>(count, (uint64_t)1 << 22);
+    const uint64_t batch = std::min<uint64_t>(count, kSplitEmit);
+    const size_t rsz = sizeof(R);
+    if (s->emitted.bytes < 7 * batch * rsz) s->emitted.alloc(7 * batch * rsz);
+    if (s->emitdraw.bytes < batch * sizeof(uint32_t)) s->emitdraw.alloc(batch * sizeof(uint32_t));
+    unsigned long long* claims = s->next.as<unsigned long long>();
+    hipEvent_t ev_start, ev_join, ev_end[2];
+    SG_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    SG_HIP(hipEventCreate(&ev_start));
+    for (int g = 0; g < 2; g++) SG_HIP(hipEventCreate(&ev_end[g]));
+    int walk_launches = 0;
     for (uint64_t b = 0; b < count; b += batch) {
         McArgs ab = a;
         ab.first = first + b;
         ab.count = std::min(batch, count - b);
         ab.emitted = s->emitted.p;
         ab.emit_draw = s->emitdraw.as<uint32_t>();
-        ab.next_packet = pc + 2;
-        SG_HIP(hipMemsetAsync(pc + 2, 0, sizeof(unsigned long long), 0));
+        ab.next_packet = claims;
+        // the previous batch is finished on both streams: emit on stream 0, then both groups start
+        if (G == 2) {
+            SG_HIP(hipEventRecord(ev_join, st[1]));
+            SG_HIP(hipStreamWaitEvent(st[0], ev_join, 0));
+        }
+        SG_HIP(hipMemsetAsync(claims, 0, sizeof(unsigned long long), st[0]));
         int emb = (int)std::min<uint64_t>((ab.count + 255) / 256, (uint64_t)g_cus * 16);
-        hipLaunchKernelGGL(k_emit<R>, dim3(emb), dim3(256), 0, 0, ab, s->emitted.as<R>(), s->emitdraw.as<uint32_t>());
+        hipLaunchKernelGGL(k_emit<R>, dim3(emb), dim3(256), 0, st[0], ab, s->emitted.as<R>(), s->emitdraw.as<uint32_t>());
         SG_HIP(hipGetLastError());
-        if (b > 0) hipLaunchKernelGGL(k_revive, dim3((P + 255) / 256), dim3(256), 0, 0, sl.u + (size_t)SU_STATE * P, P);
-        const bool last = b + batch >= count;
+        for (int g = 0; g < G && b > 0; g++)
+            hipLaunchKernelGGL(k_revive, dim3((Pg[g] + 255) / 256), dim3(256), 0, st[0], sl[g].u + (size_t)SU_STATE * Pg[g],
+                               Pg[g]);
+        if (b == 0) SG_HIP(hipEventRecord(ev_start, st[0]));
+        if (G == 2) {
+            SG_HIP(hipEventRecord(ev_join, st[0]));
+            SG_HIP(hipStreamWaitEvent(st[1], ev_join, 0));
+        }
         for (;;) {
-            SG_HIP(hipMemsetAsync(pc + 1, 0, sizeof(unsigned long long), 0));
-            hipLaunchKernelGGL(k_events<R>, dim3(eblocks), dim3(ebs), eshm, 0, ab, sl, q);
-            SG_HIP(hipGetLastError());
-            unsigned long long h[2];   // requests appended, packets claimed
-            SG_HIP(hipMemcpy(h, pc + 1, sizeof h, hipMemcpyDeviceToHost));
-            SG_REQUIRE(h[0] <= (unsigned long long)cap, "walk queue overflow");
-            if (h[0] > 0) {
-                SG_HIP(hipMemsetAsync(pc + 3, 0, sizeof(unsigned long long), 0));
-                lt.start();
-                hipLaunchKernelGGL((k_walks<W, ABS>), dim3(blocks), dim3(bs), shm, 0, v, ab, q, sl, pc + 3, nlds);
-                SG_HIP(hipGetLastError());
-                lt.stop();
+            for (int p = 0; p < kSplitCheck; p++)
+                for (int g = 0; g < G; g++) {
+                    SG_HIP(hipMemsetAsync(pc[g] + 1, 0, sizeof(unsigned long long), st[g]));
+                    hipLaunchKernelGGL(k_events<R>, dim3((Pg[g] + ebs - 1) / ebs), dim3(ebs), eshm, st[g], ab, sl[g], q[g]);
+                    SG_HIP(hipGetLastError());
+                    SG_HIP(hipMemsetAsync(pc[g] + 3, 0, sizeof(unsigned long long), st[g]));
+                    hipLaunchKernelGGL((k_walks<W, ABS>), dim3(blocks), dim3(bs), shm, st[g], v, ab, q[g], sl[g], pc[g] + 3,
+                                       nlds);
+                    SG_HIP(hipGetLastError());
+                    walk_launches++;
+                }
+            // progress after the last pass of each group: packets claimed, walks each group queued
+            if (G == 2) {
+                SG_HIP(hipEventRecord(ev_join, st[1]));
+                SG_HIP(hipStreamWaitEvent(st[0], ev_join, 0));
             }
-            // all packets of the batch claimed: go on to the next batch, or, after the last one, stop
-            // once a pass has queued no walk (no packet is waiting for one)
-            if (h[1] >= ab.count && (!last || h[0] == 0)) break;
+            SG_HIP(hipMemcpyAsync(g_split_host, claims, sizeof(unsigned long long), hipMemcpyDeviceToHost, st[0]));
+            for (int g = 0; g < G; g++)
+                SG_HIP(hipMemcpyAsync(g_split_host + 1 + g, pc[g] + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                      st[0]));
+            SG_HIP(hipStreamSynchronize(st[0]));
+            bool queued = false;
+            for (int g = 0; g < G; g++) {
+                SG_REQUIRE(g_split_host[1 + g] <= (unsigned long long)(Pg[g] * (a.ni + 1)), "walk queue overflow");
+                queued = queued || g_split_host[1 + g] > 0;
+            }
+            if (g_split_host[0] >= ab.count && !queued) break;
         }
     }
-    hipLaunchKernelGGL(k_slot_counts, dim3(std::min<int64_t>((P + 255) / 256, 1024)), dim3(256), 0, 0, sl.u, P,
-                       a.counters);
+    for (int g = 0; g < G; g++)
+        hipLaunchKernelGGL(k_slot_counts, dim3(std::min<int64_t>((Pg[g] + 255) / 256, 1024)), dim3(256), 0, st[0], sl[g].u,
+                           Pg[g], a.counters);
     SG_HIP(hipGetLastError());
+    for (int g = 0; g < G; g++) SG_HIP(hipEventRecord(ev_end[g], st[g]));
+    SG_HIP(hipStreamSynchronize(st[0]));
+    if (G == 2) SG_HIP(hipStreamSynchronize(st[1]));
+    float ms = 0, ms1 = 0;
+    SG_HIP(hipEventElapsedTime(&ms, ev_start, ev_end[0]));
+    if (G == 2) SG_HIP(hipEventElapsedTime(&ms1, ev_start, ev_end[1]));
+    lt.add(std::max(ms, ms1), walk_launches);
+    (void)hipEventDestroy(ev_start);
+    (void)hipEventDestroy(ev_join);
+    for (int g = 0; g < 2; g++) (void)hipEventDestroy(ev_end[g]);
 }
 
 template <bool ABS>
@@ -1019,7 +1088,7 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
         counters[10] = (uint64_t)(ms * 1000.0);
         counters[11] = blocks;
         counters[12] = (uint64_t)(lt.total_ms() * 1000.0);
-        counters[13] = lt.ev.size();
+        counters[13] = lt.launches();
     }
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH
@@ -1192,7 +1261,7 @@ int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const u
         counters[10] = (uint64_t)(ms * 1000.0);
         counters[11] = blocks;
         counters[12] = (uint64_t)(lt.total_ms() * 1000.0);
-        counters[13] = lt.ev.size();
+        counters[13] = lt.launches();
     }
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH

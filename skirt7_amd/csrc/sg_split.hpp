@@ -23,8 +23,10 @@ namespace sg {
 enum SlotR { SR_RX = 0, SR_RY, SR_RZ, SR_KX, SR_KY, SR_KZ, SR_L, SR_RES, SR_NR };
 // The last four slot words count the slot's packets, scatterings, detected peel-offs and started
 // walks over a run; they are summed once at the end (no per-pass atomics).
-enum SlotU { SU_PLO = 0, SU_PHI, SU_DRAW, SU_W0, SU_W1, SU_W2, SU_W3, SU_NSCATT, SU_STATE, SU_CPK, SU_CSCAT, SU_CPEEL,
-             SU_CWALK, SU_NU };
+// SU_HNODE..SU_HK: the leaf the packet's position lies in, as the last propagation walk left it
+// (node -1: unknown), handed to the walks that start there.
+enum SlotU { SU_PLO = 0, SU_PHI, SU_DRAW, SU_W0, SU_W1, SU_W2, SU_W3, SU_NSCATT, SU_HNODE, SU_HI, SU_HJ, SU_HK,
+             SU_STATE, SU_CPK, SU_CSCAT, SU_CPEEL, SU_CWALK, SU_NU };
 enum PacketState : uint32_t { PS_FRESH = 0, PS_OWN, PS_PROP, PS_DEAD };
 enum WalkKind : uint32_t { WK_PEEL = 0, WK_OWN = 1, WK_PROP = 2 };
 
@@ -43,10 +45,15 @@ template <typename R> __host__ __device__ constexpr size_t slot_bytes(int64_t n)
 
 // Walk requests: origin (3), direction (3, unused for peel-offs: the instrument's direction is
 // taken from its record) and a value (peel-off luminosity, own-path luminosity, or propagation
-// target optical depth) in the run's real type; a tag (kind | instrument << 2) and an index
-// (packet slot, or pixel for a peel-off).
+// target optical depth) in the run's real type; a tag (kind | instrument << 2), an index (packet
+// slot, or pixel for a peel-off) and the start leaf hint (node, indices) of the walker.
 enum QueueR { QR_OX = 0, QR_OY, QR_OZ, QR_DX, QR_DY, QR_DZ, QR_VAL, QR_NR };
-enum QueueU { QU_TAG = 0, QU_IDX, QU_NU };
+enum QueueU { QU_TAG = 0, QU_IDX, QU_HN, QU_HI, QU_HJ, QU_HK, QU_NU };
+
+// Start-leaf hint of a walk.
+struct LeafHint {
+    int32_t n, i, j, k;
+};
 template <typename R> struct WalkQueue {
     R* r;
     uint32_t* u;
@@ -83,7 +90,7 @@ template <typename R> struct StagedQueue {
     int cap;
     unsigned* cnt;
     __device__ __forceinline__ void push(uint32_t kind, uint32_t inst, uint32_t idx, R ox, R oy, R oz, R dx, R dy,
-                                         R dz, R val) const
+                                         R dz, R val, const LeafHint& hint) const
     {
         uint64_t mask = __ballot(1);
         int lane = __lane_id();
@@ -98,6 +105,8 @@ template <typename R> struct StagedQueue {
         r[QR_VAL * cap + i] = val;
         u[QU_TAG * cap + i] = kind | (inst << 2);
         u[QU_IDX * cap + i] = idx;
+        u[QU_HN * cap + i] = (uint32_t)hint.n; u[QU_HI * cap + i] = (uint32_t)hint.i;
+        u[QU_HJ * cap + i] = (uint32_t)hint.j; u[QU_HK * cap + i] = (uint32_t)hint.k;
     }
 };
 template <typename R> __host__ __device__ constexpr size_t staged_bytes(int cap)
@@ -110,7 +119,8 @@ template <typename R> __host__ __device__ constexpr size_t staged_bytes(int cap)
 // walk request or, without a dust system, the detection itself.
 template <typename R, class Q>
 __device__ __forceinline__ uint32_t peel_offs(const McArgs& a, const InstrDev* instr, const Q& q,
-                                              bool scattering, R rx, R ry, R rz, R kx, R ky, R kz, R L)
+                                              bool scattering, R rx, R ry, R rz, R kx, R ky, R kz, R L,
+                                              const LeafHint& hint)
 {
     const R gg = (R)a.g;
     uint32_t n = 0;
@@ -127,7 +137,7 @@ __device__ __forceinline__ uint32_t peel_offs(const McArgs& a, const InstrDev* i
         }
         n++;
         if (!a.has_dust) unsafeAtomicAdd(in.frame + (size_t)a.ell * in.Nx * in.Ny + pix, (double)Lp);
-        else q.push(WK_PEEL, (uint32_t)i, (uint32_t)pix, rx, ry, rz, (R)0, (R)0, (R)0, Lp);
+        else q.push(WK_PEEL, (uint32_t)i, (uint32_t)pix, rx, ry, rz, (R)0, (R)0, (R)0, Lp, hint);
     }
     return n;
 }
@@ -189,6 +199,8 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
         rng.plo = sl.fu(SU_PLO, i); rng.phi = sl.fu(SU_PHI, i); rng.draw = sl.fu(SU_DRAW, i);
         rng.w0 = sl.fu(SU_W0, i); rng.w1 = sl.fu(SU_W1, i); rng.w2 = sl.fu(SU_W2, i); rng.w3 = sl.fu(SU_W3, i);
         int nscatt = (int)sl.fu(SU_NSCATT, i);
+        LeafHint hint{(int32_t)sl.fu(SU_HNODE, i), (int32_t)sl.fu(SU_HI, i), (int32_t)sl.fu(SU_HJ, i),
+                      (int32_t)sl.fu(SU_HK, i)};
         // at most a few transitions per pass: a finished packet is replaced by a new one at once; a slot
         // without a dust system detects its peel-offs directly and takes up to four packets per pass
         for (int step = 0; step < 4; step++) {
@@ -211,11 +223,12 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
                     philox10(rng.w0, rng.w1, rng.w2, rng.w3, rng.k0, rng.k1);
                 }
                 nscatt = 0;
-                uint32_t np = peel_offs<R>(a, s_instr, sq, false, rx, ry, rz, kx, ky, kz, L);
+                hint.n = -1;   // a new position: the leaf is not known yet
+                uint32_t np = peel_offs<R>(a, s_instr, sq, false, rx, ry, rz, kx, ky, kz, L, hint);
                 c_npeel += np;
                 if (!a.has_dust) continue;
                 c_walks += np + 1;
-                sq.push(WK_OWN, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, L);
+                sq.push(WK_OWN, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, L, hint);
                 state = PS_OWN;
                 break;
             }
@@ -243,7 +256,7 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
                         R qq = ((R)1.0 - (R)a.xi) * pp + (R)a.xi / taupath;
                         L = L * (pp / qq);
                     }
-                    sq.push(WK_PROP, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, tt);
+                    sq.push(WK_PROP, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, tt, hint);
                     state = PS_PROP;
                     break;
                 }
@@ -255,13 +268,13 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
                 // PhotonPackage::propagate (PhotonPackage.cpp:93-96) by pathlength(tau)
                 R d = sl.fr(SR_RES, i);
                 rx = rx + d * kx; ry = ry + d * ky; rz = rz + d * kz;
-                uint32_t np = peel_offs<R>(a, s_instr, sq, true, rx, ry, rz, kx, ky, kz, L);
+                uint32_t np = peel_offs<R>(a, s_instr, sq, true, rx, ry, rz, kx, ky, kz, L, hint);
                 c_npeel += np;
                 hg_scatter<R>(rng, gg, kx, ky, kz);
                 nscatt++;
                 c_scat++;
                 c_walks += np + 1;
-                sq.push(WK_OWN, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, L);
+                sq.push(WK_OWN, 0, (uint32_t)i, rx, ry, rz, kx, ky, kz, L, hint);
                 state = PS_OWN;
                 break;
             }
@@ -272,6 +285,7 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
         sl.fu(SU_PLO, i) = rng.plo; sl.fu(SU_PHI, i) = rng.phi; sl.fu(SU_DRAW, i) = rng.draw;
         sl.fu(SU_W0, i) = rng.w0; sl.fu(SU_W1, i) = rng.w1; sl.fu(SU_W2, i) = rng.w2; sl.fu(SU_W3, i) = rng.w3;
         sl.fu(SU_NSCATT, i) = (uint32_t)nscatt;
+        sl.fu(SU_HNODE, i) = (uint32_t)hint.n;
         sl.fu(SU_CPK, i) += c_pk; sl.fu(SU_CSCAT, i) += c_scat; sl.fu(SU_CPEEL, i) += c_npeel;
         sl.fu(SU_CWALK, i) += c_walks;
     }
@@ -387,6 +401,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_WAL
             h.stage = ST_NEWPKT;
         } else if (h.stage == ST_PROP_DONE) {
             sl.fr(SR_RES, idx) = h.s;
+            if constexpr (W::kHint) {
+                // the leaf holding the interaction point: where the packet's next walks start
+                int n;
+                int32_t i, j, k;
+                h.w.hint(n, i, j, k);
+                sl.fu(SU_HNODE, idx) = (uint32_t)n;
+                sl.fu(SU_HI, idx) = (uint32_t)i; sl.fu(SU_HJ, idx) = (uint32_t)j; sl.fu(SU_HK, idx) = (uint32_t)k;
+            }
             h.stage = ST_NEWPKT;
         }
         const bool want = h.stage == ST_NEWPKT;
@@ -402,13 +424,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_WAL
                 R ox = q.fr(QR_OX, r), oy = q.fr(QR_OY, r), oz = q.fr(QR_OZ, r);
                 h.L0 = q.fr(QR_VAL, r);
                 h.tau = 0;
+                double dx, dy, dz;
                 if (kind == WK_PEEL) {
                     const InstrDev& in = s_instr[inst];
-                    h.w.init(g, s_nodes, nlds, ox, oy, oz, in.kobs[0], in.kobs[1], in.kobs[2]);
+                    dx = in.kobs[0]; dy = in.kobs[1]; dz = in.kobs[2];
+                } else {
+                    dx = q.fr(QR_DX, r); dy = q.fr(QR_DY, r); dz = q.fr(QR_DZ, r);
+                }
+                if constexpr (W::kHint)
+                    h.w.init_hinted(g, s_nodes, nlds, ox, oy, oz, dx, dy, dz, (int)q.fu(QU_HN, r), (int32_t)q.fu(QU_HI, r),
+                                    (int32_t)q.fu(QU_HJ, r), (int32_t)q.fu(QU_HK, r));
+                else
+                    h.w.init(g, s_nodes, nlds, ox, oy, oz, dx, dy, dz);
+                if (kind == WK_PEEL) {
                     h.stage = ST_PEELW;
                 } else {
-                    R dx = q.fr(QR_DX, r), dy = q.fr(QR_DY, r), dz = q.fr(QR_DZ, r);
-                    h.w.init(g, s_nodes, nlds, ox, oy, oz, dx, dy, dz);
                     if (kind == WK_OWN) h.stage = ST_OWN;
                     else {
                         h.target = h.L0;

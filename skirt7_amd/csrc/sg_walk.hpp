@@ -79,6 +79,7 @@ struct CartW64 {
     uint32_t nrecompute, nstuck;
     static constexpr int kParkWords = 25;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
+    static constexpr bool kHint = false;
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
@@ -218,6 +219,7 @@ struct CartW32 {
     uint32_t nrecompute, nstuck;
     static constexpr int kParkWords = 19;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
+    static constexpr bool kHint = false;
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz); p.io(ix); p.io(iy); p.io(iz);
@@ -429,6 +431,7 @@ struct TreeW64 {
     uint32_t steps, nrecompute, nstuck;
     static constexpr int kParkWords = 38;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
+    static constexpr bool kHint = false;
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
@@ -533,17 +536,20 @@ struct TreeW32 {
     float rho;
     int32_t rope[6];
     int m;                   // node id of the current leaf
+    int lm;                  // leaf of the last segment returned, and its indices
+    int32_t li, lj, lk;
     int ne;
     float e0, e1, e2;
     bool done;
     uint32_t steps, nrecompute, nstuck;
-    static constexpr int kParkWords = 29;
+    static constexpr int kParkWords = 33;
+    static constexpr bool kHint = true;
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz); p.io(ix); p.io(iy); p.io(iz);
         p.io(ci); p.io(cj); p.io(ck); p.io(nsub);
         p.io(rho); p.io(rope[0]); p.io(rope[1]); p.io(rope[2]); p.io(rope[3]); p.io(rope[4]); p.io(rope[5]);
-        p.io(m); p.io(ne); p.io(e0); p.io(e1); p.io(e2);
+        p.io(m); p.io(lm); p.io(li); p.io(lj); p.io(lk); p.io(ne); p.io(e0); p.io(e1); p.io(e2);
         p.io(done); p.io(steps); p.io(nrecompute); p.io(nstuck);
     }
     // cell number of a walk segment's node
@@ -659,22 +665,67 @@ struct TreeW32 {
             ne++;
         }
     }
+    __device__ __forceinline__ void set_direction(double dx, double dy, double dz)
+    {
+        kx = (float)dx; ky = (float)dy; kz = (float)dz;
+        ix = fabsf(kx) > 1e-15f ? 1.0f / kx : 0.0f;
+        iy = fabsf(ky) > 1e-15f ? 1.0f / ky : 0.0f;
+        iz = fabsf(kz) > 1e-15f ? 1.0f / kz : 0.0f;
+    }
     __device__ __forceinline__ void init(const View& t, const int32_t* lds, int nlds, double rx, double ry, double rz,
                                          double dx, double dy, double dz)
     {
-        steps = 0; done = false;
+        steps = 0; done = false; m = -1; lm = -1;
         Entry en;
         if (!en.move_inside(t.ext, t.eps, rx, ry, rz, dx, dy, dz)) { ne = 0; done = true; return; }
         ne = 0;
         for (int q = 0; q < en.ne; q++) push((float)(q == 0 ? en.e0 : q == 1 ? en.e1 : en.e2));
         x = (float)rx; y = (float)ry; z = (float)rz;
-        kx = (float)dx; ky = (float)dy; kz = (float)dz;
-        ix = fabsf(kx) > 1e-15f ? 1.0f / kx : 0.0f;
-        iy = fabsf(ky) > 1e-15f ? 1.0f / ky : 0.0f;
-        iz = fabsf(kz) > 1e-15f ? 1.0f / kz : 0.0f;
+        set_direction(dx, dy, dz);
         int32_t i, j, k;
         int n = descend_root(t, lds, nlds, -1, 0, i, j, k);
         if (n < 0) { ne = 0; done = true; return; }
+        descend_from(t, n, i, j, k);
+    }
+    // The leaf of the last segment returned (-1 before the first one): after a walk that stopped
+    // inside a segment, the leaf holding the stopping point.
+    __device__ __forceinline__ void hint(int& n, int32_t& i, int32_t& j, int32_t& k) const
+    {
+        n = lm; i = li; j = lj; k = lk;
+    }
+    // init for a start point strictly inside the grid with a candidate leaf (node hn, indices hi, hj,
+    // hk): the leaf is taken when it contains the point in the half-open sense lo <= p < hi (closed at
+    // the grid's upper walls), which is exactly the leaf the root descent selects; otherwise init.
+    __device__ __forceinline__ void init_hinted(const View& t, const int32_t* lds, int nlds, double rx, double ry,
+                                                double rz, double dx, double dy, double dz, int hn, int32_t hi,
+                                                int32_t hj, int32_t hk)
+    {
+        const double* e = t.ext;
+        const bool inside = rx > e[0] && rx < e[3] && ry > e[1] && ry < e[4] && rz > e[2] && rz < e[5];
+        if (!(inside && hn >= 0 && hn < t.nnodes)) { init(t, lds, nlds, rx, ry, rz, dx, dy, dz); return; }
+        steps = 0; done = false; m = -1; lm = -1; ne = 0;
+        x = (float)rx; y = (float)ry; z = (float)rz;
+        set_direction(dx, dy, dz);
+        uint4 a, b;
+        load(t, hn, a, b);
+        bool ok = (a.x & (1u << 15)) != 0;
+        if (ok) {
+            const uint32_t ns = a.x & 0x7fffu;
+            const float p[3] = {x, y, z};
+            const int32_t kk[3] = {hi, hj, hk};
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int n = (int)((ns >> (5 * q)) & 31u);
+                const float w = ldexpf(t.widthf[q], -n);
+                const float lo = t.extf[q] + (float)kk[q] * w, up = t.extf[q] + (float)(kk[q] + 1) * w;
+                const bool top = kk[q] + 1 == (1 << n);
+                ok = ok && p[q] >= lo && (p[q] < up || (top && p[q] <= up));
+            }
+        }
+        if (ok) { descend_loaded(t, a, b, hn, hi, hj, hk); return; }
+        int32_t i, j, k;
+        int n = descend_root(t, lds, nlds, -1, 0, i, j, k);
+        if (n < 0) { done = true; return; }
         descend_from(t, n, i, j, k);
     }
     // Chooses the exit wall and advances onto it; returns the wall (0..5) and the segment length.
@@ -736,6 +787,11 @@ struct TreeW32 {
             else descend_from(t, n, i, j, k);
         }
     }
+    __device__ __forceinline__ int rope_of(int wall) const
+    {
+        return wall == 0 ? rope[0] : wall == 1 ? rope[1] : wall == 2 ? rope[2] : wall == 3 ? rope[3]
+             : wall == 4 ? rope[4] : rope[5];
+    }
     __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds, float& rh)
     {
         if (ne) {
@@ -744,19 +800,22 @@ struct TreeW32 {
         }
         while (!done) {
             float d;
-            int segm = m;
-            float segrho = rho;
-            int wall = exit_wall(t, d);
-            int r = wall == 0 ? rope[0] : wall == 1 ? rope[1] : wall == 2 ? rope[2] : wall == 3 ? rope[3]
-                  : wall == 4 ? rope[4] : rope[5];
+            const int segm = m;
+            const float segrho = rho;
+            const int32_t si = ci, sj = cj, sk = ck;
+            const int wall = exit_wall(t, d);
+            const int r = rope_of(wall);
             if (r < 0) done = true;   // a wall of the grid: the path leaves the grid
             else {
-                int oldm = m;
                 step_through(t, lds, nlds, wall, r);
-                if (!done && m == oldm) { nstuck++; done = true; }
+                if (!done && m == segm) { nstuck++; done = true; }
             }
             if (++steps > SG_MAX_STEPS) done = true;
-            if (d > 0) { mo = segm; ds = d; rh = segrho; return true; }
+            if (d > 0) {
+                mo = segm; ds = d; rh = segrho;
+                lm = segm; li = si; lj = sj; lk = sk;
+                return true;
+            }
         }
         return false;
     }
@@ -773,19 +832,23 @@ struct TreeW32TopDown : TreeW32 {
         }
         while (!done) {
             float d;
-            int segm = m;
-            float segrho = rho;
-            int wall = exit_wall(t, d);
-            int oldm = m;
+            const int segm = m;
+            const float segrho = rho;
+            const int32_t si = ci, sj = cj, sk = ck;
+            const int wall = exit_wall(t, d);
             int32_t i, j, k;
-            int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1, i, j, k);
+            const int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1, i, j, k);
             if (n < 0) done = true;
             else {
                 descend_from(t, n, i, j, k);
-                if (m == oldm) { nstuck++; done = true; }
+                if (m == segm) { nstuck++; done = true; }
             }
             if (++steps > SG_MAX_STEPS) done = true;
-            if (d > 0) { mo = segm; ds = d; rh = segrho; return true; }
+            if (d > 0) {
+                mo = segm; ds = d; rh = segrho;
+                lm = segm; li = si; lj = sj; lk = sk;
+                return true;
+            }
         }
         return false;
     }
@@ -839,6 +902,7 @@ template <typename R> struct VoroW {
     uint32_t steps, nrecompute, nstuck;
     static constexpr int kParkWords = 24;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
+    static constexpr bool kHint = false;
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
