@@ -378,7 +378,7 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
                 peel++;
                 break;
             }
-            pixel = pix; Lpeel = Lp; h.tau = 0;
+            pixel = pix; Lpeel = Lp; h.tau = 0; h.s = 0; h.nseg = 0;
             h.w.init(g, lds, nlds, rx, ry, rz, in.kobs[0], in.kobs[1], in.kobs[2]);
             c_walks++;
             h.stage = ST_PEELW;
@@ -394,7 +394,7 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
             break;
         }
         case ST_OWN_START: {
-            h.tau = 0; h.L0 = L;
+            h.tau = 0; h.s = 0; h.nseg = 0; h.L0 = L;
             h.w.init(g, lds, nlds, rx, ry, rz, kx, ky, kz);
             c_walks++;
             h.stage = ST_OWN;
@@ -524,33 +524,27 @@ __device__ __forceinline__ int crossing(Hot<W>& h, const typename W::View& g, co
     }
     R kr = (R)a.kext * rho;        // KappaRho (DustSystem.cpp:459-485): 0 + kappaext*rho, 0 outside
     R dtau = kr * ds;
-    int kind = 0;
-    if (h.stage == ST_PEELW) {
-        h.tau += dtau;
-        kind = 2;
-        if (a.tau_cut > 0.0 && h.tau > (R)a.tau_cut) h.stage = ST_PEELW_DONE;
-    } else if (h.stage == ST_OWN) {
-        if (ABS && m >= 0) {
-            // OligoDustSystem::absorb via simulateescapeandabsorption (MonteCarloSimulation.cpp:445-458)
-            R expfactorm = -rexpm1<R>(-dtau);
-            R Lintm = h.L0 * rexp<R>(-h.tau) * expfactorm;
-            R Labsm = ((R)1.0 - (R)a.albedo) * Lintm;
-            unsafeAtomicAdd(a.labs + (size_t)a.ell * a.ncells + W::cell_of(g, m), (double)Labsm);
-        }
-        h.tau += dtau;
-        kind = 1;
-    } else {
-        // ST_PROP: cumulative s and tau per segment; stop at the first segment past the target
-        h.tprev = h.tau; h.sprev = h.s;
-        h.s += ds;
-        h.tau += dtau;
-        h.nseg++;
-        kind = 3;
-        if (h.target > (R)0 && h.tau > h.target) {
-            h.s = (h.nseg == 1) ? interp_linlin(h.target, (R)0, h.tau, (R)0, h.s)
-                                : interp_linlin(h.target, h.tprev, h.tau, h.sprev, h.s);
-            h.stage = ST_PROP_DONE;
-        }
+    const int st = h.stage;
+    if (ABS && st == ST_OWN && m >= 0) {
+        // OligoDustSystem::absorb via simulateescapeandabsorption (MonteCarloSimulation.cpp:445-458)
+        R expfactorm = -rexpm1<R>(-dtau);
+        R Lintm = h.L0 * rexp<R>(-h.tau) * expfactorm;
+        R Labsm = ((R)1.0 - (R)a.albedo) * Lintm;
+        unsafeAtomicAdd(a.labs + (size_t)a.ell * a.ncells + W::cell_of(g, m), (double)Labsm);
+    }
+    // running sums, kept for every walk kind without branching (only the propagation re-walk reads
+    // the path length and the previous segment's sums): cumulative s and tau per segment
+    h.tprev = h.tau; h.sprev = h.s;
+    h.s += ds;
+    h.tau += dtau;
+    h.nseg++;
+    const int kind = st == ST_PEELW ? 2 : st == ST_OWN ? 1 : 3;
+    if (st == ST_PEELW && a.tau_cut > 0.0 && h.tau > (R)a.tau_cut) h.stage = ST_PEELW_DONE;
+    if (st == ST_PROP && h.target > (R)0 && h.tau > h.target) {
+        // the propagation re-walk stops at the first segment past the target optical depth
+        h.s = (h.nseg == 1) ? interp_linlin(h.target, (R)0, h.tau, (R)0, h.s)
+                            : interp_linlin(h.target, h.tprev, h.tau, h.sprev, h.s);
+        h.stage = ST_PROP_DONE;
     }
     return m >= 0 ? kind : 0;
 }

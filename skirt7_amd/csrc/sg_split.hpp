@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_WAL
                 inst = tag >> 2;
                 R ox = q.fr(QR_OX, r), oy = q.fr(QR_OY, r), oz = q.fr(QR_OZ, r);
                 h.L0 = q.fr(QR_VAL, r);
-                h.tau = 0;
+                h.tau = 0; h.s = 0; h.tprev = 0; h.sprev = 0; h.nseg = 0;
                 double dx, dy, dz;
                 if (kind == WK_PEEL) {
                     const InstrDev& in = s_instr[inst];
@@ -442,7 +442,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_WAL
                     if (kind == WK_OWN) h.stage = ST_OWN;
                     else {
                         h.target = h.L0;
-                        h.s = 0; h.tprev = 0; h.sprev = 0; h.nseg = 0;
                         h.stage = ST_PROP;
                     }
                 }
