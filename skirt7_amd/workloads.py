@@ -113,3 +113,51 @@ def target_image(frame, counts, seed=4357):
     gain = counts / total if total > 0 else 1.0
     rng = np.random.default_rng(seed)
     return rng.poisson(np.clip(frame, 0.0, None) * gain).astype(np.float64), gain
+
+
+def sample_sites(geometry, n, extent=EXTENT, seed=4357):
+    """n positions drawn from a (spiral-decorated) exponential-disc density inside the box: radius from the
+    disc's R exp(-R/hR) law, height from exp(-|z|/hz), azimuth uniform, then the spiral perturbation by
+    rejection against its maximum (the same density the grid samples)."""
+    rng = np.random.default_rng(seed)
+    base = geometry.geometry if isinstance(geometry, S.SpiralStructureGeometryDecorator) else geometry
+    hR, hz = base.hR, base.hz
+    out = np.zeros((0, 3))
+    while len(out) < n:
+        m = 2 * (n - len(out)) + 1024
+        R = rng.gamma(2.0, hR, m)
+        z = rng.laplace(0.0, hz, m)
+        phi = rng.uniform(0.0, 2.0 * math.pi, m)
+        p = np.stack([R * np.cos(phi), R * np.sin(phi), z], 1)
+        inside = np.all((p > extent[:3]) & (p < extent[3:]), axis=1)
+        p = p[inside]
+        if isinstance(geometry, S.SpiralStructureGeometryDecorator):
+            w = geometry.perturbWeight
+            N = geometry.index
+            cn = math.sqrt(math.pi) * math.exp(math.lgamma(N + 1.0) - math.lgamma(N + 0.5))
+            ratio = S.geometry_density(geometry, p) / np.maximum(S.geometry_density(base, p), 1e-300)
+            p = p[rng.uniform(0.0, (1.0 - w) + w * cn, len(p)) < ratio]
+        out = np.concatenate([out, p])
+    return out[:n]
+
+
+def config2(nsites=1_000_000, nlambda=50, pixels=256, packages=2e7, seed=4357, tau_v=1.0, store_abs=True):
+    """Voronoi: nsites generator sites drawn from the config1 dust density, each cell with the analytic
+    density at its site; nlambda wavelengths log-spaced over 0.1-10 micron with kappa(lambda) =
+    kappa_V (lambda/0.55)^-1.3 normalised to a face-on tau_V, albedo 0.7 -> 0.4 and g 0.7 -> 0.3 linear in
+    the wavelength index; the absorbed luminosity per cell and wavelength is stored. Stars as config1
+    with a flat spectrum; two frames (0 and 75 deg) of pixels^2."""
+    comps, dgeom, _, _ = _galaxy(True, 1.0)
+    lam = np.logspace(-1.0, 1.0, nlambda)
+    kappa = (lam / 0.55) ** -1.3
+    idx = np.arange(nlambda) / max(1, nlambda - 1)
+    mix = S.SimpleOligoDustMix(kappa, 0.7 - 0.3 * idx, 0.7 - 0.4 * idx)
+    scale = S.faceon_normalization(dgeom, tau_v, 1.0)
+    sites = sample_sites(dgeom, nsites, EXTENT, seed)
+    grid = S.VoronoiDustGrid(EXTENT, sites)
+    rho = scale * S.geometry_density(dgeom, sites)
+    comps = [S.OligoStellarComp(c.geometry, [c.luminosities[0]] * nlambda) for c in comps]
+    ins = [S.FrameInstrument(DISTANCE, math.radians(a), 0.0, 0.0, pixels, pixels, 40.0, 40.0) for a in (0.0, 75.0)]
+    d = dict(name="config2", kind="voronoi", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
+             instr=S.InstrumentSystem(ins, nlambda), store_abs=store_abs, wavelengths=lam)
+    return _finish(d, packages, seed)

@@ -56,3 +56,35 @@ def voronoi_superset(sites):
         ids.extend(lst)
         off.append(len(ids))
     return np.array(off, np.int64), np.array(ids, np.int32)
+
+
+def point_source_box(tau0=1.0, albedo=0.6, packages=100000, seed=4357):
+    """A 3 x 3 x 3 Cartesian grid over [-1, 1]^3 with uniform dust of optical depth tau0 from the centre to a face
+    centre, a near-point stellar source at the centre, and the Monte Carlo parameters set so that every
+    packet makes exactly one own-path walk (minWeightReduction 1: the first scattering ends the packet).
+    The absorbed luminosity is then (1 - albedo) L (1 - exp(-tau0 / max|k_i|)) per packet of direction k."""
+    import math
+    from skirt7_amd import skirt as S
+    ext = np.array([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0])
+    grid = S.CartesianDustGrid.uniform(ext, 3, 3, 3)
+    mix = S.SimpleOligoDustMix([1.0], [albedo], [0.0])
+    rho = np.full(27, tau0)
+    comps = [S.OligoStellarComp(S.SersicGeometry(1.0, 1e-6), [1.0])]
+    ins = S.InstrumentSystem([S.FrameInstrument(1e4, 0.0, 0.0, 0.0, 8, 8, 2.0, 2.0)], 1)
+    d = dict(name="pointbox", kind="cartesian", grid=grid, rho=rho, mix=mix, comps=comps, instr=ins, store_abs=True)
+    d["stars"] = S.StellarSystem(comps, emissionBias=0.5)
+    d["dust"] = S.DustSystem(grid, rho, mix, writeMeanIntensity=True)
+    d["sim"] = S.OligoMonteCarloSimulation(d["dust"], d["stars"], ins, packages=packages, minWeightReduction=1.0,
+                                           minScattEvents=0, scattBias=0.5, seed=seed)
+    return d
+
+
+def absorbed_fraction_quadrature(tau0, n=2_000_000):
+    """E over isotropic directions of 1 - exp(-tau0 / max|k_i|) (Fibonacci sphere quadrature)."""
+    import math
+    i = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * i / n
+    r = np.sqrt(1.0 - z * z)
+    phi = math.pi * (3.0 - math.sqrt(5.0)) * i
+    k = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
+    return float(np.mean(1.0 - np.exp(-tau0 / np.max(np.abs(k), axis=1))))

@@ -294,3 +294,33 @@ def test_voronoi_neighbour_builder_gives_exact_paths():
     inner = np.all(np.abs(sites) < 12, axis=1)
     counts = np.diff(off)[inner]
     assert 14.5 < counts.mean() < 16.5
+
+
+def test_absorption_closed_form_point_source_box():
+    """simulateescapeandabsorption (MonteCarloSimulation.cpp:432-461): with one own-path walk per packet
+    (minWeightReduction 1: the first scattering ends the packet) the absorbed luminosity is
+    (1 - albedo) L (1 - exp(-tau(k))), tau(k) = tau0 / max|k_i| from the centre of a uniform cube; its
+    mean over isotropic directions comes from an independent quadrature. The cube is split into 3 x 3 x 3
+    cells of equal density, so the per-cell terms (1 - albedo) L exp(-tau_before) (1 - exp(-dtau)) must
+    telescope to that total: a dropped attenuation factor or a wrong albedo factor fails."""
+    from tests.helpers import absorbed_fraction_quadrature
+    sc = O.Scene()
+    edges = np.linspace(-1.0, 1.0, 4)
+    sc.cartesian(np.array([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]), edges, edges, edges)
+    tau0, albedo = 1.0, 0.6
+    sc.dust(np.full(27, tau0), [1.0], [albedo], [0.0], store_abs=True)
+    ts, tM = np.zeros(101), np.zeros(101)
+    O.lib().or_sersic_table(1.0, O.dp(ts), O.dp(tM))
+    star = np.zeros(16)
+    star[0], star[1], star[2] = 2, 1.0, 1e-6          # a Sersic sphere of negligible size at the centre
+    sc.star(star, [1.0], ts, tM)
+    sc.instrument(np.array([1e4, 0.0, 0.0, 0.0, 8, 8, 2.0, 2.0, 0.0, 0.0]))
+    n = 100000
+
+    class Sim:
+        mc = np.array([n, 1.0, 0, 0.5, 4357, 0.5, 0, 0.0])
+    _, c = sc.run_packets({"sim": Sim()}, 0, 0, n, n)
+    got = float(np.sum(sc.labs))
+    expect = (1.0 - albedo) * absorbed_fraction_quadrature(tau0)
+    assert int(c[3]) == 0     # the stop test after the first own-path walk ends every packet
+    assert abs(got - expect) < 2e-3 * expect, (got, expect)
