@@ -26,8 +26,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
-# algorithmic bytes per cell crossing (octree leaf walk: 6 bounds + kappa*rho + one neighbour link)
-BYTES_PER_CROSSING = {"mixed": 6 * 4 + 4 + 4, "fp64": 6 * 8 + 8 + 4}
+# algorithmic bytes per cell crossing: tree leaf walk 6 bounds + kappa*rho + one link; Voronoi about 15.5
+# neighbours x (4-byte id + 24-byte FP64 site) + kappa*rho; Cartesian kappa*rho + one wall coordinate
+BYTES_PER_CROSSING = {"tree": {"mixed": 6 * 4 + 4 + 4, "fp64": 6 * 8 + 8 + 4},
+                      "voronoi": {"mixed": 438, "fp64": 442},
+                      "cartesian": {"mixed": 8, "fp64": 16}}
+BYTES_PER_ABSORPTION = 16   # FP64 read-modify-write of one (cell, wavelength) entry per own-path crossing
 BYTES_PER_PEELOFF = 16   # FP64 read-modify-write of one pixel
 # HBM bytes per cell crossing of the walk kernel, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by
 # the copy-kernel calibration) on this workload
@@ -41,6 +45,9 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="config1", choices=("config1", "config2", "config4"),
+                    help="config1 octree (the metric's workload), config2 Voronoi with the absorption table, "
+                         "config4 kd-tree")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--packets", type=float, default=1e7, help="packets per step")
@@ -72,18 +79,20 @@ def copy_calibration(mib):
     log("copy calibration: 5 copies of %d MiB (%d bytes read and %d written per copy)" % (mib, 4 * n, 4 * n))
 
 
-def frames(w):
+def frames(w, ell=0):
     ins = w["instr"]
-    return [ins.frame(i)[0].copy() for i in range(len(ins.instruments))]
+    return [ins.frame(i)[ell].copy() for i in range(len(ins.instruments))]
 
 
-def run_steps(sim, first, nsteps, per, npp, precision):
+def run_steps(sim, first, nsteps, per, npp, precision, ell=0):
     """Runs nsteps batches; returns the summed counters and the transport kernel's device seconds."""
     tot = None
     for s in range(nsteps):
-        c = sim.run_packets(0, first + s * per, per, npp, precision)
+        c = sim.run_packets(ell, first + s * per, per, npp, precision)
         tot = c if tot is None else {k: tot[k] + c[k] for k in c}
-    return tot, tot["transport_us"] * 1e-6
+        log("  step %d/%d (%s): %d packets, %.3f s on the device" % (
+            s + 1, nsteps, "mixed" if precision == 1 else "fp64", c["packets"], c["kernel_us"] * 1e-6))
+    return tot, (tot["transport_us"] * 1e-6 if tot else 0.0)
 
 
 def cpu_baseline(w, seconds):
@@ -131,19 +140,32 @@ def main():
     per = int(a.packets)
     total = per * (a.steps + a.warmup)
     t0 = time.perf_counter()
-    w = workloads.config1(tau=a.tau, maxlevel=a.maxlevel, pixels=a.pixels, packages=total, tau_cut=a.tau_cut)
+    ell = 0
+    if a.workload == "config1":
+        w = workloads.config1(tau=a.tau, maxlevel=a.maxlevel, pixels=a.pixels, packages=total, tau_cut=a.tau_cut)
+        desc = "config1 octree spiral galaxy, face-on tau %.3g" % a.tau
+    elif a.workload == "config2":
+        w = workloads.config2(pixels=min(a.pixels, 256), packages=total)
+        ell = int(np.argmin(np.abs(w["wavelengths"] - 0.55)))
+        desc = "config2 Voronoi 1e6 sites, 50 wavelengths (run at %.3g micron), absorption table stored" % (
+            w["wavelengths"][ell])
+    else:
+        w = workloads.config4(pixels=min(a.pixels * 2, 1024), packages=total)
+        desc = "config4 kd-tree (alternating binary tree, TopDown) over a log-normal field"
     ncell = w["grid"].numCells()
-    log("setup: octree with %d leaves built in %.1f s on %s" % (ncell, time.perf_counter() - t0, S.device_name()))
+    gkind = "voronoi" if w["kind"] == "voronoi" else "cartesian" if w["kind"] == "cartesian" else "tree"
+    log("setup: %s with %d cells built in %.1f s on %s" % (a.workload, ncell, time.perf_counter() - t0,
+                                                            S.device_name()))
     sim = w["sim"]
     npp = total
 
     # warmup (mixed), then the timed mixed steps
-    run_steps(sim, 0, a.warmup, per, npp, SG_MIXED)
+    run_steps(sim, 0, a.warmup, per, npp, SG_MIXED, ell)
     w["instr"].clear()
     t0 = time.perf_counter()
-    cm, kmix = run_steps(sim, a.warmup * per, a.steps, per, npp, SG_MIXED)
+    cm, kmix = run_steps(sim, a.warmup * per, a.steps, per, npp, SG_MIXED, ell)
     wall = time.perf_counter() - t0
-    img_mixed = frames(w)
+    img_mixed = frames(w, ell)
     pk = cm["packets"]
     cross = cm["own_crossings"] + cm["peel_crossings"] + cm["rewalk_crossings"]
     log("mixed: %d packets in %.3f s (transport kernel %.3f s in %d launches, emission + transport %.3f s): "
@@ -154,19 +176,21 @@ def main():
            "value": pk / wall, "unit": "packets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 (walk, weights) + f64 (wall fallback, tallies)", "data": "synthetic",
-           "config": {"workload": "config1 octree spiral galaxy, face-on tau %.3g, %d leaves, %d packets per step, "
-                                  "4x%d^2 frames" % (a.tau, ncell, per, a.pixels),
-                      "packets_per_step": per, "leaves": ncell, "tau": a.tau, "instruments": 4,
-                      "pixels": a.pixels}}
+           "config": {"workload": "%s, %d cells, %d packets per step, %d frames of %d^2" % (
+                          desc, ncell, per, len(w["instr"].instruments), w["instr"].instruments[0].Nx),
+                      "packets_per_step": per, "cells": ncell, "instruments": len(w["instr"].instruments),
+                      "pixels": w["instr"].instruments[0].Nx}}
     # algorithmic bytes of the transport kernel over its measured device time (all launches)
-    bytes_launch = cross * BYTES_PER_CROSSING["mixed"] + cm["peeloffs"] * BYTES_PER_PEELOFF
+    bytes_launch = cross * BYTES_PER_CROSSING[gkind]["mixed"] + cm["peeloffs"] * BYTES_PER_PEELOFF
+    if w.get("store_abs"):
+        bytes_launch += cm["own_crossings"] * BYTES_PER_ABSORPTION
     achieved = bytes_launch / kmix
     launches = max(1, cm["transport_launches"])
     traffic = None
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if a.pipeline == tr["pipeline"]:
+        if a.pipeline == tr["pipeline"] and a.workload == "config1":
             # measured HBM bytes per crossing times this run's crossings per launch
             traffic = (tr["fetch_bytes_per_crossing"] + tr["write_bytes_per_crossing"]) * cross / launches
     except (OSError, KeyError, ValueError):
@@ -187,18 +211,18 @@ def main():
             S.set_pipeline(p)
             w["instr"].clear()
             t0 = time.perf_counter()
-            c64 = sim.run_packets(0, first, per, per, SG_FP64)
+            c64 = sim.run_packets(ell, first, per, per, SG_FP64)
             dt = time.perf_counter() - t0
             fp64[p] = (c64["packets"] / dt, c64)
             if p == 1:
-                img64 = frames(w)
+                img64 = frames(w, ell)
         S.set_pipeline(a.pipeline)
         best = max(fp64, key=lambda p: fp64[p][0])
         fp64_rate, c64 = fp64[best]
         # mixed image of the same packets (identical streams)
         w["instr"].clear()
-        sim.run_packets(0, first, per, per, SG_MIXED)
-        img32 = frames(w)
+        sim.run_packets(ell, first, per, per, SG_MIXED)
+        img32 = frames(w, ell)
         errs = [float(np.sqrt(np.mean((x - y) ** 2)) / np.sqrt(np.mean(x ** 2))) for x, y in zip(img64, img32)]
         out["fp64"] = {"packets_per_s": fp64_rate, "pipeline": best,
                        "packets_per_s_by_pipeline": {str(p): fp64[p][0] for p in fp64},
@@ -213,8 +237,8 @@ def main():
             # FP64 reference with ten times the packets (fresh packets), normalised to its own count
             ref0 = (a.warmup + a.steps) * per
             w["instr"].clear()
-            sim.run_packets(0, ref0, 10 * per, 10 * per, SG_FP64)
-            imgref = frames(w)
+            sim.run_packets(ell, ref0, 10 * per, 10 * per, SG_FP64)
+            imgref = frames(w, ell)
 
             def rms(x, y):
                 return float(np.sqrt(np.mean((x - y) ** 2)) / np.sqrt(np.mean(y ** 2)))

@@ -172,6 +172,15 @@ int sg_geometry_density(const double* geom16, int64_t n, const double* xyz, doub
 int sg_tree_subdivide(int arity, const double ext[6], const double* geom16, double scale, double total_mass,
                       int minlevel, int maxlevel, double maxmassfraction, int nsamples, uint32_t seed,
                       uint8_t* flags, int64_t cap, int64_t* nnodes);
+/* The same subdivision with node masses from a density field given per voxel of a regular
+ * nx x ny x nz grid over ext (value [i][j][k] at i*ny*nz + j*nz + k, constant inside a voxel): exact mass in
+ * each node's box, the total being the field's mass. */
+int sg_tree_subdivide_field(int arity, const double ext[6], const double* field, int nx, int ny, int nz,
+                            int minlevel, int maxlevel, double maxmassfraction, uint8_t* flags, int64_t cap,
+                            int64_t* nnodes);
+/* Cell densities from such a voxel field: mass in the cell's box over its volume, the exact mass-in-box
+ * densities of TreeDustGrid's particle path (TreeDustGrid.cpp:647-660). Cartesian and tree grids. */
+int sg_grid_field_density(sg_grid* g, const double* field, int nx, int ny, int nz, double* rho_out);
 /* DustSystem::setSampleDensityBody (DustSystem.cpp:145-172): mean of nsamples densities per cell
  * (Cartesian and tree grids), times scale. */
 int sg_grid_sample_density(sg_grid* g, const double* geom16, double scale, int nsamples, uint32_t seed,

@@ -83,6 +83,10 @@ _SIGS = {
                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _U8,
                                          ctypes.c_int64, _I64]),
     "sg_grid_sample_density": (ctypes.c_int, [_VP, _D, ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _D]),
+    "sg_tree_subdivide_field": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_double, _U8, ctypes.c_int64,
+                                               _I64]),
+    "sg_grid_field_density": (ctypes.c_int, [_VP, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
     "sg_voronoi_neighbors": (ctypes.c_int, [_D, _D, ctypes.c_int64, _I64, _I32, ctypes.c_int64, _I64]),
 }
 

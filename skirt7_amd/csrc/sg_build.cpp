@@ -250,6 +250,128 @@ template <class F> static void parallel_for(int64_t n, F f)
 
 using namespace sg;
 
+namespace sg {
+
+// TreeDustGrid::setupSelfBefore/subdivide (TreeDustGrid.cpp:49-230): breadth-first subdivision with the
+// reference's node numbering and child boxes; a node is split while its level is at most minlevel, or
+// when below maxlevel its mass (mass(box)) reaches maxmassfraction of the total.
+template <class MassFn>
+static int subdivide_tree(int arity, const double ext[6], int minlevel, int maxlevel, double maxmassfraction,
+                          double total_mass, MassFn mass, uint8_t* flags, int64_t cap, int64_t* nnodes)
+{
+    std::vector<double> b(ext, ext + 6);
+    std::vector<int16_t> level(1, 0);
+    std::vector<uint8_t> f;
+    int64_t begin = 0;
+    while (begin < (int64_t)level.size()) {
+        int64_t end = (int64_t)level.size();
+        std::vector<uint8_t> split(end - begin, 0);
+        parallel_for(end - begin, [&](int64_t i) {
+            int64_t l = begin + i;
+            int lev = level[l];
+            if (lev <= minlevel) { split[i] = 1; return; }
+            if (lev >= maxlevel) return;
+            if (mass(&b[6 * l], l) / total_mass >= maxmassfraction) split[i] = 1;
+        });
+        for (int64_t l = begin; l < end; l++) {
+            f.push_back(split[l - begin]);
+            if (!split[l - begin]) continue;
+            double q[6];
+            std::memcpy(q, &b[6 * l], sizeof q);
+            int lev = level[l] + 1;
+            if (arity == 8) {
+                double c[3] = {0.5 * (q[0] + q[3]), 0.5 * (q[1] + q[4]), 0.5 * (q[2] + q[5])};
+                for (int k = 0; k < 8; k++) {
+                    for (int a = 0; a < 3; a++) b.push_back(((k >> a) & 1) ? c[a] : q[a]);
+                    for (int a = 0; a < 3; a++) b.push_back(((k >> a) & 1) ? q[a + 3] : c[a]);
+                    level.push_back((int16_t)lev);
+                }
+            } else {
+                int a = level[l] % 3;
+                double c = 0.5 * (q[a] + q[a + 3]);
+                for (int k = 0; k < 2; k++) {
+                    double nb[6];
+                    std::memcpy(nb, q, sizeof nb);
+                    if (k == 0) nb[a + 3] = c;
+                    else nb[a] = c;
+                    b.insert(b.end(), nb, nb + 6);
+                    level.push_back((int16_t)lev);
+                }
+            }
+        }
+        begin = end;
+        if ((int64_t)level.size() > cap) { set_error("flag buffer too small"); *nnodes = (int64_t)level.size(); return -5; }
+    }
+    *nnodes = (int64_t)f.size();
+    if ((int64_t)f.size() > cap) { set_error("flag buffer too small"); return -5; }
+    std::memcpy(flags, f.data(), f.size());
+    return 0;
+}
+
+// A density given per voxel of a regular grid over a box (value [i][j][k] at i*ny*nz + j*nz + k), constant
+// inside each voxel. The mass in any axis-aligned box is exact: per axis the box covers at most a partial
+// voxel, a run of whole voxels and a partial voxel, and each of the up to 27 combinations is a
+// summed-volume lookup times its edge lengths.
+struct VoxelField {
+    double e[6];
+    int n[3];
+    double w[3];
+    std::vector<double> P;   // prefix sums, (nx+1)(ny+1)(nz+1)
+    double total = 0;
+    VoxelField(const double ext[6], const double* v, int nx, int ny, int nz)
+    {
+        std::memcpy(e, ext, sizeof e);
+        n[0] = nx; n[1] = ny; n[2] = nz;
+        for (int a = 0; a < 3; a++) w[a] = (e[a + 3] - e[a]) / n[a];
+        const size_t sy = (size_t)nz + 1, sx = (size_t)(ny + 1) * sy;
+        P.assign((size_t)(nx + 1) * sx, 0.0);
+        for (int i = 1; i <= nx; i++)
+            for (int j = 1; j <= ny; j++)
+                for (int k = 1; k <= nz; k++)
+                    P[i * sx + j * sy + k] = v[((size_t)(i - 1) * ny + (j - 1)) * nz + (k - 1)] + P[(i - 1) * sx + j * sy + k] +
+                                             P[i * sx + (j - 1) * sy + k] + P[i * sx + j * sy + k - 1] -
+                                             P[(i - 1) * sx + (j - 1) * sy + k] - P[(i - 1) * sx + j * sy + k - 1] -
+                                             P[i * sx + (j - 1) * sy + k - 1] + P[(i - 1) * sx + (j - 1) * sy + k - 1];
+        total = P.back() * w[0] * w[1] * w[2];
+    }
+    double S(int i0, int i1, int j0, int j1, int k0, int k1) const
+    {
+        const size_t sy = (size_t)n[2] + 1, sx = (size_t)(n[1] + 1) * sy;
+        auto at = [&](int i, int j, int k) { return P[i * sx + j * sy + k]; };
+        return at(i1, j1, k1) - at(i0, j1, k1) - at(i1, j0, k1) - at(i1, j1, k0) + at(i0, j0, k1) + at(i0, j1, k0) +
+               at(i1, j0, k0) - at(i0, j0, k0);
+    }
+    // voxel index ranges [i0, i1) and the edge length inside each voxel of the range, for [lo, hi] on axis a
+    int segments(int a, double lo, double hi, int* i0, int* i1, double* len) const
+    {
+        double u0 = std::max(0.0, std::min((double)n[a], (lo - e[a]) / w[a]));
+        double u1 = std::max(0.0, std::min((double)n[a], (hi - e[a]) / w[a]));
+        int f0 = std::min(n[a] - 1, (int)std::floor(u0)), f1 = std::min(n[a] - 1, (int)std::floor(u1));
+        int c = 0;
+        if (f0 == f1) { i0[c] = f0; i1[c] = f0 + 1; len[c++] = (u1 - u0) * w[a]; return c; }
+        if (f0 + 1 > u0) { i0[c] = f0; i1[c] = f0 + 1; len[c++] = (f0 + 1 - u0) * w[a]; }
+        if (f1 > f0 + 1) { i0[c] = f0 + 1; i1[c] = f1; len[c++] = w[a]; }
+        if (u1 > f1) { i0[c] = f1; i1[c] = f1 + 1; len[c++] = (u1 - f1) * w[a]; }
+        return c;
+    }
+    double mass(const double* q) const
+    {
+        int a0[3], a1[3], b0[3], b1[3], c0[3], c1[3];
+        double la[3], lb[3], lc[3];
+        int na = segments(0, q[0], q[3], a0, a1, la);
+        int nb = segments(1, q[1], q[4], b0, b1, lb);
+        int nc = segments(2, q[2], q[5], c0, c1, lc);
+        double m = 0;
+        for (int x = 0; x < na; x++)
+            for (int y = 0; y < nb; y++)
+                for (int z = 0; z < nc; z++)
+                    m += la[x] * lb[y] * lc[z] * S(a0[x], a1[x], b0[y], b1[y], c0[z], c1[z]);
+        return m;
+    }
+};
+
+} // namespace sg
+
 extern "C" {
 
 // SersicFunction (SersicFunction.cpp:15-68): 101 log-spaced radii s in [1e-6, 1e4] (units of reff)
@@ -326,61 +448,66 @@ int sg_tree_subdivide(int arity, const double ext[6], const double* geom16, doub
     if (!(arity == 8 || arity == 2)) { set_error("arity must be 8 or 2"); return -1; }
     if (maxlevel < 2 || maxlevel <= minlevel) { set_error("bad tree levels"); return -1; }
     GeomDensity dens(geom16);
-    std::vector<double> b(ext, ext + 6);
-    std::vector<int16_t> level(1, 0);
-    std::vector<int8_t> dirv(1, -1);
-    std::vector<uint8_t> f;
-    int64_t begin = 0;
-    while (begin < (int64_t)level.size()) {
-        int64_t end = (int64_t)level.size();
-        std::vector<uint8_t> split(end - begin, 0);
-        parallel_for(end - begin, [&](int64_t i) {
-            int64_t l = begin + i;
-            int lev = level[l];
-            if (lev <= minlevel) { split[i] = 1; return; }
-            if (lev >= maxlevel) return;
-            const double* q = &b[6 * l];
-            double vol = (q[3] - q[0]) * (q[4] - q[1]) * (q[5] - q[2]);
-            double sum = 0;
-            for (int j = 0; j < nsamples; j++) {
-                double u[3];
-                sample3(seed, (uint64_t)l, (uint32_t)j, u);
-                sum += scale * dens(q[0] + u[0] * (q[3] - q[0]), q[1] + u[1] * (q[4] - q[1]), q[2] + u[2] * (q[5] - q[2]));
-            }
-            double mass = sum / nsamples * vol;
-            if (mass / total_mass >= maxmassfraction) split[i] = 1;
-        });
-        for (int64_t l = begin; l < end; l++) {
-            f.push_back(split[l - begin]);
-            if (!split[l - begin]) continue;
-            double q[6];
-            std::memcpy(q, &b[6 * l], sizeof q);
-            int lev = level[l] + 1;
-            if (arity == 8) {
-                double c[3] = {0.5 * (q[0] + q[3]), 0.5 * (q[1] + q[4]), 0.5 * (q[2] + q[5])};
-                for (int k = 0; k < 8; k++) {
-                    for (int a = 0; a < 3; a++) b.push_back(((k >> a) & 1) ? c[a] : q[a]);
-                    for (int a = 0; a < 3; a++) b.push_back(((k >> a) & 1) ? q[a + 3] : c[a]);
-                    level.push_back((int16_t)lev); dirv.push_back(-1);
-                }
-            } else {
-                int a = level[l] % 3;
-                double c = 0.5 * (q[a] + q[a + 3]);
-                for (int k = 0; k < 2; k++) {
-                    double nb[6];
-                    std::memcpy(nb, q, sizeof nb);
-                    if (k == 0) nb[a + 3] = c;
-                    else nb[a] = c;
-                    b.insert(b.end(), nb, nb + 6);
-                    level.push_back((int16_t)lev); dirv.push_back(-1);
-                }
-            }
+    // node mass from nsamples counter-based density samples in the node's box
+    auto mass = [&](const double* q, int64_t l) {
+        double vol = (q[3] - q[0]) * (q[4] - q[1]) * (q[5] - q[2]);
+        double sum = 0;
+        for (int j = 0; j < nsamples; j++) {
+            double u[3];
+            sample3(seed, (uint64_t)l, (uint32_t)j, u);
+            sum += scale * dens(q[0] + u[0] * (q[3] - q[0]), q[1] + u[1] * (q[4] - q[1]), q[2] + u[2] * (q[5] - q[2]));
         }
-        begin = end;
+        return sum / nsamples * vol;
+    };
+    return subdivide_tree(arity, ext, minlevel, maxlevel, maxmassfraction, total_mass, mass, flags, cap, nnodes);
+}
+
+// The same subdivision with node masses from a voxel density field (exact mass in the node's box).
+int sg_tree_subdivide_field(int arity, const double ext[6], const double* field, int nx, int ny, int nz,
+                            int minlevel, int maxlevel, double maxmassfraction, uint8_t* flags, int64_t cap,
+                            int64_t* nnodes)
+{
+    if (!(arity == 8 || arity == 2)) { set_error("arity must be 8 or 2"); return -1; }
+    if (maxlevel < 2 || maxlevel <= minlevel) { set_error("bad tree levels"); return -1; }
+    if (nx < 1 || ny < 1 || nz < 1) { set_error("empty density field"); return -1; }
+    VoxelField vf(ext, field, nx, ny, nz);
+    if (!(vf.total > 0)) { set_error("the density field has no mass"); return -1; }
+    auto mass = [&](const double* q, int64_t) { return vf.mass(q); };
+    return subdivide_tree(arity, ext, minlevel, maxlevel, maxmassfraction, vf.total, mass, flags, cap, nnodes);
+}
+
+// Cell densities from a voxel density field: the mass in each cell's box over its volume (the exact
+// mass-in-box densities of TreeDustGrid.cpp:647-660). Cartesian and tree grids.
+int sg_grid_field_density(sg_grid* g, const double* field, int nx, int ny, int nz, double* rho_out)
+{
+    if (g->kind == GK_VORO) { set_error("Voronoi cells take the density at their site"); return -1; }
+    if (nx < 1 || ny < 1 || nz < 1) { set_error("empty density field"); return -1; }
+    double ext[6];
+    std::vector<double> hx, hy, hz;
+    int Nx = 0, Ny = 0, Nz = 0;
+    if (g->kind == GK_CART) {
+        Nx = g->cart.Nx; Ny = g->cart.Ny; Nz = g->cart.Nz;
+        hx.resize(Nx + 1); hy.resize(Ny + 1); hz.resize(Nz + 1);
+        (void)hipMemcpy(hx.data(), g->cart.xv, hx.size() * 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(hy.data(), g->cart.yv, hy.size() * 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(hz.data(), g->cart.zv, hz.size() * 8, hipMemcpyDeviceToHost);
+        ext[0] = hx[0]; ext[1] = hy[0]; ext[2] = hz[0]; ext[3] = hx[Nx]; ext[4] = hy[Ny]; ext[5] = hz[Nz];
+    } else {
+        std::memcpy(ext, g->tree.ext, sizeof ext);
     }
-    *nnodes = (int64_t)f.size();
-    if ((int64_t)f.size() > cap) { set_error("flag buffer too small"); return -5; }
-    std::memcpy(flags, f.data(), f.size());
+    VoxelField vf(ext, field, nx, ny, nz);
+    parallel_for(g->ncells, [&](int64_t m) {
+        double q[6];
+        if (g->kind == GK_CART) {
+            int i = (int)(m / ((int64_t)Nz * Ny)), j = (int)((m / Nz) % Ny), k = (int)(m % Nz);
+            q[0] = hx[i]; q[1] = hy[j]; q[2] = hz[k]; q[3] = hx[i + 1]; q[4] = hy[j + 1]; q[5] = hz[k + 1];
+        } else {
+            const TreeRec64& r = g->hrec64[m];
+            for (int a = 0; a < 3; a++) { q[a] = r.lo[a]; q[a + 3] = r.hi[a]; }
+        }
+        double vol = (q[3] - q[0]) * (q[4] - q[1]) * (q[5] - q[2]);
+        rho_out[m] = vf.mass(q) / vol;
+    });
     return 0;
 }
 

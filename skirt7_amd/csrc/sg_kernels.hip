@@ -244,8 +244,8 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     g->kind = GK_TREE;
     HostTree& h = g->ht;
     if (!h.build(arity, ext, subdivided, nnodes)) { delete g; throw Fail{-1}; }
-    if (search == SG_NEIGHBOR) h.build_neighbors();
-    else h.nb.assign(6 * h.child0.size(), {});
+    if (search == SG_NEIGHBOR) h.build_neighbors();   // TopDown needs no neighbour lists (and no ropes)
+    const bool lists = !h.nb.empty();
     int64_t nn = (int64_t)h.child0.size(), nc = (int64_t)h.idv.size();
     g->ncells = nc;
     std::vector<int32_t> words(nn);
@@ -260,9 +260,9 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
         for (int a = 0; a < 3; a++) { r.lo[a] = h.bounds[6 * l + a]; r.hi[a] = h.bounds[6 * l + 3 + a]; }
         r.rho = 0; r.node = l;
         for (int w = 0; w < 6; w++) {
-            const auto& lst = h.nb[6 * (size_t)l + w];
             off[c * 6 + w] = (int32_t)ids.size();
-            for (int32_t nbn : lst) ids.push_back(h.cellnumber[nbn]);
+            if (!lists) continue;
+            for (int32_t nbn : h.nb[6 * (size_t)l + w]) ids.push_back(h.cellnumber[nbn]);
         }
     }
     // mixed-precision records: subdivision counts per axis (a child adds one along each axis of an
@@ -286,8 +286,8 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
         cellof[l] = h.cellnumber[l];
         for (int w = 0; w < 6; w++) {
             // rope: the first (largest-overlap) neighbour, raised to this node's level when finer
-            const auto& lst = h.nb[6 * (size_t)l + w];
-            int32_t r = lst.empty() ? -1 : lst[0];
+            int32_t r = -1;
+            if (lists && !h.nb[6 * (size_t)l + w].empty()) r = h.nb[6 * (size_t)l + w][0];
             while (r >= 0 && h.level[r] > h.level[l]) r = h.father[r];
             f.rope[w] = leaf ? r : -1;
         }

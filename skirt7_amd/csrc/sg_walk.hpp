@@ -567,8 +567,10 @@ struct TreeW32 {
         b = p[1];
     }
     // Descends from node n (record a, b already loaded; indices i, j, k at its subdivision) to the
-    // leaf containing the current position and takes it over; child choice with the reference's
-    // strict "<" against the split point (OctTreeNode.cpp:184-189, BinTreeNode.cpp:326-335).
+    // leaf containing the current position and takes it over. Child choice by "<" against the split
+    // point (OctTreeNode.cpp:184-189, BinTreeNode.cpp:326-335), except that a point lying exactly on a
+    // split plane goes to the child the ray is heading into (a walk never steps back into a cell behind
+    // it; the reference instead steps a small distance past every wall).
     __device__ __forceinline__ void descend_loaded(const View& t, uint4 a, uint4 b, int n, int32_t i, int32_t j,
                                                    int32_t k)
     {
@@ -581,7 +583,9 @@ struct TreeW32 {
                 const float cx = t.extf[0] + (float)(2 * i + 1) * width(t, 0, ns + 1u);
                 const float cy = t.extf[1] + (float)(2 * j + 1) * width(t, 1, ns + (1u << 5));
                 const float cz = t.extf[2] + (float)(2 * k + 1) * width(t, 2, ns + (1u << 10));
-                const int qx = x < cx ? 0 : 1, qy = y < cy ? 0 : 1, qz = z < cz ? 0 : 1;
+                const int qx = (kx < 0.0f ? x <= cx : x < cx) ? 0 : 1;
+                const int qy = (ky < 0.0f ? y <= cy : y < cy) ? 0 : 1;
+                const int qz = (kz < 0.0f ? z <= cz : z < cz) ? 0 : 1;
                 n = c0 + qx + 2 * qy + 4 * qz;
                 i = 2 * i + qx; j = 2 * j + qy; k = 2 * k + qz;
             } else {
@@ -589,9 +593,19 @@ struct TreeW32 {
                 const int d = (int)((a.x >> 16) & 3u);
                 const uint32_t ns1 = ns + (1u << (5 * d));
                 int q;
-                if (d == 0) { q = x < t.extf[0] + (float)(2 * i + 1) * width(t, 0, ns1) ? 0 : 1; i = 2 * i + q; }
-                else if (d == 1) { q = y < t.extf[1] + (float)(2 * j + 1) * width(t, 1, ns1) ? 0 : 1; j = 2 * j + q; }
-                else { q = z < t.extf[2] + (float)(2 * k + 1) * width(t, 2, ns1) ? 0 : 1; k = 2 * k + q; }
+                if (d == 0) {
+                    const float c = t.extf[0] + (float)(2 * i + 1) * width(t, 0, ns1);
+                    q = (kx < 0.0f ? x <= c : x < c) ? 0 : 1;
+                    i = 2 * i + q;
+                } else if (d == 1) {
+                    const float c = t.extf[1] + (float)(2 * j + 1) * width(t, 1, ns1);
+                    q = (ky < 0.0f ? y <= c : y < c) ? 0 : 1;
+                    j = 2 * j + q;
+                } else {
+                    const float c = t.extf[2] + (float)(2 * k + 1) * width(t, 2, ns1);
+                    q = (kz < 0.0f ? z <= c : z < c) ? 0 : 1;
+                    k = 2 * k + q;
+                }
                 n = c0 + q;
             }
             load(t, n, a, b);
@@ -610,14 +624,16 @@ struct TreeW32 {
         load(t, n, a, b);
         descend_loaded(t, a, b, n, i, j, k);
     }
-    // Root descent in FP32 with the exit axis compared in the travel direction, so a point lying
-    // exactly on the exit wall goes to the cell beyond it. Returns the leaf's node id (and its
+    // Root descent in FP32. At the start of a walk (xa < 0) by "<" as the reference; after a step
+    // through a wall on axis xa, every axis is compared in the travel direction, so a point lying
+    // exactly on a wall goes to the cell the ray is heading into. Returns the leaf's node id (and its
     // indices) or -1.
     __device__ __forceinline__ int descend_root(const View& t, const int32_t* lds, int nlds, int xa, int xdir,
                                                 int32_t& i, int32_t& j, int32_t& k) const
     {
         const float* e = t.extf;
         const float p[3] = {x, y, z};
+        const float kd[3] = {kx, ky, kz};
         for (int a = 0; a < 3; a++) {
             bool in = p[a] >= e[a] && p[a] <= e[a + 3];
             if (a == xa) in = xdir > 0 ? (p[a] >= e[a] && p[a] < e[a + 3]) : (p[a] > e[a] && p[a] <= e[a + 3]);
@@ -633,7 +649,7 @@ struct TreeW32 {
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     const float c = e[a] + (float)(2 * kk[a] + 1) * ldexpf(t.widthf[a], -(level + 1));
-                    const bool low = (a == xa && xdir < 0) ? (p[a] <= c) : (p[a] < c);
+                    const bool low = (xa >= 0 && kd[a] < 0.0f) ? (p[a] <= c) : (p[a] < c);
                     kk[a] = 2 * kk[a] + (low ? 0 : 1);
                     q += low ? 0 : 1 << a;
                 }
@@ -643,7 +659,7 @@ struct TreeW32 {
                 for (int b = 0; b < 3; b++) {
                     if (b != a) continue;
                     const float c = e[b] + (float)(2 * kk[b] + 1) * ldexpf(t.widthf[b], -(level / 3 + 1));
-                    const bool low = (b == xa && xdir < 0) ? (p[b] <= c) : (p[b] < c);
+                    const bool low = (xa >= 0 && kd[b] < 0.0f) ? (p[b] <= c) : (p[b] < c);
                     kk[b] = 2 * kk[b] + (low ? 0 : 1);
                     q = low ? 0 : 1;
                 }

@@ -134,6 +134,25 @@ class TreeDustGrid(DustGrid):
         return cls(e, flags[:n.value].copy(), search)
 
 
+    @classmethod
+    def subdivide_field(cls, extent, field, minlevel=2, maxlevel=30, maxmassfraction=1e-6, search=SG_NEIGHBOR):
+        """The subdivision with exact node masses from a voxel density field field[i, j, k] over the box."""
+        e = as_f64(extent)
+        f = as_f64(field)
+        nx, ny, nz = f.shape
+        cap = 1 << 22
+        while True:
+            flags = np.zeros(cap, np.uint8)
+            n = ctypes.c_int64()
+            st = lib().sg_tree_subdivide_field(cls.arity, dptr(e), dptr(f), nx, ny, nz, minlevel, maxlevel,
+                                               maxmassfraction, flags.ctypes.data_as(capi._U8), cap, ctypes.byref(n))
+            if st == -5:
+                cap = max(2 * cap, n.value + 1024)
+                continue
+            check(st)
+            return cls(e, flags[:n.value].copy(), search)
+
+
 class OctTreeDustGrid(TreeDustGrid):
     """OctTreeDustGrid (OctTreeDustGrid.cpp:32-43, OctTreeNode.cpp)."""
     arity = 8
@@ -344,6 +363,14 @@ def sample_density(grid, geometry, scale, nsamples=100, seed=4357):
     out = np.zeros(grid.numCells())
     check(lib().sg_grid_sample_density(grid._h, dptr(as_f64(geometry.params())), scale, nsamples, seed,
                                        dptr(out)))
+    return out
+
+
+def field_density(grid, field):
+    """Cell densities from a voxel density field: mass in the cell's box over its volume."""
+    f = as_f64(field)
+    out = np.zeros(grid.numCells())
+    check(lib().sg_grid_field_density(grid._h, dptr(f), f.shape[0], f.shape[1], f.shape[2], dptr(out)))
     return out
 
 

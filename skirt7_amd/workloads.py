@@ -16,7 +16,7 @@ import math
 import numpy as np
 
 from . import skirt as S
-from .capi import SG_NEIGHBOR
+from .capi import SG_NEIGHBOR, SG_TOPDOWN
 
 EXTENT = np.array([-20.0, -20.0, -20.0, 20.0, 20.0, 20.0])
 DISTANCE = 1.0e4   # 10 Mpc in kpc
@@ -160,4 +160,40 @@ def config2(nsites=1_000_000, nlambda=50, pixels=256, packages=2e7, seed=4357, t
     ins = [S.FrameInstrument(DISTANCE, math.radians(a), 0.0, 0.0, pixels, pixels, 40.0, 40.0) for a in (0.0, 75.0)]
     d = dict(name="config2", kind="voronoi", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
              instr=S.InstrumentSystem(ins, nlambda), store_abs=store_abs, wavelengths=lam)
+    return _finish(d, packages, seed)
+
+
+def lognormal_field(n, sigma=1.5, slope=-11.0 / 3.0, seed=4357):
+    """A periodic n^3 log-normal density field exp(g) with g Gaussian of standard deviation sigma and power
+    spectrum P(k) ~ k^slope (k in cycles per box), normalised to unit mean."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n) * n
+    kk = np.sqrt(k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + k1[None, None, :] ** 2)
+    amp = np.zeros_like(kk)
+    amp[kk > 0] = kk[kk > 0] ** (0.5 * slope)
+    g = np.fft.ifftn(np.fft.fftn(rng.standard_normal((n, n, n))) * amp).real
+    g = (g - g.mean()) / g.std() * sigma
+    rho = np.exp(g)
+    return rho / rho.mean()
+
+
+def config4(nvox=256, maxmassfraction=9e-8, maxlevel=40, pixels=1024, packages=1e9, seed=4357, tau=1.0,
+            search=SG_TOPDOWN, half=15.0):
+    """kd-tree: a log-normal dust field (sigma_ln 1.5, P(k) ~ k^-11/3) over a 30-kpc box, given as an nvox^3
+    voxel field in place of the particle set it would be sampled from; a BinTreeDustGrid with alternating
+    split axes refined by mass fraction (9e-8 gives 1.61e7 leaves at nvox 256), with exact mass-in-box
+    cell densities; the field normalised to a
+    mean optical depth tau across the box height. Stars: an exponential disc (hR 4, hz 0.4). Eight frames of
+    pixels^2 at inclinations 30 and 60 deg, azimuths 0, 90, 180, 270 deg."""
+    ext = np.array([-half, -half, -half, half, half, half])
+    field = lognormal_field(nvox, seed=seed)
+    grid = S.BinTreeDustGrid.subdivide_field(ext, field, minlevel=3, maxlevel=maxlevel,
+                                             maxmassfraction=maxmassfraction, search=search)
+    mix = S.SimpleOligoDustMix([1.0], [0.6], [0.6])
+    rho = S.field_density(grid, field) * tau / (2.0 * half * mix.kappaext[0])
+    comps = [S.OligoStellarComp(S.ExpDiskGeometry(4.0, 0.4), [1.0e10])]
+    ins = [S.FrameInstrument(DISTANCE, math.radians(i), math.radians(a), 0.0, pixels, pixels, 2.0 * half, 2.0 * half)
+           for i in (30.0, 60.0) for a in (0.0, 90.0, 180.0, 270.0)]
+    d = dict(name="config4", kind="bintree", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=None,
+             instr=S.InstrumentSystem(ins), field=field)
     return _finish(d, packages, seed)

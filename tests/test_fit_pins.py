@@ -1,6 +1,6 @@
 """Pins of the oracle's FitSKIRT evaluation (PSF kernel, nested-loop convolution, golden-section luminosity
-fit) against what the mathematics fixes, and the product's host-side kernel builder against the oracle.
-CPU only."""
+fit) against what the mathematics fixes, the product's host-side kernel builder against the oracle, and the
+product's host-side voxel-field tree subdivision against brute-force masses. CPU only."""
 import math
 
 import numpy as np
@@ -115,3 +115,50 @@ def test_golden_section_at_the_bounds():
     assert abs(lum - 100.0) < 1e-5
     lum, _, _ = O.golden_section(ref, frame, 1.0, 10.0)
     assert abs(lum - 10.0) < 1e-6
+
+
+def test_voxel_field_tree_masses_exact():
+    """sg_tree_subdivide_field / sg_grid_field_density (setup helpers): the mass in a box of a voxel field
+    equals the brute-force sum over voxels of density x overlap volume; checked through the subdivision of
+    a tiny field, whose node masses decide every split."""
+    import ctypes
+    from skirt7_amd import capi
+    rng = np.random.default_rng(8)
+    n = 6
+    field = rng.random((n, n, n)) ** 4
+    ext = np.array([-1.0, -2.0, 0.0, 2.0, 1.0, 3.0])
+    w = (ext[3:] - ext[:3]) / n
+
+    def brute(q):
+        m = 0.0
+        for i in range(n):
+            for j in range(n):
+                for k in range(n):
+                    lo = ext[:3] + w * np.array([i, j, k])
+                    ov = np.clip(np.minimum(q[3:], lo + w) - np.maximum(q[:3], lo), 0, None)
+                    m += field[i, j, k] * np.prod(ov)
+        return m
+
+    total = brute(ext)
+    for maxmf in (0.2, 0.05, 0.01):
+        cap = 4096
+        flags = np.zeros(cap, np.uint8)
+        nn = ctypes.c_int64()
+        assert capi.lib().sg_tree_subdivide_field(2, capi.dptr(ext), capi.dptr(np.ascontiguousarray(field)), n, n, n,
+                                                  1, 12, maxmf, flags.ctypes.data_as(capi._U8), cap,
+                                                  ctypes.byref(nn)) == 0
+        # rebuild the node boxes breadth-first and check every decision with the brute-force mass
+        boxes = [ext.copy()]
+        levels = [0]
+        for l in range(nn.value):
+            q, lev = boxes[l], levels[l]
+            should = lev <= 1 or (lev < 12 and brute(q) / total >= maxmf)
+            assert bool(flags[l]) == should, (l, lev)
+            if flags[l]:
+                a = lev % 3
+                c = 0.5 * (q[a] + q[a + 3])
+                lo, hi = q.copy(), q.copy()
+                lo[a + 3] = c
+                hi[a] = c
+                boxes += [lo, hi]
+                levels += [lev + 1, lev + 1]

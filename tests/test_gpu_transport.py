@@ -108,3 +108,37 @@ def test_split_and_lane_pipelines_agree(c0_small, which):
                 assert np.allclose(a, b, rtol=1e-10, atol=1e-300)
     finally:
         S.set_pipeline(1)
+
+
+@pytest.fixture(scope="module")
+def c4_small():
+    return workloads.config4(nvox=16, maxmassfraction=2e-4, pixels=32, packages=6000)
+
+
+def test_kdtree_fp64_matches_oracle(c4_small):
+    """Alternating binary tree with TopDown search over a log-normal field: packet-by-packet parity."""
+    w = c4_small
+    n = 6000
+    w["instr"].clear()
+    c = w["sim"].run_packets(0, 0, n, n, SG_FP64)
+    gpu = frames(w)
+    sc = O.scene_from_workload(w)
+    ref, rc = sc.run_packets(w, 0, 0, n, n)
+    orc = oracle_frames(w, ref)
+    assert c["packets"] == rc[0] == n
+    assert abs(int(c["scatterings"]) - int(rc[3])) <= 2
+    assert abs(int(c["own_crossings"]) - int(rc[1])) <= 0.001 * rc[1]
+    for g, o in zip(gpu, orc):
+        assert abs(g.sum() - o.sum()) <= 1e-6 * o.sum()
+
+
+def test_kdtree_mixed_matches_fp64(c4_small):
+    w = c4_small
+    n = 6000
+    out = []
+    for prec in (SG_FP64, SG_MIXED):
+        w["instr"].clear()
+        w["sim"].run_packets(0, 0, n, n, prec)
+        out.append(frames(w))
+    for a, b in zip(*out):
+        assert abs(a.sum() - b.sum()) <= 2e-3 * a.sum()
