@@ -142,3 +142,35 @@ def test_mixed_precision_walks_close(kind):
     t32 = ds.opticaldepths(rays, 0, SG_MIXED)
     rel = np.abs(t32 - t64) / np.maximum(t64, 1e-3)
     assert np.quantile(rel, 0.99) < 1e-4, np.quantile(rel, 0.99)
+
+
+@pytest.mark.parametrize("arity,search", [(8, SG_NEIGHBOR), (8, SG_TOPDOWN), (2, SG_TOPDOWN), (2, SG_NEIGHBOR)])
+def test_mixed_walk_through_edges_and_corners(arity, search):
+    """Rays that run exactly through cell edges and corners, in every sign combination of the direction: the
+    mixed walk must go on into the cells ahead (a point on a wall belongs to the cell the ray heads into)
+    and return the same cells with positive length and the same total length as the exact walk. Walking
+    back into a cell behind would cycle through zero-length steps until the step limit and cut the path."""
+    flags = random_tree_flags(arity, 6 if arity == 8 else 12, 1.0, seed=1, minlevel=1)   # a uniform tree
+    ext = np.array([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0])
+    grid = S.TreeDustGrid.__new__(S.OctTreeDustGrid if arity == 8 else S.BinTreeDustGrid)
+    S.TreeDustGrid.__init__(grid, ext, flags, search)
+    rays = []
+    for sx in (-1, 1):
+        for sy in (-1, 1):
+            for sz in (-1, 1):
+                for k in ((0.6 * sx, 0.8 * sy, 0.0), (0.48 * sx, 0.64 * sy, 0.6 * sz), (sx / 3 ** 0.5, sy / 3 ** 0.5, sz / 3 ** 0.5)):
+                    for r in ((0.25, 0.25, 0.1), (0.25, 0.25, 0.25), (-0.5, 0.125, 0.375), (0.0, 0.0, 0.0)):
+                        rays.append(list(r) + list(k))
+    rays = np.array(rays)
+    c64, m64, d64 = grid.paths(rays, max_seg=2048, precision=SG_FP64)
+    c32, m32, d32 = grid.paths(rays, max_seg=2048, precision=SG_MIXED)
+    assert np.all(c64 >= 0) and np.all(c32 >= 0)
+    for i in range(len(rays)):
+        a = [(m, d) for m, d in zip(m64[i, :c64[i]], d64[i, :c64[i]]) if d > 1e-5]
+        b = [(m, d) for m, d in zip(m32[i, :c32[i]], d32[i, :c32[i]]) if d > 1e-5]
+        assert abs(sum(d for _, d in a) - sum(d for _, d in b)) < 1e-5, i
+        # a ray lying inside a wall plane (zero direction component) touches two cells at once; the
+        # Neighbor search then takes the first of them in the neighbour list, the descents the upper one
+        in_plane = np.any(rays[i, 3:] == 0.0)
+        if not (in_plane and search == SG_NEIGHBOR):
+            assert [m for m, _ in a] == [m for m, _ in b], i
