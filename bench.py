@@ -45,9 +45,12 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="config1", choices=("config1", "config2", "config4"),
+    ap.add_argument("--workload", default="config1", choices=("config1", "config2", "config3", "config4"),
                     help="config1 octree (the metric's workload), config2 Voronoi with the absorption table, "
-                         "config4 kd-tree")
+                         "config3 batched population + fit, config4 kd-tree")
+    ap.add_argument("--members", type=int, default=256, help="config3: population size")
+    ap.add_argument("--member-packets", type=float, nargs=2, default=(1e5, 1e7),
+                    help="config3: log-uniform range of packets per member")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--packets", type=float, default=1e7, help="packets per step")
@@ -124,6 +127,53 @@ def cpu_baseline(w, seconds):
             "crossings_per_s": float(c[1] + c[2]) / dt}
 
 
+def population(a):
+    """config3: one batched launch over the population against member-by-member runs, then the fit."""
+    from skirt7_amd import skirt as S
+    from skirt7_amd import workloads
+    from skirt7_amd.capi import SG_MIXED
+    t0 = time.perf_counter()
+    pop = workloads.config3(members=a.members, packets=tuple(a.member_packets))
+    ms = pop["members"]
+    counts = [m["packages"] for m in ms]
+    total = int(sum(counts))
+    log("setup: %d members (%d packets in all) built in %.1f s on %s" % (len(ms), total, time.perf_counter() - t0,
+                                                                         S.device_name()))
+    sims = [m["sim"] for m in ms]
+    S.run_batch(sims[:4], [min(c, 100000) for c in counts[:4]], precision=SG_MIXED)   # warmup
+    for m in ms:
+        m["instr"].clear()
+    t0 = time.perf_counter()
+    cb = S.run_batch(sims, counts, precision=SG_MIXED)
+    t_batch = time.perf_counter() - t0
+    log("batched: %d packets in %.3f s: %.3e packets/s" % (total, t_batch, total / t_batch))
+    t0 = time.perf_counter()
+    for m in ms:
+        m["sim"].run_packets(0, 0, m["packages"], m["packages"], SG_MIXED)
+    t_seq = time.perf_counter() - t0
+    log("member by member: %.3f s: %.3e packets/s" % (t_seq, total / t_seq))
+    tw = pop["target"]
+    tw["sim"].run_packets(0, 0, tw["packages"], tw["packages"], SG_MIXED)
+    ref, gain = workloads.target_image(tw["instr"].frame(0)[0], pop["target_counts"])
+    S.fit_frames([m["instr"] for m in ms[:2]], ref, pop["kernel"], gain * 1e-2, gain * 1e2)   # warmup
+    t0 = time.perf_counter()
+    lum, chi2 = S.fit_frames([m["instr"] for m in ms], ref, pop["kernel"], gain * 1e-2, gain * 1e2)
+    t_fit = time.perf_counter() - t0
+    log("fit of %d frames (%dx%d, PSF %dx%d): %.3f s" % (len(ms), ref.shape[1], ref.shape[0], pop["kernel"].dimension,
+                                                         pop["kernel"].dimension, t_fit))
+    cross = cb["own_crossings"] + cb["peel_crossings"] + cb["rewalk_crossings"]
+    out = {"metric": "packets/s on 1 MI355X, mixed vs FP64 GPU at equal image error; % HBM peak",
+           "value": total / t_batch, "unit": "packets/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": 1e3 * t_batch, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 (walk, weights) + f64 (wall fallback, tallies)", "data": "synthetic",
+           "config": {"workload": "config3 population of %d Cartesian galaxies, %d packets, one batched launch "
+                                  "per round" % (len(ms), total), "members": len(ms), "packets": total},
+           "member_by_member_packets_per_s": total / t_seq, "fit_seconds": t_fit,
+           "crossings_per_s": cross / t_batch, "counters": cb,
+           "best_fit": {"member": int(np.argmin(chi2)), "chi2": float(np.min(chi2))}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -134,6 +184,9 @@ def main():
     from skirt7_amd import workloads
     from skirt7_amd.capi import SG_FP64, SG_MIXED
 
+    if a.workload == "config3":
+        population(a)
+        return
     if a.copy_calibration > 0:
         copy_calibration(a.copy_calibration)
     S.set_pipeline(a.pipeline)

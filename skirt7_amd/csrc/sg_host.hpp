@@ -132,6 +132,5 @@ struct sg_sim {
     // work buffers of a batched run, held by the batch's first member: range counters and the
     // tables of grid views and run arguments
     sg::DevBuf bnext, btab;
-    // split pipeline, two slot groups: packet slots, walk queue and progress counters per group
-    sg::DevBuf slots[2], queue[2], prog[2];
+
 };

@@ -875,6 +875,19 @@ static const int kSplitCheck = 4;                       // passes between progre
 // Two non-blocking streams of default priority, one per slot group, and a pinned progress buffer.
 static hipStream_t g_split_stream[2] = {nullptr, nullptr};
 static unsigned long long* g_split_host = nullptr;
+// Work buffers of the split pipeline and of the emission, shared by every simulation of the process
+// (runs are synchronous and driven by one host thread): slot groups, walk queues, progress counters,
+// launch states. Allocated on first use, grown as needed, kept until the process ends.
+struct SplitPool {
+    DevBuf slots[2], queue[2], prog[2], emitted, emitdraw;
+};
+static SplitPool* g_pool = nullptr;
+static SplitPool& pool()
+{
+    if (!g_pool) g_pool = new SplitPool();
+    return *g_pool;
+}
+
 static void split_resources()
 {
     if (g_split_stream[0]) return;
@@ -901,15 +914,16 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
     unsigned long long* pc[2];   // per group: [1] queue tail, [3] requests claimed
     for (int g = 0; g < G; g++) {
         const int64_t cap = Pg[g] * (a.ni + 1);
-        if (s->slots[g].bytes < slot_bytes<R>(Pg[g])) s->slots[g].alloc(slot_bytes<R>(Pg[g]));
-        if (s->queue[g].bytes < queue_bytes<R>(cap)) s->queue[g].alloc(queue_bytes<R>(cap));
-        if (s->prog[g].bytes < 4 * sizeof(unsigned long long)) s->prog[g].alloc(4 * sizeof(unsigned long long));
-        pc[g] = s->prog[g].as<unsigned long long>();
-        sl[g] = Slots<R>{s->slots[g].as<R>(),
-                         reinterpret_cast<uint32_t*>(s->slots[g].as<unsigned char>() + (size_t)Pg[g] * SR_NR * sizeof(R)),
+        SplitPool& pl = pool();
+        if (pl.slots[g].bytes < slot_bytes<R>(Pg[g])) pl.slots[g].alloc(slot_bytes<R>(Pg[g]));
+        if (pl.queue[g].bytes < queue_bytes<R>(cap)) pl.queue[g].alloc(queue_bytes<R>(cap));
+        if (pl.prog[g].bytes < 4 * sizeof(unsigned long long)) pl.prog[g].alloc(4 * sizeof(unsigned long long));
+        pc[g] = pl.prog[g].as<unsigned long long>();
+        sl[g] = Slots<R>{pl.slots[g].as<R>(),
+                         reinterpret_cast<uint32_t*>(pl.slots[g].as<unsigned char>() + (size_t)Pg[g] * SR_NR * sizeof(R)),
                          Pg[g]};
-        q[g] = WalkQueue<R>{s->queue[g].as<R>(),
-                            reinterpret_cast<uint32_t*>(s->queue[g].as<unsigned char>() + (size_t)cap * QR_NR * sizeof(R)),
+        q[g] = WalkQueue<R>{pl.queue[g].as<R>(),
+                            reinterpret_cast<uint32_t*>(pl.queue[g].as<unsigned char>() + (size_t)cap * QR_NR * sizeof(R)),
                             cap, pc[g] + 1};
         // slot states PS_FRESH and slot counts zero (adjacent arrays)
         SG_HIP(hipMemsetAsync(sl[g].u + (size_t)SU_STATE * Pg[g], 0, (size_t)Pg[g] * (SU_NU - SU_STATE) * sizeof(uint32_t),
@@ -926,8 +940,10 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
     const size_t eshm = staged_bytes<R>(ebs * (a.ni + 1));
     const uint64_t batch = std::min<uint64_t>(count, kSplitEmit);
     const size_t rsz = sizeof(R);
-    if (s->emitted.bytes < 7 * batch * rsz) s->emitted.alloc(7 * batch * rsz);
-    if (s->emitdraw.bytes < batch * sizeof(uint32_t)) s->emitdraw.alloc(batch * sizeof(uint32_t));
+    DevBuf& emitted = pool().emitted;
+    DevBuf& emitdraw = pool().emitdraw;
+    if (emitted.bytes < 7 * batch * rsz) emitted.alloc(7 * batch * rsz);
+    if (emitdraw.bytes < batch * sizeof(uint32_t)) emitdraw.alloc(batch * sizeof(uint32_t));
     unsigned long long* claims = s->next.as<unsigned long long>();
     hipEvent_t ev_start, ev_join, ev_end[2];
     SG_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
@@ -938,8 +954,8 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
         McArgs ab = a;
         ab.first = first + b;
         ab.count = std::min(batch, count - b);
-        ab.emitted = s->emitted.p;
-        ab.emit_draw = s->emitdraw.as<uint32_t>();
+        ab.emitted = emitted.p;
+        ab.emit_draw = emitdraw.as<uint32_t>();
         ab.next_packet = claims;
         // the previous batch is finished on both streams: emit on stream 0, then both groups start
         if (G == 2) {
@@ -948,7 +964,7 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
         }
         SG_HIP(hipMemsetAsync(claims, 0, sizeof(unsigned long long), st[0]));
         int emb = (int)std::min<uint64_t>((ab.count + 255) / 256, (uint64_t)g_cus * 16);
-        hipLaunchKernelGGL(k_emit<R>, dim3(emb), dim3(256), 0, st[0], ab, s->emitted.as<R>(), s->emitdraw.as<uint32_t>());
+        hipLaunchKernelGGL(k_emit<R>, dim3(emb), dim3(256), 0, st[0], ab, emitted.as<R>(), emitdraw.as<uint32_t>());
         SG_HIP(hipGetLastError());
         for (int g = 0; g < G && b > 0; g++)
             hipLaunchKernelGGL(k_revive, dim3((Pg[g] + 255) / 256), dim3(256), 0, st[0], sl[g].u + (size_t)SU_STATE * Pg[g],
@@ -1043,8 +1059,10 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
     const bool mixed = precision == SG_MIXED;
     const size_t rsz = mixed ? sizeof(float) : sizeof(double);
     const uint64_t batch = std::min<uint64_t>(count, (uint64_t)1 << 22);
-    if (s->emitted.bytes < 7 * batch * rsz) s->emitted.alloc(7 * batch * rsz);
-    if (s->emitdraw.bytes < batch * sizeof(uint32_t)) s->emitdraw.alloc(batch * sizeof(uint32_t));
+    if (g_pipeline == 0) {
+        if (s->emitted.bytes < 7 * batch * rsz) s->emitted.alloc(7 * batch * rsz);
+        if (s->emitdraw.bytes < batch * sizeof(uint32_t)) s->emitdraw.alloc(batch * sizeof(uint32_t));
+    }
     hipEvent_t e0, e1;
     SG_HIP(hipEventCreate(&e0));
     SG_HIP(hipEventCreate(&e1));
