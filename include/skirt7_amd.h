@@ -134,7 +134,9 @@ int sg_sim_destroy(sg_sim* s);
 int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, int precision,
                uint64_t* counters);
 /* Batched population (one persistent launch over many members, each its own simulation on a
- * grid of the same kind): member i runs packets [0, counts[i]) at wavelength ells[i]. */
+ * grid of the same kind): member i runs packets [0, counts[i]) at wavelength ells[i]. With the
+ * event/walk pipeline selected, members of at least sg_set_batch_alone packets run one at a time
+ * through it and the rest share the persistent launches; each member's result is its single run. */
 int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const uint64_t* counts, int precision,
                      uint64_t* counters);
 /* ---- FitSKIRT evaluation of a population ------------------------------------------------------ */
@@ -161,6 +163,9 @@ int sg_set_pipeline(int mode);
  * threshold (lanes that must finish their walk before the wave re-arms them; by default 32 for the
  * lane pipeline and 16 for the walk kernel). */
 int sg_set_tuning(int block, int waves_per_cu, int compact_lanes);
+/* Smallest member (packets) that sg_sim_run_batch runs alone through the event/walk pipeline
+ * instead of in the shared persistent launch (0 = never; default 2^21). */
+int sg_set_batch_alone(uint64_t min_packets);
 
 /* ---- setup helpers (the reference's own setup phase, made reproducible) ---------------------- */
 /* Normalised density of an exponential-disc geometry record laid out like a stellar component

@@ -23,6 +23,7 @@ static int g_cus = 256;
 // compaction threshold 0: each pipeline's own default (32 lanes for the lane pipeline, 16 for the walk kernel)
 static int g_tune_block = 256, g_tune_waves = 0, g_tune_compact = 0;
 static int g_pipeline = 1;   // 0: persistent lane pipeline (k_mc); 1: event and walk kernels (k_events/k_walks)
+static uint64_t g_batch_alone = (uint64_t)1 << 21;   // batched members this large run alone (0: never)
 
 static void ensure_init()
 {
@@ -187,6 +188,12 @@ int sg_set_pipeline(int mode)
         return -1;
     }
     g_pipeline = mode;
+    return 0;
+}
+
+int sg_set_batch_alone(uint64_t min_packets)
+{
+    g_batch_alone = min_packets;
     return 0;
 }
 
@@ -1230,6 +1237,13 @@ int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const u
     int blocks = 0;
     LaunchTimer lt;
     std::vector<uint64_t> done(nmembers, 0);
+    // large members: their own event/walk passes, which keep the walk kernel full on their own
+    for (int i = 0; g_pipeline == 1 && g_batch_alone > 0 && i < nmembers; i++) {
+        if (counts[i] < g_batch_alone) continue;
+        if (base[i].store_abs) dispatch_split<true>(sims[i], base[i], precision, 0, counts[i], lt, blocks);
+        else dispatch_split<false>(sims[i], base[i], precision, 0, counts[i], lt, blocks);
+        done[i] = counts[i];
+    }
     int j = 0;
     for (;;) {
         // the next round: the members' remaining packets in order, split where the round is full

@@ -566,6 +566,12 @@ def set_pipeline(mode):
     check(lib().sg_set_pipeline(int(mode)))
 
 
+def set_batch_alone(min_packets):
+    """Members of run_batch with at least this many packets run one at a time through the event and walk
+    kernels; the rest share the persistent batched launches (0: every member shares them)."""
+    check(lib().sg_set_batch_alone(int(min_packets)))
+
+
 def set_tuning(block=0, waves_per_cu=0, compact_lanes=0):
     check(lib().sg_set_tuning(block, waves_per_cu, compact_lanes))
 

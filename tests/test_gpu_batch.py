@@ -58,12 +58,38 @@ def test_batch_splits_members_across_rounds(pop):
     m0 = ms[0]
     m0["instr"].clear()
     ms[1]["instr"].clear()
-    S.run_batch([m["sim"] for m in ms], counts, precision=SG_MIXED)
+    S.set_batch_alone(0)      # every member in the shared launches
+    try:
+        S.run_batch([m["sim"] for m in ms], counts, precision=SG_MIXED)
+    finally:
+        S.set_batch_alone(1 << 21)
     g = m0["instr"].frame(0)[0].copy()
     m0["instr"].clear()
     m0["sim"].run_packets(0, 0, counts[0], counts[0], SG_MIXED)
     f = m0["instr"].frame(0)[0]
     assert np.allclose(g, f, rtol=1e-9, atol=1e-300)
+
+
+@pytest.mark.parametrize("precision", [SG_FP64, SG_MIXED])
+def test_batch_runs_large_members_alone(pop, precision):
+    """Members at or above the threshold run alone through the event/walk kernels, the others share the
+    batched launches: every member's frame and the summed counters are still those of single runs."""
+    ms = pop["members"]
+    counts = sorted(m["packages"] for m in ms)
+    threshold = counts[len(counts) // 2]
+    print("threshold", threshold, "counts", [m["packages"] for m in ms], flush=True)
+    single, tot = _single_runs(ms, precision)
+    for m in ms:
+        m["instr"].clear()
+    S.set_batch_alone(threshold)
+    try:
+        cb = S.run_batch([m["sim"] for m in ms], [m["packages"] for m in ms], precision=precision)
+    finally:
+        S.set_batch_alone(1 << 21)
+    for k in EXACT:
+        assert cb[k] == tot[k], k
+    for m, f in zip(ms, single):
+        assert np.allclose(m["instr"].frame(0)[0], f, rtol=1e-11, atol=1e-300)
 
 
 def test_batch_rejects_mixed_grid_kinds(pop):
