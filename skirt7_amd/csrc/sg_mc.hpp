@@ -230,6 +230,19 @@ template <class W> struct Hot {
         p.io(nseg); p.io(stage);
     }
 };
+// DustGridPath::pathlength (DustGridPath.cpp:162-173, NR.hpp:76-95) at the end of a propagation
+// re-walk: the distance at which the cumulative optical depth reaches the target, interpolated
+// linearly in the last segment; the path's full length when the target lies beyond its end, and 0
+// for an empty path or a target of zero.
+template <class W> __device__ __forceinline__ typename W::Real prop_length(const Hot<W>& h)
+{
+    using R = typename W::Real;
+    if (h.nseg == 0 || !(h.target > (R)0)) return (R)0;
+    if (h.tau > h.target || (h.tau == h.target && h.nseg > 1))
+        return interp_linlin(h.target, h.tprev, h.tau, h.sprev, h.s);
+    return h.s;
+}
+
 template <class W> __host__ __device__ constexpr int hot_words()
 {
     return W::kParkWords + (int)(6 * sizeof(typename W::Real) / 4) + 2;
@@ -432,8 +445,8 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
             break;
         }
         case ST_PROP_DONE: {
-            // PhotonPackage::propagate (PhotonPackage.cpp:93-96): h.s holds pathlength(tau)
-            R d = h.s;
+            // PhotonPackage::propagate (PhotonPackage.cpp:93-96) by pathlength(tau)
+            R d = prop_length(h);
             rx = rx + d * kx; ry = ry + d * ky; rz = rz + d * kz;
             peel = 0; peelkind = 1;
             h.stage = ST_PEEL;
@@ -507,25 +520,16 @@ __device__ __forceinline__ int crossing(Hot<W>& h, const typename W::View& g, co
     using R = typename W::Real;
     int m;
     R ds, rho;
-    bool got = h.w.next(g, lds, nlds, m, ds, rho);
-    if (!got) {
-        if (h.stage == ST_PROP) {
-            // DustGridPath::pathlength end cases (DustGridPath.cpp:162-173, NR.hpp:76-95)
-            R res;
-            if (h.nseg == 0 || !(h.target > (R)0)) res = 0;
-            else if (h.target > h.tau) res = h.s;
-            else if (h.nseg == 1) res = h.s;
-            else res = interp_linlin(h.target, h.tprev, h.tau, h.sprev, h.s);
-            h.s = res;
-            h.stage = ST_PROP_DONE;
-        } else if (h.stage == ST_OWN) h.stage = ST_OWN_DONE;
-        else h.stage = ST_PEELW_DONE;
-        return 0;
-    }
-    R kr = (R)a.kext * rho;        // KappaRho (DustSystem.cpp:459-485): 0 + kappaext*rho, 0 outside
-    R dtau = kr * ds;
+    bool got;
+    if constexpr (W::kStep) got = h.w.step(g, lds, nlds, m, ds, rho);
+    else got = h.w.next(g, lds, nlds, m, ds, rho);
+    // a segment of positive length (a single-step walker also returns zero-length steps along
+    // edges and through corners)
+    const bool seg = got && ds > (R)0;
+    const R dsv = seg ? ds : (R)0;
+    const R dtau = seg ? ((R)a.kext * rho) * ds : (R)0;   // KappaRho (DustSystem.cpp:459-485)
     const int st = h.stage;
-    if (ABS && st == ST_OWN && m >= 0) {
+    if (ABS && st == ST_OWN && seg && m >= 0) {
         // OligoDustSystem::absorb via simulateescapeandabsorption (MonteCarloSimulation.cpp:445-458)
         R expfactorm = -rexpm1<R>(-dtau);
         R Lintm = h.L0 * rexp<R>(-h.tau) * expfactorm;
@@ -534,19 +538,20 @@ __device__ __forceinline__ int crossing(Hot<W>& h, const typename W::View& g, co
     }
     // running sums, kept for every walk kind without branching (only the propagation re-walk reads
     // the path length and the previous segment's sums): cumulative s and tau per segment
-    h.tprev = h.tau; h.sprev = h.s;
-    h.s += ds;
+    h.tprev = seg ? h.tau : h.tprev;
+    h.sprev = seg ? h.s : h.sprev;
+    h.s += dsv;
     h.tau += dtau;
-    h.nseg++;
+    h.nseg += seg ? 1 : 0;
+    // the walk ends at the end of the path, for a peel-off beyond the optical-depth cut, and for the
+    // propagation re-walk at the first segment past the target optical depth (prop_length then
+    // interpolates the interaction point)
+    const bool end = !got || (st == ST_PEELW && a.tau_cut > 0.0 && h.tau > (R)a.tau_cut)
+                     || (st == ST_PROP && h.target > (R)0 && h.tau > h.target);
+    const int done = st == ST_PROP ? ST_PROP_DONE : st == ST_OWN ? ST_OWN_DONE : ST_PEELW_DONE;
+    h.stage = end ? done : st;
     const int kind = st == ST_PEELW ? 2 : st == ST_OWN ? 1 : 3;
-    if (st == ST_PEELW && a.tau_cut > 0.0 && h.tau > (R)a.tau_cut) h.stage = ST_PEELW_DONE;
-    if (st == ST_PROP && h.target > (R)0 && h.tau > h.target) {
-        // the propagation re-walk stops at the first segment past the target optical depth
-        h.s = (h.nseg == 1) ? interp_linlin(h.target, (R)0, h.tau, (R)0, h.s)
-                            : interp_linlin(h.target, h.tprev, h.tau, h.sprev, h.s);
-        h.stage = ST_PROP_DONE;
-    }
-    return m >= 0 ? kind : 0;
+    return seg && m >= 0 ? kind : 0;
 }
 
 __device__ __forceinline__ bool walking_stage(int st) { return st == ST_PEELW || st == ST_OWN || st == ST_PROP; }

@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_WAL
             sl.fr(SR_RES, idx) = h.tau;
             h.stage = ST_NEWPKT;
         } else if (h.stage == ST_PROP_DONE) {
-            sl.fr(SR_RES, idx) = h.s;
+            sl.fr(SR_RES, idx) = prop_length(h);
             if constexpr (W::kHint) {
                 // the leaf holding the interaction point: where the packet's next walks start
                 int n;

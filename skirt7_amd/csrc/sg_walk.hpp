@@ -80,6 +80,7 @@ struct CartW64 {
     static constexpr int kParkWords = 25;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     static constexpr bool kHint = false;
+    static constexpr bool kStep = false;   // next() only
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
@@ -197,6 +198,15 @@ template <typename T> __device__ __forceinline__ int choose_axis_tree(T dx, T dy
     return 2;
 }
 
+// One of three per-axis values by flags (fy: y, fz: z, neither: x). The values pass through an empty
+// register constraint first: a select between two loads of one structure is otherwise folded into a
+// load through a selected address, which keeps the whole walker in private (scratch) memory.
+template <typename T> __device__ __forceinline__ T pick3(bool fy, bool fz, T vx, T vy, T vz)
+{
+    asm("" : "+v"(vx), "+v"(vy), "+v"(vz));
+    return fz ? vz : fy ? vy : vx;
+}
+
 // True when the FP32 exit decision is ambiguous: the two smallest wall distances agree to 1e-5
 // relative (a corner or edge crossing), or the chosen distance d is within 1e-5 of zero relative
 // to the cell, measured along the exit axis (ka: direction component, wa: cell width on that axis).
@@ -204,7 +214,7 @@ template <typename T> __device__ __forceinline__ int choose_axis_tree(T dx, T dy
 __device__ __forceinline__ bool ambiguous(float dx, float dy, float dz, float d, float ka, float wa)
 {
     float mid = fmaxf(fminf(dx, dy), fminf(fmaxf(dx, dy), dz));
-    return (mid < FLT_MAX && (mid - d) <= 1e-5f * mid) || (d * fabsf(ka) <= 1e-5f * wa);
+    return ((mid < FLT_MAX) & ((mid - d) <= 1e-5f * mid)) | (d * fabsf(ka) <= 1e-5f * wa);
 }
 
 // Mixed-precision Cartesian walk.
@@ -220,6 +230,7 @@ struct CartW32 {
     static constexpr int kParkWords = 19;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     static constexpr bool kHint = false;
+    static constexpr bool kStep = false;   // next() only
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz); p.io(ix); p.io(iy); p.io(iz);
@@ -432,6 +443,7 @@ struct TreeW64 {
     static constexpr int kParkWords = 38;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     static constexpr bool kHint = false;
+    static constexpr bool kStep = false;   // next() only
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
@@ -570,10 +582,12 @@ struct TreeW32 {
     // leaf containing the current position and takes it over. Child choice by "<" against the split
     // point (OctTreeNode.cpp:184-189, BinTreeNode.cpp:326-335), except that a point lying exactly on a
     // split plane goes to the child the ray is heading into (a walk never steps back into a cell behind
-    // it; the reference instead steps a small distance past every wall).
+    // it; the reference instead steps a small distance past every wall). Per-axis choices are selects
+    // on flags, not indexed arrays, so that nothing goes to private memory.
     __device__ __forceinline__ void descend_loaded(const View& t, uint4 a, uint4 b, int n, int32_t i, int32_t j,
                                                    int32_t k)
     {
+        const bool nx = kx < 0.0f, ny = ky < 0.0f, nz = kz < 0.0f;
         int level = 0;
         while (!(a.x & (1u << 15))) {
             const uint32_t ns = a.x & 0x7fffu;
@@ -583,29 +597,25 @@ struct TreeW32 {
                 const float cx = t.extf[0] + (float)(2 * i + 1) * width(t, 0, ns + 1u);
                 const float cy = t.extf[1] + (float)(2 * j + 1) * width(t, 1, ns + (1u << 5));
                 const float cz = t.extf[2] + (float)(2 * k + 1) * width(t, 2, ns + (1u << 10));
-                const int qx = (kx < 0.0f ? x <= cx : x < cx) ? 0 : 1;
-                const int qy = (ky < 0.0f ? y <= cy : y < cy) ? 0 : 1;
-                const int qz = (kz < 0.0f ? z <= cz : z < cz) ? 0 : 1;
+                const int qx = (nx ? x <= cx : x < cx) ? 0 : 1;
+                const int qy = (ny ? y <= cy : y < cy) ? 0 : 1;
+                const int qz = (nz ? z <= cz : z < cz) ? 0 : 1;
                 n = c0 + qx + 2 * qy + 4 * qz;
                 i = 2 * i + qx; j = 2 * j + qy; k = 2 * k + qz;
             } else {
-                // binary node: split at the midpoint of its recorded axis
-                const int d = (int)((a.x >> 16) & 3u);
-                const uint32_t ns1 = ns + (1u << (5 * d));
-                int q;
-                if (d == 0) {
-                    const float c = t.extf[0] + (float)(2 * i + 1) * width(t, 0, ns1);
-                    q = (kx < 0.0f ? x <= c : x < c) ? 0 : 1;
-                    i = 2 * i + q;
-                } else if (d == 1) {
-                    const float c = t.extf[1] + (float)(2 * j + 1) * width(t, 1, ns1);
-                    q = (ky < 0.0f ? y <= c : y < c) ? 0 : 1;
-                    j = 2 * j + q;
-                } else {
-                    const float c = t.extf[2] + (float)(2 * k + 1) * width(t, 2, ns1);
-                    q = (kz < 0.0f ? z <= c : z < c) ? 0 : 1;
-                    k = 2 * k + q;
-                }
+                // binary node: split at the midpoint of its recorded axis (bits 16-17: 0 x, 1 y, 2 z)
+                const bool fy = (a.x >> 16) & 1u, fz = (a.x >> 17) & 1u;
+                const float e = pick3(fy, fz, t.extf[0], t.extf[1], t.extf[2]);
+                const float w0 = pick3(fy, fz, t.widthf[0], t.widthf[1], t.widthf[2]);
+                const int32_t kd = fz ? k : fy ? j : i;
+                const int nd = (int)((ns >> (fz ? 10 : fy ? 5 : 0)) & 31u) + 1;
+                const float p = pick3(fy, fz, x, y, z);
+                const bool neg = fz ? nz : fy ? ny : nx;
+                const float c = e + (float)(2 * kd + 1) * ldexpf(w0, -nd);
+                const int q = (neg ? p <= c : p < c) ? 0 : 1;
+                i = (fy || fz) ? i : 2 * i + q;
+                j = fy ? 2 * j + q : j;
+                k = fz ? 2 * k + q : k;
                 n = c0 + q;
             }
             load(t, n, a, b);
@@ -632,43 +642,49 @@ struct TreeW32 {
                                                 int32_t& i, int32_t& j, int32_t& k) const
     {
         const float* e = t.extf;
-        const float p[3] = {x, y, z};
-        const float kd[3] = {kx, ky, kz};
-        for (int a = 0; a < 3; a++) {
-            bool in = p[a] >= e[a] && p[a] <= e[a + 3];
-            if (a == xa) in = xdir > 0 ? (p[a] >= e[a] && p[a] < e[a + 3]) : (p[a] > e[a] && p[a] <= e[a + 3]);
-            if (!in) return -1;
-        }
-        int32_t kk[3] = {0, 0, 0};
+        // inside the grid: closed box, except half-open on the side of the wall just crossed
+        const bool ax = xa == 0, ay = xa == 1, az = xa == 2, up = xdir > 0;
+        const bool inx = x >= e[0] && x <= e[3] && !(ax && (up ? x >= e[3] : x <= e[0]));
+        const bool iny = y >= e[1] && y <= e[4] && !(ay && (up ? y >= e[4] : y <= e[1]));
+        const bool inz = z >= e[2] && z <= e[5] && !(az && (up ? z >= e[5] : z <= e[2]));
+        if (!(inx && iny && inz)) return -1;
+        const bool dir = xa >= 0;
+        const bool nx = dir && kx < 0.0f, ny = dir && ky < 0.0f, nz = dir && kz < 0.0f;
+        int32_t ki = 0, kj = 0, kk = 0;
         int id = 0;
         int32_t w = node_word(t, lds, nlds, 0);
         int level = 0;
+        int axis = 0;   // binary tree: split axis of this level (x, y, z in turn)
         while (w >= 0) {
             int q = 0;
             if (t.arity == 8) {
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    const float c = e[a] + (float)(2 * kk[a] + 1) * ldexpf(t.widthf[a], -(level + 1));
-                    const bool low = (xa >= 0 && kd[a] < 0.0f) ? (p[a] <= c) : (p[a] < c);
-                    kk[a] = 2 * kk[a] + (low ? 0 : 1);
-                    q += low ? 0 : 1 << a;
-                }
+                const float cx = e[0] + (float)(2 * ki + 1) * ldexpf(t.widthf[0], -(level + 1));
+                const float cy = e[1] + (float)(2 * kj + 1) * ldexpf(t.widthf[1], -(level + 1));
+                const float cz = e[2] + (float)(2 * kk + 1) * ldexpf(t.widthf[2], -(level + 1));
+                const int qx = (nx ? x <= cx : x < cx) ? 0 : 1;
+                const int qy = (ny ? y <= cy : y < cy) ? 0 : 1;
+                const int qz = (nz ? z <= cz : z < cz) ? 0 : 1;
+                ki = 2 * ki + qx; kj = 2 * kj + qy; kk = 2 * kk + qz;
+                q = qx + 2 * qy + 4 * qz;
             } else {
-                const int a = level % 3;
-#pragma unroll
-                for (int b = 0; b < 3; b++) {
-                    if (b != a) continue;
-                    const float c = e[b] + (float)(2 * kk[b] + 1) * ldexpf(t.widthf[b], -(level / 3 + 1));
-                    const bool low = (xa >= 0 && kd[b] < 0.0f) ? (p[b] <= c) : (p[b] < c);
-                    kk[b] = 2 * kk[b] + (low ? 0 : 1);
-                    q = low ? 0 : 1;
-                }
+                const bool fy = axis == 1, fz = axis == 2;
+                const float ea = pick3(fy, fz, e[0], e[1], e[2]);
+                const float wa = pick3(fy, fz, t.widthf[0], t.widthf[1], t.widthf[2]);
+                const int32_t ka = fz ? kk : fy ? kj : ki;
+                const float p = pick3(fy, fz, x, y, z);
+                const bool neg = fz ? nz : fy ? ny : nx;
+                const float c = ea + (float)(2 * ka + 1) * ldexpf(wa, -(level / 3 + 1));
+                q = (neg ? p <= c : p < c) ? 0 : 1;
+                ki = (fy || fz) ? ki : 2 * ki + q;
+                kj = fy ? 2 * kj + q : kj;
+                kk = fz ? 2 * kk + q : kk;
+                axis = fz ? 0 : axis + 1;
             }
             id = w + q;
             w = node_word(t, lds, nlds, id);
             level++;
         }
-        i = kk[0]; j = kk[1]; k = kk[2];
+        i = ki; j = kj; k = kk;
         return id;
     }
     __device__ __forceinline__ void push(float ds)
@@ -744,42 +760,50 @@ struct TreeW32 {
         if (n < 0) { done = true; return; }
         descend_from(t, n, i, j, k);
     }
-    // Chooses the exit wall and advances onto it; returns the wall (0..5) and the segment length.
-    __device__ __forceinline__ int exit_wall(const View& t, float& d)
+    // Chooses the exit wall and advances onto it; returns the wall (0..5) and the segment length, and
+    // the exit axis as two flags (bx: x, by: y, neither: z). The choices are written as selects on
+    // these flags rather than on an axis number, which the compiler would turn into branches or an
+    // indexed private array.
+    __device__ __forceinline__ int exit_wall(const View& t, float& d, bool& bx, bool& by)
     {
         const float wx = width(t, 0, nsub), wy = width(t, 1, nsub), wz = width(t, 2, nsub);
-        const float xn = t.extf[0] + (float)(kx < 0.0f ? ci : ci + 1) * wx;
-        const float yn = t.extf[1] + (float)(ky < 0.0f ? cj : cj + 1) * wy;
-        const float zn = t.extf[2] + (float)(kz < 0.0f ? ck : ck + 1) * wz;
-        float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
-        float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
-        float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
-        int a = choose_axis_tree(dsx, dsy, dsz);
-        d = a == 0 ? dsx : a == 1 ? dsy : dsz;
-        if (ambiguous(dsx, dsy, dsz, d, a == 0 ? kx : a == 1 ? ky : kz, a == 0 ? wx : a == 1 ? wy : wz)) {
+        const bool nx = kx < 0.0f, ny = ky < 0.0f, nz = kz < 0.0f;
+        const float xn = t.extf[0] + (float)(nx ? ci : ci + 1) * wx;
+        const float yn = t.extf[1] + (float)(ny ? cj : cj + 1) * wy;
+        const float zn = t.extf[2] + (float)(nz ? ck : ck + 1) * wz;
+        const float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
+        const float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
+        const float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
+        // choose_axis_tree: x on ties with y or z, then y on ties with z
+        bx = dsx <= dsy && dsx <= dsz;
+        by = !bx && dsy <= dsz;
+        d = bx ? dsx : by ? dsy : dsz;
+        const bool bz = !bx && !by;
+        if (ambiguous(dsx, dsy, dsz, d, pick3(by, bz, kx, ky, kz), pick3(by, bz, wx, wy, wz))) {
             nrecompute++;
-            double X = x, Y = y, Z = z;
-            double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
-            double ddy = iy != 0.0f ? ((double)yn - Y) / (double)ky : DBL_MAX;
-            double ddz = iz != 0.0f ? ((double)zn - Z) / (double)kz : DBL_MAX;
-            a = choose_axis_tree(ddx, ddy, ddz);
-            d = (float)(a == 0 ? ddx : a == 1 ? ddy : ddz);
+            const double X = x, Y = y, Z = z;
+            const double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
+            const double ddy = iy != 0.0f ? ((double)yn - Y) / (double)ky : DBL_MAX;
+            const double ddz = iz != 0.0f ? ((double)zn - Z) / (double)kz : DBL_MAX;
+            bx = ddx <= ddy && ddx <= ddz;
+            by = !bx && ddy <= ddz;
+            d = (float)(bx ? ddx : by ? ddy : ddz);
         }
-        float dd = fmaxf(d, 0.0f);
-        if (a == 0) { x = xn; y = fmaf(ky, dd, y); z = fmaf(kz, dd, z); }
-        else if (a == 1) { x = fmaf(kx, dd, x); y = yn; z = fmaf(kz, dd, z); }
-        else { x = fmaf(kx, dd, x); y = fmaf(ky, dd, y); z = zn; }
-        float dir = a == 0 ? kx : a == 1 ? ky : kz;
-        return 2 * a + (dir < 0.0f ? 0 : 1);
+        const float dd = fmaxf(d, 0.0f);
+        x = bx ? xn : fmaf(kx, dd, x);
+        y = by ? yn : fmaf(ky, dd, y);
+        z = !bx && !by ? zn : fmaf(kz, dd, z);
+        const bool neg = bx ? nx : by ? ny : nz;
+        return (bx ? 0 : by ? 2 : 4) + (neg ? 0 : 1);
     }
-    // Moves to the leaf beyond `wall` from the node r across it (or, when the position is not in r's
-    // box, by a directed root descent).
-    __device__ __forceinline__ void step_through(const View& t, const int32_t* lds, int nlds, int wall, int r)
+    // Moves to the leaf beyond the exit wall (axis flags bx, by; direction sign s) from the node r
+    // across it (or, when the position is not in r's box, by a directed root descent).
+    __device__ __forceinline__ void step_through(const View& t, const int32_t* lds, int nlds, int wall, bool bx,
+                                                 bool by, int r)
     {
         uint4 a, b;
         load(t, r, a, b);
         const uint32_t nsr = a.x & 0x7fffu;
-        const int ax = wall >> 1;
         const int s = (wall & 1) ? 1 : -1;
         const int dx = (int)(nsub & 31u) - (int)(nsr & 31u);
         const int dy = (int)((nsub >> 5) & 31u) - (int)((nsr >> 5) & 31u);
@@ -787,9 +811,9 @@ struct TreeW32 {
         bool in = dx >= 0 && dy >= 0 && dz >= 0;
         int32_t i = 0, j = 0, k = 0;
         if (in) {
-            i = (ci + (ax == 0 ? s : 0)) >> dx;
-            j = (cj + (ax == 1 ? s : 0)) >> dy;
-            k = (ck + (ax == 2 ? s : 0)) >> dz;
+            i = (ci + (bx ? s : 0)) >> dx;
+            j = (cj + (by ? s : 0)) >> dy;
+            k = (ck + (!bx && !by ? s : 0)) >> dz;
             const float wx = width(t, 0, nsr), wy = width(t, 1, nsr), wz = width(t, 2, nsr);
             const float lx = t.extf[0] + (float)i * wx, hx = t.extf[0] + (float)(i + 1) * wx;
             const float ly = t.extf[1] + (float)j * wy, hy = t.extf[1] + (float)(j + 1) * wy;
@@ -798,74 +822,92 @@ struct TreeW32 {
         }
         if (in) descend_loaded(t, a, b, r, i, j, k);
         else {
-            int n = descend_root(t, lds, nlds, ax, s, i, j, k);
+            int n = descend_root(t, lds, nlds, wall >> 1, s, i, j, k);
             if (n < 0) done = true;
             else descend_from(t, n, i, j, k);
         }
     }
-    __device__ __forceinline__ int rope_of(int wall) const
+    // The rope across the exit wall, selected by the axis flags and the wall's side.
+    __device__ __forceinline__ int rope_of(int wall, bool bx, bool by) const
     {
-        return wall == 0 ? rope[0] : wall == 1 ? rope[1] : wall == 2 ? rope[2] : wall == 3 ? rope[3]
-             : wall == 4 ? rope[4] : rope[5];
+        const bool hi = wall & 1;
+        int32_t r0 = rope[0], r1 = rope[1], r2 = rope[2], r3 = rope[3], r4 = rope[4], r5 = rope[5];
+        asm("" : "+v"(r0), "+v"(r1), "+v"(r2), "+v"(r3), "+v"(r4), "+v"(r5));
+        return bx ? (hi ? r1 : r0) : by ? (hi ? r3 : r2) : (hi ? r5 : r4);
     }
-    __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds, float& rh)
+    // One wall step: an entry segment, or the segment from the position to the next wall, whose
+    // length is zero (or below) when the position lies on an edge or corner of the cell; false at the
+    // end of the path. Transport loops take one step per iteration and skip segments with ds <= 0,
+    // so that no loop nests inside theirs.
+    static constexpr bool kStep = true;
+    __device__ __forceinline__ bool step(const View& t, const int32_t* lds, int nlds, int& mo, float& ds, float& rh)
     {
         if (ne) {
             ds = e0; e0 = e1; e1 = e2; ne--; mo = -1; rh = 0.0f;
             return true;
         }
-        while (!done) {
-            float d;
-            const int segm = m;
-            const float segrho = rho;
-            const int32_t si = ci, sj = cj, sk = ck;
-            const int wall = exit_wall(t, d);
-            const int r = rope_of(wall);
-            if (r < 0) done = true;   // a wall of the grid: the path leaves the grid
-            else {
-                step_through(t, lds, nlds, wall, r);
-                if (!done && m == segm) { nstuck++; done = true; }
-            }
-            if (++steps > SG_MAX_STEPS) done = true;
-            if (d > 0) {
-                mo = segm; ds = d; rh = segrho;
-                lm = segm; li = si; lj = sj; lk = sk;
-                return true;
-            }
+        if (done) return false;
+        float d;
+        const int segm = m;
+        const float segrho = rho;
+        const int32_t si = ci, sj = cj, sk = ck;
+        bool bx, by;
+        const int wall = exit_wall(t, d, bx, by);
+        const int r = rope_of(wall, bx, by);
+        if (r < 0) done = true;   // a wall of the grid: the path leaves the grid
+        else {
+            step_through(t, lds, nlds, wall, bx, by, r);
+            if (!done && m == segm) { nstuck++; done = true; }
         }
+        if (++steps > SG_MAX_STEPS) done = true;
+        mo = segm; ds = d; rh = segrho;
+        const bool seg = d > 0;
+        lm = seg ? segm : lm; li = seg ? si : li; lj = seg ? sj : lj; lk = seg ? sk : lk;
+        return true;
+    }
+    // The next segment of positive length.
+    __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds, float& rh)
+    {
+        while (step(t, lds, nlds, mo, ds, rh))
+            if (ds > 0) return true;
         return false;
     }
 };
 
 // Mixed-precision tree walk with TopDown semantics (every next cell by directed root descent).
 struct TreeW32TopDown : TreeW32 {
-    __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds,
+    __device__ __forceinline__ bool step(const View& t, const int32_t* lds, int nlds, int& mo, float& ds,
                                          float& rh)
     {
         if (ne) {
             ds = e0; e0 = e1; e1 = e2; ne--; mo = -1; rh = 0.0f;
             return true;
         }
-        while (!done) {
-            float d;
-            const int segm = m;
-            const float segrho = rho;
-            const int32_t si = ci, sj = cj, sk = ck;
-            const int wall = exit_wall(t, d);
-            int32_t i, j, k;
-            const int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1, i, j, k);
-            if (n < 0) done = true;
-            else {
-                descend_from(t, n, i, j, k);
-                if (m == segm) { nstuck++; done = true; }
-            }
-            if (++steps > SG_MAX_STEPS) done = true;
-            if (d > 0) {
-                mo = segm; ds = d; rh = segrho;
-                lm = segm; li = si; lj = sj; lk = sk;
-                return true;
-            }
+        if (done) return false;
+        float d;
+        const int segm = m;
+        const float segrho = rho;
+        const int32_t si = ci, sj = cj, sk = ck;
+        bool bx, by;
+        const int wall = exit_wall(t, d, bx, by);
+        int32_t i, j, k;
+        const int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1, i, j, k);
+        if (n < 0) done = true;
+        else {
+            descend_from(t, n, i, j, k);
+            if (m == segm) { nstuck++; done = true; }
         }
+        if (++steps > SG_MAX_STEPS) done = true;
+        mo = segm; ds = d; rh = segrho;
+        const bool seg = d > 0;
+        lm = seg ? segm : lm; li = seg ? si : li; lj = seg ? sj : lj; lk = seg ? sk : lk;
+        return true;
+    }
+    __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds,
+                                         float& rh)
+    {
+        while (step(t, lds, nlds, mo, ds, rh))
+            if (ds > 0) return true;
         return false;
     }
 };
@@ -919,6 +961,7 @@ template <typename R> struct VoroW {
     static constexpr int kParkWords = 24;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     static constexpr bool kHint = false;
+    static constexpr bool kStep = false;   // next() only
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
