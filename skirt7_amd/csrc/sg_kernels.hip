@@ -810,7 +810,7 @@ static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64
     a.counters = s->counters.as<unsigned long long>();
     a.next_packet = s->next.as<unsigned long long>();
     a.error = s->err.as<int>();
-    a.compact = g_tune_compact > 0 ? g_tune_compact : (g_pipeline == 1 ? 16 : 32);
+    a.compact = g_tune_compact > 0 ? g_tune_compact : (g_pipeline == 1 ? 24 : 32);
     if (!s->scene.p) s->scene.alloc(sizeof(SceneDev));
     SG_HIP(hipMemcpy(s->scene.p, &sd, sizeof sd, hipMemcpyHostToDevice));
     a.scene = s->scene.as<SceneDev>();

@@ -554,7 +554,12 @@ __device__ __forceinline__ int crossing(Hot<W>& h, const typename W::View& g, co
     return seg && m >= 0 ? kind : 0;
 }
 
-__device__ __forceinline__ bool walking_stage(int st) { return st == ST_PEELW || st == ST_OWN || st == ST_PROP; }
+// ST_PEELW, ST_OWN or ST_PROP, as a bit test (a chain of comparisons compiles to branches)
+__device__ __forceinline__ bool walking_stage(int st)
+{
+    static_assert(ST_PEELW < 32 && ST_OWN < 32 && ST_PROP < 32, "stage bits");
+    return ((1u << ST_PEELW | 1u << ST_OWN | 1u << ST_PROP) >> (uint32_t)st) & 1u;
+}
 
 __device__ __forceinline__ void add_wave(unsigned long long* dst, unsigned long long x)
 {

@@ -361,14 +361,17 @@ __device__ __forceinline__ bool chunk_claim(unsigned long long* ctr, uint64_t nq
     return ok && want;
 }
 
-// Occupancy target of the walk kernel (waves per SIMD).
+// Occupancy target of the walk kernel (waves per SIMD): five for the mixed tree walk, which fits
+// in 96 registers, four for the others.
 #ifndef SG_WALK_WAVES_PER_EU
 #define SG_WALK_WAVES_PER_EU 4
 #endif
+template <class W> struct WalkWaves { static constexpr int value = SG_WALK_WAVES_PER_EU; };
+template <> struct WalkWaves<TreeW32> { static constexpr int value = 5; };
 
 // Walk pass: persistent lanes over the requests [0, *q.tail).
 template <class W, bool ABS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_WALK_WAVES_PER_EU, 8))) k_walks(const typename W::View g, const McArgs a, WalkQueue<typename W::Real> q,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WalkWaves<W>::value, 8))) k_walks(const typename W::View g, const McArgs a, WalkQueue<typename W::Real> q,
                                               Slots<typename W::Real> sl, unsigned long long* claim, int nlds_req)
 {
     using R = typename W::Real;

@@ -574,9 +574,14 @@ struct TreeW32 {
     // Node record n as two 16-byte loads issued together: (code, val, rope0, rope1), (rope2..rope5).
     __device__ __forceinline__ static void load(const View& t, int n, uint4& a, uint4& b)
     {
-        const uint4* p = reinterpret_cast<const uint4*>(t.node32 + n);
-        a = p[0];
-        b = p[1];
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4* p = reinterpret_cast<const u32x4*>(t.node32 + n);
+        u32x4 va = p[0], vb = p[1];
+        // both halves at once: otherwise the compiler loads the first word alone and sinks the rest
+        // of the record into the branch that uses it, a second dependent memory access per step
+        asm("" : "+v"(va), "+v"(vb));
+        a = make_uint4(va.x, va.y, va.z, va.w);
+        b = make_uint4(vb.x, vb.y, vb.z, vb.w);
     }
     // Descends from node n (record a, b already loaded; indices i, j, k at its subdivision) to the
     // leaf containing the current position and takes it over. Child choice by "<" against the split
@@ -764,13 +769,14 @@ struct TreeW32 {
     // the exit axis as two flags (bx: x, by: y, neither: z). The choices are written as selects on
     // these flags rather than on an axis number, which the compiler would turn into branches or an
     // indexed private array.
-    __device__ __forceinline__ int exit_wall(const View& t, float& d, bool& bx, bool& by)
+    __device__ __forceinline__ int exit_wall(const View& t, float& d, bool& bx, bool& by, float& xn, float& yn,
+                                             float& zn)
     {
         const float wx = width(t, 0, nsub), wy = width(t, 1, nsub), wz = width(t, 2, nsub);
         const bool nx = kx < 0.0f, ny = ky < 0.0f, nz = kz < 0.0f;
-        const float xn = t.extf[0] + (float)(nx ? ci : ci + 1) * wx;
-        const float yn = t.extf[1] + (float)(ny ? cj : cj + 1) * wy;
-        const float zn = t.extf[2] + (float)(nz ? ck : ck + 1) * wz;
+        xn = t.extf[0] + (float)(nx ? ci : ci + 1) * wx;
+        yn = t.extf[1] + (float)(ny ? cj : cj + 1) * wy;
+        zn = t.extf[2] + (float)(nz ? ck : ck + 1) * wz;
         const float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
         const float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
         const float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
@@ -797,30 +803,29 @@ struct TreeW32 {
         return (bx ? 0 : by ? 2 : 4) + (neg ? 0 : 1);
     }
     // Moves to the leaf beyond the exit wall (axis flags bx, by; direction sign s) from the node r
-    // across it (or, when the position is not in r's box, by a directed root descent).
+    // across it, or by a directed root descent when the position is not on the leaf's face. The
+    // position lies on the exit wall and, having moved away from the entry walls, at or beyond the
+    // leaf's lower walls in the travel direction: it is on the face (and so in r's box, which covers
+    // the whole face) exactly when it has not passed the exit planes xn, yn, zn of the two other axes,
+    // which rounding can do by an ulp near an edge.
     __device__ __forceinline__ void step_through(const View& t, const int32_t* lds, int nlds, int wall, bool bx,
-                                                 bool by, int r)
+                                                 bool by, int r, float xn, float yn, float zn)
     {
         uint4 a, b;
         load(t, r, a, b);
         const uint32_t nsr = a.x & 0x7fffu;
+        const bool bz = !bx && !by;
         const int s = (wall & 1) ? 1 : -1;
         const int dx = (int)(nsub & 31u) - (int)(nsr & 31u);
         const int dy = (int)((nsub >> 5) & 31u) - (int)((nsr >> 5) & 31u);
         const int dz = (int)((nsub >> 10) & 31u) - (int)((nsr >> 10) & 31u);
-        bool in = dx >= 0 && dy >= 0 && dz >= 0;
-        int32_t i = 0, j = 0, k = 0;
-        if (in) {
-            i = (ci + (bx ? s : 0)) >> dx;
-            j = (cj + (by ? s : 0)) >> dy;
-            k = (ck + (!bx && !by ? s : 0)) >> dz;
-            const float wx = width(t, 0, nsr), wy = width(t, 1, nsr), wz = width(t, 2, nsr);
-            const float lx = t.extf[0] + (float)i * wx, hx = t.extf[0] + (float)(i + 1) * wx;
-            const float ly = t.extf[1] + (float)j * wy, hy = t.extf[1] + (float)(j + 1) * wy;
-            const float lz = t.extf[2] + (float)k * wz, hz = t.extf[2] + (float)(k + 1) * wz;
-            in = x >= lx && x <= hx && y >= ly && y <= hy && z >= lz && z <= hz;
-        }
-        if (in) descend_loaded(t, a, b, r, i, j, k);
+        const bool onx = bx | (kx < 0.0f ? x >= xn : x <= xn);
+        const bool ony = by | (ky < 0.0f ? y >= yn : y <= yn);
+        const bool onz = bz | (kz < 0.0f ? z >= zn : z <= zn);
+        int32_t i = (ci + (bx ? s : 0)) >> dx;
+        int32_t j = (cj + (by ? s : 0)) >> dy;
+        int32_t k = (ck + (bz ? s : 0)) >> dz;
+        if (((dx | dy | dz) >= 0) & onx & ony & onz) descend_loaded(t, a, b, r, i, j, k);
         else {
             int n = descend_root(t, lds, nlds, wall >> 1, s, i, j, k);
             if (n < 0) done = true;
@@ -852,11 +857,12 @@ struct TreeW32 {
         const float segrho = rho;
         const int32_t si = ci, sj = cj, sk = ck;
         bool bx, by;
-        const int wall = exit_wall(t, d, bx, by);
+        float xn, yn, zn;
+        const int wall = exit_wall(t, d, bx, by, xn, yn, zn);
         const int r = rope_of(wall, bx, by);
         if (r < 0) done = true;   // a wall of the grid: the path leaves the grid
         else {
-            step_through(t, lds, nlds, wall, bx, by, r);
+            step_through(t, lds, nlds, wall, bx, by, r, xn, yn, zn);
             if (!done && m == segm) { nstuck++; done = true; }
         }
         if (++steps > SG_MAX_STEPS) done = true;
@@ -889,7 +895,8 @@ struct TreeW32TopDown : TreeW32 {
         const float segrho = rho;
         const int32_t si = ci, sj = cj, sk = ck;
         bool bx, by;
-        const int wall = exit_wall(t, d, bx, by);
+        float xn, yn, zn;
+        const int wall = exit_wall(t, d, bx, by, xn, yn, zn);
         int32_t i, j, k;
         const int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1, i, j, k);
         if (n < 0) done = true;
