@@ -233,12 +233,15 @@ def main():
                           desc, ncell, per, len(w["instr"].instruments), w["instr"].instruments[0].Nx),
                       "packets_per_step": per, "cells": ncell, "instruments": len(w["instr"].instruments),
                       "pixels": w["instr"].instruments[0].Nx}}
-    # algorithmic bytes of the transport kernel over its measured device time (all launches)
+    # algorithmic bytes of the walk kernel per launch over its average launch duration (HIP events on the
+    # launch's own stream, summed by the library); the walk passes of the two slot groups overlap, so the
+    # whole transport's rate over its device span is reported beside it
     bytes_launch = cross * BYTES_PER_CROSSING[gkind]["mixed"] + cm["peeloffs"] * BYTES_PER_PEELOFF
     if w.get("store_abs"):
         bytes_launch += cm["own_crossings"] * BYTES_PER_ABSORPTION
-    achieved = bytes_launch / kmix
     launches = max(1, cm["transport_launches"])
+    kwalk = cm["walk_kernel_us"] * 1e-6
+    achieved = bytes_launch / kwalk
     traffic = None
     try:
         with open(TRAFFIC_FILE) as f:
@@ -250,9 +253,12 @@ def main():
         traffic = None
     out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK, "traffic": traffic,
-                       "algorithmic_bytes_per_launch": bytes_launch / launches}
+                       "algorithmic_bytes_per_launch": bytes_launch / launches,
+                       "kernel": "k_walks" if a.pipeline == 1 else "k_mc",
+                       "kernel_ms_per_launch": 1e3 * kwalk / launches,
+                       "achieved_over_transport_span": bytes_launch / kmix / 1e9}
     out["crossings_per_s"] = cross / kmix
-    out["transport_kernel_ms_per_launch"] = 1e3 * kmix / max(1, cm["transport_launches"])
+    out["transport_span_ms_per_launch"] = 1e3 * kmix / launches
     out["counters"] = cm
 
     if not a.no_fp64:

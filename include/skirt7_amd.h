@@ -38,7 +38,7 @@ enum { SG_TOPDOWN = 0, SG_NEIGHBOR = 1 };   /* TreeDustGrid::SearchMethod (TreeD
 #define SG_STAR_NPARAM 16
 #define SG_INSTR_NPARAM 10
 #define SG_MC_NPARAM 8
-#define SG_NCOUNTERS 14
+#define SG_NCOUNTERS 15
 
 /* ---- library ---------------------------------------------------------------------------- */
 const char* sg_last_error(void);
@@ -130,7 +130,9 @@ int sg_sim_destroy(sg_sim* s);
  *   [0] packets [1] own-path crossings [2] peel-off crossings [3] scattering events
  *   [4] peel-offs detected [5] stuck-path escapes [6] npp [7] walks [8] re-walk crossings
  *   [9] FP64 wall recomputes [10] device microseconds (emission and transport kernels)
- *   [11] transport workgroups per launch [12] transport-kernel microseconds [13] transport launches */
+ *   [11] transport workgroups per launch [12] transport microseconds (device span of the transport
+ *   passes) [13] transport (walk-kernel) launches [14] walk-kernel microseconds: the sum over those
+ *   launches of each launch's own duration (HIP events on its stream; launches on two streams overlap) */
 int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, int precision,
                uint64_t* counters);
 /* Batched population (one persistent launch over many members, each its own simulation on a

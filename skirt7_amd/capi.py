@@ -18,11 +18,11 @@ SG_NEIGHBOR = 1
 SG_STAR_NPARAM = 16
 SG_INSTR_NPARAM = 10
 SG_MC_NPARAM = 8
-SG_NCOUNTERS = 14
+SG_NCOUNTERS = 15
 
 COUNTER_NAMES = ("packets", "own_crossings", "peel_crossings", "scatterings", "peeloffs", "stuck", "npp",
                  "walks", "rewalk_crossings", "fp64_recomputes", "kernel_us", "blocks", "transport_us",
-                 "transport_launches")
+                 "transport_launches", "walk_kernel_us")
 
 _lib = None
 

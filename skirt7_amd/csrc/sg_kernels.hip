@@ -718,8 +718,18 @@ int sg_sim_destroy(sg_sim* s)
 
 // Device time of a series of transport launches: an event pair around each launch on the default
 // stream, summed once the stream has drained.
+// Device time of a run's transport: the span of its passes (start/stop on stream 0, or a span
+// measured elsewhere) and, per launch of the walk kernel, that launch's own duration between HIP
+// events recorded on its stream just before and after it. The per-launch events come from a
+// process-wide pool that is reused from call to call (one host thread drives the library).
 struct LaunchTimer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    static std::vector<std::pair<hipEvent_t, hipEvent_t>>& kpool()
+    {
+        static std::vector<std::pair<hipEvent_t, hipEvent_t>> p;
+        return p;
+    }
+    size_t kused = 0;
     void start()
     {
         hipEvent_t a, b;
@@ -731,6 +741,19 @@ struct LaunchTimer {
     void stop() { SG_HIP(hipEventRecord(ev.back().second, 0)); }
     // a span measured elsewhere that covered `launches` launches
     void add(double ms, int launches) { extra_ms += ms; extra_launches += launches; }
+    // brackets one walk-kernel launch on stream s
+    void kbefore(hipStream_t s)
+    {
+        auto& p = kpool();
+        if (kused == p.size()) {
+            hipEvent_t a, b;
+            SG_HIP(hipEventCreate(&a));
+            SG_HIP(hipEventCreate(&b));
+            p.push_back({a, b});
+        }
+        SG_HIP(hipEventRecord(p[kused].first, s));
+    }
+    void kafter(hipStream_t s) { SG_HIP(hipEventRecord(kpool()[kused++].second, s)); }
     double extra_ms = 0;
     int extra_launches = 0;
     int launches() const { return (int)ev.size() + extra_launches; }
@@ -741,6 +764,19 @@ struct LaunchTimer {
             float ms = 0;
             SG_HIP(hipEventSynchronize(p.second));
             SG_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+            t += ms;
+        }
+        return t;
+    }
+    // summed per-launch durations: the bracketed launches and the start/stop spans (one kernel each)
+    double kernel_ms()
+    {
+        double t = total_ms() - extra_ms;
+        auto& p = kpool();
+        for (size_t i = 0; i < kused; i++) {
+            float ms = 0;
+            SG_HIP(hipEventSynchronize(p[i].second));
+            SG_HIP(hipEventElapsedTime(&ms, p[i].first, p[i].second));
             t += ms;
         }
         return t;
@@ -988,9 +1024,11 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
                     hipLaunchKernelGGL(k_events<R>, dim3((Pg[g] + ebs - 1) / ebs), dim3(ebs), eshm, st[g], ab, sl[g], q[g]);
                     SG_HIP(hipGetLastError());
                     SG_HIP(hipMemsetAsync(pc[g] + 3, 0, sizeof(unsigned long long), st[g]));
+                    lt.kbefore(st[g]);
                     hipLaunchKernelGGL((k_walks<W, ABS>), dim3(blocks), dim3(bs), shm, st[g], v, ab, q[g], sl[g], pc[g] + 3,
                                        nlds);
                     SG_HIP(hipGetLastError());
+                    lt.kafter(st[g]);
                     walk_launches++;
                 }
             // progress after the last pass of each group: packets claimed, walks each group queued
@@ -1114,6 +1152,7 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
         counters[11] = blocks;
         counters[12] = (uint64_t)(lt.total_ms() * 1000.0);
         counters[13] = lt.launches();
+        counters[14] = (uint64_t)(lt.kernel_ms() * 1000.0);
     }
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH
@@ -1294,6 +1333,7 @@ int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const u
         counters[11] = blocks;
         counters[12] = (uint64_t)(lt.total_ms() * 1000.0);
         counters[13] = lt.launches();
+        counters[14] = (uint64_t)(lt.kernel_ms() * 1000.0);
     }
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH
