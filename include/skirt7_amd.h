@@ -197,6 +197,15 @@ int sg_grid_sample_density(sg_grid* g, const double* geom16, double scale, int n
 int sg_voronoi_neighbors(const double ext[6], const double* sites, int64_t n, int64_t* offsets, int32_t* ids,
                          int64_t cap, int64_t* nids);
 
+/* ---- output --------------------------------------------------------------------------------- */
+/* FITSInOut::write (FITSInOut.cpp:32-95): one primary image of 32-bit floats, nx x ny (x nz frames when
+ * nz > 1) from data[x + nx*y + nx*ny*z], with BSCALE, BZERO, DATE, ORIGIN, BUNIT and the CRPIXn /
+ * CRVALn / CDELTn / CTYPEn keywords (reference pixels at the image centre, increments incx, incy,
+ * centre values xc, yc in xyunits). stamp: the DATE value ("yyyy-mm-ddThh:mm:ss"), or null for
+ * the current UTC time. An existing file is replaced. */
+int sg_fits_write(const char* path, const double* data, int nx, int ny, int nz, double incx, double incy, double xc,
+                  double yc, const char* dataunits, const char* xyunits, const char* stamp);
+
 #ifdef __cplusplus
 }
 #endif

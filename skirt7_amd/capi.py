@@ -50,6 +50,9 @@ _SIGS = {
     "sg_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "sg_set_pipeline": (ctypes.c_int, [ctypes.c_int]),
     "sg_set_batch_alone": (ctypes.c_int, [ctypes.c_uint64]),
+    "sg_fits_write": (ctypes.c_int, [ctypes.c_char_p, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
+                                     ctypes.c_char_p]),
     "sg_grid_create_cartesian": (ctypes.c_int, [_D, _D, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _VPP]),
     "sg_grid_create_tree": (ctypes.c_int, [ctypes.c_int, _D, _U8, ctypes.c_int64, ctypes.c_int, _VPP]),
     "sg_grid_create_voronoi": (ctypes.c_int, [_D, _D, ctypes.c_int64, _I64, _I32, _VPP]),

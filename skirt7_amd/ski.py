@@ -1,0 +1,411 @@
+"""Ski-file front end: runs an unchanged SKIRT7 parameter file (.ski) of the supported classes on the GPU
+and writes the frame instruments' FITS files as SKIRT7 does.
+
+A ski file is an XML hierarchy (doc/Part 1 - User Guide/SKIRT/ski files.txt): an element with a
+capitalised name is an object of that class, its attributes are the object's scalar properties, and a
+child element with a lower-case name is a compound property holding the object(s) nested in it. Scalar
+values follow Discover's property handlers: a number with an optional unit, a unitless value of a
+physical quantity taken in the units system's default unit (DoublePropertyHandler.cpp:169-199),
+comma-separated lists (DoubleListPropertyHandler.cpp:96-108), booleans true/t/yes/y/1
+(BoolPropertyHandler.cpp:78-83). Property names and defaults are those of the classes' Q_CLASSINFO
+declarations.
+
+Supported classes (anything else raises FatalError naming the class):
+  OligoMonteCarloSimulation (+ MonteCarloSimulation, Simulation properties), SIUnits,
+  ExtragalacticUnits, Random, OligoWavelengthGrid, StellarSystem, OligoStellarComp, ExpDiskGeometry,
+  SersicGeometry, SpiralStructureGeometryDecorator, OligoDustSystem, CompDustDistribution (one DustComp),
+  DustComp, SimpleOligoDustMix, FaceOnDustCompNormalization, DustMassDustCompNormalization,
+  CartesianDustGrid with LinMesh, OctTreeDustGrid, BinTreeDustGrid (mass-fraction criterion),
+  VoronoiDustGrid (sites from the dust density or uniform), InstrumentSystem, FrameInstrument.
+
+The transport works in parsecs (lengths) and watts (luminosities); outputs are calibrated by
+output.calibrate_frame and written by output.write_frame as <prefix>_<instrument>_total.fits.
+"""
+import os
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+from . import output
+from . import skirt as S
+from .capi import SG_MIXED, SG_NEIGHBOR, SG_TOPDOWN
+from .output import PC, FatalError, Units
+
+
+class Item:
+    """One object of the hierarchy: class name, scalar attributes, compound properties."""
+
+    def __init__(self, cls, attrs, props):
+        self.cls, self.attrs, self.props = cls, attrs, props
+
+    def one(self, name, default_cls=None):
+        items = self.props.get(name)
+        if not items:
+            if default_cls is None:
+                return None
+            return Item(default_cls, {}, {})
+        if len(items) != 1:
+            raise FatalError("property %s of %s holds %d objects, expected one" % (name, self.cls, len(items)))
+        return items[0]
+
+    def many(self, name):
+        return list(self.props.get(name, []))
+
+
+def _parse(elem):
+    props = {}
+    for child in elem:
+        if not isinstance(child.tag, str):
+            continue
+        if child.tag[:1].islower():
+            props[child.tag] = [_parse(c) for c in child if isinstance(c.tag, str)]
+        else:
+            raise FatalError("element %s inside %s is not a property" % (child.tag, elem.tag))
+    return Item(elem.tag, dict(elem.attrib), props)
+
+
+def parse(text):
+    """The simulation object of a ski document (text)."""
+    root = ET.fromstring(text)
+    if root.tag != "skirt-simulation-hierarchy":
+        raise FatalError("not a ski file: root element is " + root.tag)
+    sims = [c for c in root if isinstance(c.tag, str)]
+    if len(sims) != 1:
+        raise FatalError("a ski file holds exactly one simulation")
+    return _parse(sims[0])
+
+
+class Values:
+    """Scalar property conversion in one units system."""
+
+    def __init__(self, units):
+        self.units = units
+
+    def double(self, item, name, qty=None, default=None):
+        s = item.attrs.get(name)
+        if s is None:
+            if default is None:
+                raise FatalError("%s requires property %s" % (item.cls, name))
+            s = default
+        return self.convert(s, qty, item.cls, name)
+
+    def convert(self, s, qty, cls="", name=""):
+        seg = s.split()
+        if not seg or len(seg) > 2:
+            raise FatalError("invalid value '%s' for %s.%s" % (s, cls, name))
+        try:
+            v = float(seg[0])
+        except ValueError:
+            raise FatalError("invalid number '%s' for %s.%s" % (s, cls, name))
+        if qty is None:
+            if len(seg) != 1:
+                raise FatalError("dimensionless %s.%s has a unit: '%s'" % (cls, name, s))
+            return v
+        return self.units.in_(qty, v, seg[1] if len(seg) == 2 else None)
+
+    def doubles(self, item, name, qty=None, default=None):
+        s = item.attrs.get(name, default)
+        if s is None:
+            raise FatalError("%s requires property %s" % (item.cls, name))
+        return [self.convert(x, qty, item.cls, name) for x in s.split(",")]
+
+    @staticmethod
+    def integer(item, name, default=None):
+        s = item.attrs.get(name, default)
+        if s is None:
+            raise FatalError("%s requires property %s" % (item.cls, name))
+        return int(float(s))
+
+    @staticmethod
+    def boolean(item, name, default="false"):
+        return item.attrs.get(name, default).strip().lower() in ("true", "t", "yes", "y", "1")
+
+    @staticmethod
+    def string(item, name, default=None):
+        s = item.attrs.get(name, default)
+        if s is None:
+            raise FatalError("%s requires property %s" % (item.cls, name))
+        return s
+
+
+def _require(item, *classes):
+    if item.cls not in classes:
+        raise FatalError("class %s is not supported here (supported: %s)" % (item.cls, ", ".join(classes)))
+
+
+def read_sun_sed(path):
+    """The solar SED resource (SunSED.dat: a comment line, the count, then lambda in micron and L_lambda in
+    W/micron per line), as lambda in m and L_lambda in W/m (OligoStellarComp.cpp:39-58)."""
+    with open(path) as f:
+        f.readline()
+        tok = f.read().split()
+    n = int(tok[0])
+    vals = np.array([float(t) for t in tok[1:1 + 2 * n]]).reshape(n, 2)
+    return vals[:, 0] / 1e6, vals[:, 1] * 1e6
+
+
+def sun_luminosity(lam, sed):
+    """Solar L_lambda (W/m) at lam (m): NR::locate_fail + linear interpolation (OligoStellarComp.cpp:63-72)."""
+    lv, Lv = sed
+    if lam < lv[0] or lam > lv[-1]:
+        raise FatalError("The sun does not emit at the wavelength of the simulation (lambda = %g m)" % lam)
+    k = int(np.searchsorted(lv, lam, side="right")) - 1
+    k = min(k, len(lv) - 2)
+    return Lv[k] + (lam - lv[k]) / (lv[k + 1] - lv[k]) * (Lv[k + 1] - Lv[k])
+
+
+def _geometry(item, val):
+    """A geometry, lengths in pc."""
+    if item is None:
+        raise FatalError("a geometry is required")
+    if item.cls == "ExpDiskGeometry":
+        return S.ExpDiskGeometry(val.double(item, "radialScale", "length") / PC,
+                                 val.double(item, "axialScale", "length") / PC,
+                                 val.double(item, "radialTrunc", "length", "0") / PC,
+                                 val.double(item, "axialTrunc", "length", "0") / PC,
+                                 val.double(item, "innerRadius", "length", "0") / PC)
+    if item.cls == "SersicGeometry":
+        return S.SersicGeometry(val.double(item, "index", None, "1"), val.double(item, "radius", "length") / PC)
+    if item.cls == "SpiralStructureGeometryDecorator":
+        inner = _geometry(item.one("geometry"), val)
+        return S.SpiralStructureGeometryDecorator(
+            inner, arms=val.integer(item, "arms", "1"), pitch=val.double(item, "pitch", "posangle", "10 deg"),
+            radius=val.double(item, "radius", "length") / PC, phase=val.double(item, "phase", "posangle", "0 deg"),
+            perturbWeight=val.double(item, "perturbWeight"), index=val.integer(item, "index", "1"))
+    _require(item, "ExpDiskGeometry", "SersicGeometry", "SpiralStructureGeometryDecorator")
+
+
+def _box(item, val):
+    return np.array([val.double(item, k, "length") / PC for k in ("minX", "minY", "minZ", "maxX", "maxY", "maxZ")])
+
+
+class Spec:
+    """A ski file's parameters, converted and checked on the host: everything a run needs, with
+    lengths in pc (transport) and SI (calibration), luminosities in W per wavelength bin."""
+
+    def __init__(self, text, resources=None):
+        sim = parse(text)
+        _require(sim, "OligoMonteCarloSimulation")
+        uitem = sim.one("units", "ExtragalacticUnits")
+        _require(uitem, "ExtragalacticUnits", "SIUnits")
+        self.units = Units(uitem.cls, uitem.attrs.get("fluxOutputStyle", "Neutral"))
+        val = self.val = Values(self.units)
+
+        # MonteCarloSimulation.hpp / Simulation.hpp properties
+        self.packages = val.double(sim, "packages", None, "1e6")
+        self.minWeightReduction = val.double(sim, "minWeightReduction", None, "1e4")
+        self.minScattEvents = val.integer(sim, "minScattEvents", "0")
+        self.scattBias = val.double(sim, "scattBias", None, "0.5")
+        if val.boolean(sim, "continuousScattering", "no"):
+            raise FatalError("continuous scattering is not supported")
+        rnd = sim.one("random", "Random")
+        _require(rnd, "Random")
+        self.seed = val.integer(rnd, "seed", "4357")
+
+        # OligoWavelengthGrid: sorted wavelengths, bin widths 0.001 lambda (OligoWavelengthGrid.cpp:20-36)
+        wg = sim.one("wavelengthGrid", "OligoWavelengthGrid")
+        _require(wg, "OligoWavelengthGrid")
+        self.lambdas = np.sort(np.array(val.doubles(wg, "wavelengths", "wavelength")))
+        self.dlambdas = 0.001 * self.lambdas
+        nl = self.nlambda = len(self.lambdas)
+
+        # StellarSystem of OligoStellarComps: luminosities in solar units at each wavelength -> W per bin
+        ss = sim.one("stellarSystem", "StellarSystem")
+        _require(ss, "StellarSystem")
+        self.emissionBias = val.double(ss, "emissionBias", None, "0.5")
+        sed = None
+        self.stellar = []
+        for c in ss.many("components"):
+            _require(c, "OligoStellarComp")
+            lum = val.doubles(c, "luminosities")
+            if len(lum) != nl:
+                raise FatalError("The number of stellar component luminosities differs from the number of wavelengths")
+            if sed is None:
+                sed = read_sun_sed(_resource(resources, os.path.join("SED", "Sun", "SunSED.dat")))
+            Lw = [lum[ell] * sun_luminosity(self.lambdas[ell], sed) * self.dlambdas[ell] for ell in range(nl)]
+            self.stellar.append((_geometry(c.one("geometry"), val), Lw))
+        if not self.stellar:
+            raise FatalError("the stellar system has no components")
+
+        # frame instruments (SI values)
+        isys = sim.one("instrumentSystem", "InstrumentSystem")
+        _require(isys, "InstrumentSystem")
+        self.frames = []
+        for ins in isys.many("instruments"):
+            _require(ins, "FrameInstrument")
+            self.frames.append(dict(
+                name=val.string(ins, "instrumentName"), distance=val.double(ins, "distance", "distance"),
+                inclination=val.double(ins, "inclination", "posangle", "0"),
+                azimuth=val.double(ins, "azimuth", "posangle", "0"),
+                positionAngle=val.double(ins, "positionAngle", "posangle", "0"),
+                pixelsX=val.integer(ins, "pixelsX", "250"), pixelsY=val.integer(ins, "pixelsY", "250"),
+                fovX=val.double(ins, "fieldOfViewX", "length"), fovY=val.double(ins, "fieldOfViewY", "length"),
+                centerX=val.double(ins, "centerX", "length", "0"), centerY=val.double(ins, "centerY", "length", "0")))
+
+        # dust system (optional)
+        ds = sim.one("dustSystem")
+        self.dust = None if ds is None else self._dust(ds, nl)
+
+    def _dust(self, ds, nl):
+        val = self.val
+        _require(ds, "OligoDustSystem")
+        dd = ds.one("dustDistribution", "CompDustDistribution")
+        _require(dd, "CompDustDistribution")
+        comps = dd.many("components")
+        if len(comps) != 1:
+            raise FatalError("exactly one dust component is supported (found %d)" % len(comps))
+        dc = comps[0]
+        _require(dc, "DustComp")
+        d = dict(geometry=_geometry(dc.one("geometry"), val))
+        mix = dc.one("mix")
+        if mix is None:
+            raise FatalError("DustComp requires a mix")
+        _require(mix, "SimpleOligoDustMix")
+        d["opacities"] = val.doubles(mix, "opacities", "opacity")
+        d["albedos"] = val.doubles(mix, "albedos")
+        d["asymmetry"] = val.doubles(mix, "asymmetryParameters", None, ",".join(["0"] * nl))
+        if not (len(d["opacities"]) == len(d["albedos"]) == len(d["asymmetry"]) == nl):
+            raise FatalError("the dust mix needs one opacity, albedo and asymmetry parameter per wavelength")
+        norm = dc.one("normalization")
+        if norm is None:
+            raise FatalError("DustComp requires a normalization")
+        if norm.cls == "FaceOnDustCompNormalization":
+            lam0 = val.double(norm, "wavelength", "wavelength")
+            ells = [ell for ell in range(nl) if abs(lam0 / self.lambdas[ell] - 1.0) < 1e-5]
+            if not ells:
+                raise FatalError("Optical properties are not defined for this wavelength")
+            # FaceOnDustCompNormalization::normalizationFactor (FaceOnDustCompNormalization.cpp:67-74), in pc
+            d["scale"] = S.faceon_normalization(d["geometry"], val.double(norm, "opticalDepth"),
+                                                d["opacities"][ells[0]])
+        elif norm.cls == "DustMassDustCompNormalization":
+            # DustMassDustCompNormalization.cpp:47-51: the dust mass (kg) times the geometry's unit-mass
+            # density, which is per pc^3 here while the opacity stays in m2/kg
+            d["scale"] = val.double(norm, "dustMass", "mass") / (PC * PC)
+        else:
+            _require(norm, "FaceOnDustCompNormalization", "DustMassDustCompNormalization")
+        d["sampleCount"] = val.integer(ds, "sampleCount", "100")
+        d["writeMeanIntensity"] = val.boolean(ds, "writeMeanIntensity", "no")
+        d["grid"] = self._grid(ds.one("dustGrid", "OctTreeDustGrid"))
+        return d
+
+    def _grid(self, g):
+        val = self.val
+        spec = dict(kind=g.cls, extent=_box(g, val))
+        if g.cls == "CartesianDustGrid":
+            bins = []
+            for name in ("meshX", "meshY", "meshZ"):
+                m = g.one(name, "LinMesh")
+                _require(m, "LinMesh")
+                bins.append(val.integer(m, "numBins", "100"))
+            spec["bins"] = bins
+        elif g.cls in ("OctTreeDustGrid", "BinTreeDustGrid"):
+            for crit in ("maxOpticalDepth", "maxDensDispFraction"):
+                if val.double(g, crit, None, "0") != 0.0:
+                    raise FatalError("tree subdivision criterion %s is not supported" % crit)
+            if g.cls == "BinTreeDustGrid" and val.string(g, "directionMethod", "Alternating") != "Alternating":
+                raise FatalError("only the Alternating direction method is supported")
+            if g.cls == "OctTreeDustGrid" and val.boolean(g, "barycentric", "no"):
+                raise FatalError("barycentric octrees are not supported")
+            method = val.string(g, "searchMethod", "Neighbor")
+            if method not in ("Neighbor", "TopDown"):
+                raise FatalError("search method %s is not supported" % method)
+            spec.update(minLevel=val.integer(g, "minLevel", "2"), maxLevel=val.integer(g, "maxLevel", "6"),
+                        maxMassFraction=val.double(g, "maxMassFraction", None, "1e-6"),
+                        sampleCount=val.integer(g, "sampleCount", "100"), search=method)
+        elif g.cls == "VoronoiDustGrid":
+            spec.update(numParticles=val.integer(g, "numParticles", "500"),
+                        distribution=val.string(g, "distribution", "DustDensity"))
+            if spec["distribution"] not in ("DustDensity", "Uniform"):
+                raise FatalError("Voronoi site distribution %s is not supported" % spec["distribution"])
+        else:
+            _require(g, "CartesianDustGrid", "OctTreeDustGrid", "BinTreeDustGrid", "VoronoiDustGrid")
+        return spec
+
+
+def _resource(resources, name):
+    base = resources or os.environ.get("SKIRT7_AMD_RESOURCES")
+    if not base:
+        raise FatalError("resource " + name + " needed: pass resources= (SKIRT7's dat directory) or set "
+                         "SKIRT7_AMD_RESOURCES")
+    path = os.path.join(base, name)
+    if not os.path.exists(path):
+        raise FatalError("Could not open the data file " + path)
+    return path
+
+
+class Simulation:
+    """A ski file's objects on the GPU: grid, dust and stellar systems, instruments and the simulation."""
+
+    def __init__(self, spec, prefix="ski"):
+        self.spec, self.prefix = spec, prefix
+        sp = spec
+        comps = [S.OligoStellarComp(geom, lum) for geom, lum in sp.stellar]
+        self.stars = S.StellarSystem(comps, emissionBias=sp.emissionBias)
+        self.instr = S.InstrumentSystem(
+            [S.FrameInstrument(f["distance"] / PC, f["inclination"], f["azimuth"], f["positionAngle"], f["pixelsX"],
+                               f["pixelsY"], f["fovX"] / PC, f["fovY"] / PC, f["centerX"] / PC, f["centerY"] / PC)
+             for f in sp.frames], sp.nlambda)
+        self.grid = self.dust = self.rho = self.sites = None
+        if sp.dust is not None:
+            d = sp.dust
+            self.grid = self._grid(d["grid"], d["geometry"], d["scale"], sp.seed)
+            if d["grid"]["kind"] == "VoronoiDustGrid":
+                self.rho = d["scale"] * S.geometry_density(d["geometry"], self.sites)
+            else:
+                self.rho = S.sample_density(self.grid, d["geometry"], d["scale"], d["sampleCount"], sp.seed)
+            self.mix = S.SimpleOligoDustMix(d["opacities"], d["albedos"], d["asymmetry"])
+            self.dust = S.DustSystem(self.grid, self.rho, self.mix, writeMeanIntensity=d["writeMeanIntensity"])
+        self.sim = S.OligoMonteCarloSimulation(self.dust, self.stars, self.instr, packages=sp.packages,
+                                               minWeightReduction=sp.minWeightReduction,
+                                               minScattEvents=sp.minScattEvents, scattBias=sp.scattBias,
+                                               seed=sp.seed)
+
+    def _grid(self, g, dgeom, scale, seed):
+        ext = g["extent"]
+        if g["kind"] == "CartesianDustGrid":
+            return S.CartesianDustGrid(ext, *[S.linmesh(ext[a], ext[a + 3], g["bins"][a]) for a in range(3)])
+        if g["kind"] in ("OctTreeDustGrid", "BinTreeDustGrid"):
+            cls = S.OctTreeDustGrid if g["kind"] == "OctTreeDustGrid" else S.BinTreeDustGrid
+            return cls.subdivide(ext, dgeom, scale, total_mass=scale, minlevel=g["minLevel"], maxlevel=g["maxLevel"],
+                                 maxmassfraction=g["maxMassFraction"], nsamples=g["sampleCount"], seed=seed,
+                                 search=SG_NEIGHBOR if g["search"] == "Neighbor" else SG_TOPDOWN)
+        n = g["numParticles"]
+        if g["distribution"] == "DustDensity":
+            from . import workloads
+            self.sites = workloads.sample_sites(dgeom, n, extent=ext, seed=seed)
+        else:
+            rng = np.random.default_rng(seed)
+            self.sites = ext[:3] + rng.random((n, 3)) * (ext[3:] - ext[:3])
+        return S.VoronoiDustGrid(ext, self.sites)
+
+    def run(self, precision=SG_MIXED):
+        """The stellar emission phase: every wavelength, Npp packets each."""
+        self.instr.clear()
+        return self.sim.runstellaremission(precision)
+
+    def write(self, outdir=".", stamp=None):
+        """FrameInstrument::write for every instrument: calibrated <prefix>_<name>_total.fits files."""
+        paths = []
+        sp = self.spec
+        for i, f in enumerate(sp.frames):
+            xps, yps = f["fovX"] / f["pixelsX"], f["fovY"] / f["pixelsY"]
+            cube = output.calibrate_frame(self.instr.frame(i), sp.lambdas, sp.dlambdas, f["distance"], xps, yps,
+                                          sp.units)
+            path = os.path.join(outdir, "%s_%s_total.fits" % (self.prefix, f["name"]))
+            output.write_frame(path, cube, xps, yps, f["centerX"], f["centerY"], sp.units, stamp)
+            paths.append(path)
+        return paths
+
+
+def load(path, resources=None):
+    """A ski file from disk; the output prefix is its file name without the extension."""
+    with open(path) as f:
+        text = f.read()
+    return Simulation(Spec(text, resources), os.path.splitext(os.path.basename(path))[0])
+
+
+def run(path, outdir=None, resources=None, precision=SG_MIXED):
+    """Runs a ski file and writes its frames next to it (or into outdir); returns the FITS paths."""
+    s = load(path, resources)
+    s.run(precision)
+    return s.write(outdir or os.path.dirname(os.path.abspath(path)))

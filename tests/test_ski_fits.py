@@ -1,0 +1,248 @@
+"""Ski-file front end, units, frame calibration and FITS output (CPU).
+
+The FITS writer is pinned against the reference's own vendored cfitsio (Cfitsio/, compiled from the
+reference tree into oracle/_ref/libfitsref.so behind a shim that makes FITSInOut::write's calls): the
+files must be byte-identical for the same data, keywords and DATE value, and cfitsio must read ours back.
+The calibration is pinned by flux conservation (summing the surface brightness over the pixels' solid
+angles recovers L / (4 pi d^2) per unit wavelength) and by the relations between the flux output styles.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+import pytest
+
+from skirt7_amd import output as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FITSREF = os.path.join(ROOT, "oracle", "_ref", "libfitsref.so")
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def _fitsref():
+    if not os.path.exists(FITSREF):
+        pytest.skip("oracle/_ref/libfitsref.so not built (needs the reference tree)")
+    L = ctypes.CDLL(FITSREF)
+    L.ref_fits_write.argtypes = [ctypes.c_char_p, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
+                                 ctypes.c_char_p]
+    L.ref_fits_read.argtypes = [ctypes.c_char_p, _D, ctypes.c_long, ctypes.POINTER(ctypes.c_int)]
+    L.ref_fits_key.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _D, ctypes.c_char_p, ctypes.c_int]
+    return L
+
+
+STAMP = "2026-12-02T21:07:00"
+
+
+@pytest.mark.parametrize("shape,incx,incy,xc,yc,du,xu", [
+    ((1, 5, 7), 0.123456789, 98765.4321, -1.5e-3, 2.5e20, "W/m2/arcsec2", "pc"),
+    ((3, 4, 6), 1.0, 2.0, 0.0, 0.0, "MJy/sr", "m"),
+    ((1, 128, 256), 156.25, 156.25, 0.0, -12.5, "W/m2/micron/arcsec2", "pc"),
+    ((2, 1, 1), 1e-30, 7e100, 1.0 / 3.0, -0.0, "", "Mpc"),
+])
+def test_fits_bytes_match_reference_cfitsio(tmp_path, shape, incx, incy, xc, yc, du, xu):
+    L = _fitsref()
+    rng = np.random.default_rng(sum(shape))
+    d = rng.lognormal(0.0, 6.0, shape) * 1e-15
+    d.flat[0] = 0.0
+    if d.size > 2:
+        d.flat[1] = -2.5
+        d.flat[2] = 3.0e38
+    ref, ours = str(tmp_path / "r.fits"), str(tmp_path / "o.fits")
+    assert L.ref_fits_write(ref.encode(), np.ascontiguousarray(d).ctypes.data_as(_D), shape[2], shape[1], shape[0],
+                            incx, incy, xc, yc, du.encode(), xu.encode(), STAMP.encode()) == 0
+    O.fits_write(ours, d, incx, incy, xc, yc, du, xu, STAMP)
+    a, b = open(ref, "rb").read(), open(ours, "rb").read()
+    assert len(a) == len(b) and len(a) % 2880 == 0
+    assert a == b
+
+
+def test_fits_read_back_by_reference_cfitsio(tmp_path):
+    L = _fitsref()
+    d = np.arange(2 * 3 * 5, dtype=np.float64).reshape(2, 3, 5) * 0.1 + 1e-7
+    p = str(tmp_path / "x.fits")
+    O.fits_write(p, d, 2.5, 3.5, 1.0, -1.0, "W/m2/arcsec2", "pc")      # current UTC date
+    out = np.zeros(64)
+    dims = (ctypes.c_int * 3)()
+    assert L.ref_fits_read(p.encode(), out.ctypes.data_as(_D), 64, dims) == 0
+    assert tuple(dims) == (5, 3, 2)
+    assert np.array_equal(out[:30], d.ravel().astype(np.float32).astype(np.float64))
+    v = ctypes.c_double()
+    for key, want in (("CRPIX1", 3.0), ("CRPIX2", 2.0), ("CDELT1", 2.5), ("CDELT2", 3.5), ("CRVAL2", -1.0)):
+        assert L.ref_fits_key(p.encode(), key.encode(), ctypes.byref(v), None, 0) == 0
+        assert v.value == want, key
+    buf = ctypes.create_string_buffer(80)
+    assert L.ref_fits_key(p.encode(), b"BUNIT", None, buf, 80) == 0
+    assert buf.value == b"W/m2/arcsec2"
+    assert L.ref_fits_key(p.encode(), b"DATE", None, buf, 80) == 0
+    assert len(buf.value) == 19 and buf.value[4:5] == b"-" and buf.value[10:11] == b"T"
+
+
+def test_units_systems_and_flux_styles():
+    u = O.Units("ExtragalacticUnits")
+    assert u.unit("length") == "pc" and u.unit("distance") == "Mpc" and u.unit("wavelength") == "micron"
+    assert u.unit("surfacebrightness") == "W/m2/arcsec2"
+    assert O.Units("ExtragalacticUnits", "Frequency").unit("surfacebrightness") == "MJy/sr"
+    assert O.Units("SIUnits", "Wavelength").unit("surfacebrightness") == "W/m3/sr"
+    assert abs(u.in_("distance", 10.0, "Mpc") / (1e7 * O.PC) - 1.0) < 1e-15
+    assert u.out("length", O.PC * 250.0) == 250.0
+    assert u.in_("posangle", 90.0, "deg") == math.pi / 2
+    # an arcsec is 1/206264.806... rad, so one sr holds 4.25e10 arcsec^2
+    assert abs(1.0 / O.ARCSEC ** 2 / 4.254517029e10 - 1.0) < 1e-9
+    # styles: lambda f_lambda (neutral) = nu f_nu, and f_nu = lambda^2 f_lambda / c, in SI
+    si_l = O.Units("SIUnits", "Wavelength")
+    si_n = O.Units("SIUnits", "Neutral")
+    si_f = O.Units("SIUnits", "Frequency")
+    lam, flam = 0.55e-6, 3.7e-5
+    assert si_n.osurfacebrightness(lam, flam) == lam * si_l.osurfacebrightness(lam, flam)
+    fnu = si_f.osurfacebrightness(lam, flam)
+    nu = O.C / lam
+    assert abs(nu * fnu / si_n.osurfacebrightness(lam, flam) - 1.0) < 1e-14
+    with pytest.raises(O.FatalError):
+        u.in_("length", 1.0, "furlong")
+
+
+def test_calibration_conserves_flux():
+    """Summing the calibrated f_lambda (SI) over the pixels' solid angles gives F_lambda = L_lambda / (4 pi d^2):
+    the detected luminosity per bin over the bin width, the pixels' angular size at the distance, the
+    inverse-square dilution."""
+    rng = np.random.default_rng(2)
+    nl, ny, nx = 2, 16, 24
+    frame = rng.random((nl, ny, nx)) * 1e30                     # W per pixel per wavelength bin
+    lam = np.array([0.5e-6, 2.2e-6])
+    dl = 0.001 * lam
+    d = 10.0e6 * O.PC
+    xps, yps = 40.0 * O.PC, 25.0 * O.PC
+    f = O.calibrate_frame(frame, lam, dl, d, xps, yps, O.Units("SIUnits", "Wavelength"))
+    omega = (2.0 * math.atan(xps / (2.0 * d))) * (2.0 * math.atan(yps / (2.0 * d)))
+    for ell in range(nl):
+        F = f[ell].sum() * omega
+        assert abs(F / (frame[ell].sum() / dl[ell] / (4.0 * math.pi * d * d)) - 1.0) < 1e-12
+    # small-angle check of the pixel solid angle: (xps/d)(yps/d) up to terms in (xps/d)^2 / 12
+    assert abs(omega / (xps * yps / d ** 2) - 1.0) < 1e-10
+    # the extragalactic neutral style is lambda f_lambda in W/m2/arcsec2
+    g = O.calibrate_frame(frame, lam, dl, d, xps, yps, O.Units("ExtragalacticUnits"))
+    assert np.allclose(g[1], f[1] * lam[1] * O.ARCSEC ** 2, rtol=1e-14, atol=0)
+
+
+SUN = """# a two-segment stand-in for the solar SED (lambda in micron, L in W/micron)
+3
+0.4  1.0e26
+0.6  2.0e26
+1.0  1.0e26
+"""
+
+
+def ski_text(grid="cartesian", packages="1e4", units="ExtragalacticUnits", dust=True, style="Neutral"):
+    grids = {
+        "cartesian": """<CartesianDustGrid minX="-20 kpc" maxX="20 kpc" minY="-20 kpc" maxY="20 kpc" minZ="-20 kpc" maxZ="20 kpc">
+                  <meshX type="MoveableMesh"><LinMesh numBins="10"/></meshX>
+                  <meshY type="MoveableMesh"><LinMesh numBins="12"/></meshY>
+                  <meshZ type="MoveableMesh"><LinMesh numBins="8"/></meshZ>
+                </CartesianDustGrid>""",
+        "octree": """<OctTreeDustGrid minX="-20 kpc" maxX="20 kpc" minY="-20 kpc" maxY="20 kpc" minZ="-20 kpc" maxZ="20 kpc"
+                  minLevel="2" maxLevel="4" searchMethod="Neighbor" sampleCount="20" maxMassFraction="1e-3"/>""",
+    }
+    dust_xml = ""
+    if dust:
+        dust_xml = """
+        <dustSystem type="DustSystem">
+          <OligoDustSystem sampleCount="10" writeMeanIntensity="false">
+            <dustDistribution type="DustDistribution">
+              <CompDustDistribution>
+                <components type="DustComp">
+                  <DustComp>
+                    <geometry type="Geometry"><ExpDiskGeometry radialScale="6 kpc" axialScale="250 pc"/></geometry>
+                    <mix type="DustMix"><SimpleOligoDustMix opacities="1 m2/kg" albedos="0.6" asymmetryParameters="0.6"/></mix>
+                    <normalization type="DustCompNormalization">
+                      <FaceOnDustCompNormalization wavelength="0.55 micron" opticalDepth="1"/>
+                    </normalization>
+                  </DustComp>
+                </components>
+              </CompDustDistribution>
+            </dustDistribution>
+            <dustGrid type="DustGrid">%s</dustGrid>
+          </OligoDustSystem>
+        </dustSystem>""" % grids[grid]
+    return """<?xml version="1.0" encoding="UTF-8"?>
+<skirt-simulation-hierarchy type="MonteCarloSimulation" format="6.1">
+  <OligoMonteCarloSimulation packages="%s" minWeightReduction="1e4" scattBias="0.5">
+    <random type="Random"><Random seed="4357"/></random>
+    <units type="Units"><%s fluxOutputStyle="%s"/></units>
+    <instrumentSystem type="InstrumentSystem">
+      <InstrumentSystem>
+        <instruments type="Instrument">
+          <FrameInstrument instrumentName="i0" distance="10 Mpc" inclination="0 deg" azimuth="0 deg" positionAngle="0 deg"
+             fieldOfViewX="40 kpc" pixelsX="32" centerX="0 pc" fieldOfViewY="40 kpc" pixelsY="24" centerY="0 pc"/>
+          <FrameInstrument instrumentName="i88" distance="10 Mpc" inclination="88 deg"
+             fieldOfViewX="4e4 pc" pixelsX="32" fieldOfViewY="2e4 pc" pixelsY="16"/>
+        </instruments>
+      </InstrumentSystem>
+    </instrumentSystem>
+    <wavelengthGrid type="OligoWavelengthGrid"><OligoWavelengthGrid wavelengths="0.55 micron"/></wavelengthGrid>
+    <stellarSystem type="StellarSystem">
+      <StellarSystem emissionBias="0.5">
+        <components type="StellarComp">
+          <OligoStellarComp luminosities="8e9">
+            <geometry type="Geometry"><ExpDiskGeometry radialScale="4 kpc" axialScale="400 pc"/></geometry>
+          </OligoStellarComp>
+          <OligoStellarComp luminosities="2e9">
+            <geometry type="Geometry"><SersicGeometry index="4" radius="1 kpc"/></geometry>
+          </OligoStellarComp>
+        </components>
+      </StellarSystem>
+    </stellarSystem>%s
+  </OligoMonteCarloSimulation>
+</skirt-simulation-hierarchy>
+""" % (packages, units, style, dust_xml)
+
+
+@pytest.fixture
+def resources(tmp_path):
+    d = tmp_path / "dat"
+    (d / "SED" / "Sun").mkdir(parents=True)
+    (d / "SED" / "Sun" / "SunSED.dat").write_text(SUN)
+    return str(d)
+
+
+def test_ski_spec(resources):
+    from skirt7_amd import ski
+    sp = ski.Spec(ski_text(), resources)
+    assert sp.packages == 1e4 and sp.seed == 4357 and sp.scattBias == 0.5 and sp.minScattEvents == 0
+    assert sp.units.system == "ExtragalacticUnits"
+    assert np.allclose(sp.lambdas, [0.55e-6]) and np.allclose(sp.dlambdas, [0.55e-9])
+    # solar L_lambda at 0.55 micron: 1.0e26 + 0.75 (2.0e26 - 1.0e26) W/micron = 1.75e32 W/m
+    geom0, L0 = sp.stellar[0]
+    assert abs(L0[0] / (8e9 * 1.75e32 * 0.55e-9) - 1.0) < 1e-12
+    assert abs(geom0.hR - 4000.0) < 1e-9 and abs(geom0.hz - 400.0) < 1e-9     # pc
+    geom1, L1 = sp.stellar[1]
+    assert abs(L1[0] / L0[0] - 0.25) < 1e-12
+    f0, f1 = sp.frames
+    assert f0["name"] == "i0" and abs(f0["distance"] / (1e7 * O.PC) - 1.0) < 1e-15
+    assert f1["inclination"] == 88.0 * math.pi / 180.0 and f1["pixelsY"] == 16
+    assert abs(f1["fovX"] - 4e4 * O.PC) < 1e-3 * O.PC and f1["centerX"] == 0.0
+    d = sp.dust
+    assert d["grid"]["kind"] == "CartesianDustGrid" and d["grid"]["bins"] == [10, 12, 8]
+    assert np.allclose(d["grid"]["extent"], [-2e4, -2e4, -2e4, 2e4, 2e4, 2e4])
+    # face-on optical depth 1 through the dust disc: scale * kappa * SigmaZ = tau (lengths in pc)
+    assert abs(d["scale"] * d["opacities"][0] * d["geometry"].SigmaZ() - 1.0) < 1e-12
+    sp2 = ski.Spec(ski_text(grid="octree", units="SIUnits", style="Frequency"), resources)
+    g = sp2.dust["grid"]
+    assert g["kind"] == "OctTreeDustGrid" and g["maxLevel"] == 4 and g["maxMassFraction"] == 1e-3
+    assert sp2.units.unit("surfacebrightness") == "W/m2/Hz/sr"
+
+
+def test_ski_rejects_what_it_cannot_run(resources):
+    from skirt7_amd import ski
+    t = ski_text().replace("<SersicGeometry", "<PlummerGeometry")
+    with pytest.raises(O.FatalError, match="PlummerGeometry"):
+        ski.Spec(t, resources)
+    t = ski_text().replace('wavelength="0.55 micron" opticalDepth', 'wavelength="0.6 micron" opticalDepth')
+    with pytest.raises(O.FatalError, match="wavelength"):
+        ski.Spec(t, resources)
+    with pytest.raises(O.FatalError, match="SunSED"):
+        ski.Spec(ski_text(), os.path.join(resources, "missing"))
+    t = ski_text().replace('luminosities="8e9"', 'luminosities="8e9, 1e9"')
+    with pytest.raises(O.FatalError, match="luminosities"):
+        ski.Spec(t, resources)
