@@ -53,7 +53,8 @@ def parse():
                     help="config3: log-uniform range of packets per member")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--packets", type=float, default=1e7, help="packets per step")
+    ap.add_argument("--packets", type=float, default=None,
+                    help="packets per step (default 1e7 for config1, 1e6 for config2 and config4)")
     ap.add_argument("--tau", type=float, default=2.0)
     ap.add_argument("--maxlevel", type=int, default=8)
     ap.add_argument("--pixels", type=int, default=512)
@@ -190,6 +191,8 @@ def main():
     if a.copy_calibration > 0:
         copy_calibration(a.copy_calibration)
     S.set_pipeline(a.pipeline)
+    if a.packets is None:
+        a.packets = 1e7 if a.workload == "config1" else 1e6
     per = int(a.packets)
     total = per * (a.steps + a.warmup)
     t0 = time.perf_counter()
@@ -273,6 +276,7 @@ def main():
             c64 = sim.run_packets(ell, first, per, per, SG_FP64)
             dt = time.perf_counter() - t0
             fp64[p] = (c64["packets"] / dt, c64)
+            log("  fp64 step (pipeline %d): %d packets, %.3f s" % (p, c64["packets"], dt))
             if p == 1:
                 img64 = frames(w, ell)
         S.set_pipeline(a.pipeline)
@@ -296,7 +300,10 @@ def main():
             # FP64 reference with ten times the packets (fresh packets), normalised to its own count
             ref0 = (a.warmup + a.steps) * per
             w["instr"].clear()
-            sim.run_packets(ell, ref0, 10 * per, 10 * per, SG_FP64)
+            for k in range(10):
+                t0 = time.perf_counter()
+                sim.run_packets(ell, ref0 + k * per, per, 10 * per, SG_FP64)
+                log("  fp64 reference part %d/10: %d packets, %.3f s" % (k + 1, per, time.perf_counter() - t0))
             imgref = frames(w, ell)
 
             def rms(x, y):

@@ -168,6 +168,11 @@ int sg_set_tuning(int block, int waves_per_cu, int compact_lanes);
 /* Smallest member (packets) that sg_sim_run_batch runs alone through the event/walk pipeline
  * instead of in the shared persistent launch (0 = never; default 2^21). */
 int sg_set_batch_alone(uint64_t min_packets);
+/* Mixed-precision walks of trees built with the TopDown search method: 0 (default) follow the ropes
+ * between leaves, as for Neighbor trees (the same cells: the search method only decides how the next
+ * cell is found, TreeDustGrid.cpp:371-643); 1 descend from the root at every step as TopDown does.
+ * FP64 walks always use the tree's own method. */
+int sg_set_mixed_topdown(int descend);
 
 /* ---- setup helpers (the reference's own setup phase, made reproducible) ---------------------- */
 /* Normalised density of an exponential-disc geometry record laid out like a stellar component

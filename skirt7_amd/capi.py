@@ -50,6 +50,7 @@ _SIGS = {
     "sg_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "sg_set_pipeline": (ctypes.c_int, [ctypes.c_int]),
     "sg_set_batch_alone": (ctypes.c_int, [ctypes.c_uint64]),
+    "sg_set_mixed_topdown": (ctypes.c_int, [ctypes.c_int]),
     "sg_fits_write": (ctypes.c_int, [ctypes.c_char_p, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
                                      ctypes.c_char_p]),

@@ -80,6 +80,42 @@ bool HostTree::build(int ar, const double e[6], const uint8_t* flags, int64_t nn
     return true;
 }
 
+// Ropes from the root down, in id order (children after their parents): a child's wall inside its
+// parent leads to the sibling across it; a wall on the parent's boundary leads through the parent's
+// rope R: to R itself when R is a leaf or coarser than the parent, else to R's child adjacent to the
+// child's wall (R is then at the parent's level and, in an alternating binary tree, split along the
+// same axis).
+void HostTree::build_ropes(std::vector<int32_t>& rope) const
+{
+    const int64_t n = (int64_t)child0.size();
+    rope.assign(6 * (size_t)n, -1);
+    for (int64_t p = 0; p < n; p++) {
+        const int32_t c0 = child0[p];
+        if (c0 < 0) continue;
+        const int nch = arity == 8 ? 8 : 2;
+        for (int q = 0; q < nch; q++) {
+            const int64_t c = c0 + q;
+            for (int w = 0; w < 6; w++) {
+                const int a = w >> 1, side = w & 1;
+                int32_t r;
+                // the child's position along the wall's axis: 0 lower half, 1 upper half, -1 whole extent
+                const int pos = arity == 8 ? ((q >> a) & 1) : (dir[p] == a ? q : -1);
+                if (pos >= 0 && pos != side) {
+                    r = arity == 8 ? c0 + (q ^ (1 << a)) : c0 + (1 - q);   // sibling across an inner wall
+                } else {
+                    const int32_t R = rope[6 * (size_t)p + w];
+                    if (R < 0 || child0[R] < 0 || level[R] != level[p]) r = R;
+                    else if (arity == 8) r = child0[R] + (q ^ (1 << a));
+                    else if (dir[R] == a) r = child0[R] + (1 - side);
+                    else if (dir[R] == dir[p]) r = child0[R] + q;
+                    else r = R;
+                }
+                rope[6 * (size_t)c + w] = r;
+            }
+        }
+    }
+}
+
 // Neighbour lists, built by the same incremental procedure as OctTreeNode::addneighbors /
 // BinTreeNode::addneighbors (called for every node in id order, TreeDustGrid.cpp:156-161) and
 // ordered by decreasing overlap area with std::sort (TreeNode.cpp:97-158), so that the first

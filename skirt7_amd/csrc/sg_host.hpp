@@ -75,6 +75,9 @@ struct HostTree {
     std::vector<std::vector<int32_t>> nb;   // 6 lists per node (node ids), built on request
     bool build(int arity, const double ext[6], const uint8_t* flags, int64_t nnodes);
     void build_neighbors();
+    // Ropes of every node (6 per node, walls in the order -x +x -y +y -z +z): the node across the
+    // wall at the same level, or the coarser leaf covering the whole wall, or -1 on the grid boundary.
+    void build_ropes(std::vector<int32_t>& rope) const;
 };
 
 } // namespace sg

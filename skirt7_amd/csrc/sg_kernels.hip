@@ -24,6 +24,11 @@ static int g_cus = 256;
 static int g_tune_block = 256, g_tune_waves = 0, g_tune_compact = 0;
 static int g_pipeline = 1;   // 0: persistent lane pipeline (k_mc); 1: event and walk kernels (k_events/k_walks)
 static uint64_t g_batch_alone = (uint64_t)1 << 21;   // batched members this large run alone (0: never)
+static int g_topdown_descent = 0;   // 1: mixed walks of TopDown trees descend from the root at every step
+
+// Mixed-precision tree walks follow the ropes (the same cells as any search method) unless TopDown
+// descents were asked for a TopDown tree.
+static bool rope_walk(const sg_grid* g) { return g->tree.search == SG_NEIGHBOR || !g_topdown_descent; }
 
 static void ensure_init()
 {
@@ -197,6 +202,12 @@ int sg_set_batch_alone(uint64_t min_packets)
     return 0;
 }
 
+int sg_set_mixed_topdown(int descend)
+{
+    g_topdown_descent = descend ? 1 : 0;
+    return 0;
+}
+
 int sg_set_tuning(int block, int waves_per_cu, int compact_lanes)
 {
     if (block == 64 || block == 128 || block == 256) g_tune_block = block;
@@ -251,7 +262,7 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     g->kind = GK_TREE;
     HostTree& h = g->ht;
     if (!h.build(arity, ext, subdivided, nnodes)) { delete g; throw Fail{-1}; }
-    if (search == SG_NEIGHBOR) h.build_neighbors();   // TopDown needs no neighbour lists (and no ropes)
+    if (search == SG_NEIGHBOR) h.build_neighbors();   // TopDown needs no neighbour lists
     const bool lists = !h.nb.empty();
     int64_t nn = (int64_t)h.child0.size(), nc = (int64_t)h.idv.size();
     g->ncells = nc;
@@ -275,6 +286,8 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     // mixed-precision records: subdivision counts per axis (a child adds one along each axis of an
     // octree, or along its parent's split axis of a binary tree), density or first child, ropes
     std::vector<int32_t> cellof(nn);
+    std::vector<int32_t> ropes;
+    h.build_ropes(ropes);
     std::vector<uint32_t> nsub(nn, 0);   // nx | ny << 5 | nz << 10
     for (int64_t l = 0; l < nn; l++) {
         const int32_t c0 = h.child0[l];
@@ -292,11 +305,14 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
         f.val = leaf ? 0u : (uint32_t)h.child0[l];
         cellof[l] = h.cellnumber[l];
         for (int w = 0; w < 6; w++) {
-            // rope: the first (largest-overlap) neighbour, raised to this node's level when finer
-            int32_t r = -1;
-            if (lists && !h.nb[6 * (size_t)l + w].empty()) r = h.nb[6 * (size_t)l + w][0];
-            while (r >= 0 && h.level[r] > h.level[l]) r = h.father[r];
-            f.rope[w] = leaf ? r : -1;
+            f.rope[w] = leaf ? ropes[6 * (size_t)l + w] : -1;
+            if (leaf && lists) {
+                // consistency with the neighbour lists: the first (largest-overlap) neighbour, raised to
+                // this node's level when finer, is the same rope
+                int32_t r = h.nb[6 * (size_t)l + w].empty() ? -1 : h.nb[6 * (size_t)l + w][0];
+                while (r >= 0 && h.level[r] > h.level[l]) r = h.father[r];
+                SG_REQUIRE(r == f.rope[w], "tree ropes disagree with the neighbour lists");
+            }
         }
     }
     g->cellof.upload(cellof);
@@ -466,7 +482,7 @@ int sg_grid_paths(sg_grid* g, int64_t n, const double* rk, int max_seg, int prec
         else launch_paths<CartW64>(g->cart, n, d, max_seg, oc, om, od);
     } else if (g->kind == GK_TREE) {
         if (!mixed) launch_paths<TreeW64>(g->tree, n, d, max_seg, oc, om, od);
-        else if (g->tree.search == SG_NEIGHBOR) launch_paths<TreeW32>(g->tree, n, d, max_seg, oc, om, od);
+        else if (rope_walk(g)) launch_paths<TreeW32>(g->tree, n, d, max_seg, oc, om, od);
         else launch_paths<TreeW32TopDown>(g->tree, n, d, max_seg, oc, om, od);
     } else {
         if (mixed) launch_paths<VoroW<float>>(g->voro, n, d, max_seg, oc, om, od);
@@ -543,7 +559,7 @@ static int tau_common(sg_dust* d, int ell, int64_t n, const double* rk, const do
         else launch_tau<CartW64>(g->cart, n, ri, ke, ti, mode, o);
     } else if (g->kind == GK_TREE) {
         if (!mixed) launch_tau<TreeW64>(g->tree, n, ri, ke, ti, mode, o);
-        else if (g->tree.search == SG_NEIGHBOR) launch_tau<TreeW32>(g->tree, n, ri, ke, ti, mode, o);
+        else if (rope_walk(g)) launch_tau<TreeW32>(g->tree, n, ri, ke, ti, mode, o);
         else launch_tau<TreeW32TopDown>(g->tree, n, ri, ke, ti, mode, o);
     } else {
         if (mixed) launch_tau<VoroW<float>>(g->voro, n, ri, ke, ti, mode, o);
@@ -896,7 +912,7 @@ template <bool ABS> static void dispatch_mc(sg_sim* s, const McArgs& a, int prec
         else launch_mc<CartW64, ABS>(g->cart, a, blocks, s->cold, s->args);
     } else if (g->kind == GK_TREE) {
         if (!mixed) launch_mc<TreeW64, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
-        else if (g->tree.search == SG_NEIGHBOR) launch_mc<TreeW32, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
+        else if (rope_walk(g)) launch_mc<TreeW32, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
         else launch_mc<TreeW32TopDown, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
     } else {
         if (mixed) launch_mc<VoroW<float>, ABS>(g->voro, a, blocks, s->cold, s->args);
@@ -1082,7 +1098,7 @@ static void dispatch_split(sg_sim* s, const McArgs& a, int precision, uint64_t f
         else run_split<CartW64, ABS>(s, g->cart, a, first, count, 0, lt, blocks);
     } else if (g->kind == GK_TREE) {
         if (!mixed) run_split<TreeW64, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
-        else if (g->tree.search == SG_NEIGHBOR) run_split<TreeW32, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
+        else if (rope_walk(g)) run_split<TreeW32, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
         else run_split<TreeW32TopDown, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
     } else {
         if (mixed) run_split<VoroW<float>, ABS>(s, g->voro, a, first, count, 0, lt, blocks);
@@ -1223,7 +1239,7 @@ static void dispatch_batch(sg_sim* s0, const std::vector<McArgs>& rg, const std:
         else launch_batch<CartW64, ABS>(s0, rg, owner, blocks, lt);
     } else if (g->kind == GK_TREE) {
         if (!mixed) launch_batch<TreeW64, ABS>(s0, rg, owner, blocks, lt);
-        else if (g->tree.search == SG_NEIGHBOR) launch_batch<TreeW32, ABS>(s0, rg, owner, blocks, lt);
+        else if (rope_walk(g)) launch_batch<TreeW32, ABS>(s0, rg, owner, blocks, lt);
         else launch_batch<TreeW32TopDown, ABS>(s0, rg, owner, blocks, lt);
     } else {
         if (mixed) launch_batch<VoroW<float>, ABS>(s0, rg, owner, blocks, lt);

@@ -362,12 +362,18 @@ __device__ __forceinline__ bool chunk_claim(unsigned long long* ctr, uint64_t nq
 }
 
 // Occupancy target of the walk kernel (waves per SIMD): five for the mixed tree walk, which fits
-// in 96 registers, four for the others.
+// in 96 registers; three for the exact tree walk, whose FP64 state needs about 200 registers (two
+// waves do not spill but hide less of the root descents' latency, four spill more); four for the
+// others.
 #ifndef SG_WALK_WAVES_PER_EU
 #define SG_WALK_WAVES_PER_EU 4
 #endif
 template <class W> struct WalkWaves { static constexpr int value = SG_WALK_WAVES_PER_EU; };
 template <> struct WalkWaves<TreeW32> { static constexpr int value = 5; };
+#ifndef SG_W64_WAVES
+#define SG_W64_WAVES 3
+#endif
+template <> struct WalkWaves<TreeW64> { static constexpr int value = SG_W64_WAVES; };
 
 // Walk pass: persistent lanes over the requests [0, *q.tail).
 template <class W, bool ABS>

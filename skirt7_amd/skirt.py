@@ -572,6 +572,12 @@ def set_batch_alone(min_packets):
     check(lib().sg_set_batch_alone(int(min_packets)))
 
 
+def set_mixed_topdown(descend):
+    """Mixed-precision walks of TopDown trees: False (default) follow the ropes between leaves (the same
+    cells); True descend from the root at every step."""
+    check(lib().sg_set_mixed_topdown(1 if descend else 0))
+
+
 def set_tuning(block=0, waves_per_cu=0, compact_lanes=0):
     check(lib().sg_set_tuning(block, waves_per_cu, compact_lanes))
 

@@ -122,24 +122,32 @@ def test_opticaldepth_and_pathlength_bitexact(kind):
     assert np.array_equal(np.nan_to_num(s, nan=-1.0), np.nan_to_num(sref, nan=-1.0))
 
 
-@pytest.mark.parametrize("kind", ["cartesian", "octree_neighbor", "octree_topdown"])
+@pytest.mark.parametrize("kind", ["cartesian", "octree_neighbor", "octree_topdown", "octree_topdown_descent",
+                                  "bintree_topdown", "bintree_topdown_descent"])
 def test_mixed_precision_walks_close(kind):
+    """Mixed walks against the exact walk of the tree's own search method: a TopDown tree's mixed walk
+    follows the ropes by default and descends from the root at every step when asked to."""
     if kind == "cartesian":
         g = S.CartesianDustGrid.uniform(EXT, 50, 50, 50)
         sc = O.Scene().cartesian(g.extent, g.xv, g.yv, g.zv)
     else:
         search = SG_NEIGHBOR if kind.endswith("neighbor") else SG_TOPDOWN
-        flags = random_tree_flags(8, 5, 0.35, seed=11)
-        g = S.OctTreeDustGrid(EXT, flags, search)
-        sc = O.Scene().tree(8, EXT, flags, search)
+        arity = 8 if kind.startswith("octree") else 2
+        flags = random_tree_flags(arity, 5 if arity == 8 else 12, 0.35 if arity == 8 else 0.55, seed=11)
+        g = (S.OctTreeDustGrid if arity == 8 else S.BinTreeDustGrid)(EXT, flags, search)
+        sc = O.Scene().tree(arity, EXT, flags, search)
     ds = _dust_pair(g, sc, 5)
     rays = random_rays(3000, 10, special=False)
-    c64, m64, d64 = g.paths(rays, 4096, SG_FP64)
-    c32, m32, d32 = g.paths(rays, 4096, SG_MIXED)
+    S.set_mixed_topdown(kind.endswith("descent"))
+    try:
+        c64, m64, d64 = g.paths(rays, 4096, SG_FP64)
+        c32, m32, d32 = g.paths(rays, 4096, SG_MIXED)
+        t64 = ds.opticaldepths(rays, 0, SG_FP64)
+        t32 = ds.opticaldepths(rays, 0, SG_MIXED)
+    finally:
+        S.set_mixed_topdown(False)
     same = np.array([c64[i] == c32[i] and np.array_equal(m64[i, :c64[i]], m32[i, :c32[i]]) for i in range(len(rays))])
     assert same.mean() > 0.98, same.mean()
-    t64 = ds.opticaldepths(rays, 0, SG_FP64)
-    t32 = ds.opticaldepths(rays, 0, SG_MIXED)
     rel = np.abs(t32 - t64) / np.maximum(t64, 1e-3)
     assert np.quantile(rel, 0.99) < 1e-4, np.quantile(rel, 0.99)
 
