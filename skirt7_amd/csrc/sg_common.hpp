@@ -78,6 +78,7 @@ struct VoroView {
     const int32_t* nb;               // neighbour ids, walls -1..-6
     const int32_t* boff;             // bucket offsets, nbk^3 + 1
     const int32_t* bids;             // site ids per bucket
+    const int32_t* brep;             // start site per bucket for the greedy location, nbk^3
     int32_t nsites;
     int nbk;
     double ext[6];

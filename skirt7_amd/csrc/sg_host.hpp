@@ -95,7 +95,7 @@ struct sg_grid {
     std::vector<sg::TreeNode32> hnode32;
     sg::TreeView tree{};
     // Voronoi
-    sg::DevBuf site, voff, vnb, boff, bids;
+    sg::DevBuf site, voff, vnb, boff, bids, brep;
     std::vector<sg::VoroSite> hsite;
     sg::VoroView voro{};
     void set_rho(const double* r);

@@ -140,7 +140,7 @@ __global__ void k_whichcell_voro(const VoroView g, int64_t n, const double* r, i
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = voro_nearest(g, r[3 * i], r[3 * i + 1], r[3 * i + 2]);
+    out[i] = voro_locate(g, r[3 * i], r[3 * i + 1], r[3 * i + 2], -1);
 }
 
 } // namespace sg
@@ -376,14 +376,48 @@ int sg_grid_create_voronoi(const double ext[6], const double* sites, int64_t n, 
     for (size_t q = 1; q < cnt.size(); q++) cnt[q] += cnt[q - 1];
     std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1), bid(n);
     for (int64_t i = 0; i < n; i++) bid[fill[bucket[i]]++] = (int32_t)i;
+    // start sites of the greedy location: in a bucket with sites the one nearest its centre; an empty
+    // bucket takes the start site of the nearest bucket with sites (breadth-first over face neighbours)
+    const size_t nq = (size_t)nbk * nbk * nbk;
+    std::vector<int32_t> rep(nq, -1), frontier, next;
+    for (size_t q = 0; q < nq; q++) {
+        if (cnt[q] == cnt[q + 1]) continue;
+        const int qi = (int)(q / ((size_t)nbk * nbk)), qj = (int)(q / nbk % nbk), qk = (int)(q % nbk);
+        const double cx = ext[0] + (qi + 0.5) * (ext[3] - ext[0]) / nbk;
+        const double cy = ext[1] + (qj + 0.5) * (ext[4] - ext[1]) / nbk;
+        const double cz = ext[2] + (qk + 0.5) * (ext[5] - ext[2]) / nbk;
+        double bd = DBL_MAX;
+        for (int32_t t = cnt[q]; t < cnt[q + 1]; t++) {
+            const VoroSite& s = g->hsite[bid[t]];
+            const double d = (s.x - cx) * (s.x - cx) + (s.y - cy) * (s.y - cy) + (s.z - cz) * (s.z - cz);
+            if (d < bd) { bd = d; rep[q] = bid[t]; }
+        }
+        frontier.push_back((int32_t)q);
+    }
+    while (!frontier.empty()) {
+        next.clear();
+        for (int32_t q : frontier) {
+            const int qi = q / (nbk * nbk), qj = q / nbk % nbk, qk = q % nbk;
+            const int nbq[6][3] = {{qi - 1, qj, qk}, {qi + 1, qj, qk}, {qi, qj - 1, qk},
+                                   {qi, qj + 1, qk}, {qi, qj, qk - 1}, {qi, qj, qk + 1}};
+            for (const auto& c : nbq) {
+                if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= nbk || c[1] >= nbk || c[2] >= nbk) continue;
+                const int32_t r = (c[0] * nbk + c[1]) * nbk + c[2];
+                if (rep[r] < 0) { rep[r] = rep[q]; next.push_back(r); }
+            }
+        }
+        frontier.swap(next);
+    }
     g->boff.upload(cnt);
     g->bids.upload(bid);
+    g->brep.upload(rep);
     VoroView& v = g->voro;
     v.site = g->site.as<VoroSite>();
     v.off = g->voff.as<int32_t>();
     v.nb = g->vnb.as<int32_t>();
     v.boff = g->boff.as<int32_t>();
     v.bids = g->bids.as<int32_t>();
+    v.brep = g->brep.as<int32_t>();
     v.nsites = (int32_t)n;
     v.nbk = nbk;
     for (int i = 0; i < 6; i++) v.ext[i] = ext[i];

@@ -955,35 +955,91 @@ __device__ int voro_nearest(const VoroView& g, double x, double y, double z)
     return best;
 }
 
+// VoronoiMesh::cellIndex by a greedy walk over the neighbour lists from a start site (a known
+// nearby cell, else the start site of the point's bucket): move to the neighbour nearest the point
+// while one is nearer than the current site (ties to the lower index). A point of the box nearer to
+// site h than to all of h's neighbours lies in h's cell (the segment from h to the point leaves the
+// cell through a face shared with a nearer neighbour), so the walk ends at the nearest site. When
+// some neighbour is within 1e-9 (relative, squared distances) as near as the final site, the point
+// may sit on a bisector, where rounding or a tie decides: the exact search settles it. Each move
+// strictly decreases (distance, index), so the walk ends.
+__device__ int voro_locate(const VoroView& g, double x, double y, double z, int start)
+{
+    if (!box_contains(g.ext, x, y, z)) return -1;
+    int cur = start;
+    if (cur < 0 || cur >= g.nsites) {
+        const int n = g.nbk;
+        const int ci = min(n - 1, max(0, (int)(n * (x - g.ext[0]) / (g.ext[3] - g.ext[0]))));
+        const int cj = min(n - 1, max(0, (int)(n * (y - g.ext[1]) / (g.ext[4] - g.ext[1]))));
+        const int ck = min(n - 1, max(0, (int)(n * (z - g.ext[2]) / (g.ext[5] - g.ext[2]))));
+        cur = g.brep[(ci * n + cj) * n + ck];
+    }
+    VoroSite s = g.site[cur];
+    double dc = (x - s.x) * (x - s.x) + (y - s.y) * (y - s.y) + (z - s.z) * (z - s.z);
+    bool thin = false;
+    for (int it = 0; it < g.nsites; it++) {
+        int best = cur;
+        double db = dc;
+        thin = false;
+        const int t1 = g.off[cur + 1];
+        for (int t = g.off[cur]; t < t1; t++) {
+            const int mi = g.nb[t];
+            if (mi < 0) continue;
+            const VoroSite p = g.site[mi];
+            const double d = (x - p.x) * (x - p.x) + (y - p.y) * (y - p.y) + (z - p.z) * (z - p.z);
+            thin = thin || d - dc <= 1e-9 * d;
+            if (d < db || (d == db && mi < best)) { db = d; best = mi; }
+        }
+        if (best == cur) break;
+        cur = best;
+        dc = db;
+    }
+    return thin ? voro_nearest(g, x, y, z) : cur;
+}
+
 // VoronoiMesh::path (VoronoiMesh.cpp:778-873): the bisector-plane tests stay FP64 in both
 // precision modes; Real only sets the type of the emitted segment lengths.
 template <typename R> struct VoroW {
     using Real = R;
     using View = VoroView;
     double x, y, z, kx, ky, kz;
-    int mr;
+    int mr, lm;
     Entry en;
     bool done;
     uint32_t steps, nrecompute, nstuck;
-    static constexpr int kParkWords = 24;
+    static constexpr int kParkWords = 25;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
-    static constexpr bool kHint = false;
+    static constexpr bool kHint = true;
     static constexpr bool kStep = false;   // next() only
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
-        p.io(mr); p.io(en.ne); p.io(en.e0); p.io(en.e1); p.io(en.e2);
+        p.io(mr); p.io(lm); p.io(en.ne); p.io(en.e0); p.io(en.e1); p.io(en.e2);
         p.io(done); p.io(steps); p.io(nrecompute); p.io(nstuck);
     }
 
-    __device__ __forceinline__ void init(const View& g, const int32_t*, int, double rx, double ry, double rz, double dx, double dy,
-                         double dz)
+    __device__ __forceinline__ void init(const View& g, const int32_t* lds, int nlds, double rx, double ry, double rz,
+                                         double dx, double dy, double dz)
+    {
+        init_hinted(g, lds, nlds, rx, ry, rz, dx, dy, dz, -1, 0, 0, 0);
+    }
+    // The cell of the last segment returned (-1 before the first one): after a walk that stopped
+    // inside a segment, the cell holding the stopping point.
+    __device__ __forceinline__ void hint(int& n, int32_t& i, int32_t& j, int32_t& k) const
+    {
+        n = lm; i = 0; j = 0; k = 0;
+    }
+    // init with a candidate cell hn (or -1): the greedy location starts there; the cell found is the
+    // nearest site whatever the start
+    __device__ __forceinline__ void init_hinted(const View& g, const int32_t*, int, double rx, double ry, double rz,
+                                                double dx, double dy, double dz, int hn, int32_t, int32_t, int32_t)
     {
         kx = dx; ky = dy; kz = dz;
-        steps = 0; done = false;
+        steps = 0; done = false; lm = -1;
         if (!en.move_inside(g.ext, g.eps, rx, ry, rz, kx, ky, kz)) { en.ne = 0; done = true; return; }
         x = rx; y = ry; z = rz;
-        mr = voro_nearest(g, x, y, z);
+        // a start point moved onto the box from outside is not where the candidate was found
+        mr = voro_locate(g, x, y, z, en.ne ? -1 : hn);
         if (mr < 0) { en.ne = 0; done = true; }
     }
     __device__ __forceinline__ bool next(const View& g, const int32_t*, int, int& mo, R& ds, R& rh)
@@ -1023,7 +1079,7 @@ template <typename R> struct VoroW {
             if (mq == NO_INDEX) {
                 nstuck++;
                 x += kx * g.eps; y += ky * g.eps; z += kz * g.eps;
-                mr = voro_nearest(g, x, y, z);
+                mr = voro_locate(g, x, y, z, mr);
                 if (mr < 0) done = true;
                 continue;
             }
@@ -1033,6 +1089,7 @@ template <typename R> struct VoroW {
             mr = mq;
             if (mr < 0) done = true;
             mo = segm; ds = (R)sq; rh = (R)segrho;
+            lm = segm;
             return true;
         }
         return false;

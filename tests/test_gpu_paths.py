@@ -71,6 +71,26 @@ def test_voronoi_bitexact():
     assert_paths_equal(g, sc, random_rays(1500, 6))
 
 
+def test_voronoi_clustered_sites():
+    """Sites concentrated in a thin disc (the config2 density) with the builder's true neighbour
+    lists: the greedy point location over the lists (walk starts, whichcell) must find the exact
+    nearest site, and the walks must be the oracle's bit for bit."""
+    from skirt7_amd import workloads
+    _, dgeom, _, _ = workloads._galaxy(True, 1.0)
+    sites = workloads.sample_sites(dgeom, 20000, EXT, seed=21)
+    g = S.VoronoiDustGrid(EXT, sites)
+    sc = O.Scene().voronoi(EXT, g.sites, g.nb_offsets, g.nb_ids)
+    rng = np.random.default_rng(22)
+    pts = np.concatenate([workloads.sample_sites(dgeom, 3000, EXT, seed=23), rng.uniform(-20.5, 20.5, (1000, 3)),
+                          sites[:200]])
+    got = g.whichcell(pts)
+    ref = np.array([sc.whichcell(p) for p in pts])
+    assert np.array_equal(got, ref)
+    rays = random_rays(600, 24)
+    rays[:300, :3] = workloads.sample_sites(dgeom, 300, EXT, seed=25)
+    assert_paths_equal(g, sc, rays)
+
+
 def test_whichcell_all_grids():
     rng = np.random.default_rng(9)
     pts = rng.uniform(-25, 25, (4000, 3))
