@@ -165,6 +165,19 @@ int sg_set_pipeline(int mode);
  * threshold (lanes that must finish their walk before the wave re-arms them; by default 32 for the
  * lane pipeline and 16 for the walk kernel). */
 int sg_set_tuning(int block, int waves_per_cu, int compact_lanes);
+/* In-flight packet slots of the event/walk pipeline, split over its two slot groups (0 = the
+ * default 2^22; otherwise 1024..2^24). No reference counterpart: SKIRT7's chunk size
+ * (MonteCarloSimulation.cpp:68-114) sets the CPU work granularity, this the GPU's. */
+int sg_set_slots(int64_t slots);
+/* Launch-parameter autotuner (no reference counterpart; the GPU analogue of the chunk rule of
+ * MonteCarloSimulation::setChunkParams, MonteCarloSimulation.cpp:68-114): times sg_sim_run on the
+ * sample of packets [first, first+sample) over the compaction threshold (8..48 lanes), then the
+ * in-flight slots (2^20..2^23, event/walk pipeline) or waves per CU (lane pipeline), then the block
+ * size (64, 128, 256), keeping the fastest of each in turn, and leaves them set for later runs.
+ * best (may be null) receives {block, slots or waves per CU, compaction threshold}; rate (may be
+ * null) the sample's packets per second with them. The results of a run do not depend on these
+ * parameters. The tallies the sample adds are removed again (frames and absorption as before). */
+int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precision, int64_t* best, double* rate);
 /* Smallest member (packets) that sg_sim_run_batch runs alone through the event/walk pipeline
  * instead of in the shared persistent launch (0 = never; default 2^21). */
 int sg_set_batch_alone(uint64_t min_packets);

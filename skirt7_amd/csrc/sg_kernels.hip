@@ -22,6 +22,7 @@ static int g_device = -1;
 static int g_cus = 256;
 // compaction threshold 0: each pipeline's own default (32 lanes for the lane pipeline, 16 for the walk kernel)
 static int g_tune_block = 256, g_tune_waves = 0, g_tune_compact = 0;
+static int64_t g_tune_slots = 0;   // in-flight packet slots of the event/walk pipeline (0: its default)
 static int g_pipeline = 1;   // 0: persistent lane pipeline (k_mc); 1: event and walk kernels (k_events/k_walks)
 static uint64_t g_batch_alone = (uint64_t)1 << 21;   // batched members this large run alone (0: never)
 static int g_topdown_descent = 0;   // 1: mixed walks of TopDown trees descend from the root at every step
@@ -357,10 +358,14 @@ int sg_grid_create_voronoi(const double ext[6], const double* sites, int64_t n, 
     std::vector<int32_t> off(n + 1);
     for (int64_t i = 0; i <= n; i++) off[i] = (int32_t)nb_offsets[i];
     g->voff.upload(off);
-    std::vector<int32_t> ids(nb_ids, nb_ids + nb_offsets[n]);
-    for (int32_t v : ids) SG_REQUIRE(v >= -6 && v < n && v != -99, "neighbour id out of range");
-    if (ids.empty()) ids.push_back(-1);
-    g->vnb.upload(ids);
+    std::vector<VoroNbr> ent(std::max<int64_t>(1, nb_offsets[n]));
+    ent[0] = VoroNbr{0.0, 0.0, 0.0, -1, 0};
+    for (int64_t t = 0; t < nb_offsets[n]; t++) {
+        const int32_t v = nb_ids[t];
+        SG_REQUIRE(v >= -6 && v < n && v != -99, "neighbour id out of range");
+        ent[t] = v >= 0 ? VoroNbr{g->hsite[v].x, g->hsite[v].y, g->hsite[v].z, v, 0} : VoroNbr{0.0, 0.0, 0.0, v, 0};
+    }
+    g->vnb.upload(ent);
     // bucket grid for the nearest-site search: about 2 sites per bucket
     int nbk = std::max(1, std::min(256, (int)std::cbrt((double)n / 2.0)));
     std::vector<int32_t> cnt((size_t)nbk * nbk * nbk + 1, 0), bucket(n);
@@ -414,7 +419,7 @@ int sg_grid_create_voronoi(const double ext[6], const double* sites, int64_t n, 
     VoroView& v = g->voro;
     v.site = g->site.as<VoroSite>();
     v.off = g->voff.as<int32_t>();
-    v.nb = g->vnb.as<int32_t>();
+    v.nbr = g->vnb.as<VoroNbr>();
     v.boff = g->boff.as<int32_t>();
     v.bids = g->bids.as<int32_t>();
     v.brep = g->brep.as<int32_t>();
@@ -999,7 +1004,8 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
     using R = typename W::Real;
     split_resources();
     hipStream_t* st = g_split_stream;
-    const int64_t P = (int64_t)std::max<uint64_t>(1, std::min<uint64_t>(count, (uint64_t)kSplitSlots));
+    const int64_t slots = g_tune_slots > 0 ? g_tune_slots : kSplitSlots;
+    const int64_t P = (int64_t)std::max<uint64_t>(1, std::min<uint64_t>(count, (uint64_t)slots));
     const int G = P >= 2 ? 2 : 1;
     const int64_t Pg[2] = {G == 2 ? (P + 1) / 2 : P, G == 2 ? P / 2 : 0};
     Slots<R> sl[2];
@@ -1022,7 +1028,7 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
         SG_HIP(hipMemsetAsync(sl[g].u + (size_t)SU_STATE * Pg[g], 0, (size_t)Pg[g] * (SU_NU - SU_STATE) * sizeof(uint32_t),
                               st[g]));
     }
-    const int bs = 256;
+    const int bs = g_tune_block;
     const size_t shm = (size_t)nlds * sizeof(int32_t);
     int per = 0;
     SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walks<W, ABS>, bs, shm));
@@ -1208,6 +1214,99 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
     SG_CATCH
 }
 
+int sg_set_slots(int64_t slots)
+{
+    SG_TRY
+    SG_REQUIRE(slots == 0 || (slots >= 1024 && slots <= ((int64_t)1 << 24)), "slots must be 0 or 1024..2^24");
+    g_tune_slots = slots;
+    SG_CATCH
+}
+
+// Coordinate search over the launch parameters of the current pipeline, each candidate timed on the
+// same sample of packets (the histories do not depend on these parameters, only the speed does):
+// the compaction threshold, then the in-flight packet slots (event/walk pipeline) or the waves per
+// CU (lane pipeline), then the block size. Every candidate runs twice and the faster run counts, so
+// that work-buffer growth stays out of the comparison. The sample's tallies are taken back out: the
+// frames and the absorption table hold what they held before.
+int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precision, int64_t* best, double* rate)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(s != nullptr && sample > 0, "autotune needs a simulation and a positive sample");
+    std::vector<std::pair<DevBuf*, void*>> keep;
+    auto save = [&](void* p, size_t bytes) {
+        auto* c = new DevBuf();
+        c->alloc(bytes);
+        SG_HIP(hipMemcpy(c->p, p, bytes, hipMemcpyDeviceToDevice));
+        keep.push_back({c, p});
+    };
+    for (DevBuf* f : s->instr->frames) save(f->p, f->bytes);
+    if (s->dust && s->dust->store_abs) save(s->dust->labs.p, s->dust->labs.bytes);
+    auto trial = [&]() {
+        double r = 0;
+        for (int rep = 0; rep < 2; rep++) {
+            uint64_t c[32] = {0};
+            if (sg_sim_run(s, ell, first, sample, sample, precision, c) != 0) throw std::runtime_error(sg_last_error());
+            r = std::max(r, (double)c[0] / std::max(1e-9, (double)c[10] * 1e-6));
+        }
+        return r;
+    };
+    int block = g_tune_block;
+    int compact = g_tune_compact > 0 ? g_tune_compact : (g_pipeline == 1 ? 24 : 32);
+    int64_t slots = g_tune_slots > 0 ? g_tune_slots : kSplitSlots;
+    int waves = g_tune_waves;
+    g_tune_compact = compact;
+    double top = 0;
+    try {
+        top = trial();
+        for (int c : {8, 16, 24, 32, 40, 48}) {
+            if (c == compact) continue;
+            g_tune_compact = c;
+            const double r = trial();
+            if (r > top) { top = r; compact = c; }
+        }
+        g_tune_compact = compact;
+        if (g_pipeline == 1) {
+            for (int64_t p : {(int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22, (int64_t)1 << 23}) {
+                if (p == slots) continue;
+                g_tune_slots = p;
+                const double r = trial();
+                if (r > top) { top = r; slots = p; }
+            }
+            g_tune_slots = slots;
+        } else {
+            for (int w : {4, 8, 12, 16}) {
+                if (w == waves) continue;
+                g_tune_waves = w;
+                const double r = trial();
+                if (r > top) { top = r; waves = w; }
+            }
+            g_tune_waves = waves;
+        }
+        for (int b : {64, 128, 256}) {
+            if (b == block) continue;
+            g_tune_block = b;
+            const double r = trial();
+            if (r > top) { top = r; block = b; }
+        }
+        g_tune_block = block;
+    } catch (...) {
+        g_tune_block = block; g_tune_compact = compact; g_tune_slots = slots; g_tune_waves = waves;
+        for (auto& k : keep) {
+            (void)hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice);
+            delete k.first;
+        }
+        throw;
+    }
+    for (auto& k : keep) {
+        SG_HIP(hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice));
+        delete k.first;
+    }
+    if (best) { best[0] = block; best[1] = g_pipeline == 1 ? slots : waves; best[2] = compact; }
+    if (rate) *rate = top;
+    SG_CATCH
+}
+
 } // extern "C"
 
 // One round of a batched run: emission for every range, then the batched transport kernel. The
@@ -1386,6 +1485,99 @@ int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const u
         counters[14] = (uint64_t)(lt.kernel_ms() * 1000.0);
     }
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
+    SG_CATCH
+}
+
+int sg_set_slots(int64_t slots)
+{
+    SG_TRY
+    SG_REQUIRE(slots == 0 || (slots >= 1024 && slots <= ((int64_t)1 << 24)), "slots must be 0 or 1024..2^24");
+    g_tune_slots = slots;
+    SG_CATCH
+}
+
+// Coordinate search over the launch parameters of the current pipeline, each candidate timed on the
+// same sample of packets (the histories do not depend on these parameters, only the speed does):
+// the compaction threshold, then the in-flight packet slots (event/walk pipeline) or the waves per
+// CU (lane pipeline), then the block size. Every candidate runs twice and the faster run counts, so
+// that work-buffer growth stays out of the comparison. The sample's tallies are taken back out: the
+// frames and the absorption table hold what they held before.
+int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precision, int64_t* best, double* rate)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(s != nullptr && sample > 0, "autotune needs a simulation and a positive sample");
+    std::vector<std::pair<DevBuf*, void*>> keep;
+    auto save = [&](void* p, size_t bytes) {
+        auto* c = new DevBuf();
+        c->alloc(bytes);
+        SG_HIP(hipMemcpy(c->p, p, bytes, hipMemcpyDeviceToDevice));
+        keep.push_back({c, p});
+    };
+    for (DevBuf* f : s->instr->frames) save(f->p, f->bytes);
+    if (s->dust && s->dust->store_abs) save(s->dust->labs.p, s->dust->labs.bytes);
+    auto trial = [&]() {
+        double r = 0;
+        for (int rep = 0; rep < 2; rep++) {
+            uint64_t c[32] = {0};
+            if (sg_sim_run(s, ell, first, sample, sample, precision, c) != 0) throw std::runtime_error(sg_last_error());
+            r = std::max(r, (double)c[0] / std::max(1e-9, (double)c[10] * 1e-6));
+        }
+        return r;
+    };
+    int block = g_tune_block;
+    int compact = g_tune_compact > 0 ? g_tune_compact : (g_pipeline == 1 ? 24 : 32);
+    int64_t slots = g_tune_slots > 0 ? g_tune_slots : kSplitSlots;
+    int waves = g_tune_waves;
+    g_tune_compact = compact;
+    double top = 0;
+    try {
+        top = trial();
+        for (int c : {8, 16, 24, 32, 40, 48}) {
+            if (c == compact) continue;
+            g_tune_compact = c;
+            const double r = trial();
+            if (r > top) { top = r; compact = c; }
+        }
+        g_tune_compact = compact;
+        if (g_pipeline == 1) {
+            for (int64_t p : {(int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22, (int64_t)1 << 23}) {
+                if (p == slots) continue;
+                g_tune_slots = p;
+                const double r = trial();
+                if (r > top) { top = r; slots = p; }
+            }
+            g_tune_slots = slots;
+        } else {
+            for (int w : {4, 8, 12, 16}) {
+                if (w == waves) continue;
+                g_tune_waves = w;
+                const double r = trial();
+                if (r > top) { top = r; waves = w; }
+            }
+            g_tune_waves = waves;
+        }
+        for (int b : {64, 128, 256}) {
+            if (b == block) continue;
+            g_tune_block = b;
+            const double r = trial();
+            if (r > top) { top = r; block = b; }
+        }
+        g_tune_block = block;
+    } catch (...) {
+        g_tune_block = block; g_tune_compact = compact; g_tune_slots = slots; g_tune_waves = waves;
+        for (auto& k : keep) {
+            (void)hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice);
+            delete k.first;
+        }
+        throw;
+    }
+    for (auto& k : keep) {
+        SG_HIP(hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice));
+        delete k.first;
+    }
+    if (best) { best[0] = block; best[1] = g_pipeline == 1 ? slots : waves; best[2] = compact; }
+    if (rate) *rate = top;
     SG_CATCH
 }
 

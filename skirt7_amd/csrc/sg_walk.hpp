@@ -983,9 +983,9 @@ __device__ int voro_locate(const VoroView& g, double x, double y, double z, int 
         thin = false;
         const int t1 = g.off[cur + 1];
         for (int t = g.off[cur]; t < t1; t++) {
-            const int mi = g.nb[t];
+            const VoroNbr p = g.nbr[t];
+            const int mi = p.id;
             if (mi < 0) continue;
-            const VoroSite p = g.site[mi];
             const double d = (x - p.x) * (x - p.x) + (y - p.y) * (y - p.y) + (z - p.z) * (z - p.z);
             thin = thin || d - dc <= 1e-9 * d;
             if (d < db || (d == db && mi < best)) { db = d; best = mi; }
@@ -1052,10 +1052,10 @@ template <typename R> struct VoroW {
             int mq = NO_INDEX;
             int t0 = g.off[mr], t1 = g.off[mr + 1];
             for (int t = t0; t < t1; t++) {
-                int mi = g.nb[t];
+                const VoroNbr pi = g.nbr[t];
+                const int mi = pi.id;
                 double si = 0;
                 if (mi >= 0) {
-                    VoroSite pi = g.site[mi];
                     double nx = pi.x - pr.x, ny = pi.y - pr.y, nz = pi.z - pr.z;
                     double ndotk = nx * kx + ny * ky + nz * kz;
                     if (ndotk > 0) {

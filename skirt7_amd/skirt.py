@@ -582,6 +582,24 @@ def set_tuning(block=0, waves_per_cu=0, compact_lanes=0):
     check(lib().sg_set_tuning(block, waves_per_cu, compact_lanes))
 
 
+def set_slots(slots=0):
+    """In-flight packet slots of the event/walk pipeline (0: the default)."""
+    check(lib().sg_set_slots(int(slots)))
+
+
+def autotune(sim, ell=0, first=0, sample=2_000_000, precision=1):
+    """Times the simulation on packets [first, first+sample) over the launch parameters (compaction
+    threshold, in-flight slots or waves per CU, block size), keeps the fastest set for later runs and
+    returns it with the sample's rate; the tallies are left as they were."""
+    best = np.zeros(3, np.int64)
+    rate = ctypes.c_double(0.0)
+    check(lib().sg_autotune(sim._h, int(ell), int(first), int(sample), int(precision),
+                            best.ctypes.data_as(capi._I64), ctypes.byref(rate)))
+    key = "slots" if get_pipeline() == 1 else "waves_per_cu"
+    return {"block": int(best[0]), key: int(best[1]), "compact_lanes": int(best[2]),
+            "sample_packets_per_s": rate.value}
+
+
 def device_name():
     buf = ctypes.create_string_buffer(256)
     check(lib().sg_device_name(buf, 256))
