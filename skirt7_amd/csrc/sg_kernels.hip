@@ -1488,97 +1488,4 @@ int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const u
     SG_CATCH
 }
 
-int sg_set_slots(int64_t slots)
-{
-    SG_TRY
-    SG_REQUIRE(slots == 0 || (slots >= 1024 && slots <= ((int64_t)1 << 24)), "slots must be 0 or 1024..2^24");
-    g_tune_slots = slots;
-    SG_CATCH
-}
-
-// Coordinate search over the launch parameters of the current pipeline, each candidate timed on the
-// same sample of packets (the histories do not depend on these parameters, only the speed does):
-// the compaction threshold, then the in-flight packet slots (event/walk pipeline) or the waves per
-// CU (lane pipeline), then the block size. Every candidate runs twice and the faster run counts, so
-// that work-buffer growth stays out of the comparison. The sample's tallies are taken back out: the
-// frames and the absorption table hold what they held before.
-int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precision, int64_t* best, double* rate)
-{
-    SG_TRY
-    ensure_init();
-    SG_REQUIRE(s != nullptr && sample > 0, "autotune needs a simulation and a positive sample");
-    std::vector<std::pair<DevBuf*, void*>> keep;
-    auto save = [&](void* p, size_t bytes) {
-        auto* c = new DevBuf();
-        c->alloc(bytes);
-        SG_HIP(hipMemcpy(c->p, p, bytes, hipMemcpyDeviceToDevice));
-        keep.push_back({c, p});
-    };
-    for (DevBuf* f : s->instr->frames) save(f->p, f->bytes);
-    if (s->dust && s->dust->store_abs) save(s->dust->labs.p, s->dust->labs.bytes);
-    auto trial = [&]() {
-        double r = 0;
-        for (int rep = 0; rep < 2; rep++) {
-            uint64_t c[32] = {0};
-            if (sg_sim_run(s, ell, first, sample, sample, precision, c) != 0) throw std::runtime_error(sg_last_error());
-            r = std::max(r, (double)c[0] / std::max(1e-9, (double)c[10] * 1e-6));
-        }
-        return r;
-    };
-    int block = g_tune_block;
-    int compact = g_tune_compact > 0 ? g_tune_compact : (g_pipeline == 1 ? 24 : 32);
-    int64_t slots = g_tune_slots > 0 ? g_tune_slots : kSplitSlots;
-    int waves = g_tune_waves;
-    g_tune_compact = compact;
-    double top = 0;
-    try {
-        top = trial();
-        for (int c : {8, 16, 24, 32, 40, 48}) {
-            if (c == compact) continue;
-            g_tune_compact = c;
-            const double r = trial();
-            if (r > top) { top = r; compact = c; }
-        }
-        g_tune_compact = compact;
-        if (g_pipeline == 1) {
-            for (int64_t p : {(int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22, (int64_t)1 << 23}) {
-                if (p == slots) continue;
-                g_tune_slots = p;
-                const double r = trial();
-                if (r > top) { top = r; slots = p; }
-            }
-            g_tune_slots = slots;
-        } else {
-            for (int w : {4, 8, 12, 16}) {
-                if (w == waves) continue;
-                g_tune_waves = w;
-                const double r = trial();
-                if (r > top) { top = r; waves = w; }
-            }
-            g_tune_waves = waves;
-        }
-        for (int b : {64, 128, 256}) {
-            if (b == block) continue;
-            g_tune_block = b;
-            const double r = trial();
-            if (r > top) { top = r; block = b; }
-        }
-        g_tune_block = block;
-    } catch (...) {
-        g_tune_block = block; g_tune_compact = compact; g_tune_slots = slots; g_tune_waves = waves;
-        for (auto& k : keep) {
-            (void)hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice);
-            delete k.first;
-        }
-        throw;
-    }
-    for (auto& k : keep) {
-        SG_HIP(hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice));
-        delete k.first;
-    }
-    if (best) { best[0] = block; best[1] = g_pipeline == 1 ? slots : waves; best[2] = compact; }
-    if (rate) *rate = top;
-    SG_CATCH
-}
-
 } // extern "C"
