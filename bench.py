@@ -66,6 +66,13 @@ def parse():
                     help="1: event and walk kernels (default); 0: single lane-pipeline kernel")
     ap.add_argument("--no-equal-error", action="store_true",
                     help="skip the FP64 reference run with ten times the packets")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch parameters instead of tuning them on a sample first")
+    ap.add_argument("--tuning", default=None, metavar="BLOCK,SLOTS,COMPACT",
+                    help="fixed launch parameters of the mixed steps instead of the autotuner (e.g. the ones a "
+                         "default run picked, for a profiled run whose kernel list holds no tuning trials)")
+    ap.add_argument("--autotune-sample", type=float, default=None,
+                    help="packets per autotuner trial (default: one step; a quarter of that for FP64)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (5 times), the reference kernel for scaling "
                          "the profiler's FETCH_SIZE / WRITE_SIZE counters to HBM bytes")
@@ -214,6 +221,19 @@ def main():
                                                             S.device_name()))
     sim = w["sim"]
     npp = total
+    sample = int(a.autotune_sample or per)
+    tuning = {}
+    if a.tuning:
+        b, sl, c = (int(float(x)) for x in a.tuning.split(","))
+        S.set_tuning(b, 0, c)
+        S.set_slots(sl)
+        tuning["mixed"] = {"block": b, "slots": sl, "compact_lanes": c, "fixed": True}
+        a.no_autotune = True
+    elif not a.no_autotune:
+        # launch parameters picked on a sample of the workload's own packets, outside the timed region
+        t0 = time.perf_counter()
+        tuning["mixed"] = S.autotune(sim, ell, 0, sample, SG_MIXED)
+        log("autotune (mixed, %d-packet trials, %.1f s): %s" % (sample, time.perf_counter() - t0, tuning["mixed"]))
 
     # warmup (mixed), then the timed mixed steps
     run_steps(sim, 0, a.warmup, per, npp, SG_MIXED, ell)
@@ -235,7 +255,7 @@ def main():
            "config": {"workload": "%s, %d cells, %d packets per step, %d frames of %d^2" % (
                           desc, ncell, per, len(w["instr"].instruments), w["instr"].instruments[0].Nx),
                       "packets_per_step": per, "cells": ncell, "instruments": len(w["instr"].instruments),
-                      "pixels": w["instr"].instruments[0].Nx}}
+                      "pixels": w["instr"].instruments[0].Nx, "tuning": tuning}}
     # algorithmic bytes of the walk kernel per launch over its average launch duration (HIP events on the
     # launch's own stream, summed by the library); the walk passes of the two slot groups overlap, so the
     # whole transport's rate over its device span is reported beside it
@@ -271,6 +291,11 @@ def main():
         img64 = None
         for p in (1, 0):
             S.set_pipeline(p)
+            if not a.no_autotune:
+                S.set_tuning(-1, -1, -1)
+                S.set_slots(0)
+                tuning["fp64_pipeline%d" % p] = S.autotune(sim, ell, first, max(1, sample // 4), SG_FP64)
+                log("autotune (fp64, pipeline %d): %s" % (p, tuning["fp64_pipeline%d" % p]))
             w["instr"].clear()
             t0 = time.perf_counter()
             c64 = sim.run_packets(ell, first, per, per, SG_FP64)

@@ -161,9 +161,10 @@ int sg_fit_frames(int n, sg_instruments* const* instr, int instrument, int ell, 
  *   packet state machine and queues walks, and a persistent walk kernel over that queue;
  * 0 one persistent kernel whose lanes run the state machine and the walks in turn. */
 int sg_set_pipeline(int mode);
-/* Launch tuning used by sg_sim_run (0 = keep current): block size, waves per CU, compaction
- * threshold (lanes that must finish their walk before the wave re-arms them; by default 32 for the
- * lane pipeline and 16 for the walk kernel). */
+/* Launch tuning used by sg_sim_run (0 = keep current, negative = back to the default): block size
+ * (default 256), waves per CU (default: as many as fit), compaction threshold (lanes that must
+ * finish their walk before the wave re-arms them; by default 32 for the lane pipeline and 24 for
+ * the walk kernel). */
 int sg_set_tuning(int block, int waves_per_cu, int compact_lanes);
 /* In-flight packet slots of the event/walk pipeline, split over its two slot groups (0 = the
  * default 2^22; otherwise 1024..2^24). No reference counterpart: SKIRT7's chunk size
@@ -172,7 +173,8 @@ int sg_set_slots(int64_t slots);
 /* Launch-parameter autotuner (no reference counterpart; the GPU analogue of the chunk rule of
  * MonteCarloSimulation::setChunkParams, MonteCarloSimulation.cpp:68-114): times sg_sim_run on the
  * sample of packets [first, first+sample) over the compaction threshold (8..48 lanes), then the
- * in-flight slots (2^20..2^23, event/walk pipeline) or waves per CU (lane pipeline), then the block
+ * in-flight slots (2^20..2^23 up to the sample size, event/walk pipeline) or waves per CU (lane
+ * pipeline), then the block
  * size (64, 128, 256), keeping the fastest of each in turn, and leaves them set for later runs.
  * best (may be null) receives {block, slots or waves per CU, compaction threshold}; rate (may be
  * null) the sample's packets per second with them. The results of a run do not depend on these

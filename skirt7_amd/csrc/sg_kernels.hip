@@ -20,7 +20,7 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 static int g_device = -1;
 static int g_cus = 256;
-// compaction threshold 0: each pipeline's own default (32 lanes for the lane pipeline, 16 for the walk kernel)
+// compaction threshold 0: each pipeline's own default (32 lanes for the lane pipeline, 24 for the walk kernel)
 static int g_tune_block = 256, g_tune_waves = 0, g_tune_compact = 0;
 static int64_t g_tune_slots = 0;   // in-flight packet slots of the event/walk pipeline (0: its default)
 static int g_pipeline = 1;   // 0: persistent lane pipeline (k_mc); 1: event and walk kernels (k_events/k_walks)
@@ -212,8 +212,11 @@ int sg_set_mixed_topdown(int descend)
 int sg_set_tuning(int block, int waves_per_cu, int compact_lanes)
 {
     if (block == 64 || block == 128 || block == 256) g_tune_block = block;
+    if (block < 0) g_tune_block = 256;
     if (waves_per_cu > 0) g_tune_waves = waves_per_cu;
+    if (waves_per_cu < 0) g_tune_waves = 0;
     if (compact_lanes > 0 && compact_lanes <= 64) g_tune_compact = compact_lanes;
+    if (compact_lanes < 0) g_tune_compact = 0;
     return 0;
 }
 
@@ -357,6 +360,9 @@ int sg_grid_create_voronoi(const double ext[6], const double* sites, int64_t n, 
     g->site.upload(g->hsite);
     std::vector<int32_t> off(n + 1);
     for (int64_t i = 0; i <= n; i++) off[i] = (int32_t)nb_offsets[i];
+    SG_REQUIRE(nb_offsets[0] == 0, "neighbour offsets must start at 0");
+    for (int64_t i = 0; i < n; i++)
+        SG_REQUIRE(nb_offsets[i + 1] > nb_offsets[i], "every Voronoi cell needs at least one neighbour or wall");
     g->voff.upload(off);
     std::vector<VoroNbr> ent(std::max<int64_t>(1, nb_offsets[n]));
     ent[0] = VoroNbr{0.0, 0.0, 0.0, -1, 0};
@@ -1268,7 +1274,7 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
         g_tune_compact = compact;
         if (g_pipeline == 1) {
             for (int64_t p : {(int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22, (int64_t)1 << 23}) {
-                if (p == slots) continue;
+                if (p == slots || (uint64_t)p > sample) continue;   // beyond the sample: all slots alike
                 g_tune_slots = p;
                 const double r = trial();
                 if (r > top) { top = r; slots = p; }

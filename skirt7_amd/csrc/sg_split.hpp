@@ -374,6 +374,7 @@ template <> struct WalkWaves<TreeW32> { static constexpr int value = 5; };
 #define SG_W64_WAVES 3
 #endif
 template <> struct WalkWaves<TreeW64> { static constexpr int value = SG_W64_WAVES; };
+template <> struct WalkWaves<VoroW<double>> { static constexpr int value = 3; };   // 4 would spill
 
 // Walk pass: persistent lanes over the requests [0, *q.tail).
 template <class W, bool ABS>

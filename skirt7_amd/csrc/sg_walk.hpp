@@ -981,9 +981,11 @@ __device__ int voro_locate(const VoroView& g, double x, double y, double z, int 
         int best = cur;
         double db = dc;
         thin = false;
-        const int t1 = g.off[cur + 1];
-        for (int t = g.off[cur]; t < t1; t++) {
-            const VoroNbr p = g.nbr[t];
+        const int t0 = g.off[cur], t1 = g.off[cur + 1];
+        VoroNbr ahead = g.nbr[t0];
+        for (int t = t0; t < t1; t++) {
+            const VoroNbr p = ahead;
+            if (t + 1 < t1) ahead = g.nbr[t + 1];
             const int mi = p.id;
             if (mi < 0) continue;
             const double d = (x - p.x) * (x - p.x) + (y - p.y) * (y - p.y) + (z - p.z) * (z - p.z);
@@ -1050,9 +1052,13 @@ template <typename R> struct VoroW {
             double sq = DBL_MAX;
             const int NO_INDEX = -99;
             int mq = NO_INDEX;
-            int t0 = g.off[mr], t1 = g.off[mr + 1];
+            const int t0 = g.off[mr], t1 = g.off[mr + 1];
+            // every cell has at least one entry (checked at creation); the next entry is loaded
+            // while the current one is tested
+            VoroNbr ahead = g.nbr[t0];
             for (int t = t0; t < t1; t++) {
-                const VoroNbr pi = g.nbr[t];
+                const VoroNbr pi = ahead;
+                if (t + 1 < t1) ahead = g.nbr[t + 1];
                 const int mi = pi.id;
                 double si = 0;
                 if (mi >= 0) {

@@ -561,9 +561,13 @@ def fit_frames(instrument_systems, reference, kernel, minlum, maxlum, instrument
     return (lum, chi2, frames) if return_frames else (lum, chi2)
 
 
+_pipeline_mode = [1]
+
+
 def set_pipeline(mode):
     """1: event and walk kernels over device queues (default); 0: one persistent lane-pipeline kernel."""
     check(lib().sg_set_pipeline(int(mode)))
+    _pipeline_mode[0] = int(mode)
 
 
 def set_batch_alone(min_packets):
@@ -579,7 +583,12 @@ def set_mixed_topdown(descend):
 
 
 def set_tuning(block=0, waves_per_cu=0, compact_lanes=0):
+    """Launch parameters of sg_sim_run: 0 keeps the current value, a negative value restores the default."""
     check(lib().sg_set_tuning(block, waves_per_cu, compact_lanes))
+
+
+def get_pipeline():
+    return _pipeline_mode[0]
 
 
 def set_slots(slots=0):

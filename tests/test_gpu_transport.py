@@ -142,3 +142,29 @@ def test_kdtree_mixed_matches_fp64(c4_small):
         out.append(frames(w))
     for a, b in zip(*out):
         assert abs(a.sum() - b.sum()) <= 2e-3 * a.sum()
+
+
+def test_autotune_keeps_tallies_and_results(c0_small):
+    """The autotuner times the sample with several launch parameter sets and leaves the frames as they
+    were; a run with the parameters it picked gives the image of the default parameters."""
+    from skirt7_amd import skirt as S
+    w = c0_small
+    n = 6000
+    try:
+        w["instr"].clear()
+        w["sim"].run_packets(0, 0, n, n, SG_FP64)
+        before = [f.copy() for f in frames(w)]
+        best = S.autotune(w["sim"], ell=0, first=0, sample=4000, precision=SG_FP64)
+        assert best["block"] in (64, 128, 256)
+        assert best["compact_lanes"] in (8, 16, 24, 32, 40, 48)
+        assert best["slots"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+        assert best["sample_packets_per_s"] > 0
+        for a, b in zip(before, frames(w)):
+            assert np.array_equal(a, b)
+        w["instr"].clear()
+        w["sim"].run_packets(0, 0, n, n, SG_FP64)
+        for a, b in zip(before, frames(w)):
+            np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-300)
+    finally:
+        S.set_tuning(-1, -1, -1)
+        S.set_slots(0)
