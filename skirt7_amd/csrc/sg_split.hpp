@@ -375,6 +375,7 @@ template <> struct WalkWaves<TreeW32> { static constexpr int value = 5; };
 #endif
 template <> struct WalkWaves<TreeW64> { static constexpr int value = SG_W64_WAVES; };
 template <> struct WalkWaves<VoroW<double>> { static constexpr int value = 3; };   // 4 would spill
+template <> struct WalkWaves<VoroW<float>> { static constexpr int value = 3; };
 
 // Walk pass: persistent lanes over the requests [0, *q.tail).
 template <class W, bool ABS>

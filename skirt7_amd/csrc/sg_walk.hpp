@@ -1044,6 +1044,33 @@ template <typename R> struct VoroW {
         mr = voro_locate(g, x, y, z, en.ne ? -1 : hn);
         if (mr < 0) { en.ne = 0; done = true; }
     }
+    // The distance along the ray to the bisector plane (or wall) of one neighbour entry of the cell with
+    // site pr, taken when it lies ahead and is the nearest so far (VoronoiMesh.cpp:806-858).
+    __device__ __forceinline__ void plane(const View& g, const VoroSite& pr, const VoroNbr& pi, double& sq,
+                                          int& mq) const
+    {
+        const int mi = pi.id;
+        double si = 0;
+        if (mi >= 0) {
+            double nx = pi.x - pr.x, ny = pi.y - pr.y, nz = pi.z - pr.z;
+            double ndotk = nx * kx + ny * ky + nz * kz;
+            if (ndotk > 0) {
+                double px = 0.5 * (pi.x + pr.x), py = 0.5 * (pi.y + pr.y), pz = 0.5 * (pi.z + pr.z);
+                si = (nx * (px - x) + ny * (py - y) + nz * (pz - z)) / ndotk;
+            }
+        } else {
+            switch (mi) {
+            case -1: si = (g.ext[0] - x) / kx; break;
+            case -2: si = (g.ext[3] - x) / kx; break;
+            case -3: si = (g.ext[1] - y) / ky; break;
+            case -4: si = (g.ext[4] - y) / ky; break;
+            case -5: si = (g.ext[2] - z) / kz; break;
+            case -6: si = (g.ext[5] - z) / kz; break;
+            default: si = 0;
+            }
+        }
+        if (si > 0 && si < sq) { sq = si; mq = mi; }
+    }
     __device__ __forceinline__ bool next(const View& g, const int32_t*, int, int& mo, R& ds, R& rh)
     {
         if (en.ne) { ds = (R)en.pop(); mo = -1; rh = (R)0; return true; }
@@ -1053,33 +1080,17 @@ template <typename R> struct VoroW {
             const int NO_INDEX = -99;
             int mq = NO_INDEX;
             const int t0 = g.off[mr], t1 = g.off[mr + 1];
-            // every cell has at least one entry (checked at creation); the next entry is loaded
-            // while the current one is tested
-            VoroNbr ahead = g.nbr[t0];
-            for (int t = t0; t < t1; t++) {
-                const VoroNbr pi = ahead;
-                if (t + 1 < t1) ahead = g.nbr[t + 1];
-                const int mi = pi.id;
-                double si = 0;
-                if (mi >= 0) {
-                    double nx = pi.x - pr.x, ny = pi.y - pr.y, nz = pi.z - pr.z;
-                    double ndotk = nx * kx + ny * ky + nz * kz;
-                    if (ndotk > 0) {
-                        double px = 0.5 * (pi.x + pr.x), py = 0.5 * (pi.y + pr.y), pz = 0.5 * (pi.z + pr.z);
-                        si = (nx * (px - x) + ny * (py - y) + nz * (pz - z)) / ndotk;
-                    }
-                } else {
-                    switch (mi) {
-                    case -1: si = (g.ext[0] - x) / kx; break;
-                    case -2: si = (g.ext[3] - x) / kx; break;
-                    case -3: si = (g.ext[1] - y) / ky; break;
-                    case -4: si = (g.ext[4] - y) / ky; break;
-                    case -5: si = (g.ext[2] - z) / kz; break;
-                    case -6: si = (g.ext[5] - z) / kz; break;
-                    default: si = 0;
-                    }
-                }
-                if (si > 0 && si < sq) { sq = si; mq = mi; }
+            // entries in blocks of four loaded together (every cell has at least one entry, checked at
+            // creation; past the end the block repeats the last entry, whose second test changes
+            // nothing: the nearest plane ahead is taken on a strict "<")
+            for (int t = t0; t < t1; t += 4) {
+                const int last = t1 - 1;
+                const VoroNbr q0 = g.nbr[t], q1 = g.nbr[min(t + 1, last)];
+                const VoroNbr q2 = g.nbr[min(t + 2, last)], q3 = g.nbr[min(t + 3, last)];
+                plane(g, pr, q0, sq, mq);
+                plane(g, pr, q1, sq, mq);
+                plane(g, pr, q2, sq, mq);
+                plane(g, pr, q3, sq, mq);
             }
             if (++steps > SG_MAX_STEPS) { done = true; return false; }
             if (mq == NO_INDEX) {

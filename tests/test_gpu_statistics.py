@@ -1,0 +1,127 @@
+"""Full runs against the reference's own random-number scheme: images agree within Monte Carlo noise.
+
+The oracle's reference mode (or_run_reference) draws every packet from per-thread MT19937 streams seeded
+as Random.cpp:41-56 and splits the packets with the chunk rule of setChunkParams
+(MonteCarloSimulation.cpp:68-114); the GPU gives every packet its own Philox stream. The two runs share
+no random numbers, so they can only agree statistically: over K independent batches per side, every
+(rebinned) pixel's mean and standard error give a z-score. The z-scores must be centred with unit
+spread, and the total detected luminosity of every instrument must agree within 3 sigma. The same holds
+for the absorbed luminosity per cell group of the Voronoi absorption table.
+"""
+import numpy as np
+import pytest
+
+from skirt7_amd import workloads
+from skirt7_amd.capi import SG_FP64, SG_MIXED
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+K = 8          # independent batches per side
+N = 20000      # packets per batch
+
+
+def rebin(frame, f):
+    ny, nx = frame.shape
+    return frame.reshape(ny // f, f, nx // f, f).sum(axis=(1, 3))
+
+
+def split_frames(w, flat):
+    out, off = [], 0
+    for ins in w["instr"].instruments:
+        n = ins.Nx * ins.Ny
+        out.append(flat[off:off + n].reshape(ins.Ny, ins.Nx))
+        off += n
+    return out
+
+
+def zscores(a, b, rel=0.1):
+    """a, b: (K, n) batch values; z per column that both sides sample well (standard error below rel of
+    the mean, where the batch means are close to normal)."""
+    ma, mb = a.mean(0), b.mean(0)
+    va, vb = a.var(0, ddof=1) / a.shape[0], b.var(0, ddof=1) / b.shape[0]
+    keep = (ma > 0) & (mb > 0) & (va > 0) & (vb > 0) & (np.sqrt(va) < rel * ma) & (np.sqrt(vb) < rel * mb)
+    return (ma[keep] - mb[keep]) / np.sqrt(va[keep] + vb[keep])
+
+
+def check_z(z, what):
+    assert len(z) >= 50, (what, len(z))
+    # K = 8 batches: each side's standard error has 7 degrees of freedom, so the spread of z is somewhat
+    # above 1 (about 1.1-1.2 for Student-t tails)
+    assert abs(z.mean()) < 0.3, (what, z.mean())
+    assert 0.7 < z.std() < 1.5, (what, z.std())
+    assert np.mean(np.abs(z) > 4.0) < 0.02, (what, np.mean(np.abs(z) > 4.0))
+
+
+@pytest.fixture(scope="module")
+def c0_stat():
+    return workloads.config0(grid_n=16, pixels=48, packages=N)
+
+
+@pytest.fixture(scope="module")
+def c0_reference(c0_stat):
+    w = c0_stat
+    sc = O.scene_from_workload(w)
+    batches = []
+    for k in range(K):
+        mc = np.array(w["sim"].mc, dtype=np.float64)
+        mc[0] = N
+        mc[4] = 4357 + 1000 + 16 * k   # thread t of a run seeds seed + t: no stream shared between batches
+        flat, c = sc.run_reference(mc, threads=8)
+        assert c[0] >= N
+        batches.append(split_frames(w, flat))
+    return batches
+
+
+@pytest.mark.parametrize("precision", [SG_FP64, SG_MIXED])
+def test_images_agree_with_reference_streams(c0_stat, c0_reference, precision):
+    w = c0_stat
+    gpu = []
+    for k in range(K):
+        w["instr"].clear()
+        w["sim"].run_packets(0, k * N, N, N, precision)
+        gpu.append([w["instr"].frame(i)[0].copy() for i in range(len(w["instr"].instruments))])
+    zs = []
+    for i in range(len(w["instr"].instruments)):
+        g = np.array([rebin(b[i], 3).ravel() for b in gpu])
+        r = np.array([rebin(b[i], 3).ravel() for b in c0_reference])
+        zt = zscores(g.sum(1, keepdims=True), r.sum(1, keepdims=True))
+        print("instrument %d: totals gpu %.6e ref %.6e, z %s" % (i, g.sum(1).mean(), r.sum(1).mean(), zt), flush=True)
+        assert len(zt) == 1 and abs(zt[0]) < 3.0, ("total flux", i, zt)
+        zs.append(zscores(g, r))
+    z = np.concatenate(zs)
+    print("pixel z-scores: %d pixels, mean %.3f, std %.3f" % (len(z), z.mean(), z.std()), flush=True)
+    check_z(z, "pixels")
+
+
+def test_absorption_agrees_with_reference_streams():
+    """The Voronoi absorption table (OligoDustSystem::absorb) per group of 16 cells, in FP64 and mixed."""
+    w = workloads.config2(nsites=3000, nlambda=1, pixels=16, packages=N)
+    sc = O.scene_from_workload(w)
+    ncell = w["grid"].numCells()
+    groups = np.arange(ncell) // 16
+
+    def grouped(labs):
+        return np.bincount(groups, weights=labs, minlength=groups.max() + 1)
+
+    ref = []
+    for k in range(K):
+        mc = np.array(w["sim"].mc, dtype=np.float64)
+        mc[0] = N
+        mc[4] = 4357 + 2000 + 16 * k
+        sc.run_reference(mc, threads=8)
+        ref.append(grouped(sc.labs.reshape(ncell, -1)[:, 0]))
+    ref = np.array(ref)
+    for precision in (SG_FP64, SG_MIXED):
+        gpu = []
+        for k in range(K):
+            w["dust"].clear()
+            w["sim"].run_packets(0, k * N, N, N, precision)
+            gpu.append(grouped(w["dust"].Labs()[:, 0]))
+        gpu = np.array(gpu)
+        z = zscores(gpu, ref)
+        zt = zscores(gpu.sum(1, keepdims=True), ref.sum(1, keepdims=True))
+        print("absorption (precision %d): %d groups, z mean %.3f std %.3f; total z %s" % (
+            precision, len(z), z.mean(), z.std(), zt), flush=True)
+        check_z(z, ("absorption", precision))
+        assert len(zt) == 1 and abs(zt[0]) < 3.0, ("total absorbed", precision, zt)
