@@ -45,9 +45,10 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="config1", choices=("config1", "config2", "config3", "config4"),
-                    help="config1 octree (the metric's workload), config2 Voronoi with the absorption table, "
-                         "config3 batched population + fit, config4 kd-tree")
+    ap.add_argument("--workload", default="config1", choices=("config0", "config1", "config2", "config3", "config4"),
+                    help="config1 octree (the metric's workload), config0 plain Cartesian 50^3 (the CPU-runnable "
+                         "case), config2 Voronoi with the absorption table, config3 batched population + fit, "
+                         "config4 kd-tree")
     ap.add_argument("--members", type=int, default=256, help="config3: population size")
     ap.add_argument("--member-packets", type=float, nargs=2, default=(1e5, 1e7),
                     help="config3: log-uniform range of packets per member")
@@ -204,7 +205,10 @@ def main():
     total = per * (a.steps + a.warmup)
     t0 = time.perf_counter()
     ell = 0
-    if a.workload == "config1":
+    if a.workload == "config0":
+        w = workloads.config0(pixels=min(a.pixels, 256), packages=total)
+        desc = "config0 Cartesian 50^3 exponential disc + Sersic bulge, face-on tau 1"
+    elif a.workload == "config1":
         w = workloads.config1(tau=a.tau, maxlevel=a.maxlevel, pixels=a.pixels, packages=total, tau_cut=a.tau_cut)
         desc = "config1 octree spiral galaxy, face-on tau %.3g" % a.tau
     elif a.workload == "config2":
