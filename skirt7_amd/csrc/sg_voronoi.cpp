@@ -12,7 +12,6 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
-#include <unordered_map>
 #include <vector>
 
 #include "../../include/skirt7_amd.h"
@@ -27,20 +26,30 @@ struct V3 {
 inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
-// Convex polyhedron as faces over a shared vertex pool.
+// Convex polyhedron as faces over a shared vertex pool: face k is fv[fo[k] .. fo[k+1]) with tag[k].
+// All storage is reused from clip to clip (no allocation once the buffers have grown).
 struct Cell {
-    std::vector<V3> v;
-    std::vector<std::vector<int>> f;
-    std::vector<int> tag;
+    std::vector<V3> v, nv2;
+    std::vector<int> fv, fo, tag;
+    // clip work buffers
+    std::vector<double> s;
+    std::vector<int> nfv, nfo, ntag, loop, remap;
+    std::vector<std::pair<uint64_t, int>> cut;   // edge -> new vertex
+    std::vector<std::pair<int, int>> nxt;        // new face edges: from -> to along the cut
 
     void box(const double* e)
     {
-        v.clear(); f.clear(); tag.clear();
+        v.clear(); fv.clear(); fo.clear(); tag.clear();
         for (int q = 0; q < 8; q++)
             v.push_back({(q & 1) ? e[3] : e[0], (q & 2) ? e[4] : e[1], (q & 4) ? e[5] : e[2]});
         // faces ordered counter-clockwise seen from outside; tags are the wall ids
-        f = {{0, 4, 6, 2}, {1, 3, 7, 5}, {0, 1, 5, 4}, {2, 6, 7, 3}, {0, 2, 3, 1}, {4, 5, 7, 6}};
-        tag = {-1, -2, -3, -4, -5, -6};
+        static const int faces[6][4] = {{0, 4, 6, 2}, {1, 3, 7, 5}, {0, 1, 5, 4}, {2, 6, 7, 3}, {0, 2, 3, 1}, {4, 5, 7, 6}};
+        fo.push_back(0);
+        for (int k = 0; k < 6; k++) {
+            for (int q = 0; q < 4; q++) fv.push_back(faces[k][q]);
+            fo.push_back((int)fv.size());
+            tag.push_back(-1 - k);
+        }
     }
 
     double radius2(V3 p) const
@@ -53,82 +62,91 @@ struct Cell {
     // Keeps the part with dot(n, x) <= d. Returns true when the plane cut the cell.
     bool clip(V3 n, double d, int t, double eps)
     {
-        size_t nv = v.size();
-        std::vector<double> s(nv);
+        const int nv = (int)v.size();
+        s.resize(nv);
         bool any_out = false;
-        for (size_t i = 0; i < nv; i++) {
+        for (int i = 0; i < nv; i++) {
             s[i] = dot(n, v[i]) - d;
             if (s[i] > eps) any_out = true;
         }
         if (!any_out) return false;
-        std::unordered_map<uint64_t, int> cut;   // edge -> new vertex
+        cut.clear();
         auto edge_point = [&](int a, int b) {
             uint64_t key = a < b ? ((uint64_t)a << 32 | (uint32_t)b) : ((uint64_t)b << 32 | (uint32_t)a);
-            auto it = cut.find(key);
-            if (it != cut.end()) return it->second;
+            for (const auto& c : cut)
+                if (c.first == key) return c.second;
             double ta = s[a] / (s[a] - s[b]);
             V3 p{v[a].x + ta * (v[b].x - v[a].x), v[a].y + ta * (v[b].y - v[a].y), v[a].z + ta * (v[b].z - v[a].z)};
             int id = (int)v.size();
             v.push_back(p);
-            cut.emplace(key, id);
+            cut.push_back({key, id});
             return id;
         };
-        std::vector<std::vector<int>> nf;
-        std::vector<int> ntag;
-        std::unordered_map<int, int> next;   // new face edges: from -> to along the cut
-        for (size_t k = 0; k < f.size(); k++) {
-            const auto& poly = f[k];
-            std::vector<int> out;
-            int m = (int)poly.size();
+        nfv.clear(); nfo.clear(); ntag.clear(); nxt.clear();
+        nfo.push_back(0);
+        const int nf = (int)tag.size();
+        for (int k = 0; k < nf; k++) {
+            const int f0 = fo[k], m = fo[k + 1] - f0;
+            const int start = (int)nfv.size();
             int enter = -1, leave = -1;
             for (int i = 0; i < m; i++) {
-                int a = poly[i], b = poly[(i + 1) % m];
+                int a = fv[f0 + i], b = fv[f0 + (i + 1) % m];
                 bool ina = s[a] <= eps, inb = s[b] <= eps;
-                if (ina) out.push_back(a);
+                if (ina) nfv.push_back(a);
                 if (ina && !inb) {
                     int c = (s[a] < -eps) ? edge_point(a, b) : a;
-                    if (c != a) out.push_back(c);
+                    if (c != a) nfv.push_back(c);
                     leave = c;
                 } else if (!ina && inb) {
                     int c = (s[b] < -eps) ? edge_point(a, b) : b;
-                    if (c != b) out.push_back(c);
+                    if (c != b) nfv.push_back(c);
                     enter = c;
                 }
             }
-            if (out.size() >= 3) {
-                nf.push_back(out);
+            if ((int)nfv.size() - start >= 3) {
+                nfo.push_back((int)nfv.size());
                 ntag.push_back(tag[k]);
                 // the cut runs along this face from `leave` to `enter`; the new face traverses it backwards
-                if (enter >= 0 && leave >= 0 && enter != leave) next[enter] = leave;
+                if (enter >= 0 && leave >= 0 && enter != leave) {
+                    bool set = false;
+                    for (auto& e : nxt)
+                        if (e.first == enter) { e.second = leave; set = true; break; }
+                    if (!set) nxt.push_back({enter, leave});
+                }
+            } else {
+                nfv.resize(start);
             }
         }
         // assemble the new face from the chained cut edges
-        if (!next.empty()) {
-            std::vector<int> loop;
-            int start = next.begin()->first, cur = start;
-            for (size_t guard = 0; guard <= next.size() + 1; guard++) {
+        if (!nxt.empty()) {
+            loop.clear();
+            int start = nxt[0].first, cur = start;
+            for (size_t guard = 0; guard <= nxt.size() + 1; guard++) {
                 loop.push_back(cur);
-                auto it = next.find(cur);
-                if (it == next.end()) break;
-                cur = it->second;
+                int to = -1;
+                for (const auto& e : nxt)
+                    if (e.first == cur) { to = e.second; break; }
+                if (to < 0) break;
+                cur = to;
                 if (cur == start) break;
             }
             if (loop.size() >= 3) {
-                nf.push_back(loop);
+                nfv.insert(nfv.end(), loop.begin(), loop.end());
+                nfo.push_back((int)nfv.size());
                 ntag.push_back(t);
             }
         }
-        f.swap(nf);
-        tag.swap(ntag);
         // compact the vertex pool
-        std::vector<int> remap(v.size(), -1);
-        std::vector<V3> nv2;
-        for (auto& poly : f)
-            for (int& i : poly) {
-                if (remap[i] < 0) { remap[i] = (int)nv2.size(); nv2.push_back(v[i]); }
-                i = remap[i];
-            }
+        remap.assign(v.size(), -1);
+        nv2.clear();
+        for (int& i : nfv) {
+            if (remap[i] < 0) { remap[i] = (int)nv2.size(); nv2.push_back(v[i]); }
+            i = remap[i];
+        }
         v.swap(nv2);
+        fv.swap(nfv);
+        fo.swap(nfo);
+        tag.swap(ntag);
         return true;
     }
 };
@@ -179,12 +197,20 @@ bool voronoi_neighbors(const double ext[6], const double* sites, int64_t n, std:
                             for (int c = std::max(0, ck - ring); c <= std::min(nb - 1, ck + ring); c++) {
                                 if (std::max(std::abs(a - ci), std::max(std::abs(b - cj), std::abs(c - ck))) != ring)
                                     continue;
+                                // a bucket whose box lies beyond twice the radius holds no candidate
+                                const double bx = std::max(0.0, std::max(e[0] + a * w[0] - p.x, p.x - (e[0] + (a + 1) * w[0])));
+                                const double by = std::max(0.0, std::max(e[1] + b * w[1] - p.y, p.y - (e[1] + (b + 1) * w[1])));
+                                const double bz = std::max(0.0, std::max(e[2] + c * w[2] - p.z, p.z - (e[2] + (c + 1) * w[2])));
+                                if (bx * bx + by * by + bz * bz > 4.0 * R2 * (1.0 + 1e-9)) continue;
                                 int q = (a * nb + b) * nb + c;
                                 for (int t = cnt[q]; t < cnt[q + 1]; t++) {
                                     int32_t j = bid[t];
                                     if (j == i) continue;
                                     V3 pj{sites[3 * j], sites[3 * j + 1], sites[3 * j + 2]};
-                                    cand.push_back({dot(pj - p, pj - p), j});
+                                    // beyond twice the current radius a site cannot clip (the loop
+                                    // below stops there, and the radius only shrinks)
+                                    const double dj = dot(pj - p, pj - p);
+                                    if (dj <= 4.0 * R2) cand.push_back({dj, j});
                                 }
                             }
                     std::sort(cand.begin(), cand.end());

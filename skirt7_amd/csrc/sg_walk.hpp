@@ -538,7 +538,9 @@ struct TreeW64 {
 // next cell is found by descending from the rope of the exit wall (the node across it that covers
 // the whole wall), one 32-byte node record per level; a directed root descent covers the rare case
 // of a point that left the rope node's box through an edge or corner. The walker carries the
-// current leaf's integer indices along x, y, z; every box is ext + [k, k + 1] * width / 2^n.
+// current leaf's integer indices along x, y, z; every box is ext + [k, k + 1] * width / 2^n. Every
+// wall and split plane is evaluated as one fused multiply-add (a single rounding of the exact value),
+// so a leaf's wall and the split plane its parent used for it are the same float.
 struct TreeW32 {
     using Real = float;
     using View = TreeView;
@@ -599,9 +601,9 @@ struct TreeW32 {
             const int c0 = (int)a.y;
             if (t.arity == 8) {
                 // split points ext + (2k + 1) * width / 2^(n + 1)
-                const float cx = t.extf[0] + (float)(2 * i + 1) * width(t, 0, ns + 1u);
-                const float cy = t.extf[1] + (float)(2 * j + 1) * width(t, 1, ns + (1u << 5));
-                const float cz = t.extf[2] + (float)(2 * k + 1) * width(t, 2, ns + (1u << 10));
+                const float cx = fmaf((float)(2 * i + 1), width(t, 0, ns + 1u), t.extf[0]);
+                const float cy = fmaf((float)(2 * j + 1), width(t, 1, ns + (1u << 5)), t.extf[1]);
+                const float cz = fmaf((float)(2 * k + 1), width(t, 2, ns + (1u << 10)), t.extf[2]);
                 const int qx = (nx ? x <= cx : x < cx) ? 0 : 1;
                 const int qy = (ny ? y <= cy : y < cy) ? 0 : 1;
                 const int qz = (nz ? z <= cz : z < cz) ? 0 : 1;
@@ -616,7 +618,7 @@ struct TreeW32 {
                 const int nd = (int)((ns >> (fz ? 10 : fy ? 5 : 0)) & 31u) + 1;
                 const float p = pick3(fy, fz, x, y, z);
                 const bool neg = fz ? nz : fy ? ny : nx;
-                const float c = e + (float)(2 * kd + 1) * ldexpf(w0, -nd);
+                const float c = fmaf((float)(2 * kd + 1), ldexpf(w0, -nd), e);
                 const int q = (neg ? p <= c : p < c) ? 0 : 1;
                 i = (fy || fz) ? i : 2 * i + q;
                 j = fy ? 2 * j + q : j;
@@ -663,9 +665,9 @@ struct TreeW32 {
         while (w >= 0) {
             int q = 0;
             if (t.arity == 8) {
-                const float cx = e[0] + (float)(2 * ki + 1) * ldexpf(t.widthf[0], -(level + 1));
-                const float cy = e[1] + (float)(2 * kj + 1) * ldexpf(t.widthf[1], -(level + 1));
-                const float cz = e[2] + (float)(2 * kk + 1) * ldexpf(t.widthf[2], -(level + 1));
+                const float cx = fmaf((float)(2 * ki + 1), ldexpf(t.widthf[0], -(level + 1)), e[0]);
+                const float cy = fmaf((float)(2 * kj + 1), ldexpf(t.widthf[1], -(level + 1)), e[1]);
+                const float cz = fmaf((float)(2 * kk + 1), ldexpf(t.widthf[2], -(level + 1)), e[2]);
                 const int qx = (nx ? x <= cx : x < cx) ? 0 : 1;
                 const int qy = (ny ? y <= cy : y < cy) ? 0 : 1;
                 const int qz = (nz ? z <= cz : z < cz) ? 0 : 1;
@@ -678,7 +680,7 @@ struct TreeW32 {
                 const int32_t ka = fz ? kk : fy ? kj : ki;
                 const float p = pick3(fy, fz, x, y, z);
                 const bool neg = fz ? nz : fy ? ny : nx;
-                const float c = ea + (float)(2 * ka + 1) * ldexpf(wa, -(level / 3 + 1));
+                const float c = fmaf((float)(2 * ka + 1), ldexpf(wa, -(level / 3 + 1)), ea);
                 q = (neg ? p <= c : p < c) ? 0 : 1;
                 ki = (fy || fz) ? ki : 2 * ki + q;
                 kj = fy ? 2 * kj + q : kj;
@@ -754,7 +756,7 @@ struct TreeW32 {
             for (int q = 0; q < 3; q++) {
                 const int n = (int)((ns >> (5 * q)) & 31u);
                 const float w = ldexpf(t.widthf[q], -n);
-                const float lo = t.extf[q] + (float)kk[q] * w, up = t.extf[q] + (float)(kk[q] + 1) * w;
+                const float lo = fmaf((float)kk[q], w, t.extf[q]), up = fmaf((float)(kk[q] + 1), w, t.extf[q]);
                 const bool top = kk[q] + 1 == (1 << n);
                 ok = ok && p[q] >= lo && (p[q] < up || (top && p[q] <= up));
             }
@@ -774,9 +776,9 @@ struct TreeW32 {
     {
         const float wx = width(t, 0, nsub), wy = width(t, 1, nsub), wz = width(t, 2, nsub);
         const bool nx = kx < 0.0f, ny = ky < 0.0f, nz = kz < 0.0f;
-        xn = t.extf[0] + (float)(nx ? ci : ci + 1) * wx;
-        yn = t.extf[1] + (float)(ny ? cj : cj + 1) * wy;
-        zn = t.extf[2] + (float)(nz ? ck : ck + 1) * wz;
+        xn = fmaf((float)(nx ? ci : ci + 1), wx, t.extf[0]);
+        yn = fmaf((float)(ny ? cj : cj + 1), wy, t.extf[1]);
+        zn = fmaf((float)(nz ? ck : ck + 1), wz, t.extf[2]);
         const float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
         const float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
         const float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
