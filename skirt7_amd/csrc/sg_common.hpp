@@ -40,7 +40,9 @@ struct CartView {
 //          (bit 15) and the split axis of an internal binary-tree node (bits 16-17);
 //   val  : a leaf's density (float bits) or an internal node's first child;
 //   rope : for a leaf, the node across each wall that covers the whole wall (same level or coarser;
-//          -1 on the grid boundary): the next cell is found by descending from it.
+//          -1 on the grid boundary): the next cell is found by descending from it. For an internal
+//          node the first words hold its split planes as floats (x, y, z for an octree node, the
+//          plane of its split axis for a binary node), so a descent compares without computing them.
 // Node bounds are not stored: along axis b a node with integer index k_b spans
 // ext_b + [k_b, k_b + 1] * width_b / 2^n_b, and the walker carries the current leaf's k_b.
 struct alignas(32) TreeNode32 {

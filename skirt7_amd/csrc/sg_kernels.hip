@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -300,6 +301,27 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
             for (int q = 0; q < arity; q++) nsub[c0 + q] = nsub[l] + add;
         }
     }
+    // integer indices of every node along x, y, z at its own subdivision (children in the order of
+    // OctTreeNode.cpp:38-49: bit a of the child number is the upper half along axis a)
+    std::vector<int32_t> kidx(3 * (size_t)nn, 0);
+    for (int64_t l = 0; l < nn; l++) {
+        const int32_t c0 = h.child0[l];
+        if (c0 < 0) continue;
+        for (int q = 0; q < arity; q++)
+            for (int a = 0; a < 3; a++) {
+                const int32_t k = kidx[3 * (size_t)l + a];
+                const bool split = arity == 8 || h.dir[l] == a;
+                const int bit = arity == 8 ? (q >> a) & 1 : q;
+                kidx[3 * (size_t)(c0 + q) + a] = split ? 2 * k + bit : k;
+            }
+    }
+    std::array<float, 3> extf3, widthf3;
+    for (int a = 0; a < 3; a++) { extf3[a] = (float)ext[a]; widthf3[a] = (float)(ext[a + 3] - ext[a]); }
+    // split plane of an internal node along axis a, exactly as the walker would compute it
+    auto split_plane = [&](int64_t l, int a) {
+        const int n = (int)((nsub[l] >> (5 * a)) & 31u) + 1;
+        return std::fmaf((float)(2 * kidx[3 * (size_t)l + a] + 1), std::ldexp(widthf3[a], -n), extf3[a]);
+    };
     for (int64_t l = 0; l < nn; l++) {
         uint32_t nx = nsub[l] & 31u, ny = (nsub[l] >> 5) & 31u, nz = (nsub[l] >> 10) & 31u;
         SG_REQUIRE(nx < 31 && ny < 31 && nz < 31, "tree too deep for the mixed-precision records (30 subdivisions)");
@@ -308,8 +330,20 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
         f.code = nsub[l] | (leaf ? 1u << 15 : 0u) | ((uint32_t)(leaf ? 0 : h.dir[l]) << 16);
         f.val = leaf ? 0u : (uint32_t)h.child0[l];
         cellof[l] = h.cellnumber[l];
+        if (!leaf) {
+            // an internal node holds its split planes where a leaf holds its ropes: x, y, z for an
+            // octree node, the one plane of its split axis for a binary node
+            float pl[3] = {0.0f, 0.0f, 0.0f};
+            if (arity == 8) for (int a = 0; a < 3; a++) pl[a] = split_plane(l, a);
+            else pl[0] = split_plane(l, h.dir[l]);
+            for (int w = 0; w < 6; w++) {
+                float v = w < 3 ? pl[w] : 0.0f;
+                std::memcpy(&f.rope[w], &v, sizeof v);
+            }
+            continue;
+        }
         for (int w = 0; w < 6; w++) {
-            f.rope[w] = leaf ? ropes[6 * (size_t)l + w] : -1;
+            f.rope[w] = ropes[6 * (size_t)l + w];
             if (leaf && lists) {
                 // consistency with the neighbour lists: the first (largest-overlap) neighbour, raised to
                 // this node's level when finer, is the same rope

@@ -597,13 +597,10 @@ struct TreeW32 {
         const bool nx = kx < 0.0f, ny = ky < 0.0f, nz = kz < 0.0f;
         int level = 0;
         while (!(a.x & (1u << 15))) {
-            const uint32_t ns = a.x & 0x7fffu;
             const int c0 = (int)a.y;
             if (t.arity == 8) {
-                // split points ext + (2k + 1) * width / 2^(n + 1)
-                const float cx = fmaf((float)(2 * i + 1), width(t, 0, ns + 1u), t.extf[0]);
-                const float cy = fmaf((float)(2 * j + 1), width(t, 1, ns + (1u << 5)), t.extf[1]);
-                const float cz = fmaf((float)(2 * k + 1), width(t, 2, ns + (1u << 10)), t.extf[2]);
+                // split planes ext + (2k + 1) * width / 2^(n + 1), held by the internal node's record
+                const float cx = __uint_as_float(a.z), cy = __uint_as_float(a.w), cz = __uint_as_float(b.x);
                 const int qx = (nx ? x <= cx : x < cx) ? 0 : 1;
                 const int qy = (ny ? y <= cy : y < cy) ? 0 : 1;
                 const int qz = (nz ? z <= cz : z < cz) ? 0 : 1;
@@ -612,13 +609,9 @@ struct TreeW32 {
             } else {
                 // binary node: split at the midpoint of its recorded axis (bits 16-17: 0 x, 1 y, 2 z)
                 const bool fy = (a.x >> 16) & 1u, fz = (a.x >> 17) & 1u;
-                const float e = pick3(fy, fz, t.extf[0], t.extf[1], t.extf[2]);
-                const float w0 = pick3(fy, fz, t.widthf[0], t.widthf[1], t.widthf[2]);
-                const int32_t kd = fz ? k : fy ? j : i;
-                const int nd = (int)((ns >> (fz ? 10 : fy ? 5 : 0)) & 31u) + 1;
                 const float p = pick3(fy, fz, x, y, z);
                 const bool neg = fz ? nz : fy ? ny : nx;
-                const float c = fmaf((float)(2 * kd + 1), ldexpf(w0, -nd), e);
+                const float c = __uint_as_float(a.z);   // the split plane, held by the record
                 const int q = (neg ? p <= c : p < c) ? 0 : 1;
                 i = (fy || fz) ? i : 2 * i + q;
                 j = fy ? 2 * j + q : j;
