@@ -6,9 +6,10 @@
 //               one 64-byte FP64 record per cell (bounds, rho) with the full neighbour lists per
 //               (cell, wall) in CSR form; one 32-byte mixed-precision record per node (subdivision
 //               counts, density or first child, six ropes).
-//   Voronoi   : one 32-byte record per site (x, y, z, rho); neighbour lists in CSR form with
-//               walls -1..-6, each entry a 32-byte record holding the neighbour's site; a uniform
-//               bucket grid over the sites with one start site per bucket for cellIndex.
+//   Voronoi   : one 32-byte record per site (x, y, z, rho); per cell a 32-byte header (its site and
+//               density) followed by its neighbour entries (walls -1..-6), each a 32-byte record
+//               holding the neighbour's site, the cells' blocks in Morton order of their sites; a
+//               uniform bucket grid over the sites with one start site per bucket for cellIndex.
 //   Tallies   : FP64, instrument frames [ell][pixel], absorption [ell][m].
 #pragma once
 #include <hip/hip_runtime.h>
@@ -80,8 +81,8 @@ struct alignas(32) VoroNbr { double x, y, z; int32_t id, pad; };
 
 struct VoroView {
     const VoroSite* site;
-    const int32_t* off;              // CSR offsets, nsites + 1
-    const VoroNbr* nbr;              // neighbour entries, walls -1..-6
+    const int2* off;                 // per cell: its header record in nbr, and its entry count
+    const VoroNbr* nbr;              // per cell a header (site, density) then its entries, walls -1..-6
     const int32_t* boff;             // bucket offsets, nbk^3 + 1
     const int32_t* bids;             // site ids per bucket
     const int32_t* brep;             // start site per bucket for the greedy location, nbk^3

@@ -97,6 +97,7 @@ struct sg_grid {
     // Voronoi
     sg::DevBuf site, voff, vnb, boff, bids, brep;
     std::vector<sg::VoroSite> hsite;
+    std::vector<sg::VoroNbr> hvnb;     // device list records (headers and neighbour entries)
     sg::VoroView voro{};
     void set_rho(const double* r);
 };

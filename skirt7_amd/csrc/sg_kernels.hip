@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -271,8 +272,30 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     const bool lists = !h.nb.empty();
     int64_t nn = (int64_t)h.child0.size(), nc = (int64_t)h.idv.size();
     g->ncells = nc;
+    // Device node numbering: the breadth-first prefix of whole sibling groups that the LDS stage holds
+    // keeps its ids; every deeper sibling group follows in depth-first order of its parent, so that
+    // leaves that are neighbours in space, which a walk meets one after another, lie near each other
+    // in memory (a node record fetched at random brings a whole cache line in). Sibling groups stay
+    // contiguous, so a node's children are still its first child + q. Host-side ids stay breadth-first.
+    std::vector<int32_t> nid(nn, -1);
+    const int64_t top = nn <= SG_LDS_NODES ? nn : 1 + arity * ((SG_LDS_NODES - 1) / arity);
+    for (int64_t l = 0; l < top; l++) nid[l] = (int32_t)l;
+    {
+        int64_t next = top;
+        std::vector<int32_t> stack{0};
+        while (!stack.empty()) {
+            const int32_t l = stack.back();
+            stack.pop_back();
+            const int32_t c0 = h.child0[l];
+            if (c0 < 0) continue;
+            if (nid[c0] < 0)
+                for (int q = 0; q < arity; q++) nid[c0 + q] = (int32_t)next++;
+            for (int q = arity - 1; q >= 0; q--) stack.push_back(c0 + q);
+        }
+        SG_REQUIRE(next == nn, "tree renumbering missed nodes");
+    }
     std::vector<int32_t> words(nn);
-    for (int64_t l = 0; l < nn; l++) words[l] = h.child0[l] >= 0 ? h.child0[l] : -(h.cellnumber[l] + 1);
+    for (int64_t l = 0; l < nn; l++) words[nid[l]] = h.child0[l] >= 0 ? nid[h.child0[l]] : -(h.cellnumber[l] + 1);
     g->node.upload(words);
     g->hrec64.assign(nc, TreeRec64{});
     g->hnode32.assign(nn, TreeNode32{});
@@ -281,7 +304,7 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
         int l = h.idv[c];
         TreeRec64& r = g->hrec64[c];
         for (int a = 0; a < 3; a++) { r.lo[a] = h.bounds[6 * l + a]; r.hi[a] = h.bounds[6 * l + 3 + a]; }
-        r.rho = 0; r.node = l;
+        r.rho = 0; r.node = nid[l];
         for (int w = 0; w < 6; w++) {
             off[c * 6 + w] = (int32_t)ids.size();
             if (!lists) continue;
@@ -325,11 +348,11 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     for (int64_t l = 0; l < nn; l++) {
         uint32_t nx = nsub[l] & 31u, ny = (nsub[l] >> 5) & 31u, nz = (nsub[l] >> 10) & 31u;
         SG_REQUIRE(nx < 31 && ny < 31 && nz < 31, "tree too deep for the mixed-precision records (30 subdivisions)");
-        TreeNode32& f = g->hnode32[l];
+        TreeNode32& f = g->hnode32[nid[l]];
         const bool leaf = h.child0[l] < 0;
         f.code = nsub[l] | (leaf ? 1u << 15 : 0u) | ((uint32_t)(leaf ? 0 : h.dir[l]) << 16);
-        f.val = leaf ? 0u : (uint32_t)h.child0[l];
-        cellof[l] = h.cellnumber[l];
+        f.val = leaf ? 0u : (uint32_t)nid[h.child0[l]];
+        cellof[nid[l]] = h.cellnumber[l];
         if (!leaf) {
             // an internal node holds its split planes where a leaf holds its ropes: x, y, z for an
             // octree node, the one plane of its split axis for a binary node
@@ -343,13 +366,14 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
             continue;
         }
         for (int w = 0; w < 6; w++) {
-            f.rope[w] = ropes[6 * (size_t)l + w];
+            const int32_t rope = ropes[6 * (size_t)l + w];
+            f.rope[w] = rope >= 0 ? nid[rope] : -1;
             if (leaf && lists) {
                 // consistency with the neighbour lists: the first (largest-overlap) neighbour, raised to
                 // this node's level when finer, is the same rope
                 int32_t r = h.nb[6 * (size_t)l + w].empty() ? -1 : h.nb[6 * (size_t)l + w][0];
                 while (r >= 0 && h.level[r] > h.level[l]) r = h.father[r];
-                SG_REQUIRE(r == f.rope[w], "tree ropes disagree with the neighbour lists");
+                SG_REQUIRE(r == rope, "tree ropes disagree with the neighbour lists");
             }
         }
     }
@@ -392,20 +416,45 @@ int sg_grid_create_voronoi(const double ext[6], const double* sites, int64_t n, 
     g->hsite.resize(n);
     for (int64_t i = 0; i < n; i++) g->hsite[i] = VoroSite{sites[3 * i], sites[3 * i + 1], sites[3 * i + 2], 0.0};
     g->site.upload(g->hsite);
-    std::vector<int32_t> off(n + 1);
-    for (int64_t i = 0; i <= n; i++) off[i] = (int32_t)nb_offsets[i];
     SG_REQUIRE(nb_offsets[0] == 0, "neighbour offsets must start at 0");
     for (int64_t i = 0; i < n; i++)
         SG_REQUIRE(nb_offsets[i + 1] > nb_offsets[i], "every Voronoi cell needs at least one neighbour or wall");
-    g->voff.upload(off);
-    std::vector<VoroNbr> ent(std::max<int64_t>(1, nb_offsets[n]));
-    ent[0] = VoroNbr{0.0, 0.0, 0.0, -1, 0};
-    for (int64_t t = 0; t < nb_offsets[n]; t++) {
-        const int32_t v = nb_ids[t];
-        SG_REQUIRE(v >= -6 && v < n && v != -99, "neighbour id out of range");
-        ent[t] = v >= 0 ? VoroNbr{g->hsite[v].x, g->hsite[v].y, g->hsite[v].z, v, 0} : VoroNbr{0.0, 0.0, 0.0, v, 0};
+    for (int64_t t = 0; t < nb_offsets[n]; t++)
+        SG_REQUIRE(nb_ids[t] >= -6 && nb_ids[t] < n && nb_ids[t] != -99, "neighbour id out of range");
+    // Device lists: per cell a header record (its own site and density) followed by its neighbour
+    // entries; the cells' blocks in Morton order of their sites, so that cells a walk crosses one
+    // after another, which are neighbours in space, lie near each other in memory. voff holds each
+    // cell's (first record, entry count).
+    std::vector<uint64_t> key(n);
+    for (int64_t i = 0; i < n; i++) {
+        uint64_t k = 0;
+        for (int a = 0; a < 3; a++) {
+            const double u = (sites[3 * i + a] - ext[a]) / (ext[a + 3] - ext[a]);
+            const uint64_t c = (uint64_t)std::min(2097151.0, std::max(0.0, u * 2097152.0));
+            for (int b = 0; b < 21; b++) k |= ((c >> b) & 1ull) << (3 * b + a);
+        }
+        key[i] = k;
     }
-    g->vnb.upload(ent);
+    std::vector<int32_t> order(n);
+    for (int64_t i = 0; i < n; i++) order[i] = (int32_t)i;
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+    std::vector<int32_t> be(2 * n);
+    g->hvnb.clear();
+    g->hvnb.reserve(n + nb_offsets[n]);
+    for (int32_t m : order) {
+        be[2 * m] = (int32_t)g->hvnb.size();
+        be[2 * m + 1] = (int32_t)(nb_offsets[m + 1] - nb_offsets[m]);
+        VoroNbr head{g->hsite[m].x, g->hsite[m].y, g->hsite[m].z, 0, 0};   // density set by set_rho
+        g->hvnb.push_back(head);
+        for (int64_t t = nb_offsets[m]; t < nb_offsets[m + 1]; t++) {
+            const int32_t v = nb_ids[t];
+            g->hvnb.push_back(v >= 0 ? VoroNbr{g->hsite[v].x, g->hsite[v].y, g->hsite[v].z, v, 0}
+                                     : VoroNbr{0.0, 0.0, 0.0, v, 0});
+        }
+    }
+    SG_REQUIRE(g->hvnb.size() < ((size_t)1 << 31), "neighbour lists too long");
+    g->voff.upload(be);
+    g->vnb.upload(g->hvnb);
     // bucket grid for the nearest-site search: about 2 sites per bucket
     int nbk = std::max(1, std::min(256, (int)std::cbrt((double)n / 2.0)));
     std::vector<int32_t> cnt((size_t)nbk * nbk * nbk + 1, 0), bucket(n);
@@ -458,7 +507,7 @@ int sg_grid_create_voronoi(const double ext[6], const double* sites, int64_t n, 
     g->brep.upload(rep);
     VoroView& v = g->voro;
     v.site = g->site.as<VoroSite>();
-    v.off = g->voff.as<int32_t>();
+    v.off = g->voff.as<int2>();
     v.nbr = g->vnb.as<VoroNbr>();
     v.boff = g->boff.as<int32_t>();
     v.bids = g->bids.as<int32_t>();
@@ -501,6 +550,14 @@ void sg_grid::set_rho(const double* r)
     } else {
         for (int64_t c = 0; c < ncells; c++) hsite[c].rho = r[c];
         SG_HIP(hipMemcpy(site.p, hsite.data(), ncells * sizeof(VoroSite), hipMemcpyHostToDevice));
+        // the density in each cell's list header (the double in place of the entry's id and pad)
+        std::vector<int32_t> be(2 * ncells);
+        SG_HIP(hipMemcpy(be.data(), voff.p, be.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        static_assert(sizeof(VoroNbr) == sizeof(VoroSite) && offsetof(VoroSite, rho) == offsetof(VoroNbr, id),
+                      "a list header is a site record in a list entry's place");
+        for (int64_t c = 0; c < ncells; c++)
+            std::memcpy(reinterpret_cast<char*>(&hvnb[be[2 * c]]) + offsetof(VoroSite, rho), &r[c], sizeof(double));
+        SG_HIP(hipMemcpy(vnb.p, hvnb.data(), hvnb.size() * sizeof(VoroNbr), hipMemcpyHostToDevice));
     }
 }
 

@@ -976,7 +976,8 @@ __device__ int voro_locate(const VoroView& g, double x, double y, double z, int 
         int best = cur;
         double db = dc;
         thin = false;
-        const int t0 = g.off[cur], t1 = g.off[cur + 1];
+        const int2 be = g.off[cur];
+        const int t0 = be.x + 1, t1 = be.x + 1 + be.y;
         VoroNbr ahead = g.nbr[t0];
         for (int t = t0; t < t1; t++) {
             const VoroNbr p = ahead;
@@ -1070,11 +1071,13 @@ template <typename R> struct VoroW {
     {
         if (en.ne) { ds = (R)en.pop(); mo = -1; rh = (R)0; return true; }
         while (!done) {
-            VoroSite pr = g.site[mr];
+            // the cell's header record (its site and density), then its entries, contiguous
+            const int2 be = g.off[mr];
+            const VoroSite pr = *reinterpret_cast<const VoroSite*>(g.nbr + be.x);
             double sq = DBL_MAX;
             const int NO_INDEX = -99;
             int mq = NO_INDEX;
-            const int t0 = g.off[mr], t1 = g.off[mr + 1];
+            const int t0 = be.x + 1, t1 = be.x + 1 + be.y;
             // entries in blocks of four loaded together (every cell has at least one entry, checked at
             // creation; past the end the block repeats the last entry, whose second test changes
             // nothing: the nearest plane ahead is taken on a strict "<")
