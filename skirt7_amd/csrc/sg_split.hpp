@@ -460,14 +460,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WalkWa
         }
         uint64_t alive = __ballot(h.stage != ST_DEAD);
         if (!alive) break;
-        int need_n = max(1, min(thr, __popcll(alive) >> 1));
+        // wave-uniform, kept in a scalar register
+        const int need_n = __builtin_amdgcn_readfirstlane(max(1, min(thr, __popcll(alive) >> 1)));
+        bool walk = walking_stage(h.stage);   // carried around the loop: only crossing() changes the stage
         for (;;) {
             int kind = 0;
-            if (walking_stage(h.stage)) kind = crossing<W, ABS>(h, g, s_nodes, nlds, a);
+            if (walk) kind = crossing<W, ABS>(h, g, s_nodes, nlds, a);
             n_own += __popcll(__ballot(kind == 1));
             n_peel += __popcll(__ballot(kind == 2));
             n_re += __popcll(__ballot(kind == 3));
-            bool walk = walking_stage(h.stage);
+            walk = walking_stage(h.stage);
             uint64_t idle = __ballot(!walk && h.stage != ST_DEAD);
             if (__popcll(idle) >= need_n || !__ballot(walk)) break;
         }

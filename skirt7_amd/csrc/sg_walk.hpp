@@ -790,7 +790,7 @@ struct TreeW32 {
             by = !bx && ddy <= ddz;
             d = (float)(bx ? ddx : by ? ddy : ddz);
         }
-        const float dd = fmaxf(d, 0.0f);
+        const float dd = d > 0.0f ? d : 0.0f;   // a select: fmaxf would first canonicalize d
         x = bx ? xn : fmaf(kx, dd, x);
         y = by ? yn : fmaf(ky, dd, y);
         z = !bx && !by ? zn : fmaf(kz, dd, z);
