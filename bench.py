@@ -64,7 +64,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--tau-cut", type=float, default=0.0)
     ap.add_argument("--pipeline", type=int, default=1, choices=(0, 1),
-                    help="1: event and walk kernels (default); 0: single lane-pipeline kernel")
+                    help="1: event and walk kernels (default); 0: single lane-pipeline kernel (the autotuner, "
+                         "unless switched off or given --tuning, picks the faster for the mixed steps)")
     ap.add_argument("--no-equal-error", action="store_true",
                     help="skip the FP64 reference run with ten times the packets")
     ap.add_argument("--no-autotune", action="store_true",
@@ -130,10 +131,19 @@ def cpu_baseline(w, seconds):
                     break
     except OSError:
         pass
+    # the same restatement on one thread, over a sample a twentieth the size
+    mc1 = mc.copy()
+    mc1[0] = max(1000.0, mc[0] / 20)
+    t1 = time.perf_counter()
+    _, c1 = sc.run_reference(mc1, 1)
+    dt1 = time.perf_counter() - t1
     return {"value": float(c[0]) / dt, "unit": "packets/s", "cores": threads, "kind": "port",
             "sample": "%d packets of the same workload, MT19937 streams, %d threads on %s, %.1f s" % (
                 int(c[0]), threads, cpu, dt),
-            "crossings_per_s": float(c[1] + c[2]) / dt}
+            "crossings_per_s": float(c[1] + c[2]) / dt,
+            "single_thread": {"value": float(c1[0]) / dt1, "unit": "packets/s",
+                              "sample": "%d packets, 1 thread, %.1f s" % (int(c1[0]), dt1)},
+            "compiler": "g++ -O2 -ffp-contract=off (the reference's qmake release level)"}
 
 
 def population(a):
@@ -231,12 +241,13 @@ def main():
         b, sl, c = (int(float(x)) for x in a.tuning.split(","))
         S.set_tuning(b, 0, c)
         S.set_slots(sl)
-        tuning["mixed"] = {"block": b, "slots": sl, "compact_lanes": c, "fixed": True}
+        tuning["mixed"] = {"pipeline": a.pipeline, "block": b, "slots": sl, "compact_lanes": c, "fixed": True}
         a.no_autotune = True
     elif not a.no_autotune:
         # launch parameters picked on a sample of the workload's own packets, outside the timed region
         t0 = time.perf_counter()
-        tuning["mixed"] = S.autotune(sim, ell, 0, sample, SG_MIXED)
+        tuning["mixed"] = S.autotune(sim, ell, 0, sample, SG_MIXED, choose_pipeline=True)
+        a.pipeline = tuning["mixed"]["pipeline"]
         log("autotune (mixed, %d-packet trials, %.1f s): %s" % (sample, time.perf_counter() - t0, tuning["mixed"]))
 
     # warmup (mixed), then the timed mixed steps

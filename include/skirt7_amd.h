@@ -176,10 +176,13 @@ int sg_set_slots(int64_t slots);
  * in-flight slots (2^20..2^23 up to the sample size, event/walk pipeline) or waves per CU (lane
  * pipeline), then the block
  * size (64, 128, 256), keeping the fastest of each in turn, and leaves them set for later runs.
- * best (may be null) receives {block, slots or waves per CU, compaction threshold}; rate (may be
- * null) the sample's packets per second with them. The results of a run do not depend on these
- * parameters. The tallies the sample adds are removed again (frames and absorption as before). */
-int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precision, int64_t* best, double* rate);
+ * With choose_pipeline nonzero it first times both transport organisations of sg_set_pipeline with
+ * their default parameters and keeps the faster. best (may be null) receives {block, slots or waves
+ * per CU, compaction threshold, pipeline}; rate (may be null) the sample's packets per second with
+ * them. The results of a run do not depend on these choices. The tallies the sample adds are
+ * removed again (frames and absorption as before). */
+int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precision, int choose_pipeline,
+                int64_t* best, double* rate);
 /* Smallest member (packets) that sg_sim_run_batch runs alone through the event/walk pipeline
  * instead of in the shared persistent launch (0 = never; default 2^21). */
 int sg_set_batch_alone(uint64_t min_packets);

@@ -52,8 +52,8 @@ _SIGS = {
     "sg_set_batch_alone": (ctypes.c_int, [ctypes.c_uint64]),
     "sg_set_mixed_topdown": (ctypes.c_int, [ctypes.c_int]),
     "sg_set_slots": (ctypes.c_int, [ctypes.c_int64]),
-    "sg_autotune": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _I64,
-                                   _D]),
+    "sg_autotune": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                   _I64, _D]),
     "sg_fits_write": (ctypes.c_int, [ctypes.c_char_p, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p,
                                      ctypes.c_char_p]),

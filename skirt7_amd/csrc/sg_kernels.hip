@@ -1325,7 +1325,8 @@ int sg_set_slots(int64_t slots)
 // CU (lane pipeline), then the block size. Every candidate runs twice and the faster run counts, so
 // that work-buffer growth stays out of the comparison. The sample's tallies are taken back out: the
 // frames and the absorption table hold what they held before.
-int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precision, int64_t* best, double* rate)
+int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precision, int choose_pipeline,
+                int64_t* best, double* rate)
 {
     SG_TRY
     ensure_init();
@@ -1348,12 +1349,35 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
         }
         return r;
     };
+    const int pipeline0 = g_pipeline;
+    const int block0 = g_tune_block, compact0 = g_tune_compact, waves0 = g_tune_waves;
+    const int64_t slots0 = g_tune_slots;
+    double top = 0;
+    if (choose_pipeline) {
+        // the transport organisation first, each with its default launch parameters
+        try {
+            double r[2] = {0, 0};
+            for (int p = 0; p < 2; p++) {
+                g_pipeline = p;
+                g_tune_block = 256; g_tune_compact = 0; g_tune_waves = 0; g_tune_slots = 0;
+                r[p] = trial();
+            }
+            g_pipeline = r[1] >= r[0] ? 1 : 0;
+        } catch (...) {
+            g_pipeline = pipeline0; g_tune_block = block0; g_tune_compact = compact0; g_tune_waves = waves0;
+            g_tune_slots = slots0;
+            for (auto& k : keep) {
+                (void)hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice);
+                delete k.first;
+            }
+            throw;
+        }
+    }
     int block = g_tune_block;
     int compact = g_tune_compact > 0 ? g_tune_compact : (g_pipeline == 1 ? 24 : 32);
     int64_t slots = g_tune_slots > 0 ? g_tune_slots : kSplitSlots;
     int waves = g_tune_waves;
     g_tune_compact = compact;
-    double top = 0;
     try {
         top = trial();
         for (int c : {8, 16, 24, 32, 40, 48}) {
@@ -1399,7 +1423,7 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
         SG_HIP(hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice));
         delete k.first;
     }
-    if (best) { best[0] = block; best[1] = g_pipeline == 1 ? slots : waves; best[2] = compact; }
+    if (best) { best[0] = block; best[1] = g_pipeline == 1 ? slots : waves; best[2] = compact; best[3] = g_pipeline; }
     if (rate) *rate = top;
     SG_CATCH
 }
