@@ -158,7 +158,13 @@ def test_autotune_keeps_tallies_and_results(c0_small):
         assert best["block"] in (64, 128, 256)
         assert best["compact_lanes"] in (8, 16, 24, 32, 40, 48)
         assert best["slots"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+        assert best["pipeline"] == 1
         assert best["sample_packets_per_s"] > 0
+        for a, b in zip(before, frames(w)):
+            assert np.array_equal(a, b)
+        # choosing the transport organisation too
+        best = S.autotune(w["sim"], ell=0, first=0, sample=4000, precision=SG_FP64, choose_pipeline=True)
+        assert best["pipeline"] in (0, 1) and S.get_pipeline() == best["pipeline"]
         for a, b in zip(before, frames(w)):
             assert np.array_equal(a, b)
         w["instr"].clear()
@@ -168,3 +174,4 @@ def test_autotune_keeps_tallies_and_results(c0_small):
     finally:
         S.set_tuning(-1, -1, -1)
         S.set_slots(0)
+        S.set_pipeline(1)
