@@ -53,14 +53,7 @@ def check_z(z, what):
     assert np.mean(np.abs(z) > 4.0) < 0.02, (what, np.mean(np.abs(z) > 4.0))
 
 
-@pytest.fixture(scope="module")
-def c0_stat():
-    return workloads.config0(grid_n=16, pixels=48, packages=N)
-
-
-@pytest.fixture(scope="module")
-def c0_reference(c0_stat):
-    w = c0_stat
+def reference_batches(w):
     sc = O.scene_from_workload(w)
     batches = []
     for k in range(K):
@@ -73,9 +66,24 @@ def c0_reference(c0_stat):
     return batches
 
 
+@pytest.fixture(scope="module")
+def c0_stat():
+    w = workloads.config0(grid_n=16, pixels=48, packages=N)
+    return w, reference_batches(w)
+
+
+@pytest.fixture(scope="module")
+def c1_stat():
+    """A small config1: octree to level 5 over the spiral galaxy (the oracle walks it with the Neighbor
+    method, the GPU's mixed walks follow the ropes)."""
+    w = workloads.config1(maxlevel=5, pixels=48, packages=N, nsamples=4)
+    return w, reference_batches(w)
+
+
 @pytest.mark.parametrize("precision", [SG_FP64, SG_MIXED])
-def test_images_agree_with_reference_streams(c0_stat, c0_reference, precision):
-    w = c0_stat
+@pytest.mark.parametrize("which", ["cartesian", "octree"])
+def test_images_agree_with_reference_streams(c0_stat, c1_stat, which, precision):
+    w, ref_batches = c0_stat if which == "cartesian" else c1_stat
     gpu = []
     for k in range(K):
         w["instr"].clear()
@@ -84,7 +92,7 @@ def test_images_agree_with_reference_streams(c0_stat, c0_reference, precision):
     zs = []
     for i in range(len(w["instr"].instruments)):
         g = np.array([rebin(b[i], 3).ravel() for b in gpu])
-        r = np.array([rebin(b[i], 3).ravel() for b in c0_reference])
+        r = np.array([rebin(b[i], 3).ravel() for b in ref_batches])
         zt = zscores(g.sum(1, keepdims=True), r.sum(1, keepdims=True))
         print("instrument %d: totals gpu %.6e ref %.6e, z %s" % (i, g.sum(1).mean(), r.sum(1).mean(), zt), flush=True)
         assert len(zt) == 1 and abs(zt[0]) < 3.0, ("total flux", i, zt)

@@ -86,12 +86,17 @@ def test_scheduling_independent(c0_small):
 EXACT = ("packets", "own_crossings", "peel_crossings", "rewalk_crossings", "scatterings", "peeloffs", "walks")
 
 
-@pytest.mark.parametrize("which", ["cartesian", "octree"])
+@pytest.mark.parametrize("which", ["cartesian", "octree", "voronoi"])
 def test_split_and_lane_pipelines_agree(c0_small, which):
     """The event/walk kernels and the single lane-pipeline kernel run the same packet histories: identical
     event and crossing counts, images equal up to the order of the tally additions."""
     from skirt7_amd import skirt as S
-    w = c0_small if which == "cartesian" else workloads.config1(maxlevel=5, pixels=32, packages=8000, nsamples=4)
+    if which == "cartesian":
+        w = c0_small
+    elif which == "octree":
+        w = workloads.config1(maxlevel=5, pixels=32, packages=8000, nsamples=4)
+    else:
+        w = workloads.config2(nsites=3000, nlambda=1, pixels=32, packages=8000)
     n = 8000
     try:
         for prec in (SG_FP64, SG_MIXED):
