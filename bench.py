@@ -3,8 +3,8 @@
 Workload (config1): stellar exponential disc + Sersic bulge (B/T 0.2), both discs with a two-arm
 spiral (pitch 20 deg, weight 0.3); dust disc face-on tau (default 2); octree over +-20 kpc refined to
 level 8 by dust mass fraction 1e-6; four 512x512 frames at 0/45/75/89 deg; HG g=0.6, albedo 0.6.
-A step is one pass of the transport loop over a batch of packets (default 1e7; the configuration's
-1e8 packets are ten steps) with all inputs resident in HBM. The timed region is the mixed-precision
+A step is one pass of the transport loop over a batch of packets (default 2e7: the configuration's
+1e8 packets are the five timed steps) with all inputs resident in HBM. The timed region is the mixed-precision
 path. The FP64 baseline is timed on one step of the same packets with both transport pipelines (the
 faster counts). Image errors: mixed vs FP64 on identical packet streams, and both against an FP64
 reference run with ten times the packets (the equal-image-error condition of the metric).
@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--packets", type=float, default=None,
-                    help="packets per step (default 1e7 for config1, 1e6 for config2 and config4)")
+                    help="packets per step (default 2e7 for config1, 1e6 for the others)")
     ap.add_argument("--tau", type=float, default=2.0)
     ap.add_argument("--maxlevel", type=int, default=8)
     ap.add_argument("--pixels", type=int, default=512)
@@ -210,7 +210,7 @@ def main():
         copy_calibration(a.copy_calibration)
     S.set_pipeline(a.pipeline)
     if a.packets is None:
-        a.packets = 1e7 if a.workload == "config1" else 1e6
+        a.packets = 2e7 if a.workload == "config1" else 1e6
     per = int(a.packets)
     total = per * (a.steps + a.warmup)
     t0 = time.perf_counter()

@@ -173,7 +173,7 @@ int sg_set_slots(int64_t slots);
 /* Launch-parameter autotuner (no reference counterpart; the GPU analogue of the chunk rule of
  * MonteCarloSimulation::setChunkParams, MonteCarloSimulation.cpp:68-114): times sg_sim_run on the
  * sample of packets [first, first+sample) over the compaction threshold (8..48 lanes), then the
- * in-flight slots (2^20..2^23 up to the sample size, event/walk pipeline) or waves per CU (lane
+ * in-flight slots (2^20..2^24 up to the sample size, event/walk pipeline) or waves per CU (lane
  * pipeline), then the block
  * size (64, 128, 256), keeping the fastest of each in turn, and leaves them set for later runs.
  * With choose_pipeline nonzero it first times both transport organisations of sg_set_pipeline with

@@ -1388,7 +1388,7 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
         }
         g_tune_compact = compact;
         if (g_pipeline == 1) {
-            for (int64_t p : {(int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22, (int64_t)1 << 23}) {
+            for (int64_t p : {(int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22, (int64_t)1 << 23, (int64_t)1 << 24}) {
                 if (p == slots || (uint64_t)p > sample) continue;   // beyond the sample: all slots alike
                 g_tune_slots = p;
                 const double r = trial();

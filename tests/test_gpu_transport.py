@@ -162,7 +162,7 @@ def test_autotune_keeps_tallies_and_results(c0_small):
         best = S.autotune(w["sim"], ell=0, first=0, sample=4000, precision=SG_FP64)
         assert best["block"] in (64, 128, 256)
         assert best["compact_lanes"] in (8, 16, 24, 32, 40, 48)
-        assert best["slots"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+        assert best["slots"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
         assert best["pipeline"] == 1
         assert best["sample_packets_per_s"] > 0
         for a, b in zip(before, frames(w)):
