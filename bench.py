@@ -1,6 +1,6 @@
 """Benchmark: photon packets per second on one MI355X for the octree configuration.
 
-Workload (config1): stellar exponential disc + Sersic bulge (B/T 0.2), both discs with a two-arm
+Workload (spiral_octree): stellar exponential disc + Sersic bulge (B/T 0.2), both discs with a two-arm
 spiral (pitch 20 deg, weight 0.3); dust disc face-on tau (default 2); octree over +-20 kpc refined to
 level 8 by dust mass fraction 1e-6; four 512x512 frames at 0/45/75/89 deg; HG g=0.6, albedo 0.6.
 A step is one pass of the transport loop over a batch of packets (default 2e7: the configuration's
@@ -35,7 +35,7 @@ BYTES_PER_ABSORPTION = 16   # FP64 read-modify-write of one (cell, wavelength) e
 BYTES_PER_PEELOFF = 16   # FP64 read-modify-write of one pixel
 # HBM bytes per cell crossing of the walk kernel, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by
 # the copy-kernel calibration) on this workload
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_walks_config1_octree_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_walks_spiral_octree_traffic.json")
 
 
 def log(msg):
@@ -45,17 +45,19 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="config1", choices=("config0", "config1", "config2", "config3", "config4"),
-                    help="config1 octree (the metric's workload), config0 plain Cartesian 50^3 (the CPU-runnable "
-                         "case), config2 Voronoi with the absorption table, config3 batched population + fit, "
-                         "config4 kd-tree")
-    ap.add_argument("--members", type=int, default=256, help="config3: population size")
+    ap.add_argument("--workload", default="spiral_octree",
+                    choices=("disc_cartesian", "spiral_octree", "spiral_voronoi", "fit_population", "lognormal_kdtree"),
+                    help="spiral_octree: octree over a spiral galaxy (the metric's workload); disc_cartesian: plain "
+                         "Cartesian 50^3 (the CPU-runnable case); spiral_voronoi: 1e6-site Voronoi grid with the "
+                         "absorption table; fit_population: batched population + fit; lognormal_kdtree: kd-tree over a "
+                         "log-normal field")
+    ap.add_argument("--members", type=int, default=256, help="fit_population: population size")
     ap.add_argument("--member-packets", type=float, nargs=2, default=(1e5, 1e7),
-                    help="config3: log-uniform range of packets per member")
+                    help="fit_population: log-uniform range of packets per member")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--packets", type=float, default=None,
-                    help="packets per step (default 2e7 for config1, 1e6 for the others)")
+                    help="packets per step (default 2e7 for spiral_octree, 1e6 for the others)")
     ap.add_argument("--tau", type=float, default=2.0)
     ap.add_argument("--maxlevel", type=int, default=8)
     ap.add_argument("--pixels", type=int, default=512)
@@ -147,12 +149,12 @@ def cpu_baseline(w, seconds):
 
 
 def population(a):
-    """config3: one batched launch over the population against member-by-member runs, then the fit."""
+    """fit_population: one batched launch over the population against member-by-member runs, then the fit."""
     from skirt7_amd import skirt as S
     from skirt7_amd import workloads
     from skirt7_amd.capi import SG_MIXED
     t0 = time.perf_counter()
-    pop = workloads.config3(members=a.members, packets=tuple(a.member_packets))
+    pop = workloads.fit_population(members=a.members, packets=tuple(a.member_packets))
     ms = pop["members"]
     counts = [m["packages"] for m in ms]
     total = int(sum(counts))
@@ -185,7 +187,7 @@ def population(a):
            "value": total / t_batch, "unit": "packets/s", "n_gpus": 1, "steps": 1, "warmup": 1,
            "ms_per_step": 1e3 * t_batch, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 (walk, weights) + f64 (wall fallback, tallies)", "data": "synthetic",
-           "config": {"workload": "config3 population of %d Cartesian galaxies, %d packets, one batched launch "
+           "config": {"workload": "fitting population of %d Cartesian galaxies, %d packets, one batched launch "
                                   "per round" % (len(ms), total), "members": len(ms), "packets": total},
            "member_by_member_packets_per_s": total / t_seq, "fit_seconds": t_fit,
            "crossings_per_s": cross / t_batch, "counters": cb,
@@ -203,32 +205,32 @@ def main():
     from skirt7_amd import workloads
     from skirt7_amd.capi import SG_FP64, SG_MIXED
 
-    if a.workload == "config3":
+    if a.workload == "fit_population":
         population(a)
         return
     if a.copy_calibration > 0:
         copy_calibration(a.copy_calibration)
     S.set_pipeline(a.pipeline)
     if a.packets is None:
-        a.packets = 2e7 if a.workload == "config1" else 1e6
+        a.packets = 2e7 if a.workload == "spiral_octree" else 1e6
     per = int(a.packets)
     total = per * (a.steps + a.warmup)
     t0 = time.perf_counter()
     ell = 0
-    if a.workload == "config0":
-        w = workloads.config0(pixels=min(a.pixels, 256), packages=total)
-        desc = "config0 Cartesian 50^3 exponential disc + Sersic bulge, face-on tau 1"
-    elif a.workload == "config1":
-        w = workloads.config1(tau=a.tau, maxlevel=a.maxlevel, pixels=a.pixels, packages=total, tau_cut=a.tau_cut)
-        desc = "config1 octree spiral galaxy, face-on tau %.3g" % a.tau
-    elif a.workload == "config2":
-        w = workloads.config2(pixels=min(a.pixels, 256), packages=total)
+    if a.workload == "disc_cartesian":
+        w = workloads.disc_cartesian(pixels=min(a.pixels, 256), packages=total)
+        desc = "Cartesian 50^3 grid, exponential disc + Sersic bulge, face-on tau 1"
+    elif a.workload == "spiral_octree":
+        w = workloads.spiral_octree(tau=a.tau, maxlevel=a.maxlevel, pixels=a.pixels, packages=total, tau_cut=a.tau_cut)
+        desc = "octree over a spiral galaxy, face-on tau %.3g" % a.tau
+    elif a.workload == "spiral_voronoi":
+        w = workloads.spiral_voronoi(pixels=min(a.pixels, 256), packages=total)
         ell = int(np.argmin(np.abs(w["wavelengths"] - 0.55)))
-        desc = "config2 Voronoi 1e6 sites, 50 wavelengths (run at %.3g micron), absorption table stored" % (
+        desc = "Voronoi grid of 1e6 sites, 50 wavelengths (run at %.3g micron), absorption table stored" % (
             w["wavelengths"][ell])
     else:
-        w = workloads.config4(pixels=min(a.pixels * 2, 1024), packages=total)
-        desc = "config4 kd-tree (alternating binary tree, TopDown) over a log-normal field"
+        w = workloads.lognormal_kdtree(pixels=min(a.pixels * 2, 1024), packages=total)
+        desc = "kd-tree (alternating binary tree, TopDown) over a log-normal field"
     ncell = w["grid"].numCells()
     gkind = "voronoi" if w["kind"] == "voronoi" else "cartesian" if w["kind"] == "cartesian" else "tree"
     log("setup: %s with %d cells built in %.1f s on %s" % (a.workload, ncell, time.perf_counter() - t0,
@@ -284,7 +286,7 @@ def main():
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if a.pipeline == tr["pipeline"] and a.workload == "config1":
+        if a.pipeline == tr["pipeline"] and a.workload == "spiral_octree":
             # measured HBM bytes per crossing times this run's crossings per launch
             traffic = (tr["fetch_bytes_per_crossing"] + tr["write_bytes_per_crossing"]) * cross / launches
     except (OSError, KeyError, ValueError):

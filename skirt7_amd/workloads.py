@@ -1,12 +1,12 @@
 """Synthetic simulation setups of the benchmark configurations (all lengths in kpc, angles in radians).
 
-config0: plain FP64 reference case -- stellar exponential disc (hR 4, hz 0.4) plus Sersic n=4 bulge
+disc_cartesian: plain FP64 reference case -- stellar exponential disc (hR 4, hz 0.4) plus Sersic n=4 bulge
          (reff 1, B/T 0.2), dust exponential disc (hR 6, hz 0.25, face-on tau_V 1), HG g=0.6 albedo 0.6,
          Cartesian 50^3 over +-20, three 256x256 frames over 40 kpc at inclinations 0/60/88 deg.
-config1: the same galaxy with a two-arm logarithmic spiral (pitch 20 deg, weight 0.3) on stars and dust,
+spiral_octree: the same galaxy with a two-arm logarithmic spiral (pitch 20 deg, weight 0.3) on stars and dust,
          face-on tau in {0.5, 2, 5}, octree to level 8 by dust mass fraction 1e-6, four 512x512 frames at
          0/45/75/89 deg.
-config3: a population of small Cartesian galaxies run in one batched launch, each fitted to a
+fit_population: a population of small Cartesian galaxies run in one batched launch, each fitted to a
          Poisson-noised target by PSF convolution and golden section (FitSKIRT's evaluation step).
 Sizes can be scaled down for tests; every array a run depends on is kept in the returned dict so that
 the CPU oracle can be given the same inputs.
@@ -53,27 +53,27 @@ def _finish(d, packages, seed, tau_cut=0.0):
     return d
 
 
-def config0(grid_n=50, pixels=256, packages=1e6, seed=4357, nsamples=100, tau=1.0):
+def disc_cartesian(grid_n=50, pixels=256, packages=1e6, seed=4357, nsamples=100, tau=1.0):
     comps, dgeom, mix, scale = _galaxy(False, tau)
     grid = S.CartesianDustGrid.uniform(EXTENT, grid_n, grid_n, grid_n)
     rho = S.sample_density(grid, dgeom, scale, nsamples, seed)
-    d = dict(name="config0", kind="cartesian", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
+    d = dict(name="disc_cartesian", kind="cartesian", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
              instr=_frames((0.0, 60.0, 88.0), pixels))
     return _finish(d, packages, seed)
 
 
-def config1(tau=2.0, maxlevel=8, pixels=512, packages=1e8, seed=4357, nsamples=100, search=SG_NEIGHBOR,
+def spiral_octree(tau=2.0, maxlevel=8, pixels=512, packages=1e8, seed=4357, nsamples=100, search=SG_NEIGHBOR,
             maxmassfraction=1e-6, tau_cut=0.0):
     comps, dgeom, mix, scale = _galaxy(True, tau)
     grid = S.OctTreeDustGrid.subdivide(EXTENT, dgeom, scale, total_mass=scale, minlevel=2, maxlevel=maxlevel,
                                        maxmassfraction=maxmassfraction, nsamples=nsamples, seed=seed, search=search)
     rho = S.sample_density(grid, dgeom, scale, nsamples, seed)
-    d = dict(name="config1", kind="octree", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
+    d = dict(name="spiral_octree", kind="octree", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
              instr=_frames((0.0, 45.0, 75.0, 89.0), pixels))
     return _finish(d, packages, seed, tau_cut)
 
 
-def config3(members=256, packets=(1.0e5, 1.0e7), grid_n=(32, 128), pixels=(256, 128), seed=4357, nsamples=8,
+def fit_population(members=256, packets=(1.0e5, 1.0e7), grid_n=(32, 128), pixels=(256, 128), seed=4357, nsamples=8,
             fwhm=3.0, kernel_dim=6, target_counts=1.0e7, store_target=True):
     """Batched population: `members` galaxies (Cartesian grids with N per axis drawn uniformly from grid_n,
     dust hR U(2,8), hz U(0.1,0.6), face-on tau U(0.1,5), inclination U(85,90) deg, B/T U(0,0.5), packets
@@ -97,13 +97,13 @@ def config3(members=256, packets=(1.0e5, 1.0e7), grid_n=(32, 128), pixels=(256, 
         rho = S.sample_density(grid, dgeom, scale, nsamples, mseed)
         instr = S.InstrumentSystem([S.FrameInstrument(DISTANCE, math.radians(incl), 0.0, 0.0, nx_pix, ny_pix, 40.0,
                                                       40.0 * ny_pix / nx_pix)])
-        d = dict(name="config3-target" if is_target else "config3-member%d" % i, kind="cartesian", grid=grid,
+        d = dict(name="fit_population-target" if is_target else "fit_population-member%d" % i, kind="cartesian", grid=grid,
                  rho=rho, mix=mix, comps=comps, dgeom=dgeom, instr=instr,
                  params=dict(hR=hR, hz=hz, tau=tau, inclination=incl, bt=bt, grid=(nx, ny, nz)))
         out.append(_finish(d, npk, mseed))
     target = out.pop()
     kernel = S.GaussianKernel(fwhm, kernel_dim)
-    return dict(name="config3", members=out, target=target, kernel=kernel, target_counts=target_counts)
+    return dict(name="fit_population", members=out, target=target, kernel=kernel, target_counts=target_counts)
 
 
 def target_image(frame, counts, seed=4357):
@@ -141,11 +141,11 @@ def sample_sites(geometry, n, extent=EXTENT, seed=4357):
     return out[:n]
 
 
-def config2(nsites=1_000_000, nlambda=50, pixels=256, packages=2e7, seed=4357, tau_v=1.0, store_abs=True):
-    """Voronoi: nsites generator sites drawn from the config1 dust density, each cell with the analytic
+def spiral_voronoi(nsites=1_000_000, nlambda=50, pixels=256, packages=2e7, seed=4357, tau_v=1.0, store_abs=True):
+    """Voronoi: nsites generator sites drawn from the spiral_octree dust density, each cell with the analytic
     density at its site; nlambda wavelengths log-spaced over 0.1-10 micron with kappa(lambda) =
     kappa_V (lambda/0.55)^-1.3 normalised to a face-on tau_V, albedo 0.7 -> 0.4 and g 0.7 -> 0.3 linear in
-    the wavelength index; the absorbed luminosity per cell and wavelength is stored. Stars as config1
+    the wavelength index; the absorbed luminosity per cell and wavelength is stored. Stars as spiral_octree
     with a flat spectrum; two frames (0 and 75 deg) of pixels^2."""
     comps, dgeom, _, _ = _galaxy(True, 1.0)
     lam = np.logspace(-1.0, 1.0, nlambda)
@@ -158,7 +158,7 @@ def config2(nsites=1_000_000, nlambda=50, pixels=256, packages=2e7, seed=4357, t
     rho = scale * S.geometry_density(dgeom, sites)
     comps = [S.OligoStellarComp(c.geometry, [c.luminosities[0]] * nlambda) for c in comps]
     ins = [S.FrameInstrument(DISTANCE, math.radians(a), 0.0, 0.0, pixels, pixels, 40.0, 40.0) for a in (0.0, 75.0)]
-    d = dict(name="config2", kind="voronoi", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
+    d = dict(name="spiral_voronoi", kind="voronoi", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
              instr=S.InstrumentSystem(ins, nlambda), store_abs=store_abs, wavelengths=lam)
     return _finish(d, packages, seed)
 
@@ -177,7 +177,7 @@ def lognormal_field(n, sigma=1.5, slope=-11.0 / 3.0, seed=4357):
     return rho / rho.mean()
 
 
-def config4(nvox=256, maxmassfraction=9e-8, maxlevel=40, pixels=1024, packages=1e9, seed=4357, tau=1.0,
+def lognormal_kdtree(nvox=256, maxmassfraction=9e-8, maxlevel=40, pixels=1024, packages=1e9, seed=4357, tau=1.0,
             search=SG_TOPDOWN, half=15.0):
     """kd-tree: a log-normal dust field (sigma_ln 1.5, P(k) ~ k^-11/3) over a 30-kpc box, given as an nvox^3
     voxel field in place of the particle set it would be sampled from; a BinTreeDustGrid with alternating
@@ -194,6 +194,6 @@ def config4(nvox=256, maxmassfraction=9e-8, maxlevel=40, pixels=1024, packages=1
     comps = [S.OligoStellarComp(S.ExpDiskGeometry(4.0, 0.4), [1.0e10])]
     ins = [S.FrameInstrument(DISTANCE, math.radians(i), math.radians(a), 0.0, pixels, pixels, 2.0 * half, 2.0 * half)
            for i in (30.0, 60.0) for a in (0.0, 90.0, 180.0, 270.0)]
-    d = dict(name="config4", kind="bintree", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=None,
+    d = dict(name="lognormal_kdtree", kind="bintree", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=None,
              instr=S.InstrumentSystem(ins), field=field)
     return _finish(d, packages, seed)

@@ -1,5 +1,5 @@
 """ctypes access to the CPU oracle (oracle/_build/liboracle.so) for tests, smoke() and bench.py's
-CPU baseline. The oracle is the checker; it is never part of the product path."""
+CPU baseline. The oracle is test infrastructure; it is never part of the product path."""
 import ctypes
 import os
 

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def c2_small():
-    return workloads.config2(nsites=3000, nlambda=3, pixels=32, packages=4000)
+    return workloads.spiral_voronoi(nsites=3000, nlambda=3, pixels=32, packages=4000)
 
 
 def test_absorption_matches_oracle_voronoi(c2_small):

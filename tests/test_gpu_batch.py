@@ -20,7 +20,7 @@ EXACT = ("packets", "own_crossings", "peel_crossings", "scatterings", "peeloffs"
 
 @pytest.fixture(scope="module")
 def pop():
-    return workloads.config3(members=6, packets=(2.0e3, 3.0e4), grid_n=(8, 24), pixels=(64, 32), nsamples=2)
+    return workloads.fit_population(members=6, packets=(2.0e3, 3.0e4), grid_n=(8, 24), pixels=(64, 32), nsamples=2)
 
 
 def _single_runs(ms, precision):
@@ -93,7 +93,7 @@ def test_batch_runs_large_members_alone(pop, precision):
 
 
 def test_batch_rejects_mixed_grid_kinds(pop):
-    w = workloads.config1(maxlevel=3, pixels=16, packages=100, nsamples=2)
+    w = workloads.spiral_octree(maxlevel=3, pixels=16, packages=100, nsamples=2)
     with pytest.raises(S.FatalError):
         S.run_batch([pop["members"][0]["sim"], w["sim"]], [10, 10])
     with pytest.raises(S.FatalError):
@@ -127,7 +127,7 @@ def test_fit_matches_oracle(pop):
 
 def test_fit_full_frame_size():
     """256 x 128 frames and the 6-pixel FWHM-3 PSF of the population workload."""
-    pop = workloads.config3(members=3, packets=(5.0e3, 1.0e4), grid_n=(16, 24), nsamples=2)
+    pop = workloads.fit_population(members=3, packets=(5.0e3, 1.0e4), grid_n=(16, 24), nsamples=2)
     ms = pop["members"]
     S.run_batch([m["sim"] for m in ms], [m["packages"] for m in ms], precision=SG_MIXED)
     frame0 = ms[0]["instr"].frame(0)[0]

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def c0_tiny():
-    return workloads.config0(grid_n=10, pixels=24, packages=3000)
+    return workloads.disc_cartesian(grid_n=10, pixels=24, packages=3000)
 
 
 def frames(w):

@@ -72,7 +72,7 @@ def test_voronoi_bitexact():
 
 
 def test_voronoi_clustered_sites():
-    """Sites concentrated in a thin disc (the config2 density) with the builder's true neighbour
+    """Sites concentrated in a thin disc (the spiral_voronoi density) with the builder's true neighbour
     lists: the greedy point location over the lists (walk starts, whichcell) must find the exact
     nearest site, and the walks must be the oracle's bit for bit."""
     from skirt7_amd import workloads

@@ -68,15 +68,15 @@ def reference_batches(w):
 
 @pytest.fixture(scope="module")
 def c0_stat():
-    w = workloads.config0(grid_n=16, pixels=48, packages=N)
+    w = workloads.disc_cartesian(grid_n=16, pixels=48, packages=N)
     return w, reference_batches(w)
 
 
 @pytest.fixture(scope="module")
 def c1_stat():
-    """A small config1: octree to level 5 over the spiral galaxy (the oracle walks it with the Neighbor
+    """A small spiral_octree: octree to level 5 over the spiral galaxy (the oracle walks it with the Neighbor
     method, the GPU's mixed walks follow the ropes)."""
-    w = workloads.config1(maxlevel=5, pixels=48, packages=N, nsamples=4)
+    w = workloads.spiral_octree(maxlevel=5, pixels=48, packages=N, nsamples=4)
     return w, reference_batches(w)
 
 
@@ -104,7 +104,7 @@ def test_images_agree_with_reference_streams(c0_stat, c1_stat, which, precision)
 
 def test_absorption_agrees_with_reference_streams():
     """The Voronoi absorption table (OligoDustSystem::absorb) per group of 16 cells, in FP64 and mixed."""
-    w = workloads.config2(nsites=3000, nlambda=1, pixels=16, packages=N)
+    w = workloads.spiral_voronoi(nsites=3000, nlambda=1, pixels=16, packages=N)
     sc = O.scene_from_workload(w)
     ncell = w["grid"].numCells()
     groups = np.arange(ncell) // 16

@@ -31,7 +31,7 @@ def oracle_frames(w, ref):
 
 @pytest.fixture(scope="module")
 def c0_small():
-    return workloads.config0(grid_n=16, pixels=48, packages=20000)
+    return workloads.disc_cartesian(grid_n=16, pixels=48, packages=20000)
 
 
 def test_fp64_matches_oracle_packet_by_packet(c0_small):
@@ -94,9 +94,9 @@ def test_split_and_lane_pipelines_agree(c0_small, which):
     if which == "cartesian":
         w = c0_small
     elif which == "octree":
-        w = workloads.config1(maxlevel=5, pixels=32, packages=8000, nsamples=4)
+        w = workloads.spiral_octree(maxlevel=5, pixels=32, packages=8000, nsamples=4)
     else:
-        w = workloads.config2(nsites=3000, nlambda=1, pixels=32, packages=8000)
+        w = workloads.spiral_voronoi(nsites=3000, nlambda=1, pixels=32, packages=8000)
     n = 8000
     try:
         for prec in (SG_FP64, SG_MIXED):
@@ -117,7 +117,7 @@ def test_split_and_lane_pipelines_agree(c0_small, which):
 
 @pytest.fixture(scope="module")
 def c4_small():
-    return workloads.config4(nvox=16, maxmassfraction=2e-4, pixels=32, packages=6000)
+    return workloads.lognormal_kdtree(nvox=16, maxmassfraction=2e-4, pixels=32, packages=6000)
 
 
 def test_kdtree_fp64_matches_oracle(c4_small):
