@@ -6,6 +6,8 @@ At this size the oracle still checks in seconds what it can: exact FP64 walks on
 checked through properties that do not depend on the size: every packet counted once and none stuck, a run
 split into two packet ranges equal to the run in one piece (the streams are keyed by the packet index), and
 mixed precision equal to FP64 on the same streams up to the rare packets whose history a rounding flips.
+The Voronoi workload's full grid (1e6 sites drawn from the spiral disc's density) is checked the same way
+for its walks and its point location.
 """
 import numpy as np
 import pytest
@@ -84,3 +86,18 @@ def test_mixed_equals_fp64_on_the_same_streams(full):
         assert abs(a.sum() - b.sum()) <= 1e-3 * b.sum(), (a.sum(), b.sum())
         rel = np.linalg.norm(a - b) / np.linalg.norm(b)
         assert rel < 0.03, rel
+
+
+def test_voronoi_full_size_walks_and_location():
+    w = workloads.spiral_voronoi(nlambda=1, pixels=32, packages=1e4)
+    g = w["grid"]
+    assert g.numCells() == 1000000
+    sc = O.Scene().voronoi(g.extent, g.sites, g.nb_offsets, g.nb_ids)
+    rng = np.random.default_rng(41)
+    lo, hi = g.extent[:3], g.extent[3:]
+    pts = np.concatenate([g.sites[rng.integers(0, len(g.sites), 500)] + rng.normal(0.0, 1e-3, (500, 3)),
+                          lo + rng.random((500, 3)) * (hi - lo)])
+    assert np.array_equal(g.whichcell(pts), np.array([sc.whichcell(p) for p in pts]))
+    rays = random_rays(300, 42, g.extent, special=False)
+    rays[:150, :3] = g.sites[rng.integers(0, len(g.sites), 150)]
+    assert_paths_equal(g, sc, rays, maxseg=16384)
