@@ -1074,10 +1074,12 @@ struct PP {
 };
 
 // StellarSystem.cpp:116-158
-void launch(const Scene& sc, PP& pp, int ell, double L, Rng& rng, double emissionBias, const std::vector<double>& Ltot)
+void launch(const Scene& sc, PP& pp, int ell, double L, Rng& rng, double emissionBias, const std::vector<double>& Ltot,
+            int* hout = nullptr)
 {
     int N = (int)sc.stars.size();
     int h = 0;
+    if (hout) *hout = 0;
     double Lw = L;
     if (N > 1) {
         double X = rng.uniform();
@@ -1089,6 +1091,7 @@ void launch(const Scene& sc, PP& pp, int ell, double L, Rng& rng, double emissio
             for (int i = 0; i <= N; i++) Xv[i] /= Xv[N];
             h = nr_locate_clip(Xv.data(), N + 1, (X - emissionBias) / (1.0 - emissionBias));
         }
+        if (hout) *hout = h;
         double Lh = sc.stars[h].lum[ell];
         if (!(Lh > 0)) { pp.L = 0; pp.ell = ell; return; }
         double Lmean = Ltot[ell] / N;
@@ -1576,6 +1579,71 @@ double or_hg_value(double g, const double k[3], const double kobs[3])
     double cosalpha = k[0] * kobs[0] + k[1] * kobs[1] + k[2] * kobs[2];
     double t = 1.0 + g * g - 2 * g * cosalpha;
     return (1.0 - g) * (1.0 + g) / std::sqrt(t * t * t);
+}
+
+// ------------------------------------------------------------------ sampling building blocks
+// (each packet i of [first, first+n) draws from its own Philox stream keyed (seed, 0) at wavelength 0,
+// as the transport does)
+
+// GeometricStellarComp::launch position (with the spiral decorator when the component has one) of
+// stellar component comp: x, y, z per packet.
+int or_star_positions(void* s, int comp, uint32_t seed, uint64_t first, uint64_t n, double* out)
+{
+    Scene* sc = (Scene*)s;
+    if (comp < 0 || comp >= (int)sc->stars.size()) { g_err = "no such stellar component"; return -1; }
+    Rng rng;
+    rng.use_mt = false;
+    rng.key[0] = seed;
+    for (uint64_t i = 0; i < n; i++) {
+        rng.start_packet(first + i);
+        star_position(sc->stars[comp], rng, out + 3 * i);
+    }
+    return 0;
+}
+
+// StellarSystem::launch with luminosity 1: the chosen component and the packet's luminosity weight.
+int or_launch_sample(void* s, int ell, double bias, uint32_t seed, uint64_t first, uint64_t n, int32_t* comp,
+                     double* weight)
+{
+    Scene* sc = (Scene*)s;
+    if (sc->stars.empty()) { g_err = "no stellar components"; return -1; }
+    std::vector<double> Ltot = total_lum(*sc);
+    Rng rng;
+    rng.use_mt = false;
+    rng.key[0] = seed;
+    for (uint64_t i = 0; i < n; i++) {
+        rng.start_packet(first + i);
+        PP pp;
+        int h = 0;
+        launch(*sc, pp, ell, 1.0, rng, bias, Ltot, &h);
+        comp[i] = h;
+        weight[i] = pp.L;
+    }
+    return 0;
+}
+
+// Random::exponcutoff(xmax) for n packets.
+void or_exponcutoff_sample(double xmax, uint32_t seed, uint64_t first, uint64_t n, double* out)
+{
+    Rng rng;
+    rng.use_mt = false;
+    rng.key[0] = seed;
+    for (uint64_t i = 0; i < n; i++) {
+        rng.start_packet(first + i);
+        out[i] = exponcutoff(rng, xmax);
+    }
+}
+
+// SingleFrameInstrument::pixelondetector of n positions for the instrument record p10; the
+// instrument's observer direction (DistantInstrument::bfkobs) in kobs.
+void or_pixelondetector(const double* p10, int64_t n, const double* r, int32_t* pix, double* kobs)
+{
+    Instrument in;
+    in.distance = p10[0]; in.incl = p10[1]; in.azim = p10[2]; in.pa = p10[3];
+    in.Nx = (int)p10[4]; in.Ny = (int)p10[5]; in.fovx = p10[6]; in.fovy = p10[7]; in.xc = p10[8]; in.yc = p10[9];
+    setup_instrument(in);
+    for (int64_t i = 0; i < n; i++) pix[i] = pixelondetector(in, r + 3 * i);
+    if (kobs) std::memcpy(kobs, in.kobs, sizeof in.kobs);
 }
 
 // ------------------------------------------------------------------ FitSKIRT evaluation

@@ -73,6 +73,12 @@ void   or_sersic_table(double n, double* s101, double* M101);
 void   or_hg_sample(double g, const double k[3], double u1, double u2, double out[3]);
 double or_hg_value(double g, const double k[3], const double kobs[3]);
 const char* or_last_error(void);
+// sampling building blocks on the per-packet Philox streams keyed (seed, 0), packets [first, first+n)
+int    or_star_positions(void* s, int comp, uint32_t seed, uint64_t first, uint64_t n, double* xyz);
+int    or_launch_sample(void* s, int ell, double bias, uint32_t seed, uint64_t first, uint64_t n, int32_t* comp,
+                        double* weight);
+void   or_exponcutoff_sample(double xmax, uint32_t seed, uint64_t first, uint64_t n, double* out);
+void   or_pixelondetector(const double* params10, int64_t n, const double* xyz, int32_t* pix, double* kobs);
 
 // FitSKIRT evaluation (one stellar component)
 //   GaussianKernel::setupSelfBefore (GaussianKernel.cpp:19-46) + ConvolutionKernel::setupSelfAfter

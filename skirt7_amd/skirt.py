@@ -375,7 +375,11 @@ def field_density(grid, field):
 
 
 def faceon_normalization(geometry, tau, kappaext):
-    """FaceOnDustCompNormalization::normalizationFactor (FaceOnDustCompNormalization.cpp:67-74)."""
+    """FaceOnDustCompNormalization::normalizationFactor (FaceOnDustCompNormalization.cpp:67-74): only an
+    axisymmetric geometry (an AxGeometry: here the exponential disc) has a face-on surface density; a Sersic
+    sphere or a spiral decorator (a GenGeometry) raises the reference's fatal error."""
+    if not isinstance(geometry, ExpDiskGeometry):
+        raise FatalError("Geometry is not axisymmetric")
     return tau / (geometry.SigmaZ() * kappaext)
 
 

@@ -34,7 +34,10 @@ def _galaxy(spiral=False, dust_tau=1.0, bt=0.2, ltot=1.0e10, dust_hR=6.0, dust_h
     if bt > 0:
         comps.append(S.OligoStellarComp(S.SersicGeometry(4.0, 1.0), [ltot * bt]))
     mix = S.SimpleOligoDustMix([1.0], [0.6], [0.6])
-    scale = S.faceon_normalization(dgeom, dust_tau, mix.kappaext[0])
+    # face-on optical depth of the dust disc; with the spiral, that of the underlying disc (the perturbation
+    # averages to one over azimuth). SKIRT7's FaceOnDustCompNormalization refuses the decorator itself, so a
+    # ski file of this galaxy states the same normalisation as a dust mass.
+    scale = S.faceon_normalization(dgeom.geometry if spiral else dgeom, dust_tau, mix.kappaext[0])
     return comps, dgeom, mix, scale
 
 
@@ -152,7 +155,7 @@ def spiral_voronoi(nsites=1_000_000, nlambda=50, pixels=256, packages=2e7, seed=
     kappa = (lam / 0.55) ** -1.3
     idx = np.arange(nlambda) / max(1, nlambda - 1)
     mix = S.SimpleOligoDustMix(kappa, 0.7 - 0.3 * idx, 0.7 - 0.4 * idx)
-    scale = S.faceon_normalization(dgeom, tau_v, 1.0)
+    scale = S.faceon_normalization(dgeom.geometry, tau_v, 1.0)
     sites = sample_sites(dgeom, nsites, EXTENT, seed)
     grid = S.VoronoiDustGrid(EXTENT, sites)
     rho = scale * S.geometry_density(dgeom, sites)

@@ -49,6 +49,11 @@ _SIGS = {
     "or_nr_locate_fail": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_double]),
     "or_nr_locate_tau": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_double]),
     "or_nr_interp_loglog": (ctypes.c_double, [ctypes.c_double] * 5),
+    "or_star_positions": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, _D]),
+    "or_launch_sample": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64,
+                                        ctypes.c_uint64, _I32, _D]),
+    "or_exponcutoff_sample": (None, [ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, _D]),
+    "or_pixelondetector": (None, [_D, ctypes.c_int64, _D, _I32, _D]),
 }
 
 _lib = None
@@ -136,6 +141,21 @@ class Scene:
         assert lib().or_scene_add_instrument(self.h, dp(p)) == 0
         self.ninstr.append((int(p[4]), int(p[5])))
         return self
+
+    def star_positions(self, comp, n, seed=4357, first=0):
+        """GeometricStellarComp::launch positions of component comp on packet streams [first, first+n)."""
+        out = np.zeros((n, 3))
+        rc = lib().or_star_positions(self.h, comp, seed, first, n, dp(out))
+        assert rc == 0, lib().or_last_error()
+        return out
+
+    def launch_sample(self, n, ell=0, bias=0.5, seed=4357, first=0):
+        """StellarSystem::launch with L = 1: (component index, luminosity weight) per packet."""
+        comp = np.zeros(n, np.int32)
+        w = np.zeros(n)
+        rc = lib().or_launch_sample(self.h, ell, bias, seed, first, n, comp.ctypes.data_as(_I32), dp(w))
+        assert rc == 0, lib().or_last_error()
+        return comp, w
 
     def path(self, r, k, maxseg=100000):
         rk = f64(np.concatenate([r, k]))
@@ -231,3 +251,19 @@ def scene_from_workload(w):
     for ins in w["instr"].instruments:
         sc.instrument(ins.p)
     return sc
+
+
+def exponcutoff_sample(xmax, n, seed=4357, first=0):
+    """Random::exponcutoff(xmax) on packet streams [first, first+n)."""
+    out = np.zeros(n)
+    lib().or_exponcutoff_sample(xmax, seed, first, n, dp(out))
+    return out
+
+
+def pixelondetector(params10, xyz):
+    """SingleFrameInstrument::pixelondetector of positions xyz[n, 3]: (pixel indices, bfkobs)."""
+    r = f64(xyz).reshape(-1, 3)
+    pix = np.zeros(r.shape[0], np.int32)
+    kobs = np.zeros(3)
+    lib().or_pixelondetector(dp(f64(params10)), r.shape[0], dp(r), pix.ctypes.data_as(_I32), dp(kobs))
+    return pix, kobs

@@ -246,3 +246,73 @@ def test_ski_rejects_what_it_cannot_run(resources):
     t = ski_text().replace('luminosities="8e9"', 'luminosities="8e9, 1e9"')
     with pytest.raises(O.FatalError, match="luminosities"):
         ski.Spec(t, resources)
+
+
+REF_SUN = "/root/reference/dat/SED/Sun/SunSED.dat"
+
+
+def test_reference_sun_sed(tmp_path):
+    """The reference's own solar SED resource (dat/SED/Sun/SunSED.dat): read as OligoStellarComp does
+    (micron -> m, W/micron -> W/m), its integral is the solar luminosity of Units.cpp (3.839e26 W), the
+    lookup equals the reference's NR::locate_fail + NR::interpolate_linlin (compiled from the SKIRT7 tree)
+    on and between the grid points, and a ski file run against that resource directory gets
+    L = luminosity x L_sun,lambda x dlambda per bin."""
+    import ctypes
+    from skirt7_amd import ski
+    if not os.path.exists(REF_SUN) or not os.path.exists(O_REF_SHIM()):
+        pytest.skip("the SKIRT7 tree (and its compiled numerics) is not present")
+    lv, Lv = ski.read_sun_sed(REF_SUN)
+    assert len(lv) == 1099 and np.all(np.diff(lv) > 0)
+    assert abs(np.trapezoid(Lv, lv) / 3.839e26 - 1.0) < 0.01
+    R = ctypes.CDLL(O_REF_SHIM())
+    D = ctypes.POINTER(ctypes.c_double)
+    R.ref_locate_fail.restype = ctypes.c_int
+    R.ref_locate_fail.argtypes = [D, ctypes.c_int, ctypes.c_double]
+    R.ref_interpolate_linlin.restype = ctypes.c_double
+    R.ref_interpolate_linlin.argtypes = [ctypes.c_double] * 5
+    rng = np.random.default_rng(4)
+    lams = np.concatenate([lv[1:-1:7], rng.uniform(lv[0], lv[-1], 400), [lv[0], lv[-1]]])
+    for lam in lams:
+        k = R.ref_locate_fail(lv.ctypes.data_as(D), len(lv), lam)
+        assert k >= 0
+        expect = R.ref_interpolate_linlin(lam, lv[k], lv[k + 1], Lv[k], Lv[k + 1])
+        assert ski.sun_luminosity(lam, (lv, Lv)) == expect
+    assert R.ref_locate_fail(lv.ctypes.data_as(D), len(lv), lv[0] * 0.99) < 0
+    with pytest.raises(O.FatalError, match="does not emit"):
+        ski.sun_luminosity(lv[0] * 0.99, (lv, Lv))
+    res = tmp_path / "dat"
+    res.mkdir()
+    os.symlink(os.path.dirname(os.path.dirname(REF_SUN)), str(res / "SED"))
+    sp = ski.Spec(ski_text(), str(res))
+    geom0, L0 = sp.stellar[0]
+    lam = 0.55e-6
+    k = R.ref_locate_fail(lv.ctypes.data_as(D), len(lv), lam)
+    Lsun = R.ref_interpolate_linlin(lam, lv[k], lv[k + 1], Lv[k], Lv[k + 1])
+    assert abs(L0[0] / (8e9 * Lsun * 0.55e-9) - 1.0) < 1e-14
+    # the Sun emits about 5e26 W per micron at 0.55 micron (4 pi AU^2 x 1.87 W/m2/nm)
+    assert 4.5e32 < Lsun < 6.0e32
+
+
+def O_REF_SHIM():
+    from tests import oracle_lib
+    return oracle_lib.REFSHIM
+
+
+def test_faceon_normalization_needs_an_axisymmetric_geometry(resources):
+    """FaceOnDustCompNormalization.cpp:67-74: a geometry that is not an AxGeometry (a spiral decorator, a
+    GenGeometry, or a Sersic sphere) raises "Geometry is not axisymmetric"."""
+    from skirt7_amd import ski
+    disc = '<ExpDiskGeometry radialScale="6 kpc" axialScale="250 pc"/>'
+    spiral = ('<SpiralStructureGeometryDecorator arms="2" pitch="20 deg" radius="4 kpc" perturbWeight="0.3">'
+              '<geometry type="Geometry">%s</geometry></SpiralStructureGeometryDecorator>' % disc)
+    t = ski_text().replace(disc, spiral)
+    assert spiral in t
+    with pytest.raises(O.FatalError, match="Geometry is not axisymmetric"):
+        ski.Spec(t, resources)
+    t = ski_text().replace(disc, '<SersicGeometry index="2" radius="3 kpc"/>')
+    with pytest.raises(O.FatalError, match="Geometry is not axisymmetric"):
+        ski.Spec(t, resources)
+    # the same spiral dust normalised by its dust mass is accepted
+    t = ski_text().replace(disc, spiral).replace('<FaceOnDustCompNormalization wavelength="0.55 micron" opticalDepth="1"/>',
+                                                 '<DustMassDustCompNormalization dustMass="1e7 Msun"/>')
+    assert ski.Spec(t, resources).dust["scale"] > 0
