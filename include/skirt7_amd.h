@@ -191,6 +191,12 @@ int sg_set_batch_alone(uint64_t min_packets);
  * cell is found, TreeDustGrid.cpp:371-643); 1 descend from the root at every step as TopDown does.
  * FP64 walks always use the tree's own method. */
 int sg_set_mixed_topdown(int descend);
+/* Order in which the walk kernel of the event/walk pipeline takes a pass's walks: 1 sorted by a
+ * coherence key (peel-offs by instrument and pixel, other walks by start point), 0 (default) as
+ * appended. The
+ * order changes no packet history, only the order of the tally additions. No counterpart in the
+ * reference, whose threads walk one packet at a time (MonteCarloSimulation.cpp:259-297). */
+int sg_set_walk_order(int sorted);
 
 /* ---- setup helpers (the reference's own setup phase, made reproducible) ---------------------- */
 /* Normalised density of an exponential-disc geometry record laid out like a stellar component

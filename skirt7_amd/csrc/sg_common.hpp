@@ -5,7 +5,7 @@
 //   Tree      : node words in breadth-first id order (>= 0 first child id, < 0 leaf: -(m+1));
 //               one 64-byte FP64 record per cell (bounds, rho) with the full neighbour lists per
 //               (cell, wall) in CSR form; one 32-byte mixed-precision record per node (subdivision
-//               counts, density or first child, six ropes).
+//               counts, density or first child, six ropes or split planes).
 //   Voronoi   : one 32-byte record per site (x, y, z, rho); per cell a 32-byte header (its site and
 //               density) followed by its neighbour entries (walls -1..-6), each a 32-byte record
 //               holding the neighbour's site, the cells' blocks in Morton order of their sites; a
@@ -36,16 +36,21 @@ struct CartView {
     float wminf;                     // smallest cell width over the three axes
 };
 
-// Mixed-precision walk record, one per tree NODE (breadth-first id), 32 bytes:
+// Mixed-precision walk record, one per tree NODE (device id), 32 bytes, read as two 16-byte loads:
 //   code : subdivisions of the root along x, y, z (5 bits each: nx | ny << 5 | nz << 10), leaf flag
 //          (bit 15) and the split axis of an internal binary-tree node (bits 16-17);
 //   val  : a leaf's density (float bits) or an internal node's first child;
-//   rope : for a leaf, the node across each wall that covers the whole wall (same level or coarser;
-//          -1 on the grid boundary): the next cell is found by descending from it. For an internal
-//          node the first words hold its split planes as floats (x, y, z for an octree node, the
-//          plane of its split axis for a binary node), so a descent compares without computing them.
+//   rope : for a leaf, the node across each wall that covers the whole wall (same level or coarser
+//          along every axis; -1 on the grid boundary): the next cell is found by descending from it.
+//          For an internal node the first words hold its split planes as floats (x, y, z for an
+//          octree node, the plane of its split axis for a binary node), so a descent compares without
+//          computing them.
 // Node bounds are not stored: along axis b a node with integer index k_b spans
-// ext_b + [k_b, k_b + 1] * width_b / 2^n_b, and the walker carries the current leaf's k_b.
+// ext_b + [k_b, k_b + 1] * width_b / 2^n_b, each wall one fused multiply-add (a single rounding of
+// the exact value, so that neighbouring leaves share their wall floats exactly and a parent's split
+// plane equals its children's common wall); the walker carries the current leaf's k_b. A node's
+// indices are also kept packed (21 bits per axis) for the walks that start in a given leaf.
+constexpr uint32_t kTreeLeaf = 1u << 15;
 struct alignas(32) TreeNode32 {
     uint32_t code;
     uint32_t val;
@@ -63,6 +68,7 @@ struct TreeView {
     const TreeRec64* rec64;
     const TreeNode32* node32;        // mixed-precision walk records
     const int32_t* cellof;           // cell number per node (-1 for internal nodes)
+    const uint64_t* kpack;           // integer indices per node, i | j << 21 | k << 42
     const int32_t* nboff;            // CSR offsets, index cell*6 + wall
     const int32_t* nbids;            // neighbour cell numbers
     int32_t nnodes, ncells;
@@ -149,6 +155,9 @@ struct McArgs {
     // and the number of random draws the emission consumed
     const void* emitted;
     const uint32_t* emit_draw;
+    // coherence keys of walk requests (event/walk pipeline): start points are binned on a raster of
+    // the grid box, t = (r - key_lo) * key_scale in [0, 16)
+    double key_lo[3], key_scale[3];
 };
 
 // ------------------------------------------------------------------------- Philox4x32-10

@@ -28,6 +28,7 @@ static int64_t g_tune_slots = 0;   // in-flight packet slots of the event/walk p
 static int g_pipeline = 1;   // 0: persistent lane pipeline (k_mc); 1: event and walk kernels (k_events/k_walks)
 static uint64_t g_batch_alone = (uint64_t)1 << 21;   // batched members this large run alone (0: never)
 static int g_topdown_descent = 0;   // 1: mixed walks of TopDown trees descend from the root at every step
+static int g_walk_order = 0;        // 1: the walk kernel takes each pass's walks in coherence-key order
 
 // Mixed-precision tree walks follow the ropes (the same cells as any search method) unless TopDown
 // descents were asked for a TopDown tree.
@@ -211,6 +212,12 @@ int sg_set_mixed_topdown(int descend)
     return 0;
 }
 
+int sg_set_walk_order(int sorted)
+{
+    g_walk_order = sorted ? 1 : 0;
+    return 0;
+}
+
 int sg_set_tuning(int block, int waves_per_cu, int compact_lanes)
 {
     if (block == 64 || block == 128 || block == 256) g_tune_block = block;
@@ -314,6 +321,7 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     // mixed-precision records: subdivision counts per axis (a child adds one along each axis of an
     // octree, or along its parent's split axis of a binary tree), density or first child, ropes
     std::vector<int32_t> cellof(nn);
+    std::vector<uint64_t> kpack(nn);
     std::vector<int32_t> ropes;
     h.build_ropes(ropes);
     std::vector<uint32_t> nsub(nn, 0);   // nx | ny << 5 | nz << 10
@@ -340,19 +348,23 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     }
     std::array<float, 3> extf3, widthf3;
     for (int a = 0; a < 3; a++) { extf3[a] = (float)ext[a]; widthf3[a] = (float)(ext[a + 3] - ext[a]); }
-    // split plane of an internal node along axis a, exactly as the walker would compute it
+    // split plane of an internal node along axis a, exactly as the walker computes walls: one fused
+    // multiply-add, so that every wall is the same float wherever it is computed
     auto split_plane = [&](int64_t l, int a) {
         const int n = (int)((nsub[l] >> (5 * a)) & 31u) + 1;
         return std::fmaf((float)(2 * kidx[3 * (size_t)l + a] + 1), std::ldexp(widthf3[a], -n), extf3[a]);
     };
     for (int64_t l = 0; l < nn; l++) {
         uint32_t nx = nsub[l] & 31u, ny = (nsub[l] >> 5) & 31u, nz = (nsub[l] >> 10) & 31u;
-        SG_REQUIRE(nx < 31 && ny < 31 && nz < 31, "tree too deep for the mixed-precision records (30 subdivisions)");
+        SG_REQUIRE(nx <= 21 && ny <= 21 && nz <= 21,
+                   "tree too deep for the mixed-precision records (21 subdivisions per axis)");
         TreeNode32& f = g->hnode32[nid[l]];
         const bool leaf = h.child0[l] < 0;
-        f.code = nsub[l] | (leaf ? 1u << 15 : 0u) | ((uint32_t)(leaf ? 0 : h.dir[l]) << 16);
+        f.code = nsub[l] | (leaf ? kTreeLeaf : 0u) | ((uint32_t)(leaf ? 0 : h.dir[l]) << 16);
         f.val = leaf ? 0u : (uint32_t)nid[h.child0[l]];
         cellof[nid[l]] = h.cellnumber[l];
+        kpack[nid[l]] = (uint64_t)kidx[3 * (size_t)l] | (uint64_t)kidx[3 * (size_t)l + 1] << 21 |
+                        (uint64_t)kidx[3 * (size_t)l + 2] << 42;
         if (!leaf) {
             // an internal node holds its split planes where a leaf holds its ropes: x, y, z for an
             // octree node, the one plane of its split axis for a binary node
@@ -368,7 +380,12 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
         for (int w = 0; w < 6; w++) {
             const int32_t rope = ropes[6 * (size_t)l + w];
             f.rope[w] = rope >= 0 ? nid[rope] : -1;
-            if (leaf && lists) {
+            // the walker steps into a rope's indices by shifting them down per axis
+            if (rope >= 0)
+                for (int a = 0; a < 3; a++)
+                    SG_REQUIRE(((nsub[rope] >> (5 * a)) & 31u) <= ((nsub[l] >> (5 * a)) & 31u),
+                               "a tree rope is finer than its leaf");
+            if (lists) {
                 // consistency with the neighbour lists: the first (largest-overlap) neighbour, raised to
                 // this node's level when finer, is the same rope
                 int32_t r = h.nb[6 * (size_t)l + w].empty() ? -1 : h.nb[6 * (size_t)l + w][0];
@@ -378,6 +395,7 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
         }
     }
     g->cellof.upload(cellof);
+    g->kpack.upload(kpack);
     off[nc * 6] = (int32_t)ids.size();
     SG_REQUIRE(ids.size() < ((size_t)1 << 31), "neighbour lists too long");
     g->nboff.upload(off);
@@ -390,6 +408,7 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     t.rec64 = g->rec64.as<TreeRec64>();
     t.node32 = g->node32.as<TreeNode32>();
     t.cellof = g->cellof.as<int32_t>();
+    t.kpack = g->kpack.as<uint64_t>();
     t.nboff = g->nboff.as<int32_t>();
     t.nbids = g->nbids.as<int32_t>();
     t.nnodes = (int32_t)nn; t.ncells = (int32_t)nc;
@@ -964,6 +983,18 @@ static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64
         a.store_abs = s->dust->store_abs;
         a.labs = s->dust->store_abs ? s->dust->labs.as<double>() : nullptr;
         a.ncells = s->dust->grid->ncells;
+        // the raster of start points for the walk order: the grid's box
+        const sg_grid* gr = s->dust->grid;
+        double box[6];
+        if (gr->kind == GK_CART) {
+            const CartView& c = gr->cart;
+            box[0] = c.xmin; box[1] = c.ymin; box[2] = c.zmin; box[3] = c.xmax; box[4] = c.ymax; box[5] = c.zmax;
+        } else if (gr->kind == GK_TREE) for (int i = 0; i < 6; i++) box[i] = gr->tree.ext[i];
+        else for (int i = 0; i < 6; i++) box[i] = gr->voro.ext[i];
+        for (int i = 0; i < 3; i++) {
+            a.key_lo[i] = box[i];
+            a.key_scale[i] = box[i + 3] > box[i] ? 16.0 / (box[i + 3] - box[i]) : 0.0;
+        }
     }
     a.ni = s->instr->n;
     for (int i = 0; i < a.ni; i++) sd.instr[i] = s->instr->dev[i];
@@ -1075,6 +1106,7 @@ static unsigned long long* g_split_host = nullptr;
 // launch states. Allocated on first use, grown as needed, kept until the process ends.
 struct SplitPool {
     DevBuf slots[2], queue[2], prog[2], emitted, emitdraw;
+    DevBuf bins[2];   // per group: bin counts (zero between passes: cleared once, then by each pass's scan), offsets
 };
 static SplitPool* g_pool = nullptr;
 static SplitPool& pool()
@@ -1111,16 +1143,33 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
     for (int g = 0; g < G; g++) {
         const int64_t cap = Pg[g] * (a.ni + 1);
         SplitPool& pl = pool();
+        const bool sorted = g_walk_order != 0;
         if (pl.slots[g].bytes < slot_bytes<R>(Pg[g])) pl.slots[g].alloc(slot_bytes<R>(Pg[g]));
-        if (pl.queue[g].bytes < queue_bytes<R>(cap)) pl.queue[g].alloc(queue_bytes<R>(cap));
+        if (pl.queue[g].bytes < queue_bytes<R>(cap, sorted)) pl.queue[g].alloc(queue_bytes<R>(cap, sorted));
+        if (sorted && pl.bins[g].bytes == 0) {
+            pl.bins[g].alloc(2 * (size_t)kKeyBins * sizeof(uint32_t));
+            SG_HIP(hipMemsetAsync(pl.bins[g].p, 0, kKeyBins * sizeof(uint32_t), st[g]));
+        }
         if (pl.prog[g].bytes < 4 * sizeof(unsigned long long)) pl.prog[g].alloc(4 * sizeof(unsigned long long));
         pc[g] = pl.prog[g].as<unsigned long long>();
         sl[g] = Slots<R>{pl.slots[g].as<R>(),
                          reinterpret_cast<uint32_t*>(pl.slots[g].as<unsigned char>() + (size_t)Pg[g] * SR_NR * sizeof(R)),
                          Pg[g]};
-        q[g] = WalkQueue<R>{pl.queue[g].as<R>(),
-                            reinterpret_cast<uint32_t*>(pl.queue[g].as<unsigned char>() + (size_t)cap * QR_NR * sizeof(R)),
-                            cap, pc[g] + 1};
+        // requests, then (sorted order) the sorted requests, keys, ranks, bin counts and offsets
+        unsigned char* qb = pl.queue[g].as<unsigned char>();
+        q[g] = WalkQueue<R>{};
+        q[g].req = reinterpret_cast<WalkReq<R>*>(qb);
+        q[g].cap = cap;
+        q[g].tail = pc[g] + 1;
+        if (sorted) {
+            unsigned char* p = qb + (size_t)cap * sizeof(WalkReq<R>);
+            q[g].sorted = reinterpret_cast<WalkReq<R>*>(p);
+            p += (size_t)cap * sizeof(WalkReq<R>);
+            q[g].key = reinterpret_cast<uint32_t*>(p);
+            q[g].rank = q[g].key + cap;
+            q[g].hist = pl.bins[g].as<uint32_t>();
+            q[g].off = q[g].hist + kKeyBins;
+        }
         // slot states PS_FRESH and slot counts zero (adjacent arrays)
         SG_HIP(hipMemsetAsync(sl[g].u + (size_t)SU_STATE * Pg[g], 0, (size_t)Pg[g] * (SU_NU - SU_STATE) * sizeof(uint32_t),
                               st[g]));
@@ -1176,6 +1225,11 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
                     SG_HIP(hipMemsetAsync(pc[g] + 1, 0, sizeof(unsigned long long), st[g]));
                     hipLaunchKernelGGL(k_events<R>, dim3((Pg[g] + ebs - 1) / ebs), dim3(ebs), eshm, st[g], ab, sl[g], q[g]);
                     SG_HIP(hipGetLastError());
+                    if (q[g].sorted) {
+                        hipLaunchKernelGGL(k_key_scan, dim3(1), dim3(1024), 0, st[g], q[g].hist, q[g].off);
+                        hipLaunchKernelGGL(k_key_scatter<R>, dim3(g_cus * 8), dim3(256), 0, st[g], q[g], ab.error);
+                        SG_HIP(hipGetLastError());
+                    }
                     SG_HIP(hipMemsetAsync(pc[g] + 3, 0, sizeof(unsigned long long), st[g]));
                     lt.kbefore(st[g]);
                     hipLaunchKernelGGL((k_walks<W, ABS>), dim3(blocks), dim3(bs), shm, st[g], v, ab, q[g], sl[g], pc[g] + 3,
@@ -1307,6 +1361,8 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
         counters[13] = lt.launches();
         counters[14] = (uint64_t)(lt.kernel_ms() * 1000.0);
     }
+    SG_REQUIRE(!(err & 2), "walk order: a request's sorted place lies beyond the pass's requests");
+    SG_REQUIRE(!(err & 4), "walk queue: a request names no instrument pixel or packet slot");
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH
 }
@@ -1605,6 +1661,8 @@ int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const u
         counters[13] = lt.launches();
         counters[14] = (uint64_t)(lt.kernel_ms() * 1000.0);
     }
+    SG_REQUIRE(!(err & 2), "walk order: a request's sorted place lies beyond the pass's requests");
+    SG_REQUIRE(!(err & 4), "walk queue: a request names no instrument pixel or packet slot");
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH
 }

@@ -13,8 +13,11 @@
 
 namespace sg {
 
-enum Stage : int { ST_NEWPKT = 0, ST_PEEL, ST_PEELW, ST_OWN_START, ST_OWN, ST_OWN_DONE, ST_PROP, ST_PROP_DONE,
-                   ST_PEELW_DONE, ST_SCATTER, ST_DEAD };
+// Each walking stage is followed by its finished stage (a walk ends by stepping its stage by one).
+enum Stage : int { ST_NEWPKT = 0, ST_PEEL, ST_PEELW, ST_PEELW_DONE, ST_OWN_START, ST_OWN, ST_OWN_DONE, ST_PROP,
+                   ST_PROP_DONE, ST_SCATTER, ST_DEAD };
+static_assert(ST_PEELW_DONE == ST_PEELW + 1 && ST_OWN_DONE == ST_OWN + 1 && ST_PROP_DONE == ST_PROP + 1,
+              "a finished stage follows its walking stage");
 
 // ------------------------------------------------------------------------- emission (FP64)
 
@@ -218,17 +221,30 @@ template <typename R> __device__ __forceinline__ int pixel_on_detector(const Ins
 // read and written when a lane is re-armed. This keeps the crossing loop's register footprint low
 // enough for several waves per SIMD.
 
+// limit: the optical depth beyond which the walk ends (the propagation target, or the peel-off
+// optical-depth cut; infinite for none); nent: the walk's entry segments (outside the grid).
 template <class W> struct Hot {
     using R = typename W::Real;
     W w;
-    R tau, s, tprev, sprev, target, L0;
-    int nseg, stage;
+    R tau, s, tprev, sprev, limit, L0;
+    int nseg, nent, stage;
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         w.park(p);
-        p.io(tau); p.io(s); p.io(tprev); p.io(sprev); p.io(target); p.io(L0);
-        p.io(nseg); p.io(stage);
+        p.io(tau); p.io(s); p.io(tprev); p.io(sprev); p.io(limit); p.io(L0);
+        p.io(nseg); p.io(nent); p.io(stage);
     }
+    // After w.init: the running sums start at zero. stage: ST_PEELW, ST_OWN or ST_PROP; lim: the end
+    // optical depth (> 0), anything else for none.
+    __device__ __forceinline__ void start_walk(int st, R lim)
+    {
+        tau = 0; s = 0; tprev = 0; sprev = 0; nseg = 0;
+        nent = w.entries();
+        limit = lim > (R)0 ? lim : (R)INFINITY;
+        stage = st;
+    }
+    // cell crossings of the finished walk (segments inside the grid)
+    __device__ __forceinline__ uint32_t crossings() const { return (uint32_t)(nseg - nent); }
 };
 // DustGridPath::pathlength (DustGridPath.cpp:162-173, NR.hpp:76-95) at the end of a propagation
 // re-walk: the distance at which the cumulative optical depth reaches the target, interpolated
@@ -237,15 +253,15 @@ template <class W> struct Hot {
 template <class W> __device__ __forceinline__ typename W::Real prop_length(const Hot<W>& h)
 {
     using R = typename W::Real;
-    if (h.nseg == 0 || !(h.target > (R)0)) return (R)0;
-    if (h.tau > h.target || (h.tau == h.target && h.nseg > 1))
-        return interp_linlin(h.target, h.tprev, h.tau, h.sprev, h.s);
+    if (h.nseg == 0 || !(h.limit < (R)INFINITY)) return (R)0;
+    if (h.tau > h.limit || (h.tau == h.limit && h.nseg > 1))
+        return interp_linlin(h.limit, h.tprev, h.tau, h.sprev, h.s);
     return h.s;
 }
 
 template <class W> __host__ __device__ constexpr int hot_words()
 {
-    return W::kParkWords + (int)(6 * sizeof(typename W::Real) / 4) + 2;
+    return W::kParkWords + (int)(6 * sizeof(typename W::Real) / 4) + 3;
 }
 
 // Hot state is parked in LDS (structure-of-arrays, one slot per lane) while a wave re-arms lanes,
@@ -391,10 +407,10 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
                 peel++;
                 break;
             }
-            pixel = pix; Lpeel = Lp; h.tau = 0; h.s = 0; h.nseg = 0;
+            pixel = pix; Lpeel = Lp;
             h.w.init(g, lds, nlds, rx, ry, rz, in.kobs[0], in.kobs[1], in.kobs[2]);
+            h.start_walk(ST_PEELW, (R)a.tau_cut);
             c_walks++;
-            h.stage = ST_PEELW;
             armed = true;
             break;
         }
@@ -407,10 +423,10 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
             break;
         }
         case ST_OWN_START: {
-            h.tau = 0; h.s = 0; h.nseg = 0; h.L0 = L;
+            h.L0 = L;
             h.w.init(g, lds, nlds, rx, ry, rz, kx, ky, kz);
+            h.start_walk(ST_OWN, (R)0);
             c_walks++;
-            h.stage = ST_OWN;
             armed = true;
             break;
         }
@@ -437,10 +453,8 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
                 R q = ((R)1.0 - (R)a.xi) * p + (R)a.xi / taupath;
                 L = L * (p / q);
             }
-            h.target = tt;
-            h.tau = 0; h.s = 0; h.tprev = 0; h.sprev = 0; h.nseg = 0;
             h.w.init(g, lds, nlds, rx, ry, rz, kx, ky, kz);
-            h.stage = ST_PROP;
+            h.start_walk(ST_PROP, tt);
             armed = true;
             break;
         }
@@ -510,12 +524,13 @@ __device__ __noinline__ void service_call(uint32_t* park, int nl, int lane, Cold
     lds_store(q, park, nl, lane);
 }
 
-// One crossing of the current walk; marks the walk finished when the path ends (or, for the
-// propagation re-walk, when the target optical depth is reached). Returns which kind of cell
-// crossing was made: 0 none (entry segment or end of path), 1 own path, 2 peel-off, 3 re-walk.
+// One step of the current walk; marks the walk finished (its stage steps to the finished one) when
+// the path ends, or when the optical depth passes the walk's limit (the propagation target, or the
+// peel-off cut). Returns whether the walk goes on. Crossings are counted per walk when it ends
+// (segments minus entry segments), so nothing here is counted across the wave.
 template <class W, bool ABS>
-__device__ __forceinline__ int crossing(Hot<W>& h, const typename W::View& g, const int32_t* lds, int nlds,
-                                        const McArgs& a)
+__device__ __forceinline__ bool crossing(Hot<W>& h, const typename W::View& g, const int32_t* lds, int nlds,
+                                         const McArgs& a)
 {
     using R = typename W::Real;
     int m;
@@ -528,8 +543,7 @@ __device__ __forceinline__ int crossing(Hot<W>& h, const typename W::View& g, co
     const bool seg = got && ds > (R)0;
     const R dsv = seg ? ds : (R)0;
     const R dtau = seg ? ((R)a.kext * rho) * ds : (R)0;   // KappaRho (DustSystem.cpp:459-485)
-    const int st = h.stage;
-    if (ABS && st == ST_OWN && seg && m >= 0) {
+    if (ABS && h.stage == ST_OWN && seg && m >= 0) {
         // OligoDustSystem::absorb via simulateescapeandabsorption (MonteCarloSimulation.cpp:445-458)
         R expfactorm = -rexpm1<R>(-dtau);
         R Lintm = h.L0 * rexp<R>(-h.tau) * expfactorm;
@@ -546,12 +560,19 @@ __device__ __forceinline__ int crossing(Hot<W>& h, const typename W::View& g, co
     // the walk ends at the end of the path, for a peel-off beyond the optical-depth cut, and for the
     // propagation re-walk at the first segment past the target optical depth (prop_length then
     // interpolates the interaction point)
-    const bool end = !got || (st == ST_PEELW && a.tau_cut > 0.0 && h.tau > (R)a.tau_cut)
-                     || (st == ST_PROP && h.target > (R)0 && h.tau > h.target);
-    const int done = st == ST_PROP ? ST_PROP_DONE : st == ST_OWN ? ST_OWN_DONE : ST_PEELW_DONE;
-    h.stage = end ? done : st;
-    const int kind = st == ST_PEELW ? 2 : st == ST_OWN ? 1 : 3;
-    return seg && m >= 0 ? kind : 0;
+    const bool end = !got || h.tau > h.limit;
+    h.stage += end ? 1 : 0;
+    return !end;
+}
+
+// Adds a finished walk's crossings to the lane's count of its kind (own path, peel-off, re-walk).
+template <class W> __device__ __forceinline__ void count_walk(const Hot<W>& h, uint32_t& n_own, uint32_t& n_peel,
+                                                               uint32_t& n_re)
+{
+    const uint32_t c = h.crossings();
+    n_own += h.stage == ST_OWN_DONE ? c : 0u;
+    n_peel += h.stage == ST_PEELW_DONE ? c : 0u;
+    n_re += h.stage == ST_PROP_DONE ? c : 0u;
 }
 
 // ST_PEELW, ST_OWN or ST_PROP, as a bit test (a chain of comparisons compiles to branches)
@@ -559,6 +580,12 @@ __device__ __forceinline__ bool walking_stage(int st)
 {
     static_assert(ST_PEELW < 32 && ST_OWN < 32 && ST_PROP < 32, "stage bits");
     return ((1u << ST_PEELW | 1u << ST_OWN | 1u << ST_PROP) >> (uint32_t)st) & 1u;
+}
+
+// ST_PEELW_DONE, ST_OWN_DONE or ST_PROP_DONE
+__device__ __forceinline__ bool finished_stage(int st)
+{
+    return ((1u << ST_PEELW_DONE | 1u << ST_OWN_DONE | 1u << ST_PROP_DONE) >> (uint32_t)st) & 1u;
 }
 
 __device__ __forceinline__ void add_wave(unsigned long long* dst, unsigned long long x)
@@ -593,11 +620,13 @@ __device__ __forceinline__ void drain(const typename W::View& g, const McArgs& a
     c.fu(CU_PK) = 0; c.fu(CU_SCAT) = 0; c.fu(CU_NPEEL) = 0; c.fu(CU_WALKS) = 0;
     Hot<W> h;
     h.stage = ST_NEWPKT;
+    h.nseg = 0; h.nent = 0;
     h.w.nrecompute = 0; h.w.nstuck = 0;
-    unsigned long long n_own = 0, n_peel = 0, n_re = 0;   // wave-uniform counts
+    uint32_t n_own = 0, n_peel = 0, n_re = 0;   // the lane's crossings by walk kind
     const int thr = a.compact;
     for (;;) {
         bool need = !walking_stage(h.stage) && h.stage != ST_DEAD;
+        if (finished_stage(h.stage)) count_walk(h, n_own, n_peel, n_re);
 #if SG_PARK
         if (__ballot(need)) {
             // park every lane's hot state, re-arm the idle lanes, reload
@@ -608,25 +637,20 @@ __device__ __forceinline__ void drain(const typename W::View& g, const McArgs& a
 #else
         if (need) service<W, ABS>(h, c, g, s_nodes, nlds, a, s_instr);
 #endif
-        uint64_t alive = __ballot(h.stage != ST_DEAD);
+        const uint64_t alive = __ballot(h.stage != ST_DEAD);
         if (!alive) break;
-        int need_n = max(1, min(thr, __popcll(alive) >> 1));
+        const int need_n = max(1, min(thr, __popcll(alive) >> 1));
+        bool walk = walking_stage(h.stage);
         for (;;) {
-            int kind = 0;
-            if (walking_stage(h.stage)) kind = crossing<W, ABS>(h, g, s_nodes, nlds, a);
-            n_own += __popcll(__ballot(kind == 1));
-            n_peel += __popcll(__ballot(kind == 2));
-            n_re += __popcll(__ballot(kind == 3));
-            bool walk = walking_stage(h.stage);
-            uint64_t idle = __ballot(!walk && h.stage != ST_DEAD);
-            if (__popcll(idle) >= need_n || !__ballot(walk)) break;
+            if (walk) walk = crossing<W, ABS>(h, g, s_nodes, nlds, a);
+            // lanes alive and not walking are idle (alive is fixed while the wave walks)
+            const uint64_t wm = __ballot(walk);
+            if (__popcll(alive & ~wm) >= need_n || !wm) break;
         }
     }
-    if (__lane_id() == 0) {
-        if (n_own) atomicAdd(a.counters + 1, n_own);
-        if (n_peel) atomicAdd(a.counters + 2, n_peel);
-        if (n_re) atomicAdd(a.counters + 8, n_re);
-    }
+    add_wave(a.counters + 1, n_own);
+    add_wave(a.counters + 2, n_peel);
+    add_wave(a.counters + 8, n_re);
     add_wave(a.counters + 0, c.fu(CU_PK));
     add_wave(a.counters + 3, c.fu(CU_SCAT));
     add_wave(a.counters + 4, c.fu(CU_NPEEL));

@@ -23,9 +23,9 @@ namespace sg {
 enum SlotR { SR_RX = 0, SR_RY, SR_RZ, SR_KX, SR_KY, SR_KZ, SR_L, SR_RES, SR_NR };
 // The last four slot words count the slot's packets, scatterings, detected peel-offs and started
 // walks over a run; they are summed once at the end (no per-pass atomics).
-// SU_HNODE..SU_HK: the leaf the packet's position lies in, as the last propagation walk left it
-// (node -1: unknown), handed to the walks that start there.
-enum SlotU { SU_PLO = 0, SU_PHI, SU_DRAW, SU_W0, SU_W1, SU_W2, SU_W3, SU_NSCATT, SU_HNODE, SU_HI, SU_HJ, SU_HK,
+// SU_HINT: the leaf (tree node, or Voronoi cell) the packet's position lies in, as the last
+// propagation walk left it (-1: unknown), handed to the walks that start there.
+enum SlotU { SU_PLO = 0, SU_PHI, SU_DRAW, SU_W0, SU_W1, SU_W2, SU_W3, SU_NSCATT, SU_HINT,
              SU_STATE, SU_CPK, SU_CSCAT, SU_CPEEL, SU_CWALK, SU_NU };
 enum PacketState : uint32_t { PS_FRESH = 0, PS_OWN, PS_PROP, PS_DEAD };
 enum WalkKind : uint32_t { WK_PEEL = 0, WK_OWN = 1, WK_PROP = 2 };
@@ -46,25 +46,83 @@ template <typename R> __host__ __device__ constexpr size_t slot_bytes(int64_t n)
 // Walk requests: origin (3), direction (3, unused for peel-offs: the instrument's direction is
 // taken from its record) and a value (peel-off luminosity, own-path luminosity, or propagation
 // target optical depth) in the run's real type; a tag (kind | instrument << 2), an index (packet
-// slot, or pixel for a peel-off) and the start leaf hint (node, indices) of the walker.
+// slot, or pixel for a peel-off) and the start leaf hint of the walker. In the LDS staging area
+// they are structure-of-arrays (QR_*, QU_* fields); in the queues one record per request, so that
+// a request moved to its sorted place, or read by a lane, is one contiguous block.
 enum QueueR { QR_OX = 0, QR_OY, QR_OZ, QR_DX, QR_DY, QR_DZ, QR_VAL, QR_NR };
-enum QueueU { QU_TAG = 0, QU_IDX, QU_HN, QU_HI, QU_HJ, QU_HK, QU_NU };
-
-// Start-leaf hint of a walk.
-struct LeafHint {
-    int32_t n, i, j, k;
+enum QueueU { QU_TAG = 0, QU_IDX, QU_HINT, QU_NU };
+template <typename R> struct alignas(16) WalkReq {
+    R o[3], d[3], val;
+    uint32_t tag, idx, hint;
 };
+static_assert(sizeof(WalkReq<float>) == 48 && sizeof(WalkReq<double>) == 80, "walk request records");
+
+// Walk order. The walks of a pass are sorted by a coherence key before the walk kernel takes them:
+// peel-offs by instrument and the Morton order of their pixel (rays toward one instrument through
+// neighbouring pixels cross the same cells), own-path and propagation walks by the Morton order of
+// their start point. Lanes and waves that run at the same time then touch a small part of the grid,
+// which the L2 cache of their XCD holds. A counting sort: the event kernel counts each request into
+// its key's bin and keeps its rank there, a one-block scan turns the counts into bin offsets (and
+// clears them), and a scatter moves each request to offset + rank. Histories do not depend on the
+// order (only the order of the tally additions changes).
+constexpr int kKeyBits = 16;
+constexpr uint32_t kKeyBins = 1u << kKeyBits;
 template <typename R> struct WalkQueue {
-    R* r;
-    uint32_t* u;
+    WalkReq<R>* req;            // requests in the order the event kernel appended them
+    WalkReq<R>* sorted;         // the same in key order (null: the walk kernel reads req)
+    uint32_t* key;              // per request: its bin, and its rank within the bin
+    uint32_t* rank;
+    uint32_t* hist;             // kKeyBins counts (zero between passes; a buffer of its own, cleared once)
+    uint32_t* off;              // kKeyBins bin offsets
     int64_t cap;
     unsigned long long* tail;   // requests appended by the event kernel
-    __device__ __forceinline__ R& fr(int f, int64_t i) const { return r[f * cap + i]; }
-    __device__ __forceinline__ uint32_t& fu(int f, int64_t i) const { return u[f * cap + i]; }
 };
-template <typename R> __host__ __device__ constexpr size_t queue_bytes(int64_t cap)
+// requests, sorted requests, keys and ranks (the bin counts and offsets are held apart)
+template <typename R> __host__ __device__ constexpr size_t queue_bytes(int64_t cap, bool sort)
 {
-    return (size_t)cap * (QR_NR * sizeof(R) + QU_NU * sizeof(uint32_t));
+    return (size_t)cap * sizeof(WalkReq<R>) * (sort ? 2 : 1) + (sort ? (size_t)cap * 8 : 0);
+}
+
+// 2-D and 3-D Morton codes (bit interleaving) of small integer coordinates.
+__device__ __forceinline__ uint32_t spread2(uint32_t v)   // 8 bits -> every second bit
+{
+    v &= 0xffu;
+    v = (v | (v << 4)) & 0x0f0fu;
+    v = (v | (v << 2)) & 0x3333u;
+    v = (v | (v << 1)) & 0x5555u;
+    return v;
+}
+__device__ __forceinline__ uint32_t spread3(uint32_t v)   // 5 bits -> every third bit
+{
+    v &= 0x1fu;
+    v = (v | (v << 8)) & 0x100fu;
+    v = (v | (v << 4)) & 0x10c3u;
+    v = (v | (v << 2)) & 0x1249u;
+    return v;
+}
+// The coherence key of a request: 4 bits of class (peel-off instrument 0-7, own path 8, propagation
+// 9) above 12 bits of Morton order (a peel-off's pixel on a 64 x 64 raster of its frame; a start
+// point on a 16 x 16 x 16 raster of the grid box).
+template <typename R>
+__device__ __forceinline__ uint32_t walk_key(const McArgs& a, const InstrDev* instr, uint32_t tag, uint32_t idx, R ox,
+                                             R oy, R oz)
+{
+    const uint32_t kind = tag & 3u;
+    if (kind == WK_PEEL) {
+        const InstrDev& in = instr[tag >> 2];
+        const uint32_t i = idx % (uint32_t)in.Nx, j = idx / (uint32_t)in.Nx;
+        const uint32_t qi = (uint32_t)(((uint64_t)i << 6) / (uint32_t)in.Nx);
+        const uint32_t qj = (uint32_t)(((uint64_t)j << 6) / (uint32_t)in.Ny);
+        return ((tag >> 2) << 12) | spread2(qi) | (spread2(qj) << 1);
+    }
+    auto q = [](double v, double lo, double sc) {
+        const double t = (v - lo) * sc;
+        return (uint32_t)(!(t > 0.0) ? 0.0 : t > 15.0 ? 15.0 : t);   // NaN to 0
+    };
+    const uint32_t qx = q((double)ox, a.key_lo[0], a.key_scale[0]);
+    const uint32_t qy = q((double)oy, a.key_lo[1], a.key_scale[1]);
+    const uint32_t qz = q((double)oz, a.key_lo[2], a.key_scale[2]);
+    return ((kind == WK_OWN ? 8u : 9u) << 12) | spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
 }
 
 
@@ -90,7 +148,7 @@ template <typename R> struct StagedQueue {
     int cap;
     unsigned* cnt;
     __device__ __forceinline__ void push(uint32_t kind, uint32_t inst, uint32_t idx, R ox, R oy, R oz, R dx, R dy,
-                                         R dz, R val, const LeafHint& hint) const
+                                         R dz, R val, int32_t hint) const
     {
         uint64_t mask = __ballot(1);
         int lane = __lane_id();
@@ -105,8 +163,7 @@ template <typename R> struct StagedQueue {
         r[QR_VAL * cap + i] = val;
         u[QU_TAG * cap + i] = kind | (inst << 2);
         u[QU_IDX * cap + i] = idx;
-        u[QU_HN * cap + i] = (uint32_t)hint.n; u[QU_HI * cap + i] = (uint32_t)hint.i;
-        u[QU_HJ * cap + i] = (uint32_t)hint.j; u[QU_HK * cap + i] = (uint32_t)hint.k;
+        u[QU_HINT * cap + i] = (uint32_t)hint;
     }
 };
 template <typename R> __host__ __device__ constexpr size_t staged_bytes(int cap)
@@ -120,7 +177,7 @@ template <typename R> __host__ __device__ constexpr size_t staged_bytes(int cap)
 template <typename R, class Q>
 __device__ __forceinline__ uint32_t peel_offs(const McArgs& a, const InstrDev* instr, const Q& q,
                                               bool scattering, R rx, R ry, R rz, R kx, R ky, R kz, R L,
-                                              const LeafHint& hint)
+                                              int32_t hint)
 {
     const R gg = (R)a.g;
     uint32_t n = 0;
@@ -199,8 +256,7 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
         rng.plo = sl.fu(SU_PLO, i); rng.phi = sl.fu(SU_PHI, i); rng.draw = sl.fu(SU_DRAW, i);
         rng.w0 = sl.fu(SU_W0, i); rng.w1 = sl.fu(SU_W1, i); rng.w2 = sl.fu(SU_W2, i); rng.w3 = sl.fu(SU_W3, i);
         int nscatt = (int)sl.fu(SU_NSCATT, i);
-        LeafHint hint{(int32_t)sl.fu(SU_HNODE, i), (int32_t)sl.fu(SU_HI, i), (int32_t)sl.fu(SU_HJ, i),
-                      (int32_t)sl.fu(SU_HK, i)};
+        int32_t hint = (int32_t)sl.fu(SU_HINT, i);
         // at most a few transitions per pass: a finished packet is replaced by a new one at once; a slot
         // without a dust system detects its peel-offs directly and takes up to four packets per pass
         for (int step = 0; step < 4; step++) {
@@ -223,7 +279,7 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
                     philox10(rng.w0, rng.w1, rng.w2, rng.w3, rng.k0, rng.k1);
                 }
                 nscatt = 0;
-                hint.n = -1;   // a new position: the leaf is not known yet
+                hint = -1;   // a new position: the leaf is not known yet
                 uint32_t np = peel_offs<R>(a, s_instr, sq, false, rx, ry, rz, kx, ky, kz, L, hint);
                 c_npeel += np;
                 if (!a.has_dust) continue;
@@ -285,7 +341,7 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
         sl.fu(SU_PLO, i) = rng.plo; sl.fu(SU_PHI, i) = rng.phi; sl.fu(SU_DRAW, i) = rng.draw;
         sl.fu(SU_W0, i) = rng.w0; sl.fu(SU_W1, i) = rng.w1; sl.fu(SU_W2, i) = rng.w2; sl.fu(SU_W3, i) = rng.w3;
         sl.fu(SU_NSCATT, i) = (uint32_t)nscatt;
-        sl.fu(SU_HNODE, i) = (uint32_t)hint.n;
+        sl.fu(SU_HINT, i) = (uint32_t)hint;
         sl.fu(SU_CPK, i) += c_pk; sl.fu(SU_CSCAT, i) += c_scat; sl.fu(SU_CPEEL, i) += c_npeel;
         sl.fu(SU_CWALK, i) += c_walks;
     }
@@ -297,10 +353,72 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
     const unsigned n = s_cnt;
     const uint64_t base = s_base;
     for (unsigned j = threadIdx.x; j < n; j += blockDim.x) {
+        WalkReq<R> w;
+        w.o[0] = sq.r[QR_OX * cap + j]; w.o[1] = sq.r[QR_OY * cap + j]; w.o[2] = sq.r[QR_OZ * cap + j];
+        w.d[0] = sq.r[QR_DX * cap + j]; w.d[1] = sq.r[QR_DY * cap + j]; w.d[2] = sq.r[QR_DZ * cap + j];
+        w.val = sq.r[QR_VAL * cap + j];
+        w.tag = sq.u[QU_TAG * cap + j]; w.idx = sq.u[QU_IDX * cap + j]; w.hint = sq.u[QU_HINT * cap + j];
+        q.req[base + j] = w;
+        if (q.sorted) {
+            const uint32_t k = walk_key<R>(a, s_instr, w.tag, w.idx, w.o[0], w.o[1], w.o[2]);
+            q.key[base + j] = k;
+            q.rank[base + j] = atomicAdd(q.hist + k, 1u);
+        }
+    }
+}
+
+// Bin offsets of the walk order: exclusive prefix sum of the bin counts, which are cleared for the
+// next pass. One block of 1024 threads; each thread sums kKeyBins / 1024 consecutive bins.
+__global__ void __launch_bounds__(1024) k_key_scan(uint32_t* hist, uint32_t* off)
+{
+    constexpr int per = (int)(kKeyBins / 1024);
+    __shared__ uint32_t s_wave[16];
+    const int t = threadIdx.x;
+    uint32_t v[per];
+    uint32_t sum = 0;
 #pragma unroll
-        for (int f = 0; f < QR_NR; f++) q.fr(f, base + j) = sq.r[f * cap + j];
+    for (int k = 0; k < per; k++) { v[k] = hist[t * per + k]; sum += v[k]; }
+    // inclusive scan of the per-thread sums: within each wave, then across the 16 waves
+    uint32_t x = sum;
+    const int lane = __lane_id(), w = t >> 6;
 #pragma unroll
-        for (int f = 0; f < QU_NU; f++) q.fu(f, base + j) = sq.u[f * cap + j];
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[w] = x;
+    __syncthreads();
+    if (t < 64) {
+        uint32_t z = t < 16 ? s_wave[t] : 0u;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const uint32_t y = __shfl_up(z, o);
+            if (lane >= o) z += y;
+        }
+        if (t < 16) s_wave[t] = z;
+    }
+    __syncthreads();
+    uint32_t run = x - sum + (w ? s_wave[w - 1] : 0u);
+#pragma unroll
+    for (int k = 0; k < per; k++) {
+        off[t * per + k] = run;
+        run += v[k];
+        hist[t * per + k] = 0u;
+    }
+}
+
+// Moves each request of the pass to its place in the walk order. A place beyond the pass's requests
+// (bin counts that were not zero when the pass began) is reported through the error flag (bit 2)
+// and not written.
+template <typename R>
+__global__ void __launch_bounds__(256) k_key_scatter(WalkQueue<R> q, int* error)
+{
+    const uint64_t n = *q.tail;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = q.key[j];
+        const uint64_t pos = (uint64_t)q.off[k < kKeyBins ? k : 0] + q.rank[j];
+        if (k < kKeyBins && pos < n) q.sorted[pos] = q.req[j];
+        else atomicOr(error, 2);
     }
 }
 
@@ -394,91 +512,82 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WalkWa
     if (threadIdx.x < SG_MAX_INSTR) s_instr[threadIdx.x] = a.scene->instr[threadIdx.x];
     __syncthreads();
     const uint64_t nq = *q.tail;
+    const WalkReq<R>* reqs = q.sorted ? q.sorted : q.req;
     Hot<W> h;
     h.stage = ST_NEWPKT;   // idle: needs a request
+    h.nseg = 0; h.nent = 0;
     h.w.nrecompute = 0; h.w.nstuck = 0;
     uint32_t idx = 0, inst = 0;
-    unsigned long long n_own = 0, n_peel = 0, n_re = 0;   // wave-uniform counts
-    uint64_t ccur = 0, cend = 0;                          // the wave's chunk of requests
+    uint32_t n_own = 0, n_peel = 0, n_re = 0;   // the lane's crossings by walk kind
+    uint64_t ccur = 0, cend = 0;                 // the wave's chunk of requests
     const int thr = a.compact;
     for (;;) {
         if (h.stage == ST_PEELW_DONE) {
             // FrameInstrument::detect (FrameInstrument.cpp:33-47)
             const InstrDev& in = s_instr[inst];
             unsafeAtomicAdd(in.frame + (size_t)a.ell * in.Nx * in.Ny + idx, (double)(h.L0 * rexp<R>(-h.tau)));
-            h.stage = ST_NEWPKT;
         } else if (h.stage == ST_OWN_DONE) {
             sl.fr(SR_RES, idx) = h.tau;
-            h.stage = ST_NEWPKT;
         } else if (h.stage == ST_PROP_DONE) {
             sl.fr(SR_RES, idx) = prop_length(h);
-            if constexpr (W::kHint) {
-                // the leaf holding the interaction point: where the packet's next walks start
-                int n;
-                int32_t i, j, k;
-                h.w.hint(n, i, j, k);
-                sl.fu(SU_HNODE, idx) = (uint32_t)n;
-                sl.fu(SU_HI, idx) = (uint32_t)i; sl.fu(SU_HJ, idx) = (uint32_t)j; sl.fu(SU_HK, idx) = (uint32_t)k;
-            }
+            // the leaf holding the interaction point: where the packet's next walks start
+            if constexpr (W::kHint) sl.fu(SU_HINT, idx) = (uint32_t)h.w.hint();
+        }
+        if (finished_stage(h.stage)) {
+            count_walk(h, n_own, n_peel, n_re);
             h.stage = ST_NEWPKT;
         }
         const bool want = h.stage == ST_NEWPKT;
         uint64_t r = 0;
         const bool got = chunk_claim(claim, nq, want, ccur, cend, r);
         if (want) {
+            const WalkReq<R> w = got ? reqs[r] : WalkReq<R>{};
+            const uint32_t kind = w.tag & 3u;
+            const uint32_t wi = w.tag >> 2;
+            // a request that names no pixel of an instrument or no packet slot is reported (error bit 2)
+            // and dropped, never followed; the lane stays idle and takes the next one
+            const uint32_t npix = wi < (uint32_t)a.ni ? (uint32_t)(s_instr[wi].Nx * s_instr[wi].Ny) : 0u;
+            const bool valid = kind == WK_PEEL ? w.idx < npix : kind <= WK_PROP && (int64_t)w.idx < sl.n;
             if (!got) h.stage = ST_DEAD;
+            else if (!valid) atomicOr(a.error, 4);
             else {
-                const uint32_t tag = q.fu(QU_TAG, r);
-                const uint32_t kind = tag & 3u;
-                idx = q.fu(QU_IDX, r);
-                inst = tag >> 2;
-                R ox = q.fr(QR_OX, r), oy = q.fr(QR_OY, r), oz = q.fr(QR_OZ, r);
-                h.L0 = q.fr(QR_VAL, r);
-                h.tau = 0; h.s = 0; h.tprev = 0; h.sprev = 0; h.nseg = 0;
+                idx = w.idx;
+                inst = wi;
+                const R ox = w.o[0], oy = w.o[1], oz = w.o[2];
+                const R val = w.val;
                 double dx, dy, dz;
                 if (kind == WK_PEEL) {
                     const InstrDev& in = s_instr[inst];
                     dx = in.kobs[0]; dy = in.kobs[1]; dz = in.kobs[2];
                 } else {
-                    dx = q.fr(QR_DX, r); dy = q.fr(QR_DY, r); dz = q.fr(QR_DZ, r);
+                    dx = w.d[0]; dy = w.d[1]; dz = w.d[2];
                 }
                 if constexpr (W::kHint)
-                    h.w.init_hinted(g, s_nodes, nlds, ox, oy, oz, dx, dy, dz, (int)q.fu(QU_HN, r), (int32_t)q.fu(QU_HI, r),
-                                    (int32_t)q.fu(QU_HJ, r), (int32_t)q.fu(QU_HK, r));
+                    h.w.init_hinted(g, s_nodes, nlds, ox, oy, oz, dx, dy, dz, (int)w.hint);
                 else
                     h.w.init(g, s_nodes, nlds, ox, oy, oz, dx, dy, dz);
-                if (kind == WK_PEEL) {
-                    h.stage = ST_PEELW;
-                } else {
-                    if (kind == WK_OWN) h.stage = ST_OWN;
-                    else {
-                        h.target = h.L0;
-                        h.stage = ST_PROP;
-                    }
-                }
+                // the value: a peel-off's or own path's luminosity, or the propagation target depth
+                h.L0 = val;
+                if (kind == WK_PEEL) h.start_walk(ST_PEELW, (R)a.tau_cut);
+                else if (kind == WK_OWN) h.start_walk(ST_OWN, (R)0);
+                else h.start_walk(ST_PROP, val);
             }
         }
-        uint64_t alive = __ballot(h.stage != ST_DEAD);
+        const uint64_t alive = __ballot(h.stage != ST_DEAD);
         if (!alive) break;
         // wave-uniform, kept in a scalar register
         const int need_n = __builtin_amdgcn_readfirstlane(max(1, min(thr, __popcll(alive) >> 1)));
-        bool walk = walking_stage(h.stage);   // carried around the loop: only crossing() changes the stage
+        bool walk = walking_stage(h.stage);
         for (;;) {
-            int kind = 0;
-            if (walk) kind = crossing<W, ABS>(h, g, s_nodes, nlds, a);
-            n_own += __popcll(__ballot(kind == 1));
-            n_peel += __popcll(__ballot(kind == 2));
-            n_re += __popcll(__ballot(kind == 3));
-            walk = walking_stage(h.stage);
-            uint64_t idle = __ballot(!walk && h.stage != ST_DEAD);
-            if (__popcll(idle) >= need_n || !__ballot(walk)) break;
+            if (walk) walk = crossing<W, ABS>(h, g, s_nodes, nlds, a);
+            // lanes alive and not walking are idle (alive is fixed while the wave walks)
+            const uint64_t wm = __ballot(walk);
+            if (__popcll(alive & ~wm) >= need_n || !wm) break;
         }
     }
-    if (__lane_id() == 0) {
-        if (n_own) atomicAdd(a.counters + 1, n_own);
-        if (n_peel) atomicAdd(a.counters + 2, n_peel);
-        if (n_re) atomicAdd(a.counters + 8, n_re);
-    }
+    add_wave(a.counters + 1, n_own);
+    add_wave(a.counters + 2, n_peel);
+    add_wave(a.counters + 8, n_re);
     add_wave(a.counters + 5, h.w.nstuck);
     add_wave(a.counters + 9, h.w.nrecompute);
 }

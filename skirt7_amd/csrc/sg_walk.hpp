@@ -80,6 +80,7 @@ struct CartW64 {
     static constexpr int kParkWords = 25;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     static constexpr bool kHint = false;
+    __device__ __forceinline__ int entries() const { return ne; }
     static constexpr bool kStep = false;   // next() only
     template <class P> __device__ __forceinline__ void park(P& p)
     {
@@ -230,6 +231,7 @@ struct CartW32 {
     static constexpr int kParkWords = 19;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     static constexpr bool kHint = false;
+    __device__ __forceinline__ int entries() const { return ne; }
     static constexpr bool kStep = false;   // next() only
     template <class P> __device__ __forceinline__ void park(P& p)
     {
@@ -443,6 +445,7 @@ struct TreeW64 {
     static constexpr int kParkWords = 38;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     static constexpr bool kHint = false;
+    __device__ __forceinline__ int entries() const { return en.ne; }
     static constexpr bool kStep = false;   // next() only
     template <class P> __device__ __forceinline__ void park(P& p)
     {
@@ -534,45 +537,46 @@ struct TreeW64 {
     }
 };
 
-// Mixed-precision tree walk (Neighbor semantics): FP32 position snapped onto the exit wall; the
-// next cell is found by descending from the rope of the exit wall (the node across it that covers
-// the whole wall), one 32-byte node record per level; a directed root descent covers the rare case
-// of a point that left the rope node's box through an edge or corner. The walker carries the
-// current leaf's integer indices along x, y, z; every box is ext + [k, k + 1] * width / 2^n. Every
-// wall and split plane is evaluated as one fused multiply-add (a single rounding of the exact value),
-// so a leaf's wall and the split plane its parent used for it are the same float.
+// Mixed-precision tree walk (Neighbor semantics). Every step reads one 32-byte node record (two
+// 16-byte loads issued together). The walker carries the current leaf's integer indices along x, y, z
+// and its subdivision counts; its exit planes (the walls on the sides the ray heads to, each
+// ext + k * width / 2^n as one fused multiply-add, the same float wherever it is computed) and its
+// ropes across them are computed when the leaf is taken over, so a step is: three guarded FP32
+// distances, the smallest (FP64 decides when the FP32 decision is ambiguous: the two smallest agree
+// to 1e-5 relative, an edge or corner crossing, or the smallest is below 1e-5 of the second, a
+// position on or just past a wall after rounding), the position snapped onto the exit wall, and the
+// next leaf: the rope's node, or a descent from it when the rope is coarser. A directed root
+// descent covers the rare case of a point that left the rope node's face through an edge or corner
+// by rounding.
 struct TreeW32 {
     using Real = float;
     using View = TreeView;
     float x, y, z, kx, ky, kz, ix, iy, iz;
-    int32_t ci, cj, ck;      // integer indices of the current leaf
+    float gx, gy, gz;        // 0, or FLT_MAX along an axis the direction is parallel to (|k| < 1e-15)
+    float px, py, pz;        // exit planes of the current leaf
+    int32_t ci, cj, ck;      // its integer indices along x, y, z
     uint32_t nsub;           // its subdivision counts, nx | ny << 5 | nz << 10
+    int32_t rx, ry, rz;      // its ropes across the exit walls
     float rho;
-    int32_t rope[6];
-    int m;                   // node id of the current leaf
-    int lm;                  // leaf of the last segment returned, and its indices
-    int32_t li, lj, lk;
+    int m;                   // its node id
+    int lm;                  // node of the last segment returned
     int ne;
     float e0, e1, e2;
     bool done;
     uint32_t steps, nrecompute, nstuck;
-    static constexpr int kParkWords = 33;
+    static constexpr int kParkWords = 34;
     static constexpr bool kHint = true;
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz); p.io(ix); p.io(iy); p.io(iz);
-        p.io(ci); p.io(cj); p.io(ck); p.io(nsub);
-        p.io(rho); p.io(rope[0]); p.io(rope[1]); p.io(rope[2]); p.io(rope[3]); p.io(rope[4]); p.io(rope[5]);
-        p.io(m); p.io(lm); p.io(li); p.io(lj); p.io(lk); p.io(ne); p.io(e0); p.io(e1); p.io(e2);
+        p.io(gx); p.io(gy); p.io(gz); p.io(px); p.io(py); p.io(pz); p.io(ci); p.io(cj); p.io(ck); p.io(nsub);
+        p.io(rx); p.io(ry); p.io(rz); p.io(rho); p.io(m); p.io(lm); p.io(ne); p.io(e0); p.io(e1); p.io(e2);
         p.io(done); p.io(steps); p.io(nrecompute); p.io(nstuck);
     }
     // cell number of a walk segment's node
     __device__ __forceinline__ static int cell_of(const View& t, int n) { return t.cellof[n]; }
+    __device__ __forceinline__ int entries() const { return ne; }
 
-    __device__ __forceinline__ static float width(const View& t, int a, uint32_t ns)
-    {
-        return ldexpf(t.widthf[a], -(int)((ns >> (5 * a)) & 31u));
-    }
     // Node record n as two 16-byte loads issued together: (code, val, rope0, rope1), (rope2..rope5).
     __device__ __forceinline__ static void load(const View& t, int n, uint4& a, uint4& b)
     {
@@ -585,48 +589,64 @@ struct TreeW32 {
         a = make_uint4(va.x, va.y, va.z, va.w);
         b = make_uint4(vb.x, vb.y, vb.z, vb.w);
     }
-    // Descends from node n (record a, b already loaded; indices i, j, k at its subdivision) to the
-    // leaf containing the current position and takes it over. Child choice by "<" against the split
-    // point (OctTreeNode.cpp:184-189, BinTreeNode.cpp:326-335), except that a point lying exactly on a
-    // split plane goes to the child the ray is heading into (a walk never steps back into a cell behind
-    // it; the reference instead steps a small distance past every wall). Per-axis choices are selects
-    // on flags, not indexed arrays, so that nothing goes to private memory.
+    // Takes over leaf n (record a, b; indices i, j, k at its subdivision): exit planes and ropes on the
+    // sides the ray heads to.
+    __device__ __forceinline__ void take(const View& t, uint4 a, uint4 b, int n, int32_t i, int32_t j, int32_t k)
+    {
+        const bool nx = kx < 0.0f, ny = ky < 0.0f, nz = kz < 0.0f;
+        const uint32_t ns = a.x & 0x7fffu;
+        const float wx = ldexpf(t.widthf[0], -(int)(ns & 31u));
+        const float wy = ldexpf(t.widthf[1], -(int)((ns >> 5) & 31u));
+        const float wz = ldexpf(t.widthf[2], -(int)((ns >> 10) & 31u));
+        px = fmaf((float)(nx ? i : i + 1), wx, t.extf[0]);
+        py = fmaf((float)(ny ? j : j + 1), wy, t.extf[1]);
+        pz = fmaf((float)(nz ? k : k + 1), wz, t.extf[2]);
+        ci = i; cj = j; ck = k;
+        nsub = ns;
+        rho = __uint_as_float(a.y);
+        rx = (int32_t)(nx ? a.z : a.w);
+        ry = (int32_t)(ny ? b.x : b.y);
+        rz = (int32_t)(nz ? b.z : b.w);
+        m = n;
+    }
+    // Descends from node n (record a, b already loaded; indices i, j, k at its subdivision) to the leaf
+    // containing the current position and takes it over. Child choice by "<" against the split point
+    // (OctTreeNode.cpp:184-189, BinTreeNode.cpp:326-335), except that a point lying exactly on a split
+    // plane goes to the child the ray is heading into (a walk never steps back into a cell behind it;
+    // the reference instead steps a small distance past every wall). Per-axis choices are selects on
+    // flags, not indexed arrays, so that nothing goes to private memory.
     __device__ __forceinline__ void descend_loaded(const View& t, uint4 a, uint4 b, int n, int32_t i, int32_t j,
                                                    int32_t k)
     {
-        const bool nx = kx < 0.0f, ny = ky < 0.0f, nz = kz < 0.0f;
+        // upper child: beyond the plane, or on it while heading up (bitwise forms stay in mask registers)
+        const bool ux = !(kx < 0.0f), uy = !(ky < 0.0f), uz = !(kz < 0.0f);
         int level = 0;
-        while (!(a.x & (1u << 15))) {
+        while (!(a.x & kTreeLeaf)) {
+            if (++level > 128) { done = true; return; }   // no tree the records describe is this deep
             const int c0 = (int)a.y;
             if (t.arity == 8) {
                 // split planes ext + (2k + 1) * width / 2^(n + 1), held by the internal node's record
                 const float cx = __uint_as_float(a.z), cy = __uint_as_float(a.w), cz = __uint_as_float(b.x);
-                const int qx = (nx ? x <= cx : x < cx) ? 0 : 1;
-                const int qy = (ny ? y <= cy : y < cy) ? 0 : 1;
-                const int qz = (nz ? z <= cz : z < cz) ? 0 : 1;
-                n = c0 + qx + 2 * qy + 4 * qz;
-                i = 2 * i + qx; j = 2 * j + qy; k = 2 * k + qz;
+                const bool qx = (x > cx) | ((x == cx) & ux);
+                const bool qy = (y > cy) | ((y == cy) & uy);
+                const bool qz = (z > cz) | ((z == cz) & uz);
+                n = c0 + (qx ? 1 : 0) + (qy ? 2 : 0) + (qz ? 4 : 0);
+                i = 2 * i + (qx ? 1 : 0); j = 2 * j + (qy ? 1 : 0); k = 2 * k + (qz ? 1 : 0);
             } else {
                 // binary node: split at the midpoint of its recorded axis (bits 16-17: 0 x, 1 y, 2 z)
                 const bool fy = (a.x >> 16) & 1u, fz = (a.x >> 17) & 1u;
                 const float p = pick3(fy, fz, x, y, z);
-                const bool neg = fz ? nz : fy ? ny : nx;
+                const bool up = fz ? uz : fy ? uy : ux;
                 const float c = __uint_as_float(a.z);   // the split plane, held by the record
-                const int q = (neg ? p <= c : p < c) ? 0 : 1;
+                const int q = ((p > c) | ((p == c) & up)) ? 1 : 0;
                 i = (fy || fz) ? i : 2 * i + q;
                 j = fy ? 2 * j + q : j;
                 k = fz ? 2 * k + q : k;
                 n = c0 + q;
             }
             load(t, n, a, b);
-            if (++level > 64) return;
         }
-        ci = i; cj = j; ck = k;
-        nsub = a.x & 0x7fffu;
-        rho = __uint_as_float(a.y);
-        rope[0] = (int32_t)a.z; rope[1] = (int32_t)a.w;
-        rope[2] = (int32_t)b.x; rope[3] = (int32_t)b.y; rope[4] = (int32_t)b.z; rope[5] = (int32_t)b.w;
-        m = n;
+        take(t, a, b, n, i, j, k);
     }
     __device__ __forceinline__ void descend_from(const View& t, int n, int32_t i, int32_t j, int32_t k)
     {
@@ -634,10 +654,11 @@ struct TreeW32 {
         load(t, n, a, b);
         descend_loaded(t, a, b, n, i, j, k);
     }
-    // Root descent in FP32. At the start of a walk (xa < 0) by "<" as the reference; after a step
-    // through a wall on axis xa, every axis is compared in the travel direction, so a point lying
-    // exactly on a wall goes to the cell the ray is heading into. Returns the leaf's node id (and its
-    // indices) or -1.
+    // Root descent in FP32 over the node words (the top levels from LDS), with every split plane
+    // computed by the same single-rounding formula as the records hold. At the start of a walk (xa < 0)
+    // by "<" as the reference; after a step through a wall on axis xa, every axis is compared in the
+    // travel direction, so a point lying exactly on a wall goes to the cell the ray is heading into.
+    // Returns the leaf's node id (and its indices) or -1.
     __device__ __forceinline__ int descend_root(const View& t, const int32_t* lds, int nlds, int xa, int xdir,
                                                 int32_t& i, int32_t& j, int32_t& k) const
     {
@@ -700,47 +721,50 @@ struct TreeW32 {
     __device__ __forceinline__ void set_direction(double dx, double dy, double dz)
     {
         kx = (float)dx; ky = (float)dy; kz = (float)dz;
-        ix = fabsf(kx) > 1e-15f ? 1.0f / kx : 0.0f;
-        iy = fabsf(ky) > 1e-15f ? 1.0f / ky : 0.0f;
-        iz = fabsf(kz) > 1e-15f ? 1.0f / kz : 0.0f;
+        const bool ox = fabsf(kx) > 1e-15f, oy = fabsf(ky) > 1e-15f, oz = fabsf(kz) > 1e-15f;
+        ix = ox ? 1.0f / kx : 0.0f;
+        iy = oy ? 1.0f / ky : 0.0f;
+        iz = oz ? 1.0f / kz : 0.0f;
+        gx = ox ? 0.0f : FLT_MAX;
+        gy = oy ? 0.0f : FLT_MAX;
+        gz = oz ? 0.0f : FLT_MAX;
     }
-    __device__ __forceinline__ void init(const View& t, const int32_t* lds, int nlds, double rx, double ry, double rz,
+    __device__ __forceinline__ void init(const View& t, const int32_t* lds, int nlds, double rx_, double ry_, double rz_,
                                          double dx, double dy, double dz)
     {
         steps = 0; done = false; m = -1; lm = -1;
         Entry en;
-        if (!en.move_inside(t.ext, t.eps, rx, ry, rz, dx, dy, dz)) { ne = 0; done = true; return; }
+        if (!en.move_inside(t.ext, t.eps, rx_, ry_, rz_, dx, dy, dz)) { ne = 0; done = true; return; }
         ne = 0;
         for (int q = 0; q < en.ne; q++) push((float)(q == 0 ? en.e0 : q == 1 ? en.e1 : en.e2));
-        x = (float)rx; y = (float)ry; z = (float)rz;
+        x = (float)rx_; y = (float)ry_; z = (float)rz_;
         set_direction(dx, dy, dz);
         int32_t i, j, k;
-        int n = descend_root(t, lds, nlds, -1, 0, i, j, k);
+        const int n = descend_root(t, lds, nlds, -1, 0, i, j, k);
         if (n < 0) { ne = 0; done = true; return; }
         descend_from(t, n, i, j, k);
     }
     // The leaf of the last segment returned (-1 before the first one): after a walk that stopped
     // inside a segment, the leaf holding the stopping point.
-    __device__ __forceinline__ void hint(int& n, int32_t& i, int32_t& j, int32_t& k) const
-    {
-        n = lm; i = li; j = lj; k = lk;
-    }
-    // init for a start point strictly inside the grid with a candidate leaf (node hn, indices hi, hj,
-    // hk): the leaf is taken when it contains the point in the half-open sense lo <= p < hi (closed at
-    // the grid's upper walls), which is exactly the leaf the root descent selects; otherwise init.
-    __device__ __forceinline__ void init_hinted(const View& t, const int32_t* lds, int nlds, double rx, double ry,
-                                                double rz, double dx, double dy, double dz, int hn, int32_t hi,
-                                                int32_t hj, int32_t hk)
+    __device__ __forceinline__ int hint() const { return lm; }
+    // init for a start point strictly inside the grid with a candidate leaf hn: the leaf is taken when
+    // it contains the point in the half-open sense lo <= p < hi (closed at the grid's upper walls),
+    // which is exactly the leaf the root descent selects; otherwise init.
+    __device__ __forceinline__ void init_hinted(const View& t, const int32_t* lds, int nlds, double rx_, double ry_,
+                                                double rz_, double dx, double dy, double dz, int hn)
     {
         const double* e = t.ext;
-        const bool inside = rx > e[0] && rx < e[3] && ry > e[1] && ry < e[4] && rz > e[2] && rz < e[5];
-        if (!(inside && hn >= 0 && hn < t.nnodes)) { init(t, lds, nlds, rx, ry, rz, dx, dy, dz); return; }
+        const bool inside = rx_ > e[0] && rx_ < e[3] && ry_ > e[1] && ry_ < e[4] && rz_ > e[2] && rz_ < e[5];
+        if (!(inside && hn >= 0 && hn < t.nnodes)) { init(t, lds, nlds, rx_, ry_, rz_, dx, dy, dz); return; }
         steps = 0; done = false; m = -1; lm = -1; ne = 0;
-        x = (float)rx; y = (float)ry; z = (float)rz;
+        x = (float)rx_; y = (float)ry_; z = (float)rz_;
         set_direction(dx, dy, dz);
         uint4 a, b;
         load(t, hn, a, b);
-        bool ok = (a.x & (1u << 15)) != 0;
+        const uint64_t kp = t.kpack[hn];
+        const int32_t hi = (int32_t)(kp & 0x1fffffu), hj = (int32_t)((kp >> 21) & 0x1fffffu),
+                      hk = (int32_t)((kp >> 42) & 0x1fffffu);
+        bool ok = (a.x & kTreeLeaf) != 0;
         if (ok) {
             const uint32_t ns = a.x & 0x7fffu;
             const float p[3] = {x, y, z};
@@ -754,91 +778,48 @@ struct TreeW32 {
                 ok = ok && p[q] >= lo && (p[q] < up || (top && p[q] <= up));
             }
         }
-        if (ok) { descend_loaded(t, a, b, hn, hi, hj, hk); return; }
+        if (ok) { take(t, a, b, hn, hi, hj, hk); return; }
         int32_t i, j, k;
-        int n = descend_root(t, lds, nlds, -1, 0, i, j, k);
+        const int n = descend_root(t, lds, nlds, -1, 0, i, j, k);
         if (n < 0) { done = true; return; }
         descend_from(t, n, i, j, k);
     }
-    // Chooses the exit wall and advances onto it; returns the wall (0..5) and the segment length, and
-    // the exit axis as two flags (bx: x, by: y, neither: z). The choices are written as selects on
-    // these flags rather than on an axis number, which the compiler would turn into branches or an
-    // indexed private array.
-    __device__ __forceinline__ int exit_wall(const View& t, float& d, bool& bx, bool& by, float& xn, float& yn,
-                                             float& zn)
+    // Chooses the exit wall and moves the position onto it; returns the segment length and the exit
+    // axis as flags (fx: x, fy: y, neither: z). The choices are selects on these flags, never an axis
+    // number, which the compiler would turn into branches or an indexed private array.
+    __device__ __forceinline__ float exit_wall(bool& fx, bool& fy)
     {
-        const float wx = width(t, 0, nsub), wy = width(t, 1, nsub), wz = width(t, 2, nsub);
-        const bool nx = kx < 0.0f, ny = ky < 0.0f, nz = kz < 0.0f;
-        xn = fmaf((float)(nx ? ci : ci + 1), wx, t.extf[0]);
-        yn = fmaf((float)(ny ? cj : cj + 1), wy, t.extf[1]);
-        zn = fmaf((float)(nz ? ck : ck + 1), wz, t.extf[2]);
-        const float dsx = ix != 0.0f ? (xn - x) * ix : FLT_MAX;
-        const float dsy = iy != 0.0f ? (yn - y) * iy : FLT_MAX;
-        const float dsz = iz != 0.0f ? (zn - z) * iz : FLT_MAX;
-        // choose_axis_tree: x on ties with y or z, then y on ties with z
-        bx = dsx <= dsy && dsx <= dsz;
-        by = !bx && dsy <= dsz;
-        d = bx ? dsx : by ? dsy : dsz;
-        const bool bz = !bx && !by;
-        if (ambiguous(dsx, dsy, dsz, d, pick3(by, bz, kx, ky, kz), pick3(by, bz, wx, wy, wz))) {
+        // guarded distances: an axis parallel to the direction gives FLT_MAX
+        const float dsx = fmaf(px - x, ix, gx);
+        const float dsy = fmaf(py - y, iy, gy);
+        const float dsz = fmaf(pz - z, iz, gz);
+        float d = fminf(fminf(dsx, dsy), dsz);
+        const float mid = __builtin_amdgcn_fmed3f(dsx, dsy, dsz);
+        // tree tie rule (TreeDustGrid.cpp:451-466): x on ties with y or z, then y on ties with z
+        fx = dsx == d;
+        fy = !fx && dsy == d;
+        const float tol = 1e-5f * mid;
+        if ((mid < FLT_MAX) & (((mid - d) <= tol) | (d <= tol))) {
             nrecompute++;
             const double X = x, Y = y, Z = z;
-            const double ddx = ix != 0.0f ? ((double)xn - X) / (double)kx : DBL_MAX;
-            const double ddy = iy != 0.0f ? ((double)yn - Y) / (double)ky : DBL_MAX;
-            const double ddz = iz != 0.0f ? ((double)zn - Z) / (double)kz : DBL_MAX;
-            bx = ddx <= ddy && ddx <= ddz;
-            by = !bx && ddy <= ddz;
-            d = (float)(bx ? ddx : by ? ddy : ddz);
+            const double ddx = ix != 0.0f ? ((double)px - X) / (double)kx : DBL_MAX;
+            const double ddy = iy != 0.0f ? ((double)py - Y) / (double)ky : DBL_MAX;
+            const double ddz = iz != 0.0f ? ((double)pz - Z) / (double)kz : DBL_MAX;
+            fx = ddx <= ddy && ddx <= ddz;
+            fy = !fx && ddy <= ddz;
+            d = (float)(fx ? ddx : fy ? ddy : ddz);
         }
         const float dd = d > 0.0f ? d : 0.0f;   // a select: fmaxf would first canonicalize d
-        x = bx ? xn : fmaf(kx, dd, x);
-        y = by ? yn : fmaf(ky, dd, y);
-        z = !bx && !by ? zn : fmaf(kz, dd, z);
-        const bool neg = bx ? nx : by ? ny : nz;
-        return (bx ? 0 : by ? 2 : 4) + (neg ? 0 : 1);
+        const bool fz = !fx && !fy;
+        x = fx ? px : fmaf(kx, dd, x);
+        y = fy ? py : fmaf(ky, dd, y);
+        z = fz ? pz : fmaf(kz, dd, z);
+        return d;
     }
-    // Moves to the leaf beyond the exit wall (axis flags bx, by; direction sign s) from the node r
-    // across it, or by a directed root descent when the position is not on the leaf's face. The
-    // position lies on the exit wall and, having moved away from the entry walls, at or beyond the
-    // leaf's lower walls in the travel direction: it is on the face (and so in r's box, which covers
-    // the whole face) exactly when it has not passed the exit planes xn, yn, zn of the two other axes,
-    // which rounding can do by an ulp near an edge.
-    __device__ __forceinline__ void step_through(const View& t, const int32_t* lds, int nlds, int wall, bool bx,
-                                                 bool by, int r, float xn, float yn, float zn)
-    {
-        uint4 a, b;
-        load(t, r, a, b);
-        const uint32_t nsr = a.x & 0x7fffu;
-        const bool bz = !bx && !by;
-        const int s = (wall & 1) ? 1 : -1;
-        const int dx = (int)(nsub & 31u) - (int)(nsr & 31u);
-        const int dy = (int)((nsub >> 5) & 31u) - (int)((nsr >> 5) & 31u);
-        const int dz = (int)((nsub >> 10) & 31u) - (int)((nsr >> 10) & 31u);
-        const bool onx = bx | (kx < 0.0f ? x >= xn : x <= xn);
-        const bool ony = by | (ky < 0.0f ? y >= yn : y <= yn);
-        const bool onz = bz | (kz < 0.0f ? z >= zn : z <= zn);
-        int32_t i = (ci + (bx ? s : 0)) >> dx;
-        int32_t j = (cj + (by ? s : 0)) >> dy;
-        int32_t k = (ck + (bz ? s : 0)) >> dz;
-        if (((dx | dy | dz) >= 0) & onx & ony & onz) descend_loaded(t, a, b, r, i, j, k);
-        else {
-            int n = descend_root(t, lds, nlds, wall >> 1, s, i, j, k);
-            if (n < 0) done = true;
-            else descend_from(t, n, i, j, k);
-        }
-    }
-    // The rope across the exit wall, selected by the axis flags and the wall's side.
-    __device__ __forceinline__ int rope_of(int wall, bool bx, bool by) const
-    {
-        const bool hi = wall & 1;
-        int32_t r0 = rope[0], r1 = rope[1], r2 = rope[2], r3 = rope[3], r4 = rope[4], r5 = rope[5];
-        asm("" : "+v"(r0), "+v"(r1), "+v"(r2), "+v"(r3), "+v"(r4), "+v"(r5));
-        return bx ? (hi ? r1 : r0) : by ? (hi ? r3 : r2) : (hi ? r5 : r4);
-    }
-    // One wall step: an entry segment, or the segment from the position to the next wall, whose
-    // length is zero (or below) when the position lies on an edge or corner of the cell; false at the
-    // end of the path. Transport loops take one step per iteration and skip segments with ds <= 0,
-    // so that no loop nests inside theirs.
+    // One wall step: an entry segment, or the segment from the position to the next wall, whose length
+    // is zero (or below) when the position lies on an edge or corner of the cell; false at the end of
+    // the path. Transport loops take one step per iteration and skip segments with ds <= 0, so that no
+    // loop nests inside theirs.
     static constexpr bool kStep = true;
     __device__ __forceinline__ bool step(const View& t, const int32_t* lds, int nlds, int& mo, float& ds, float& rh)
     {
@@ -847,23 +828,44 @@ struct TreeW32 {
             return true;
         }
         if (done) return false;
-        float d;
         const int segm = m;
         const float segrho = rho;
-        const int32_t si = ci, sj = cj, sk = ck;
-        bool bx, by;
-        float xn, yn, zn;
-        const int wall = exit_wall(t, d, bx, by, xn, yn, zn);
-        const int r = rope_of(wall, bx, by);
+        bool fx, fy;
+        const float d = exit_wall(fx, fy);
+        const bool fz = !fx && !fy;
+        int32_t r0 = rx, r1 = ry, r2 = rz;
+        asm("" : "+v"(r0), "+v"(r1), "+v"(r2));
+        const int r = fx ? r0 : fy ? r1 : r2;
         if (r < 0) done = true;   // a wall of the grid: the path leaves the grid
         else {
-            step_through(t, lds, nlds, wall, bx, by, r, xn, yn, zn);
+            uint4 a, b;
+            load(t, r, a, b);
+            // the position lies on the exit wall and, having moved away from the entry walls, at or
+            // beyond the leaf's lower walls in the travel direction: it is on the leaf's face (and so in
+            // the rope node's box, which covers the whole face) exactly when it has not passed the exit
+            // planes of the two other axes, which rounding can do by an ulp near an edge. Not passed:
+            // (p - x) k >= 0 (zero on the exit axis, where the position lies on the plane)
+            const float on = fminf(fminf((px - x) * kx, (py - y) * ky), (pz - z) * kz);
+            // indices across the exit wall at the rope node's subdivision (a rope is never finer than
+            // its leaf along any axis, so the packed per-axis differences do not borrow)
+            const uint32_t dn = nsub - (a.x & 0x7fffu);
+            const int32_t i = (ci + (fx ? (kx < 0.0f ? -1 : 1) : 0)) >> (dn & 31u);
+            const int32_t j = (cj + (fy ? (ky < 0.0f ? -1 : 1) : 0)) >> ((dn >> 5) & 31u);
+            const int32_t k = (ck + (fz ? (kz < 0.0f ? -1 : 1) : 0)) >> ((dn >> 10) & 31u);
+            if (on >= 0.0f) descend_loaded(t, a, b, r, i, j, k);
+            else {
+                const int xa = fx ? 0 : fy ? 1 : 2;
+                const float ka = fx ? kx : fy ? ky : kz;
+                int32_t i2, j2, k2;
+                const int n = descend_root(t, lds, nlds, xa, ka < 0.0f ? -1 : 1, i2, j2, k2);
+                if (n < 0) done = true;
+                else descend_from(t, n, i2, j2, k2);
+            }
             if (!done && m == segm) { nstuck++; done = true; }
         }
         if (++steps > SG_MAX_STEPS) done = true;
         mo = segm; ds = d; rh = segrho;
-        const bool seg = d > 0;
-        lm = seg ? segm : lm; li = seg ? si : li; lj = seg ? sj : lj; lk = seg ? sk : lk;
+        lm = d > 0.0f ? segm : lm;
         return true;
     }
     // The next segment of positive length.
@@ -885,15 +887,14 @@ struct TreeW32TopDown : TreeW32 {
             return true;
         }
         if (done) return false;
-        float d;
         const int segm = m;
         const float segrho = rho;
-        const int32_t si = ci, sj = cj, sk = ck;
-        bool bx, by;
-        float xn, yn, zn;
-        const int wall = exit_wall(t, d, bx, by, xn, yn, zn);
+        bool fx, fy;
+        const float d = exit_wall(fx, fy);
+        const int xa = fx ? 0 : fy ? 1 : 2;
+        const float ka = fx ? kx : fy ? ky : kz;
         int32_t i, j, k;
-        const int n = descend_root(t, lds, nlds, wall >> 1, (wall & 1) ? 1 : -1, i, j, k);
+        const int n = descend_root(t, lds, nlds, xa, ka < 0.0f ? -1 : 1, i, j, k);
         if (n < 0) done = true;
         else {
             descend_from(t, n, i, j, k);
@@ -901,8 +902,7 @@ struct TreeW32TopDown : TreeW32 {
         }
         if (++steps > SG_MAX_STEPS) done = true;
         mo = segm; ds = d; rh = segrho;
-        const bool seg = d > 0;
-        lm = seg ? segm : lm; li = seg ? si : li; lj = seg ? sj : lj; lk = seg ? sk : lk;
+        lm = d > 0.0f ? segm : lm;
         return true;
     }
     __device__ __forceinline__ bool next(const View& t, const int32_t* lds, int nlds, int& mo, float& ds,
@@ -1014,6 +1014,7 @@ template <typename R> struct VoroW {
     static constexpr int kParkWords = 25;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     static constexpr bool kHint = true;
+    __device__ __forceinline__ int entries() const { return en.ne; }
     static constexpr bool kStep = false;   // next() only
     template <class P> __device__ __forceinline__ void park(P& p)
     {
@@ -1025,18 +1026,15 @@ template <typename R> struct VoroW {
     __device__ __forceinline__ void init(const View& g, const int32_t* lds, int nlds, double rx, double ry, double rz,
                                          double dx, double dy, double dz)
     {
-        init_hinted(g, lds, nlds, rx, ry, rz, dx, dy, dz, -1, 0, 0, 0);
+        init_hinted(g, lds, nlds, rx, ry, rz, dx, dy, dz, -1);
     }
     // The cell of the last segment returned (-1 before the first one): after a walk that stopped
     // inside a segment, the cell holding the stopping point.
-    __device__ __forceinline__ void hint(int& n, int32_t& i, int32_t& j, int32_t& k) const
-    {
-        n = lm; i = 0; j = 0; k = 0;
-    }
+    __device__ __forceinline__ int hint() const { return lm; }
     // init with a candidate cell hn (or -1): the greedy location starts there; the cell found is the
     // nearest site whatever the start
     __device__ __forceinline__ void init_hinted(const View& g, const int32_t*, int, double rx, double ry, double rz,
-                                                double dx, double dy, double dz, int hn, int32_t, int32_t, int32_t)
+                                                double dx, double dy, double dz, int hn)
     {
         kx = dx; ky = dy; kz = dz;
         steps = 0; done = false; lm = -1;

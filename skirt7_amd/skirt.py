@@ -586,6 +586,12 @@ def set_mixed_topdown(descend):
     check(lib().sg_set_mixed_topdown(1 if descend else 0))
 
 
+def set_walk_order(sorted_=False):
+    """Event/walk pipeline: True takes each pass's walks sorted by a coherence key (peel-offs by instrument
+    and pixel, other walks by start point), False (default) as the event kernel appended them."""
+    check(lib().sg_set_walk_order(1 if sorted_ else 0))
+
+
 def set_tuning(block=0, waves_per_cu=0, compact_lanes=0):
     """Launch parameters of sg_sim_run: 0 keeps the current value, a negative value restores the default."""
     check(lib().sg_set_tuning(block, waves_per_cu, compact_lanes))
