@@ -135,6 +135,16 @@ int sg_sim_destroy(sg_sim* s);
  *   launches of each launch's own duration (HIP events on its stream; launches on two streams overlap) */
 int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, int precision,
                uint64_t* counters);
+/* sg_sim_run that allocates no device memory. No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same packet histories and tallies as sg_sim_run (event/walk kernels on one slot group of up to
+ * 2^15 in-flight packets, one stream), with all work memory in a static 64 MiB device array that each
+ * GPU the library runs on holds until the process ends, set up by sg_init (call it first). At most
+ * 2^19 packets per call (the array holds their launch states): larger ranges are refused with an
+ * error, split them over several calls. Calls that overlap on one GPU race (they share the array);
+ * each call clears what it needs. counters as sg_sim_run, without the timings [10], [12]-[14]. */
+int sg_sim_run_static(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, int precision,
+                      uint64_t* counters);
 /* Batched population (one persistent launch over many members, each its own simulation on a
  * grid of the same kind): member i runs packets [0, counts[i]) at wavelength ells[i]. With the
  * event/walk pipeline selected, members of at least sg_set_batch_alone packets run one at a time

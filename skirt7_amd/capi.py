@@ -83,6 +83,8 @@ _SIGS = {
     "sg_sim_destroy": (ctypes.c_int, [_VP]),
     "sg_sim_run": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                   ctypes.c_int, _U64]),
+    "sg_sim_run_static": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                         ctypes.c_int, _U64]),
     "sg_sim_run_batch": (ctypes.c_int, [ctypes.c_int, _VPP, _I32, _U64, ctypes.c_int, _U64]),
     "sg_gaussian_kernel": (ctypes.c_int, [ctypes.c_double, ctypes.c_int, _D]),
     "sg_fit_frames": (ctypes.c_int, [ctypes.c_int, _VPP, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int,

@@ -151,6 +151,9 @@ __global__ void k_whichcell_voro(const VoroView g, int64_t n, const double* r, i
 
 using namespace sg;
 
+static void split_resources();
+static void static_scratch_setup(int device);
+
 // ========================================================================= C-ABI
 
 #define SG_TRY try {
@@ -177,6 +180,10 @@ int sg_init(int device)
                std::string("this library is built for gfx950, found ") + p.gcnArchName);
     g_cus = p.multiProcessorCount;
     g_device = device;
+    // the transport streams, the pinned progress buffer and this device's static work array are set up
+    // here, so that sg_sim_run_static allocates nothing
+    split_resources();
+    static_scratch_setup(device);
     SG_CATCH
 }
 
@@ -876,6 +883,7 @@ int sg_sim_create(sg_dust* dust, sg_stars* stars, sg_instruments* instr, const d
     s->counters.alloc(SG_NCOUNTERS * sizeof(unsigned long long));
     s->next.alloc(sizeof(unsigned long long));
     s->err.alloc(sizeof(int));
+    s->scene.alloc(sizeof(SceneDev));
     *out = s;
     SG_CATCH
 }
@@ -1030,7 +1038,6 @@ static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64
     a.next_packet = s->next.as<unsigned long long>();
     a.error = s->err.as<int>();
     a.compact = g_tune_compact > 0 ? g_tune_compact : (g_pipeline == 1 ? 24 : 32);
-    if (!s->scene.p) s->scene.alloc(sizeof(SceneDev));
     SG_HIP(hipMemcpy(s->scene.p, &sd, sizeof sd, hipMemcpyHostToDevice));
     a.scene = s->scene.as<SceneDev>();
 }
@@ -1362,6 +1369,167 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
         counters[14] = (uint64_t)(lt.kernel_ms() * 1000.0);
     }
     SG_REQUIRE(!(err & 2), "walk order: a request's sorted place lies beyond the pass's requests");
+    SG_REQUIRE(!(err & 4), "walk queue: a request names no instrument pixel or packet slot");
+    SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
+    SG_CATCH
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- transport without device allocation
+//
+// sg_sim_run_static runs the event/walk pipeline on one slot group and one stream with all of its work
+// memory (packet slots, walk queue, progress counters, the launch states of its packets) in `scratch`,
+// a file-scope device array of kStaticScratchBytes that every device the library runs on holds until
+// the process ends. Host code never uses the array's name as an address: sg_init takes this device's
+// address of it with hipGetSymbolAddress and keeps it in g_scratch. Counters and flags are cleared by
+// a fill kernel, not by hipMemsetAsync. Calls that overlap on one device race, since they share it.
+constexpr size_t kStaticScratchBytes = (size_t)64 << 20;
+constexpr uint64_t kStaticMaxPackets = (uint64_t)1 << 19;
+constexpr int64_t kStaticSlots = (int64_t)1 << 15;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+static void* g_scratch[64] = {};
+
+static void static_scratch_setup(int device)
+{
+    SG_REQUIRE(device < 64, "device index beyond the static work array table");
+    if (g_scratch[device]) return;
+    void* p = nullptr;
+    SG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SG_REQUIRE(p != nullptr, "no device address for the static work array");
+    g_scratch[device] = p;
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+static void fill_u32(void* p, int64_t n, hipStream_t st)
+{
+    const int64_t b = std::min<int64_t>((n + 255) / 256, (int64_t)g_cus * 4);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::max<int64_t>(b, 1)), dim3(256), 0, st, static_cast<uint32_t*>(p), n,
+                       0u);
+    SG_HIP(hipGetLastError());
+}
+
+static size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+template <class W, bool ABS>
+static void run_static(sg_sim* s, const typename W::View& v, const McArgs& a0, int nlds, int& blocks)
+{
+    using R = typename W::Real;
+    hipStream_t st = g_split_stream[0];
+    const int64_t P = (int64_t)std::max<uint64_t>(1, std::min<uint64_t>(a0.count, (uint64_t)kStaticSlots));
+    const int64_t cap = P * (a0.ni + 1);
+    // layout of the static array: slots, walk requests, progress counters, launch states, draw counts
+    const size_t o_q = align256(slot_bytes<R>(P));
+    const size_t o_pc = o_q + align256((size_t)cap * sizeof(WalkReq<R>));
+    const size_t o_em = o_pc + 256;
+    const size_t o_ed = o_em + align256(7 * a0.count * sizeof(R));
+    const size_t need = o_ed + align256(a0.count * sizeof(uint32_t));
+    SG_REQUIRE(need <= kStaticScratchBytes, "sg_sim_run_static: the run needs " + std::to_string(need) +
+                                                " bytes of work memory, more than the static array holds");
+    unsigned char* base = static_cast<unsigned char*>(g_scratch[g_device]);
+    SG_REQUIRE(base != nullptr, "sg_sim_run_static: the static work array of this device is not set up");
+    Slots<R> sl{reinterpret_cast<R*>(base), reinterpret_cast<uint32_t*>(base + (size_t)P * SR_NR * sizeof(R)), P};
+    unsigned long long* pc = reinterpret_cast<unsigned long long*>(base + o_pc);   // [1] queue tail, [3] claims
+    WalkQueue<R> q{};
+    q.req = reinterpret_cast<WalkReq<R>*>(base + o_q);
+    q.cap = cap;
+    q.tail = pc + 1;
+    McArgs a = a0;
+    a.emitted = base + o_em;
+    a.emit_draw = reinterpret_cast<uint32_t*>(base + o_ed);
+    a.next_packet = s->next.as<unsigned long long>();
+    // cleared by kernels: slot states (PS_FRESH) and counts, progress, the packet claims, counters, error flag
+    fill_u32(sl.u + (size_t)SU_STATE * P, P * (SU_NU - SU_STATE), st);
+    fill_u32(pc, 8, st);
+    fill_u32(s->next.p, 2, st);
+    fill_u32(s->counters.p, 2 * SG_NCOUNTERS, st);
+    fill_u32(s->err.p, 1, st);
+    const int bs = g_tune_block;
+    const size_t shm = (size_t)nlds * sizeof(int32_t);
+    int per = 0;
+    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walks<W, ABS>, bs, shm));
+    blocks = std::max(1, per) * g_cus;
+    int ebs = 256;
+    while (ebs > 64 && staged_bytes<R>(ebs * (a.ni + 1)) > 64 * 1024) ebs /= 2;
+    const size_t eshm = staged_bytes<R>(ebs * (a.ni + 1));
+    if (a.count > 0) {
+        const int emb = (int)std::min<uint64_t>((a.count + 255) / 256, (uint64_t)g_cus * 16);
+        hipLaunchKernelGGL(k_emit<R>, dim3(emb), dim3(256), 0, st, a, static_cast<R*>(const_cast<void*>(a.emitted)),
+                           const_cast<uint32_t*>(a.emit_draw));
+        SG_HIP(hipGetLastError());
+    }
+    for (;;) {
+        for (int p = 0; p < kSplitCheck; p++) {
+            fill_u32(pc + 1, 2, st);
+            hipLaunchKernelGGL(k_events<R>, dim3((P + ebs - 1) / ebs), dim3(ebs), eshm, st, a, sl, q);
+            SG_HIP(hipGetLastError());
+            fill_u32(pc + 3, 2, st);
+            hipLaunchKernelGGL((k_walks<W, ABS>), dim3(blocks), dim3(bs), shm, st, v, a, q, sl, pc + 3, nlds);
+            SG_HIP(hipGetLastError());
+        }
+        SG_HIP(hipMemcpyAsync(g_split_host, a.next_packet, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        SG_HIP(hipMemcpyAsync(g_split_host + 1, pc + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        SG_HIP(hipStreamSynchronize(st));
+        SG_REQUIRE(g_split_host[1] <= (unsigned long long)cap, "walk queue overflow");
+        if (g_split_host[0] >= a.count && g_split_host[1] == 0) break;
+    }
+    hipLaunchKernelGGL(k_slot_counts, dim3(std::min<int64_t>((P + 255) / 256, 1024)), dim3(256), 0, st, sl.u, P, a.counters);
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipStreamSynchronize(st));
+}
+
+template <bool ABS>
+static void dispatch_static(sg_sim* s, const McArgs& a, int precision, int& blocks)
+{
+    const bool mixed = precision == SG_MIXED;
+    sg_grid* g = s->dust ? s->dust->grid : nullptr;
+    if (!g) {
+        CartView v{};
+        if (mixed) run_static<CartW32, false>(s, v, a, 0, blocks);
+        else run_static<CartW64, false>(s, v, a, 0, blocks);
+    } else if (g->kind == GK_CART) {
+        if (mixed) run_static<CartW32, ABS>(s, g->cart, a, 0, blocks);
+        else run_static<CartW64, ABS>(s, g->cart, a, 0, blocks);
+    } else if (g->kind == GK_TREE) {
+        if (!mixed) run_static<TreeW64, ABS>(s, g->tree, a, SG_LDS_NODES, blocks);
+        else if (rope_walk(g)) run_static<TreeW32, ABS>(s, g->tree, a, SG_LDS_NODES, blocks);
+        else run_static<TreeW32TopDown, ABS>(s, g->tree, a, SG_LDS_NODES, blocks);
+    } else {
+        if (mixed) run_static<VoroW<float>, ABS>(s, g->voro, a, 0, blocks);
+        else run_static<VoroW<double>, ABS>(s, g->voro, a, 0, blocks);
+    }
+}
+
+extern "C" {
+
+int sg_sim_run_static(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, int precision,
+                      uint64_t* counters)
+{
+    SG_TRY
+    SG_REQUIRE(g_device >= 0, "sg_sim_run_static: call sg_init first (the static work array is set up there)");
+    SG_REQUIRE(npp > 0, "npp must be positive");
+    SG_REQUIRE(precision == SG_MIXED || precision == SG_FP64, "precision must be SG_FP64 or SG_MIXED");
+    SG_REQUIRE(count <= kStaticMaxPackets, "sg_sim_run_static: at most " + std::to_string(kStaticMaxPackets) +
+                                               " packets per call (the static work array holds their launch states); "
+                                               "split the packet range over several calls");
+    McArgs a;
+    fill_args(s, ell, first, count, npp, a);
+    int blocks = 0;
+    if (a.store_abs) dispatch_static<true>(s, a, precision, blocks);
+    else dispatch_static<false>(s, a, precision, blocks);
+    unsigned long long c[SG_NCOUNTERS];
+    SG_HIP(hipMemcpy(c, s->counters.p, sizeof c, hipMemcpyDeviceToHost));
+    int err = 0;
+    SG_HIP(hipMemcpy(&err, s->err.p, sizeof err, hipMemcpyDeviceToHost));
+    if (counters) {
+        for (int i = 0; i < SG_NCOUNTERS; i++) counters[i] = c[i];
+        counters[6] = npp;
+        counters[11] = blocks;
+    }
     SG_REQUIRE(!(err & 4), "walk queue: a request names no instrument pixel or packet slot");
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH

@@ -497,6 +497,15 @@ class OligoMonteCarloSimulation:
         self.counters = dict(zip(capi.COUNTER_NAMES, (int(x) for x in c)))
         return self.counters
 
+    def run_packets_static(self, ell, first, count, npp, precision=SG_FP64):
+        """run_packets without any device allocation (work memory in the library's static device array);
+        at most 2^19 packets per call."""
+        c = np.zeros(capi.SG_NCOUNTERS, np.uint64)
+        check(lib().sg_sim_run_static(self._h, ell, int(first), int(count), int(npp), precision,
+                                      c.ctypes.data_as(capi._U64)))
+        self.counters = dict(zip(capi.COUNTER_NAMES, (int(x) for x in c)))
+        return self.counters
+
     def runstellaremission(self, precision=SG_FP64):
         """The stellar emission phase (MonteCarloSimulation.cpp:241-255): all wavelengths, Npp packets each."""
         npp = self.setChunkParams(1)
