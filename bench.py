@@ -59,9 +59,17 @@ def parse():
     ap.add_argument("--packets", type=float, default=None,
                     help="packets per step (default 2e7 for spiral_octree, 1e6 for the others)")
     ap.add_argument("--tau", type=float, default=2.0)
+    ap.add_argument("--tau-variants", type=float, nargs="*", default=[0.5, 5.0],
+                    help="spiral_octree: the other face-on optical depths to time (two steps each, rate only)")
     ap.add_argument("--maxlevel", type=int, default=8)
     ap.add_argument("--pixels", type=int, default=512)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="target duration of each multi-threaded CPU baseline sample")
+    ap.add_argument("--ref-mult", type=int, default=10,
+                    help="the equal-image-error reference runs this many times the timed steps' packets in FP64")
+    ap.add_argument("--slot-groups", type=int, default=2, choices=(1, 2),
+                    help="slot groups of the timed steps (2: alternating on two streams; 1: one stream, as the "
+                         "roofline step)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--tau-cut", type=float, default=0.0)
@@ -113,42 +121,63 @@ def run_steps(sim, first, nsteps, per, npp, precision, ell=0):
     return tot, (tot["transport_us"] * 1e-6 if tot else 0.0)
 
 
-def cpu_baseline(w, seconds):
-    from tests import oracle_lib as O
-    sc = O.scene_from_workload(w)
-    threads = min(os.cpu_count() or 1, 16)
-    mc = np.array(w["sim"].mc)
-    n = 2000.0
+def _cpu_model():
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return cpu
+
+
+def _time_reference(sc, mc0, threads, seconds, start=2000.0):
+    """Runs the oracle's reference mode (MT19937 streams, the reference's chunk rule) on a sample of the
+    workload sized to take about `seconds`; returns (packets, crossings, seconds)."""
+    mc = np.array(mc0)
+    n = start
     while True:
         mc[0] = n
         t0 = time.perf_counter()
         _, c = sc.run_reference(mc, threads)
         dt = time.perf_counter() - t0
         if dt * 3 > seconds or n >= 1e7:
-            break
+            return float(c[0]), float(c[1] + c[2]), dt
         n = min(1e7, n * max(2.0, 0.9 * seconds / max(dt, 1e-3)))
-    cpu = platform.processor() or platform.machine()
+
+
+def cpu_baseline(w, seconds):
+    """The CPU restatement of the reference loop on this host: the -O2 build (the reference's qmake release
+    level) on the host threads this process may use, capped at 16 (the CPU share of one GPU on these
+    machines), the -O3 -march=x86-64-v3 build on as many, and the -O2 build on one thread."""
+    from tests import oracle_lib as O
+    nproc = os.cpu_count() or 1
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    # the same restatement on one thread, over a sample a twentieth the size
-    mc1 = mc.copy()
-    mc1[0] = max(1000.0, mc[0] / 20)
-    t1 = time.perf_counter()
-    _, c1 = sc.run_reference(mc1, 1)
-    dt1 = time.perf_counter() - t1
-    return {"value": float(c[0]) / dt, "unit": "packets/s", "cores": threads, "kind": "port",
-            "sample": "%d packets of the same workload, MT19937 streams, %d threads on %s, %.1f s" % (
-                int(c[0]), threads, cpu, dt),
-            "crossings_per_s": float(c[1] + c[2]) / dt,
-            "single_thread": {"value": float(c1[0]) / dt1, "unit": "packets/s",
-                              "sample": "%d packets, 1 thread, %.1f s" % (int(c1[0]), dt1)},
-            "compiler": "g++ -O2 -ffp-contract=off (the reference's qmake release level)"}
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = nproc
+    threads = max(1, min(avail, 16))
+    cpu = _cpu_model()
+    mc = np.array(w["sim"].mc)
+    sc = O.scene_from_workload(w)
+    pk, cr, dt = _time_reference(sc, mc, threads, seconds)
+    sc3 = O.scene_from_workload(w, "v3")
+    pk3, cr3, dt3 = _time_reference(sc3, mc, threads, seconds)
+    pk1, cr1, dt1 = _time_reference(sc, mc, 1, seconds / 4, start=500.0)
+    return {"value": pk / dt, "unit": "packets/s", "cores": threads, "kind": "port",
+            "sample": "%d packets of the same workload, MT19937 streams and the reference's chunk rule, %d threads "
+                      "on %s, %.1f s" % (int(pk), threads, cpu, dt),
+            "crossings_per_s": cr / dt, "cpu_model": cpu, "nproc": nproc, "affinity_cpus": avail,
+            "compiler": "g++ -O2 -ffp-contract=off (the reference's qmake release level)",
+            "o3_x86_64_v3": {"value": pk3 / dt3, "unit": "packets/s", "cores": threads,
+                             "crossings_per_s": cr3 / dt3,
+                             "sample": "%d packets, %d threads, %.1f s" % (int(pk3), threads, dt3),
+                             "compiler": "g++ -O3 -march=x86-64-v3 (AVX2, FMA; -march=native of the build host is "
+                                         "not portable to this machine)"},
+            "single_thread": {"value": pk1 / dt1, "unit": "packets/s", "crossings_per_s": cr1 / dt1,
+                              "sample": "%d packets, 1 thread, %.1f s" % (int(pk1), dt1)}}
 
 
 def population(a):
@@ -215,6 +244,7 @@ def main():
         copy_calibration(a.copy_calibration)
     S.set_pipeline(a.pipeline)
     S.set_walk_order(bool(a.walk_order))
+    S.set_slot_groups(a.slot_groups)
     if a.packets is None:
         a.packets = 2e7 if a.workload == "spiral_octree" else 1e6
     per = int(a.packets)
@@ -277,39 +307,83 @@ def main():
                           desc, ncell, per, len(w["instr"].instruments), w["instr"].instruments[0].Nx),
                       "packets_per_step": per, "cells": ncell, "instruments": len(w["instr"].instruments),
                       "pixels": w["instr"].instruments[0].Nx, "tuning": tuning}}
-    # algorithmic bytes of the walk kernel per launch over its average launch duration (HIP events on the
-    # launch's own stream, summed by the library); the walk passes of the two slot groups overlap, so the
-    # whole transport's rate over its device span is reported beside it
-    bytes_launch = cross * BYTES_PER_CROSSING[gkind]["mixed"] + cm["peeloffs"] * BYTES_PER_PEELOFF
-    if w.get("store_abs"):
-        bytes_launch += cm["own_crossings"] * BYTES_PER_ABSORPTION
+    def algorithmic_bytes(c):
+        """Algorithmic bytes of a run's walks: crossings x bytes per crossing, peel-off tallies, absorption."""
+        b = (c["own_crossings"] + c["peel_crossings"] + c["rewalk_crossings"]) * BYTES_PER_CROSSING[gkind]["mixed"]
+        b += c["peeloffs"] * BYTES_PER_PEELOFF
+        if w.get("store_abs"):
+            b += c["own_crossings"] * BYTES_PER_ABSORPTION
+        return b
+
+    # Three figures for the walk kernel. (1) frac: its algorithmic bytes per launch over its own launch
+    # duration, measured on a step run with one slot group on one stream, so that each launch has the device to
+    # itself (HIP events on that stream). (2) overlapped: the same ratio over the timed steps, whose two slot
+    # groups' passes share the device, so a launch's duration includes time shared with the other stream's
+    # kernels. (3) device: the timed steps' algorithmic bytes over their wall time.
+    bytes_steps = algorithmic_bytes(cm)
     launches = max(1, cm["transport_launches"])
     kwalk = cm["walk_kernel_us"] * 1e-6
-    achieved = bytes_launch / kwalk
-    traffic = None
+    S.set_slot_groups(1)
+    w["instr"].clear()
+    c1 = sim.run_packets(ell, a.warmup * per, per, npp, SG_MIXED)
+    S.set_slot_groups(a.slot_groups)
+    w["instr"].clear()
+    b1 = algorithmic_bytes(c1)
+    l1 = max(1, c1["transport_launches"])
+    k1 = c1["walk_kernel_us"] * 1e-6
+    achieved = b1 / k1
+    cross1 = c1["own_crossings"] + c1["peel_crossings"] + c1["rewalk_crossings"]
+    log("walk kernel alone (one slot group, one stream): %d launches, %.3f ms each, %.3e B each: %.0f GB/s" % (
+        l1, 1e3 * k1 / l1, b1 / l1, achieved / 1e9))
+    traffic, tsrc = None, None
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if a.pipeline == tr["pipeline"] and a.workload == "spiral_octree":
-            # measured HBM bytes per crossing times this run's crossings per launch
-            traffic = (tr["fetch_bytes_per_crossing"] + tr["write_bytes_per_crossing"]) * cross / launches
+        if a.pipeline == tr["pipeline"] and a.workload == tr["workload_name"]:
+            # HBM bytes per crossing measured by rocprofv3 on this workload, times this run's crossings per launch
+            traffic = (tr["fetch_bytes_per_crossing"] + tr["write_bytes_per_crossing"]) * cross1 / l1
+            tsrc = "rocprofv3 FETCH_SIZE + WRITE_SIZE per crossing of k_walks on this workload (%s) x this run's " \
+                   "crossings per launch" % os.path.relpath(TRAFFIC_FILE, ROOT)
     except (OSError, KeyError, ValueError):
         traffic = None
     out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK, "traffic": traffic,
-                       "algorithmic_bytes_per_launch": bytes_launch / launches,
+                       "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": tsrc,
                        "kernel": "k_walks" if a.pipeline == 1 else "k_mc",
-                       "kernel_ms_per_launch": 1e3 * kwalk / launches,
-                       "achieved_over_transport_span": bytes_launch / kmix / 1e9}
+                       "measured_on": "one step of %d packets with one slot group on one stream (each launch alone "
+                                      "on the device)" % per,
+                       "algorithmic_bytes_per_launch": b1 / l1, "kernel_ms_per_launch": 1e3 * k1 / l1,
+                       "launches": l1,
+                       "overlapped": {"frac": bytes_steps / kwalk / HBM_PEAK, "kernel_ms_per_launch": 1e3 * kwalk / launches,
+                                      "note": "timed steps, two slot groups on two streams: a launch's duration "
+                                              "includes time shared with the other stream's kernels"},
+                       "device": {"frac": bytes_steps / wall / HBM_PEAK, "achieved": bytes_steps / wall / 1e9,
+                                  "note": "algorithmic bytes of the timed steps over their wall time"}}
     out["crossings_per_s"] = cross / kmix
     out["transport_span_ms_per_launch"] = 1e3 * kmix / launches
     out["counters"] = cm
 
+    if a.workload == "spiral_octree" and a.tau_variants:
+        # the configuration's other face-on optical depths: same grid refinement, the density scaled; the
+        # launch parameters of the headline run, one warmup and two timed steps each
+        out["tau_variants"] = {}
+        for tv in a.tau_variants:
+            wt = workloads.spiral_octree(tau=tv, maxlevel=a.maxlevel, pixels=a.pixels, packages=3 * per,
+                                         tau_cut=a.tau_cut)
+            run_steps(wt["sim"], 0, 1, per, 3 * per, SG_MIXED, ell)
+            t0 = time.perf_counter()
+            ct, _ = run_steps(wt["sim"], per, 2, per, 3 * per, SG_MIXED, ell)
+            dt = time.perf_counter() - t0
+            crt = ct["own_crossings"] + ct["peel_crossings"] + ct["rewalk_crossings"]
+            out["tau_variants"]["%g" % tv] = {"packets_per_s": ct["packets"] / dt, "crossings_per_s": crt / dt,
+                                              "device_frac": algorithmic_bytes(ct) / dt / HBM_PEAK,
+                                              "packets": ct["packets"], "scatterings": ct["scatterings"]}
+            log("tau %g: %.3e packets/s, %.3e crossings/s" % (tv, ct["packets"] / dt, crt / dt))
+            del wt
+
     if not a.no_fp64:
-        # FP64 baseline: the first timed step's packets with each pipeline; the faster one counts
+        # FP64 baseline rate: one timed step's packets with each pipeline (autotuned); the faster one counts
         first = a.warmup * per
         fp64 = {}
-        img64 = None
         for p in (1, 0):
             S.set_pipeline(p)
             if not a.no_autotune:
@@ -319,50 +393,65 @@ def main():
                 log("autotune (fp64, pipeline %d): %s" % (p, tuning["fp64_pipeline%d" % p]))
             w["instr"].clear()
             t0 = time.perf_counter()
-            c64 = sim.run_packets(ell, first, per, per, SG_FP64)
+            c64 = sim.run_packets(ell, first, per, npp, SG_FP64)
             dt = time.perf_counter() - t0
             fp64[p] = (c64["packets"] / dt, c64)
             log("  fp64 step (pipeline %d): %d packets, %.3f s" % (p, c64["packets"], dt))
-            if p == 1:
-                img64 = frames(w, ell)
-        S.set_pipeline(a.pipeline)
         best = max(fp64, key=lambda p: fp64[p][0])
         fp64_rate, c64 = fp64[best]
-        # mixed image of the same packets (identical streams)
-        w["instr"].clear()
-        sim.run_packets(ell, first, per, per, SG_MIXED)
-        img32 = frames(w, ell)
-        errs = [float(np.sqrt(np.mean((x - y) ** 2)) / np.sqrt(np.mean(x ** 2))) for x, y in zip(img64, img32)]
         out["fp64"] = {"packets_per_s": fp64_rate, "pipeline": best,
                        "packets_per_s_by_pipeline": {str(p): fp64[p][0] for p in fp64},
                        "crossings_per_s": (c64["own_crossings"] + c64["peel_crossings"] + c64["rewalk_crossings"])
                        / (c64["transport_us"] * 1e-6),
-                       "speedup_mixed_over_fp64": out["value"] / fp64_rate,
-                       "image_rms_error_mixed_vs_fp64": errs}
-        log("fp64: %.3e packets/s (pipeline %d; %s); mixed/fp64 = %.2fx; image rms error mixed vs fp64 %s" % (
-            fp64_rate, best, {p: "%.3e" % fp64[p][0] for p in fp64}, out["value"] / fp64_rate,
-            ["%.2e" % e for e in errs]))
+                       "speedup_mixed_over_fp64": out["value"] / fp64_rate}
+        log("fp64: %.3e packets/s (pipeline %d; %s); mixed/fp64 = %.2fx" % (
+            fp64_rate, best, {p: "%.3e" % fp64[p][0] for p in fp64}, out["value"] / fp64_rate))
         if not a.no_equal_error:
-            # FP64 reference with ten times the packets (fresh packets), normalised to its own count
+            # Equal image error. The mixed image of the timed steps (their steps x per packets) and the FP64
+            # image of the same packets (identical random streams), both against an FP64 reference run of
+            # ref_mult times as many fresh packets; every image normalised to a full run.
+            S.set_pipeline(best)
+            w["instr"].clear()
+            t0 = time.perf_counter()
+            for s_ in range(a.steps):
+                sim.run_packets(ell, first + s_ * per, per, npp, SG_FP64)
+            log("  fp64 image of the timed steps' packets: %.1f s" % (time.perf_counter() - t0))
+            n_img = a.steps * per
+            img64 = [x * (npp / n_img) for x in frames(w, ell)]
+            img32 = [x * (npp / n_img) for x in img_mixed]
+            nref = a.ref_mult * n_img
             ref0 = (a.warmup + a.steps) * per
             w["instr"].clear()
-            for k in range(10):
-                t0 = time.perf_counter()
-                sim.run_packets(ell, ref0 + k * per, per, 10 * per, SG_FP64)
-                log("  fp64 reference part %d/10: %d packets, %.3f s" % (k + 1, per, time.perf_counter() - t0))
+            t0 = time.perf_counter()
+            for k in range(a.ref_mult * a.steps):
+                sim.run_packets(ell, ref0 + k * per, per, nref, SG_FP64)
+                if (k + 1) % 10 == 0:
+                    log("  fp64 reference: %d of %d packets, %.1f s" % ((k + 1) * per, nref, time.perf_counter() - t0))
             imgref = frames(w, ell)
+            S.set_pipeline(a.pipeline)
 
             def rms(x, y):
                 return float(np.sqrt(np.mean((x - y) ** 2)) / np.sqrt(np.mean(y ** 2)))
             e32 = [rms(x, r) for x, r in zip(img32, imgref)]
             e64 = [rms(x, r) for x, r in zip(img64, imgref)]
-            out["fp64"]["image_rms_error_vs_fp64_10x"] = {"mixed": e32, "fp64": e64}
-            log("image rms error vs FP64 with 10x packets: mixed %s, fp64 %s" % (
-                ["%.4f" % e for e in e32], ["%.4f" % e for e in e64]))
+            d32 = [rms(x, y) for x, y in zip(img32, img64)]
+            # the error of n packets alone (the reference's own noise, 1/ref_mult of the variance, removed), and
+            # the packet count at which the worst frame's error would be 1%
+            worst = max(e32)
+            own = worst / math.sqrt(1.0 + 1.0 / a.ref_mult)
+            out["fp64"]["equal_image_error"] = {
+                "packets": n_img, "reference_packets": nref, "rms_error_mixed": e32, "rms_error_fp64": e64,
+                "rms_difference_mixed_vs_fp64_same_packets": d32, "bound": 0.01,
+                "within_bound": bool(worst <= 0.01),
+                "packets_for_1pct_rms": float(n_img * (own / 0.01) ** 2)}
+            log("image rms error vs FP64 with %dx packets: mixed %s, fp64 %s; mixed vs fp64 on the same packets %s" % (
+                a.ref_mult, ["%.4f" % e for e in e32], ["%.4f" % e for e in e64], ["%.1e" % e for e in d32]))
 
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
-        log("cpu: %.3e packets/s on %d threads" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"]))
+        log("cpu: %.3e packets/s on %d threads (-O2), %.3e (-O3 x86-64-v3), %.3e on 1 thread" % (
+            out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"], out["cpu_baseline"]["o3_x86_64_v3"]["value"],
+            out["cpu_baseline"]["single_thread"]["value"]))
     print(json.dumps(out), flush=True)
 
 

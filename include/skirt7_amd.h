@@ -207,6 +207,11 @@ int sg_set_mixed_topdown(int descend);
  * order changes no packet history, only the order of the tally additions. No counterpart in the
  * reference, whose threads walk one packet at a time (MonteCarloSimulation.cpp:259-297). */
 int sg_set_walk_order(int sorted);
+/* Slot groups of the event/walk pipeline: 2 (default) alternate on two streams, so that one group's
+ * event pass and the drain of its walk pass overlap the other group's walks; 1 runs every pass on one
+ * stream, one kernel at a time (each walk launch then has the device to itself, which is how its own
+ * duration is measured). No counterpart in the reference. */
+int sg_set_slot_groups(int groups);
 
 /* ---- setup helpers (the reference's own setup phase, made reproducible) ---------------------- */
 /* Normalised density of an exponential-disc geometry record laid out like a stellar component

@@ -29,6 +29,7 @@ static int g_pipeline = 1;   // 0: persistent lane pipeline (k_mc); 1: event and
 static uint64_t g_batch_alone = (uint64_t)1 << 21;   // batched members this large run alone (0: never)
 static int g_topdown_descent = 0;   // 1: mixed walks of TopDown trees descend from the root at every step
 static int g_walk_order = 0;        // 1: the walk kernel takes each pass's walks in coherence-key order
+static int g_slot_groups = 2;       // slot groups of the event/walk pipeline (2: alternating on two streams)
 
 // Mixed-precision tree walks follow the ropes (the same cells as any search method) unless TopDown
 // descents were asked for a TopDown tree.
@@ -216,6 +217,16 @@ int sg_set_batch_alone(uint64_t min_packets)
 int sg_set_mixed_topdown(int descend)
 {
     g_topdown_descent = descend ? 1 : 0;
+    return 0;
+}
+
+int sg_set_slot_groups(int groups)
+{
+    if (groups != 1 && groups != 2) {
+        sg::set_error("slot groups must be 1 or 2");
+        return -1;
+    }
+    g_slot_groups = groups;
     return 0;
 }
 
@@ -1142,7 +1153,7 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
     hipStream_t* st = g_split_stream;
     const int64_t slots = g_tune_slots > 0 ? g_tune_slots : kSplitSlots;
     const int64_t P = (int64_t)std::max<uint64_t>(1, std::min<uint64_t>(count, (uint64_t)slots));
-    const int G = P >= 2 ? 2 : 1;
+    const int G = (P >= 2 && g_slot_groups == 2) ? 2 : 1;
     const int64_t Pg[2] = {G == 2 ? (P + 1) / 2 : P, G == 2 ? P / 2 : 0};
     Slots<R> sl[2];
     WalkQueue<R> q[2];

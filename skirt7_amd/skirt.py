@@ -601,6 +601,12 @@ def set_walk_order(sorted_=False):
     check(lib().sg_set_walk_order(1 if sorted_ else 0))
 
 
+def set_slot_groups(groups=2):
+    """Event/walk pipeline: 2 (default) slot groups alternating on two streams; 1 runs every pass on one stream,
+    one kernel at a time."""
+    check(lib().sg_set_slot_groups(int(groups)))
+
+
 def set_tuning(block=0, waves_per_cu=0, compact_lanes=0):
     """Launch parameters of sg_sim_run: 0 keeps the current value, a negative value restores the default."""
     check(lib().sg_set_tuning(block, waves_per_cu, compact_lanes))

@@ -7,6 +7,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+# the same restatement built with -O3 -march=x86-64-v3 (the CPU baseline's optimised build)
+ORACLE_V3 = os.path.join(ROOT, "oracle", "_build", "liboracle_v3.so")
 REFSHIM = os.path.join(ROOT, "oracle", "_ref", "libnrref.so")
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -57,20 +59,25 @@ _SIGS = {
 }
 
 _lib = None
+_libs = {}
 
 
-def lib():
+def lib(variant=None):
+    """The oracle library: the -O2 build (default), or variant "v3" (-O3 -march=x86-64-v3)."""
     global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE):
+    path = ORACLE_V3 if variant == "v3" else ORACLE
+    if path not in _libs:
+        if not os.path.exists(path):
             raise RuntimeError("oracle not built (make -C oracle)")
-        L = ctypes.CDLL(ORACLE)
+        L = ctypes.CDLL(path)
         for n, (r, a) in _SIGS.items():
             f = getattr(L, n)
             f.restype = r
             f.argtypes = a
-        _lib = L
-    return _lib
+        _libs[path] = L
+    if path == ORACLE:
+        _lib = _libs[path]
+    return _libs[path]
 
 
 def f64(a):
@@ -84,15 +91,16 @@ def dp(a):
 class Scene:
     """An oracle scene: grid, dust, stars and instruments (all copies of the caller's arrays)."""
 
-    def __init__(self):
-        self.h = lib().or_scene_create()
+    def __init__(self, variant=None):
+        self.L = lib(variant)
+        self.h = self.L.or_scene_create()
         self._keep = []
         self.ninstr = []
         self.nlambda = 1
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.or_scene_destroy(self.h)
+        if getattr(self, "h", None) and getattr(self, "L", None) is not None:
+            self.L.or_scene_destroy(self.h)
             self.h = None
 
     def keep(self, *a):
@@ -101,14 +109,14 @@ class Scene:
 
     def cartesian(self, extent, xv, yv, zv):
         e, x, y, z = self.keep(f64(extent), f64(xv), f64(yv), f64(zv))
-        assert lib().or_scene_set_cartesian(self.h, dp(e), dp(x), len(x), dp(y), len(y), dp(z), len(z)) == 0
+        assert self.L.or_scene_set_cartesian(self.h, dp(e), dp(x), len(x), dp(y), len(y), dp(z), len(z)) == 0
         return self
 
     def tree(self, arity, extent, flags, search):
         e = self.keep(f64(extent))
         fl = self.keep(np.ascontiguousarray(flags, dtype=np.uint8))
-        rc = lib().or_scene_set_tree(self.h, arity, dp(e), fl.ctypes.data_as(_U8), len(fl), search)
-        assert rc == 0, lib().or_last_error()
+        rc = self.L.or_scene_set_tree(self.h, arity, dp(e), fl.ctypes.data_as(_U8), len(fl), search)
+        assert rc == 0, self.L.or_last_error()
         return self
 
     def voronoi(self, extent, sites, off, ids):
@@ -116,7 +124,7 @@ class Scene:
         s = self.keep(f64(sites).reshape(-1, 3))
         o = self.keep(np.ascontiguousarray(off, dtype=np.int64))
         i = self.keep(np.ascontiguousarray(ids, dtype=np.int32))
-        assert lib().or_scene_set_voronoi(self.h, dp(e), dp(s), s.shape[0], o.ctypes.data_as(_I64),
+        assert self.L.or_scene_set_voronoi(self.h, dp(e), dp(s), s.shape[0], o.ctypes.data_as(_I64),
                                           i.ctypes.data_as(_I32)) == 0
         return self
 
@@ -124,57 +132,57 @@ class Scene:
         rho, k, a, gg = self.keep(f64(rho), f64(kext), f64(albedo), f64(g))
         self.nlambda = len(k)
         self.ncells = len(rho)
-        assert lib().or_scene_set_dust(self.h, dp(rho), len(rho), len(k), dp(k), dp(a), dp(gg), int(store_abs)) == 0
+        assert self.L.or_scene_set_dust(self.h, dp(rho), len(rho), len(k), dp(k), dp(a), dp(gg), int(store_abs)) == 0
         return self
 
     def star(self, params, lum, ts=None, tM=None):
         p, l = self.keep(f64(params), f64(lum))
         if ts is not None:
             s, M = self.keep(f64(ts), f64(tM))
-            assert lib().or_scene_add_star(self.h, dp(p), dp(l), dp(s), dp(M), len(s)) == 0
+            assert self.L.or_scene_add_star(self.h, dp(p), dp(l), dp(s), dp(M), len(s)) == 0
         else:
-            assert lib().or_scene_add_star(self.h, dp(p), dp(l), None, None, 0) == 0
+            assert self.L.or_scene_add_star(self.h, dp(p), dp(l), None, None, 0) == 0
         return self
 
     def instrument(self, params):
         p = self.keep(f64(params))
-        assert lib().or_scene_add_instrument(self.h, dp(p)) == 0
+        assert self.L.or_scene_add_instrument(self.h, dp(p)) == 0
         self.ninstr.append((int(p[4]), int(p[5])))
         return self
 
     def star_positions(self, comp, n, seed=4357, first=0):
         """GeometricStellarComp::launch positions of component comp on packet streams [first, first+n)."""
         out = np.zeros((n, 3))
-        rc = lib().or_star_positions(self.h, comp, seed, first, n, dp(out))
-        assert rc == 0, lib().or_last_error()
+        rc = self.L.or_star_positions(self.h, comp, seed, first, n, dp(out))
+        assert rc == 0, self.L.or_last_error()
         return out
 
     def launch_sample(self, n, ell=0, bias=0.5, seed=4357, first=0):
         """StellarSystem::launch with L = 1: (component index, luminosity weight) per packet."""
         comp = np.zeros(n, np.int32)
         w = np.zeros(n)
-        rc = lib().or_launch_sample(self.h, ell, bias, seed, first, n, comp.ctypes.data_as(_I32), dp(w))
-        assert rc == 0, lib().or_last_error()
+        rc = self.L.or_launch_sample(self.h, ell, bias, seed, first, n, comp.ctypes.data_as(_I32), dp(w))
+        assert rc == 0, self.L.or_last_error()
         return comp, w
 
     def path(self, r, k, maxseg=100000):
         rk = f64(np.concatenate([r, k]))
         m = np.zeros(maxseg, np.int32)
         ds = np.zeros(maxseg)
-        n = lib().or_scene_path(self.h, dp(rk), maxseg, m.ctypes.data_as(_I32), dp(ds))
+        n = self.L.or_scene_path(self.h, dp(rk), maxseg, m.ctypes.data_as(_I32), dp(ds))
         assert n >= 0
         return m[:n].copy(), ds[:n].copy()
 
     def whichcell(self, r):
-        return lib().or_scene_whichcell(self.h, dp(f64(r)))
+        return self.L.or_scene_whichcell(self.h, dp(f64(r)))
 
     def opticaldepth(self, r, k, ell=0):
-        return lib().or_scene_opticaldepth(self.h, dp(f64(np.concatenate([r, k]))), ell)
+        return self.L.or_scene_opticaldepth(self.h, dp(f64(np.concatenate([r, k]))), ell)
 
     def pathlength(self, r, k, taus, ell=0):
         t = f64(taus)
         out = np.zeros(len(t))
-        lib().or_scene_pathlength(self.h, dp(f64(np.concatenate([r, k]))), ell, len(t), dp(t), dp(out))
+        self.L.or_scene_pathlength(self.h, dp(f64(np.concatenate([r, k]))), ell, len(t), dp(t), dp(out))
         return out
 
     def _buffers(self):
@@ -187,9 +195,9 @@ class Scene:
         frames, labs = self._buffers()
         c = np.zeros(8, np.uint64)
         mc = f64(w["sim"].mc)
-        rc = lib().or_run_packets(self.h, dp(mc), ell, first, count, npp, threads, dp(frames), dp(labs),
+        rc = self.L.or_run_packets(self.h, dp(mc), ell, first, count, npp, threads, dp(frames), dp(labs),
                                   c.ctypes.data_as(_U64))
-        assert rc == 0, lib().or_last_error()
+        assert rc == 0, self.L.or_last_error()
         self.labs = labs
         return frames, c
 
@@ -197,8 +205,8 @@ class Scene:
         frames, labs = self._buffers()
         c = np.zeros(8, np.uint64)
         m = f64(mc)
-        rc = lib().or_run_reference(self.h, dp(m), threads, dp(frames), dp(labs), c.ctypes.data_as(_U64))
-        assert rc == 0, lib().or_last_error()
+        rc = self.L.or_run_reference(self.h, dp(m), threads, dp(frames), dp(labs), c.ctypes.data_as(_U64))
+        assert rc == 0, self.L.or_last_error()
         self.labs = labs
         return frames, c
 
@@ -229,9 +237,9 @@ def golden_section(ref, frame, minlum, maxlum):
     return float(lum[0]), float(chi2[0]), fr
 
 
-def scene_from_workload(w):
+def scene_from_workload(w, variant=None):
     """An oracle scene with exactly the inputs of a skirt7_amd.workloads setup."""
-    sc = Scene()
+    sc = Scene(variant)
     g = w["grid"]
     if w["kind"] == "cartesian":
         sc.cartesian(g.extent, g.xv, g.yv, g.zv)
