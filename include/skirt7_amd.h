@@ -212,6 +212,10 @@ int sg_set_walk_order(int sorted);
  * stream, one kernel at a time (each walk launch then has the device to itself, which is how its own
  * duration is measured). No counterpart in the reference. */
 int sg_set_slot_groups(int groups);
+/* Packets whose launch states sg_sim_run emits at once (0: the default, 2^25 for the event/walk pipeline and
+ * 2^22 for the lane pipeline; otherwise at least 64): a longer range is transported batch by batch, with the
+ * same packet histories. No counterpart in the reference. */
+int sg_set_emit_batch(uint64_t packets);
 
 /* ---- setup helpers (the reference's own setup phase, made reproducible) ---------------------- */
 /* Normalised density of an exponential-disc geometry record laid out like a stellar component

@@ -53,6 +53,7 @@ _SIGS = {
     "sg_set_mixed_topdown": (ctypes.c_int, [ctypes.c_int]),
     "sg_set_walk_order": (ctypes.c_int, [ctypes.c_int]),
     "sg_set_slot_groups": (ctypes.c_int, [ctypes.c_int]),
+    "sg_set_emit_batch": (ctypes.c_int, [ctypes.c_uint64]),
     "sg_set_slots": (ctypes.c_int, [ctypes.c_int64]),
     "sg_autotune": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                                    _I64, _D]),

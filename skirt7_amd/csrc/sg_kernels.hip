@@ -30,6 +30,7 @@ static uint64_t g_batch_alone = (uint64_t)1 << 21;   // batched members this lar
 static int g_topdown_descent = 0;   // 1: mixed walks of TopDown trees descend from the root at every step
 static int g_walk_order = 0;        // 1: the walk kernel takes each pass's walks in coherence-key order
 static int g_slot_groups = 2;       // slot groups of the event/walk pipeline (2: alternating on two streams)
+static uint64_t g_emit_batch = 0;   // packets emitted at once (0: each pipeline's default)
 
 // Mixed-precision tree walks follow the ropes (the same cells as any search method) unless TopDown
 // descents were asked for a TopDown tree.
@@ -217,6 +218,16 @@ int sg_set_batch_alone(uint64_t min_packets)
 int sg_set_mixed_topdown(int descend)
 {
     g_topdown_descent = descend ? 1 : 0;
+    return 0;
+}
+
+int sg_set_emit_batch(uint64_t packets)
+{
+    if (packets != 0 && packets < 64) {
+        sg::set_error("emission batch must be 0 (default) or at least 64 packets");
+        return -1;
+    }
+    g_emit_batch = packets;
     return 0;
 }
 
@@ -1201,7 +1212,7 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
     int ebs = 256;
     while (ebs > 64 && staged_bytes<R>(ebs * (a.ni + 1)) > 64 * 1024) ebs /= 2;
     const size_t eshm = staged_bytes<R>(ebs * (a.ni + 1));
-    const uint64_t batch = std::min<uint64_t>(count, kSplitEmit);
+    const uint64_t batch = std::min<uint64_t>(count, g_emit_batch ? g_emit_batch : kSplitEmit);
     const size_t rsz = sizeof(R);
     DevBuf& emitted = pool().emitted;
     DevBuf& emitdraw = pool().emitdraw;
@@ -1328,7 +1339,7 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
     s->err.zero();
     const bool mixed = precision == SG_MIXED;
     const size_t rsz = mixed ? sizeof(float) : sizeof(double);
-    const uint64_t batch = std::min<uint64_t>(count, (uint64_t)1 << 22);
+    const uint64_t batch = std::min<uint64_t>(count, g_emit_batch ? g_emit_batch : (uint64_t)1 << 22);
     if (g_pipeline == 0) {
         if (s->emitted.bytes < 7 * batch * rsz) s->emitted.alloc(7 * batch * rsz);
         if (s->emitdraw.bytes < batch * sizeof(uint32_t)) s->emitdraw.alloc(batch * sizeof(uint32_t));

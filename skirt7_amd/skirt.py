@@ -601,6 +601,11 @@ def set_walk_order(sorted_=False):
     check(lib().sg_set_walk_order(1 if sorted_ else 0))
 
 
+def set_emit_batch(packets=0):
+    """Packets emitted at once by run_packets (0: the default); longer ranges are transported batch by batch."""
+    check(lib().sg_set_emit_batch(int(packets)))
+
+
 def set_slot_groups(groups=2):
     """Event/walk pipeline: 2 (default) slot groups alternating on two streams; 1 runs every pass on one stream,
     one kernel at a time."""
