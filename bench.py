@@ -52,6 +52,8 @@ def parse():
                          "absorption table; fit_population: batched population + fit; lognormal_kdtree: kd-tree over a "
                          "log-normal field")
     ap.add_argument("--members", type=int, default=256, help="fit_population: population size")
+    ap.add_argument("--one-wavelength", action="store_true",
+                    help="spiral_voronoi: run only the 0.55 micron wavelength instead of all 50 per step")
     ap.add_argument("--member-packets", type=float, nargs=2, default=(1e5, 1e7),
                     help="fit_population: log-uniform range of packets per member")
     ap.add_argument("--steps", type=int, default=5)
@@ -111,13 +113,18 @@ def frames(w, ell=0):
 
 
 def run_steps(sim, first, nsteps, per, npp, precision, ell=0):
-    """Runs nsteps batches; returns the summed counters and the transport kernel's device seconds."""
+    """Runs nsteps batches of packets [first + s per, first + (s + 1) per) at wavelength ell (or at each of a list
+    of wavelengths in turn); returns the summed counters and the transport kernel's device seconds."""
+    ells = list(ell) if isinstance(ell, (list, tuple, range)) else [ell]
     tot = None
     for s in range(nsteps):
-        c = sim.run_packets(ell, first + s * per, per, npp, precision)
-        tot = c if tot is None else {k: tot[k] + c[k] for k in c}
+        st = None
+        for e in ells:
+            c = sim.run_packets(e, first + s * per, per, npp, precision)
+            st = c if st is None else {k: st[k] + c[k] for k in c}
+        tot = st if tot is None else {k: tot[k] + st[k] for k in st}
         log("  step %d/%d (%s): %d packets, %.3f s on the device" % (
-            s + 1, nsteps, "mixed" if precision == 1 else "fp64", c["packets"], c["kernel_us"] * 1e-6))
+            s + 1, nsteps, "mixed" if precision == 1 else "fp64", st["packets"], st["kernel_us"] * 1e-6))
     return tot, (tot["transport_us"] * 1e-6 if tot else 0.0)
 
 
@@ -251,6 +258,7 @@ def main():
     total = per * (a.steps + a.warmup)
     t0 = time.perf_counter()
     ell = 0
+    ells = None
     if a.workload == "disc_cartesian":
         w = workloads.disc_cartesian(pixels=min(a.pixels, 256), packages=total)
         desc = "Cartesian 50^3 grid, exponential disc + Sersic bulge, face-on tau 1"
@@ -260,11 +268,17 @@ def main():
     elif a.workload == "spiral_voronoi":
         w = workloads.spiral_voronoi(pixels=min(a.pixels, 256), packages=total)
         ell = int(np.argmin(np.abs(w["wavelengths"] - 0.55)))
-        desc = "Voronoi grid of 1e6 sites, 50 wavelengths (run at %.3g micron), absorption table stored" % (
-            w["wavelengths"][ell])
+        if a.one_wavelength:
+            desc = "Voronoi grid of 1e6 sites, 50 wavelengths (run at %.3g micron), absorption table stored" % (
+                w["wavelengths"][ell])
+        else:
+            # a step runs its packets at every wavelength (each wavelength its own Npp packets, as the reference)
+            ells = list(range(len(w["wavelengths"])))
+            desc = "Voronoi grid of 1e6 sites, all 50 wavelengths per step (1e6 x 50 absorption table stored)"
     else:
         w = workloads.lognormal_kdtree(pixels=min(a.pixels * 2, 1024), packages=total)
         desc = "kd-tree (alternating binary tree, TopDown) over a log-normal field"
+    ells = ells or [ell]
     ncell = w["grid"].numCells()
     gkind = "voronoi" if w["kind"] == "voronoi" else "cartesian" if w["kind"] == "cartesian" else "tree"
     log("setup: %s with %d cells built in %.1f s on %s" % (a.workload, ncell, time.perf_counter() - t0,
@@ -287,10 +301,12 @@ def main():
         log("autotune (mixed, %d-packet trials, %.1f s): %s" % (sample, time.perf_counter() - t0, tuning["mixed"]))
 
     # warmup (mixed), then the timed mixed steps
-    run_steps(sim, 0, a.warmup, per, npp, SG_MIXED, ell)
+    run_steps(sim, 0, a.warmup, per, npp, SG_MIXED, ells)
     w["instr"].clear()
+    if w.get("store_abs"):
+        w["dust"].clear()
     t0 = time.perf_counter()
-    cm, kmix = run_steps(sim, a.warmup * per, a.steps, per, npp, SG_MIXED, ell)
+    cm, kmix = run_steps(sim, a.warmup * per, a.steps, per, npp, SG_MIXED, ells)
     wall = time.perf_counter() - t0
     img_mixed = frames(w, ell)
     pk = cm["packets"]
@@ -304,8 +320,9 @@ def main():
            "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 (walk, weights) + f64 (wall fallback, tallies)", "data": "synthetic",
            "config": {"workload": "%s, %d cells, %d packets per step, %d frames of %d^2" % (
-                          desc, ncell, per, len(w["instr"].instruments), w["instr"].instruments[0].Nx),
-                      "packets_per_step": per, "cells": ncell, "instruments": len(w["instr"].instruments),
+                          desc, ncell, per * len(ells), len(w["instr"].instruments), w["instr"].instruments[0].Nx),
+                      "packets_per_step": per * len(ells), "wavelengths_per_step": len(ells), "cells": ncell,
+                      "instruments": len(w["instr"].instruments),
                       "pixels": w["instr"].instruments[0].Nx, "tuning": tuning}}
     def algorithmic_bytes(c):
         """Algorithmic bytes of a run's walks: crossings x bytes per crossing, peel-off tallies, absorption."""
@@ -325,7 +342,7 @@ def main():
     kwalk = cm["walk_kernel_us"] * 1e-6
     S.set_slot_groups(1)
     w["instr"].clear()
-    c1 = sim.run_packets(ell, a.warmup * per, per, npp, SG_MIXED)
+    c1, _ = run_steps(sim, a.warmup * per, 1, per, npp, SG_MIXED, ells)
     S.set_slot_groups(a.slot_groups)
     w["instr"].clear()
     b1 = algorithmic_bytes(c1)
@@ -393,7 +410,7 @@ def main():
                 log("autotune (fp64, pipeline %d): %s" % (p, tuning["fp64_pipeline%d" % p]))
             w["instr"].clear()
             t0 = time.perf_counter()
-            c64 = sim.run_packets(ell, first, per, npp, SG_FP64)
+            c64, _ = run_steps(sim, first, 1, per, npp, SG_FP64, ells)
             dt = time.perf_counter() - t0
             fp64[p] = (c64["packets"] / dt, c64)
             log("  fp64 step (pipeline %d): %d packets, %.3f s" % (p, c64["packets"], dt))
@@ -406,7 +423,7 @@ def main():
                        "speedup_mixed_over_fp64": out["value"] / fp64_rate}
         log("fp64: %.3e packets/s (pipeline %d; %s); mixed/fp64 = %.2fx" % (
             fp64_rate, best, {p: "%.3e" % fp64[p][0] for p in fp64}, out["value"] / fp64_rate))
-        if not a.no_equal_error:
+        if not a.no_equal_error and len(ells) == 1:
             # Equal image error. The mixed image of the timed steps (their steps x per packets) and the FP64
             # image of the same packets (identical random streams), both against an FP64 reference run of
             # ref_mult times as many fresh packets; every image normalised to a full run.

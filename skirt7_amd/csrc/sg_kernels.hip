@@ -313,28 +313,47 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     // leaves that are neighbours in space, which a walk meets one after another, lie near each other
     // in memory (a node record fetched at random brings a whole cache line in). Sibling groups stay
     // contiguous, so a node's children are still its first child + q. Host-side ids stay breadth-first.
+    // Below that prefix, sibling groups that hold an internal node come first, then the groups made of
+    // leaves only, each part in depth-first order: the walks read leaf records, and a cache line then holds
+    // leaves of one small subtree instead of leaves mixed with internal nodes (half the nodes of a binary
+    // tree), which a walk reaches only when a rope leads to a finer region.
     std::vector<int32_t> nid(nn, -1);
     const int64_t top = nn <= SG_LDS_NODES ? nn : 1 + arity * ((SG_LDS_NODES - 1) / arity);
+    int64_t ndev = nn;   // device ids in use, padding included
     for (int64_t l = 0; l < top; l++) nid[l] = (int32_t)l;
     {
-        int64_t next = top;
+        std::vector<int32_t> groups;   // first child of each sibling group below the prefix, depth-first
         std::vector<int32_t> stack{0};
         while (!stack.empty()) {
             const int32_t l = stack.back();
             stack.pop_back();
             const int32_t c0 = h.child0[l];
             if (c0 < 0) continue;
-            if (nid[c0] < 0)
-                for (int q = 0; q < arity; q++) nid[c0 + q] = (int32_t)next++;
+            if (nid[c0] < 0) groups.push_back(c0);
             for (int q = arity - 1; q >= 0; q--) stack.push_back(c0 + q);
         }
-        SG_REQUIRE(next == nn, "tree renumbering missed nodes");
+        int64_t next = top, used = top;
+        for (int pass = 0; pass < 2; pass++) {
+            // the leaf groups start on a 128-byte line (four 32-byte records), so that every group of two
+            // or eight lies within whole lines; the skipped ids hold no node
+            if (pass == 1) next = (next + 3) / 4 * 4;
+            for (int32_t c0 : groups) {
+                bool leaves = true;
+                for (int q = 0; q < arity; q++) leaves = leaves && h.child0[c0 + q] < 0;
+                if (leaves != (pass == 1)) continue;
+                for (int q = 0; q < arity; q++) nid[c0 + q] = (int32_t)next++;
+                used += arity;
+            }
+        }
+        SG_REQUIRE(used == nn, "tree renumbering missed nodes");
+        ndev = next;
     }
-    std::vector<int32_t> words(nn);
+    SG_REQUIRE(ndev < (int64_t)1 << 31, "too many tree nodes");
+    std::vector<int32_t> words(ndev, 0);
     for (int64_t l = 0; l < nn; l++) words[nid[l]] = h.child0[l] >= 0 ? nid[h.child0[l]] : -(h.cellnumber[l] + 1);
     g->node.upload(words);
     g->hrec64.assign(nc, TreeRec64{});
-    g->hnode32.assign(nn, TreeNode32{});
+    g->hnode32.assign(ndev, TreeNode32{});
     std::vector<int32_t> off(nc * 6 + 1, 0), ids;
     for (int64_t c = 0; c < nc; c++) {
         int l = h.idv[c];
@@ -349,8 +368,8 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     }
     // mixed-precision records: subdivision counts per axis (a child adds one along each axis of an
     // octree, or along its parent's split axis of a binary tree), density or first child, ropes
-    std::vector<int32_t> cellof(nn);
-    std::vector<uint64_t> kpack(nn);
+    std::vector<int32_t> cellof(ndev, -1);
+    std::vector<uint64_t> kpack(ndev, 0);
     std::vector<int32_t> ropes;
     h.build_ropes(ropes);
     std::vector<uint32_t> nsub(nn, 0);   // nx | ny << 5 | nz << 10
@@ -440,7 +459,7 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     t.kpack = g->kpack.as<uint64_t>();
     t.nboff = g->nboff.as<int32_t>();
     t.nbids = g->nbids.as<int32_t>();
-    t.nnodes = (int32_t)nn; t.ncells = (int32_t)nc;
+    t.nnodes = (int32_t)ndev; t.ncells = (int32_t)nc;
     t.arity = arity; t.search = search;
     for (int i = 0; i < 6; i++) { t.ext[i] = ext[i]; t.extf[i] = (float)ext[i]; }
     for (int a = 0; a < 3; a++) t.widthf[a] = (float)(ext[a + 3] - ext[a]);
