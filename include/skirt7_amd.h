@@ -32,7 +32,7 @@ typedef struct sg_instruments sg_instruments;  /* InstrumentSystem of FrameInstr
 typedef struct sg_sim sg_sim;              /* MonteCarloSimulation            */
 
 enum { SG_FP64 = 0, SG_MIXED = 1 };
-enum { SG_TOPDOWN = 0, SG_NEIGHBOR = 1 };   /* TreeDustGrid::SearchMethod (TreeDustGrid.hpp:50) */
+enum { SG_TOPDOWN = 0, SG_NEIGHBOR = 1, SG_BOOKKEEPING = 2 };   /* TreeDustGrid::SearchMethod (TreeDustGrid.hpp:149) */
 
 /* Number of doubles in the parameter records below. */
 #define SG_STAR_NPARAM 16
@@ -187,8 +187,9 @@ int sg_set_slots(int64_t slots);
  * pipeline), then the block
  * size (64, 128, 256), keeping the fastest of each in turn, and leaves them set for later runs.
  * With choose_pipeline nonzero it first times both transport organisations of sg_set_pipeline with
- * their default parameters and keeps the faster. best (may be null) receives {block, slots or waves
- * per CU, compaction threshold, pipeline}; rate (may be null) the sample's packets per second with
+ * their default parameters and keeps the faster; the event/walk pipeline also tries both walk orders
+ * (sg_set_walk_order). best (may be null) receives {block, slots or waves per CU, compaction threshold,
+ * pipeline, walk order}; rate (may be null) the sample's packets per second with
  * them. The results of a run do not depend on these choices. The tallies the sample adds are
  * removed again (frames and absorption as before). */
 int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precision, int choose_pipeline,

@@ -15,6 +15,7 @@ SG_FP64 = 0
 SG_MIXED = 1
 SG_TOPDOWN = 0
 SG_NEIGHBOR = 1
+SG_BOOKKEEPING = 2
 SG_STAR_NPARAM = 16
 SG_INSTR_NPARAM = 10
 SG_MC_NPARAM = 8

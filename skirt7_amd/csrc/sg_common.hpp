@@ -69,11 +69,17 @@ struct TreeView {
     const TreeNode32* node32;        // mixed-precision walk records
     const int32_t* cellof;           // cell number per node (-1 for internal nodes)
     const uint64_t* kpack;           // integer indices per node, i | j << 21 | k << 42
+    // Bookkeeping search (octrees): per node in the reference's breadth-first order, its box (6 doubles),
+    // first child (-1 for a leaf), father (-1 for the root) and cell number (-1 for an internal node)
+    const double* bkb;
+    const int32_t* bkchild0;
+    const int32_t* bkfather;
+    const int32_t* bkcell;
     const int32_t* nboff;            // CSR offsets, index cell*6 + wall
     const int32_t* nbids;            // neighbour cell numbers
     int32_t nnodes, ncells;
     int arity;                       // 8 octree, 2 alternating binary tree
-    int search;                      // 0 TopDown, 1 Neighbor
+    int search;                      // 0 TopDown, 1 Neighbor, 2 Bookkeeping
     double ext[6];
     float extf[6];
     float widthf[3];                 // root box widths

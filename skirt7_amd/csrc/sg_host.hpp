@@ -90,7 +90,7 @@ struct sg_grid {
     sg::CartView cart{};
     // tree
     sg::HostTree ht;
-    sg::DevBuf node, rec64, node32, cellof, kpack, nboff, nbids;
+    sg::DevBuf node, rec64, node32, cellof, kpack, nboff, nbids, bkb, bkchild0, bkfather, bkcell;
     std::vector<sg::TreeRec64> hrec64;
     std::vector<sg::TreeNode32> hnode32;
     sg::TreeView tree{};

@@ -298,7 +298,10 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     SG_TRY
     ensure_init();
     SG_REQUIRE(arity == 8 || arity == 2, "arity must be 8 (octree) or 2 (binary tree)");
-    SG_REQUIRE(search == SG_TOPDOWN || search == SG_NEIGHBOR, "search method must be TopDown or Neighbor");
+    SG_REQUIRE(search == SG_TOPDOWN || search == SG_NEIGHBOR || search == SG_BOOKKEEPING,
+               "search method must be TopDown, Neighbor or Bookkeeping");
+    // BinTreeDustGrid::setupSelfBefore (BinTreeDustGrid.cpp:23-24)
+    SG_REQUIRE(!(search == SG_BOOKKEEPING && arity != 8), "Bookkeeping method is not compatible with binary tree");
     SG_REQUIRE(nnodes >= 1 && nnodes < (int64_t)1 << 31, "bad node count");
     auto* g = new sg_grid();
     g->kind = GK_TREE;
@@ -461,6 +464,19 @@ int sg_grid_create_tree(int arity, const double ext[6], const uint8_t* subdivide
     t.nbids = g->nbids.as<int32_t>();
     t.nnodes = (int32_t)ndev; t.ncells = (int32_t)nc;
     t.arity = arity; t.search = search;
+    if (search == SG_BOOKKEEPING) {
+        // the node arrays of the Bookkeeping walk, in the reference's breadth-first order
+        std::vector<int32_t> fa(nn), cn(nn), c0(nn);
+        for (int64_t l = 0; l < nn; l++) { fa[l] = h.father[l]; cn[l] = h.cellnumber[l]; c0[l] = h.child0[l]; }
+        g->bkb.upload(h.bounds.data(), 6 * (size_t)nn);
+        g->bkchild0.upload(c0);
+        g->bkfather.upload(fa);
+        g->bkcell.upload(cn);
+        t.bkb = g->bkb.as<double>();
+        t.bkchild0 = g->bkchild0.as<int32_t>();
+        t.bkfather = g->bkfather.as<int32_t>();
+        t.bkcell = g->bkcell.as<int32_t>();
+    }
     for (int i = 0; i < 6; i++) { t.ext[i] = ext[i]; t.extf[i] = (float)ext[i]; }
     for (int a = 0; a < 3; a++) t.widthf[a] = (float)(ext[a + 3] - ext[a]);
     t.eps = h.eps;
@@ -684,7 +700,8 @@ int sg_grid_paths(sg_grid* g, int64_t n, const double* rk, int max_seg, int prec
         if (mixed) launch_paths<CartW32>(g->cart, n, d, max_seg, oc, om, od);
         else launch_paths<CartW64>(g->cart, n, d, max_seg, oc, om, od);
     } else if (g->kind == GK_TREE) {
-        if (!mixed) launch_paths<TreeW64>(g->tree, n, d, max_seg, oc, om, od);
+        if (!mixed && g->tree.search == SG_BOOKKEEPING) launch_paths<TreeW64Book>(g->tree, n, d, max_seg, oc, om, od);
+        else if (!mixed) launch_paths<TreeW64>(g->tree, n, d, max_seg, oc, om, od);
         else if (rope_walk(g)) launch_paths<TreeW32>(g->tree, n, d, max_seg, oc, om, od);
         else launch_paths<TreeW32TopDown>(g->tree, n, d, max_seg, oc, om, od);
     } else {
@@ -761,7 +778,8 @@ static int tau_common(sg_dust* d, int ell, int64_t n, const double* rk, const do
         if (mixed) launch_tau<CartW32>(g->cart, n, ri, ke, ti, mode, o);
         else launch_tau<CartW64>(g->cart, n, ri, ke, ti, mode, o);
     } else if (g->kind == GK_TREE) {
-        if (!mixed) launch_tau<TreeW64>(g->tree, n, ri, ke, ti, mode, o);
+        if (!mixed && g->tree.search == SG_BOOKKEEPING) launch_tau<TreeW64Book>(g->tree, n, ri, ke, ti, mode, o);
+        else if (!mixed) launch_tau<TreeW64>(g->tree, n, ri, ke, ti, mode, o);
         else if (rope_walk(g)) launch_tau<TreeW32>(g->tree, n, ri, ke, ti, mode, o);
         else launch_tau<TreeW32TopDown>(g->tree, n, ri, ke, ti, mode, o);
     } else {
@@ -1126,7 +1144,8 @@ template <bool ABS> static void dispatch_mc(sg_sim* s, const McArgs& a, int prec
         if (mixed) launch_mc<CartW32, ABS>(g->cart, a, blocks, s->cold, s->args);
         else launch_mc<CartW64, ABS>(g->cart, a, blocks, s->cold, s->args);
     } else if (g->kind == GK_TREE) {
-        if (!mixed) launch_mc<TreeW64, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
+        if (!mixed && g->tree.search == SG_BOOKKEEPING) launch_mc<TreeW64Book, ABS>(g->tree, a, blocks, s->cold, s->args, 0);
+        else if (!mixed) launch_mc<TreeW64, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
         else if (rope_walk(g)) launch_mc<TreeW32, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
         else launch_mc<TreeW32TopDown, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
     } else {
@@ -1336,7 +1355,8 @@ static void dispatch_split(sg_sim* s, const McArgs& a, int precision, uint64_t f
         if (mixed) run_split<CartW32, ABS>(s, g->cart, a, first, count, 0, lt, blocks);
         else run_split<CartW64, ABS>(s, g->cart, a, first, count, 0, lt, blocks);
     } else if (g->kind == GK_TREE) {
-        if (!mixed) run_split<TreeW64, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
+        if (!mixed && g->tree.search == SG_BOOKKEEPING) run_split<TreeW64Book, ABS>(s, g->tree, a, first, count, 0, lt, blocks);
+        else if (!mixed) run_split<TreeW64, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
         else if (rope_walk(g)) run_split<TreeW32, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
         else run_split<TreeW32TopDown, ABS>(s, g->tree, a, first, count, SG_LDS_NODES, lt, blocks);
     } else {
@@ -1536,7 +1556,8 @@ static void dispatch_static(sg_sim* s, const McArgs& a, int precision, int& bloc
         if (mixed) run_static<CartW32, ABS>(s, g->cart, a, 0, blocks);
         else run_static<CartW64, ABS>(s, g->cart, a, 0, blocks);
     } else if (g->kind == GK_TREE) {
-        if (!mixed) run_static<TreeW64, ABS>(s, g->tree, a, SG_LDS_NODES, blocks);
+        if (!mixed && g->tree.search == SG_BOOKKEEPING) run_static<TreeW64Book, ABS>(s, g->tree, a, 0, blocks);
+        else if (!mixed) run_static<TreeW64, ABS>(s, g->tree, a, SG_LDS_NODES, blocks);
         else if (rope_walk(g)) run_static<TreeW32, ABS>(s, g->tree, a, SG_LDS_NODES, blocks);
         else run_static<TreeW32TopDown, ABS>(s, g->tree, a, SG_LDS_NODES, blocks);
     } else {
@@ -1615,6 +1636,7 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
         return r;
     };
     const int pipeline0 = g_pipeline;
+    const int order0 = g_walk_order;
     const int block0 = g_tune_block, compact0 = g_tune_compact, waves0 = g_tune_waves;
     const int64_t slots0 = g_tune_slots;
     double top = 0;
@@ -1630,7 +1652,7 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
             g_pipeline = r[1] >= r[0] ? 1 : 0;
         } catch (...) {
             g_pipeline = pipeline0; g_tune_block = block0; g_tune_compact = compact0; g_tune_waves = waves0;
-            g_tune_slots = slots0;
+            g_tune_slots = slots0; g_walk_order = order0;
             for (auto& k : keep) {
                 (void)hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice);
                 delete k.first;
@@ -1642,6 +1664,7 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
     int compact = g_tune_compact > 0 ? g_tune_compact : (g_pipeline == 1 ? 24 : 32);
     int64_t slots = g_tune_slots > 0 ? g_tune_slots : kSplitSlots;
     int waves = g_tune_waves;
+    int order = g_walk_order;
     g_tune_compact = compact;
     try {
         top = trial();
@@ -1660,6 +1683,12 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
                 if (r > top) { top = r; slots = p; }
             }
             g_tune_slots = slots;
+            // the walk order: as appended, or sorted by coherence key (which pays where the walks' grid
+            // data does not fit the caches)
+            g_walk_order = 1 - order;
+            const double r = trial();
+            if (r > top) { top = r; order = g_walk_order; }
+            g_walk_order = order;
         } else {
             for (int w : {4, 8, 12, 16}) {
                 if (w == waves) continue;
@@ -1678,6 +1707,7 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
         g_tune_block = block;
     } catch (...) {
         g_tune_block = block; g_tune_compact = compact; g_tune_slots = slots; g_tune_waves = waves;
+        g_walk_order = order;
         for (auto& k : keep) {
             (void)hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice);
             delete k.first;
@@ -1688,7 +1718,10 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
         SG_HIP(hipMemcpy(k.second, k.first->p, k.first->bytes, hipMemcpyDeviceToDevice));
         delete k.first;
     }
-    if (best) { best[0] = block; best[1] = g_pipeline == 1 ? slots : waves; best[2] = compact; best[3] = g_pipeline; }
+    if (best) {
+        best[0] = block; best[1] = g_pipeline == 1 ? slots : waves; best[2] = compact; best[3] = g_pipeline;
+        best[4] = g_pipeline == 1 ? order : 0;
+    }
     if (rate) *rate = top;
     SG_CATCH
 }
@@ -1757,7 +1790,8 @@ static void dispatch_batch(sg_sim* s0, const std::vector<McArgs>& rg, const std:
         if (mixed) launch_batch<CartW32, ABS>(s0, rg, owner, blocks, lt);
         else launch_batch<CartW64, ABS>(s0, rg, owner, blocks, lt);
     } else if (g->kind == GK_TREE) {
-        if (!mixed) launch_batch<TreeW64, ABS>(s0, rg, owner, blocks, lt);
+        if (!mixed && g->tree.search == SG_BOOKKEEPING) launch_batch<TreeW64Book, ABS>(s0, rg, owner, blocks, lt);
+        else if (!mixed) launch_batch<TreeW64, ABS>(s0, rg, owner, blocks, lt);
         else if (rope_walk(g)) launch_batch<TreeW32, ABS>(s0, rg, owner, blocks, lt);
         else launch_batch<TreeW32TopDown, ABS>(s0, rg, owner, blocks, lt);
     } else {

@@ -537,6 +537,136 @@ struct TreeW64 {
     }
 };
 
+// TreeDustGrid::path, Bookkeeping method (TreeDustGrid.cpp:508-640), exact FP64, octrees only. The nodes are
+// addressed in the reference's breadth-first creation order, in which every sibling group of eight starts
+// at an index 1 + 8k, so a node's octant is (l - 1) % 8: the next cell is found by climbing to the first
+// ancestor that has a sibling across the exit wall, stepping to that sibling (l -+ 1, 2 or 4) and
+// descending to the leaf by the upper planes of child 0's box (a point on a plane goes to the lower child,
+// the reference's <=). The tie rules are this method's own: x on <= with both, y on < x and <= z, z on <
+// both; the exit coordinate is set to the wall exactly and no small step is added.
+struct TreeW64Book {
+    using Real = double;
+    using View = TreeView;
+    double x, y, z, kx, ky, kz;
+    int l;                   // current node, breadth-first index
+    Entry en;
+    bool done;
+    uint32_t steps, nrecompute, nstuck;
+    static constexpr int kParkWords = 25;
+    // segments carry cell numbers
+    __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
+    static constexpr bool kHint = false;
+    __device__ __forceinline__ int entries() const { return en.ne; }
+    static constexpr bool kStep = false;   // next() only
+    template <class P> __device__ __forceinline__ void park(P& p)
+    {
+        p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
+        p.io(l); p.io(en.ne); p.io(en.e0); p.io(en.e1); p.io(en.e2);
+        p.io(done); p.io(steps); p.io(nrecompute); p.io(nstuck);
+    }
+    __device__ __forceinline__ void init(const View& t, const int32_t*, int, double rx, double ry, double rz, double dx,
+                                         double dy, double dz)
+    {
+        kx = dx; ky = dy; kz = dz;
+        steps = 0; done = false; l = 0;
+        if (!en.move_inside(t.ext, t.eps, rx, ry, rz, kx, ky, kz)) { en.ne = 0; done = true; return; }
+        x = rx; y = ry; z = rz;
+        // TreeNode::whichnode from the root (TreeNode.cpp:70-80): inside the closed root box, then
+        // OctTreeNode::child by "<" against child 0's upper corner (OctTreeNode.cpp:184-189)
+        const double* b = t.bkb;
+        if (!(x >= b[0] && x <= b[3] && y >= b[1] && y <= b[4] && z >= b[2] && z <= b[5])) {
+            en.ne = 0; done = true; return;
+        }
+        int n = 0;
+        for (int level = 0; t.bkchild0[n] >= 0 && level < 64; level++) {
+            const int c0 = t.bkchild0[n];
+            const double* cb = t.bkb + 6 * (size_t)c0;
+            n = c0 + (x < cb[3] ? 0 : 1) + (y < cb[4] ? 0 : 2) + (z < cb[5] ? 0 : 4);
+        }
+        l = n;
+    }
+    // child q of node n
+    __device__ __forceinline__ static int child(const View& t, int n, int q) { return t.bkchild0[n] + q; }
+    __device__ __forceinline__ bool next(const View& t, const int32_t*, int, int& mo, double& ds, double& rh)
+    {
+        if (en.ne) { ds = en.pop(); mo = -1; rh = 0.0; return true; }
+        while (!done) {
+            const double* nb = t.bkb + 6 * (size_t)l;
+            const double xnext = (kx < 0.0) ? nb[0] : nb[3];
+            const double ynext = (ky < 0.0) ? nb[1] : nb[4];
+            const double znext = (kz < 0.0) ? nb[2] : nb[5];
+            const double dsx = (fabs(kx) > 1e-15) ? (xnext - x) / kx : DBL_MAX;
+            const double dsy = (fabs(ky) > 1e-15) ? (ynext - y) / ky : DBL_MAX;
+            const double dsz = (fabs(kz) > 1e-15) ? (znext - z) / kz : DBL_MAX;
+            const int cell = t.bkcell[l];
+            double d;
+            if (dsx <= dsy && dsx <= dsz) {
+                d = dsx;
+                x = xnext; y += ky * dsx; z += kz * dsx;
+                // climb while the node lies on the side of its father the ray leaves through
+                for (;;) {
+                    const int oct = ((l - 1) % 8) + 1;
+                    if (!((kx < 0.0) ? (oct % 2 == 1) : (oct % 2 == 0))) break;
+                    l = t.bkfather[l];
+                    if (l == 0) { done = true; break; }
+                }
+                if (!done) {
+                    l += (kx < 0.0) ? -1 : 1;
+                    while (t.bkcell[l] == -1) {
+                        const double* cb = t.bkb + 6 * (size_t)child(t, l, 0);
+                        const double yM = cb[4], zM = cb[5];
+                        if (kx < 0.0) l = (y <= yM) ? child(t, l, (z <= zM) ? 1 : 5) : child(t, l, (z <= zM) ? 3 : 7);
+                        else l = (y <= yM) ? child(t, l, (z <= zM) ? 0 : 4) : child(t, l, (z <= zM) ? 2 : 6);
+                    }
+                }
+            } else if (dsy < dsx && dsy <= dsz) {
+                d = dsy;
+                x += kx * dsy; y = ynext; z += kz * dsy;
+                for (;;) {
+                    if (!((ky < 0.0) ? ((l - 1) % 4 < 2) : ((l - 1) % 4 > 1))) break;
+                    l = t.bkfather[l];
+                    if (l == 0) { done = true; break; }
+                }
+                if (!done) {
+                    l += (ky < 0.0) ? -2 : 2;
+                    while (t.bkcell[l] == -1) {
+                        const double* cb = t.bkb + 6 * (size_t)child(t, l, 0);
+                        const double xM = cb[3], zM = cb[5];
+                        if (ky < 0.0) l = (x <= xM) ? child(t, l, (z <= zM) ? 2 : 6) : child(t, l, (z <= zM) ? 3 : 7);
+                        else l = (x <= xM) ? child(t, l, (z <= zM) ? 0 : 4) : child(t, l, (z <= zM) ? 1 : 5);
+                    }
+                }
+            } else if (dsz < dsx && dsz < dsy) {
+                d = dsz;
+                x += kx * dsz; y += ky * dsz; z = znext;
+                for (;;) {
+                    const int oct = ((l - 1) % 8) + 1;
+                    if (!((kz < 0.0) ? (oct < 5) : (oct > 4))) break;
+                    l = t.bkfather[l];
+                    if (l == 0) { done = true; break; }
+                }
+                if (!done) {
+                    l += (kz < 0.0) ? -4 : 4;
+                    while (t.bkcell[l] == -1) {
+                        const double* cb = t.bkb + 6 * (size_t)child(t, l, 0);
+                        const double xM = cb[3], yM = cb[4];
+                        if (kz < 0.0) l = (x <= xM) ? child(t, l, (y <= yM) ? 4 : 6) : child(t, l, (y <= yM) ? 5 : 7);
+                        else l = (x <= xM) ? child(t, l, (y <= yM) ? 0 : 2) : child(t, l, (y <= yM) ? 1 : 3);
+                    }
+                }
+            } else {
+                // only with NaN input, where the reference would not terminate
+                done = true;
+                return false;
+            }
+            if (++steps > SG_MAX_STEPS) done = true;
+            // DustGridPath::addSegment keeps only segments of positive length
+            if (d > 0) { mo = cell; ds = d; rh = t.rec64[cell].rho; return true; }
+        }
+        return false;
+    }
+};
+
 // Mixed-precision tree walk (Neighbor semantics). Every step reads one 32-byte node record (two
 // 16-byte loads issued together). The walker carries the current leaf's integer indices along x, y, z
 // and its subdivision counts; its exit planes (the walls on the sides the ray heads to, each

@@ -28,8 +28,11 @@ import numpy as np
 
 from . import output
 from . import skirt as S
-from .capi import SG_MIXED, SG_NEIGHBOR, SG_TOPDOWN
+from .capi import SG_BOOKKEEPING, SG_MIXED, SG_NEIGHBOR, SG_TOPDOWN
 from .output import PC, FatalError, Units
+
+# TreeDustGrid::SearchMethod names (TreeDustGrid.hpp:47-50)
+_SEARCH = {"TopDown": SG_TOPDOWN, "Neighbor": SG_NEIGHBOR, "Bookkeeping": SG_BOOKKEEPING}
 
 
 class Item:
@@ -307,8 +310,11 @@ class Spec:
             if g.cls == "OctTreeDustGrid" and val.boolean(g, "barycentric", "no"):
                 raise FatalError("barycentric octrees are not supported")
             method = val.string(g, "searchMethod", "Neighbor")
-            if method not in ("Neighbor", "TopDown"):
+            if method not in ("Neighbor", "TopDown", "Bookkeeping"):
                 raise FatalError("search method %s is not supported" % method)
+            # BinTreeDustGrid::setupSelfBefore (BinTreeDustGrid.cpp:23-24)
+            if method == "Bookkeeping" and g.cls == "BinTreeDustGrid":
+                raise FatalError("Bookkeeping method is not compatible with binary tree")
             spec.update(minLevel=val.integer(g, "minLevel", "2"), maxLevel=val.integer(g, "maxLevel", "6"),
                         maxMassFraction=val.double(g, "maxMassFraction", None, "1e-6"),
                         sampleCount=val.integer(g, "sampleCount", "100"), search=method)
@@ -368,7 +374,7 @@ class Simulation:
             cls = S.OctTreeDustGrid if g["kind"] == "OctTreeDustGrid" else S.BinTreeDustGrid
             return cls.subdivide(ext, dgeom, scale, total_mass=scale, minlevel=g["minLevel"], maxlevel=g["maxLevel"],
                                  maxmassfraction=g["maxMassFraction"], nsamples=g["sampleCount"], seed=seed,
-                                 search=SG_NEIGHBOR if g["search"] == "Neighbor" else SG_TOPDOWN)
+                                 search=_SEARCH[g["search"]])
         n = g["numParticles"]
         if g["distribution"] == "DustDensity":
             from . import workloads

@@ -631,14 +631,17 @@ def autotune(sim, ell=0, first=0, sample=2_000_000, precision=1, choose_pipeline
     organisation when choose_pipeline, then compaction threshold, in-flight slots or waves per CU, block
     size), keeps the fastest set for later runs and returns it with the sample's rate; the tallies are left
     as they were."""
-    best = np.zeros(4, np.int64)
+    best = np.zeros(5, np.int64)
     rate = ctypes.c_double(0.0)
     check(lib().sg_autotune(sim._h, int(ell), int(first), int(sample), int(precision), int(bool(choose_pipeline)),
                             best.ctypes.data_as(capi._I64), ctypes.byref(rate)))
     _pipeline_mode[0] = int(best[3])
     key = "slots" if best[3] == 1 else "waves_per_cu"
-    return {"pipeline": int(best[3]), "block": int(best[0]), key: int(best[1]), "compact_lanes": int(best[2]),
-            "sample_packets_per_s": rate.value}
+    out = {"pipeline": int(best[3]), "block": int(best[0]), key: int(best[1]), "compact_lanes": int(best[2]),
+           "sample_packets_per_s": rate.value}
+    if best[3] == 1:
+        out["walk_order"] = int(best[4])
+    return out
 
 
 def device_name():

@@ -8,6 +8,7 @@
   the frames of FP64 runs at every wavelength against the oracle packet by packet.
 - config4's full kd-tree (1.61e7 leaves): FP64 walks bit for bit against the oracle, and the mixed rope walk
   against them.
+- The Bookkeeping search method (octrees): FP64 packets against the oracle, mixed against FP64.
 - A packet range emitted in several batches equals the same range emitted at once (both pipelines, both
   precisions).
 """
@@ -16,7 +17,7 @@ import pytest
 
 from skirt7_amd import skirt as S
 from skirt7_amd import workloads
-from skirt7_amd.capi import SG_FP64, SG_MIXED
+from skirt7_amd.capi import SG_BOOKKEEPING, SG_FP64, SG_MIXED
 from tests import oracle_lib as O
 from tests.helpers import random_rays
 from tests.test_gpu_paths import assert_paths_equal
@@ -151,3 +152,20 @@ def test_emission_in_batches_equals_one_batch(pipeline, precision):
     for a, b in zip(f_many, f_one):
         assert b.sum() > 0
         np.testing.assert_allclose(a, b, rtol=1e-12, atol=0.0)
+
+
+def test_bookkeeping_octree_transport():
+    w = workloads.spiral_octree(tau=2.0, maxlevel=6, pixels=64, packages=20000, nsamples=20, search=SG_BOOKKEEPING)
+    sc = O.scene_from_workload(w)
+    print("bookkeeping octree: 20000 packets FP64 against the oracle", flush=True)
+    assert_matches_oracle(w, sc, 0, 20000, 20000)
+    n = 200000
+    w["instr"].clear()
+    w["sim"].run_packets(0, 0, n, n, SG_FP64)
+    f64 = snapshot(w)
+    w["instr"].clear()
+    cm = w["sim"].run_packets(0, 0, n, n, SG_MIXED)
+    assert cm["stuck"] == 0
+    for a, b in zip(snapshot(w), f64):
+        assert abs(a.sum() - b.sum()) <= 1e-3 * b.sum()
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 0.05

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from skirt7_amd import skirt as S
-from skirt7_amd.capi import SG_FP64, SG_MIXED, SG_NEIGHBOR, SG_TOPDOWN
+from skirt7_amd.capi import SG_BOOKKEEPING, SG_FP64, SG_MIXED, SG_NEIGHBOR, SG_TOPDOWN
 from tests import oracle_lib as O
 from tests.helpers import EXT, random_rays, random_tree_flags, voronoi_superset
 
@@ -45,13 +45,28 @@ def test_cartesian_nonuniform_bitexact():
     assert_paths_equal(g, sc, random_rays(2000, 2, ext))
 
 
-@pytest.mark.parametrize("search", [SG_TOPDOWN, SG_NEIGHBOR])
+@pytest.mark.parametrize("search", [SG_TOPDOWN, SG_NEIGHBOR, SG_BOOKKEEPING])
 def test_octree_bitexact(search):
     flags = random_tree_flags(8, 5, 0.35, seed=11, minlevel=1)
     g = S.OctTreeDustGrid(EXT, flags, search)
     sc = O.Scene().tree(8, EXT, flags, search)
     assert g.numCells() == O.lib().or_scene_num_cells(sc.h)
     assert_paths_equal(g, sc, random_rays(3000, 3))
+
+
+def test_octree_bookkeeping_deep_and_uniform():
+    """The Bookkeeping walk (TreeDustGrid.cpp:508-640) on a deep random octree and on a uniform one, whose rays
+    run along cell walls and through edges and corners, where its own tie rules decide."""
+    for flags, seed in ((random_tree_flags(8, 7, 0.25, seed=21, minlevel=1), 61),
+                        (random_tree_flags(8, 3, 1.0, seed=22, minlevel=1), 62)):
+        g = S.OctTreeDustGrid(EXT, flags, SG_BOOKKEEPING)
+        sc = O.Scene().tree(8, EXT, flags, SG_BOOKKEEPING)
+        assert_paths_equal(g, sc, random_rays(3000, seed))
+
+
+def test_bintree_refuses_bookkeeping():
+    with pytest.raises(S.FatalError, match="Bookkeeping method is not compatible with binary tree"):
+        S.BinTreeDustGrid(EXT, random_tree_flags(2, 6, 0.5, seed=13, minlevel=2), SG_BOOKKEEPING)
 
 
 @pytest.mark.parametrize("search", [SG_TOPDOWN, SG_NEIGHBOR])

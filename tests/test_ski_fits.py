@@ -233,6 +233,19 @@ def test_ski_spec(resources):
     assert sp2.units.unit("surfacebrightness") == "W/m2/Hz/sr"
 
 
+def test_ski_tree_search_methods(resources):
+    """searchMethod TopDown, Neighbor or Bookkeeping for an octree (TreeDustGrid.hpp:47-50); Bookkeeping on a
+    binary tree is the reference's fatal error (BinTreeDustGrid.cpp:23-24)."""
+    from skirt7_amd import ski
+    for m in ("TopDown", "Neighbor", "Bookkeeping"):
+        t = ski_text(grid="octree").replace('searchMethod="Neighbor"', 'searchMethod="%s"' % m)
+        assert ski.Spec(t, resources).dust["grid"]["search"] == m
+    t = ski_text(grid="octree").replace('searchMethod="Neighbor"', 'searchMethod="Bookkeeping"')
+    t = t.replace("<OctTreeDustGrid ", "<BinTreeDustGrid ")
+    with pytest.raises(O.FatalError, match="Bookkeeping method is not compatible with binary tree"):
+        ski.Spec(t, resources)
+
+
 def test_ski_rejects_what_it_cannot_run(resources):
     from skirt7_amd import ski
     t = ski_text().replace("<SersicGeometry", "<PlummerGeometry")
