@@ -10,8 +10,9 @@
  *    throws FatalError, e.g. DustSystem.cpp:953-955; the host wrapper rethrows its own).
  *  - Pointers are host pointers; sizes are element counts. Positions in the caller's length
  *    unit, luminosities in the caller's power unit, angles in radians.
- *  - Objects are immutable after creation except for their tallies; one host thread drives
- *    one device stream (the reference's Parallel threads become device lanes).
+ *  - Objects are immutable after creation except for their tallies and the in-place updates that
+ *    pooled fit members take (sg_stars_update, sg_instruments_update, sg_grid_resample_density);
+ *    one host thread drives one device stream (the reference's Parallel threads become device lanes).
  *  - precision: SG_FP64 reproduces the reference arithmetic (cell-index sequences bit-exact);
  *    SG_MIXED runs positions, directions, optical depths and weights in FP32, with FP64
  *    wall-distance recompute near ties and FP64 tallies.
@@ -101,6 +102,11 @@ int sg_dust_clear_absorption(sg_dust* d);
 int sg_stars_create(int ncomp, const double* params, int nlambda, const double* lum, const double* sersic_s,
                     const double* sersic_M, int ntable, double emission_bias, sg_stars** out);
 int sg_stars_destroy(sg_stars* s);
+/* The same setup into an existing stellar system (same component count): FitSKIRT evaluates each individual
+ * with its own stellar parameters (AdjustableSkirtSimulation::performWith, AdjustableSkirtSimulation.cpp:119-141,
+ * rebuilds the whole simulation); here a pooled member takes the new records in place. */
+int sg_stars_update(sg_stars* s, int ncomp, const double* params, const double* lum, const double* sersic_s,
+                    const double* sersic_M, int ntable, double emission_bias);
 /* SersicFunction table helper (SersicFunction.cpp:15-68): 101 radii s and masses M. */
 int sg_sersic_table(double n, double* s101, double* M101);
 
@@ -109,6 +115,10 @@ int sg_sersic_table(double n, double* s101, double* M101);
  * (DistantInstrument.cpp:27-50, SingleFrameInstrument.cpp:26-42). */
 int sg_instruments_create(int n, const double* params, int nlambda, sg_instruments** out);
 int sg_instruments_destroy(sg_instruments* in);
+/* New parameter records for an existing instrument system, as sg_instruments_create reads them (the pixel
+ * counts stay; the frames keep their contents): a pooled fit member's viewing angles (FitSKIRT, see
+ * sg_stars_update). */
+int sg_instruments_update(sg_instruments* in, const double* params);
 /* FrameInstrument::_distftotv (FrameInstrument.cpp:33-47): instrument i's raw detected
  * luminosities, nlambda x Ny x Nx doubles ([ell][j][i] = reference's ell*Nframep + i + Nx*j). */
 int sg_instruments_read(sg_instruments* in, int i, double* frame);
@@ -165,6 +175,33 @@ int sg_gaussian_kernel(double fwhm, int dimension, double* out);
 int sg_fit_frames(int n, sg_instruments* const* instr, int instrument, int ell, const double* kernel, int kx,
                   int ky, const double* ref, double minlum, double maxlum, double* lum, double* chi2,
                   double* convolved);
+
+/* ---- FitSKIRT's genetic algorithm (Optimization) --------------------------------------------- */
+/* Optimization::setupSelfBefore (FitSKIRTcore/Optimization.cpp:42-90): GAlib's steady-state GA over one
+ * real gene per parameter range [lower[k], upper[k]], population popsize, generations generations,
+ * mutation and crossover probabilities pmut and pcross, minimising the score. seed is the GAlib seed
+ * (OligoFitScheme's fixedSeed uses 4357, Optimization.cpp:156-163; 0 takes one from the clock). The
+ * algorithm makes GAlib's random draws and choices, so a fit repeats the reference's sequence of
+ * individuals for the same seed and the same scores. Host code; it allocates no device memory. */
+typedef struct sg_ga sg_ga;
+int sg_ga_create(int ngenes, const double* lower, const double* upper, int popsize, int generations, double pmut,
+                 double pcross, uint32_t seed, sg_ga** out);
+int sg_ga_destroy(sg_ga* ga);
+/* Optimization::initialize and Optimization::step (Optimization.cpp:156-163,260-263) up to the evaluation
+ * of the population: afterwards sg_ga_pending lists the individuals to evaluate. */
+int sg_ga_initialize(sg_ga* ga);
+int sg_ga_step(sg_ga* ga);
+/* The individuals that Optimization::PopEvaluate would evaluate (Optimization.cpp:301-333), in its order:
+ * *n of them, their genes (n x ngenes doubles) into genes when it is not null. */
+int sg_ga_pending(sg_ga* ga, int* n, double* genes);
+/* Their scores (chi2, Optimization.cpp:345-357), in the same order; completes initialize or step. */
+int sg_ga_set_scores(sg_ga* ga, int n, const double* scores);
+/* Generations completed and Optimization::done (Optimization.cpp:149-152). */
+int sg_ga_state(sg_ga* ga, int* generation, int* done);
+/* The best individual found (GAStatistics::bestIndividual) and its score. */
+int sg_ga_best(sg_ga* ga, double* genes, double* score);
+/* The population (popsize x ngenes genes and popsize scores) in GAlib's population order. */
+int sg_ga_population(sg_ga* ga, double* genes, double* scores);
 
 /* Transport organisation used by sg_sim_run (both give the same per-packet histories):
  * 1 (default) alternating kernels -- an event kernel over in-flight packet slots that runs the
@@ -241,6 +278,11 @@ int sg_grid_field_density(sg_grid* g, const double* field, int nx, int ny, int n
  * (Cartesian and tree grids), times scale. */
 int sg_grid_sample_density(sg_grid* g, const double* geom16, double scale, int nsamples, uint32_t seed,
                            double* rho_out);
+/* The same sampling on the device, stored as a Cartesian grid's densities in place of the current ones
+ * (the positions are the same counter-based draws; densities agree with sg_grid_sample_density to rounding
+ * of the device's exp/log/pow). Exponential-disc geometries (spiral-decorated or not); queued on the
+ * default stream without waiting: a FitSKIRT member's new dust density without a host round trip. */
+int sg_grid_resample_density(sg_grid* g, const double* geom16, double scale, int nsamples, uint32_t seed);
 /* Voronoi neighbour lists of n sites in the box ext (VoronoiMesh::buildMesh, VoronoiMesh.cpp:297-405):
  * offsets[n+1] and at most cap ids (walls -1..-6); *nids receives the total; -5 if cap is too small. */
 int sg_voronoi_neighbors(const double ext[6], const double* sites, int64_t n, int64_t* offsets, int32_t* ids,

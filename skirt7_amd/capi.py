@@ -77,9 +77,11 @@ _SIGS = {
     "sg_stars_create": (ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, _D, _D, ctypes.c_int, ctypes.c_double,
                                        _VPP]),
     "sg_stars_destroy": (ctypes.c_int, [_VP]),
+    "sg_stars_update": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_double]),
     "sg_sersic_table": (ctypes.c_int, [ctypes.c_double, _D, _D]),
     "sg_instruments_create": (ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _VPP]),
     "sg_instruments_destroy": (ctypes.c_int, [_VP]),
+    "sg_instruments_update": (ctypes.c_int, [_VP, _D]),
     "sg_instruments_read": (ctypes.c_int, [_VP, ctypes.c_int, _D]),
     "sg_instruments_clear": (ctypes.c_int, [_VP]),
     "sg_sim_create": (ctypes.c_int, [_VP, _VP, _VP, _D, _VPP]),
@@ -97,6 +99,17 @@ _SIGS = {
                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _U8,
                                          ctypes.c_int64, _I64]),
     "sg_grid_sample_density": (ctypes.c_int, [_VP, _D, ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _D]),
+    "sg_grid_resample_density": (ctypes.c_int, [_VP, _D, ctypes.c_double, ctypes.c_int, ctypes.c_uint32]),
+    "sg_ga_create": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_uint32, _VPP]),
+    "sg_ga_destroy": (ctypes.c_int, [_VP]),
+    "sg_ga_initialize": (ctypes.c_int, [_VP]),
+    "sg_ga_step": (ctypes.c_int, [_VP]),
+    "sg_ga_pending": (ctypes.c_int, [_VP, _I32, _D]),
+    "sg_ga_set_scores": (ctypes.c_int, [_VP, ctypes.c_int, _D]),
+    "sg_ga_state": (ctypes.c_int, [_VP, _I32, _I32]),
+    "sg_ga_best": (ctypes.c_int, [_VP, _D, _D]),
+    "sg_ga_population": (ctypes.c_int, [_VP, _D, _D]),
     "sg_tree_subdivide_field": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, ctypes.c_int, ctypes.c_double, _U8, ctypes.c_int64,
                                                _I64]),

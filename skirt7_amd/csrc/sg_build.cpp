@@ -264,6 +264,14 @@ struct GeomDensity {
     }
 };
 
+void disc_density_constants(const double* geom16, double* rho0, double* tanp, double* CN)
+{
+    GeomDensity d(geom16);
+    *rho0 = d.rho0;
+    *tanp = d.tanp;
+    *CN = d.CN;
+}
+
 template <class F> static void parallel_for(int64_t n, F f)
 {
     int T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));

@@ -11,6 +11,8 @@ namespace sg {
 
 void set_error(const std::string& msg);
 double lngamma(double a);
+// Normalisation and spiral constants of an exponential-disc geometry record (rho0, tan(pitch), C_N).
+void disc_density_constants(const double* geom16, double* rho0, double* tanp, double* CN);
 struct Fail {
     int code;
 };

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -114,6 +115,52 @@ __global__ void __launch_bounds__(256) k_tau(const typename W::View g, int64_t n
         else res = interp_linlin(target, tprev, tau, sprev, s);
     }
     out[i] = (double)res;
+}
+
+// An exponential disc, optionally spiral-decorated, as the host sampler evaluates it (ExpDiskGeometry.cpp:103-114,
+// SpiralStructureGeometryDecorator.cpp:165-172): normalisation and spiral constants come from the host.
+struct DiscDev {
+    double hR, hz, Rmax, zmax, Rmin, rho0;
+    double m, tanp, R0, phi0, w, CN;
+    int spiral, N;
+};
+
+__device__ double disc_density(const DiscDev& d, double x, double y, double z)
+{
+    double R = sqrt(x * x + y * y), absz = fabs(z);
+    if (d.Rmax > 0.0 && R > d.Rmax) return 0.0;
+    if (d.zmax > 0.0 && absz > d.zmax) return 0.0;
+    if (R < d.Rmin) return 0.0;
+    double rho = d.rho0 * exp(-R / d.hR) * exp(-absz / d.hz);
+    if (d.spiral) {
+        double phi = atan2(y, x);
+        double gamma = log(R / d.R0) / d.tanp + d.phi0 + 0.5 * M_PI / d.m;
+        rho *= (1.0 - d.w) + d.w * d.CN * pow(sin(0.5 * d.m * (gamma - phi)), (double)(2 * d.N));
+    }
+    return rho;
+}
+
+// DustSystem::setSampleDensityBody (DustSystem.cpp:145-172) for a Cartesian grid: cell m's density is scale
+// times the mean of nsamples disc densities at positions in the cell's box drawn from the counter-based
+// stream (seed, m, sample) that the host sampler uses; written as the grid's FP64 and FP32 densities.
+__global__ void k_resample_cart(const double* xv, const double* yv, const double* zv, int Nx, int Ny, int Nz,
+                                DiscDev d, double scale, int nsamples, uint32_t seed, double* rho, float* rhof)
+{
+    int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (int64_t)Nx * Ny * Nz) return;
+    int i = (int)(m / ((int64_t)Nz * Ny)), j = (int)((m / Nz) % Ny), k = (int)(m % Nz);
+    double x0 = xv[i], y0 = yv[j], z0 = zv[k], x1 = xv[i + 1], y1 = yv[j + 1], z1 = zv[k + 1];
+    double sum = 0;
+    for (int q = 0; q < nsamples; q++) {
+        uint32_t c0 = (uint32_t)m, c1 = (uint32_t)((uint64_t)m >> 32), c2 = (uint32_t)q, c3 = 0x5e7u;
+        philox10(c0, c1, c2, c3, seed, 0x9d2c5680u);
+        const double s32 = 1.0 / 4294967296.0;
+        double u0 = ((double)c0 + 0.5) * s32, u1 = ((double)c1 + 0.5) * s32, u2 = ((double)c2 + 0.5) * s32;
+        sum += disc_density(d, x0 + u0 * (x1 - x0), y0 + u1 * (y1 - y0), z0 + u2 * (z1 - z0));
+    }
+    double r = scale * sum / nsamples;
+    rho[m] = r;
+    rhof[m] = (float)r;
 }
 
 __global__ void k_whichcell_cart(const CartView g, int64_t n, const double* r, int32_t* out)
@@ -742,6 +789,27 @@ int sg_dust_create(sg_grid* g, const double* rho, int64_t ncells, int nlambda, c
     SG_CATCH
 }
 
+int sg_grid_resample_density(sg_grid* g, const double* geom16, double scale, int nsamples, uint32_t seed)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(g && geom16, "null grid or geometry record");
+    SG_REQUIRE(g->kind == GK_CART, "device density sampling needs a Cartesian grid (tree and Voronoi grids take "
+                                   "their structure from the density: a new density means a new grid)");
+    SG_REQUIRE((int)geom16[0] == 1 && geom16[1] > 0 && geom16[2] > 0, "the dust geometry must be an exponential disc");
+    SG_REQUIRE(nsamples >= 1 && scale >= 0 && std::isfinite(scale), "need at least one sample and a finite scale");
+    DiscDev d{};
+    d.hR = geom16[1]; d.hz = geom16[2]; d.Rmax = geom16[3]; d.zmax = geom16[4]; d.Rmin = geom16[5];
+    d.spiral = geom16[6] != 0;
+    d.m = geom16[7]; d.R0 = geom16[9]; d.phi0 = geom16[10]; d.w = geom16[11]; d.N = (int)geom16[12];
+    disc_density_constants(geom16, &d.rho0, &d.tanp, &d.CN);
+    const CartView& c = g->cart;
+    hipLaunchKernelGGL(k_resample_cart, dim3(blocks_for(g->ncells, 256)), dim3(256), 0, 0, c.xv, c.yv, c.zv, c.Nx,
+                       c.Ny, c.Nz, d, scale, nsamples, seed, g->rho.as<double>(), g->rhof.as<float>());
+    SG_HIP(hipGetLastError());
+    SG_CATCH
+}
+
 int sg_dust_destroy(sg_dust* d)
 {
     delete d;
@@ -824,36 +892,56 @@ int sg_dust_clear_absorption(sg_dust* d)
 
 // ---------------------------------------------------------------- stars
 
+// StellarSystem and its components' setup from the parameter records (the Sersic tables to the device).
+static void stars_fill(sg_stars* s, int ncomp, const double* params, const double* lum, const double* sersic_s,
+                       const double* sersic_M, int ntable, double emission_bias)
+{
+    SG_REQUIRE(ncomp >= 1 && ncomp <= SG_MAX_STARS, "between 1 and 4 stellar components are supported");
+    SG_REQUIRE(emission_bias >= 0 && emission_bias <= 1, "emission bias must be in [0,1]");
+    for (int c = 0; c < ncomp; c++) {
+        int kind = (int)params[c * SG_STAR_NPARAM];
+        SG_REQUIRE(kind == 1 || kind == 2, "stellar geometry must be 1 (exponential disc) or 2 (Sersic)");
+        if (kind == 2) SG_REQUIRE(sersic_s && sersic_M && ntable >= 2, "Sersic component needs its mass table");
+    }
+    s->params.assign(params, params + ncomp * SG_STAR_NPARAM);
+    s->lum.assign(lum, lum + (size_t)ncomp * s->nlambda);
+    s->bias = emission_bias;
+    while ((int)s->tables.size() < ncomp) s->tables.push_back(new DevBuf());
+    int nser = 0;
+    for (int c = 0; c < ncomp; c++) {
+        if ((int)params[c * SG_STAR_NPARAM] != 2) continue;
+        std::vector<double> both;
+        both.insert(both.end(), sersic_s + nser * ntable, sersic_s + (nser + 1) * ntable);
+        both.insert(both.end(), sersic_M + nser * ntable, sersic_M + (nser + 1) * ntable);
+        DevBuf* tb = s->tables[c];
+        if (tb->bytes == both.size() * sizeof(double))
+            SG_HIP(hipMemcpy(tb->p, both.data(), tb->bytes, hipMemcpyHostToDevice));
+        else
+            tb->upload(both);
+        nser++;
+    }
+    s->ts.resize(ncomp);
+}
+
 int sg_stars_create(int ncomp, const double* params, int nlambda, const double* lum, const double* sersic_s,
                     const double* sersic_M, int ntable, double emission_bias, sg_stars** out)
 {
     SG_TRY
     ensure_init();
-    SG_REQUIRE(ncomp >= 1 && ncomp <= SG_MAX_STARS, "between 1 and 4 stellar components are supported");
-    SG_REQUIRE(emission_bias >= 0 && emission_bias <= 1, "emission bias must be in [0,1]");
-    auto* s = new sg_stars();
+    std::unique_ptr<sg_stars> s(new sg_stars());
     s->nlambda = nlambda;
-    s->params.assign(params, params + ncomp * SG_STAR_NPARAM);
-    s->lum.assign(lum, lum + (size_t)ncomp * nlambda);
-    s->bias = emission_bias;
-    int nser = 0;
-    for (int c = 0; c < ncomp; c++) {
-        int kind = (int)params[c * SG_STAR_NPARAM];
-        SG_REQUIRE(kind == 1 || kind == 2, "stellar geometry must be 1 (exponential disc) or 2 (Sersic)");
-        auto* tb = new DevBuf();
-        s->tables.push_back(tb);
-        if (kind == 2) {
-            SG_REQUIRE(sersic_s && sersic_M && ntable >= 2, "Sersic component needs its mass table");
-            std::vector<double> both;
-            both.insert(both.end(), sersic_s + nser * ntable, sersic_s + (nser + 1) * ntable);
-            both.insert(both.end(), sersic_M + nser * ntable, sersic_M + (nser + 1) * ntable);
-            tb->upload(both);
-            nser++;
-        }
-    }
-    s->ts.resize(ncomp);
-    *out = s;
-    (void)ntable;
+    stars_fill(s.get(), ncomp, params, lum, sersic_s, sersic_M, ntable, emission_bias);
+    *out = s.release();
+    SG_CATCH
+}
+
+int sg_stars_update(sg_stars* s, int ncomp, const double* params, const double* lum, const double* sersic_s,
+                    const double* sersic_M, int ntable, double emission_bias)
+{
+    SG_TRY
+    SG_REQUIRE(s && params && lum, "null stellar system or parameters");
+    SG_REQUIRE(ncomp == (int)(s->params.size() / SG_STAR_NPARAM), "a stellar system update keeps the number of components");
+    stars_fill(s, ncomp, params, lum, sersic_s, sersic_M, ntable, emission_bias);
     SG_CATCH
 }
 
@@ -865,6 +953,29 @@ int sg_stars_destroy(sg_stars* s)
 
 // ---------------------------------------------------------------- instruments
 
+// DistantInstrument::setupSelfBefore (DistantInstrument.cpp:27-50) and SingleFrameInstrument::setupSelfBefore
+// (SingleFrameInstrument.cpp:26-42) for one parameter record; the frame pointer is left null.
+static InstrDev instrument_setup(const double* p)
+{
+    InstrDev d{};
+    double incl = p[1], azim = p[2], pa = p[3];
+    d.costheta = std::cos(incl); d.sintheta = std::sin(incl);
+    d.cosphi = std::cos(azim); d.sinphi = std::sin(azim);
+    d.cospa = std::cos(pa); d.sinpa = std::sin(pa);
+    const double eps = 1e-8;
+    if (incl <= eps) { d.kobs[0] = 0; d.kobs[1] = 0; d.kobs[2] = 1; }
+    else if (incl >= M_PI - eps) { d.kobs[0] = 0; d.kobs[1] = 0; d.kobs[2] = -1; }
+    else {
+        double st = std::sin(incl);
+        d.kobs[0] = st * std::cos(azim); d.kobs[1] = st * std::sin(azim); d.kobs[2] = std::cos(incl);
+    }
+    d.Nx = (int)p[4]; d.Ny = (int)p[5];
+    SG_REQUIRE(d.Nx > 0 && d.Ny > 0 && p[6] > 0 && p[7] > 0, "instrument needs pixels and a field of view");
+    d.xpmin = p[8] - 0.5 * p[6]; d.xpsiz = p[6] / d.Nx;
+    d.ypmin = p[9] - 0.5 * p[7]; d.ypsiz = p[7] / d.Ny;
+    return d;
+}
+
 int sg_instruments_create(int n, const double* params, int nlambda, sg_instruments** out)
 {
     SG_TRY
@@ -874,25 +985,7 @@ int sg_instruments_create(int n, const double* params, int nlambda, sg_instrumen
     in->n = n;
     in->nlambda = nlambda;
     for (int i = 0; i < n; i++) {
-        const double* p = params + i * SG_INSTR_NPARAM;
-        InstrDev d{};
-        double incl = p[1], azim = p[2], pa = p[3];
-        // DistantInstrument::setupSelfBefore (DistantInstrument.cpp:27-50)
-        d.costheta = std::cos(incl); d.sintheta = std::sin(incl);
-        d.cosphi = std::cos(azim); d.sinphi = std::sin(azim);
-        d.cospa = std::cos(pa); d.sinpa = std::sin(pa);
-        const double eps = 1e-8;
-        if (incl <= eps) { d.kobs[0] = 0; d.kobs[1] = 0; d.kobs[2] = 1; }
-        else if (incl >= M_PI - eps) { d.kobs[0] = 0; d.kobs[1] = 0; d.kobs[2] = -1; }
-        else {
-            double st = std::sin(incl);
-            d.kobs[0] = st * std::cos(azim); d.kobs[1] = st * std::sin(azim); d.kobs[2] = std::cos(incl);
-        }
-        // SingleFrameInstrument::setupSelfBefore (SingleFrameInstrument.cpp:26-42)
-        d.Nx = (int)p[4]; d.Ny = (int)p[5];
-        SG_REQUIRE(d.Nx > 0 && d.Ny > 0 && p[6] > 0 && p[7] > 0, "instrument needs pixels and a field of view");
-        d.xpmin = p[8] - 0.5 * p[6]; d.xpsiz = p[6] / d.Nx;
-        d.ypmin = p[9] - 0.5 * p[7]; d.ypsiz = p[7] / d.Ny;
+        InstrDev d = instrument_setup(params + i * SG_INSTR_NPARAM);
         auto* f = new DevBuf();
         f->alloc((size_t)nlambda * d.Nx * d.Ny * sizeof(double));
         f->zero();
@@ -901,6 +994,21 @@ int sg_instruments_create(int n, const double* params, int nlambda, sg_instrumen
         in->dev.push_back(d);
     }
     *out = in;
+    SG_CATCH
+}
+
+int sg_instruments_update(sg_instruments* in, const double* params)
+{
+    SG_TRY
+    SG_REQUIRE(in && params, "null instrument system or parameters");
+    std::vector<InstrDev> next(in->n);
+    for (int i = 0; i < in->n; i++) {
+        next[i] = instrument_setup(params + i * SG_INSTR_NPARAM);
+        SG_REQUIRE(next[i].Nx == in->dev[i].Nx && next[i].Ny == in->dev[i].Ny,
+                   "an instrument update keeps the number of pixels");
+        next[i].frame = in->dev[i].frame;
+    }
+    in->dev = next;
     SG_CATCH
 }
 

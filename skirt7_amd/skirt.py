@@ -366,6 +366,11 @@ def sample_density(grid, geometry, scale, nsamples=100, seed=4357):
     return out
 
 
+def resample_density(grid, geometry, scale, nsamples=100, seed=4357):
+    """The same sampling on the device, stored as a Cartesian grid's densities (no host round trip)."""
+    check(lib().sg_grid_resample_density(grid._h, dptr(as_f64(geometry.params())), scale, nsamples, seed))
+
+
 def field_density(grid, field):
     """Cell densities from a voxel density field: mass in the cell's box over its volume."""
     f = as_f64(field)
@@ -402,18 +407,32 @@ class StellarSystem:
         self.components = list(components)
         self.emissionBias = emissionBias
         nl = len(self.components[0].luminosities)
-        params = np.concatenate([c.geometry.params() for c in self.components])
-        lum = np.concatenate([c.luminosities for c in self.components])
-        ts = [c.geometry.s for c in self.components if getattr(c.geometry, "s", None) is not None]
-        tM = [c.geometry.M for c in self.components if getattr(c.geometry, "M", None) is not None]
-        s_arr = as_f64(np.concatenate(ts)) if ts else np.zeros(1)
-        M_arr = as_f64(np.concatenate(tM)) if tM else np.zeros(1)
-        self._keep = (params, lum, s_arr, M_arr)
+        params, lum, s_arr, M_arr = self._records(self.components)
         h = _handle()
-        check(lib().sg_stars_create(len(self.components), dptr(as_f64(params)), nl, dptr(as_f64(lum)),
-                                    dptr(s_arr), dptr(M_arr), 101, emissionBias, ctypes.byref(h)))
+        check(lib().sg_stars_create(len(self.components), dptr(params), nl, dptr(lum), dptr(s_arr), dptr(M_arr),
+                                    101, emissionBias, ctypes.byref(h)))
         self._h = h
         self.nlambda = nl
+
+    @staticmethod
+    def _records(components):
+        params = as_f64(np.concatenate([c.geometry.params() for c in components]))
+        lum = as_f64(np.concatenate([c.luminosities for c in components]))
+        ts = [c.geometry.s for c in components if getattr(c.geometry, "s", None) is not None]
+        tM = [c.geometry.M for c in components if getattr(c.geometry, "M", None) is not None]
+        s_arr = as_f64(np.concatenate(ts)) if ts else np.zeros(1)
+        M_arr = as_f64(np.concatenate(tM)) if tM else np.zeros(1)
+        return params, lum, s_arr, M_arr
+
+    def update(self, components, emissionBias=None):
+        """New components (same count and wavelengths) in place of the current ones."""
+        components = list(components)
+        if emissionBias is not None:
+            self.emissionBias = emissionBias
+        params, lum, s_arr, M_arr = self._records(components)
+        check(lib().sg_stars_update(self._h, len(components), dptr(params), dptr(lum), dptr(s_arr), dptr(M_arr),
+                                    101, self.emissionBias))
+        self.components = components
 
     def luminosity(self, ell):
         return sum(c.luminosities[ell] for c in self.components)
@@ -456,6 +475,15 @@ class InstrumentSystem:
 
     def clear(self):
         check(lib().sg_instruments_clear(self._h))
+
+    def update(self, instruments):
+        """New viewing parameters (same pixel counts) in place of the current ones; frames keep their tallies."""
+        instruments = list(instruments)
+        if len(instruments) != len(self.instruments):
+            raise FatalError("an instrument update keeps the number of instruments")
+        params = as_f64(np.concatenate([i.p for i in instruments])) if instruments else np.zeros(1)
+        check(lib().sg_instruments_update(self._h, dptr(params)))
+        self.instruments = instruments
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and capi._lib is not None:
