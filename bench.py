@@ -46,11 +46,14 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="spiral_octree",
-                    choices=("disc_cartesian", "spiral_octree", "spiral_voronoi", "fit_population", "lognormal_kdtree"),
+                    choices=("disc_cartesian", "spiral_octree", "spiral_voronoi", "fit_population", "lognormal_kdtree",
+                             "fitskirt"),
                     help="spiral_octree: octree over a spiral galaxy (the metric's workload); disc_cartesian: plain "
                          "Cartesian 50^3 (the CPU-runnable case); spiral_voronoi: 1e6-site Voronoi grid with the "
                          "absorption table; fit_population: batched population + fit; lognormal_kdtree: kd-tree over a "
-                         "log-normal field")
+                         "log-normal field; fitskirt: a FitSKIRT run (a step is one generation of the genetic "
+                         "algorithm, evaluated in batched launches)")
+    ap.add_argument("--popsize", type=int, default=100, help="fitskirt: population size (FitSKIRT's default)")
     ap.add_argument("--members", type=int, default=256, help="fit_population: population size")
     ap.add_argument("--one-wavelength", action="store_true",
                     help="spiral_voronoi: run only the 0.55 micron wavelength instead of all 50 per step")
@@ -234,6 +237,70 @@ def population(a):
     print(json.dumps(out), flush=True)
 
 
+def fitskirt(a):
+    """fitskirt: OligoFitScheme::runSelf (OligoFitScheme.cpp:27-34) on the GPU. The population's first
+    evaluation and `warmup` generations are untimed; each timed step is one generation: the genetic algorithm's
+    step, the new individuals' adjusted ski files parsed and taken by pooled members in place, one batched
+    transport launch and the PSF-convolved golden-section fit of every member."""
+    import tempfile
+
+    from skirt7_amd import fitskirt as F
+    from skirt7_amd import skirt as S
+    from skirt7_amd import workloads
+    from skirt7_amd.capi import SG_MIXED
+    res = tempfile.mkdtemp()
+    os.makedirs(os.path.join(res, "SED", "Sun"))
+    with open(os.path.join(res, "SED", "Sun", "SunSED.dat"), "w") as f:
+        f.write(workloads.SUN_STANDIN)
+    packets = a.packets or 1e6
+    prob = workloads.fitskirt_edgeon(packages=packets)
+    t0 = time.perf_counter()
+    data, kernel, gain = workloads.fitskirt_observation(prob, res)
+    log("observation built in %.1f s on %s" % (time.perf_counter() - t0, S.device_name()))
+    sim = F.AdjustableSkirtSimulation(prob["ski"], resources=res, precision=SG_MIXED)
+    optim = F.Optimization(popsize=a.popsize, generations=a.warmup + a.steps, pmut=0.03, pcross=0.65)
+    image = F.ReferenceImage(data, kernel, [0.2 * gain], [5.0 * gain])
+    scheme = F.OligoFitScheme(sim, prob["ranges"], [image], optim, fixedSeed=True)
+    optim.setup(scheme.ranges, 4357)
+    t0 = time.perf_counter()
+    optim.initialize()
+    scheme._evaluate_pending()
+    log("initial population of %d evaluated in %.2f s (pool built)" % (a.popsize, time.perf_counter() - t0))
+    for _ in range(a.warmup):
+        optim.step()
+        scheme._evaluate_pending()
+    sim.timing.update(prepare=0.0, transport=0.0, fit=0.0, packets=0)
+    n0 = len(scheme.records)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        tg = time.perf_counter()
+        optim.step()
+        scheme._evaluate_pending()
+        log("generation %d: %d individuals, %.3f s, best chi2 %.6g" % (optim.generation(), len(scheme.records) -
+            n0, time.perf_counter() - tg, optim.best()[1]))
+    wall = time.perf_counter() - t0
+    tm = sim.timing
+    individuals = len(scheme.records) - n0
+    best, score = optim.best()
+    out = {"metric": "packets/s on 1 MI355X, mixed vs FP64 GPU at equal image error; % HBM peak",
+           "value": tm["packets"] / wall, "unit": "packets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 (walk, weights) + f64 (wall fallback, tallies)", "data": "synthetic",
+           "config": {"workload": "FitSKIRT run: population %d, %d generations timed, %g packets per individual, "
+                                  "Cartesian %dx%dx%d, %dx%d frame, 6 parameters" % (a.popsize, a.steps, packets, 100,
+                                                                                       100, 30, 256, 128),
+                      "popsize": a.popsize, "packets_per_individual": packets},
+           "individuals_per_s": individuals / wall,
+           "seconds_per_generation": {"total": wall / a.steps, "host_prepare": tm["prepare"] / a.steps,
+                                      "transport": tm["transport"] / a.steps, "fit": tm["fit"] / a.steps,
+                                      "ga_and_other": (wall - tm["prepare"] - tm["transport"] - tm["fit"]) / a.steps},
+           "pool_rebuilds": sim.rebuilt,
+           "best": {"chi2": score, "values": {r.label: (v if not r.quantityString() else
+                                                        sim.units.out(r.quantityString(), v))
+                                              for r, v in zip(scheme.ranges, best)}}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -246,6 +313,9 @@ def main():
 
     if a.workload == "fit_population":
         population(a)
+        return
+    if a.workload == "fitskirt":
+        fitskirt(a)
         return
     if a.copy_calibration > 0:
         copy_calibration(a.copy_calibration)

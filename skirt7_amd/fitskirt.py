@@ -26,7 +26,7 @@ frames, raise FatalError.
 """
 import ctypes
 import math
-import os
+import time
 
 import numpy as np
 
@@ -261,6 +261,9 @@ class AdjustableSkirtSimulation:
                              "component is fitted by golden section" % len(base.stellar))
         self.pool = []
         self.rebuilt = 0
+        # wall seconds of the evaluations: host preparation (parsing, in-place updates, rebuilds) and the
+        # batched transport launches; packets run
+        self.timing = dict(prepare=0.0, transport=0.0, packets=0)
 
     def ncomponents(self):
         return 1
@@ -275,6 +278,7 @@ class AdjustableSkirtSimulation:
     def evaluate(self, specs):
         """Runs every individual's simulation: members updated in place, one batched launch per wavelength.
         Returns the members holding the individuals' frames, in order."""
+        t0 = time.perf_counter()
         for i, sp in enumerate(specs):
             if len(sp.stellar) != 1:
                 raise FatalError("an individual's ski file has %d stellar components; one is supported"
@@ -288,10 +292,14 @@ class AdjustableSkirtSimulation:
         sims = [m.sim for m in members]
         for m in sims:
             m.instr.clear()
+        t1 = time.perf_counter()
         nl = specs[0].nlambda
         for ell in range(nl):
             counts = [int(m.sim.setChunkParams(1)) for m in sims]
             S.run_batch([m.sim for m in sims], counts, ells=[ell] * len(sims), precision=self.precision)
+            self.timing["packets"] += sum(counts)
+        self.timing["prepare"] += t1 - t0
+        self.timing["transport"] += time.perf_counter() - t1
         return members
 
 
@@ -350,8 +358,11 @@ class OligoFitScheme:
         if len(genes) == 0:
             return np.zeros(0), np.zeros((0, len(self.images))), np.zeros((0, len(self.images)))
         sim = self.simulation
+        t0 = time.perf_counter()
         specs = sim.specs([self._replacements(g) for g in genes])
+        sim.timing["prepare"] += time.perf_counter() - t0
         members = sim.evaluate(specs)
+        t0 = time.perf_counter()
         nimg = len(self.images)
         lums, chis = np.zeros((len(genes), nimg)), np.zeros((len(genes), nimg))
         for k, ref in enumerate(self.images):
@@ -371,6 +382,7 @@ class OligoFitScheme:
                 lum, chi2 = S.fit_frames([members[i].sim.instr for i in sel], ref.data, ref.kernel, lo, hi,
                                          instrument=sim.instrument, ell=k)
                 lums[sel, k], chis[sel, k] = lum / c, chi2
+        sim.timing["fit"] = sim.timing.get("fit", 0.0) + time.perf_counter() - t0
         return chis.sum(axis=1), lums, chis
 
     def _evaluate_pending(self):

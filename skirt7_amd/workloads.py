@@ -200,3 +200,110 @@ def lognormal_kdtree(nvox=256, maxmassfraction=9e-8, maxlevel=40, pixels=1024, p
     d = dict(name="lognormal_kdtree", kind="bintree", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=None,
              instr=S.InstrumentSystem(ins), field=field)
     return _finish(d, packages, seed)
+
+
+# a two-segment stand-in for the solar SED (lambda in micron, L in W/micron): OligoStellarComp luminosities are
+# in solar units, and the fit's luminosity scale absorbs the conversion, so any positive SED serves
+SUN_STANDIN = """# a two-segment stand-in for the solar SED (lambda in micron, L in W/micron)
+3
+0.4  1.0e26
+0.6  2.0e26
+1.0  1.0e26
+"""
+
+FIT_SKI = """<?xml version="1.0" encoding="UTF-8"?>
+<skirt-simulation-hierarchy type="MonteCarloSimulation" format="6.1">
+  <OligoMonteCarloSimulation packages="[packages:%(packages)g]" minWeightReduction="1e4" scattBias="0.5">
+    <random type="Random"><Random seed="4357"/></random>
+    <units type="Units"><ExtragalacticUnits fluxOutputStyle="Neutral"/></units>
+    <instrumentSystem type="InstrumentSystem">
+      <InstrumentSystem>
+        <instruments type="Instrument">
+          <FrameInstrument instrumentName="edgeon" distance="10 Mpc" inclination="[incl:88 deg]" azimuth="0 deg"
+             positionAngle="0 deg" fieldOfViewX="40 kpc" pixelsX="%(nx)d" fieldOfViewY="%(fovy)g kpc" pixelsY="%(ny)d"/>
+        </instruments>
+      </InstrumentSystem>
+    </instrumentSystem>
+    <wavelengthGrid type="OligoWavelengthGrid"><OligoWavelengthGrid wavelengths="0.55 micron"/></wavelengthGrid>
+    <stellarSystem type="StellarSystem">
+      <StellarSystem emissionBias="0.5">
+        <components type="StellarComp">
+          <OligoStellarComp luminosities="1e10">
+            <geometry type="Geometry"><ExpDiskGeometry radialScale="[stellar_hr:4000 pc]" axialScale="[stellar_hz:400 pc]"/></geometry>
+          </OligoStellarComp>
+        </components>
+      </StellarSystem>
+    </stellarSystem>
+    <dustSystem type="DustSystem">
+      <OligoDustSystem sampleCount="%(nsamples)d" writeMeanIntensity="false">
+        <dustDistribution type="DustDistribution">
+          <CompDustDistribution>
+            <components type="DustComp">
+              <DustComp>
+                <geometry type="Geometry"><ExpDiskGeometry radialScale="[dust_hr:6000 pc]" axialScale="[dust_hz:250 pc]"/></geometry>
+                <mix type="DustMix"><SimpleOligoDustMix opacities="1 m2/kg" albedos="0.6" asymmetryParameters="0.6"/></mix>
+                <normalization type="DustCompNormalization">
+                  <FaceOnDustCompNormalization wavelength="0.55 micron" opticalDepth="[tau:1]"/>
+                </normalization>
+              </DustComp>
+            </components>
+          </CompDustDistribution>
+        </dustDistribution>
+        <dustGrid type="DustGrid">
+          <CartesianDustGrid minX="-20 kpc" maxX="20 kpc" minY="-20 kpc" maxY="20 kpc" minZ="-5 kpc" maxZ="5 kpc">
+            <meshX type="MoveableMesh"><LinMesh numBins="%(gx)d"/></meshX>
+            <meshY type="MoveableMesh"><LinMesh numBins="%(gx)d"/></meshY>
+            <meshZ type="MoveableMesh"><LinMesh numBins="%(gz)d"/></meshZ>
+          </CartesianDustGrid>
+        </dustGrid>
+      </OligoDustSystem>
+    </dustSystem>
+  </OligoMonteCarloSimulation>
+</skirt-simulation-hierarchy>
+"""
+
+
+def fitskirt_edgeon(packages=1e6, pixels=(256, 128), grid=(100, 30), nsamples=8):
+    """A FitSKIRT problem: an edge-on disc galaxy (stellar disc with a dust disc, Cartesian grid of
+    grid[0]^2 x grid[1] cells over 40 x 40 x 10 kpc) whose ski file has placeholders for six parameters (the
+    ranges of FitSKIRT's edge-on tutorial kind: stellar and dust scale lengths and heights, face-on optical
+    depth, inclination), the parameter ranges and the true values of a synthetic observation. Lengths of the
+    ranges in SI (m) and the inclination in radians, as FitSKIRT's ParameterRange holds them."""
+    from . import fitskirt as F
+    from .output import PC
+    text = FIT_SKI % dict(packages=packages, nx=pixels[0], ny=pixels[1], fovy=40.0 * pixels[1] / pixels[0],
+                          nsamples=nsamples, gx=grid[0], gz=grid[1])
+    ranges = [F.ParameterRange("stellar_hr", "length", 2500 * PC, 5500 * PC),
+              F.ParameterRange("stellar_hz", "length", 200 * PC, 700 * PC),
+              F.ParameterRange("dust_hr", "length", 3000 * PC, 9000 * PC),
+              F.ParameterRange("dust_hz", "length", 100 * PC, 500 * PC),
+              F.ParameterRange("tau", "", 0.2, 3.0),
+              F.ParameterRange("incl", "posangle", math.radians(85.0), math.radians(90.0))]
+    truth = [3800 * PC, 420 * PC, 6500 * PC, 240 * PC, 1.2, math.radians(87.5)]
+    return dict(ski=text, ranges=ranges, truth=truth)
+
+
+def fitskirt_observation(problem, resources, counts=1e6, mult=10, seed=5):
+    """The synthetic observation of fitskirt_edgeon's truth: its frame with `mult` times the packets, calibrated,
+    convolved with the 3-pixel FWHM Gaussian PSF and Poisson-noised at `counts` counts in all (the chi2's
+    sigma is Poisson). Returns (reference image, PSF kernel, gain)."""
+    from scipy.ndimage import convolve
+
+    from . import fitskirt as F
+    from . import output, ski
+    from .capi import SG_MIXED
+    units = F._units_of(problem["ski"])
+    repl = {r.label: (float(v), r.quantityString()) for r, v in zip(problem["ranges"], problem["truth"])}
+    spec0 = ski.Spec(F.adjusted_ski_content(problem["ski"], repl, units), resources)
+    repl["packages"] = (spec0.packages * mult, "")
+    spec = ski.Spec(F.adjusted_ski_content(problem["ski"], repl, units), resources)
+    sim = ski.Simulation(spec)
+    sim.run(SG_MIXED)
+    f = spec.frames[0]
+    cal = output.calibrate_frame(sim.instr.frame(0), spec.lambdas, spec.dlambdas, f["distance"],
+                                 f["fovX"] / f["pixelsX"], f["fovY"] / f["pixelsY"], spec.units)[0]
+    kernel = S.GaussianKernel(3.0, 6)
+    cal = convolve(cal, kernel.data, mode="constant")
+    gain = counts / cal.sum()
+    data = np.random.default_rng(seed).poisson(cal * gain).astype(np.float64)
+    return data, kernel, gain
