@@ -79,6 +79,17 @@ int sg_grid_paths(sg_grid* g, int64_t n, const double* rk, int max_seg, int prec
 int sg_dust_create(sg_grid* g, const double* rho, int64_t ncells, int nlambda, const double* kappaext,
                    const double* albedo, const double* asymmpar, int store_absorption, sg_dust** out);
 int sg_dust_destroy(sg_dust* d);
+/* A dust system of ncomp components (DustSystem with Ncomp > 1, DustSystem.hpp:412; CompDustDistribution):
+ * rho holds each component's cell densities (ncomp x ncells, [h][m]); kappaext, kappasca, albedo and
+ * asymmpar are ncomp x nlambda ([h][ell]) as each component's DustMix derives them (DustMix.cpp:44-91). Walks
+ * see KappaRho, the sum over components of kappaext(h) rho(m, h) (DustSystem.cpp:459-485); escape and
+ * absorption use each cell's own albedo (MonteCarloSimulation.cpp:463-508); scattering peel-offs weight the
+ * components' phase functions by kappasca(h) rho(m, h) (:318-343) and scattering draws the component with
+ * those weights (DustSystem::randomMixForPosition, DustSystem.cpp:855-869). Such systems run in the lane
+ * pipeline (sg_sim_run and batched runs; sg_sim_run_static refuses them). ncomp == 1 is sg_dust_create. */
+int sg_dust_create_multi(sg_grid* g, int ncomp, const double* rho, int64_t ncells, int nlambda, const double* kappaext,
+                         const double* kappasca, const double* albedo, const double* asymmpar, int store_absorption,
+                         sg_dust** out);
 /* DustSystem::opticaldepth (DustSystem.cpp:960-976) along n rays at wavelength ell. */
 int sg_dust_optical_depths(sg_dust* d, int ell, int64_t n, const double* rk, int precision, double* tau);
 /* DustGridPath::pathlength after fillOpticalDepth (DustGridPath.cpp:162-173): for ray i and
@@ -265,6 +276,11 @@ int sg_geometry_density(const double* geom16, int64_t n, const double* xyz, doub
 int sg_tree_subdivide(int arity, const double ext[6], const double* geom16, double scale, double total_mass,
                       int minlevel, int maxlevel, double maxmassfraction, int nsamples, uint32_t seed,
                       uint8_t* flags, int64_t cap, int64_t* nnodes);
+/* The same for a CompDustDistribution of ncomp components: the density is the sum over components of
+ * scales[h] times geometry h's density (geoms16 holds ncomp records of 16), CompDustDistribution::density. */
+int sg_tree_subdivide_multi(int arity, const double ext[6], int ncomp, const double* geoms16, const double* scales,
+                            double total_mass, int minlevel, int maxlevel, double maxmassfraction, int nsamples,
+                            uint32_t seed, uint8_t* flags, int64_t cap, int64_t* nnodes);
 /* The same subdivision with node masses from a density field given per voxel of a regular
  * nx x ny x nz grid over ext (value [i][j][k] at i*ny*nz + j*nz + k, constant inside a voxel): exact mass in
  * each node's box, the total being the field's mass. */

@@ -997,6 +997,10 @@ struct Scene {
     std::vector<double> rho;
     int nlambda = 1;
     std::vector<double> kext, alb, g;
+    // several dust components (DustSystem with Ncomp > 1): densities [h][m] and mixes [h][ell]
+    int ncomp = 1;
+    std::vector<double> rhoc, kextc, kscac, gc;
+    double density(int m, int h) const { return rhoc[(size_t)h * rho.size() + m]; }
     bool store_abs = false;
     bool has_dust = false;
     std::vector<Star> stars;
@@ -1156,6 +1160,67 @@ inline int64_t count_cells(const Path& p)
 }
 
 // FrameInstrument.cpp:33-47 with Instrument.cpp:69-72 and DustSystem.cpp:960-976
+// KappaRho (DustSystem.cpp:459-485): the sum over components of kappaext(h) rho(m, h), from zero in
+// component order
+double kappa_rho(const Scene& sc, int ell, int m)
+{
+    double res = 0;
+    if (m < 0) return res;
+    if (sc.ncomp == 1) res += sc.kext[ell] * sc.rho[m];
+    else
+        for (int h = 0; h < sc.ncomp; h++) res += sc.kextc[(size_t)h * sc.nlambda + ell] * sc.density(m, h);
+    return res;
+}
+
+// MonteCarloSimulation::peeloffscattering's phase-function value (MonteCarloSimulation.cpp:313-343) for a
+// packet with direction k scattering in cell m towards kobs: one component's Henyey-Greenstein value
+// (DustMix.cpp:648-653), or the components' values weighted by their normalised kappasca(h) rho(m, h). False
+// when the peel-off is aborted (several components, and m outside the grid or without scattering dust).
+bool scatter_phase(const Scene& sc, int ell, int m, const double k[3], const double kobs[3], double& I)
+{
+    const int H = sc.ncomp;
+    double cosalpha = k[0] * kobs[0] + k[1] * kobs[1] + k[2] * kobs[2];
+    I = 0;
+    if (H == 1) {
+        const double g = sc.g[ell];
+        double tt = 1.0 + g * g - 2 * g * cosalpha;
+        I = (1.0 - g) * (1.0 + g) / std::sqrt(tt * tt * tt);
+        return true;
+    }
+    if (m == -1) return false;
+    double wv[8], sum = 0;
+    for (int h = 0; h < H; h++) wv[h] = sc.kscac[(size_t)h * sc.nlambda + ell] * sc.density(m, h);
+    for (int h = 0; h < H; h++) sum += wv[h];
+    if (sum <= 0) return false;
+    for (int h = 0; h < H; h++) wv[h] /= sum;
+    for (int h = 0; h < H; h++) {
+        double gh = sc.gc[(size_t)h * sc.nlambda + ell];
+        double tt = 1.0 + gh * gh - 2 * gh * cosalpha;
+        I += wv[h] * ((1.0 - gh) * (1.0 + gh) / std::sqrt(tt * tt * tt));
+    }
+    return true;
+}
+
+// DustSystem::randomMixForPosition (DustSystem.cpp:855-869) for cell m and uniform deviate u: NR::cdf of
+// kappasca(h) rho(m, h), then NR::locate_clip; component 0 outside the grid.
+int random_mix_u(const Scene& sc, int ell, int m, double u)
+{
+    const int H = sc.ncomp;
+    double X[9];
+    X[0] = 0;
+    for (int h = 0; h < H; h++) X[h + 1] = X[h] + sc.kscac[(size_t)h * sc.nlambda + ell] * sc.density(m, h);
+    const double total = X[H];
+    for (int h = 0; h <= H; h++) X[h] /= total;
+    return nr_locate_clip(X, H + 1, u);
+}
+
+int random_mix(const Scene& sc, int ell, int m, Rng& rng)
+{
+    if (m < 0) return 0;
+    return random_mix_u(sc, ell, m, rng.uniform());
+}
+
+
 void detect(const Scene& sc, size_t i, const double r[3], double L, int ell, Path& path, Tally& t, Counters& c)
 {
     const Instrument& in = sc.instr[i];
@@ -1168,9 +1233,7 @@ void detect(const Scene& sc, size_t i, const double r[3], double L, int ell, Pat
         sc.grid->path(path, c.ws);
         c.peel += count_cells(path);
         c.walks++;
-        double ke = sc.kext[ell];
-        const double* rho = sc.rho.data();
-        tau = path.opticalDepth([&](int m) { double res = 0; if (m >= 0) res += ke * rho[m]; return res; });
+        tau = path.opticalDepth([&](int m) { return kappa_rho(sc, ell, m); });
     }
     c.npeel++;
     atomic_add(&t.frames[i][(size_t)ell * in.Nx * in.Ny + l], L * std::exp(-tau));
@@ -1196,9 +1259,9 @@ int one_packet(const Scene& sc, const MCParams& mp, int ell, double L, double Lt
     // peeloffemission :299-309 (isotropic geometry: probabilityForDirection == 1)
     for (size_t i = 0; i < sc.instr.size(); i++) detect(sc, i, pp.r, pp.L, ell, ppath, t, c);
     if (!sc.has_dust) return 0;
-    const double ke = sc.kext[ell], albedo = sc.alb[ell], g = sc.g[ell];
-    const double* rho = sc.rho.data();
-    auto kr = [&](int m) { double res = 0; if (m >= 0) res += ke * rho[m]; return res; };
+    const double albedo = sc.alb[ell], g = sc.g[ell];
+    const int H = sc.ncomp;
+    auto kr = [&](int m) { return kappa_rho(sc, ell, m); };
     while (true) {
         // DustSystem::fillOpticalDepth, DustSystem.cpp:935-956
         std::memcpy(path.r, pp.r, sizeof pp.r);
@@ -1212,10 +1275,33 @@ int one_packet(const Scene& sc, const MCParams& mp, int ell, double L, double Lt
             g_err = "The optical depth along the path is not a positive number";
             return -1;
         }
-        // simulateescapeandabsorption :432-461 (single component)
         double L0 = pp.L;
+        if (H > 1) {
+            // simulateescapeandabsorption :463-508 (several components): each cell's own albedo
+            double Lsca = 0.0;
+            int N = (int)path.v.size();
+            for (int n = 0; n < N; n++) {
+                int m = path.v[n].m;
+                if (m != -1) {
+                    double ksca = 0.0, kext = 0.0;
+                    for (int h = 0; h < H; h++) {
+                        double rh = sc.density(m, h);
+                        ksca += rh * sc.kscac[(size_t)h * sc.nlambda + ell];
+                        kext += rh * sc.kextc[(size_t)h * sc.nlambda + ell];
+                    }
+                    double alb = (kext > 0.0) ? ksca / kext : 0.0;
+                    double taustart = (n == 0) ? 0.0 : path.v[n - 1].tau;
+                    double expfactorm = -std::expm1(-path.v[n].dtau);
+                    double Lintm = L0 * std::exp(-taustart) * expfactorm;
+                    Lsca += alb * Lintm;
+                    if (sc.store_abs && t.labs) atomic_add(&t.labs[(size_t)m * sc.nlambda + ell], (1.0 - alb) * Lintm);
+                }
+            }
+            pp.L = Lsca;
+        }
+        // simulateescapeandabsorption :432-461 (single component)
         double expfactor = -std::expm1(-taupath);
-        if (sc.store_abs && t.labs) {
+        if (H == 1 && sc.store_abs && t.labs) {
             int N = (int)path.v.size();
             for (int n = 0; n < N; n++) {
                 int m = path.v[n].m;
@@ -1227,7 +1313,7 @@ int one_packet(const Scene& sc, const MCParams& mp, int ell, double L, double Lt
                 }
             }
         }
-        pp.L = L0 * albedo * expfactor;
+        if (H == 1) pp.L = L0 * albedo * expfactor;
         if (pp.L <= 0 || (pp.L <= Lthreshold && pp.nscatt >= mp.minfs)) break;
         // simulatepropagation :513-531
         if (taupath != 0.0) {
@@ -1243,20 +1329,23 @@ int one_packet(const Scene& sc, const MCParams& mp, int ell, double L, double Lt
             double s = path.pathlength(tau);
             pp.r[0] += s * pp.k[0]; pp.r[1] += s * pp.k[1]; pp.r[2] += s * pp.k[2];
         }
-        // peeloffscattering :313-357 (Ncomp == 1: w = phase function value)
+        // peeloffscattering :313-357: with several components the phase functions are weighted by
+        // kappasca(h) rho(m, h) at the scattering point's cell, and the peel-off is aborted outside the grid or
+        // where there is no scattering dust
+        int mscat = H > 1 ? sc.grid->whichcell(pp.r) : -1;
         for (size_t i = 0; i < sc.instr.size(); i++) {
-            const double* ko = sc.instr[i].kobs;
-            double cosalpha = pp.k[0] * ko[0] + pp.k[1] * ko[1] + pp.k[2] * ko[2];
-            double tt = 1.0 + g * g - 2 * g * cosalpha;
-            double w = (1.0 - g) * (1.0 + g) / std::sqrt(tt * tt * tt);   // DustMix.cpp:650-653
-            detect(sc, i, pp.r, pp.L * w, ell, ppath, t, c);
+            double I;
+            if (!scatter_phase(sc, ell, mscat, pp.k, sc.instr[i].kobs, I)) break;
+            detect(sc, i, pp.r, pp.L * I, ell, ppath, t, c);
         }
-        // simulatescattering :535-543, DustMix.cpp:597-601
+        // simulatescattering :535-543, DustMix.cpp:597-601, with the mix of DustSystem::randomMixForPosition
+        // (DustSystem.cpp:855-869) when there are several components
+        const double g_ = H > 1 ? sc.gc[(size_t)random_mix(sc, ell, mscat, rng) * sc.nlambda + ell] : g;
         double knew[3];
-        if (std::fabs(g) < 1e-6) random_direction(rng, knew);
+        if (std::fabs(g_) < 1e-6) random_direction(rng, knew);
         else {
-            double f = ((1.0 - g) * (1.0 + g)) / (1.0 - g + 2.0 * g * rng.uniform());
-            double costheta = (1.0 + g * g - f * f) / (2.0 * g);
+            double f = ((1.0 - g_) * (1.0 + g_)) / (1.0 - g_ + 2.0 * g_ * rng.uniform());
+            double costheta = (1.0 + g_ * g_ - f * f) / (2.0 * g_);
             random_direction_about(rng, pp.k, costheta, knew);
         }
         std::memcpy(pp.k, knew, sizeof knew);
@@ -1349,6 +1438,7 @@ int or_scene_set_dust(void* s, const double* rho, int64_t ncells, int nlambda, c
                       const double* albedo, const double* g, int store_abs)
 {
     Scene* sc = (Scene*)s;
+    sc->ncomp = 1;
     sc->rho.assign(rho, rho + ncells);
     sc->nlambda = nlambda;
     sc->kext.assign(kext, kext + nlambda);
@@ -1356,6 +1446,31 @@ int or_scene_set_dust(void* s, const double* rho, int64_t ncells, int nlambda, c
     sc->g.assign(g, g + nlambda);
     sc->store_abs = store_abs != 0;
     sc->has_dust = true;
+    return 0;
+}
+
+// A dust system of ncomp components: densities [h][m], kappaext, kappasca, albedo and g [h][ell]. The
+// single-component fields keep the first component's mix and the summed density (for callers that only
+// read those).
+int or_scene_set_dust_multi(void* s, int ncomp, const double* rho, int64_t ncells, int nlambda, const double* kext,
+                            const double* ksca, const double* albedo, const double* g, int store_abs)
+{
+    auto* sc = (Scene*)s;
+    if (ncomp < 1 || ncomp > 8) { g_err = "between 1 and 8 dust components"; return -1; }
+    sc->has_dust = true;
+    sc->nlambda = nlambda;
+    sc->ncomp = ncomp;
+    sc->rho.assign(ncells, 0.0);
+    for (int h = 0; h < ncomp; h++)
+        for (int64_t m = 0; m < ncells; m++) sc->rho[m] += rho[(size_t)h * ncells + m];
+    sc->rhoc.assign(rho, rho + (size_t)ncomp * ncells);
+    sc->kextc.assign(kext, kext + (size_t)ncomp * nlambda);
+    sc->kscac.assign(ksca, ksca + (size_t)ncomp * nlambda);
+    sc->gc.assign(g, g + (size_t)ncomp * nlambda);
+    sc->kext.assign(kext, kext + nlambda);
+    sc->alb.assign(albedo, albedo + nlambda);
+    sc->g.assign(g, g + nlambda);
+    sc->store_abs = store_abs != 0;
     return 0;
 }
 
@@ -1403,6 +1518,25 @@ int or_scene_path(void* s, const double rk[6], int maxseg, int32_t* m, double* d
 
 int or_scene_whichcell(void* s, const double r[3]) { return ((Scene*)s)->grid->whichcell(r); }
 
+// The scattering peel-off's phase value at position r (cell by whichcell) for direction k towards kobs;
+// -1 when the peel-off is aborted.
+double or_scatter_phase(void* s, int ell, const double r[3], const double k[3], const double kobs[3])
+{
+    const Scene& sc = *(Scene*)s;
+    int m = sc.ncomp > 1 ? sc.grid->whichcell(r) : -1;
+    double I;
+    return scatter_phase(sc, ell, m, k, kobs, I) ? I : -1.0;
+}
+
+// The dust component randomMixForPosition picks at r for the uniform deviate u (0 outside the grid).
+int or_random_mix(void* s, int ell, const double r[3], double u)
+{
+    const Scene& sc = *(Scene*)s;
+    if (sc.ncomp < 2) return 0;
+    int m = sc.grid->whichcell(r);
+    return m < 0 ? 0 : random_mix_u(sc, ell, m, u);
+}
+
 double or_scene_opticaldepth(void* s, const double rk[6], int ell)
 {
     Scene* sc = (Scene*)s;
@@ -1411,9 +1545,7 @@ double or_scene_opticaldepth(void* s, const double rk[6], int ell)
     std::memcpy(p.k, rk + 3, 3 * sizeof(double));
     WalkStats st;
     sc->grid->path(p, st);
-    double ke = sc->kext[ell];
-    const double* rho = sc->rho.data();
-    return p.opticalDepth([&](int m) { double res = 0; if (m >= 0) res += ke * rho[m]; return res; });
+    return p.opticalDepth([&](int m) { return kappa_rho(*sc, ell, m); });
 }
 
 void or_scene_pathlength(void* s, const double rk[6], int ell, int ntau, const double* taus, double* out)
@@ -1424,9 +1556,7 @@ void or_scene_pathlength(void* s, const double rk[6], int ell, int ntau, const d
     std::memcpy(p.k, rk + 3, 3 * sizeof(double));
     WalkStats st;
     sc->grid->path(p, st);
-    double ke = sc->kext[ell];
-    const double* rho = sc->rho.data();
-    p.fillOpticalDepth([&](int m) { double res = 0; if (m >= 0) res += ke * rho[m]; return res; });
+    p.fillOpticalDepth([&](int m) { return kappa_rho(*sc, ell, m); });
     for (int i = 0; i < ntau; i++) out[i] = p.pathlength(taus[i]);
 }
 

@@ -35,6 +35,8 @@ int64_t or_scene_num_cells(void* s);
 // medium, sources, instruments
 int or_scene_set_dust(void* s, const double* rho, int64_t ncells, int nlambda, const double* kappaext,
                       const double* albedo, const double* asymmpar, int store_absorption);
+int or_scene_set_dust_multi(void* s, int ncomp, const double* rho, int64_t ncells, int nlambda, const double* kext,
+                            const double* ksca, const double* albedo, const double* g, int store_abs);
 int or_scene_add_star(void* s, const double* params16, const double* lum, const double* table_s,
                       const double* table_M, int ntable);
 int or_scene_add_instrument(void* s, const double* params10);
@@ -42,6 +44,8 @@ int or_scene_add_instrument(void* s, const double* params10);
 // DustGrid::path on one ray; returns number of segments (or -1 if more than maxseg)
 int or_scene_path(void* s, const double rk[6], int maxseg, int32_t* m, double* ds);
 int or_scene_whichcell(void* s, const double r[3]);
+double or_scatter_phase(void* s, int ell, const double r[3], const double k[3], const double kobs[3]);
+int or_random_mix(void* s, int ell, const double r[3], double u);
 // DustSystem::opticaldepth on one ray at wavelength ell
 double or_scene_opticaldepth(void* s, const double rk[6], int ell);
 // DustGridPath::pathlength after fillOpticalDepth along one ray, for each of ntau targets

@@ -70,6 +70,8 @@ _SIGS = {
     "sg_grid_paths": (ctypes.c_int, [_VP, ctypes.c_int64, _D, ctypes.c_int, ctypes.c_int, _I32, _I32, _D]),
     "sg_dust_create": (ctypes.c_int, [_VP, _D, ctypes.c_int64, ctypes.c_int, _D, _D, _D, ctypes.c_int, _VPP]),
     "sg_dust_destroy": (ctypes.c_int, [_VP]),
+    "sg_dust_create_multi": (ctypes.c_int, [_VP, ctypes.c_int, _D, ctypes.c_int64, ctypes.c_int, _D, _D, _D, _D,
+                                            ctypes.c_int, _VPP]),
     "sg_dust_optical_depths": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int64, _D, ctypes.c_int, _D]),
     "sg_dust_path_lengths": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int64, _D, _D, ctypes.c_int, _D]),
     "sg_dust_read_absorption": (ctypes.c_int, [_VP, _D]),
@@ -99,6 +101,9 @@ _SIGS = {
                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _U8,
                                          ctypes.c_int64, _I64]),
     "sg_grid_sample_density": (ctypes.c_int, [_VP, _D, ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _D]),
+    "sg_tree_subdivide_multi": (ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_double,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_uint32, _U8, ctypes.c_int64, _I64]),
     "sg_grid_resample_density": (ctypes.c_int, [_VP, _D, ctypes.c_double, ctypes.c_int, ctypes.c_uint32]),
     "sg_ga_create": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_uint32, _VPP]),

@@ -489,9 +489,21 @@ int sg_tree_subdivide(int arity, const double ext[6], const double* geom16, doub
                       int minlevel, int maxlevel, double maxmassfraction, int nsamples, uint32_t seed,
                       uint8_t* flags, int64_t cap, int64_t* nnodes)
 {
+    return sg_tree_subdivide_multi(arity, ext, 1, geom16, &scale, total_mass, minlevel, maxlevel, maxmassfraction,
+                                   nsamples, seed, flags, cap, nnodes);
+}
+
+// The same with the density of a CompDustDistribution: the sum over components of scale(h) times the
+// geometry's density (CompDustDistribution::density), sampled at the same positions.
+int sg_tree_subdivide_multi(int arity, const double ext[6], int ncomp, const double* geoms16, const double* scales,
+                            double total_mass, int minlevel, int maxlevel, double maxmassfraction, int nsamples,
+                            uint32_t seed, uint8_t* flags, int64_t cap, int64_t* nnodes)
+{
     if (!(arity == 8 || arity == 2)) { set_error("arity must be 8 or 2"); return -1; }
     if (maxlevel < 2 || maxlevel <= minlevel) { set_error("bad tree levels"); return -1; }
-    GeomDensity dens(geom16);
+    if (ncomp < 1 || !geoms16 || !scales) { set_error("at least one dust component is needed"); return -1; }
+    std::vector<GeomDensity> dens;
+    for (int h = 0; h < ncomp; h++) dens.emplace_back(geoms16 + 16 * h);
     // node mass from nsamples counter-based density samples in the node's box
     auto mass = [&](const double* q, int64_t l) {
         double vol = (q[3] - q[0]) * (q[4] - q[1]) * (q[5] - q[2]);
@@ -499,7 +511,10 @@ int sg_tree_subdivide(int arity, const double ext[6], const double* geom16, doub
         for (int j = 0; j < nsamples; j++) {
             double u[3];
             sample3(seed, (uint64_t)l, (uint32_t)j, u);
-            sum += scale * dens(q[0] + u[0] * (q[3] - q[0]), q[1] + u[1] * (q[4] - q[1]), q[2] + u[2] * (q[5] - q[2]));
+            double x = q[0] + u[0] * (q[3] - q[0]), y = q[1] + u[1] * (q[4] - q[1]), z = q[2] + u[2] * (q[5] - q[2]);
+            double d = 0;
+            for (int h = 0; h < ncomp; h++) d += scales[h] * dens[h](x, y, z);
+            sum += d;
         }
         return sum / nsamples * vol;
     };

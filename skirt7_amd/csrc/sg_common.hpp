@@ -18,6 +18,7 @@
 
 #define SG_MAX_INSTR 8
 #define SG_MAX_STARS 4
+#define SG_MAX_DUST 8
 #define SG_LDS_NODES 4096
 #define SG_MAX_STEPS 4000000
 
@@ -132,6 +133,9 @@ struct SceneDev {
     InstrDev instr[SG_MAX_INSTR];
     StarDev star[SG_MAX_STARS];
     double Xcum[SG_MAX_STARS + 1];   // normalised cumulative luminosity
+    // dust components at the run's wavelength (used when there are two or more): scattering
+    // coefficient and asymmetry parameter of each component's mix
+    double ksca[SG_MAX_DUST], gc[SG_MAX_DUST];
 };
 
 struct McArgs {
@@ -164,6 +168,12 @@ struct McArgs {
     // coherence keys of walk requests (event/walk pipeline): start points are binned on a raster of
     // the grid box, t = (r - key_lo) * key_scale in [0, 16)
     double key_lo[3], key_scale[3];
+    // several dust components (DustSystem with Ncomp > 1): the walks see the combined kappa*rho per cell
+    // (kext is then 1); rhoc holds each component's density ([h][m], ncells apart) for the peel-off
+    // weights and the choice of the scattering mix, albm each cell's albedo at the run's wavelength
+    int ncomp;
+    const double* rhoc;
+    const double* albm;
 };
 
 // ------------------------------------------------------------------------- Philox4x32-10

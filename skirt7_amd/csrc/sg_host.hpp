@@ -110,6 +110,12 @@ struct sg_dust {
     std::vector<double> kext, albedo, g;
     bool store_abs;
     sg::DevBuf labs;   // [ell][m]
+    // several dust components: each one's density ([h][m]) and mix per wavelength ([h][ell]); the grid
+    // then holds the combined kappa*rho of the wavelength `prepared` and albm each cell's albedo there
+    int ncomp = 1;
+    std::vector<double> rhoc, kextc, kscac, gc;
+    sg::DevBuf drhoc, albm;
+    int prepared = -1;
 };
 
 struct sg_stars {

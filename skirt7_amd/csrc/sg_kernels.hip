@@ -691,6 +691,34 @@ void sg_grid::set_rho(const double* r)
     }
 }
 
+// A dust system of several components at wavelength ell: the grid's densities become the combined
+// kappa*rho of each cell (KappaRho, DustSystem.cpp:459-485: the sum over components of kappaext(h) rho(m, h),
+// in component order from zero, so that walks multiply it by 1 and give the reference's optical depths),
+// and albm each cell's albedo, ksca/kext with both sums taken over the components as
+// simulateescapeandabsorption takes them (MonteCarloSimulation.cpp:478-490; 0 where kext is 0). Nothing
+// to do when the system was last prepared for ell.
+static void dust_prepare(sg_dust* d, int ell)
+{
+    if (d->ncomp < 2 || d->prepared == ell) return;
+    const int64_t nc = d->grid->ncells;
+    const int nl = d->nlambda, H = d->ncomp;
+    std::vector<double> kr(nc), alb(nc);
+    for (int64_t m = 0; m < nc; m++) {
+        double result = 0, ksca = 0, kext = 0;
+        for (int h = 0; h < H; h++) {
+            double rho = d->rhoc[(size_t)h * nc + m];
+            result += d->kextc[(size_t)h * nl + ell] * rho;
+            ksca += rho * d->kscac[(size_t)h * nl + ell];
+            kext += rho * d->kextc[(size_t)h * nl + ell];
+        }
+        kr[m] = result;
+        alb[m] = (kext > 0.0) ? ksca / kext : 0.0;
+    }
+    d->grid->set_rho(kr.data());
+    SG_HIP(hipMemcpy(d->albm.p, alb.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+    d->prepared = ell;
+}
+
 // Host helper: copies a batch of n records to the device and back.
 struct Batch {
     DevBuf in, in2, o1, o2, o3;
@@ -789,6 +817,48 @@ int sg_dust_create(sg_grid* g, const double* rho, int64_t ncells, int nlambda, c
     SG_CATCH
 }
 
+int sg_dust_create_multi(sg_grid* g, int ncomp, const double* rho, int64_t ncells, int nlambda, const double* kappaext,
+                         const double* kappasca, const double* albedo, const double* asymmpar, int store_absorption,
+                         sg_dust** out)
+{
+    if (ncomp == 1) return sg_dust_create(g, rho, ncells, nlambda, kappaext, albedo, asymmpar, store_absorption, out);
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(ncomp >= 2 && ncomp <= SG_MAX_DUST, "between 1 and 8 dust components are supported");
+    SG_REQUIRE(ncells == g->ncells, "density array size differs from the number of cells");
+    SG_REQUIRE(nlambda >= 1, "need at least one wavelength");
+    const size_t n = (size_t)ncomp * ncells;
+    for (size_t i = 0; i < n; i++)
+        SG_REQUIRE(rho[i] >= 0 && std::isfinite(rho[i]), "dust density must be finite and non-negative");
+    std::unique_ptr<sg_dust> d(new sg_dust());
+    d->grid = g;
+    d->nlambda = nlambda;
+    d->ncomp = ncomp;
+    d->kext.assign(nlambda, 1.0);
+    d->albedo.assign(nlambda, 0.0);
+    d->g.assign(nlambda, 0.0);
+    d->rhoc.assign(rho, rho + n);
+    d->kextc.assign(kappaext, kappaext + (size_t)ncomp * nlambda);
+    d->kscac.assign(kappasca, kappasca + (size_t)ncomp * nlambda);
+    d->gc.assign(asymmpar, asymmpar + (size_t)ncomp * nlambda);
+    d->store_abs = store_absorption != 0;
+    d->drhoc.upload(d->rhoc);
+    d->albm.alloc((size_t)ncells * sizeof(double));
+    dust_prepare(d.get(), 0);
+    if (d->store_abs) {
+        d->labs.alloc((size_t)nlambda * ncells * sizeof(double));
+        d->labs.zero();
+    }
+    *out = d.release();
+    SG_CATCH
+}
+
+int sg_dust_destroy(sg_dust* d)
+{
+    delete d;
+    return 0;
+}
+
 int sg_grid_resample_density(sg_grid* g, const double* geom16, double scale, int nsamples, uint32_t seed)
 {
     SG_TRY
@@ -810,11 +880,6 @@ int sg_grid_resample_density(sg_grid* g, const double* geom16, double scale, int
     SG_CATCH
 }
 
-int sg_dust_destroy(sg_dust* d)
-{
-    delete d;
-    return 0;
-}
 
 } // extern "C"
 template <class W>
@@ -841,6 +906,7 @@ static int tau_common(sg_dust* d, int ell, int64_t n, const double* rk, const do
     double* o = b.o1.as<double>();
     sg_grid* g = d->grid;
     bool mixed = precision == SG_MIXED;
+    dust_prepare(d, ell);
     double ke = d->kext[ell];
     if (g->kind == GK_CART) {
         if (mixed) launch_tau<CartW32>(g->cart, n, ri, ke, ti, mode, o);
@@ -1154,7 +1220,17 @@ static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64
     a.tau_cut = s->mc[7];
     a.has_dust = s->dust != nullptr;
     if (s->dust) {
+        dust_prepare(s->dust, ell);
         a.kext = s->dust->kext[ell]; a.albedo = s->dust->albedo[ell]; a.g = s->dust->g[ell];
+        a.ncomp = s->dust->ncomp;
+        if (a.ncomp > 1) {
+            a.rhoc = s->dust->drhoc.as<double>();
+            a.albm = s->dust->albm.as<double>();
+            for (int h = 0; h < a.ncomp; h++) {
+                sd.ksca[h] = s->dust->kscac[(size_t)h * s->dust->nlambda + ell];
+                sd.gc[h] = s->dust->gc[(size_t)h * s->dust->nlambda + ell];
+            }
+        }
         a.store_abs = s->dust->store_abs;
         a.labs = s->dust->store_abs ? s->dust->labs.as<double>() : nullptr;
         a.ncells = s->dust->grid->ncells;
@@ -1209,7 +1285,7 @@ static void fill_args(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64
     a.scene = s->scene.as<SceneDev>();
 }
 
-template <class W, bool ABS>
+template <class W, bool ABS, bool MULTI = false>
 static void launch_mc(const typename W::View& v, const McArgs& a, int& blocks, DevBuf& cold, DevBuf& args,
                       int nlds = 0)
 {
@@ -1218,7 +1294,7 @@ static void launch_mc(const typename W::View& v, const McArgs& a, int& blocks, D
     // LDS: the hot-state parking area (only when lanes are re-armed out of line), then the top tree nodes
     size_t shm = (SG_PARK ? hot_words<W>() * 4 * (size_t)bs : 0) + (size_t)nlds * sizeof(int32_t);
     int per = 0;
-    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc<W, ABS>, bs, shm));
+    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc<W, ABS, MULTI>, bs, shm));
     if (per < 1) per = 1;
     if (g_tune_waves > 0) per = std::max(1, std::min(per, g_tune_waves * 64 / bs));
     blocks = per * g_cus;
@@ -1234,11 +1310,11 @@ static void launch_mc(const typename W::View& v, const McArgs& a, int& blocks, D
     SG_HIP(hipMemcpy(args.as<unsigned char>() + off, &a, sizeof(McArgs), hipMemcpyHostToDevice));
     const auto* vd = reinterpret_cast<const typename W::View*>(args.p);
     const auto* ad = reinterpret_cast<const McArgs*>(args.as<unsigned char>() + off);
-    hipLaunchKernelGGL((k_mc<W, ABS>), dim3(blocks), dim3(bs), shm, 0, v, a, nlds, cr, cu, nslots, vd, ad);
+    hipLaunchKernelGGL((k_mc<W, ABS, MULTI>), dim3(blocks), dim3(bs), shm, 0, v, a, nlds, cr, cu, nslots, vd, ad);
     SG_HIP(hipGetLastError());
 }
 
-template <bool ABS> static void dispatch_mc(sg_sim* s, const McArgs& a, int precision, int& blocks)
+template <bool ABS, bool MULTI = false> static void dispatch_mc(sg_sim* s, const McArgs& a, int precision, int& blocks)
 {
     bool mixed = precision == SG_MIXED;
     sg_grid* g = s->dust ? s->dust->grid : nullptr;
@@ -1249,16 +1325,29 @@ template <bool ABS> static void dispatch_mc(sg_sim* s, const McArgs& a, int prec
         return;
     }
     if (g->kind == GK_CART) {
-        if (mixed) launch_mc<CartW32, ABS>(g->cart, a, blocks, s->cold, s->args);
-        else launch_mc<CartW64, ABS>(g->cart, a, blocks, s->cold, s->args);
+        if (mixed) launch_mc<CartW32, ABS, MULTI>(g->cart, a, blocks, s->cold, s->args);
+        else launch_mc<CartW64, ABS, MULTI>(g->cart, a, blocks, s->cold, s->args);
     } else if (g->kind == GK_TREE) {
-        if (!mixed && g->tree.search == SG_BOOKKEEPING) launch_mc<TreeW64Book, ABS>(g->tree, a, blocks, s->cold, s->args, 0);
-        else if (!mixed) launch_mc<TreeW64, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
-        else if (rope_walk(g)) launch_mc<TreeW32, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
-        else launch_mc<TreeW32TopDown, ABS>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
+        if (!mixed && g->tree.search == SG_BOOKKEEPING)
+            launch_mc<TreeW64Book, ABS, MULTI>(g->tree, a, blocks, s->cold, s->args, 0);
+        else if (!mixed) launch_mc<TreeW64, ABS, MULTI>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
+        else if (rope_walk(g)) launch_mc<TreeW32, ABS, MULTI>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
+        else launch_mc<TreeW32TopDown, ABS, MULTI>(g->tree, a, blocks, s->cold, s->args, SG_LDS_NODES);
     } else {
-        if (mixed) launch_mc<VoroW<float>, ABS>(g->voro, a, blocks, s->cold, s->args);
-        else launch_mc<VoroW<double>, ABS>(g->voro, a, blocks, s->cold, s->args);
+        if (mixed) launch_mc<VoroW<float>, ABS, MULTI>(g->voro, a, blocks, s->cold, s->args);
+        else launch_mc<VoroW<double>, ABS, MULTI>(g->voro, a, blocks, s->cold, s->args);
+    }
+}
+
+// The lane pipeline's kernel for a run's arguments: absorption stored or not, one or several dust components.
+static void dispatch_mc_any(sg_sim* s, const McArgs& a, int precision, int& blocks)
+{
+    if (a.ncomp > 1) {
+        if (a.store_abs) dispatch_mc<true, true>(s, a, precision, blocks);
+        else dispatch_mc<false, true>(s, a, precision, blocks);
+    } else {
+        if (a.store_abs) dispatch_mc<true>(s, a, precision, blocks);
+        else dispatch_mc<false>(s, a, precision, blocks);
     }
 }
 // ---------------------------------------------------------------- split pipeline
@@ -1487,7 +1576,10 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
     const bool mixed = precision == SG_MIXED;
     const size_t rsz = mixed ? sizeof(float) : sizeof(double);
     const uint64_t batch = std::min<uint64_t>(count, g_emit_batch ? g_emit_batch : (uint64_t)1 << 22);
-    if (g_pipeline == 0) {
+    // several dust components run in the lane pipeline
+    const int pipeline = a.ncomp > 1 ? 0 : g_pipeline;
+    if (pipeline != g_pipeline && g_tune_compact <= 0) a.compact = 32;
+    if (pipeline == 0) {
         if (s->emitted.bytes < 7 * batch * rsz) s->emitted.alloc(7 * batch * rsz);
         if (s->emitdraw.bytes < batch * sizeof(uint32_t)) s->emitdraw.alloc(batch * sizeof(uint32_t));
     }
@@ -1497,12 +1589,12 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
     SG_HIP(hipEventRecord(e0, 0));
     int blocks = 0;
     LaunchTimer lt;
-    if (g_pipeline == 1 && count > 0) {
+    if (pipeline == 1 && count > 0) {
         if (a.store_abs) dispatch_split<true>(s, a, precision, first, count, lt, blocks);
         else dispatch_split<false>(s, a, precision, first, count, lt, blocks);
     }
     // the lane pipeline runs in batches: emission kernel, then the persistent transport kernel
-    for (uint64_t b = 0; g_pipeline == 0 && b < count; b += batch) {
+    for (uint64_t b = 0; pipeline == 0 && b < count; b += batch) {
         McArgs ab = a;
         ab.first = first + b;
         ab.count = std::min(batch, count - b);
@@ -1514,8 +1606,7 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
         else hipLaunchKernelGGL(k_emit<double>, dim3(eb), dim3(256), 0, 0, ab, s->emitted.as<double>(), s->emitdraw.as<uint32_t>());
         SG_HIP(hipGetLastError());
         lt.start();
-        if (ab.store_abs) dispatch_mc<true>(s, ab, precision, blocks);
-        else dispatch_mc<false>(s, ab, precision, blocks);
+        dispatch_mc_any(s, ab, precision, blocks);
         lt.stop();
     }
     SG_HIP(hipEventRecord(e1, 0));
@@ -1686,6 +1777,8 @@ int sg_sim_run_static(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64
     SG_REQUIRE(count <= kStaticMaxPackets, "sg_sim_run_static: at most " + std::to_string(kStaticMaxPackets) +
                                                " packets per call (the static work array holds their launch states); "
                                                "split the packet range over several calls");
+    SG_REQUIRE(!s->dust || s->dust->ncomp == 1,
+               "sg_sim_run_static: a dust system of several components needs sg_sim_run (lane pipeline)");
     McArgs a;
     fill_args(s, ell, first, count, npp, a);
     int blocks = 0;
@@ -1725,6 +1818,11 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
     SG_TRY
     ensure_init();
     SG_REQUIRE(s != nullptr && sample > 0, "autotune needs a simulation and a positive sample");
+    if (s->dust && s->dust->ncomp > 1) {
+        // several dust components always run in the lane pipeline: tune that
+        choose_pipeline = 0;
+        g_pipeline = 0;
+    }
     std::vector<std::pair<DevBuf*, void*>> keep;
     auto save = [&](void* p, size_t bytes) {
         auto* c = new DevBuf();
@@ -1953,8 +2051,30 @@ int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const u
     int blocks = 0;
     LaunchTimer lt;
     std::vector<uint64_t> done(nmembers, 0);
+    // members with several dust components: one at a time through the lane pipeline
+    for (int i = 0; i < nmembers; i++) {
+        if (base[i].ncomp < 2 || counts[i] == 0) continue;
+        for (uint64_t b = 0; b < counts[i]; b += B) {
+            McArgs ab = base[i];
+            ab.first = b;
+            ab.count = std::min(B, counts[i] - b);
+            ab.emitted = s0->emitted.p;
+            ab.emit_draw = s0->emitdraw.as<uint32_t>();
+            if (g_tune_compact <= 0) ab.compact = 32;
+            sims[i]->next.zero();
+            int eb = (int)std::min<uint64_t>((ab.count + 255) / 256, (uint64_t)g_cus * 16);
+            if (mixed) hipLaunchKernelGGL(k_emit<float>, dim3(eb), dim3(256), 0, 0, ab, s0->emitted.as<float>(), s0->emitdraw.as<uint32_t>());
+            else hipLaunchKernelGGL(k_emit<double>, dim3(eb), dim3(256), 0, 0, ab, s0->emitted.as<double>(), s0->emitdraw.as<uint32_t>());
+            SG_HIP(hipGetLastError());
+            lt.start();
+            dispatch_mc_any(sims[i], ab, precision, blocks);
+            lt.stop();
+        }
+        done[i] = counts[i];
+    }
     // large members: their own event/walk passes, which keep the walk kernel full on their own
     for (int i = 0; g_pipeline == 1 && g_batch_alone > 0 && i < nmembers; i++) {
+        if (done[i] == counts[i]) continue;
         if (counts[i] < g_batch_alone) continue;
         if (base[i].store_abs) dispatch_split<true>(sims[i], base[i], precision, 0, counts[i], lt, blocks);
         else dispatch_split<false>(sims[i], base[i], precision, 0, counts[i], lt, blocks);

@@ -228,11 +228,15 @@ template <class W> struct Hot {
     W w;
     R tau, s, tprev, sprev, limit, L0;
     int nseg, nent, stage;
+    // several dust components only: the own walk's scattered luminosity, and the cell of the walk's
+    // last segment (for a propagation re-walk, the cell of the interaction point)
+    R lsca;
+    int mcell;
     template <class P> __device__ __forceinline__ void park(P& p)
     {
         w.park(p);
         p.io(tau); p.io(s); p.io(tprev); p.io(sprev); p.io(limit); p.io(L0);
-        p.io(nseg); p.io(nent); p.io(stage);
+        p.io(nseg); p.io(nent); p.io(stage); p.io(lsca); p.io(mcell);
     }
     // After w.init: the running sums start at zero. stage: ST_PEELW, ST_OWN or ST_PROP; lim: the end
     // optical depth (> 0), anything else for none.
@@ -261,7 +265,7 @@ template <class W> __device__ __forceinline__ typename W::Real prop_length(const
 
 template <class W> __host__ __device__ constexpr int hot_words()
 {
-    return W::kParkWords + (int)(6 * sizeof(typename W::Real) / 4) + 3;
+    return W::kParkWords + (int)(7 * sizeof(typename W::Real) / 4) + 4;
 }
 
 // Hot state is parked in LDS (structure-of-arrays, one slot per lane) while a wave re-arms lanes,
@@ -279,7 +283,7 @@ template <class W> __device__ __forceinline__ void lds_load(Hot<W>& v, uint32_t*
 
 enum ColdR { CR_RX = 0, CR_RY, CR_RZ, CR_KX, CR_KY, CR_KZ, CR_L, CR_LPEEL, CR_NR };
 enum ColdU { CU_PLO = 0, CU_PHI, CU_DRAW, CU_W0, CU_W1, CU_W2, CU_W3, CU_NSCATT, CU_PEEL, CU_PEELKIND, CU_PIXEL,
-             CU_PK, CU_SCAT, CU_NPEEL, CU_WALKS, CU_NU };
+             CU_MCELL, CU_PK, CU_SCAT, CU_NPEEL, CU_WALKS, CU_NU };
 
 template <typename R> struct ColdSlot {
     R* r;          // CR_NR arrays of nl slots (device memory, one slot per launched lane)
@@ -339,16 +343,63 @@ template <typename R> __device__ __forceinline__ R rexpm1(R x);
 template <> __device__ __forceinline__ double rexpm1<double>(double x) { return expm1(x); }
 template <> __device__ __forceinline__ float rexpm1<float>(float x) { return expm1f(x); }
 
+// ------------------------------------------------------------------------- dust components
+
+// Weights of the components' phase functions at the scattering point, kappasca(h) rho(m, h)
+// (MonteCarloSimulation.cpp:318-331): false when the point lies outside the grid or has no
+// scattering dust, which aborts the scattering peel-off.
+template <typename R> __device__ __forceinline__ bool peel_weights_ok(const McArgs& a, int m)
+{
+    if (m < 0) return false;
+    double sum = 0;
+    for (int k = 0; k < a.ncomp; k++) sum += a.scene->ksca[k] * a.rhoc[(size_t)k * a.ncells + m];
+    return sum > 0;
+}
+
+// The peel-off's phase-function value at cos(alpha): each component's Henyey-Greenstein value
+// (DustMix.cpp:648-653) weighted by its normalised kappasca(h) rho(m, h) (MonteCarloSimulation.cpp:318-343)
+template <typename R> __device__ __forceinline__ R mixed_phase(const McArgs& a, int m, R cosalpha)
+{
+    double sum = 0;
+    for (int k = 0; k < a.ncomp; k++) sum += a.scene->ksca[k] * a.rhoc[(size_t)k * a.ncells + m];
+    R I = 0;
+    for (int k = 0; k < a.ncomp; k++) {
+        R w = (R)(a.scene->ksca[k] * a.rhoc[(size_t)k * a.ncells + m] / sum);
+        R gk = (R)a.scene->gc[k];
+        R t = (R)1.0 + gk * gk - (R)2 * gk * cosalpha;
+        I += w * (((R)1.0 - gk) * ((R)1.0 + gk) / sqrt(t * t * t));
+    }
+    return I;
+}
+
+// DustSystem::randomMixForPosition (DustSystem.cpp:855-869): the component drawn with probability
+// proportional to kappasca(h) rho(m, h) (NR::cdf, then NR::locate_clip on one uniform deviate); the first
+// component, without a draw, outside the grid.
+__device__ __forceinline__ int random_mix(const McArgs& a, int m, PacketRng& rng)
+{
+    if (m < 0) return 0;
+    double X[SG_MAX_DUST + 1];
+    X[0] = 0;
+    for (int k = 0; k < a.ncomp; k++) X[k + 1] = X[k] + a.scene->ksca[k] * a.rhoc[(size_t)k * a.ncells + m];
+    const double total = X[a.ncomp];
+    for (int k = 0; k <= a.ncomp; k++) X[k] /= total;
+    double u = rng.ud();
+    int h = 0;
+    for (int k = 1; k < a.ncomp; k++) h = (X[k] <= u) ? k : h;
+    return h;
+}
+
 // ------------------------------------------------------------------------- stages
 
 // Re-arms one lane: runs the packet state machine from its current stage until the lane has a
 // walk to do (PEELW, OWN or PROP) or no packets are left (DEAD).
-template <class W, bool ABS>
+template <class W, bool ABS, bool MULTI = false>
 __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Real>& c, const typename W::View& g,
                                         const int32_t* lds, int nlds, const McArgs& a, const InstrDev* instr)
 {
     using R = typename W::Real;
-    const R albedo = (R)a.albedo, gg = (R)a.g;
+    const R albedo = (R)a.albedo;
+    R gg = (R)a.g;
     R rx = c.fr(CR_RX), ry = c.fr(CR_RY), rz = c.fr(CR_RZ);
     R kx = c.fr(CR_KX), ky = c.fr(CR_KY), kz = c.fr(CR_KZ);
     R L = c.fr(CR_L), Lpeel = c.fr(CR_LPEEL);
@@ -358,6 +409,7 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
     rng.w0 = c.fu(CU_W0); rng.w1 = c.fu(CU_W1); rng.w2 = c.fu(CU_W2); rng.w3 = c.fu(CU_W3);
     int nscatt = (int)c.fu(CU_NSCATT), peel = (int)c.fu(CU_PEEL), peelkind = (int)c.fu(CU_PEELKIND);
     int pixel = (int)c.fu(CU_PIXEL);
+    int mcell = MULTI ? (int)c.fu(CU_MCELL) : -1;
     uint32_t c_pk = 0, c_scat = 0, c_npeel = 0, c_walks = 0;
     bool armed = false;
     while (!armed) {
@@ -391,6 +443,12 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
                 else h.stage = ST_SCATTER;
                 break;
             }
+            if (MULTI && peelkind && peel == 0 && !peel_weights_ok<R>(a, mcell)) {
+                // peeloffscattering aborts when the scattering point has no scattering dust
+                // (MonteCarloSimulation.cpp:321-327)
+                peel = a.ni;
+                break;
+            }
             const InstrDev& in = instr[peel];
             int pix = pixel_on_detector<R>(in, rx, ry, rz);
             if (pix < 0) { peel++; break; }
@@ -398,8 +456,11 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
             if (peelkind) {
                 // DustMix::phaseFunctionValue, Henyey-Greenstein (DustMix.cpp:648-653)
                 R cosalpha = kx * (R)in.kobs[0] + ky * (R)in.kobs[1] + kz * (R)in.kobs[2];
-                R t = (R)1.0 + gg * gg - (R)2 * gg * cosalpha;
-                Lp = L * (((R)1.0 - gg) * ((R)1.0 + gg) / sqrt(t * t * t));
+                if (MULTI) Lp = L * mixed_phase<R>(a, mcell, cosalpha);
+                else {
+                    R t = (R)1.0 + gg * gg - (R)2 * gg * cosalpha;
+                    Lp = L * (((R)1.0 - gg) * ((R)1.0 + gg) / sqrt(t * t * t));
+                }
             }
             c_npeel++;
             if (!a.has_dust) {
@@ -424,6 +485,7 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
         }
         case ST_OWN_START: {
             h.L0 = L;
+            h.lsca = (R)0;
             h.w.init(g, lds, nlds, rx, ry, rz, kx, ky, kz);
             h.start_walk(ST_OWN, (R)0);
             c_walks++;
@@ -438,12 +500,16 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
                 h.stage = ST_NEWPKT;
                 break;
             }
-            // simulateescapeandabsorption, single component (MonteCarloSimulation.cpp:432-461)
-            R expfactor = -rexpm1<R>(-taupath);
-            L = h.L0 * albedo * expfactor;
+            // simulateescapeandabsorption: one component (MonteCarloSimulation.cpp:432-461), or the sum of
+            // each cell's albedo times its interacting luminosity, gathered along the walk (:463-508)
+            if (MULTI) L = h.lsca;
+            else {
+                R expfactor = -rexpm1<R>(-taupath);
+                L = h.L0 * albedo * expfactor;
+            }
             if (L <= (R)0 || (L <= (R)a.Lthreshold && nscatt >= a.minfs)) { h.stage = ST_NEWPKT; break; }
             // simulatepropagation (MonteCarloSimulation.cpp:513-531)
-            if (taupath == (R)0) { peel = 0; peelkind = 1; h.stage = ST_PEEL; break; }
+            if (taupath == (R)0) { peel = 0; peelkind = 1; mcell = -1; h.stage = ST_PEEL; break; }
             R tt;
             if (a.xi == 0.0) tt = exponcutoff(rng, taupath);
             else {
@@ -463,12 +529,15 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
             R d = prop_length(h);
             rx = rx + d * kx; ry = ry + d * ky; rz = rz + d * kz;
             peel = 0; peelkind = 1;
+            mcell = h.mcell;
             h.stage = ST_PEEL;
             break;
         }
         case ST_SCATTER: {
             // simulatescattering, Henyey-Greenstein (DustMix.cpp:596-601) and Random::direction(k, cos)
-            // (Random.cpp:189-223)
+            // (Random.cpp:189-223); with several components, the mix drawn first
+            // (DustSystem::randomMixForPosition, DustSystem.cpp:855-869)
+            if (MULTI) gg = (R)a.scene->gc[random_mix(a, mcell, rng)];
             R nkx, nky, nkz;
             if (fabs((double)gg) < 1e-6) {
                 double a0, a1, a2;
@@ -507,20 +576,21 @@ __device__ __forceinline__ void service(Hot<W>& h, const ColdSlot<typename W::Re
     c.fu(CU_W0) = rng.w0; c.fu(CU_W1) = rng.w1; c.fu(CU_W2) = rng.w2; c.fu(CU_W3) = rng.w3;
     c.fu(CU_NSCATT) = (uint32_t)nscatt; c.fu(CU_PEEL) = (uint32_t)peel; c.fu(CU_PEELKIND) = (uint32_t)peelkind;
     c.fu(CU_PIXEL) = (uint32_t)pixel;
+    if (MULTI) c.fu(CU_MCELL) = (uint32_t)mcell;
     c.fu(CU_PK) += c_pk; c.fu(CU_SCAT) += c_scat; c.fu(CU_NPEEL) += c_npeel; c.fu(CU_WALKS) += c_walks;
 }
 
 // The re-arming step as a separate (not inlined) function: it reads the lane's parked hot state,
 // runs the state machine and parks the result. Its register demand (FP64 emission and scattering
 // math) then stays out of the crossing loop's allocation; any spilling it needs happens here only.
-template <class W, bool ABS>
+template <class W, bool ABS, bool MULTI = false>
 __device__ __noinline__ void service_call(uint32_t* park, int nl, int lane, ColdSlot<typename W::Real> c,
                                           const typename W::View* g, const int32_t* lds, int nlds, const McArgs* a,
                                           const InstrDev* instr)
 {
     Hot<W> q;
     lds_load(q, park, nl, lane);
-    service<W, ABS>(q, c, *g, lds, nlds, *a, instr);
+    service<W, ABS, MULTI>(q, c, *g, lds, nlds, *a, instr);
     lds_store(q, park, nl, lane);
 }
 
@@ -528,7 +598,7 @@ __device__ __noinline__ void service_call(uint32_t* park, int nl, int lane, Cold
 // the path ends, or when the optical depth passes the walk's limit (the propagation target, or the
 // peel-off cut). Returns whether the walk goes on. Crossings are counted per walk when it ends
 // (segments minus entry segments), so nothing here is counted across the wave.
-template <class W, bool ABS>
+template <class W, bool ABS, bool MULTI = false>
 __device__ __forceinline__ bool crossing(Hot<W>& h, const typename W::View& g, const int32_t* lds, int nlds,
                                          const McArgs& a)
 {
@@ -543,7 +613,21 @@ __device__ __forceinline__ bool crossing(Hot<W>& h, const typename W::View& g, c
     const bool seg = got && ds > (R)0;
     const R dsv = seg ? ds : (R)0;
     const R dtau = seg ? ((R)a.kext * rho) * ds : (R)0;   // KappaRho (DustSystem.cpp:459-485)
-    if (ABS && h.stage == ST_OWN && seg && m >= 0) {
+    if constexpr (MULTI) {
+        // several components: each cell's own albedo (MonteCarloSimulation.cpp:478-505), its scattered
+        // part summed for the packet, its absorbed part into the table
+        if (seg) {
+            const int64_t cell = m >= 0 ? W::cell_of(g, m) : -1;
+            if (h.stage == ST_OWN && cell >= 0) {
+                R expfactorm = -rexpm1<R>(-dtau);
+                R Lintm = h.L0 * rexp<R>(-h.tau) * expfactorm;
+                R alb = (R)a.albm[cell];
+                h.lsca += alb * Lintm;
+                if (ABS) unsafeAtomicAdd(a.labs + (size_t)a.ell * a.ncells + cell, (double)(((R)1.0 - alb) * Lintm));
+            }
+            h.mcell = (int)cell;
+        }
+    } else if (ABS && h.stage == ST_OWN && seg && m >= 0) {
         // OligoDustSystem::absorb via simulateescapeandabsorption (MonteCarloSimulation.cpp:445-458)
         R expfactorm = -rexpm1<R>(-dtau);
         R Lintm = h.L0 * rexp<R>(-h.tau) * expfactorm;
@@ -610,7 +694,7 @@ __device__ __forceinline__ void add_wave(unsigned long long* dst, unsigned long 
 // The lane pipeline of one wave over the packet range of `a`: lanes claim packets until the range
 // is exhausted and every packet the wave claimed has finished; then the wave's event counts go
 // to a.counters.
-template <class W, bool ABS>
+template <class W, bool ABS, bool MULTI = false>
 __device__ __forceinline__ void drain(const typename W::View& g, const McArgs& a, uint32_t* park,
                                       const int32_t* s_nodes, int nlds, const ColdSlot<typename W::Real>& c,
                                       const InstrDev* s_instr, const typename W::View* gdev, const McArgs* adev)
@@ -631,18 +715,18 @@ __device__ __forceinline__ void drain(const typename W::View& g, const McArgs& a
         if (__ballot(need)) {
             // park every lane's hot state, re-arm the idle lanes, reload
             lds_store(h, park, nl, lane);
-            if (need) service_call<W, ABS>(park, nl, lane, c, gdev, s_nodes, nlds, adev, s_instr);
+            if (need) service_call<W, ABS, MULTI>(park, nl, lane, c, gdev, s_nodes, nlds, adev, s_instr);
             lds_load(h, park, nl, lane);
         }
 #else
-        if (need) service<W, ABS>(h, c, g, s_nodes, nlds, a, s_instr);
+        if (need) service<W, ABS, MULTI>(h, c, g, s_nodes, nlds, a, s_instr);
 #endif
         const uint64_t alive = __ballot(h.stage != ST_DEAD);
         if (!alive) break;
         const int need_n = max(1, min(thr, __popcll(alive) >> 1));
         bool walk = walking_stage(h.stage);
         for (;;) {
-            if (walk) walk = crossing<W, ABS>(h, g, s_nodes, nlds, a);
+            if (walk) walk = crossing<W, ABS, MULTI>(h, g, s_nodes, nlds, a);
             // lanes alive and not walking are idle (alive is fixed while the wave walks)
             const uint64_t wm = __ballot(walk);
             if (__popcll(alive & ~wm) >= need_n || !wm) break;
@@ -661,7 +745,7 @@ __device__ __forceinline__ void drain(const typename W::View& g, const McArgs& a
 
 // Persistent transport kernel. Dynamic LDS: the hot-state parking area (hot_words<W>() per lane),
 // then (optionally) the top tree nodes. Cold lane state lives in device memory at cold_r/cold_u.
-template <class W, bool ABS>
+template <class W, bool ABS, bool MULTI = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_MC_WAVES_PER_EU, 8)))
 k_mc(const typename W::View g, const McArgs a, int nlds_req, typename W::Real* cold_r, uint32_t* cold_u,
      int nslots, const typename W::View* gdev, const McArgs* adev)
@@ -681,7 +765,7 @@ k_mc(const typename W::View g, const McArgs a, int nlds_req, typename W::Real* c
     c.r = cold_r; c.u = cold_u; c.nl = nslots;
     c.lane = blockIdx.x * blockDim.x + threadIdx.x;
     __syncthreads();
-    drain<W, ABS>(g, a, park, s_nodes, nlds, c, s_instr, gdev, adev);
+    drain<W, ABS, MULTI>(g, a, park, s_nodes, nlds, c, s_instr, gdev, adev);
 }
 
 // Batched transport over many packet ranges, each with its own grid view and run arguments (one

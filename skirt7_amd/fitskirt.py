@@ -205,8 +205,11 @@ def _structure(spec):
         g = d["grid"]
         grid = (g["kind"], tuple(np.asarray(g["extent"]).tolist()), tuple(sorted((k, str(v)) for k, v in g.items()
                                                                                if k not in ("kind", "extent"))))
-        if g["kind"] != "CartesianDustGrid":
-            grid = grid + (repr(d["geometry"].params().tolist()), d["scale"])
+        if g["kind"] != "CartesianDustGrid" or len(d["components"]) > 1:
+            # the density decides a tree's or Voronoi grid's structure; several components keep their own
+            # densities on the host: either way a new density means a new member
+            grid = grid + tuple((repr(c["geometry"].params().tolist()), c["scale"], tuple(c["opacities"]),
+                                 tuple(c["albedos"]), tuple(c["asymmetry"])) for c in d["components"])
         grid = grid + (tuple(d["opacities"]), tuple(d["albedos"]), tuple(d["asymmetry"]), d["writeMeanIntensity"],
                        d["sampleCount"], type(d["geometry"]).__name__)
     frames = tuple((f["name"], f["pixelsX"], f["pixelsY"]) for f in spec.frames)

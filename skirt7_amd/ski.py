@@ -255,9 +255,19 @@ class Spec:
         dd = ds.one("dustDistribution", "CompDustDistribution")
         _require(dd, "CompDustDistribution")
         comps = dd.many("components")
-        if len(comps) != 1:
-            raise FatalError("exactly one dust component is supported (found %d)" % len(comps))
-        dc = comps[0]
+        if not comps:
+            raise FatalError("the dust distribution has no components")
+        d = dict(components=[self._dustcomp(dc, nl) for dc in comps])
+        # the first component's entries at the top level, for single-component callers
+        d.update({k: v for k, v in d["components"][0].items()})
+        d["sampleCount"] = val.integer(ds, "sampleCount", "100")
+        d["writeMeanIntensity"] = val.boolean(ds, "writeMeanIntensity", "no")
+        d["grid"] = self._grid(ds.one("dustGrid", "OctTreeDustGrid"))
+        return d
+
+    def _dustcomp(self, dc, nl):
+        """One DustComp (DustComp.cpp): geometry, SimpleOligoDustMix and normalization."""
+        val = self.val
         _require(dc, "DustComp")
         d = dict(geometry=_geometry(dc.one("geometry"), val))
         mix = dc.one("mix")
@@ -286,9 +296,6 @@ class Spec:
             d["scale"] = val.double(norm, "dustMass", "mass") / (PC * PC)
         else:
             _require(norm, "FaceOnDustCompNormalization", "DustMassDustCompNormalization")
-        d["sampleCount"] = val.integer(ds, "sampleCount", "100")
-        d["writeMeanIntensity"] = val.boolean(ds, "writeMeanIntensity", "no")
-        d["grid"] = self._grid(ds.one("dustGrid", "OctTreeDustGrid"))
         return d
 
     def _grid(self, g):
@@ -354,31 +361,48 @@ class Simulation:
         self.grid = self.dust = self.rho = self.sites = None
         if sp.dust is not None:
             d = sp.dust
-            self.grid = self._grid(d["grid"], d["geometry"], d["scale"], sp.seed)
+            dcs = d["components"]
+            geoms, scales = [c["geometry"] for c in dcs], [c["scale"] for c in dcs]
+            self.grid = self._grid(d["grid"], geoms, scales, sp.seed)
+            # DustSystem::setSampleDensityBody (DustSystem.cpp:145-172): every component's density sampled at
+            # the same positions of each cell (its value at the site for a Voronoi cell)
             if d["grid"]["kind"] == "VoronoiDustGrid":
-                self.rho = d["scale"] * S.geometry_density(d["geometry"], self.sites)
+                rhos = [c["scale"] * S.geometry_density(c["geometry"], self.sites) for c in dcs]
             else:
-                self.rho = S.sample_density(self.grid, d["geometry"], d["scale"], d["sampleCount"], sp.seed)
-            self.mix = S.SimpleOligoDustMix(d["opacities"], d["albedos"], d["asymmetry"])
+                rhos = [S.sample_density(self.grid, c["geometry"], c["scale"], d["sampleCount"], sp.seed)
+                        for c in dcs]
+            mixes = [S.SimpleOligoDustMix(c["opacities"], c["albedos"], c["asymmetry"]) for c in dcs]
+            self.mix = mixes[0] if len(mixes) == 1 else mixes
+            self.rho = rhos[0] if len(rhos) == 1 else rhos
             self.dust = S.DustSystem(self.grid, self.rho, self.mix, writeMeanIntensity=d["writeMeanIntensity"])
         self.sim = S.OligoMonteCarloSimulation(self.dust, self.stars, self.instr, packages=sp.packages,
                                                minWeightReduction=sp.minWeightReduction,
                                                minScattEvents=sp.minScattEvents, scattBias=sp.scattBias,
                                                seed=sp.seed)
 
-    def _grid(self, g, dgeom, scale, seed):
+    def _grid(self, g, geoms, scales, seed):
+        """The grid of the dust system whose components have geometries geoms and normalisations scales (the
+        tree is subdivided on their summed density, Voronoi sites drawn from the components by mass)."""
         ext = g["extent"]
         if g["kind"] == "CartesianDustGrid":
             return S.CartesianDustGrid(ext, *[S.linmesh(ext[a], ext[a + 3], g["bins"][a]) for a in range(3)])
         if g["kind"] in ("OctTreeDustGrid", "BinTreeDustGrid"):
             cls = S.OctTreeDustGrid if g["kind"] == "OctTreeDustGrid" else S.BinTreeDustGrid
-            return cls.subdivide(ext, dgeom, scale, total_mass=scale, minlevel=g["minLevel"], maxlevel=g["maxLevel"],
-                                 maxmassfraction=g["maxMassFraction"], nsamples=g["sampleCount"], seed=seed,
-                                 search=_SEARCH[g["search"]])
+            return cls.subdivide(ext, geoms, scales, total_mass=float(sum(scales)), minlevel=g["minLevel"],
+                                 maxlevel=g["maxLevel"], maxmassfraction=g["maxMassFraction"],
+                                 nsamples=g["sampleCount"], seed=seed, search=_SEARCH[g["search"]])
         n = g["numParticles"]
         if g["distribution"] == "DustDensity":
             from . import workloads
-            self.sites = workloads.sample_sites(dgeom, n, extent=ext, seed=seed)
+            # CompDustDistribution::generatePosition: a component chosen with probability proportional to its
+            # mass, then a position from its geometry
+            if len(geoms) == 1:
+                self.sites = workloads.sample_sites(geoms[0], n, extent=ext, seed=seed)
+            else:
+                counts = np.random.default_rng(seed).multinomial(n, np.asarray(scales) / float(sum(scales)))
+                parts = [workloads.sample_sites(gm, int(k), extent=ext, seed=seed + h)
+                         for h, (gm, k) in enumerate(zip(geoms, counts)) if k > 0]
+                self.sites = np.concatenate(parts)[np.random.default_rng(seed + 1000).permutation(n)]
         else:
             rng = np.random.default_rng(seed)
             self.sites = ext[:3] + rng.random((n, 3)) * (ext[3:] - ext[:3])

@@ -122,15 +122,20 @@ class TreeDustGrid(DustGrid):
     @classmethod
     def subdivide(cls, extent, geometry, scale, total_mass, minlevel=2, maxlevel=6, maxmassfraction=1e-6,
                   nsamples=100, seed=4357, search=SG_NEIGHBOR):
-        """Builds the subdivision from a dust geometry density (TreeDustGrid.cpp:166-230)."""
+        """Builds the subdivision from a dust geometry density (TreeDustGrid.cpp:166-230); geometry and scale
+        may be lists, one per dust component, for a CompDustDistribution's summed density."""
         e = as_f64(extent)
         cap = 1 << 27
         flags = np.zeros(cap, np.uint8)
         n = ctypes.c_int64()
-        g = as_f64(geometry.params())
-        check(lib().sg_tree_subdivide(cls.arity, dptr(e), dptr(g), scale, total_mass, minlevel, maxlevel,
-                                      maxmassfraction, nsamples, seed, flags.ctypes.data_as(capi._U8), cap,
-                                      ctypes.byref(n)))
+        geoms = list(geometry) if isinstance(geometry, (list, tuple)) else [geometry]
+        scales = as_f64(np.atleast_1d(scale))
+        if len(scales) != len(geoms):
+            raise FatalError("one scale per dust component is needed")
+        g = as_f64(np.concatenate([x.params() for x in geoms]))
+        check(lib().sg_tree_subdivide_multi(cls.arity, dptr(e), len(geoms), dptr(g), dptr(scales), total_mass,
+                                            minlevel, maxlevel, maxmassfraction, nsamples, seed,
+                                            flags.ctypes.data_as(capi._U8), cap, ctypes.byref(n)))
         return cls(e, flags[:n.value].copy(), search)
 
 
@@ -310,18 +315,36 @@ class SimpleOligoDustMix:
 
 
 class DustSystem:
-    """OligoDustSystem with one dust component (DustSystem.cpp:935-976, OligoDustSystem.cpp:53-87):
-    the cell densities rho[m] of the component on a grid, and its dust mix."""
+    """OligoDustSystem (DustSystem.cpp:935-976, OligoDustSystem.cpp:53-87): the cell densities rho[m] of a
+    dust component on a grid and its dust mix; or, for several components (a CompDustDistribution), a list
+    of density arrays and a list of mixes, one per component."""
 
     def __init__(self, grid, rho, mix, writeMeanIntensity=False):
         self.grid = grid
-        self.mix = mix
-        self.rho = as_f64(rho)
-        self.nlambda = mix.Nlambda()
+        multi = isinstance(mix, (list, tuple))
+        self.mixes = list(mix) if multi else [mix]
+        self.mix = self.mixes[0]
+        rhos = [as_f64(r) for r in rho] if multi else [as_f64(rho)]
+        if len(rhos) != len(self.mixes):
+            raise FatalError("one density array per dust component is needed")
+        self.rhos = rhos
+        self.rho = rhos[0] if len(rhos) == 1 else sum(rhos[1:], rhos[0].copy())
+        self.ncomp = len(rhos)
+        self.nlambda = self.mix.Nlambda()
         h = _handle()
-        check(lib().sg_dust_create(grid._h, dptr(self.rho), len(self.rho), self.nlambda, dptr(mix.kappaext),
-                                   dptr(mix.albedo), dptr(mix.asymmpar), int(bool(writeMeanIntensity)),
-                                   ctypes.byref(h)))
+        if self.ncomp == 1:
+            check(lib().sg_dust_create(grid._h, dptr(rhos[0]), len(rhos[0]), self.nlambda, dptr(mix.kappaext),
+                                       dptr(mix.albedo), dptr(mix.asymmpar), int(bool(writeMeanIntensity)),
+                                       ctypes.byref(h)))
+        else:
+            if any(m.Nlambda() != self.nlambda for m in self.mixes):
+                raise FatalError("the dust mixes disagree on the number of wavelengths")
+            allrho = as_f64(np.concatenate(rhos))
+            stack = lambda name: as_f64(np.concatenate([getattr(m, name) for m in self.mixes]))
+            self._keep = (allrho, stack("kappaext"), stack("kappasca"), stack("albedo"), stack("asymmpar"))
+            check(lib().sg_dust_create_multi(grid._h, self.ncomp, dptr(allrho), len(rhos[0]), self.nlambda,
+                                             dptr(self._keep[1]), dptr(self._keep[2]), dptr(self._keep[3]),
+                                             dptr(self._keep[4]), int(bool(writeMeanIntensity)), ctypes.byref(h)))
         self._h = h
         self.writeMeanIntensity = bool(writeMeanIntensity)
 

@@ -307,3 +307,36 @@ def fitskirt_observation(problem, resources, counts=1e6, mult=10, seed=5):
     gain = counts / cal.sum()
     data = np.random.default_rng(seed).poisson(cal * gain).astype(np.float64)
     return data, kernel, gain
+
+
+def two_dust_disc(kind="cartesian", grid_n=32, maxlevel=5, nsites=20000, pixels=64, packages=1e5, seed=4357,
+                  nsamples=20, store_abs=True):
+    """Two dust components (a CompDustDistribution of two DustComps): a thick diffuse disc (hR 6, hz 0.5,
+    face-on tau 0.5 at the first wavelength, albedo 0.7, g 0.6) and a thin dense lane (hR 3, hz 0.12, face-on
+    tau 2, albedo 0.3, g 0.1), each with its own two-wavelength mix (kappa falling by 0.6 and 0.8 to the
+    second wavelength); the disc + bulge stars of disc_cartesian; frames at 0, 60 and 88 deg. kind: cartesian
+    (grid_n^3), octree (subdivided on the lane's density to maxlevel) or voronoi (nsites drawn from the
+    lane)."""
+    comps, _, _, _ = _galaxy(False, 1.0)
+    for c in comps:
+        c.luminosities = np.array([c.luminosities[0], 0.7 * c.luminosities[0]])
+    diffuse, lane = S.ExpDiskGeometry(6.0, 0.5), S.ExpDiskGeometry(3.0, 0.12)
+    mixes = [S.SimpleOligoDustMix([1.0, 0.6], [0.7, 0.65], [0.6, 0.5]),
+             S.SimpleOligoDustMix([1.0, 0.8], [0.3, 0.25], [0.1, 0.05])]
+    scales = [S.faceon_normalization(diffuse, 0.5, 1.0), S.faceon_normalization(lane, 2.0, 1.0)]
+    sites = None
+    if kind == "cartesian":
+        grid = S.CartesianDustGrid.uniform(EXTENT, grid_n, grid_n, grid_n)
+    elif kind == "octree":
+        grid = S.OctTreeDustGrid.subdivide(EXTENT, lane, scales[1], total_mass=scales[1], minlevel=2,
+                                           maxlevel=maxlevel, maxmassfraction=1e-4, nsamples=nsamples, seed=seed)
+    else:
+        sites = sample_sites(lane, nsites, EXTENT, seed)
+        grid = S.VoronoiDustGrid(EXTENT, sites)
+    if sites is not None:
+        rhos = [s * S.geometry_density(g, sites) for g, s in zip((diffuse, lane), scales)]
+    else:
+        rhos = [S.sample_density(grid, g, s, nsamples, seed) for g, s in zip((diffuse, lane), scales)]
+    d = dict(name="two_dust_disc-" + kind, kind=kind, grid=grid, rho=rhos, mix=mixes, comps=comps, dgeom=lane,
+             instr=_frames((0.0, 60.0, 88.0), pixels, nlambda=2), store_abs=store_abs)
+    return _finish(d, packages, seed)

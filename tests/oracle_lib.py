@@ -27,10 +27,14 @@ _SIGS = {
     "or_scene_set_voronoi": (ctypes.c_int, [_V, _D, _D, ctypes.c_int64, _I64, _I32]),
     "or_scene_num_cells": (ctypes.c_int64, [_V]),
     "or_scene_set_dust": (ctypes.c_int, [_V, _D, ctypes.c_int64, ctypes.c_int, _D, _D, _D, ctypes.c_int]),
+    "or_scene_set_dust_multi": (ctypes.c_int, [_V, ctypes.c_int, _D, ctypes.c_int64, ctypes.c_int, _D, _D, _D, _D,
+                                               ctypes.c_int]),
     "or_scene_add_star": (ctypes.c_int, [_V, _D, _D, _D, _D, ctypes.c_int]),
     "or_scene_add_instrument": (ctypes.c_int, [_V, _D]),
     "or_scene_path": (ctypes.c_int, [_V, _D, ctypes.c_int, _I32, _D]),
     "or_scene_whichcell": (ctypes.c_int, [_V, _D]),
+    "or_scatter_phase": (ctypes.c_double, [_V, ctypes.c_int, _D, _D, _D]),
+    "or_random_mix": (ctypes.c_int, [_V, ctypes.c_int, _D, ctypes.c_double]),
     "or_scene_opticaldepth": (ctypes.c_double, [_V, _D, ctypes.c_int]),
     "or_scene_pathlength": (None, [_V, _D, ctypes.c_int, ctypes.c_int, _D, _D]),
     "or_run_reference": (ctypes.c_int, [_V, _D, ctypes.c_int, _D, _D, _U64]),
@@ -133,6 +137,18 @@ class Scene:
         self.nlambda = len(k)
         self.ncells = len(rho)
         assert self.L.or_scene_set_dust(self.h, dp(rho), len(rho), len(k), dp(k), dp(a), dp(gg), int(store_abs)) == 0
+        return self
+
+    def dust_multi(self, rhos, mixes, store_abs=False):
+        """Several dust components: one density array and one SimpleOligoDustMix-like object per component."""
+        r = self.keep(f64(np.concatenate([f64(x) for x in rhos])))
+        cat = lambda name: self.keep(f64(np.concatenate([getattr(m, name) for m in mixes])))
+        k, ks, a, gg = cat("kappaext"), cat("kappasca"), cat("albedo"), cat("asymmpar")
+        n = len(rhos[0])
+        self.nlambda = len(mixes[0].kappaext)
+        self.ncells = n
+        assert self.L.or_scene_set_dust_multi(self.h, len(rhos), dp(r), n, len(mixes[0].kappaext), dp(k), dp(ks), dp(a),
+                                              dp(gg), int(store_abs)) == 0
         return self
 
     def star(self, params, lum, ts=None, tM=None):
@@ -248,7 +264,10 @@ def scene_from_workload(w, variant=None):
     else:
         sc.voronoi(g.extent, g.sites, g.nb_offsets, g.nb_ids)
     mix = w["mix"]
-    sc.dust(w["rho"], mix.kappaext, mix.albedo, mix.asymmpar, w.get("store_abs", False))
+    if isinstance(mix, (list, tuple)):
+        sc.dust_multi(w["rho"], mix, w.get("store_abs", False))
+    else:
+        sc.dust(w["rho"], mix.kappaext, mix.albedo, mix.asymmpar, w.get("store_abs", False))
     for c in w["comps"]:
         geo = c.geometry
         ts = getattr(geo, "s", None)

@@ -324,3 +324,111 @@ def test_absorption_closed_form_point_source_box():
     expect = (1.0 - albedo) * absorbed_fraction_quadrature(tau0)
     assert int(c[3]) == 0     # the stop test after the first own-path walk ends every packet
     assert abs(got - expect) < 2e-3 * expect, (got, expect)
+
+
+class _Mix:
+    """A SimpleOligoDustMix's derived coefficients for one wavelength."""
+
+    def __init__(self, kext, albedo, g):
+        self.kappaext = np.array([kext * (1.0 - albedo) + kext * albedo])
+        self.kappasca = np.array([kext * albedo])
+        self.albedo = np.array([albedo])
+        self.asymmpar = np.array([g])
+
+
+def test_two_component_absorption_closed_form():
+    """simulateescapeandabsorption with several components (MonteCarloSimulation.cpp:463-508): optical depths
+    from KappaRho summed over the components (DustSystem.cpp:459-485) and each cell's albedo
+    sum(rho kappasca) / sum(rho kappaext). Two components of different opacity and albedo, uniform in the
+    3 x 3 x 3 cube with kappa*rho = 1 in all: the absorbed luminosity must be (1 - 0.52) of the single
+    component's closed form (0.52 = (0.3*2*0.8 + 0.4*1*0.1) / 1): a wrong sum or albedo fails."""
+    from tests.helpers import absorbed_fraction_quadrature
+    sc = O.Scene()
+    edges = np.linspace(-1.0, 1.0, 4)
+    sc.cartesian(np.array([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]), edges, edges, edges)
+    sc.dust_multi([np.full(27, 0.3), np.full(27, 0.4)], [_Mix(2.0, 0.8, 0.5), _Mix(1.0, 0.1, -0.2)], store_abs=True)
+    ts, tM = np.zeros(101), np.zeros(101)
+    O.lib().or_sersic_table(1.0, O.dp(ts), O.dp(tM))
+    star = np.zeros(16)
+    star[0], star[1], star[2] = 2, 1.0, 1e-6
+    sc.star(star, [1.0], ts, tM)
+    sc.instrument(np.array([1e4, 0.0, 0.0, 0.0, 8, 8, 2.0, 2.0, 0.0, 0.0]))
+    n = 100000
+
+    class Sim:
+        mc = np.array([n, 1.0, 0, 0.5, 4357, 0.5, 0, 0.0])
+    _, c = sc.run_packets({"sim": Sim()}, 0, 0, n, n)
+    got = float(np.sum(sc.labs))
+    expect = (1.0 - 0.52) * absorbed_fraction_quadrature(1.0)
+    assert int(c[3]) == 0
+    assert abs(got - expect) < 2e-3 * expect, (got, expect)
+
+
+def _two_cell_scene(rho_a, rho_b, mixes):
+    sc = O.Scene()
+    sc.cartesian(np.array([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]), np.array([-1.0, 0.0, 1.0]), np.array([-1.0, 1.0]),
+                 np.array([-1.0, 1.0]))
+    sc.dust_multi([np.asarray(r, float) for r in zip(rho_a, rho_b)], mixes)
+    return sc
+
+
+def test_random_mix_intervals():
+    """DustSystem::randomMixForPosition (DustSystem.cpp:855-869): component h is picked for uniform deviates
+    u in [X_h, X_h+1) of the normalised cumulative kappasca(h) rho(m, h) (NR::cdf, NR::locate_clip): the
+    fraction of a fine grid of u landing on each component equals its weight, a u exactly on a border picks
+    the upper component, a component without density is never picked, outside the grid the first one is
+    returned."""
+    mixes = [_Mix(1.0, 0.5, 0.1), _Mix(2.0, 0.25, 0.3), _Mix(0.5, 0.9, 0.7)]
+    sc = _two_cell_scene([0.2, 1.0, 0.6], [0.0, 0.3, 0.0], mixes)   # cell 0 (x<0): rho per component
+    lib = O.lib()
+    w = np.array([0.5 * 0.2, 0.5 * 1.0, 0.45 * 0.6])
+    X = np.concatenate([[0.0], np.cumsum(w)]) / w.sum()
+    r = O.f64([-0.5, 0.0, 0.0])
+    us = (np.arange(200000) + 0.5) / 200000
+    picks = np.array([lib.or_random_mix(sc.h, 0, O.dp(r), float(u)) for u in us])
+    for h in range(3):
+        assert abs(np.mean(picks == h) - w[h] / w.sum()) < 1e-4
+    assert lib.or_random_mix(sc.h, 0, O.dp(r), float(X[1])) == 1
+    assert lib.or_random_mix(sc.h, 0, O.dp(r), float(np.nextafter(X[1], 0))) == 0
+    r2 = O.f64([0.5, 0.0, 0.0])       # cell 1: only component 1 has density
+    assert all(lib.or_random_mix(sc.h, 0, O.dp(r2), float(u)) == 1 for u in (0.0, 0.3, 0.999999))
+    assert lib.or_random_mix(sc.h, 0, O.dp(O.f64([3.0, 0.0, 0.0])), 0.7) == 0
+
+
+def test_scatter_phase_mixture():
+    """peeloffscattering's phase value with several components (MonteCarloSimulation.cpp:313-343): the
+    components' Henyey-Greenstein values weighted by normalised kappasca(h) rho(m, h). The weights sum to one,
+    so the value averages to 1 over the sphere of observer directions; identical mixes give the single mix's
+    value; in a cell where one component has no density it is the other's value; outside the grid, or where
+    there is no scattering dust, the peel-off is aborted."""
+    lib = O.lib()
+    mixes = [_Mix(1.0, 0.5, 0.7), _Mix(3.0, 0.2, -0.3)]
+    sc = _two_cell_scene([0.4, 0.9], [0.0, 0.5], mixes)
+    k = O.f64([0.3, -0.4, np.sqrt(1 - 0.25)])
+    r = O.f64([-0.5, 0.1, 0.2])
+    # average over observer directions by Gauss-Legendre in cos(theta) about k and uniform azimuth
+    x, wq = np.polynomial.legendre.leggauss(200)
+    ez = k / np.linalg.norm(k)
+    ex = np.cross(ez, [1.0, 0.0, 0.0]); ex /= np.linalg.norm(ex)
+    ey = np.cross(ez, ex)
+    tot = 0.0
+    for phi in np.linspace(0, 2 * np.pi, 16, endpoint=False):
+        for c, wc in zip(x, wq):
+            s = np.sqrt(1 - c * c)
+            ko = O.f64(c * ez + s * (np.cos(phi) * ex + np.sin(phi) * ey))
+            tot += wc * lib.or_scatter_phase(sc.h, 0, O.dp(r), O.dp(k), O.dp(ko))
+    assert abs(tot / (2 * 16) - 1.0) < 1e-6
+    # single-mix limits against one-component scenes
+    ko = O.f64([0.0, 0.6, 0.8])
+    r2 = O.f64([0.5, 0.0, 0.0])
+    one = O.Scene()
+    one.cartesian(np.array([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]), np.array([-1.0, 0.0, 1.0]), np.array([-1.0, 1.0]),
+                  np.array([-1.0, 1.0]))
+    one.dust(np.ones(2), mixes[1].kappaext, mixes[1].albedo, mixes[1].asymmpar)
+    v_one = lib.or_scatter_phase(one.h, 0, O.dp(r2), O.dp(k), O.dp(ko))
+    assert abs(lib.or_scatter_phase(sc.h, 0, O.dp(r2), O.dp(k), O.dp(ko)) - v_one) < 1e-14 * v_one
+    same = _two_cell_scene([0.4, 0.9], [0.2, 0.5], [mixes[1], mixes[1]])
+    assert abs(lib.or_scatter_phase(same.h, 0, O.dp(r), O.dp(k), O.dp(ko)) - v_one) < 1e-14 * v_one
+    assert lib.or_scatter_phase(sc.h, 0, O.dp(O.f64([2.0, 0.0, 0.0])), O.dp(k), O.dp(ko)) == -1.0
+    empty = _two_cell_scene([0.0, 0.0], [0.0, 0.0], mixes)
+    assert lib.or_scatter_phase(empty.h, 0, O.dp(r), O.dp(k), O.dp(ko)) == -1.0

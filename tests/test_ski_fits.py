@@ -329,3 +329,31 @@ def test_faceon_normalization_needs_an_axisymmetric_geometry(resources):
     t = ski_text().replace(disc, spiral).replace('<FaceOnDustCompNormalization wavelength="0.55 micron" opticalDepth="1"/>',
                                                  '<DustMassDustCompNormalization dustMass="1e7 Msun"/>')
     assert ski.Spec(t, resources).dust["scale"] > 0
+
+
+def two_dust_ski(grid="cartesian"):
+    """ski_text with a second DustComp (a thin lane of its own mix and dust-mass normalisation)."""
+    second = """
+                  <DustComp>
+                    <geometry type="Geometry"><ExpDiskGeometry radialScale="3 kpc" axialScale="120 pc"/></geometry>
+                    <mix type="DustMix"><SimpleOligoDustMix opacities="2 m2/kg" albedos="0.3" asymmetryParameters="0.1"/></mix>
+                    <normalization type="DustCompNormalization">
+                      <DustMassDustCompNormalization dustMass="1e7 Msun"/>
+                    </normalization>
+                  </DustComp>
+                </components>"""
+    return ski_text(grid=grid).replace("</DustComp>\n                </components>", "</DustComp>" + second, 1)
+
+
+def test_ski_two_dust_components(resources):
+    from skirt7_amd import ski
+    sp = ski.Spec(two_dust_ski(), resources)
+    comps = sp.dust["components"]
+    assert len(comps) == 2
+    assert comps[0]["opacities"] == [1.0] and comps[1]["opacities"] == [2.0]
+    assert comps[1]["albedos"] == [0.3] and comps[1]["asymmetry"] == [0.1]
+    # dust mass normalisation: Msun in kg over pc^2 (the geometry's density is per pc^3)
+    from skirt7_amd.output import MSUN, PC
+    assert abs(comps[1]["scale"] - 1e7 * MSUN / PC ** 2) < 1e-9 * comps[1]["scale"]
+    # the first component's entries stay at the top level
+    assert sp.dust["scale"] == comps[0]["scale"] and sp.dust["opacities"] == [1.0]
