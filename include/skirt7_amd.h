@@ -276,17 +276,27 @@ int sg_geometry_density(const double* geom16, int64_t n, const double* xyz, doub
 int sg_tree_subdivide(int arity, const double ext[6], const double* geom16, double scale, double total_mass,
                       int minlevel, int maxlevel, double maxmassfraction, int nsamples, uint32_t seed,
                       uint8_t* flags, int64_t cap, int64_t* nnodes);
-/* The same for a CompDustDistribution of ncomp components: the density is the sum over components of
- * scales[h] times geometry h's density (geoms16 holds ncomp records of 16), CompDustDistribution::density. */
+/* The same for a CompDustDistribution of ncomp components (the density is the sum over components of
+ * scales[h] times geometry h's density; geoms16 holds ncomp records of 16, CompDustDistribution::density),
+ * with all three stopping criteria of TreeDustGrid::subdivide (TreeDustGrid.cpp:194-220): a node below
+ * maxlevel is split when no criterion is set (all three 0), or when its mass fraction reaches
+ * maxmassfraction, its optical depth kappaV M / V^(2/3) (kappaV 2600 m2/kg) reaches maxopticaldepth or its
+ * samples' density dispersion (max - min) / max reaches maxdensdisp (criteria at 0 are off). */
 int sg_tree_subdivide_multi(int arity, const double ext[6], int ncomp, const double* geoms16, const double* scales,
-                            double total_mass, int minlevel, int maxlevel, double maxmassfraction, int nsamples,
-                            uint32_t seed, uint8_t* flags, int64_t cap, int64_t* nnodes);
+                            double total_mass, int minlevel, int maxlevel, double maxmassfraction,
+                            double maxopticaldepth, double maxdensdisp, int nsamples, uint32_t seed, uint8_t* flags,
+                            int64_t cap, int64_t* nnodes);
 /* The same subdivision with node masses from a density field given per voxel of a regular
  * nx x ny x nz grid over ext (value [i][j][k] at i*ny*nz + j*nz + k, constant inside a voxel): exact mass in
  * each node's box, the total being the field's mass. */
 int sg_tree_subdivide_field(int arity, const double ext[6], const double* field, int nx, int ny, int nz,
                             int minlevel, int maxlevel, double maxmassfraction, uint8_t* flags, int64_t cap,
                             int64_t* nnodes);
+/* The same with the optical-depth criterion (TreeNodeBoxDensityCalculator::opticalDepth); a field's exact
+ * node masses give no density dispersion, as in the reference. */
+int sg_tree_subdivide_field_od(int arity, const double ext[6], const double* field, int nx, int ny, int nz,
+                               int minlevel, int maxlevel, double maxmassfraction, double maxopticaldepth,
+                               uint8_t* flags, int64_t cap, int64_t* nnodes);
 /* Cell densities from such a voxel field: mass in the cell's box over its volume, the exact mass-in-box
  * densities of TreeDustGrid's particle path (TreeDustGrid.cpp:647-660). Cartesian and tree grids. */
 int sg_grid_field_density(sg_grid* g, const double* field, int nx, int ny, int nz, double* rho_out);

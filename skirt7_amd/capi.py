@@ -102,8 +102,12 @@ _SIGS = {
                                          ctypes.c_int64, _I64]),
     "sg_grid_sample_density": (ctypes.c_int, [_VP, _D, ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _D]),
     "sg_tree_subdivide_multi": (ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_double,
-                                               ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
-                                               ctypes.c_uint32, _U8, ctypes.c_int64, _I64]),
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _U8, ctypes.c_int64,
+                                               _I64]),
+    "sg_tree_subdivide_field_od": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _U8,
+                                                  ctypes.c_int64, _I64]),
     "sg_grid_resample_density": (ctypes.c_int, [_VP, _D, ctypes.c_double, ctypes.c_int, ctypes.c_uint32]),
     "sg_ga_create": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_uint32, _VPP]),

@@ -296,12 +296,39 @@ using namespace sg;
 
 namespace sg {
 
+// The stopping criteria of TreeDustGrid::subdivide (TreeDustGrid.cpp:194-220): a node below maxlevel is
+// split when no criterion is set, or when at least one set criterion is not met -- its mass reaches
+// maxmassfraction of the total, its optical depth kappaV M / V^(2/3) (TreeNodeSampleDensityCalculator.cpp,
+// kappaV = 2600 m2/kg, Units.cpp:29) reaches maxopticaldepth, or the dispersion (max - min) / max of its
+// density samples reaches maxdensdisp.
+struct Criteria {
+    double maxmassfraction = 0, maxopticaldepth = 0, maxdensdisp = 0;
+};
+// A node's mass and, for sampled densities, the smallest and largest sample (negative when not known).
+struct NodeStats {
+    double mass, minrho = -1, maxrho = -1;
+};
+static bool need_division(const Criteria& c, const NodeStats& st, const double* q, double total_mass)
+{
+    if (c.maxmassfraction == 0 && c.maxopticaldepth == 0 && c.maxdensdisp == 0) return true;
+    if (c.maxmassfraction > 0 && st.mass / total_mass >= c.maxmassfraction) return true;
+    if (c.maxopticaldepth > 0) {
+        double vol = (q[3] - q[0]) * (q[4] - q[1]) * (q[5] - q[2]);
+        if (2600.0 * st.mass / std::pow(vol, 2. / 3.) >= c.maxopticaldepth) return true;
+    }
+    if (c.maxdensdisp > 0) {
+        double disp = st.maxrho > 0 ? (st.maxrho - st.minrho) / st.maxrho : 0;
+        if (disp >= c.maxdensdisp) return true;
+    }
+    return false;
+}
+
 // TreeDustGrid::setupSelfBefore/subdivide (TreeDustGrid.cpp:49-230): breadth-first subdivision with the
 // reference's node numbering and child boxes; a node is split while its level is at most minlevel, or
-// when below maxlevel its mass (mass(box)) reaches maxmassfraction of the total.
-template <class MassFn>
-static int subdivide_tree(int arity, const double ext[6], int minlevel, int maxlevel, double maxmassfraction,
-                          double total_mass, MassFn mass, uint8_t* flags, int64_t cap, int64_t* nnodes)
+// when below maxlevel the criteria above ask for it (stats(box, node) gives its mass and samples).
+template <class StatFn>
+static int subdivide_tree(int arity, const double ext[6], int minlevel, int maxlevel, const Criteria& crit,
+                          double total_mass, StatFn stats, uint8_t* flags, int64_t cap, int64_t* nnodes)
 {
     std::vector<double> b(ext, ext + 6);
     std::vector<int16_t> level(1, 0);
@@ -315,7 +342,7 @@ static int subdivide_tree(int arity, const double ext[6], int minlevel, int maxl
             int lev = level[l];
             if (lev <= minlevel) { split[i] = 1; return; }
             if (lev >= maxlevel) return;
-            if (mass(&b[6 * l], l) / total_mass >= maxmassfraction) split[i] = 1;
+            if (need_division(crit, stats(&b[6 * l], l), &b[6 * l], total_mass)) split[i] = 1;
         });
         for (int64_t l = begin; l < end; l++) {
             f.push_back(split[l - begin]);
@@ -490,24 +517,30 @@ int sg_tree_subdivide(int arity, const double ext[6], const double* geom16, doub
                       uint8_t* flags, int64_t cap, int64_t* nnodes)
 {
     return sg_tree_subdivide_multi(arity, ext, 1, geom16, &scale, total_mass, minlevel, maxlevel, maxmassfraction,
-                                   nsamples, seed, flags, cap, nnodes);
+                                   0.0, 0.0, nsamples, seed, flags, cap, nnodes);
 }
 
 // The same with the density of a CompDustDistribution: the sum over components of scale(h) times the
 // geometry's density (CompDustDistribution::density), sampled at the same positions.
 int sg_tree_subdivide_multi(int arity, const double ext[6], int ncomp, const double* geoms16, const double* scales,
-                            double total_mass, int minlevel, int maxlevel, double maxmassfraction, int nsamples,
-                            uint32_t seed, uint8_t* flags, int64_t cap, int64_t* nnodes)
+                            double total_mass, int minlevel, int maxlevel, double maxmassfraction,
+                            double maxopticaldepth, double maxdensdisp, int nsamples, uint32_t seed, uint8_t* flags,
+                            int64_t cap, int64_t* nnodes)
 {
     if (!(arity == 8 || arity == 2)) { set_error("arity must be 8 or 2"); return -1; }
     if (maxlevel < 2 || maxlevel <= minlevel) { set_error("bad tree levels"); return -1; }
     if (ncomp < 1 || !geoms16 || !scales) { set_error("at least one dust component is needed"); return -1; }
+    if (nsamples < 1) { set_error("at least one density sample per node is needed"); return -1; }
     std::vector<GeomDensity> dens;
     for (int h = 0; h < ncomp; h++) dens.emplace_back(geoms16 + 16 * h);
-    // node mass from nsamples counter-based density samples in the node's box
-    auto mass = [&](const double* q, int64_t l) {
+    Criteria crit;
+    crit.maxmassfraction = maxmassfraction;
+    crit.maxopticaldepth = maxopticaldepth;
+    crit.maxdensdisp = maxdensdisp;
+    // TreeNodeSampleDensityCalculator: nsamples counter-based density samples in the node's box
+    auto stats = [&](const double* q, int64_t l) {
         double vol = (q[3] - q[0]) * (q[4] - q[1]) * (q[5] - q[2]);
-        double sum = 0;
+        double sum = 0, lo = 0, hi = 0;
         for (int j = 0; j < nsamples; j++) {
             double u[3];
             sample3(seed, (uint64_t)l, (uint32_t)j, u);
@@ -515,10 +548,16 @@ int sg_tree_subdivide_multi(int arity, const double ext[6], int ncomp, const dou
             double d = 0;
             for (int h = 0; h < ncomp; h++) d += scales[h] * dens[h](x, y, z);
             sum += d;
+            lo = j ? std::min(lo, d) : d;
+            hi = j ? std::max(hi, d) : d;
         }
-        return sum / nsamples * vol;
+        NodeStats st;
+        st.mass = sum / nsamples * vol;
+        st.minrho = lo;
+        st.maxrho = hi;
+        return st;
     };
-    return subdivide_tree(arity, ext, minlevel, maxlevel, maxmassfraction, total_mass, mass, flags, cap, nnodes);
+    return subdivide_tree(arity, ext, minlevel, maxlevel, crit, total_mass, stats, flags, cap, nnodes);
 }
 
 // The same subdivision with node masses from a voxel density field (exact mass in the node's box).
@@ -526,13 +565,30 @@ int sg_tree_subdivide_field(int arity, const double ext[6], const double* field,
                             int minlevel, int maxlevel, double maxmassfraction, uint8_t* flags, int64_t cap,
                             int64_t* nnodes)
 {
+    return sg_tree_subdivide_field_od(arity, ext, field, nx, ny, nz, minlevel, maxlevel, maxmassfraction, 0.0, flags,
+                                      cap, nnodes);
+}
+
+// With the optical-depth criterion too; TreeNodeBoxDensityCalculator gives masses only, so there is no
+// density-dispersion criterion here (the reference throws "Calculation is not supported").
+int sg_tree_subdivide_field_od(int arity, const double ext[6], const double* field, int nx, int ny, int nz,
+                               int minlevel, int maxlevel, double maxmassfraction, double maxopticaldepth,
+                               uint8_t* flags, int64_t cap, int64_t* nnodes)
+{
     if (!(arity == 8 || arity == 2)) { set_error("arity must be 8 or 2"); return -1; }
     if (maxlevel < 2 || maxlevel <= minlevel) { set_error("bad tree levels"); return -1; }
     if (nx < 1 || ny < 1 || nz < 1) { set_error("empty density field"); return -1; }
     VoxelField vf(ext, field, nx, ny, nz);
     if (!(vf.total > 0)) { set_error("the density field has no mass"); return -1; }
-    auto mass = [&](const double* q, int64_t) { return vf.mass(q); };
-    return subdivide_tree(arity, ext, minlevel, maxlevel, maxmassfraction, vf.total, mass, flags, cap, nnodes);
+    Criteria crit;
+    crit.maxmassfraction = maxmassfraction;
+    crit.maxopticaldepth = maxopticaldepth;
+    auto stats = [&](const double* q, int64_t) {
+        NodeStats st;
+        st.mass = vf.mass(q);
+        return st;
+    };
+    return subdivide_tree(arity, ext, minlevel, maxlevel, crit, vf.total, stats, flags, cap, nnodes);
 }
 
 // Cell densities from a voxel density field: the mass in each cell's box over its volume (the exact

@@ -309,9 +309,6 @@ class Spec:
                 bins.append(val.integer(m, "numBins", "100"))
             spec["bins"] = bins
         elif g.cls in ("OctTreeDustGrid", "BinTreeDustGrid"):
-            for crit in ("maxOpticalDepth", "maxDensDispFraction"):
-                if val.double(g, crit, None, "0") != 0.0:
-                    raise FatalError("tree subdivision criterion %s is not supported" % crit)
             if g.cls == "BinTreeDustGrid" and val.string(g, "directionMethod", "Alternating") != "Alternating":
                 raise FatalError("only the Alternating direction method is supported")
             if g.cls == "OctTreeDustGrid" and val.boolean(g, "barycentric", "no"):
@@ -324,6 +321,8 @@ class Spec:
                 raise FatalError("Bookkeeping method is not compatible with binary tree")
             spec.update(minLevel=val.integer(g, "minLevel", "2"), maxLevel=val.integer(g, "maxLevel", "6"),
                         maxMassFraction=val.double(g, "maxMassFraction", None, "1e-6"),
+                        maxOpticalDepth=val.double(g, "maxOpticalDepth", None, "0"),
+                        maxDensDispFraction=val.double(g, "maxDensDispFraction", None, "0"),
                         sampleCount=val.integer(g, "sampleCount", "100"), search=method)
         elif g.cls == "VoronoiDustGrid":
             spec.update(numParticles=val.integer(g, "numParticles", "500"),
@@ -390,7 +389,8 @@ class Simulation:
             cls = S.OctTreeDustGrid if g["kind"] == "OctTreeDustGrid" else S.BinTreeDustGrid
             return cls.subdivide(ext, geoms, scales, total_mass=float(sum(scales)), minlevel=g["minLevel"],
                                  maxlevel=g["maxLevel"], maxmassfraction=g["maxMassFraction"],
-                                 nsamples=g["sampleCount"], seed=seed, search=_SEARCH[g["search"]])
+                                 nsamples=g["sampleCount"], seed=seed, search=_SEARCH[g["search"]],
+                                 maxopticaldepth=g["maxOpticalDepth"], maxdensdispfraction=g["maxDensDispFraction"])
         n = g["numParticles"]
         if g["distribution"] == "DustDensity":
             from . import workloads
@@ -406,6 +406,8 @@ class Simulation:
         else:
             rng = np.random.default_rng(seed)
             self.sites = ext[:3] + rng.random((n, 3)) * (ext[3:] - ext[:3])
+        # VoronoiMesh::buildMesh orders the cells by x and drops particles too close to another
+        self.sites = S.voronoi_cells(ext, self.sites)
         return S.VoronoiDustGrid(ext, self.sites)
 
     def run(self, precision=SG_MIXED):

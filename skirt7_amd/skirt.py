@@ -121,9 +121,10 @@ class TreeDustGrid(DustGrid):
 
     @classmethod
     def subdivide(cls, extent, geometry, scale, total_mass, minlevel=2, maxlevel=6, maxmassfraction=1e-6,
-                  nsamples=100, seed=4357, search=SG_NEIGHBOR):
-        """Builds the subdivision from a dust geometry density (TreeDustGrid.cpp:166-230); geometry and scale
-        may be lists, one per dust component, for a CompDustDistribution's summed density."""
+                  nsamples=100, seed=4357, search=SG_NEIGHBOR, maxopticaldepth=0.0, maxdensdispfraction=0.0):
+        """Builds the subdivision from a dust geometry density (TreeDustGrid.cpp:166-230) with the reference's
+        stopping criteria (0 switches one off; all three off subdivides to maxlevel); geometry and scale may be
+        lists, one per dust component, for a CompDustDistribution's summed density."""
         e = as_f64(extent)
         cap = 1 << 27
         flags = np.zeros(cap, np.uint8)
@@ -134,14 +135,17 @@ class TreeDustGrid(DustGrid):
             raise FatalError("one scale per dust component is needed")
         g = as_f64(np.concatenate([x.params() for x in geoms]))
         check(lib().sg_tree_subdivide_multi(cls.arity, dptr(e), len(geoms), dptr(g), dptr(scales), total_mass,
-                                            minlevel, maxlevel, maxmassfraction, nsamples, seed,
-                                            flags.ctypes.data_as(capi._U8), cap, ctypes.byref(n)))
+                                            minlevel, maxlevel, maxmassfraction, maxopticaldepth,
+                                            maxdensdispfraction, nsamples, seed, flags.ctypes.data_as(capi._U8),
+                                            cap, ctypes.byref(n)))
         return cls(e, flags[:n.value].copy(), search)
 
 
     @classmethod
-    def subdivide_field(cls, extent, field, minlevel=2, maxlevel=30, maxmassfraction=1e-6, search=SG_NEIGHBOR):
-        """The subdivision with exact node masses from a voxel density field field[i, j, k] over the box."""
+    def subdivide_field(cls, extent, field, minlevel=2, maxlevel=30, maxmassfraction=1e-6, search=SG_NEIGHBOR,
+                        maxopticaldepth=0.0):
+        """The subdivision with exact node masses from a voxel density field field[i, j, k] over the box (mass
+        fraction and optical-depth criteria)."""
         e = as_f64(extent)
         f = as_f64(field)
         nx, ny, nz = f.shape
@@ -149,8 +153,9 @@ class TreeDustGrid(DustGrid):
         while True:
             flags = np.zeros(cap, np.uint8)
             n = ctypes.c_int64()
-            st = lib().sg_tree_subdivide_field(cls.arity, dptr(e), dptr(f), nx, ny, nz, minlevel, maxlevel,
-                                               maxmassfraction, flags.ctypes.data_as(capi._U8), cap, ctypes.byref(n))
+            st = lib().sg_tree_subdivide_field_od(cls.arity, dptr(e), dptr(f), nx, ny, nz, minlevel, maxlevel,
+                                                  maxmassfraction, maxopticaldepth, flags.ctypes.data_as(capi._U8),
+                                                  cap, ctypes.byref(n))
             if st == -5:
                 cap = max(2 * cap, n.value + 1024)
                 continue
@@ -187,9 +192,33 @@ def voronoi_neighbors(extent, sites):
         return off, ids[:nids.value].copy()
 
 
+def voronoi_cells(extent, particles):
+    """VoronoiMesh::buildMesh with nearby particles removed (VoronoiMesh.cpp:297-350, as VoronoiDustGrid builds
+    its mesh from generated or imported particles): the particles sorted on x; one that lies within
+    eps = 1e-12 |box widths| of a later particle in that order is dropped; the survivors, in sorted order, are
+    the cells (cell m is the m-th survivor). Returns their positions."""
+    e = as_f64(extent)
+    p = as_f64(particles).reshape(-1, 3)
+    n = p.shape[0]
+    order = np.argsort(p[:, 0], kind="stable")
+    x = p[order, 0]
+    eps = 1e-12 * float(np.linalg.norm(e[3:] - e[:3]))
+    keep = np.ones(n, bool)
+    for i in np.flatnonzero(np.diff(x) < eps):
+        j = i + 1
+        while j < n and x[j] - x[i] < eps:
+            d = p[order[j]] - p[order[i]]
+            if float(d @ d) < eps * eps:
+                keep[i] = False
+                break
+            j += 1
+    return p[order[keep]]
+
+
 class VoronoiDustGrid(DustGrid):
     """VoronoiDustGrid / VoronoiMesh (VoronoiMesh.cpp:297-405,541-570,778-873) from generator sites and
-    neighbour lists (walls -1..-6); the lists are built from the sites when not given."""
+    neighbour lists (walls -1..-6); the lists are built from the sites when not given. The sites are the cells
+    in the given order; voronoi_cells turns a particle set into them as the reference's mesh does."""
 
     def __init__(self, extent, sites, nb_offsets=None, nb_ids=None):
         self.extent = as_f64(extent)

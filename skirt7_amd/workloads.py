@@ -156,7 +156,7 @@ def spiral_voronoi(nsites=1_000_000, nlambda=50, pixels=256, packages=2e7, seed=
     idx = np.arange(nlambda) / max(1, nlambda - 1)
     mix = S.SimpleOligoDustMix(kappa, 0.7 - 0.3 * idx, 0.7 - 0.4 * idx)
     scale = S.faceon_normalization(dgeom.geometry, tau_v, 1.0)
-    sites = sample_sites(dgeom, nsites, EXTENT, seed)
+    sites = S.voronoi_cells(EXTENT, sample_sites(dgeom, nsites, EXTENT, seed))
     grid = S.VoronoiDustGrid(EXTENT, sites)
     rho = scale * S.geometry_density(dgeom, sites)
     comps = [S.OligoStellarComp(c.geometry, [c.luminosities[0]] * nlambda) for c in comps]
@@ -331,7 +331,7 @@ def two_dust_disc(kind="cartesian", grid_n=32, maxlevel=5, nsites=20000, pixels=
         grid = S.OctTreeDustGrid.subdivide(EXTENT, lane, scales[1], total_mass=scales[1], minlevel=2,
                                            maxlevel=maxlevel, maxmassfraction=1e-4, nsamples=nsamples, seed=seed)
     else:
-        sites = sample_sites(lane, nsites, EXTENT, seed)
+        sites = S.voronoi_cells(EXTENT, sample_sites(lane, nsites, EXTENT, seed))
         grid = S.VoronoiDustGrid(EXTENT, sites)
     if sites is not None:
         rhos = [s * S.geometry_density(g, sites) for g, s in zip((diffuse, lane), scales)]

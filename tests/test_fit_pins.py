@@ -162,3 +162,133 @@ def test_voxel_field_tree_masses_exact():
                 hi[a] = c
                 boxes += [lo, hi]
                 levels += [lev + 1, lev + 1]
+
+
+def test_voronoi_cells_order_and_nearby_removal():
+    """VoronoiMesh::buildMesh with removeNearby (VoronoiMesh.cpp:297-350): cells are the particles sorted on x;
+    a particle within eps = 1e-12 |widths| of one later in that order is dropped (the later one stays), so a
+    set with planted near-duplicates loses exactly those, and particles at least eps apart all stay."""
+    from skirt7_amd import skirt as S
+    ext = np.array([-1.0, -2.0, -3.0, 1.0, 2.0, 3.0])
+    eps = 1e-12 * np.linalg.norm([2.0, 4.0, 6.0])
+    rng = np.random.default_rng(3)
+    p = rng.uniform(-1, 1, (500, 3)) * [1, 2, 3]
+    dup = p[[5, 17, 300]] + np.array([[0.3, 0.0, 0.0], [0.0, 0.5, 0.0], [0.2, 0.2, 0.2]]) * eps
+    far = p[[7]] + np.array([[0.0, 0.0, 1.5]]) * eps            # same x, farther than eps: stays
+    allp = np.concatenate([p, dup, far])
+    cells = S.voronoi_cells(ext, allp)
+    assert cells.shape[0] == allp.shape[0] - 3
+    assert np.all(np.diff(cells[:, 0]) >= 0)
+    # every original particle position or its near twin is present once
+    for q in p:
+        d = np.linalg.norm(cells - q, axis=1)
+        assert np.sum(d < eps) == 1
+    assert np.any(np.all(cells == far[0], axis=1))
+    # which of a pair stays: the later one in x order
+    a = np.array([[0.1, 0.2, 0.3], [0.1 + 0.5 * eps, 0.2, 0.3]])
+    kept = S.voronoi_cells(ext, a[::-1])
+    assert kept.shape == (1, 3) and np.array_equal(kept[0], a[1])
+
+
+def test_tree_criteria_optical_depth_and_combinations():
+    """TreeDustGrid::subdivide's stopping criteria (TreeDustGrid.cpp:194-220) on a voxel field's exact node
+    masses: a node below maxlevel is split when no criterion is set, or when its mass fraction reaches
+    maxMassFraction or its optical depth kappaV M / V^(2/3) (kappaV = 2600) reaches maxOpticalDepth. Every
+    decision is checked against brute-force masses, for the optical depth alone, both criteria together and
+    none (every node split to maxlevel)."""
+    import ctypes
+    from skirt7_amd import capi
+    rng = np.random.default_rng(9)
+    n = 5
+    field = 2e-3 * rng.random((n, n, n)) ** 3
+    ext = np.array([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0])
+    w = (ext[3:] - ext[:3]) / n
+    idx = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    vlo = ext[:3] + w * idx
+    fv = field.reshape(-1)
+
+    def brute(q):
+        ov = np.clip(np.minimum(q[3:], vlo + w) - np.maximum(q[:3], vlo), 0, None)
+        return float(np.sum(fv * np.prod(ov, axis=1)))
+
+    total = brute(ext)
+    for maxmf, maxod, maxlevel in ((0.0, 0.05, 5), (0.02, 0.2, 5), (0.0, 0.0, 3)):
+        cap = 1 << 20
+        flags = np.zeros(cap, np.uint8)
+        nn = ctypes.c_int64()
+        assert capi.lib().sg_tree_subdivide_field_od(8, capi.dptr(ext), capi.dptr(np.ascontiguousarray(field)), n, n,
+                                                     n, 1, maxlevel, maxmf, maxod, flags.ctypes.data_as(capi._U8),
+                                                     cap, ctypes.byref(nn)) == 0
+        boxes, levels = [ext.copy()], [0]
+        nsplit_by_od = nstop = 0
+        for l in range(nn.value):
+            q, lev = boxes[l], levels[l]
+            m = brute(q)
+            vol = np.prod(q[3:] - q[:3])
+            by_mf = maxmf > 0 and m / total >= maxmf
+            by_od = maxod > 0 and 2600.0 * m / vol ** (2.0 / 3.0) >= maxod
+            should = lev <= 1 or (lev < maxlevel and ((maxmf == 0 and maxod == 0) or by_mf or by_od))
+            assert bool(flags[l]) == should, (l, lev, maxmf, maxod)
+            nsplit_by_od += bool(flags[l]) and lev > 1 and by_od and not by_mf
+            nstop += (not flags[l]) and 1 < lev < maxlevel
+            if flags[l]:
+                c = 0.5 * (q[:3] + q[3:])
+                for k in range(8):
+                    b = np.array([c[a] if (k >> a) & 1 else q[a] for a in range(3)] +
+                                 [q[a + 3] if (k >> a) & 1 else c[a] for a in range(3)])
+                    boxes.append(b)
+                    levels.append(lev + 1)
+        if maxod > 0:
+            assert nsplit_by_od > 0 and nstop > 0
+        if maxmf == 0 and maxod == 0:
+            assert nn.value == sum(8 ** k for k in range(maxlevel + 1))
+
+
+def test_tree_density_dispersion_criterion():
+    """The density-dispersion criterion (TreeNodeSampleDensityCalculator::densityDispersion, (max - min) / max
+    of the node's density samples) on an exponential disc: a node split by it alone has a true dispersion of
+    at least the threshold (samples lie inside the node, so their spread cannot exceed the node's; the disc's
+    extremes in a box are at the points nearest to and farthest from the centre in R and |z|), and with many
+    samples most unsplit nodes below maxlevel have a true dispersion under it."""
+    from skirt7_amd import skirt as S
+    hR, hz = 5.0, 1.0
+    disc = S.ExpDiskGeometry(hR, hz)
+    ext = np.array([-10.0, -10.0, -2.0, 10.0, 10.0, 2.0])
+    f = 0.6
+
+    def true_disp(q):
+        x = np.clip(0.0, q[0], q[3]); y = np.clip(0.0, q[1], q[4]); z = np.clip(0.0, q[2], q[5])
+        rmin, zmin = np.hypot(x, y), abs(z)
+        rmax = np.hypot(max(abs(q[0]), abs(q[3])), max(abs(q[1]), abs(q[4])))
+        zmax = max(abs(q[2]), abs(q[5]))
+        hi = np.exp(-rmin / hR - zmin / hz)
+        lo = np.exp(-rmax / hR - zmax / hz)
+        return (hi - lo) / hi
+
+    import ctypes
+    from skirt7_amd import capi
+    cap = 1 << 20
+    buf = np.zeros(cap, np.uint8)
+    nn = ctypes.c_int64()
+    assert capi.lib().sg_tree_subdivide_multi(8, capi.dptr(ext), 1, capi.dptr(capi.as_f64(disc.params())),
+                                              capi.dptr(np.ones(1)), 1.0, 1, 5, 0.0, 0.0, f, 400, 11,
+                                              buf.ctypes.data_as(capi._U8), cap, ctypes.byref(nn)) == 0
+    flags = buf[:nn.value]
+    boxes, levels = [ext.copy()], [0]
+    unsplit_ok = unsplit = 0
+    for l in range(len(flags)):
+        q, lev = boxes[l], levels[l]
+        if 1 < lev < 5:
+            if flags[l]:
+                assert true_disp(q) >= f - 1e-12, (l, lev, true_disp(q))
+            else:
+                unsplit += 1
+                unsplit_ok += true_disp(q) < f + 0.05
+        if flags[l]:
+            c = 0.5 * (q[:3] + q[3:])
+            for k in range(8):
+                boxes.append(np.array([c[a] if (k >> a) & 1 else q[a] for a in range(3)] +
+                                      [q[a + 3] if (k >> a) & 1 else c[a] for a in range(3)]))
+                levels.append(lev + 1)
+    assert int(np.sum(flags)) > 10 and unsplit > 10
+    assert unsplit_ok >= 0.9 * unsplit
