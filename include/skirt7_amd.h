@@ -172,6 +172,31 @@ int sg_sim_run_static(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64
  * through it and the rest share the persistent launches; each member's result is its single run. */
 int sg_sim_run_batch(int nmembers, sg_sim* const* sims, const int* ells, const uint64_t* counts, int precision,
                      uint64_t* counters);
+/* ---- PanMonteCarloSimulation: dust emission (PanMonteCarloSimulation.cpp:106-340) ------------------- */
+/* PanDustSystem::calculatedustemission for one dust component with a grey-body emissivity: per cell the
+ * mean intensity from the absorbed luminosity (DustSystem::meanintensityv, DustSystem.cpp:911-931; rho is the
+ * component's density, kappa*rho per grid length unit, length_unit the grid unit in metres), each population's
+ * equilibrium temperature by inverting its Planck-integrated absorption table (DustMix::equilibrium and
+ * invplanckabs, DustMix.cpp:674-697; Tv has nT+1 temperatures, planckabs npop x (nT+1) values), the emissivity
+ * sum(sigmaabs B(lambda, T)) / mu (GreyBodyDustEmissivity.cpp:20-38) times the bin widths, normalised
+ * (DustLib.cpp): dustlum is ncells x nlambda ([m][ell]). labsbol receives PanDustSystem::Labsbolv (the row sums
+ * of labs_stellar and of labs_dust, which may be null). Tables are ncells x nlambda as sg_dust_read_absorption
+ * returns them; wavelengths and widths in metres. Cartesian and tree grids. */
+int sg_dust_emission_spectra(sg_dust* d, const double* rho, const double* labs_stellar, const double* labs_dust,
+                             int npop, const double* sigmaabs, double mu, const double* kappaabs, const double* lambdas,
+                             const double* dlambdas, int nT, const double* Tv, const double* planckabs,
+                             double length_unit, double* dustlum, double* labsbol);
+/* Dust photon packets [first, first+count) at wavelength ell emitted by the cells with luminosities Lv
+ * (ncells values: Labsbol times the cell's spectrum at ell), each with luminosity sum(Lv)/npp times its bias
+ * weight: PanMonteCarloSimulation::dodustemissionchunk (PanMonteCarloSimulation.cpp:246-340; the cell drawn
+ * uniformly for a fraction `bias` of the packets, PanDustSystem::emissionBias) with emission and scattering
+ * peel-offs, or dodustselfabsorptionchunk (:160-204: bias 0, peeloffs 0, store_absorption 1). The packets'
+ * transport is the stellar packets' (lane pipeline). Their random streams are keyed by (seed, member ^
+ * stream * 0x9E3779B9): give each emission phase or self-absorption cycle its own stream. One dust component,
+ * Cartesian and tree grids. counters as sg_sim_run without the timings. */
+int sg_sim_run_dust(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, const double* Lv, double bias,
+                    int peeloffs, int store_absorption, uint32_t stream, int precision, uint64_t* counters);
+
 /* ---- FitSKIRT evaluation of a population ------------------------------------------------------ */
 /* GaussianKernel (GaussianKernel.cpp:19-46) normalised to unit sum (ConvolutionKernel.cpp:23-28):
  * dimension x dimension doubles, pixel (x, y) at x + dimension*y. */

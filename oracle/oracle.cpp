@@ -201,6 +201,8 @@ struct Grid {
     virtual void path(Path& p, WalkStats& st) const = 0;
     virtual int whichcell(const double r[3]) const = 0;
     virtual int64_t ncells() const = 0;
+    // the box of cell m (xmin ymin zmin xmax ymax zmax); false where cells are not boxes
+    virtual bool cellbox(int64_t, double*) const { return false; }
 };
 
 // ===================================================================== Cartesian grid
@@ -212,6 +214,12 @@ struct CartesianGrid : Grid {
     int Nx, Ny, Nz;
     int index(int i, int j, int k) const { return k + Nz * j + Nz * Ny * i; }
     int64_t ncells() const override { return (int64_t)Nx * Ny * Nz; }
+    bool cellbox(int64_t m, double* q) const override   // CartesianDustGrid::box (CartesianDustGrid.cpp:326-334)
+    {
+        int i = (int)(m / ((int64_t)Nz * Ny)), j = (int)((m / Nz) % Ny), k = (int)(m % Nz);
+        q[0] = xv[i]; q[1] = yv[j]; q[2] = zv[k]; q[3] = xv[i + 1]; q[4] = yv[j + 1]; q[5] = zv[k + 1];
+        return true;
+    }
     int whichcell(const double r[3]) const override
     {
         int i = nr_locate_fail(xv.data(), Nx + 1, r[0]);
@@ -320,6 +328,11 @@ struct TreeGrid : Grid {
     std::vector<int> cellnumber, idv;
 
     int64_t ncells() const override { return (int64_t)idv.size(); }
+    bool cellbox(int64_t m, double* q) const override   // the leaf node's extent (TreeDustGrid.cpp:364-367)
+    {
+        std::memcpy(q, nodes[idv[m]].b, sizeof(double) * 6);
+        return true;
+    }
 
     static bool contains(const double* bb, const double* r)   // Box.hpp:108-109
     {
@@ -1248,6 +1261,9 @@ struct MCParams {
     uint32_t member;
 };
 
+int transport_packet(const Scene& sc, const MCParams& mp, int ell, PP& pp, double Lthreshold, Rng& rng, Path& path,
+                     Path& ppath, Tally& t, Counters& c, bool peeloffs, bool store_abs);
+
 // MonteCarloSimulation.cpp:259-297 body for one packet, with :299-357, :432-543
 int one_packet(const Scene& sc, const MCParams& mp, int ell, double L, double Lthreshold, Rng& rng,
                const std::vector<double>& Ltot, Path& path, Path& ppath, Tally& t, Counters& c)
@@ -1256,8 +1272,17 @@ int one_packet(const Scene& sc, const MCParams& mp, int ell, double L, double Lt
     launch(sc, pp, ell, L, rng, mp.emissionBias, Ltot);
     c.pk++;
     if (!(pp.L > 0)) return 0;
-    // peeloffemission :299-309 (isotropic geometry: probabilityForDirection == 1)
-    for (size_t i = 0; i < sc.instr.size(); i++) detect(sc, i, pp.r, pp.L, ell, ppath, t, c);
+    return transport_packet(sc, mp, ell, pp, Lthreshold, rng, path, ppath, t, c, true, sc.store_abs);
+}
+
+// A launched packet: its emission peel-offs (when peeloffs), then the cycle of escape/absorption
+// (absorption tallied when store_abs), propagation, scattering peel-offs (when peeloffs) and scattering
+// (:259-297, :299-357, :432-543; PanMonteCarloSimulation.cpp:170-187 without any peel-offs).
+int transport_packet(const Scene& sc, const MCParams& mp, int ell, PP& pp, double Lthreshold, Rng& rng, Path& path,
+                     Path& ppath, Tally& t, Counters& c, bool peeloffs, bool store_abs)
+{
+    // peeloffemission :299-309 (isotropic emission: probabilityForDirection == 1)
+    for (size_t i = 0; peeloffs && i < sc.instr.size(); i++) detect(sc, i, pp.r, pp.L, ell, ppath, t, c);
     if (!sc.has_dust) return 0;
     const double albedo = sc.alb[ell], g = sc.g[ell];
     const int H = sc.ncomp;
@@ -1294,14 +1319,14 @@ int one_packet(const Scene& sc, const MCParams& mp, int ell, double L, double Lt
                     double expfactorm = -std::expm1(-path.v[n].dtau);
                     double Lintm = L0 * std::exp(-taustart) * expfactorm;
                     Lsca += alb * Lintm;
-                    if (sc.store_abs && t.labs) atomic_add(&t.labs[(size_t)m * sc.nlambda + ell], (1.0 - alb) * Lintm);
+                    if (store_abs && t.labs) atomic_add(&t.labs[(size_t)m * sc.nlambda + ell], (1.0 - alb) * Lintm);
                 }
             }
             pp.L = Lsca;
         }
         // simulateescapeandabsorption :432-461 (single component)
         double expfactor = -std::expm1(-taupath);
-        if (H == 1 && sc.store_abs && t.labs) {
+        if (H == 1 && store_abs && t.labs) {
             int N = (int)path.v.size();
             for (int n = 0; n < N; n++) {
                 int m = path.v[n].m;
@@ -1333,7 +1358,7 @@ int one_packet(const Scene& sc, const MCParams& mp, int ell, double L, double Lt
         // kappasca(h) rho(m, h) at the scattering point's cell, and the peel-off is aborted outside the grid or
         // where there is no scattering dust
         int mscat = H > 1 ? sc.grid->whichcell(pp.r) : -1;
-        for (size_t i = 0; i < sc.instr.size(); i++) {
+        for (size_t i = 0; peeloffs && i < sc.instr.size(); i++) {
             double I;
             if (!scatter_phase(sc, ell, mscat, pp.k, sc.instr[i].kobs, I)) break;
             detect(sc, i, pp.r, pp.L * I, ell, ppath, t, c);
@@ -1845,6 +1870,147 @@ void or_golden_section(const double* ref, double* frame, int npix, double minlum
     *chi2 = chi;
     *lum = lumvalue;
     std::copy(sim.begin(), sim.end(), frame);
+}
+
+} // extern "C"
+
+// ---------------- panchromatic dust emission (PanMonteCarloSimulation.cpp:106-340, PanDustSystem.cpp)
+
+namespace {
+// PlanckFunction::operator() (PlanckFunction.cpp:21-30), constants of Units.cpp:17-19
+double planck_b(double lambda, double T)
+{
+    const double h = 6.62606957e-34, c = 2.99792458e8, k = 1.3806488e-23;
+    double x = h * c / (lambda * k * T);
+    return 2.0 * h * c * c / std::pow(lambda, 5) / (std::exp(x) - 1.0);
+}
+}
+
+extern "C" {
+
+// The dust's normalised emission spectrum per cell and its bolometric absorbed luminosity, for one
+// component with a grey-body emissivity: DustSystem::meanintensityv (DustSystem.cpp:911-931) with the grid's
+// kappa*rho per length unit (so kappa rho V in SI is kappa rho V len2), DustMix::equilibrium and invplanckabs
+// (DustMix.cpp:674-697), GreyBodyDustEmissivity::emissivity (GreyBodyDustEmissivity.cpp:20-38), DustLib's
+// conversion to luminosities per bin and normalisation (DustLib.cpp), PanDustSystem::Labsbolv.
+int or_dust_spectra(void* s, const double* labs_s, const double* labs_d, int npop, const double* sigmaabs, double mu,
+                    const double* kappaabs, const double* lambda, const double* dlambda, int nT, const double* Tv,
+                    const double* planck, double len2, double* dustlum, double* labsbol)
+{
+    const Scene& sc = *(Scene*)s;
+    const int nl = sc.nlambda;
+    const int64_t nc = sc.grid->ncells();
+    for (int64_t m = 0; m < nc; m++) {
+        double q[6];
+        if (!sc.grid->cellbox(m, q)) { g_err = "dust emission needs cells that are boxes"; return -1; }
+        const double vol = (q[3] - q[0]) * (q[4] - q[1]) * (q[5] - q[2]);
+        const double fac = 4.0 * M_PI * vol * len2;
+        const double* ls = labs_s + (size_t)m * nl;
+        const double* ld = labs_d ? labs_d + (size_t)m * nl : nullptr;
+        std::vector<double> T(npop);
+        for (int c = 0; c < npop; c++) {
+            double planckabs = 0.0;
+            for (int ell = 0; ell < nl; ell++) {
+                double labs = ls[ell] + (ld ? ld[ell] : 0.0);
+                double kappaabsrho = 0.0 + kappaabs[ell] * sc.rho[m];
+                double J = labs / (kappaabsrho * fac) / dlambda[ell];
+                J = std::isfinite(J) ? J : 0.0;
+                planckabs += sigmaabs[(size_t)c * nl + ell] * J * dlambda[ell];
+            }
+            const double* P = planck + (size_t)c * (nT + 1);
+            int p = nr_locate_clip(P, nT + 1, planckabs);
+            T[c] = Tv[p] + ((planckabs - P[p]) / (P[p + 1] - P[p])) * (Tv[p + 1] - Tv[p]);
+        }
+        double* out = dustlum + (size_t)m * nl;
+        double total = 0.0;
+        for (int ell = 0; ell < nl; ell++) {
+            double ev = 0.0;
+            for (int c = 0; c < npop; c++) ev += sigmaabs[(size_t)c * nl + ell] * planck_b(lambda[ell], T[c]);
+            ev /= mu;
+            out[ell] = ev * dlambda[ell];
+            total += out[ell];
+        }
+        if (total > 0)
+            for (int ell = 0; ell < nl; ell++) out[ell] /= total;
+        double bs = 0.0, bd = 0.0;
+        for (int ell = 0; ell < nl; ell++) bs += ls[ell];
+        if (ld)
+            for (int ell = 0; ell < nl; ell++) bd += ld[ell];
+        labsbol[m] = bs + bd;
+    }
+    return 0;
+}
+
+// Dust photon packets [first, first+count) at wavelength ell from per-cell luminosities Lv
+// (PanMonteCarloSimulation::dodustemissionchunk :273-340 with emission bias xi, or dodustselfabsorptionchunk
+// :170-187 with xi = 0 and no peel-offs), on the Philox packet streams keyed (seed, member ^ stream*0x9E3779B9).
+int or_run_dust_packets(void* s, const double* mc8, int ell, uint64_t first, uint64_t count, uint64_t npp,
+                        const double* Lv, double xi, int peeloffs, int store_abs, uint32_t stream, int threads,
+                        double* frames, double* labs, uint64_t* counters)
+{
+    Scene* sc = (Scene*)s;
+    MCParams mp;
+    parse_mc(mc8, mp);
+    const int64_t nc = sc->grid->ncells();
+    double Ltot = 0;
+    for (int64_t m = 0; m < nc; m++) Ltot += Lv[m];
+    for (int i = 0; i < 8; i++) counters[i] = 0;
+    if (!(Ltot > 0)) return 0;
+    // NR::cdf
+    std::vector<double> cum(nc + 1, 0.0);
+    for (int64_t m = 0; m < nc; m++) cum[m + 1] = cum[m] + Lv[m];
+    const double norm = cum[nc];
+    for (int64_t m = 0; m <= nc; m++) cum[m] /= norm;
+    const double Lmean = Ltot / nc, Lem = Ltot / (double)npp, Lthreshold = Lem / mp.minWeightReduction;
+    Tally t = make_tally(*sc, frames, labs);
+    int T = std::max(1, threads);
+    std::atomic<uint64_t> next(0);
+    std::atomic<int> failed(0);
+    std::vector<Counters> cs(T);
+    auto worker = [&](int tid) {
+        Rng rng;
+        rng.use_mt = false;
+        rng.key[0] = mp.seed;
+        rng.key[1] = mp.member ^ (stream * 0x9E3779B9u);
+        rng.ell = (uint32_t)ell;
+        Path path, ppath;
+        while (!failed) {
+            uint64_t i0 = next.fetch_add(256);
+            if (i0 >= count) break;
+            uint64_t i1 = std::min(count, i0 + 256);
+            for (uint64_t i = i0; i < i1; i++) {
+                rng.start_packet(first + i);
+                double X = rng.uniform();
+                int64_t m;
+                if (X < xi) m = std::max<int64_t>(0, std::min<int64_t>(nc - 1, (int64_t)(nc * X / xi)));
+                else m = nr_locate_clip(cum.data(), (int)(nc + 1), (X - xi) / (1 - xi));
+                double weight = 1.0 / (1 - xi + xi * Lmean / Lv[m]);
+                double q[6];
+                sc->grid->cellbox(m, q);
+                PP pp;
+                pp.ell = ell;
+                pp.nscatt = 0;
+                double fx = rng.uniform();
+                double fy = rng.uniform();
+                double fz = rng.uniform();
+                pp.r[0] = q[0] + fx * (q[3] - q[0]);
+                pp.r[1] = q[1] + fy * (q[4] - q[1]);
+                pp.r[2] = q[2] + fz * (q[5] - q[2]);
+                random_direction(rng, pp.k);
+                pp.L = Lem * weight;
+                cs[tid].pk++;
+                if (!(pp.L > 0)) continue;
+                if (transport_packet(*sc, mp, ell, pp, Lthreshold, rng, path, ppath, t, cs[tid], peeloffs != 0,
+                                     store_abs != 0) != 0) { failed = 1; break; }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int i = 1; i < T; i++) th.emplace_back(worker, i);
+    worker(0);
+    for (auto& x : th) x.join();
+    for (auto& c : cs) fold(counters, c, npp);
+    return failed ? -1 : 0;
 }
 
 } // extern "C"

@@ -94,6 +94,13 @@ void   or_convolve_nested(const double* image, int Nx, int Ny, const double* ker
 void   or_golden_section(const double* ref, double* frame, int npix, double minlum, double maxlum, double* lum,
                          double* chi2);
 
+int or_dust_spectra(void* s, const double* labs_s, const double* labs_d, int npop, const double* sigmaabs, double mu,
+                    const double* kappaabs, const double* lambda, const double* dlambda, int nT, const double* Tv,
+                    const double* planck, double len2, double* dustlum, double* labsbol);
+int or_run_dust_packets(void* s, const double* mc8, int ell, uint64_t first, uint64_t count, uint64_t npp,
+                        const double* Lv, double xi, int peeloffs, int store_abs, uint32_t stream, int threads,
+                        double* frames, double* labs, uint64_t* counters);
+
 #ifdef __cplusplus
 }
 #endif

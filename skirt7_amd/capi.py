@@ -94,6 +94,11 @@ _SIGS = {
                                          ctypes.c_int, _U64]),
     "sg_sim_run_batch": (ctypes.c_int, [ctypes.c_int, _VPP, _I32, _U64, ctypes.c_int, _U64]),
     "sg_gaussian_kernel": (ctypes.c_int, [ctypes.c_double, ctypes.c_int, _D]),
+    "sg_dust_emission_spectra": (ctypes.c_int, [_VP, _D, _D, _D, ctypes.c_int, _D, ctypes.c_double, _D, _D, _D,
+                                                ctypes.c_int, _D, _D, ctypes.c_double, _D, _D]),
+    "sg_sim_run_dust": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _D,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
+                                       _U64]),
     "sg_fit_frames": (ctypes.c_int, [ctypes.c_int, _VPP, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int,
                                      _D, ctypes.c_double, ctypes.c_double, _D, _D, _D]),
     "sg_geometry_density": (ctypes.c_int, [_D, ctypes.c_int64, _D, _D]),

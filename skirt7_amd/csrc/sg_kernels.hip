@@ -15,6 +15,7 @@
 #include "sg_host.hpp"
 #include "sg_mc.hpp"
 #include "sg_split.hpp"
+#include "sg_pan.hpp"
 
 namespace sg {
 
@@ -1630,6 +1631,145 @@ int sg_sim_run(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp,
     }
     SG_REQUIRE(!(err & 2), "walk order: a request's sorted place lies beyond the pass's requests");
     SG_REQUIRE(!(err & 4), "walk queue: a request names no instrument pixel or packet slot");
+    SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
+    SG_CATCH
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- panchromatic dust emission
+
+static CellBoxes cell_boxes(const sg_grid* g)
+{
+    CellBoxes cb{};
+    cb.kind = g->kind;
+    if (g->kind == GK_CART) {
+        cb.xv = g->cart.xv; cb.yv = g->cart.yv; cb.zv = g->cart.zv;
+        cb.Nx = g->cart.Nx; cb.Ny = g->cart.Ny; cb.Nz = g->cart.Nz;
+    } else {
+        cb.rec = g->tree.rec64;
+    }
+    return cb;
+}
+
+extern "C" {
+
+int sg_dust_emission_spectra(sg_dust* d, const double* rho, const double* labs_stellar, const double* labs_dust,
+                             int npop, const double* sigmaabs, double mu, const double* kappaabs, const double* lambdas,
+                             const double* dlambdas, int nT, const double* Tv, const double* planckabs,
+                             double length_unit, double* dustlum, double* labsbol)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(d && rho && labs_stellar && sigmaabs && kappaabs && lambdas && dlambdas && Tv && planckabs && dustlum &&
+               labsbol, "null argument");
+    SG_REQUIRE(d->ncomp == 1, "dust emission supports one dust component");
+    SG_REQUIRE(d->grid->kind == GK_CART || d->grid->kind == GK_TREE,
+               "dust emission needs cells that are boxes (Cartesian or tree grids)");
+    SG_REQUIRE(npop >= 1 && npop <= 8 && nT >= 2 && mu > 0, "between 1 and 8 dust populations, a Planck table and a "
+                                                            "positive mass per hydrogen atom are needed");
+    const int64_t nc = d->grid->ncells;
+    const int nl = d->nlambda;
+    DevBuf b_rho, b_ls, b_ld, b_sig, b_ka, b_lam, b_dl, b_T, b_P, b_out, b_bol;
+    b_rho.upload(rho, nc);
+    b_ls.upload(labs_stellar, (size_t)nc * nl);
+    if (labs_dust) b_ld.upload(labs_dust, (size_t)nc * nl);
+    b_sig.upload(sigmaabs, (size_t)npop * nl);
+    b_ka.upload(kappaabs, nl);
+    b_lam.upload(lambdas, nl);
+    b_dl.upload(dlambdas, nl);
+    b_T.upload(Tv, nT + 1);
+    b_P.upload(planckabs, (size_t)npop * (nT + 1));
+    b_out.alloc((size_t)nc * nl * sizeof(double));
+    b_bol.alloc((size_t)nc * sizeof(double));
+    SpectraArgs a{};
+    a.nl = nl; a.npop = npop; a.nT = nT;
+    a.lambda = b_lam.as<double>(); a.dlambda = b_dl.as<double>();
+    a.sigmaabs = b_sig.as<double>(); a.kappaabs = b_ka.as<double>(); a.mu = mu;
+    a.Tv = b_T.as<double>(); a.planck = b_P.as<double>();
+    a.labs_s = b_ls.as<double>(); a.labs_d = labs_dust ? b_ld.as<double>() : nullptr;
+    a.rho = b_rho.as<double>(); a.len2 = length_unit * length_unit; a.ncells = nc;
+    a.dustlum = b_out.as<double>(); a.labsbol = b_bol.as<double>();
+    hipLaunchKernelGGL(k_dust_spectra, dim3(blocks_for(nc, 256)), dim3(256), 0, 0, a, cell_boxes(d->grid));
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipMemcpy(dustlum, b_out.p, (size_t)nc * nl * sizeof(double), hipMemcpyDeviceToHost));
+    SG_HIP(hipMemcpy(labsbol, b_bol.p, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost));
+    SG_CATCH
+}
+
+int sg_sim_run_dust(sg_sim* s, int ell, uint64_t first, uint64_t count, uint64_t npp, const double* Lv, double bias,
+                    int peeloffs, int store_absorption, uint32_t stream, int precision, uint64_t* counters)
+{
+    SG_TRY
+    ensure_init();
+    SG_REQUIRE(s && s->dust && Lv, "dust emission needs a simulation with a dust system and the cells' luminosities");
+    SG_REQUIRE(npp > 0, "npp must be positive");
+    SG_REQUIRE(bias >= 0 && bias < 1, "the dust emission bias must be in [0, 1)");
+    SG_REQUIRE(s->dust->ncomp == 1, "dust emission supports one dust component");
+    const sg_grid* g = s->dust->grid;
+    SG_REQUIRE(g->kind == GK_CART || g->kind == GK_TREE,
+               "dust emission needs cells that are boxes (Cartesian or tree grids)");
+    const int64_t nc = g->ncells;
+    if (counters)
+        for (int i = 0; i < SG_NCOUNTERS; i++) counters[i] = 0;
+    // NR::cdf of the cells' luminosities, in the reference's order (PanMonteCarloSimulation.cpp:249-255,281-282)
+    std::vector<double> cum(nc + 1, 0.0);
+    for (int64_t m = 0; m < nc; m++) {
+        SG_REQUIRE(Lv[m] >= 0 && std::isfinite(Lv[m]), "cell luminosities must be finite and non-negative");
+        cum[m + 1] = cum[m] + Lv[m];
+    }
+    double Ltot = 0;
+    for (int64_t m = 0; m < nc; m++) Ltot += Lv[m];
+    if (!(Ltot > 0) || count == 0) return 0;
+    const double norm = cum[nc];
+    for (int64_t m = 0; m <= nc; m++) cum[m] /= norm;
+    McArgs a;
+    fill_args(s, ell, first, count, npp, a);
+    a.member = a.member ^ (stream * 0x9E3779B9u);
+    a.Lpkt = Ltot / (double)npp;
+    a.Lthreshold = a.Lpkt / s->mc[1];
+    a.store_abs = store_absorption && s->dust->store_abs;
+    if (!a.store_abs) a.labs = nullptr;
+    if (!peeloffs) a.ni = 0;
+    if (g_tune_compact <= 0) a.compact = 32;
+    DevBuf b_cum, b_lv;
+    b_cum.upload(cum);
+    b_lv.upload(Lv, nc);
+    DustLaunch dl{};
+    dl.cum = b_cum.as<double>(); dl.Lv = b_lv.as<double>(); dl.ncells = nc;
+    dl.xi = bias; dl.Lmean = Ltot / nc; dl.Lem = Ltot / (double)npp;
+    dl.cb = cell_boxes(g);
+    s->counters.zero();
+    s->err.zero();
+    const bool mixed = precision == SG_MIXED;
+    const size_t rsz = mixed ? sizeof(float) : sizeof(double);
+    const uint64_t batch = std::min<uint64_t>(count, g_emit_batch ? g_emit_batch : (uint64_t)1 << 22);
+    if (s->emitted.bytes < 7 * batch * rsz) s->emitted.alloc(7 * batch * rsz);
+    if (s->emitdraw.bytes < batch * sizeof(uint32_t)) s->emitdraw.alloc(batch * sizeof(uint32_t));
+    int blocks = 0;
+    for (uint64_t b = 0; b < count; b += batch) {
+        McArgs ab = a;
+        ab.first = first + b;
+        ab.count = std::min(batch, count - b);
+        ab.emitted = s->emitted.p;
+        ab.emit_draw = s->emitdraw.as<uint32_t>();
+        s->next.zero();
+        int eb = (int)std::min<uint64_t>((ab.count + 255) / 256, (uint64_t)g_cus * 16);
+        if (mixed) hipLaunchKernelGGL(k_emit_dust<float>, dim3(eb), dim3(256), 0, 0, ab, dl, s->emitted.as<float>(), s->emitdraw.as<uint32_t>());
+        else hipLaunchKernelGGL(k_emit_dust<double>, dim3(eb), dim3(256), 0, 0, ab, dl, s->emitted.as<double>(), s->emitdraw.as<uint32_t>());
+        SG_HIP(hipGetLastError());
+        dispatch_mc_any(s, ab, precision, blocks);
+    }
+    SG_HIP(hipDeviceSynchronize());
+    unsigned long long c[SG_NCOUNTERS];
+    SG_HIP(hipMemcpy(c, s->counters.p, sizeof c, hipMemcpyDeviceToHost));
+    int err = 0;
+    SG_HIP(hipMemcpy(&err, s->err.p, sizeof err, hipMemcpyDeviceToHost));
+    if (counters) {
+        for (int i = 0; i < SG_NCOUNTERS; i++) counters[i] = c[i];
+        counters[6] = npp;
+        counters[11] = blocks;
+    }
     SG_REQUIRE(err == 0, "The optical depth along the path is not a positive number");
     SG_CATCH
 }

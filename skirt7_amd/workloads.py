@@ -340,3 +340,36 @@ def two_dust_disc(kind="cartesian", grid_n=32, maxlevel=5, nsites=20000, pixels=
     d = dict(name="two_dust_disc-" + kind, kind=kind, grid=grid, rho=rhos, mix=mixes, comps=comps, dgeom=lane,
              instr=_frames((0.0, 60.0, 88.0), pixels, nlambda=2), store_abs=store_abs)
     return _finish(d, packages, seed)
+
+
+def pan_disc(kind="cartesian", nlambda=30, grid_n=24, maxlevel=5, pixels=48, packages=2e4, seed=4357, tau=1.0,
+             nsamples=20, selfAbsorption=False, cycles=0, precision=None):
+    """A panchromatic disc galaxy (PanMonteCarloSimulation): a stellar exponential disc (hR 4, hz 0.4) with a
+    6000 K blackbody spectrum of 1e10 solar luminosities, a dust disc (hR 6, hz 0.3) of Benchmark1DDustMix
+    dust with face-on optical depth `tau` at 0.55 micron, nlambda wavelengths log-spaced over 0.1-1000 micron,
+    frames at 0 and 88 deg (nlambda planes each); grid: cartesian (grid_n^3) or octree (to maxlevel). Lengths
+    in kpc, luminosities in W."""
+    from . import panchromatic as P
+    from .capi import SG_MIXED
+    from .output import PC
+    lam, dl = P.log_wavelength_grid(1e-7, 1e-3, nlambda)
+    mix = P.Benchmark1DDustMix(lam, dl)
+    sed = P.planck(lam, 6000.0) * dl
+    lum = 1e10 * 3.846e26 * sed / sed.sum()
+    comps = [S.OligoStellarComp(S.ExpDiskGeometry(4.0, 0.4), lum)]
+    dgeom = S.ExpDiskGeometry(6.0, 0.3)
+    ellv = int(np.argmin(np.abs(np.log(lam / 0.55e-6))))
+    scale = S.faceon_normalization(dgeom, tau, mix.kappaext[ellv])
+    if kind == "cartesian":
+        grid = S.CartesianDustGrid.uniform(EXTENT, grid_n, grid_n, grid_n)
+    else:
+        grid = S.OctTreeDustGrid.subdivide(EXTENT, dgeom, scale, total_mass=scale, minlevel=2, maxlevel=maxlevel,
+                                           maxmassfraction=1e-4, nsamples=nsamples, seed=seed)
+    rho = S.sample_density(grid, dgeom, scale, nsamples, seed)
+    dust = P.PanDustSystem(grid, rho, mix, selfAbsorption=selfAbsorption, cycles=cycles, length_unit=1000.0 * PC)
+    instr = _frames((0.0, 88.0), pixels, nlambda=nlambda)
+    stars = S.StellarSystem(comps, emissionBias=0.5)
+    pan = P.PanMonteCarloSimulation(stars, dust, instr, packages=packages, seed=seed,
+                                    precision=SG_MIXED if precision is None else precision)
+    return dict(name="pan_disc-" + kind, kind=kind, grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
+                instr=instr, stars=stars, dust=dust.ds, pan=pan, sim=pan.sim, packages=packages, lambdas=lam)

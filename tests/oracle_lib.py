@@ -34,6 +34,11 @@ _SIGS = {
     "or_scene_path": (ctypes.c_int, [_V, _D, ctypes.c_int, _I32, _D]),
     "or_scene_whichcell": (ctypes.c_int, [_V, _D]),
     "or_scatter_phase": (ctypes.c_double, [_V, ctypes.c_int, _D, _D, _D]),
+    "or_dust_spectra": (ctypes.c_int, [_V, _D, _D, ctypes.c_int, _D, ctypes.c_double, _D, _D, _D, ctypes.c_int, _D,
+                                       _D, ctypes.c_double, _D, _D]),
+    "or_run_dust_packets": (ctypes.c_int, [_V, _D, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                           _D, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                           ctypes.c_int, _D, _D, _U64]),
     "or_random_mix": (ctypes.c_int, [_V, ctypes.c_int, _D, ctypes.c_double]),
     "or_scene_opticaldepth": (ctypes.c_double, [_V, _D, ctypes.c_int]),
     "or_scene_pathlength": (None, [_V, _D, ctypes.c_int, ctypes.c_int, _D, _D]),
@@ -213,6 +218,29 @@ class Scene:
         mc = f64(w["sim"].mc)
         rc = self.L.or_run_packets(self.h, dp(mc), ell, first, count, npp, threads, dp(frames), dp(labs),
                                   c.ctypes.data_as(_U64))
+        assert rc == 0, self.L.or_last_error()
+        self.labs = labs
+        return frames, c
+
+    def dust_spectra(self, mix, labs_s, labs_d=None, len2=None):
+        """or_dust_spectra for a panchromatic mix (skirt7_amd.panchromatic.PanDustMix): (dustlum, labsbol)."""
+        from skirt7_amd.output import PC
+        nc, nl = self.ncells, self.nlambda
+        out, bol = np.zeros((nc, nl)), np.zeros(nc)
+        ls = f64(labs_s)
+        ld = f64(labs_d) if labs_d is not None else None
+        rc = self.L.or_dust_spectra(self.h, dp(ls), dp(ld) if ld is not None else None, mix.Npop(),
+                                    dp(f64(mix.sigmaabsvv)), mix.mu, dp(f64(mix.kappaabs)), dp(f64(mix.lambdas)),
+                                    dp(f64(mix.dlambdas)), mix.NT, dp(f64(mix.Tv)), dp(f64(mix.planckabs)),
+                                    PC * PC if len2 is None else len2, dp(out), dp(bol))
+        assert rc == 0, self.L.or_last_error()
+        return out, bol
+
+    def run_dust_packets(self, mc, ell, first, count, npp, Lv, xi, peeloffs, store_abs, stream, threads=16):
+        frames, labs = self._buffers()
+        c = np.zeros(8, np.uint64)
+        rc = self.L.or_run_dust_packets(self.h, dp(f64(mc)), ell, first, count, npp, dp(f64(Lv)), xi, int(peeloffs),
+                                        int(store_abs), stream, threads, dp(frames), dp(labs), c.ctypes.data_as(_U64))
         assert rc == 0, self.L.or_last_error()
         self.labs = labs
         return frames, c
