@@ -347,12 +347,13 @@ def main():
             desc = "Voronoi grid of 1e6 sites, all 50 wavelengths per step (1e6 x 50 absorption table stored)"
     else:
         w = workloads.lognormal_kdtree(pixels=min(a.pixels * 2, 1024), packages=total)
-        desc = "kd-tree (alternating binary tree, TopDown) over a log-normal field"
+        desc = ("kd-tree (alternating binary tree, TopDown) over 1e7 SPH particles sampled from a log-normal field, "
+                "exact mass-in-box densities")
     ells = ells or [ell]
     ncell = w["grid"].numCells()
     gkind = "voronoi" if w["kind"] == "voronoi" else "cartesian" if w["kind"] == "cartesian" else "tree"
-    log("setup: %s with %d cells built in %.1f s on %s" % (a.workload, ncell, time.perf_counter() - t0,
-                                                            S.device_name()))
+    setup_s = time.perf_counter() - t0
+    log("setup: %s with %d cells built in %.1f s on %s" % (a.workload, ncell, setup_s, S.device_name()))
     sim = w["sim"]
     npp = total
     sample = int(a.autotune_sample or per)
@@ -393,7 +394,10 @@ def main():
                           desc, ncell, per * len(ells), len(w["instr"].instruments), w["instr"].instruments[0].Nx),
                       "packets_per_step": per * len(ells), "wavelengths_per_step": len(ells), "cells": ncell,
                       "instruments": len(w["instr"].instruments),
-                      "pixels": w["instr"].instruments[0].Nx, "tuning": tuning}}
+                      "pixels": w["instr"].instruments[0].Nx, "tuning": tuning},
+           # whole setup before the timed region: synthetic input generation, the grid built (tree subdivision
+           # on the device for the octree and kd-tree workloads), cell densities, scene upload
+           "setup_s": setup_s}
     def algorithmic_bytes(c):
         """Algorithmic bytes of a run's walks: crossings x bytes per crossing, peel-off tallies, absorption."""
         b = (c["own_crossings"] + c["peel_crossings"] + c["rewalk_crossings"]) * BYTES_PER_CROSSING[gkind]["mixed"]

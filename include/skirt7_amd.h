@@ -31,6 +31,7 @@ typedef struct sg_dust sg_dust;            /* DustSystem (one dust component) */
 typedef struct sg_stars sg_stars;          /* StellarSystem                   */
 typedef struct sg_instruments sg_instruments;  /* InstrumentSystem of FrameInstruments */
 typedef struct sg_sim sg_sim;              /* MonteCarloSimulation            */
+typedef struct sg_sph sg_sph;              /* SPHDustDistribution's particles */
 
 enum { SG_FP64 = 0, SG_MIXED = 1 };
 enum { SG_TOPDOWN = 0, SG_NEIGHBOR = 1, SG_BOOKKEEPING = 2 };   /* TreeDustGrid::SearchMethod (TreeDustGrid.hpp:149) */
@@ -334,6 +335,54 @@ int sg_grid_sample_density(sg_grid* g, const double* geom16, double scale, int n
  * of the device's exp/log/pow). Exponential-disc geometries (spiral-decorated or not); queued on the
  * default stream without waiting: a FitSKIRT member's new dust density without a host round trip. */
 int sg_grid_resample_density(sg_grid* g, const double* geom16, double scale, int nsamples, uint32_t seed);
+/* ---- device grid builders and the SPH particle medium ---------------------------------------- */
+/* SPHDustDistribution::setupSelfBefore (SPHDustDistribution.cpp:38-100): n rows of x y z h M Z T (lengths
+ * in the grid's unit, masses in the caller's mass unit: the reference's pc and Msun factors are the
+ * caller's); a particle with T > Tmax (both positive) is ignored; a negative total gas or metal mass
+ * suppresses all dust; dust fraction fdust > 0. Builds the 20^3 particle grid and each cell's list
+ * (SPHGasParticleGrid.cpp:94-132) on the host and holds the particles on the device. */
+int sg_sph_create(const double* rows, int64_t n, double fdust, double Tmax, sg_sph** out);
+int sg_sph_destroy(sg_sph* s);
+/* Particles kept, SPHDustDistribution::mass (SPHDustDistribution.cpp:248-256), the grid's 3 x 21 borders
+ * (infinite at both ends; any pointer may be null) and the total number of list entries. */
+int sg_sph_info(const sg_sph* s, int64_t* count, double* mass, double* borders, int64_t* list_entries);
+/* Particle grid cell (i, j, k)'s list (particle indices in the order kept): its length in *count and up
+ * to cap entries in out. */
+int sg_sph_cell_list(const sg_sph* s, int i, int j, int k, int32_t* out, int64_t cap, int64_t* count);
+/* SPHDustDistribution::massInBox (SPHDustDistribution.cpp:226-236) of n boxes (xmin ymin zmin xmax ymax
+ * zmax), on the device. */
+int sg_sph_mass_in_box(const sg_sph* s, const double* boxes, int64_t n, double* out);
+/* SPHDustDistribution::density (SPHDustDistribution.cpp:192-202) at n positions, on the device. */
+int sg_sph_density(const sg_sph* s, const double* pos, int64_t n, double* out);
+/* TreeDustGrid::setupSelfBefore / subdivide (TreeDustGrid.cpp:49-230) over the particle set, built on the
+ * device level by level: node masses from the mass in the node's box (TreeNodeBoxDensityCalculator), or,
+ * when maxdensdisp > 0 (TreeDustGrid.cpp:70), from nsamples density samples per node at the counter-based
+ * positions of sg_tree_subdivide_multi; the three stopping criteria as there. Flags as
+ * sg_tree_subdivide; masses (may be null, else cap entries) receives each node's mass: with masses in box,
+ * every node beyond minlevel (those at maxlevel too, for the cells' densities), the reference's sum term
+ * for term; with samples, the nodes the criteria evaluated. -1 elsewhere. */
+int sg_tree_subdivide_sph(int arity, const double ext[6], const sg_sph* s, int minlevel, int maxlevel,
+                          double maxmassfraction, double maxopticaldepth, double maxdensdisp, int nsamples,
+                          uint32_t seed, uint8_t* flags, double* masses, int64_t cap, int64_t* nnodes);
+/* sg_tree_subdivide_multi on the device (exponential-disc geometries): the same sample positions, the
+ * densities to the rounding of the device's exp/log/pow, the same flags. */
+int sg_tree_subdivide_device(int arity, const double ext[6], int ncomp, const double* geoms16, const double* scales,
+                             double total_mass, int minlevel, int maxlevel, double maxmassfraction,
+                             double maxopticaldepth, double maxdensdisp, int nsamples, uint32_t seed, uint8_t* flags,
+                             int64_t cap, int64_t* nnodes);
+/* TreeDustGrid::density (TreeDustGrid.cpp:655-660) over the particle set: each leaf's mass in its box over
+ * its volume, for every cell of a tree grid. node_masses (nnodes entries, may be null) are the masses that
+ * sg_tree_subdivide_sph returned for this tree; without them the masses are evaluated on the device from
+ * the particle grid's lists. */
+int sg_grid_sph_density(sg_grid* g, const sg_sph* s, const double* node_masses, int64_t nnodes, double* rho_out);
+/* sg_grid_sample_density on the device (exponential-disc geometries): the same positions, the densities to
+ * the rounding of the device's exp/log/pow. Cartesian and tree grids. */
+int sg_grid_sample_density_device(sg_grid* g, const double* geom16, double scale, int nsamples, uint32_t seed,
+                                  double* rho_out);
+/* The distance from each of n points to its k-th nearest other point (k <= 64), on the device: the
+ * smoothing lengths of a particle set generated for a workload (no counterpart in the reference, which
+ * reads them from the particle file). */
+int sg_knn_distance(const double* pos, int64_t n, int k, double* out);
 /* Voronoi neighbour lists of n sites in the box ext (VoronoiMesh::buildMesh, VoronoiMesh.cpp:297-405):
  * offsets[n+1] and at most cap ids (walls -1..-6); *nids receives the total; -5 if cap is too small. */
 int sg_voronoi_neighbors(const double ext[6], const double* sites, int64_t n, int64_t* offsets, int32_t* ids,

@@ -8,7 +8,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", f) for f in ("sg_kernels.hip", "sg_fit.hip", "sg_build.cpp", "sg_voronoi.cpp", "sg_io.cpp", "sg_ga.cpp")]
+SRC = [os.path.join(HERE, "csrc", f) for f in ("sg_kernels.hip", "sg_fit.hip", "sg_treebuild.hip", "sg_build.cpp", "sg_voronoi.cpp", "sg_io.cpp", "sg_ga.cpp")]
 OUT = os.path.join(HERE, "lib", "libskirt7_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",

@@ -129,6 +129,21 @@ _SIGS = {
                                                _I64]),
     "sg_grid_field_density": (ctypes.c_int, [_VP, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
     "sg_voronoi_neighbors": (ctypes.c_int, [_D, _D, ctypes.c_int64, _I64, _I32, ctypes.c_int64, _I64]),
+    "sg_sph_create": (ctypes.c_int, [_D, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _VPP]),
+    "sg_sph_destroy": (ctypes.c_int, [_VP]),
+    "sg_sph_info": (ctypes.c_int, [_VP, _I64, _D, _D, _I64]),
+    "sg_sph_cell_list": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I32, ctypes.c_int64, _I64]),
+    "sg_sph_mass_in_box": (ctypes.c_int, [_VP, _D, ctypes.c_int64, _D]),
+    "sg_sph_density": (ctypes.c_int, [_VP, _D, ctypes.c_int64, _D]),
+    "sg_tree_subdivide_sph": (ctypes.c_int, [ctypes.c_int, _D, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _U8, _D,
+                                             ctypes.c_int64, _I64]),
+    "sg_tree_subdivide_device": (ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_int, ctypes.c_uint32, _U8, ctypes.c_int64, _I64]),
+    "sg_grid_sph_density": (ctypes.c_int, [_VP, _VP, _D, ctypes.c_int64, _D]),
+    "sg_knn_distance": (ctypes.c_int, [_D, ctypes.c_int64, ctypes.c_int, _D]),
+    "sg_grid_sample_density_device": (ctypes.c_int, [_VP, _D, ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _D]),
 }
 
 EXPORTED = tuple(_SIGS)

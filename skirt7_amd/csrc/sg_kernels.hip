@@ -16,6 +16,7 @@
 #include "sg_mc.hpp"
 #include "sg_split.hpp"
 #include "sg_pan.hpp"
+#include "sg_geom.hpp"
 
 namespace sg {
 
@@ -118,29 +119,6 @@ __global__ void __launch_bounds__(256) k_tau(const typename W::View g, int64_t n
     out[i] = (double)res;
 }
 
-// An exponential disc, optionally spiral-decorated, as the host sampler evaluates it (ExpDiskGeometry.cpp:103-114,
-// SpiralStructureGeometryDecorator.cpp:165-172): normalisation and spiral constants come from the host.
-struct DiscDev {
-    double hR, hz, Rmax, zmax, Rmin, rho0;
-    double m, tanp, R0, phi0, w, CN;
-    int spiral, N;
-};
-
-__device__ double disc_density(const DiscDev& d, double x, double y, double z)
-{
-    double R = sqrt(x * x + y * y), absz = fabs(z);
-    if (d.Rmax > 0.0 && R > d.Rmax) return 0.0;
-    if (d.zmax > 0.0 && absz > d.zmax) return 0.0;
-    if (R < d.Rmin) return 0.0;
-    double rho = d.rho0 * exp(-R / d.hR) * exp(-absz / d.hz);
-    if (d.spiral) {
-        double phi = atan2(y, x);
-        double gamma = log(R / d.R0) / d.tanp + d.phi0 + 0.5 * M_PI / d.m;
-        rho *= (1.0 - d.w) + d.w * d.CN * pow(sin(0.5 * d.m * (gamma - phi)), (double)(2 * d.N));
-    }
-    return rho;
-}
-
 // DustSystem::setSampleDensityBody (DustSystem.cpp:145-172) for a Cartesian grid: cell m's density is scale
 // times the mean of nsamples disc densities at positions in the cell's box drawn from the counter-based
 // stream (seed, m, sample) that the host sampler uses; written as the grid's FP64 and FP32 densities.
@@ -153,11 +131,9 @@ __global__ void k_resample_cart(const double* xv, const double* yv, const double
     double x0 = xv[i], y0 = yv[j], z0 = zv[k], x1 = xv[i + 1], y1 = yv[j + 1], z1 = zv[k + 1];
     double sum = 0;
     for (int q = 0; q < nsamples; q++) {
-        uint32_t c0 = (uint32_t)m, c1 = (uint32_t)((uint64_t)m >> 32), c2 = (uint32_t)q, c3 = 0x5e7u;
-        philox10(c0, c1, c2, c3, seed, 0x9d2c5680u);
-        const double s32 = 1.0 / 4294967296.0;
-        double u0 = ((double)c0 + 0.5) * s32, u1 = ((double)c1 + 0.5) * s32, u2 = ((double)c2 + 0.5) * s32;
-        sum += disc_density(d, x0 + u0 * (x1 - x0), y0 + u1 * (y1 - y0), z0 + u2 * (z1 - z0));
+        double u[3];
+        setup_sample3(seed, (uint64_t)m, (uint32_t)q, u);
+        sum += disc_density(d, x0 + u[0] * (x1 - x0), y0 + u[1] * (y1 - y0), z0 + u[2] * (z1 - z0));
     }
     double r = scale * sum / nsamples;
     rho[m] = r;
@@ -869,11 +845,7 @@ int sg_grid_resample_density(sg_grid* g, const double* geom16, double scale, int
                                    "their structure from the density: a new density means a new grid)");
     SG_REQUIRE((int)geom16[0] == 1 && geom16[1] > 0 && geom16[2] > 0, "the dust geometry must be an exponential disc");
     SG_REQUIRE(nsamples >= 1 && scale >= 0 && std::isfinite(scale), "need at least one sample and a finite scale");
-    DiscDev d{};
-    d.hR = geom16[1]; d.hz = geom16[2]; d.Rmax = geom16[3]; d.zmax = geom16[4]; d.Rmin = geom16[5];
-    d.spiral = geom16[6] != 0;
-    d.m = geom16[7]; d.R0 = geom16[9]; d.phi0 = geom16[10]; d.w = geom16[11]; d.N = (int)geom16[12];
-    disc_density_constants(geom16, &d.rho0, &d.tanp, &d.CN);
+    const DiscDev d = disc_dev(geom16);
     const CartView& c = g->cart;
     hipLaunchKernelGGL(k_resample_cart, dim3(blocks_for(g->ncells, 256)), dim3(256), 0, 0, c.xv, c.yv, c.zv, c.Nx,
                        c.Ny, c.Nz, d, scale, nsamples, seed, g->rho.as<double>(), g->rhof.as<float>());

@@ -142,6 +142,58 @@ class TreeDustGrid(DustGrid):
 
 
     @classmethod
+    def subdivide_device(cls, extent, geometry, scale, total_mass, minlevel=2, maxlevel=6, maxmassfraction=1e-6,
+                         nsamples=100, seed=4357, search=SG_NEIGHBOR, maxopticaldepth=0.0, maxdensdispfraction=0.0):
+        """subdivide built on the device, level by level (exponential-disc geometries): the same sample
+        positions and stopping criteria, so the same subdivision up to the rounding of the device's exp, log
+        and pow at a node whose criterion value lies within it of the threshold."""
+        e = as_f64(extent)
+        geoms = list(geometry) if isinstance(geometry, (list, tuple)) else [geometry]
+        scales = as_f64(np.atleast_1d(scale))
+        if len(scales) != len(geoms):
+            raise FatalError("one scale per dust component is needed")
+        g = as_f64(np.concatenate([x.params() for x in geoms]))
+        cap = max(1 << 22, int(4.0 / maxmassfraction) if maxmassfraction > 0 else 0)
+        while True:
+            flags = np.zeros(cap, np.uint8)
+            n = ctypes.c_int64()
+            st = lib().sg_tree_subdivide_device(cls.arity, dptr(e), len(geoms), dptr(g), dptr(scales), total_mass,
+                                                minlevel, maxlevel, maxmassfraction, maxopticaldepth,
+                                                maxdensdispfraction, nsamples, seed, flags.ctypes.data_as(capi._U8),
+                                                cap, ctypes.byref(n))
+            if st == -5:
+                cap = n.value
+                continue
+            check(st)
+            return cls(e, flags[:n.value].copy(), search)
+
+    @classmethod
+    def subdivide_sph(cls, extent, sph, minlevel=2, maxlevel=6, maxmassfraction=1e-6, maxopticaldepth=0.0,
+                      maxdensdispfraction=0.0, nsamples=100, seed=4357, search=SG_NEIGHBOR, masses=False):
+        """TreeDustGrid over an SPHDustDistribution (TreeDustGrid.cpp:49-230), built on the device: node masses
+        from the particles' mass in the node's box (TreeNodeBoxDensityCalculator), or from density samples
+        when the dispersion criterion is set (TreeDustGrid.cpp:70). With masses=True also returns each node's
+        mass (with masses in box: every node beyond minlevel; with samples: the nodes the criteria evaluated;
+        -1 elsewhere)."""
+        e = as_f64(extent)
+        # a build that outgrows cap reports its node count: the second try fits
+        cap = max(1 << 22, int(4.0 / maxmassfraction) if maxmassfraction > 0 else 0)
+        while True:
+            flags = np.zeros(cap, np.uint8)
+            mass = np.zeros(cap) if masses else None
+            n = ctypes.c_int64()
+            st = lib().sg_tree_subdivide_sph(cls.arity, dptr(e), sph._h, minlevel, maxlevel, maxmassfraction,
+                                             maxopticaldepth, maxdensdispfraction, nsamples, seed,
+                                             flags.ctypes.data_as(capi._U8), dptr(mass) if masses else None, cap,
+                                             ctypes.byref(n))
+            if st == -5:
+                cap = n.value
+                continue
+            check(st)
+            grid = cls(e, flags[:n.value].copy(), search)
+            return (grid, mass[:n.value].copy()) if masses else grid
+
+    @classmethod
     def subdivide_field(cls, extent, field, minlevel=2, maxlevel=30, maxmassfraction=1e-6, search=SG_NEIGHBOR,
                         maxopticaldepth=0.0):
         """The subdivision with exact node masses from a voxel density field field[i, j, k] over the box (mass
@@ -157,7 +209,7 @@ class TreeDustGrid(DustGrid):
                                                   maxmassfraction, maxopticaldepth, flags.ctypes.data_as(capi._U8),
                                                   cap, ctypes.byref(n))
             if st == -5:
-                cap = max(2 * cap, n.value + 1024)
+                cap = n.value
                 continue
             check(st)
             return cls(e, flags[:n.value].copy(), search)
@@ -410,17 +462,92 @@ class DustSystem:
             self._h = None
 
 
-def sample_density(grid, geometry, scale, nsamples=100, seed=4357):
-    """Cell densities by DustSystem::setSampleDensityBody (DustSystem.cpp:145-172)."""
+def sample_density(grid, geometry, scale, nsamples=100, seed=4357, device=False):
+    """Cell densities by DustSystem::setSampleDensityBody (DustSystem.cpp:145-172), on the host or (device=True,
+    exponential discs) on the device at the same positions."""
     out = np.zeros(grid.numCells())
-    check(lib().sg_grid_sample_density(grid._h, dptr(as_f64(geometry.params())), scale, nsamples, seed,
-                                       dptr(out)))
+    fn = lib().sg_grid_sample_density_device if device else lib().sg_grid_sample_density
+    check(fn(grid._h, dptr(as_f64(geometry.params())), scale, nsamples, seed, dptr(out)))
     return out
 
 
 def resample_density(grid, geometry, scale, nsamples=100, seed=4357):
     """The same sampling on the device, stored as a Cartesian grid's densities (no host round trip)."""
     check(lib().sg_grid_resample_density(grid._h, dptr(as_f64(geometry.params())), scale, nsamples, seed))
+
+
+class SPHDustDistribution:
+    """SPHDustDistribution (SPHDustDistribution.cpp:38-256) from a table of SPH gas particles, rows of
+    x y z h M Z T (lengths in the grid's unit, masses in any mass unit): particles hotter than
+    max_temperature are ignored (when both are positive), the dust is dust_fraction of the metal mass, a
+    negative total gas or metal mass suppresses all dust. Densities and masses in a box are evaluated on the
+    device over the reference's 20^3 particle grid (SPHGasParticleGrid.cpp)."""
+
+    def __init__(self, particles, dust_fraction, max_temperature=0.0):
+        rows = as_f64(particles).reshape(-1, 7)
+        h = _handle()
+        check(lib().sg_sph_create(dptr(rows), rows.shape[0], dust_fraction, max_temperature, ctypes.byref(h)))
+        self._h = h
+        cnt, mass, ent = ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
+        self.grid_borders = np.zeros((3, 21))
+        check(lib().sg_sph_info(self._h, ctypes.byref(cnt), ctypes.byref(mass), dptr(self.grid_borders),
+                                ctypes.byref(ent)))
+        self.count = cnt.value
+        self.list_entries = ent.value
+        self._mass = mass.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and capi._lib is not None:
+            capi._lib.sg_sph_destroy(self._h)
+            self._h = None
+
+    def mass(self):
+        """SPHDustDistribution::mass (SPHDustDistribution.cpp:248-256)."""
+        return self._mass
+
+    def massInBox(self, boxes):
+        """SPHDustDistribution::massInBox (SPHDustDistribution.cpp:226-236) of boxes[n, 6] (min then max)."""
+        b = as_f64(boxes).reshape(-1, 6)
+        out = np.zeros(b.shape[0])
+        check(lib().sg_sph_mass_in_box(self._h, dptr(b), b.shape[0], dptr(out)))
+        return out
+
+    def density(self, positions):
+        """SPHDustDistribution::density (SPHDustDistribution.cpp:192-202) at positions[n, 3]."""
+        p = as_f64(positions).reshape(-1, 3)
+        out = np.zeros(p.shape[0])
+        check(lib().sg_sph_density(self._h, dptr(p), p.shape[0], dptr(out)))
+        return out
+
+    def cell_list(self, i, j, k):
+        """The particle grid's list of cell (i, j, k) (SPHGasParticleGrid.cpp:94-132)."""
+        n = ctypes.c_int64()
+        check(lib().sg_sph_cell_list(self._h, i, j, k, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, np.int32)
+        check(lib().sg_sph_cell_list(self._h, i, j, k, out.ctypes.data_as(capi._I32), n.value, ctypes.byref(n)))
+        return out
+
+
+def sph_density(grid, sph, node_masses=None):
+    """TreeDustGrid::density (TreeDustGrid.cpp:655-660) over an SPHDustDistribution: each cell's mass in its
+    box over its volume; node_masses, the masses subdivide_sph(masses=True) returned for this tree, serve as
+    the cells' masses, else they are evaluated on the device."""
+    out = np.zeros(grid.numCells())
+    if node_masses is None:
+        check(lib().sg_grid_sph_density(grid._h, sph._h, None, 0, dptr(out)))
+    else:
+        m = as_f64(node_masses)
+        check(lib().sg_grid_sph_density(grid._h, sph._h, dptr(m), len(m), dptr(out)))
+    return out
+
+
+def knn_distance(positions, k=32):
+    """Distance from each point to its k-th nearest other point (on the device): smoothing lengths for a
+    generated particle set."""
+    p = as_f64(positions).reshape(-1, 3)
+    out = np.zeros(p.shape[0])
+    check(lib().sg_knn_distance(dptr(p), p.shape[0], k, dptr(out)))
+    return out
 
 
 def field_density(grid, field):

@@ -66,11 +66,14 @@ def disc_cartesian(grid_n=50, pixels=256, packages=1e6, seed=4357, nsamples=100,
 
 
 def spiral_octree(tau=2.0, maxlevel=8, pixels=512, packages=1e8, seed=4357, nsamples=100, search=SG_NEIGHBOR,
-            maxmassfraction=1e-6, tau_cut=0.0):
+            maxmassfraction=1e-6, tau_cut=0.0, device_setup=True):
+    """Octree over the spiral galaxy's dust; the subdivision and the cells' density samples on the device
+    (device_setup, the default) or on the host, at the same sample positions."""
     comps, dgeom, mix, scale = _galaxy(True, tau)
-    grid = S.OctTreeDustGrid.subdivide(EXTENT, dgeom, scale, total_mass=scale, minlevel=2, maxlevel=maxlevel,
-                                       maxmassfraction=maxmassfraction, nsamples=nsamples, seed=seed, search=search)
-    rho = S.sample_density(grid, dgeom, scale, nsamples, seed)
+    build = S.OctTreeDustGrid.subdivide_device if device_setup else S.OctTreeDustGrid.subdivide
+    grid = build(EXTENT, dgeom, scale, total_mass=scale, minlevel=2, maxlevel=maxlevel,
+                 maxmassfraction=maxmassfraction, nsamples=nsamples, seed=seed, search=search)
+    rho = S.sample_density(grid, dgeom, scale, nsamples, seed, device=device_setup)
     d = dict(name="spiral_octree", kind="octree", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=dgeom,
              instr=_frames((0.0, 45.0, 75.0, 89.0), pixels))
     return _finish(d, packages, seed, tau_cut)
@@ -180,25 +183,60 @@ def lognormal_field(n, sigma=1.5, slope=-11.0 / 3.0, seed=4357):
     return rho / rho.mean()
 
 
-def lognormal_kdtree(nvox=256, maxmassfraction=9e-8, maxlevel=40, pixels=1024, packages=1e9, seed=4357, tau=1.0,
-            search=SG_TOPDOWN, half=15.0):
-    """kd-tree: a log-normal dust field (sigma_ln 1.5, P(k) ~ k^-11/3) over a 30-kpc box, given as an nvox^3
-    voxel field in place of the particle set it would be sampled from; a BinTreeDustGrid with alternating
-    split axes refined by mass fraction (9e-8 gives 1.61e7 leaves at nvox 256), with exact mass-in-box
-    cell densities; the field normalised to a
-    mean optical depth tau across the box height. Stars: an exponential disc (hR 4, hz 0.4). Eight frames of
-    pixels^2 at inclinations 30 and 60 deg, azimuths 0, 90, 180, 270 deg."""
-    ext = np.array([-half, -half, -half, half, half, half])
+def lognormal_particles(nparticles, nvox=256, seed=4357, half=15.0, neighbours=32, total_mass=1.0):
+    """An SPH particle table (rows x y z h M Z T) sampled from the log-normal field over a box of half-width
+    half: each particle in a voxel drawn with probability proportional to the voxel's density, at a uniform
+    position inside it; smoothing length h the distance to the neighbours-th nearest other particle; equal
+    masses summing to total_mass; metallicity 1 and no temperature."""
     field = lognormal_field(nvox, seed=seed)
-    grid = S.BinTreeDustGrid.subdivide_field(ext, field, minlevel=3, maxlevel=maxlevel,
-                                             maxmassfraction=maxmassfraction, search=search)
+    rng = np.random.default_rng(seed + 1)
+    n = int(nparticles)
+    cdf = np.cumsum(field.ravel())
+    idx = np.minimum(np.searchsorted(cdf, rng.random(n) * cdf[-1], side="right"), cdf.size - 1)
+    ijk = np.stack(np.unravel_index(idx, field.shape), axis=1)
+    pos = -half + (ijk + rng.random((n, 3))) * (2.0 * half / nvox)
+    h = S.knn_distance(pos, neighbours)
+    return np.column_stack([pos, h, np.full(n, total_mass / n), np.ones(n), np.zeros(n)])
+
+
+def lognormal_kdtree(nvox=256, maxmassfraction=None, maxlevel=40, pixels=1024, packages=1e9, seed=4357, tau=1.0,
+                     search=SG_TOPDOWN, half=15.0, medium="particles", particles=1e7, neighbours=32):
+    """kd-tree over a log-normal dust medium (sigma_ln 1.5, P(k) ~ k^-11/3) in a 30-kpc box: by default an SPH
+    particle set (SPHDustDistribution) of `particles` particles sampled from the field with smoothing lengths
+    enclosing `neighbours` neighbours, the BinTreeDustGrid (alternating split axes) built over it on the device
+    with node masses from the particles' mass in each node's box, and the cells' exact mass-in-box densities
+    (TreeDustGrid.cpp:655-660); medium="voxels" takes the nvox^3 voxel field itself as the medium instead. The
+    medium's mass gives a mean optical depth tau across the box height. maxmassfraction (default 9e-8) sets the
+    leaf count: 1.615e7 for the particles and 1.61e7 for the voxel field at the defaults. Stars: an
+    exponential disc (hR 4, hz 0.4). Eight frames of pixels^2 at inclinations 30 and 60 deg, azimuths 0, 90,
+    180, 270 deg."""
+    ext = np.array([-half, -half, -half, half, half, half])
     mix = S.SimpleOligoDustMix([1.0], [0.6], [0.6])
-    rho = S.field_density(grid, field) * tau / (2.0 * half * mix.kappaext[0])
+    d = dict(name="lognormal_kdtree", kind="bintree", mix=mix, dgeom=None)
+    if medium == "voxels":
+        field = lognormal_field(nvox, seed=seed)
+        grid = S.BinTreeDustGrid.subdivide_field(ext, field, minlevel=3, maxlevel=maxlevel,
+                                                 maxmassfraction=9e-8 if maxmassfraction is None else maxmassfraction,
+                                                 search=search)
+        rho = S.field_density(grid, field) * tau / (2.0 * half * mix.kappaext[0])
+        d["field"] = field
+    elif medium == "particles":
+        # mean density tau / (2 half kappa) over the box volume (2 half)^3
+        mass = tau * (2.0 * half) ** 2 / mix.kappaext[0]
+        rows = lognormal_particles(particles, nvox, seed, half, neighbours, mass)
+        sph = S.SPHDustDistribution(rows, 1.0)
+        grid, masses = S.BinTreeDustGrid.subdivide_sph(ext, sph, minlevel=3, maxlevel=maxlevel,
+                                                       maxmassfraction=9e-8 if maxmassfraction is None
+                                                       else maxmassfraction, search=search, masses=True)
+        rho = S.sph_density(grid, sph, masses)
+        d["sph"] = sph
+        d["particles"] = rows
+    else:
+        raise ValueError("medium is 'particles' or 'voxels'")
     comps = [S.OligoStellarComp(S.ExpDiskGeometry(4.0, 0.4), [1.0e10])]
     ins = [S.FrameInstrument(DISTANCE, math.radians(i), math.radians(a), 0.0, pixels, pixels, 2.0 * half, 2.0 * half)
            for i in (30.0, 60.0) for a in (0.0, 90.0, 180.0, 270.0)]
-    d = dict(name="lognormal_kdtree", kind="bintree", grid=grid, rho=rho, mix=mix, comps=comps, dgeom=None,
-             instr=S.InstrumentSystem(ins), field=field)
+    d.update(grid=grid, rho=rho, comps=comps, instr=S.InstrumentSystem(ins))
     return _finish(d, packages, seed)
 
 

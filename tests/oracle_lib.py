@@ -65,6 +65,18 @@ _SIGS = {
                                         ctypes.c_uint64, _I32, _D]),
     "or_exponcutoff_sample": (None, [ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, _D]),
     "or_pixelondetector": (None, [_D, ctypes.c_int64, _D, _I32, _D]),
+    "or_sph_last_error": (ctypes.c_char_p, []),
+    "or_sph_create": (_V, [_D, ctypes.c_int64, ctypes.c_double, ctypes.c_double]),
+    "or_sph_destroy": (None, [_V]),
+    "or_sph_count": (ctypes.c_int64, [_V]),
+    "or_sph_mass": (ctypes.c_double, [_V]),
+    "or_sph_myerf": (ctypes.c_double, [ctypes.c_double]),
+    "or_sph_grid": (None, [_V, _D]),
+    "or_sph_cell_list": (ctypes.c_int64, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I32, ctypes.c_int64]),
+    "or_sph_mass_in_box": (None, [_V, _D, ctypes.c_int64, _D]),
+    "or_sph_density": (None, [_V, _D, ctypes.c_int64, _D]),
+    "or_sph_subdivide": (ctypes.c_int, [_V, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, _U8, _D, ctypes.c_int64, _I64]),
 }
 
 _lib = None
@@ -322,3 +334,61 @@ def pixelondetector(params10, xyz):
     kobs = np.zeros(3)
     lib().or_pixelondetector(dp(f64(params10)), r.shape[0], dp(r), pix.ctypes.data_as(_I32), dp(kobs))
     return pix, kobs
+
+
+class SphMedium:
+    """The oracle's SPHDustDistribution (oracle/sph.cpp) over rows of x y z h M Z T."""
+
+    def __init__(self, rows, fdust=1.0, tmax=0.0):
+        self.L = lib()
+        self.rows = f64(rows).reshape(-1, 7)
+        self.h = self.L.or_sph_create(dp(self.rows), self.rows.shape[0], fdust, tmax)
+
+    def __del__(self):
+        if getattr(self, "h", None) and getattr(self, "L", None) is not None:
+            self.L.or_sph_destroy(self.h)
+            self.h = None
+
+    def count(self):
+        return self.L.or_sph_count(self.h)
+
+    def mass(self):
+        return self.L.or_sph_mass(self.h)
+
+    def grid(self):
+        out = np.zeros((3, 21))
+        self.L.or_sph_grid(self.h, dp(out))
+        return out
+
+    def cell_list(self, i, j, k):
+        n = self.L.or_sph_cell_list(self.h, i, j, k, None, 0)
+        out = np.zeros(n, np.int32)
+        self.L.or_sph_cell_list(self.h, i, j, k, out.ctypes.data_as(_I32), n)
+        return out
+
+    def mass_in_box(self, boxes):
+        b = f64(boxes).reshape(-1, 6)
+        out = np.zeros(b.shape[0])
+        self.L.or_sph_mass_in_box(self.h, dp(b), b.shape[0], dp(out))
+        return out
+
+    def density(self, pos):
+        p = f64(pos).reshape(-1, 3)
+        out = np.zeros(p.shape[0])
+        self.L.or_sph_density(self.h, dp(p), p.shape[0], dp(out))
+        return out
+
+    def subdivide(self, arity, extent, minlevel, maxlevel, maxmassfraction, maxopticaldepth=0.0, cap=1 << 22):
+        """TreeDustGrid's subdivision with the box calculator: (flags, masses) in creation order."""
+        e = f64(extent)
+        flags = np.zeros(cap, np.uint8)
+        masses = np.zeros(cap)
+        n = ctypes.c_int64()
+        rc = self.L.or_sph_subdivide(self.h, arity, dp(e), minlevel, maxlevel, maxmassfraction, maxopticaldepth,
+                                     flags.ctypes.data_as(_U8), dp(masses), cap, ctypes.byref(n))
+        assert rc == 0, self.L.or_sph_last_error()
+        return flags[:n.value].copy(), masses[:n.value].copy()
+
+
+def myerf(x):
+    return lib().or_sph_myerf(float(x))

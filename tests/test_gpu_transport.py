@@ -117,7 +117,7 @@ def test_split_and_lane_pipelines_agree(c0_small, which):
 
 @pytest.fixture(scope="module")
 def c4_small():
-    return workloads.lognormal_kdtree(nvox=16, maxmassfraction=2e-4, pixels=32, packages=6000)
+    return workloads.lognormal_kdtree(nvox=16, maxmassfraction=2e-4, pixels=32, packages=6000, medium="voxels")
 
 
 def test_kdtree_fp64_matches_oracle(c4_small):
