@@ -13,9 +13,10 @@ declarations.
 Supported classes (anything else raises FatalError naming the class):
   OligoMonteCarloSimulation (+ MonteCarloSimulation, Simulation properties), SIUnits,
   ExtragalacticUnits, Random, OligoWavelengthGrid, StellarSystem, OligoStellarComp, ExpDiskGeometry,
-  SersicGeometry, SpiralStructureGeometryDecorator, OligoDustSystem, CompDustDistribution (one DustComp),
+  SersicGeometry, SpiralStructureGeometryDecorator, OligoDustSystem, CompDustDistribution (DustComps),
+  SPHDustDistribution (particle file read from the ski file's directory; tree dust grids only),
   DustComp, SimpleOligoDustMix, FaceOnDustCompNormalization, DustMassDustCompNormalization,
-  CartesianDustGrid with LinMesh, OctTreeDustGrid, BinTreeDustGrid (mass-fraction criterion),
+  CartesianDustGrid with LinMesh, OctTreeDustGrid, BinTreeDustGrid (the three stopping criteria),
   VoronoiDustGrid (sites from the dust density or uniform), InstrumentSystem, FrameInstrument.
 
 The transport works in parsecs (lengths) and watts (luminosities); outputs are calibrated by
@@ -29,7 +30,7 @@ import numpy as np
 from . import output
 from . import skirt as S
 from .capi import SG_BOOKKEEPING, SG_MIXED, SG_NEIGHBOR, SG_TOPDOWN
-from .output import PC, FatalError, Units
+from .output import MSUN, PC, FatalError, Units
 
 # TreeDustGrid::SearchMethod names (TreeDustGrid.hpp:47-50)
 _SEARCH = {"TopDown": SG_TOPDOWN, "Neighbor": SG_NEIGHBOR, "Bookkeeping": SG_BOOKKEEPING}
@@ -186,7 +187,8 @@ class Spec:
     """A ski file's parameters, converted and checked on the host: everything a run needs, with
     lengths in pc (transport) and SI (calibration), luminosities in W per wavelength bin."""
 
-    def __init__(self, text, resources=None):
+    def __init__(self, text, resources=None, inputs=None):
+        self.inputs = inputs or "."
         sim = parse(text)
         _require(sim, "OligoMonteCarloSimulation")
         uitem = sim.one("units", "ExtragalacticUnits")
@@ -253,7 +255,9 @@ class Spec:
         val = self.val
         _require(ds, "OligoDustSystem")
         dd = ds.one("dustDistribution", "CompDustDistribution")
-        _require(dd, "CompDustDistribution")
+        _require(dd, "CompDustDistribution", "SPHDustDistribution")
+        if dd.cls == "SPHDustDistribution":
+            return self._sph(ds, dd, nl)
         comps = dd.many("components")
         if not comps:
             raise FatalError("the dust distribution has no components")
@@ -263,6 +267,35 @@ class Spec:
         d["sampleCount"] = val.integer(ds, "sampleCount", "100")
         d["writeMeanIntensity"] = val.boolean(ds, "writeMeanIntensity", "no")
         d["grid"] = self._grid(ds.one("dustGrid", "OctTreeDustGrid"))
+        return d
+
+    def _sph(self, ds, dd, nl):
+        """SPHDustDistribution (SPHDustDistribution.cpp:38-100): the particle file's rows x y z h M Z [T]
+        (pc, pc, pc, pc, Msun, metallicity, K; TextInFile::readRow with the temperature optional), the dust
+        fraction of the metals and the maximum temperature, and its SimpleOligoDustMix."""
+        val = self.val
+        fdust = val.double(dd, "dustFraction", None, "0.3")
+        if not fdust > 0:
+            raise FatalError("The dust fraction should be positive")
+        tmax = val.double(dd, "maximumTemperature", "temperature", "75000 K")
+        fn = val.string(dd, "filename")
+        rows = read_particles(os.path.join(self.inputs, fn))
+        mix = dd.one("dustMix", "InterstellarDustMix")
+        if mix is None:
+            raise FatalError("Dust mix was not set")
+        _require(mix, "SimpleOligoDustMix")
+        d = dict(sph=dict(rows=rows, dustFraction=fdust, maximumTemperature=tmax))
+        d["opacities"] = val.doubles(mix, "opacities", "opacity")
+        d["albedos"] = val.doubles(mix, "albedos")
+        d["asymmetry"] = val.doubles(mix, "asymmetryParameters", None, ",".join(["0"] * nl))
+        if not (len(d["opacities"]) == len(d["albedos"]) == len(d["asymmetry"]) == nl):
+            raise FatalError("the dust mix needs one opacity, albedo and asymmetry parameter per wavelength")
+        d["sampleCount"] = val.integer(ds, "sampleCount", "100")
+        d["writeMeanIntensity"] = val.boolean(ds, "writeMeanIntensity", "no")
+        d["grid"] = self._grid(ds.one("dustGrid", "OctTreeDustGrid"))
+        if d["grid"]["kind"] not in ("OctTreeDustGrid", "BinTreeDustGrid"):
+            raise FatalError("an SPH dust distribution is supported on tree dust grids (their cells' exact mass in "
+                             "box, TreeDustGrid::density)")
         return d
 
     def _dustcomp(self, dc, nl):
@@ -334,6 +367,33 @@ class Spec:
         return spec
 
 
+def read_particles(path):
+    """SPH gas particles from a column text file as SPHDustDistribution reads it (TextInFile::readRow with
+    seven columns, the last optional: x y z h M Z T): blank lines and lines starting with # are skipped, a
+    missing temperature reads as 0, extra columns are ignored. Returns an (n, 7) array in the file's units."""
+    rows = []
+    try:
+        f = open(path)
+    except OSError:
+        raise FatalError("Could not open the data file " + path)
+    with f:
+        for line in f:
+            t = line.split()
+            if not t or t[0].startswith("#"):
+                continue
+            # stringstream reads: a seventh read after trailing blanks fails rather than reaching the end
+            if len(t) < 6 or (len(t) == 6 and line.rstrip("\n").rstrip("\r") != line.rstrip()):
+                if len(t) < 6:
+                    raise FatalError("One or more required value(s) on text line are missing")
+                raise FatalError("Input text is not formatted as a floating point number")
+            try:
+                v = [float(x) for x in t[:7]]
+            except ValueError:
+                raise FatalError("Input text is not formatted as a floating point number")
+            rows.append(v + [0.0] * (7 - len(v)))
+    return np.array(rows, dtype=np.float64).reshape(-1, 7)
+
+
 def _resource(resources, name):
     base = resources or os.environ.get("SKIRT7_AMD_RESOURCES")
     if not base:
@@ -357,8 +417,10 @@ class Simulation:
             [S.FrameInstrument(f["distance"] / PC, f["inclination"], f["azimuth"], f["positionAngle"], f["pixelsX"],
                                f["pixelsY"], f["fovX"] / PC, f["fovY"] / PC, f["centerX"] / PC, f["centerY"] / PC)
              for f in sp.frames], sp.nlambda)
-        self.grid = self.dust = self.rho = self.sites = None
-        if sp.dust is not None:
+        self.grid = self.dust = self.rho = self.sites = self.sph = None
+        if sp.dust is not None and "sph" in sp.dust:
+            self._sph_dust(sp.dust, sp.seed)
+        elif sp.dust is not None:
             d = sp.dust
             dcs = d["components"]
             geoms, scales = [c["geometry"] for c in dcs], [c["scale"] for c in dcs]
@@ -378,6 +440,26 @@ class Simulation:
                                                minWeightReduction=sp.minWeightReduction,
                                                minScattEvents=sp.minScattEvents, scattBias=sp.scattBias,
                                                seed=sp.seed)
+
+    def _sph_dust(self, d, seed):
+        """The SPH particles in the transport's units (pc; masses in kg over pc^2, so that kappa rho ds is an
+        optical depth with rho per pc^3 and ds in pc, as for DustMassDustCompNormalization), the tree built over
+        them on the device and its cells' mass-in-box densities (TreeDustGrid::density)."""
+        p = d["sph"]
+        rows = np.array(p["rows"], dtype=np.float64, copy=True)
+        rows[:, 4] *= MSUN / (PC * PC)
+        self.sph = S.SPHDustDistribution(rows, p["dustFraction"], p["maximumTemperature"])
+        g = d["grid"]
+        cls = S.OctTreeDustGrid if g["kind"] == "OctTreeDustGrid" else S.BinTreeDustGrid
+        self.grid, masses = cls.subdivide_sph(g["extent"], self.sph, minlevel=g["minLevel"], maxlevel=g["maxLevel"],
+                                              maxmassfraction=g["maxMassFraction"],
+                                              maxopticaldepth=g["maxOpticalDepth"],
+                                              maxdensdispfraction=g["maxDensDispFraction"], nsamples=g["sampleCount"],
+                                              seed=seed, search=_SEARCH[g["search"]], masses=True)
+        # with samples (the dispersion criterion) the masses at maxlevel were not evaluated: evaluate the cells
+        self.rho = S.sph_density(self.grid, self.sph, None if g["maxDensDispFraction"] > 0 else masses)
+        self.mix = S.SimpleOligoDustMix(d["opacities"], d["albedos"], d["asymmetry"])
+        self.dust = S.DustSystem(self.grid, self.rho, self.mix, writeMeanIntensity=d["writeMeanIntensity"])
 
     def _grid(self, g, geoms, scales, seed):
         """The grid of the dust system whose components have geometries geoms and normalisations scales (the
@@ -433,7 +515,8 @@ def load(path, resources=None):
     """A ski file from disk; the output prefix is its file name without the extension."""
     with open(path) as f:
         text = f.read()
-    return Simulation(Spec(text, resources), os.path.splitext(os.path.basename(path))[0])
+    return Simulation(Spec(text, resources, os.path.dirname(os.path.abspath(path))),
+                      os.path.splitext(os.path.basename(path))[0])
 
 
 def run(path, outdir=None, resources=None, precision=SG_MIXED):

@@ -214,3 +214,45 @@ def test_particle_kdtree_transport_against_oracle():
         g = w["instr"].frame(i)[0]
         assert o.sum() > 0
         assert abs(g.sum() - o.sum()) <= 1e-6 * o.sum()
+
+
+@pytest.mark.parametrize("grid", ["bintree", "octree"])
+def test_ski_file_with_sph_distribution(tmp_path, grid):
+    """A ski file with an SPHDustDistribution over a particle file and a tree grid: the tree equals the oracle's
+    subdivision of the same particles (converted to the transport's units), its densities the oracle's mass in
+    each cell over its volume, and FP64 packets the oracle's."""
+    from skirt7_amd import ski
+    from skirt7_amd.output import MSUN, PC
+    from tests.test_ski_fits import SUN, sph_ski, write_particles
+    d = tmp_path / "dat"
+    (d / "SED" / "Sun").mkdir(parents=True)
+    (d / "SED" / "Sun" / "SunSED.dat").write_text(SUN)
+    rows = write_particles(tmp_path / "particles.dat")
+    sim = ski.Simulation(ski.Spec(sph_ski(grid), str(d), str(tmp_path)))
+    conv = rows.copy()
+    conv[:, 4] *= MSUN / PC ** 2
+    ora = O.SphMedium(conv, 0.4, 2e4)
+    assert sim.sph.count == ora.count() < len(rows)
+    ext = np.array([-1e4, -1e4, -1e4, 1e4, 1e4, 1e4])
+    arity, lo, hi = (2, 3, 24) if grid == "bintree" else (8, 1, 7)
+    flags, masses = ora.subdivide(arity, ext, lo, hi, 2e-3)
+    assert len(flags) > 500 and np.array_equal(sim.grid.subdivided, flags)
+    boxes, _ = _tree_nodes(flags, arity, ext)
+    leaves = boxes[flags == 0]
+    assert np.array_equal(sim.rho, ora.mass_in_box(leaves) / np.prod(leaves[:, 3:] - leaves[:, :3], axis=1))
+    w = dict(kind="bintree" if grid == "bintree" else "octree", grid=sim.grid, rho=sim.rho, mix=sim.mix,
+             comps=sim.stars.components, instr=sim.instr, sim=sim.sim)
+    sc = O.scene_from_workload(w)
+    n = 20000
+    sim.instr.clear()
+    c = sim.sim.run_packets(0, 0, n, n, SG_FP64)
+    ref, rc = sc.run_packets(w, 0, 0, n, n, threads=16)
+    assert c["packets"] == rc[0] == n and c["scatterings"] > 0
+    off = 0
+    for i, ins in enumerate(sim.instr.instruments):
+        npx = ins.Nx * ins.Ny
+        o = ref[off:off + npx].reshape(ins.Ny, ins.Nx)
+        off += npx
+        g = sim.instr.frame(i)[0]
+        assert o.sum() > 0
+        assert abs(g.sum() - o.sum()) <= 1e-6 * o.sum()

@@ -357,3 +357,79 @@ def test_ski_two_dust_components(resources):
     assert abs(comps[1]["scale"] - 1e7 * MSUN / PC ** 2) < 1e-9 * comps[1]["scale"]
     # the first component's entries stay at the top level
     assert sp.dust["scale"] == comps[0]["scale"] and sp.dust["opacities"] == [1.0]
+
+
+def sph_ski(grid="bintree", criteria='maxMassFraction="2e-3"'):
+    """ski_text with an SPHDustDistribution over particles.dat and a tree grid."""
+    grids = {"bintree": '<BinTreeDustGrid minX="-10 kpc" maxX="10 kpc" minY="-10 kpc" maxY="10 kpc" minZ="-10 kpc" '
+                        'maxZ="10 kpc" minLevel="3" maxLevel="24" searchMethod="TopDown" %s/>' % criteria,
+             "octree": '<OctTreeDustGrid minX="-10 kpc" maxX="10 kpc" minY="-10 kpc" maxY="10 kpc" minZ="-10 kpc" '
+                       'maxZ="10 kpc" minLevel="1" maxLevel="7" %s/>' % criteria}
+    sph = """
+            <dustDistribution type="DustDistribution">
+              <SPHDustDistribution filename="particles.dat" dustFraction="0.4" maximumTemperature="2e4 K">
+                <dustMix type="DustMix"><SimpleOligoDustMix opacities="1 m2/kg" albedos="0.6" asymmetryParameters="0.6"/></dustMix>
+              </SPHDustDistribution>
+            </dustDistribution>
+            <dustGrid type="DustGrid">%s</dustGrid>""" % grids[grid]
+    text = ski_text(grid="octree")
+    start = text.index("<dustDistribution")
+    end = text.index("</dustGrid>") + len("</dustGrid>")
+    return text[:start] + sph.strip() + text[end:]
+
+
+def write_particles(path, n=6000, seed=3):
+    """A particle file (pc, pc, pc, pc, Msun, Z, K): clustered, with comments, a blank line and a few rows
+    without temperature; a few particles hotter than the maximum temperature."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-6000, 6000, (8, 3))
+    pos = np.clip(c[rng.integers(0, 8, n)] + rng.normal(0, 1500, (n, 3)), -9900, 9900)
+    h = cKDTree(pos).query(pos, 33)[0][:, -1]
+    M = rng.uniform(1e5, 3e5, n)
+    Z = rng.uniform(0.01, 0.03, n)
+    T = rng.uniform(5e3, 1.5e4, n)
+    T[::97] = 5e4
+    lines = ["# x y z h M Z T", ""]
+    for i in range(n):
+        if i % 50 == 7:
+            lines.append("%.17g %.17g %.17g %.17g %.17g %.17g" % (pos[i, 0], pos[i, 1], pos[i, 2], h[i], M[i], Z[i]))
+        else:
+            lines.append("%.17g %.17g %.17g %.17g %.17g %.17g %.17g" % (pos[i, 0], pos[i, 1], pos[i, 2], h[i], M[i],
+                                                                        Z[i], T[i]))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    rows = np.column_stack([pos, h, M, Z, T])
+    rows[np.arange(n) % 50 == 7, 6] = 0.0
+    return rows
+
+
+def test_ski_sph_distribution(tmp_path, resources):
+    from skirt7_amd import ski
+    rows = write_particles(tmp_path / "particles.dat")
+    sp = ski.Spec(sph_ski(), resources, str(tmp_path))
+    d = sp.dust
+    assert d["sph"]["dustFraction"] == 0.4 and d["sph"]["maximumTemperature"] == 2e4
+    assert np.array_equal(d["sph"]["rows"], rows)
+    assert d["opacities"] == [1.0] and d["grid"]["kind"] == "BinTreeDustGrid"
+    # defaults of SPHDustDistribution (SPHDustDistribution.hpp: dustFraction 0.3, maximumTemperature 75000 K)
+    sp2 = ski.Spec(sph_ski().replace(' dustFraction="0.4" maximumTemperature="2e4 K"', ""), resources, str(tmp_path))
+    assert sp2.dust["sph"]["dustFraction"] == 0.3 and sp2.dust["sph"]["maximumTemperature"] == 75000.0
+    # a missing file, a missing column, a non-number, a Cartesian grid
+    from skirt7_amd.output import FatalError
+    with pytest.raises(FatalError, match="Could not open"):
+        ski.Spec(sph_ski(), resources, str(tmp_path / "nowhere"))
+    (tmp_path / "bad").mkdir()
+    (tmp_path / "bad" / "particles.dat").write_text("1 2 3 4 5\n")
+    with pytest.raises(FatalError, match="missing"):
+        ski.Spec(sph_ski(), resources, str(tmp_path / "bad"))
+    (tmp_path / "bad" / "particles.dat").write_text("1 2 3 4 5 x 7\n")
+    with pytest.raises(FatalError, match="floating point"):
+        ski.Spec(sph_ski(), resources, str(tmp_path / "bad"))
+    cart = sph_ski().replace(sph_ski()[sph_ski().index("<BinTreeDustGrid"):sph_ski().index("/>", sph_ski().index(
+        "<BinTreeDustGrid")) + 2], '<CartesianDustGrid minX="-1 kpc" maxX="1 kpc" minY="-1 kpc" maxY="1 kpc" '
+        'minZ="-1 kpc" maxZ="1 kpc"><meshX type="MoveableMesh"><LinMesh numBins="2"/></meshX><meshY '
+        'type="MoveableMesh"><LinMesh numBins="2"/></meshY><meshZ type="MoveableMesh"><LinMesh numBins="2"/></meshZ>'
+        '</CartesianDustGrid>')
+    with pytest.raises(FatalError, match="tree dust grids"):
+        ski.Spec(cart, resources, str(tmp_path))
