@@ -41,6 +41,86 @@ def log_wavelength_grid(lambdamin, lambdamax, n):
     return lam, hi - lo
 
 
+def wavelength_bins(lam):
+    """Each wavelength's bin borders in a PanWavelengthGrid (PanWavelengthGrid.cpp:24-33): the geometric means
+    of neighbouring wavelengths, the outer bins ending at the first and last wavelength. (lo, hi)."""
+    lam = np.asarray(lam, float)
+    n = len(lam)
+    lo = np.array([lam[0]] + [math.sqrt(lam[i - 1] * lam[i]) for i in range(1, n)])
+    hi = np.array([math.sqrt(lam[i] * lam[i + 1]) for i in range(n - 1)] + [lam[-1]])
+    return lo, hi
+
+
+def nearest_wavelength(lam, x):
+    """WavelengthGrid::nearest (WavelengthGrid.cpp:99-106): the index of the grid wavelength nearest to x on a
+    logarithmic scale, -1 outside the grid."""
+    lam = np.asarray(lam, float)
+    if x < lam[0] or x > lam[-1]:
+        return -1
+    ell = len(lam) - 2 if x == lam[-1] else int(np.searchsorted(lam, x, side="right")) - 1
+    return ell if x < math.sqrt(lam[ell] * lam[ell + 1]) else ell + 1
+
+
+def normalized_sed(Lv):
+    """SED::setluminosities (SED.cpp): the luminosities normalised to a unit sum."""
+    Lv = np.asarray(Lv, float)
+    total = 0.0
+    for v in Lv:
+        total += v
+    if not total > 0:
+        raise FatalError("The total luminosity in the SED is zero or negative (%g)" % total)
+    return Lv / total
+
+
+def blackbody_sed(lam, T):
+    """BlackBodySED::setupSelfBefore (BlackBodySED.cpp:25-55): each bin's luminosity the trapezoidal average of
+    B(lambda) lambda over 101 points log-spaced across the bin, times ln 10 dlog lambda; normalised."""
+    if not T > 0:
+        raise FatalError("the black body temperature T should be positive")
+    lo, hi = wavelength_bins(lam)
+    Lv = np.zeros(len(lam))
+    N = 100
+    for ell in range(len(lam)):
+        a, b = math.log10(lo[ell]), math.log10(hi[ell])
+        d = (b - a) / N
+        acc = 0.0
+        for i in range(N + 1):
+            w = 0.5 if i in (0, N) else 1.0
+            x = 10.0 ** (a + i * d)
+            acc += w * float(planck(np.array([x]), T)[0]) * x
+        Lv[ell] = acc * math.log(10.0) * d
+    return normalized_sed(Lv)
+
+
+def interpolate_loglog(x, x1, x2, f1, f2):
+    """NR::interpolate_loglog (NR.hpp:321-346): linear in log x and, where both values are positive, in log f."""
+    x, x1, x2 = math.log10(x), math.log10(x1), math.log10(x2)
+    logf = f1 > 0 and f2 > 0
+    if logf:
+        f1, f2 = math.log10(f1), math.log10(f2)
+    fx = f1 + ((x - x1) / (x2 - x1)) * (f2 - f1)
+    return 10.0 ** fx if logf else fx
+
+
+def sun_sed(lam, dlam, data):
+    """SunSED::setupSelfBefore (SunSED.cpp:25-52) on the grid: the emissivities of SunSED.dat (lambda in m, j)
+    resampled by NR::resample with log-log interpolation (zero beyond the table), times the bin widths;
+    normalised (SED::setemissivities)."""
+    lv, jv = np.asarray(data[0], float), np.asarray(data[1], float)
+    out = np.zeros(len(lam))
+    for l, x in enumerate(lam):
+        if abs(1.0 - x / lv[0]) < 1e-5:
+            out[l] = jv[0]
+        elif abs(1.0 - x / lv[-1]) < 1e-5:
+            out[l] = jv[-1]
+        elif x < lv[0] or x > lv[-1]:
+            out[l] = 0.0
+        else:
+            k = len(lv) - 2 if x == lv[-1] else int(np.searchsorted(lv, x, side="right")) - 1
+            out[l] = interpolate_loglog(x, lv[k], lv[k + 1], jv[k], jv[k + 1])
+    return normalized_sed(out * np.asarray(dlam, float))
+
+
 def powgrid(xmin, xmax, n, ratio):
     """NR::powgrid (Fundamentals/NR.hpp:189-204): n bins whose widths grow by `ratio` from first to last."""
     if abs(ratio - 1.0) < 1e-3:

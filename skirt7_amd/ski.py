@@ -11,7 +11,9 @@ comma-separated lists (DoubleListPropertyHandler.cpp:96-108), booleans true/t/ye
 declarations.
 
 Supported classes (anything else raises FatalError naming the class):
-  OligoMonteCarloSimulation (+ MonteCarloSimulation, Simulation properties), SIUnits,
+  OligoMonteCarloSimulation and PanMonteCarloSimulation (+ MonteCarloSimulation, Simulation properties),
+  LogWavelengthGrid, PanStellarComp with BlackBodySED or SunSED and BolLuminosityStellarCompNormalization,
+  PanDustSystem (one DustComp of Benchmark1DDustMix, GreyBodyDustEmissivity, AllCellsDustLib), SIUnits,
   ExtragalacticUnits, Random, OligoWavelengthGrid, StellarSystem, OligoStellarComp, ExpDiskGeometry,
   SersicGeometry, SpiralStructureGeometryDecorator, OligoDustSystem, CompDustDistribution (DustComps),
   SPHDustDistribution (particle file read from the ski file's directory; tree dust grids only),
@@ -30,7 +32,8 @@ import numpy as np
 from . import output
 from . import skirt as S
 from .capi import SG_BOOKKEEPING, SG_MIXED, SG_NEIGHBOR, SG_TOPDOWN
-from .output import MSUN, PC, FatalError, Units
+from . import panchromatic as P
+from .output import LSUN, MSUN, PC, FatalError, Units
 
 # TreeDustGrid::SearchMethod names (TreeDustGrid.hpp:47-50)
 _SEARCH = {"TopDown": SG_TOPDOWN, "Neighbor": SG_NEIGHBOR, "Bookkeeping": SG_BOOKKEEPING}
@@ -190,7 +193,8 @@ class Spec:
     def __init__(self, text, resources=None, inputs=None):
         self.inputs = inputs or "."
         sim = parse(text)
-        _require(sim, "OligoMonteCarloSimulation")
+        _require(sim, "OligoMonteCarloSimulation", "PanMonteCarloSimulation")
+        self.pan = sim.cls == "PanMonteCarloSimulation"
         uitem = sim.one("units", "ExtragalacticUnits")
         _require(uitem, "ExtragalacticUnits", "SIUnits")
         self.units = Units(uitem.cls, uitem.attrs.get("fluxOutputStyle", "Neutral"))
@@ -207,20 +211,32 @@ class Spec:
         _require(rnd, "Random")
         self.seed = val.integer(rnd, "seed", "4357")
 
-        # OligoWavelengthGrid: sorted wavelengths, bin widths 0.001 lambda (OligoWavelengthGrid.cpp:20-36)
-        wg = sim.one("wavelengthGrid", "OligoWavelengthGrid")
-        _require(wg, "OligoWavelengthGrid")
-        self.lambdas = np.sort(np.array(val.doubles(wg, "wavelengths", "wavelength")))
-        self.dlambdas = 0.001 * self.lambdas
+        if self.pan:
+            # LogWavelengthGrid (LogWavelengthGrid.cpp:20-31) with PanWavelengthGrid's bins
+            wg = sim.one("wavelengthGrid", "LogWavelengthGrid")
+            _require(wg, "LogWavelengthGrid")
+            self.lambdas, self.dlambdas = P.log_wavelength_grid(val.double(wg, "minWavelength", "wavelength"),
+                                                                val.double(wg, "maxWavelength", "wavelength"),
+                                                                val.integer(wg, "points", "25"))
+        else:
+            # OligoWavelengthGrid: sorted wavelengths, bin widths 0.001 lambda (OligoWavelengthGrid.cpp:20-36)
+            wg = sim.one("wavelengthGrid", "OligoWavelengthGrid")
+            _require(wg, "OligoWavelengthGrid")
+            self.lambdas = np.sort(np.array(val.doubles(wg, "wavelengths", "wavelength")))
+            self.dlambdas = 0.001 * self.lambdas
         nl = self.nlambda = len(self.lambdas)
 
-        # StellarSystem of OligoStellarComps: luminosities in solar units at each wavelength -> W per bin
+        # StellarSystem of OligoStellarComps (luminosities in solar units at each wavelength) or PanStellarComps
+        # (an SED and a bolometric luminosity): W per bin
         ss = sim.one("stellarSystem", "StellarSystem")
         _require(ss, "StellarSystem")
         self.emissionBias = val.double(ss, "emissionBias", None, "0.5")
         sed = None
         self.stellar = []
         for c in ss.many("components"):
+            if self.pan:
+                self.stellar.append((_geometry(c.one("geometry"), val), self._pan_luminosities(c, resources)))
+                continue
             _require(c, "OligoStellarComp")
             lum = val.doubles(c, "luminosities")
             if len(lum) != nl:
@@ -251,8 +267,35 @@ class Spec:
         ds = sim.one("dustSystem")
         self.dust = None if ds is None else self._dust(ds, nl)
 
+    def _pan_luminosities(self, c, resources):
+        """PanStellarComp::setupSelfAfter (PanStellarComp.cpp:30-50): the normalisation's total luminosity times
+        the normalised SED (BlackBodySED or SunSED) at each wavelength of the grid; W per bin."""
+        val = self.val
+        _require(c, "PanStellarComp")
+        sedi = c.one("sed")
+        if sedi is None:
+            raise FatalError("PanStellarComp requires an SED")
+        if sedi.cls == "BlackBodySED":
+            sed = P.blackbody_sed(self.lambdas, val.double(sedi, "temperature", "temperature"))
+        elif sedi.cls == "SunSED":
+            sed = P.sun_sed(self.lambdas, self.dlambdas,
+                            read_sun_sed(_resource(resources, os.path.join("SED", "Sun", "SunSED.dat"))))
+        else:
+            _require(sedi, "BlackBodySED", "SunSED")
+        norm = c.one("normalization")
+        if norm is None:
+            raise FatalError("PanStellarComp requires a normalization")
+        _require(norm, "BolLuminosityStellarCompNormalization")
+        # BolLuminosityStellarCompNormalization.cpp:24-28: solar units converted with Units::Lsun
+        L = val.double(norm, "luminosity")
+        if not L > 0:
+            raise FatalError("The bolometric luminosity should be positive")
+        return list(L * LSUN * sed)
+
     def _dust(self, ds, nl):
         val = self.val
+        if self.pan:
+            return self._pan_dust(ds, nl)
         _require(ds, "OligoDustSystem")
         dd = ds.one("dustDistribution", "CompDustDistribution")
         _require(dd, "CompDustDistribution", "SPHDustDistribution")
@@ -267,6 +310,59 @@ class Spec:
         d["sampleCount"] = val.integer(ds, "sampleCount", "100")
         d["writeMeanIntensity"] = val.boolean(ds, "writeMeanIntensity", "no")
         d["grid"] = self._grid(ds.one("dustGrid", "OctTreeDustGrid"))
+        return d
+
+    def _pan_dust(self, ds, nl):
+        """PanDustSystem (PanDustSystem.cpp:30-70) with one DustComp of Benchmark1DDustMix dust, the grey-body
+        emissivity and the all-cells library (or no dust emission), emission bias and boost, self-absorption."""
+        val = self.val
+        _require(ds, "PanDustSystem")
+        # PanDustSystem::setupSelfBefore runs before its components are set up (PanDustSystem.cpp:61-64)
+        if P.nearest_wavelength(self.lambdas, 0.55e-6) < 0:
+            raise FatalError("Wavelength range should include 0.55 micron for a panchromatic simulation with dust")
+        dd = ds.one("dustDistribution", "CompDustDistribution")
+        _require(dd, "CompDustDistribution")
+        comps = dd.many("components")
+        if len(comps) != 1:
+            raise FatalError("a panchromatic dust system of one dust component is supported")
+        dc = comps[0]
+        _require(dc, "DustComp")
+        d = dict(geometry=_geometry(dc.one("geometry"), val))
+        mix = dc.one("mix")
+        if mix is None:
+            raise FatalError("DustComp requires a mix")
+        _require(mix, "Benchmark1DDustMix")
+        d["panmix"] = P.Benchmark1DDustMix(self.lambdas, self.dlambdas)
+        norm = dc.one("normalization")
+        if norm is None:
+            raise FatalError("DustComp requires a normalization")
+        if norm.cls == "FaceOnDustCompNormalization":
+            ell = P.nearest_wavelength(self.lambdas, val.double(norm, "wavelength", "wavelength"))
+            if ell < 0:
+                raise FatalError("Optical properties are not defined for this wavelength")
+            d["scale"] = S.faceon_normalization(d["geometry"], val.double(norm, "opticalDepth"),
+                                                d["panmix"].kappaext[ell])
+        elif norm.cls == "DustMassDustCompNormalization":
+            d["scale"] = val.double(norm, "dustMass", "mass") / (PC * PC)
+        else:
+            _require(norm, "FaceOnDustCompNormalization", "DustMassDustCompNormalization")
+        emis = ds.one("dustEmissivity")
+        d["dustEmission"] = emis is not None
+        if emis is not None:
+            _require(emis, "GreyBodyDustEmissivity")
+            lib_ = ds.one("dustLib")
+            if lib_ is None:
+                raise FatalError("There should be a dust library when dust emission is turned on")
+            _require(lib_, "AllCellsDustLib")
+        d["emissionBias"] = val.double(ds, "emissionBias", None, "0.5")
+        d["emissionBoost"] = val.double(ds, "emissionBoost", None, "1")
+        d["selfAbsorption"] = d["dustEmission"] and val.boolean(ds, "selfAbsorption", "no")
+        d["cycles"] = val.integer(ds, "cycles", "0")
+        d["sampleCount"] = val.integer(ds, "sampleCount", "100")
+        d["grid"] = self._grid(ds.one("dustGrid", "OctTreeDustGrid"))
+        if d["grid"]["kind"] == "VoronoiDustGrid":
+            raise FatalError("panchromatic dust emission is supported on Cartesian and tree dust grids")
+        d["components"] = [dict(geometry=d["geometry"], scale=d["scale"])]
         return d
 
     def _sph(self, ds, dd, nl):
@@ -417,7 +513,10 @@ class Simulation:
             [S.FrameInstrument(f["distance"] / PC, f["inclination"], f["azimuth"], f["positionAngle"], f["pixelsX"],
                                f["pixelsY"], f["fovX"] / PC, f["fovY"] / PC, f["centerX"] / PC, f["centerY"] / PC)
              for f in sp.frames], sp.nlambda)
-        self.grid = self.dust = self.rho = self.sites = self.sph = None
+        self.grid = self.dust = self.rho = self.sites = self.sph = self.pan = None
+        if sp.pan and sp.dust is not None:
+            self._pan(sp)
+            return
         if sp.dust is not None and "sph" in sp.dust:
             self._sph_dust(sp.dust, sp.seed)
         elif sp.dust is not None:
@@ -440,6 +539,22 @@ class Simulation:
                                                minWeightReduction=sp.minWeightReduction,
                                                minScattEvents=sp.minScattEvents, scattBias=sp.scattBias,
                                                seed=sp.seed)
+
+    def _pan(self, sp):
+        """PanMonteCarloSimulation's objects: the grid and sampled densities as for an oligochromatic run, the
+        PanDustSystem (lengths in pc) and the simulation."""
+        d = sp.dust
+        self.grid = self._grid(d["grid"], [d["geometry"]], [d["scale"]], sp.seed)
+        self.rho = S.sample_density(self.grid, d["geometry"], d["scale"], d["sampleCount"], sp.seed)
+        self.mix = d["panmix"]
+        pdust = P.PanDustSystem(self.grid, self.rho, self.mix, dustEmission=d["dustEmission"],
+                                selfAbsorption=d["selfAbsorption"], cycles=d["cycles"],
+                                emissionBias=d["emissionBias"], emissionBoost=d["emissionBoost"], length_unit=PC)
+        self.dust = pdust.ds
+        self.pan = P.PanMonteCarloSimulation(self.stars, pdust, self.instr, packages=sp.packages,
+                                             minWeightReduction=sp.minWeightReduction,
+                                             minScattEvents=sp.minScattEvents, scattBias=sp.scattBias, seed=sp.seed)
+        self.sim = self.pan.sim
 
     def _sph_dust(self, d, seed):
         """The SPH particles in the transport's units (pc; masses in kg over pc^2, so that kappa rho ds is an
@@ -493,8 +608,12 @@ class Simulation:
         return S.VoronoiDustGrid(ext, self.sites)
 
     def run(self, precision=SG_MIXED):
-        """The stellar emission phase: every wavelength, Npp packets each."""
+        """The stellar emission phase: every wavelength, Npp packets each; for a panchromatic simulation with
+        dust, PanMonteCarloSimulation::runSelf (stellar emission, self-absorption cycles, dust emission)."""
         self.instr.clear()
+        if self.pan is not None:
+            self.pan.precision = precision
+            return self.pan.runSelf()
         return self.sim.runstellaremission(precision)
 
     def write(self, outdir=".", stamp=None):

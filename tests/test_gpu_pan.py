@@ -11,6 +11,7 @@
 import numpy as np
 import pytest
 
+from skirt7_amd import panchromatic as P
 from skirt7_amd import workloads
 from skirt7_amd.capi import SG_FP64, SG_MIXED
 from tests import oracle_lib as O
@@ -115,3 +116,40 @@ def test_whole_run_with_self_absorption():
     # the absorbed dust luminosity settles: the last cycles of the last stage change little
     last = [tot for (stage, cycle, tot, eps) in pan.log if stage == 2]
     assert abs(last[-1] / last[-2] - 1) < 0.05
+
+
+def test_ski_file_panchromatic_run(tmp_path):
+    """A PanMonteCarloSimulation ski file (LogWavelengthGrid, PanStellarComp with a black-body SED, PanDustSystem
+    of Benchmark1DDustMix dust with self-absorption) runs as the same objects built through the Python API:
+    the same frames at every wavelength (to the order of the tallies' additions), with the dust's far-infrared
+    emission in them."""
+    import math
+    from skirt7_amd import ski
+    from skirt7_amd import skirt as S
+    from skirt7_amd.output import LSUN, PC
+    from tests.test_ski_fits import SUN, pan_ski
+    d = tmp_path / "dat"
+    (d / "SED" / "Sun").mkdir(parents=True)
+    (d / "SED" / "Sun" / "SunSED.dat").write_text(SUN)
+    sim = ski.Simulation(ski.Spec(pan_ski(selfabs=True), str(d)))
+    sim.run()
+    got = [sim.instr.frame(i).copy() for i in range(2)]
+    lam, dl = P.log_wavelength_grid(1e-7, 1e-3, 20)
+    mix = P.Benchmark1DDustMix(lam, dl)
+    ext = np.array([-2e4, -2e4, -2e4, 2e4, 2e4, 2e4])
+    grid = S.CartesianDustGrid(ext, S.linmesh(-2e4, 2e4, 16), S.linmesh(-2e4, 2e4, 16), S.linmesh(-2e4, 2e4, 12))
+    dgeom = S.ExpDiskGeometry(6000.0, 300.0)
+    scale = S.faceon_normalization(dgeom, 1.0, mix.kappaext[P.nearest_wavelength(lam, 0.55e-6)])
+    rho = S.sample_density(grid, dgeom, scale, 20, 4357)
+    dust = P.PanDustSystem(grid, rho, mix, selfAbsorption=True, cycles=2, length_unit=PC)
+    stars = S.StellarSystem([S.OligoStellarComp(S.ExpDiskGeometry(4000.0, 400.0),
+                                                list(1e10 * LSUN * P.blackbody_sed(lam, 6000.0)))], emissionBias=0.5)
+    instr = S.InstrumentSystem([S.FrameInstrument(1e7, 0.0, 0.0, 0.0, 24, 24, 4e4, 4e4),
+                                S.FrameInstrument(1e7, math.radians(88.0), 0.0, 0.0, 24, 12, 4e4, 2e4)], 20)
+    pan = P.PanMonteCarloSimulation(stars, dust, instr, packages=2e4, seed=4357)
+    pan.runSelf()
+    for i in range(2):
+        # the same packets; the tallies' FP64 atomic additions come in a different order from run to run
+        np.testing.assert_allclose(got[i], instr.frame(i), rtol=1e-11, atol=0)
+    fir = int(np.argmin(np.abs(lam - 1e-4)))
+    assert got[0][fir].sum() > 0

@@ -433,3 +433,113 @@ def test_ski_sph_distribution(tmp_path, resources):
         '</CartesianDustGrid>')
     with pytest.raises(FatalError, match="tree dust grids"):
         ski.Spec(cart, resources, str(tmp_path))
+
+
+def pan_ski(emission=True, selfabs=False, sed='<BlackBodySED temperature="6000 K"/>', points=20, comps=1,
+            minw="0.1 micron"):
+    """A PanMonteCarloSimulation ski file: log wavelength grid, one PanStellarComp, a PanDustSystem of
+    Benchmark1DDustMix dust on a Cartesian grid (grey-body emissivity and all-cells library when emission)."""
+    emis = ('<dustEmissivity type="DustEmissivity"><GreyBodyDustEmissivity/></dustEmissivity>'
+            '<dustLib type="DustLib"><AllCellsDustLib/></dustLib>') if emission else ""
+    comp = """<DustComp>
+                    <geometry type="Geometry"><ExpDiskGeometry radialScale="6 kpc" axialScale="300 pc"/></geometry>
+                    <mix type="DustMix"><Benchmark1DDustMix/></mix>
+                    <normalization type="DustCompNormalization">
+                      <FaceOnDustCompNormalization wavelength="0.55 micron" opticalDepth="1"/>
+                    </normalization>
+                  </DustComp>"""
+    return """<?xml version="1.0" encoding="UTF-8"?>
+<skirt-simulation-hierarchy type="MonteCarloSimulation" format="6.1">
+  <PanMonteCarloSimulation packages="2e4" minWeightReduction="1e4" scattBias="0.5">
+    <random type="Random"><Random seed="4357"/></random>
+    <units type="Units"><ExtragalacticUnits fluxOutputStyle="Neutral"/></units>
+    <instrumentSystem type="InstrumentSystem">
+      <InstrumentSystem>
+        <instruments type="Instrument">
+          <FrameInstrument instrumentName="faceon" distance="10 Mpc" inclination="0 deg" azimuth="0 deg"
+             positionAngle="0 deg" fieldOfViewX="40 kpc" pixelsX="24" fieldOfViewY="40 kpc" pixelsY="24"/>
+          <FrameInstrument instrumentName="edgeon" distance="10 Mpc" inclination="88 deg" azimuth="0 deg"
+             positionAngle="0 deg" fieldOfViewX="40 kpc" pixelsX="24" fieldOfViewY="20 kpc" pixelsY="12"/>
+        </instruments>
+      </InstrumentSystem>
+    </instrumentSystem>
+    <wavelengthGrid type="PanWavelengthGrid"><LogWavelengthGrid minWavelength="%s" maxWavelength="1000 micron" points="%d"/></wavelengthGrid>
+    <stellarSystem type="StellarSystem">
+      <StellarSystem emissionBias="0.5">
+        <components type="StellarComp">
+          <PanStellarComp>
+            <geometry type="Geometry"><ExpDiskGeometry radialScale="4 kpc" axialScale="400 pc"/></geometry>
+            <sed type="StellarSED">%s</sed>
+            <normalization type="StellarCompNormalization"><BolLuminosityStellarCompNormalization luminosity="1e10"/></normalization>
+          </PanStellarComp>
+        </components>
+      </StellarSystem>
+    </stellarSystem>
+    <dustSystem type="PanDustSystem">
+      <PanDustSystem sampleCount="20" emissionBias="0.5" emissionBoost="1" selfAbsorption="%s" cycles="2">
+        <dustDistribution type="DustDistribution">
+          <CompDustDistribution>
+            <components type="DustComp">
+              %s
+            </components>
+          </CompDustDistribution>
+        </dustDistribution>
+        <dustGrid type="DustGrid">
+          <CartesianDustGrid minX="-20 kpc" maxX="20 kpc" minY="-20 kpc" maxY="20 kpc" minZ="-20 kpc" maxZ="20 kpc">
+            <meshX type="MoveableMesh"><LinMesh numBins="16"/></meshX>
+            <meshY type="MoveableMesh"><LinMesh numBins="16"/></meshY>
+            <meshZ type="MoveableMesh"><LinMesh numBins="12"/></meshZ>
+          </CartesianDustGrid>
+        </dustGrid>
+        %s
+      </PanDustSystem>
+    </dustSystem>
+  </PanMonteCarloSimulation>
+</skirt-simulation-hierarchy>
+""" % (minw, points, sed, "true" if selfabs else "false", "\n".join([comp] * comps), emis)
+
+
+def test_ski_panchromatic(resources):
+    import math
+    from skirt7_amd import panchromatic as P
+    from skirt7_amd import ski
+    from skirt7_amd.output import FatalError, LSUN
+    sp = ski.Spec(pan_ski(), resources)
+    assert sp.pan and sp.nlambda == 20
+    lam, dl = P.log_wavelength_grid(1e-7, 1e-3, 20)
+    assert np.allclose(sp.lambdas, lam, rtol=1e-14) and np.allclose(sp.dlambdas, dl, rtol=1e-14)
+    geom, Lw = sp.stellar[0]
+    Lw = np.array(Lw)
+    assert abs(Lw.sum() / (1e10 * LSUN) - 1) < 1e-12
+    # each bin holds the integral of the Planck function over the bin (quadrature), normalised
+    from scipy.integrate import quad
+    lo, hi = P.wavelength_bins(lam)
+    bb = np.array([quad(lambda x: float(P.planck(np.array([x]), 6000.0)[0]), a, b, epsrel=1e-10)[0]
+                   for a, b in zip(lo, hi)])
+    bb /= bb.sum()
+    main = bb > 1e-12
+    # (the 101-point trapezoid in log lambda: 1.3e-4 where a bin spans the Wien tail)
+    assert np.max(np.abs(Lw[main] / (1e10 * LSUN) / bb[main] - 1)) < 5e-4
+    d = sp.dust
+    assert d["dustEmission"] and not d["selfAbsorption"] and d["cycles"] == 2
+    ell = P.nearest_wavelength(lam, 0.55e-6)
+    assert abs(lam[ell] / 0.55e-6 - 1) < 0.5
+    # face-on normalisation with the Benchmark1D mix's extinction at the nearest grid wavelength
+    assert abs(d["scale"] - S_faceon(geom_d=d["geometry"], kappa=d["panmix"].kappaext[ell])) < 1e-12 * d["scale"]
+    # a solar SED from the resource file: normalised, zero only beyond the table
+    sun = ski.Spec(pan_ski(sed="<SunSED/>"), resources)
+    Ls = np.array(sun.stellar[0][1])
+    assert abs(Ls.sum() / (1e10 * LSUN) - 1) < 1e-12 and np.all(Ls >= 0)
+    # the reference's errors
+    with pytest.raises(FatalError, match="dust library"):
+        ski.Spec(pan_ski().replace('<dustLib type="DustLib"><AllCellsDustLib/></dustLib>', ""), resources)
+    with pytest.raises(FatalError, match="0.55 micron"):
+        ski.Spec(pan_ski(minw="1 micron"), resources)
+    with pytest.raises(FatalError, match="one dust component"):
+        ski.Spec(pan_ski(comps=2), resources)
+    assert not ski.Spec(pan_ski(emission=False), resources).dust["dustEmission"]
+
+
+def S_faceon(geom_d, kappa):
+    from skirt7_amd import skirt as S
+    return S.faceon_normalization(geom_d, 1.0, kappa)
