@@ -81,9 +81,9 @@ def parse():
     ap.add_argument("--pipeline", type=int, default=1, choices=(0, 1),
                     help="1: event and walk kernels (default); 0: single lane-pipeline kernel (the autotuner, "
                          "unless switched off or given --tuning, picks the faster for the mixed steps)")
-    ap.add_argument("--walk-order", type=int, default=0, choices=(0, 1),
-                    help="1: the walk kernel takes each pass's walks sorted by a coherence key; 0 (default): as "
-                         "the event kernel appended them")
+    ap.add_argument("--walk-order", type=int, default=0, choices=(0, 1, 2),
+                    help="1: the walk kernel takes each pass's walks sorted by a coherence key; 2: sorted and "
+                         "claimed by XCD segment; 0 (default): as the event kernel appended them")
     ap.add_argument("--no-equal-error", action="store_true",
                     help="skip the FP64 reference run with ten times the packets")
     ap.add_argument("--no-autotune", action="store_true",
@@ -320,7 +320,7 @@ def main():
     if a.copy_calibration > 0:
         copy_calibration(a.copy_calibration)
     S.set_pipeline(a.pipeline)
-    S.set_walk_order(bool(a.walk_order))
+    S.set_walk_order(a.walk_order)
     S.set_slot_groups(a.slot_groups)
     if a.packets is None:
         a.packets = 2e7 if a.workload == "spiral_octree" else 1e6

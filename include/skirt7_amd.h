@@ -261,8 +261,8 @@ int sg_set_slots(int64_t slots);
  * pipeline), then the block
  * size (64, 128, 256), keeping the fastest of each in turn, and leaves them set for later runs.
  * With choose_pipeline nonzero it first times both transport organisations of sg_set_pipeline with
- * their default parameters and keeps the faster; the event/walk pipeline also tries both walk orders
- * (sg_set_walk_order). best (may be null) receives {block, slots or waves per CU, compaction threshold,
+ * their default parameters and keeps the faster; the event/walk pipeline also tries the three walk
+ * orders (sg_set_walk_order). best (may be null) receives {block, slots or waves per CU, compaction threshold,
  * pipeline, walk order}; rate (may be null) the sample's packets per second with
  * them. The results of a run do not depend on these choices. The tallies the sample adds are
  * removed again (frames and absorption as before). */
@@ -277,11 +277,12 @@ int sg_set_batch_alone(uint64_t min_packets);
  * FP64 walks always use the tree's own method. */
 int sg_set_mixed_topdown(int descend);
 /* Order in which the walk kernel of the event/walk pipeline takes a pass's walks: 1 sorted by a
- * coherence key (peel-offs by instrument and pixel, other walks by start point), 0 (default) as
- * appended. The
- * order changes no packet history, only the order of the tally additions. No counterpart in the
- * reference, whose threads walk one packet at a time (MonteCarloSimulation.cpp:259-297). */
-int sg_set_walk_order(int sorted);
+ * coherence key (peel-offs by instrument and pixel, other walks by start point), 2 sorted and cut into
+ * eight contiguous segments, each workgroup taking walks from its XCD's segment first (so that walks near
+ * each other in the order share that XCD's L2), 0 (default) as appended. The order changes no packet
+ * history, only the order of the tally additions. No counterpart in the reference, whose threads walk
+ * one packet at a time (MonteCarloSimulation.cpp:259-297). */
+int sg_set_walk_order(int order);
 /* Slot groups of the event/walk pipeline: 2 (default) alternate on two streams, so that one group's
  * event pass and the drain of its walk pass overlap the other group's walks; 1 runs every pass on one
  * stream, one kernel at a time (each walk launch then has the device to itself, which is how its own

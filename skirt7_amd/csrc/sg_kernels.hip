@@ -31,7 +31,8 @@ static int64_t g_tune_slots = 0;   // in-flight packet slots of the event/walk p
 static int g_pipeline = 1;   // 0: persistent lane pipeline (k_mc); 1: event and walk kernels (k_events/k_walks)
 static uint64_t g_batch_alone = (uint64_t)1 << 21;   // batched members this large run alone (0: never)
 static int g_topdown_descent = 0;   // 1: mixed walks of TopDown trees descend from the root at every step
-static int g_walk_order = 0;        // 1: the walk kernel takes each pass's walks in coherence-key order
+static int g_walk_order = 0;        // 1: the walk kernel takes each pass's walks in coherence-key order; 2: so,
+                                    // claimed by XCD segment
 static int g_slot_groups = 2;       // slot groups of the event/walk pipeline (2: alternating on two streams)
 static uint64_t g_emit_batch = 0;   // packets emitted at once (0: each pipeline's default)
 
@@ -265,9 +266,13 @@ int sg_set_slot_groups(int groups)
     return 0;
 }
 
-int sg_set_walk_order(int sorted)
+int sg_set_walk_order(int order)
 {
-    g_walk_order = sorted ? 1 : 0;
+    if (order < 0 || order > 2) {
+        sg::set_error("walk order must be 0 (as appended), 1 (sorted) or 2 (sorted, claimed by XCD segment)");
+        return -1;
+    }
+    g_walk_order = order;
     return 0;
 }
 
@@ -1387,7 +1392,8 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
             pl.bins[g].alloc(2 * (size_t)kKeyBins * sizeof(uint32_t));
             SG_HIP(hipMemsetAsync(pl.bins[g].p, 0, kKeyBins * sizeof(uint32_t), st[g]));
         }
-        if (pl.prog[g].bytes < 4 * sizeof(unsigned long long)) pl.prog[g].alloc(4 * sizeof(unsigned long long));
+        // [1] queue tail, [3] requests claimed, [4..11] requests claimed per XCD segment
+        if (pl.prog[g].bytes < 12 * sizeof(unsigned long long)) pl.prog[g].alloc(12 * sizeof(unsigned long long));
         pc[g] = pl.prog[g].as<unsigned long long>();
         sl[g] = Slots<R>{pl.slots[g].as<R>(),
                          reinterpret_cast<uint32_t*>(pl.slots[g].as<unsigned char>() + (size_t)Pg[g] * SR_NR * sizeof(R)),
@@ -1467,10 +1473,10 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
                         hipLaunchKernelGGL(k_key_scatter<R>, dim3(g_cus * 8), dim3(256), 0, st[g], q[g], ab.error);
                         SG_HIP(hipGetLastError());
                     }
-                    SG_HIP(hipMemsetAsync(pc[g] + 3, 0, sizeof(unsigned long long), st[g]));
+                    SG_HIP(hipMemsetAsync(pc[g] + 3, 0, (1 + kXcdSegs) * sizeof(unsigned long long), st[g]));
                     lt.kbefore(st[g]);
                     hipLaunchKernelGGL((k_walks<W, ABS>), dim3(blocks), dim3(bs), shm, st[g], v, ab, q[g], sl[g], pc[g] + 3,
-                                       nlds);
+                                       nlds, g_walk_order == 2 ? kXcdSegs : 0);
                     SG_HIP(hipGetLastError());
                     lt.kafter(st[g]);
                     walk_launches++;
@@ -1840,7 +1846,7 @@ static void run_static(sg_sim* s, const typename W::View& v, const McArgs& a0, i
             hipLaunchKernelGGL(k_events<R>, dim3((P + ebs - 1) / ebs), dim3(ebs), eshm, st, a, sl, q);
             SG_HIP(hipGetLastError());
             fill_u32(pc + 3, 2, st);
-            hipLaunchKernelGGL((k_walks<W, ABS>), dim3(blocks), dim3(bs), shm, st, v, a, q, sl, pc + 3, nlds);
+            hipLaunchKernelGGL((k_walks<W, ABS>), dim3(blocks), dim3(bs), shm, st, v, a, q, sl, pc + 3, nlds, 0);
             SG_HIP(hipGetLastError());
         }
         SG_HIP(hipMemcpyAsync(g_split_host, a.next_packet, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -2001,11 +2007,14 @@ int sg_autotune(sg_sim* s, int ell, uint64_t first, uint64_t sample, int precisi
                 if (r > top) { top = r; slots = p; }
             }
             g_tune_slots = slots;
-            // the walk order: as appended, or sorted by coherence key (which pays where the walks' grid
-            // data does not fit the caches)
-            g_walk_order = 1 - order;
-            const double r = trial();
-            if (r > top) { top = r; order = g_walk_order; }
+            // the walk order: as appended, sorted by coherence key (which pays where the walks' grid data
+            // does not fit the caches), or sorted and claimed by XCD segment
+            for (int o : {0, 1, 2}) {
+                if (o == order) continue;
+                g_walk_order = o;
+                const double r = trial();
+                if (r > top) { top = r; order = o; }
+            }
             g_walk_order = order;
         } else {
             for (int w : {4, 8, 12, 16}) {

@@ -125,7 +125,6 @@ __device__ __forceinline__ uint32_t walk_key(const McArgs& a, const InstrDev* in
     return ((kind == WK_OWN ? 8u : 9u) << 12) | spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
 }
 
-
 // Appends one request per active lane (one atomic per wave); returns this lane's queue position.
 __device__ __forceinline__ uint64_t wave_append(unsigned long long* tail)
 {
@@ -479,6 +478,48 @@ __device__ __forceinline__ bool chunk_claim(unsigned long long* ctr, uint64_t nq
     return ok && want;
 }
 
+// Request claims by XCD segment: the requests [0, nq) of a pass, in walk order, are cut into kXcdSegs
+// contiguous segments; a wave takes chunks from the segment of its workgroup's XCD (workgroups are dispatched
+// to the XCDs in turn, so blockIdx.x mod 8 names the XCD) and moves on to the following segments once that one
+// is exhausted. Walks that are neighbours in the walk order (start points or pixels close together) thus meet
+// in the same XCD's L2. ctr holds one counter per segment. Lanes the wave's chunk does not serve stay idle
+// until the next claim; exhausted (wave-uniform) becomes true when every segment is exhausted.
+constexpr int kXcdSegs = 8;
+__device__ __forceinline__ bool seg_claim(unsigned long long* ctr, uint64_t nq, bool want, int& seg, int& tried,
+                                          uint64_t& cur, uint64_t& end, uint64_t& r, bool& exhausted)
+{
+    const uint64_t mask = __ballot(want);
+    const uint64_t k = (uint64_t)__popcll(mask);
+    if (k == 0 || exhausted) return false;
+    const int lane = __lane_id();
+    const uint64_t rank = (uint64_t)__popcll(mask & (lane ? (~0ull >> (64 - lane)) : 0ull));
+    if (end == cur) {
+        while (tried < kXcdSegs) {
+            const uint64_t s0 = nq * (uint64_t)seg / kXcdSegs, s1 = nq * (uint64_t)(seg + 1) / kXcdSegs;
+            unsigned long long b = 0;
+            if (lane == 0) b = atomicAdd(ctr + seg, (unsigned long long)kClaimChunk);
+            const uint32_t lo = __shfl((uint32_t)b, 0), hi = __shfl((uint32_t)(b >> 32), 0);
+            const uint64_t nb = s0 + (((uint64_t)hi << 32) | lo);
+            if (nb < s1) {
+                cur = nb;
+                end = min(nb + kClaimChunk, s1);
+                break;
+            }
+            seg = (seg + 1) % kXcdSegs;
+            tried++;
+        }
+        if (end == cur) {
+            exhausted = true;
+            return false;
+        }
+    }
+    const uint64_t take = min(k, end - cur);
+    const bool ok = rank < take;
+    if (ok) r = cur + rank;
+    cur += take;
+    return ok && want;
+}
+
 // Occupancy target of the walk kernel (waves per SIMD): five for the mixed tree walk, which fits
 // in 96 registers; three for the exact tree walk, whose FP64 state needs about 200 registers (two
 // waves do not spill but hide less of the root descents' latency, four spill more); four for the
@@ -498,7 +539,8 @@ template <> struct WalkWaves<VoroW<float>> { static constexpr int value = 3; };
 // Walk pass: persistent lanes over the requests [0, *q.tail).
 template <class W, bool ABS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WalkWaves<W>::value, 8))) k_walks(const typename W::View g, const McArgs a, WalkQueue<typename W::Real> q,
-                                              Slots<typename W::Real> sl, unsigned long long* claim, int nlds_req)
+                                              Slots<typename W::Real> sl, unsigned long long* claim, int nlds_req,
+                                              int xcd_segments)
 {
     using R = typename W::Real;
     __shared__ InstrDev s_instr[SG_MAX_INSTR];
@@ -520,6 +562,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WalkWa
     uint32_t idx = 0, inst = 0;
     uint32_t n_own = 0, n_peel = 0, n_re = 0;   // the lane's crossings by walk kind
     uint64_t ccur = 0, cend = 0;                 // the wave's chunk of requests
+    int seg = blockIdx.x % kXcdSegs, tried = 0;  // XCD segments: the wave's segment, segments left behind
+    bool exhausted = false;
     const int thr = a.compact;
     for (;;) {
         if (h.stage == ST_PEELW_DONE) {
@@ -539,8 +583,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WalkWa
         }
         const bool want = h.stage == ST_NEWPKT;
         uint64_t r = 0;
-        const bool got = chunk_claim(claim, nq, want, ccur, cend, r);
-        if (want) {
+        const bool got = xcd_segments ? seg_claim(claim + 1, nq, want, seg, tried, ccur, cend, r, exhausted)
+                                      : chunk_claim(claim, nq, want, ccur, cend, r);
+        if (want && !got && xcd_segments && !exhausted) {
+            // not served by this claim: the lane stays idle and claims again
+        } else if (want) {
             const WalkReq<R> w = got ? reqs[r] : WalkReq<R>{};
             const uint32_t kind = w.tag & 3u;
             const uint32_t wi = w.tag >> 2;

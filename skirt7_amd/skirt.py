@@ -802,10 +802,12 @@ def set_mixed_topdown(descend):
     check(lib().sg_set_mixed_topdown(1 if descend else 0))
 
 
-def set_walk_order(sorted_=False):
-    """Event/walk pipeline: True takes each pass's walks sorted by a coherence key (peel-offs by instrument
-    and pixel, other walks by start point), False (default) as the event kernel appended them."""
-    check(lib().sg_set_walk_order(1 if sorted_ else 0))
+def set_walk_order(order=0):
+    """Event/walk pipeline: 1 (or True) takes each pass's walks sorted by a coherence key (peel-offs by
+    instrument and pixel, other walks by start point), 2 sorted and claimed by XCD segment (each workgroup
+    first takes the walks of its XCD's eighth of the order), 0 (default, or False) as the event kernel
+    appended them."""
+    check(lib().sg_set_walk_order(int(order)))
 
 
 def set_emit_batch(packets=0):

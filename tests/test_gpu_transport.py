@@ -115,6 +115,37 @@ def test_split_and_lane_pipelines_agree(c0_small, which):
         S.set_pipeline(1)
 
 
+@pytest.mark.parametrize("which", ["octree", "voronoi"])
+def test_walk_orders_agree(which):
+    """The walk kernel taking a pass's walks as appended, sorted by coherence key, or sorted and claimed by
+    XCD segment runs the same packet histories: identical counters, images equal up to the order of the tally
+    additions (FP64 and mixed; a run large enough that passes hold many claim chunks per segment)."""
+    from skirt7_amd import skirt as S
+    if which == "octree":
+        w = workloads.spiral_octree(maxlevel=6, pixels=64, packages=200000, nsamples=4)
+    else:
+        w = workloads.spiral_voronoi(nsites=20000, nlambda=1, pixels=48, packages=100000)
+    n = int(w["packages"])
+    try:
+        S.set_pipeline(1)
+        for prec in (SG_FP64, SG_MIXED):
+            res = []
+            for order in (0, 1, 2):
+                S.set_walk_order(order)
+                w["instr"].clear()
+                c = w["sim"].run_packets(0, 0, n, n, prec)
+                res.append((c, frames(w)))
+            (c0, f0) = res[0]
+            for c, f in res[1:]:
+                for k in EXACT:
+                    assert c[k] == c0[k], (prec, k, c[k], c0[k])
+                for a, b in zip(f, f0):
+                    np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-300)
+            assert c0["peel_crossings"] > 1e5
+    finally:
+        S.set_walk_order(0)
+
+
 @pytest.fixture(scope="module")
 def c4_small():
     return workloads.lognormal_kdtree(nvox=16, maxmassfraction=2e-4, pixels=32, packages=6000, medium="voxels")
