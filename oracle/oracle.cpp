@@ -5,19 +5,27 @@
 // never does. Every routine cites the SKIRT7 file:line it restates (paths relative to the
 // SKIRT7 source tree). Parity status of each routine is listed at the end of this header.
 //
-// Deliberate scope: one dust component (the Ncomp==1 branches of MonteCarloSimulation.cpp),
-// unpolarised Henyey-Greenstein scattering, distant frame instruments, isotropic emission
-// from exponential-disc / Sersic geometries with the optional spiral perturbation.
+// Deliberate scope: dust systems of one or several components, unpolarised Henyey-Greenstein
+// scattering, distant frame instruments, isotropic emission from exponential-disc / Sersic
+// geometries with the optional spiral perturbation, and panchromatic dust emission (emission
+// spectra with a grey-body emissivity, dust packet launch). The SPH medium and its tree
+// subdivision are restated in oracle/sph.cpp.
 //
-// Parity pins (tests/test_oracle_*.py):
+// Parity pins (tests/test_oracle_pins.py, test_pan_pins.py, test_fit_pins.py, test_sph_pins.py):
 //   - NR locate/locate_clip/locate_fail/lingrid/interpolate: against the reference's own
 //     Fundamentals/NR.hpp compiled into oracle/_ref (recipe oracle/ref/Makefile);
 //   - MT19937 with 69069 seeding: against numpy's MT19937 with the injected state;
 //   - Philox4x32-10: against the Random123 known-answer vectors;
 //   - grid walks: closed-form rays, brute-force midpoint classification, chord-length sums;
 //   - HG sampling/value, exponcutoff, LambertW_{-1}, Sersic table: closed forms;
-//   - escape/absorption: energy conservation identity.
-//   Parity unpinned: none of the reference's golden outputs exist (the tree ships no tests).
+//   - emission radii and heights, Sersic radii, spiral azimuths, the component choice and bias
+//     weight, the detector projection: distribution tests against closed forms and quadratures;
+//   - escape/absorption: energy conservation identity; several components: closed forms of the
+//     combined extinction, the albedo mix and the per-component phase weights;
+//   - emission spectra and dust launch: Stefan-Boltzmann, equilibrium temperatures of Planck
+//     radiation fields, flux conservation in thin media.
+//   The reference ships no golden outputs for these routines (its tree holds no tests), so the
+//   pins are closed forms, brute force and the reference's own compiled numerics.
 
 #include "oracle.hpp"
 
