@@ -11,6 +11,8 @@
 // recompute of the wall distances when the smallest two are within 1e-5 of each other or the
 // smallest is within 1e-5 of zero (relative to the second).
 #pragma once
+#include <type_traits>
+
 #include "sg_common.hpp"
 
 namespace sg {
@@ -1133,6 +1135,32 @@ __device__ int voro_locate(const VoroView& g, double x, double y, double z, int 
 
 // VoronoiMesh::path (VoronoiMesh.cpp:778-873): the bisector-plane tests stay FP64 in both
 // precision modes; Real only sets the type of the emitted segment lengths.
+// The distance along the ray from (x, y, z) to the bisector plane of the sites p (the cell's) and q (a
+// neighbour's), when the plane lies ahead (n.k > 0, n = q - p), else 0 (VoronoiMesh.cpp:806-858).
+__device__ __forceinline__ double voro_site_si(double x, double y, double z, double kx, double ky, double kz, double px,
+                                               double py, double pz, double qx, double qy, double qz)
+{
+    const double nx = qx - px, ny = qy - py, nz = qz - pz;
+    const double ndotk = nx * kx + ny * ky + nz * kz;
+    if (!(ndotk > 0)) return 0;
+    const double mx = 0.5 * (qx + px), my = 0.5 * (qy + py), mz = 0.5 * (qz + pz);
+    return (nx * (mx - x) + ny * (my - y) + nz * (mz - z)) / ndotk;
+}
+// The distance to the box wall of entry code mi (-1..-6: -x +x -y +y -z +z).
+__device__ __forceinline__ double voro_wall_si(const VoroView& g, double x, double y, double z, double kx, double ky,
+                                               double kz, int mi)
+{
+    switch (mi) {
+    case -1: return (g.ext[0] - x) / kx;
+    case -2: return (g.ext[3] - x) / kx;
+    case -3: return (g.ext[1] - y) / ky;
+    case -4: return (g.ext[4] - y) / ky;
+    case -5: return (g.ext[2] - z) / kz;
+    case -6: return (g.ext[5] - z) / kz;
+    default: return 0;
+    }
+}
+
 template <typename R> struct VoroW {
     using Real = R;
     using View = VoroView;
@@ -1180,25 +1208,8 @@ template <typename R> struct VoroW {
                                           int& mq) const
     {
         const int mi = pi.id;
-        double si = 0;
-        if (mi >= 0) {
-            double nx = pi.x - pr.x, ny = pi.y - pr.y, nz = pi.z - pr.z;
-            double ndotk = nx * kx + ny * ky + nz * kz;
-            if (ndotk > 0) {
-                double px = 0.5 * (pi.x + pr.x), py = 0.5 * (pi.y + pr.y), pz = 0.5 * (pi.z + pr.z);
-                si = (nx * (px - x) + ny * (py - y) + nz * (pz - z)) / ndotk;
-            }
-        } else {
-            switch (mi) {
-            case -1: si = (g.ext[0] - x) / kx; break;
-            case -2: si = (g.ext[3] - x) / kx; break;
-            case -3: si = (g.ext[1] - y) / ky; break;
-            case -4: si = (g.ext[4] - y) / ky; break;
-            case -5: si = (g.ext[2] - z) / kz; break;
-            case -6: si = (g.ext[5] - z) / kz; break;
-            default: si = 0;
-            }
-        }
+        const double si = mi >= 0 ? voro_site_si(x, y, z, kx, ky, kz, pr.x, pr.y, pr.z, pi.x, pi.y, pi.z)
+                                  : voro_wall_si(g, x, y, z, kx, ky, kz, mi);
         if (si > 0 && si < sq) { sq = si; mq = mi; }
     }
     __device__ __forceinline__ bool next(const View& g, const int32_t*, int, int& mo, R& ds, R& rh)

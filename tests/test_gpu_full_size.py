@@ -16,7 +16,7 @@ from skirt7_amd import workloads
 from skirt7_amd.capi import SG_FP64, SG_MIXED
 from tests import oracle_lib as O
 from tests.helpers import random_rays
-from tests.test_gpu_paths import assert_paths_equal
+from tests.test_gpu_paths import assert_mixed_voronoi_exact, assert_paths_equal
 from tests.test_gpu_transport import frames, oracle_frames
 
 pytestmark = pytest.mark.gpu
@@ -101,3 +101,6 @@ def test_voronoi_full_size_walks_and_location():
     rays = random_rays(300, 42, g.extent, special=False)
     rays[:150, :3] = g.sites[rng.integers(0, len(g.sites), 150)]
     assert_paths_equal(g, sc, rays, maxseg=16384)
+    more = random_rays(3000, 43, g.extent, special=False)
+    more[:1500, :3] = g.sites[rng.integers(0, len(g.sites), 1500)]
+    assert_mixed_voronoi_exact(g, more, maxseg=16384)

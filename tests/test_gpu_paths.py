@@ -27,6 +27,19 @@ def assert_paths_equal(grid, scene, rays, maxseg=4096):
     return counts
 
 
+def assert_mixed_voronoi_exact(grid, rays, maxseg=4096):
+    """The mixed-precision Voronoi walk evaluates the bisector planes in FP64 and rounds only what it returns
+    (lengths and densities) to FP32: its cells, and its lengths rounded to FP32, are the FP64 walk's."""
+    c64, m64, d64 = grid.paths(rays, maxseg, SG_FP64)
+    c32, m32, d32 = grid.paths(rays, maxseg, SG_MIXED)
+    assert np.array_equal(c32, c64)
+    for i in range(len(rays)):
+        n = c64[i]
+        assert np.array_equal(m32[i, :n], m64[i, :n]), ("cells", i, rays[i].tolist())
+        assert np.array_equal(d32[i, :n], d64[i, :n].astype(np.float32).astype(np.float64)), ("lengths", i)
+    return c64
+
+
 def test_cartesian_uniform_bitexact():
     g = S.CartesianDustGrid.uniform(EXT, 50, 50, 50)
     sc = O.Scene().cartesian(g.extent, g.xv, g.yv, g.zv)
@@ -84,6 +97,31 @@ def test_voronoi_bitexact():
     g = S.VoronoiDustGrid(EXT, sites, off, ids)
     sc = O.Scene().voronoi(EXT, sites, off, ids)
     assert_paths_equal(g, sc, random_rays(1500, 6))
+    # 125 entries a cell: past the 64 the FP32 pass marks
+    assert_mixed_voronoi_exact(g, random_rays(1500, 6))
+
+
+@pytest.mark.parametrize("shift,unit", [(0.0, 1.0), (1.0e5, 1.0), (0.0, 3.0857e16), (-7.0, 1e-3)])
+def test_voronoi_lattice_ties_mixed(shift, unit):
+    """Sites on a regular 5x5x5 lattice (every bisector plane axis-aligned or diagonal, many equidistant
+    planes) with rays along the axes and diagonals, through lattice points, edges and corners: planes
+    parallel to the ray and equidistant planes (taken first in list order), with the mixed walk the FP64
+    walk bit for bit, also in a box far from the origin and at the scales of metres and millimetres."""
+    ext = EXT * unit + shift
+    ax = (np.arange(5) * 8.0 - 16.0) * unit + shift
+    sites = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    off, ids = voronoi_superset(sites)
+    g = S.VoronoiDustGrid(ext, sites, off, ids)
+    rays = random_rays(1200, 31, ext)
+    rng = np.random.default_rng(32)
+    # starts on lattice points, on cell faces and edges (half-way between sites)
+    pick = rng.integers(0, 5, (600, 3))
+    half = rng.integers(0, 2, (600, 3)) * 4.0 * unit
+    rays[:600, :3] = ax[pick] + half
+    if shift == 0.0 and unit == 1.0:
+        sc = O.Scene().voronoi(ext, sites, off, ids)
+        assert_paths_equal(g, sc, rays[::4])
+    assert_mixed_voronoi_exact(g, rays)
 
 
 def test_voronoi_clustered_sites():
@@ -104,6 +142,7 @@ def test_voronoi_clustered_sites():
     rays = random_rays(600, 24)
     rays[:300, :3] = workloads.sample_sites(dgeom, 300, EXT, seed=25)
     assert_paths_equal(g, sc, rays)
+    assert_mixed_voronoi_exact(g, rays)
 
 
 def test_whichcell_all_grids():
