@@ -1161,23 +1161,37 @@ __device__ __forceinline__ double voro_wall_si(const VoroView& g, double x, doub
     }
 }
 
+#ifndef SG_VORO_GROUP
+#define SG_VORO_GROUP 16
+#endif
+// Voronoi walk (VoronoiMesh.cpp:778-873) as single steps: a step tests one group of entries (16, loaded
+// together) of the current cell's list in list order (the reference's loop, cut into pieces), and the
+// step that ends the list moves the position and returns the cell's segment; the other steps return no
+// segment. A wave then runs as many plane tests as its lanes need group by group, instead of running
+// every cell's loop to the length of the wave's longest list. The cell's list offset is loaded as soon
+// as the cell is entered.
 template <typename R> struct VoroW {
     using Real = R;
     using View = VoroView;
+    static constexpr int NO_INDEX = -99;
+    static constexpr int kGroup = SG_VORO_GROUP;   // entries tested per step
     double x, y, z, kx, ky, kz;
-    int mr, lm;
+    double sqc;              // the current cell's nearest plane ahead so far, and its entry (mqc)
+    int mr, lm, mqc;
+    int hb, tcur, tend;      // the current cell's header record, next entry to test, end of its entries
     Entry en;
-    bool done;
+    bool done, segout;
     uint32_t steps, nrecompute, nstuck;
-    static constexpr int kParkWords = 25;
+    static constexpr int kParkWords = 31;
     __device__ __forceinline__ static int cell_of(const View&, int n) { return n; }
     static constexpr bool kHint = true;
     __device__ __forceinline__ int entries() const { return en.ne; }
-    static constexpr bool kStep = false;   // next() only
+    static constexpr bool kStep = true;
     template <class P> __device__ __forceinline__ void park(P& p)
     {
-        p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz);
-        p.io(mr); p.io(lm); p.io(en.ne); p.io(en.e0); p.io(en.e1); p.io(en.e2);
+        p.io(x); p.io(y); p.io(z); p.io(kx); p.io(ky); p.io(kz); p.io(sqc);
+        p.io(mr); p.io(lm); p.io(mqc); p.io(hb); p.io(tcur); p.io(tend);
+        p.io(en.ne); p.io(en.e0); p.io(en.e1); p.io(en.e2);
         p.io(done); p.io(steps); p.io(nrecompute); p.io(nstuck);
     }
 
@@ -1189,6 +1203,13 @@ template <typename R> struct VoroW {
     // The cell of the last segment returned (-1 before the first one): after a walk that stopped
     // inside a segment, the cell holding the stopping point.
     __device__ __forceinline__ int hint() const { return lm; }
+    // Enters cell mr: its list's place, and no plane found yet.
+    __device__ __forceinline__ void enter(const View& g)
+    {
+        const int2 be = g.off[mr];
+        hb = be.x; tcur = be.x + 1; tend = be.x + 1 + be.y;
+        sqc = DBL_MAX; mqc = NO_INDEX;
+    }
     // init with a candidate cell hn (or -1): the greedy location starts there; the cell found is the
     // nearest site whatever the start
     __device__ __forceinline__ void init_hinted(const View& g, const int32_t*, int, double rx, double ry, double rz,
@@ -1196,11 +1217,13 @@ template <typename R> struct VoroW {
     {
         kx = dx; ky = dy; kz = dz;
         steps = 0; done = false; lm = -1;
+        hb = 0; tcur = 0; tend = 0; sqc = DBL_MAX; mqc = NO_INDEX;
         if (!en.move_inside(g.ext, g.eps, rx, ry, rz, kx, ky, kz)) { en.ne = 0; done = true; return; }
         x = rx; y = ry; z = rz;
         // a start point moved onto the box from outside is not where the candidate was found
         mr = voro_locate(g, x, y, z, en.ne ? -1 : hn);
-        if (mr < 0) { en.ne = 0; done = true; }
+        if (mr < 0) { en.ne = 0; done = true; return; }
+        enter(g);
     }
     // The distance along the ray to the bisector plane (or wall) of one neighbour entry of the cell with
     // site pr, taken when it lies ahead and is the nearest so far (VoronoiMesh.cpp:806-858).
@@ -1212,53 +1235,54 @@ template <typename R> struct VoroW {
                                   : voro_wall_si(g, x, y, z, kx, ky, kz, mi);
         if (si > 0 && si < sq) { sq = si; mq = mi; }
     }
-    __device__ __forceinline__ bool next(const View& g, const int32_t*, int, int& mo, R& ds, R& rh)
+    // One step: an entry segment; or kGroup more entries of the cell's list (no segment, segout false);
+    // or, when they end the list, the cell's segment (the position moves onto the next cell). False at
+    // the end of the path.
+    __device__ __forceinline__ bool step(const View& g, const int32_t*, int, int& mo, R& ds, R& rh)
     {
+        segout = true;
         if (en.ne) { ds = (R)en.pop(); mo = -1; rh = (R)0; return true; }
-        while (!done) {
-            // the cell's header record (its site and density), then its entries, contiguous
-            const int2 be = g.off[mr];
-            const VoroSite pr = *reinterpret_cast<const VoroSite*>(g.nbr + be.x);
-            double sq = DBL_MAX;
-            const int NO_INDEX = -99;
-            int mq = NO_INDEX;
-            const int t0 = be.x + 1, t1 = be.x + 1 + be.y;
-            // entries in blocks of eight loaded together, so that a cell's list (15.5 entries on
-            // average) takes about two memory round trips (every cell has at least one entry, checked
-            // at creation; past the end the block repeats the last entry, whose second test changes
-            // nothing: the nearest plane ahead is taken on a strict "<")
-            for (int t = t0; t < t1; t += 8) {
-                const int last = t1 - 1;
-                const VoroNbr q0 = g.nbr[t], q1 = g.nbr[min(t + 1, last)];
-                const VoroNbr q2 = g.nbr[min(t + 2, last)], q3 = g.nbr[min(t + 3, last)];
-                const VoroNbr q4 = g.nbr[min(t + 4, last)], q5 = g.nbr[min(t + 5, last)];
-                const VoroNbr q6 = g.nbr[min(t + 6, last)], q7 = g.nbr[min(t + 7, last)];
-                plane(g, pr, q0, sq, mq);
-                plane(g, pr, q1, sq, mq);
-                plane(g, pr, q2, sq, mq);
-                plane(g, pr, q3, sq, mq);
-                plane(g, pr, q4, sq, mq);
-                plane(g, pr, q5, sq, mq);
-                plane(g, pr, q6, sq, mq);
-                plane(g, pr, q7, sq, mq);
-            }
-            if (++steps > SG_MAX_STEPS) { done = true; return false; }
-            if (mq == NO_INDEX) {
-                nstuck++;
-                x += kx * g.eps; y += ky * g.eps; z += kz * g.eps;
-                mr = voro_locate(g, x, y, z, mr);
-                if (mr < 0) done = true;
-                continue;
-            }
-            int segm = mr;
-            double segrho = pr.rho;
-            x += (sq + g.eps) * kx; y += (sq + g.eps) * ky; z += (sq + g.eps) * kz;
-            mr = mq;
-            if (mr < 0) done = true;
-            mo = segm; ds = (R)sq; rh = (R)segrho;
-            lm = segm;
+        if (done) return false;
+        // the cell's header record (its site and density), then its entries, contiguous; every cell has at
+        // least one entry (checked at creation), and past the end a group repeats the last entry, whose
+        // second test changes nothing: the nearest plane ahead is taken on a strict "<"
+        const VoroSite pr = *reinterpret_cast<const VoroSite*>(g.nbr + hb);
+        const int t = tcur, last = tend - 1;
+        VoroNbr q[kGroup];
+#pragma unroll
+        for (int i = 0; i < kGroup; i++) q[i] = g.nbr[min(t + i, last)];
+        double sq = sqc;
+        int mq = mqc;
+#pragma unroll
+        for (int i = 0; i < kGroup; i++) plane(g, pr, q[i], sq, mq);
+        sqc = sq; mqc = mq;
+        tcur = t + kGroup;
+        if (tcur < tend) { segout = false; mo = -1; ds = (R)0; rh = (R)0; return true; }
+        if (++steps > SG_MAX_STEPS) { done = true; return false; }
+        if (mq == NO_INDEX) {
+            // no plane ahead: move on by eps and locate the point again
+            nstuck++;
+            x += kx * g.eps; y += ky * g.eps; z += kz * g.eps;
+            mr = voro_locate(g, x, y, z, mr);
+            if (mr < 0) { done = true; return false; }
+            enter(g);
+            segout = false; mo = -1; ds = (R)0; rh = (R)0;
             return true;
         }
+        const int segm = mr;
+        x += (sq + g.eps) * kx; y += (sq + g.eps) * ky; z += (sq + g.eps) * kz;
+        mr = mq;
+        if (mr < 0) done = true;
+        else enter(g);
+        mo = segm; ds = (R)sq; rh = (R)pr.rho;
+        lm = segm;
+        return true;
+    }
+    // The next segment (the reference's DustGridPath, every segment the cell loop returns).
+    __device__ __forceinline__ bool next(const View& g, const int32_t* lds, int nlds, int& mo, R& ds, R& rh)
+    {
+        while (step(g, lds, nlds, mo, ds, rh))
+            if (segout) return true;
         return false;
     }
 };
