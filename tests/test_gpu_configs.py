@@ -1,12 +1,12 @@
-"""The survey's configurations as specified, beyond the headline run.
+"""The benchmark workloads at their specified sizes, beyond the headline run.
 
-- config1 at its other face-on optical depths (0.5 and 5) on the full 718509-leaf octree: FP64 packets
+- The spiral-galaxy octree workload at its other face-on optical depths (0.5 and 5) on the full 718509-leaf octree: FP64 packets
   followed packet by packet against the oracle, and mixed precision against FP64 on the same streams.
-- config0 at its real size (Cartesian 50^3, three 256x256 frames, 1e6 packets): every packet against the
+- The exponential-disc Cartesian workload at its real size (50^3, three 256x256 frames, 1e6 packets): every packet against the
   oracle on the same Philox streams.
-- config2's 50 wavelengths on the full 1e6-site Voronoi mesh: the absorbed-luminosity table (1e6 x 50) and
+- The Voronoi workload's 50 wavelengths on the full 1e6-site mesh: the absorbed-luminosity table (1e6 x 50) and
   the frames of FP64 runs at every wavelength against the oracle packet by packet.
-- config4's full kd-tree (1.61e7 leaves): FP64 walks bit for bit against the oracle, and the mixed rope walk
+- The particle kd-tree workload's full tree (1.61e7 leaves): FP64 walks bit for bit against the oracle, and the mixed rope walk
   against them.
 - The Bookkeeping search method (octrees): FP64 packets against the oracle, mixed against FP64.
 - A packet range emitted in several batches equals the same range emitted at once (both pipelines, both
@@ -71,12 +71,12 @@ def test_octree_other_optical_depths(tau):
         assert np.linalg.norm(a - b) / np.linalg.norm(b) < 0.03
 
 
-def test_cartesian_config0_full_size_against_oracle():
+def test_cartesian_disc_full_size_against_oracle():
     w = workloads.disc_cartesian(packages=1e6)
     assert w["grid"].numCells() == 125000
     assert [i.Nx for i in w["instr"].instruments] == [256, 256, 256]
     sc = O.scene_from_workload(w)
-    print("config0: 1e6 packets FP64 against the oracle", flush=True)
+    print("Cartesian 50^3 disc: 1e6 packets FP64 against the oracle", flush=True)
     assert_matches_oracle(w, sc, 0, 1000000, 1000000, match=0.98)
 
 
@@ -87,7 +87,7 @@ def test_voronoi_fifty_wavelengths_absorption_table():
     n = 300
     w["dust"].clear()
     w["instr"].clear()
-    print("config2: %d packets at each of 50 wavelengths, FP64" % n, flush=True)
+    print("Voronoi 1e6 sites: %d packets at each of 50 wavelengths, FP64" % n, flush=True)
     for ell in range(50):
         w["sim"].run_packets(ell, 0, n, n, SG_FP64)
     labs = w["dust"].Labs()
@@ -112,7 +112,7 @@ def test_kdtree_full_size_walks():
     assert 1.6e7 < g.numCells() < 1.62e7
     sc = O.scene_from_workload(w)
     rays = random_rays(300, 51, g.extent)
-    print("config4: 300 FP64 walks on %d leaves" % g.numCells(), flush=True)
+    print("particle kd-tree: 300 FP64 walks on %d leaves" % g.numCells(), flush=True)
     counts = assert_paths_equal(g, sc, rays, maxseg=32768)
     assert counts.max() > 200
     # the mixed rope walk (TopDown tree: the mixed walk follows the ropes) against the exact walks: the same
