@@ -520,15 +520,19 @@ __device__ __forceinline__ bool seg_claim(unsigned long long* ctr, uint64_t nq, 
     return ok && want;
 }
 
-// Occupancy target of the walk kernel (waves per SIMD): five for the mixed tree walk, which fits
-// in 96 registers; three for the exact tree walk, whose FP64 state needs about 200 registers (two
+// Occupancy target of the walk kernel (waves per SIMD): four for the mixed tree walk, which then
+// takes 108 registers and spills nothing (five fit in 96 registers with 44 bytes of spills per lane
+// and ran 1-2% slower, six slower still); three for the exact tree walk, whose FP64 state needs about 200 registers (two
 // waves do not spill but hide less of the root descents' latency, four spill more); four for the
 // others.
 #ifndef SG_WALK_WAVES_PER_EU
 #define SG_WALK_WAVES_PER_EU 4
 #endif
 template <class W> struct WalkWaves { static constexpr int value = SG_WALK_WAVES_PER_EU; };
-template <> struct WalkWaves<TreeW32> { static constexpr int value = 5; };
+#ifndef SG_W32_WAVES
+#define SG_W32_WAVES 4
+#endif
+template <> struct WalkWaves<TreeW32> { static constexpr int value = SG_W32_WAVES; };
 #ifndef SG_W64_WAVES
 #define SG_W64_WAVES 3
 #endif
