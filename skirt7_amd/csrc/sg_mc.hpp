@@ -687,6 +687,20 @@ __device__ __forceinline__ void add_wave(unsigned long long* dst, unsigned long 
 #ifndef SG_MC_WAVES_PER_EU
 #define SG_MC_WAVES_PER_EU 2
 #endif
+#ifndef SG_MC_CART32_WAVES
+#define SG_MC_CART32_WAVES 3
+#endif
+#ifndef SG_MC_TREE32_WAVES
+#define SG_MC_TREE32_WAVES 4
+#endif
+// Per walker, as measured with the lane pipeline: three waves per SIMD for the mixed Cartesian walk
+// (32% faster than two on the Cartesian disc workload, four 17% slower than three); four for the mixed
+// tree walk (on the spiral octree 31% faster than two despite the spills, five collapses); two for the
+// others, the Voronoi walk above all (its FP64 state spills at three, 26% slower; at one it spills
+// nothing but hides too little latency, 29% slower).
+template <class W> struct McWaves { static constexpr int value = SG_MC_WAVES_PER_EU; };
+template <> struct McWaves<CartW32> { static constexpr int value = SG_MC_CART32_WAVES; };
+template <> struct McWaves<TreeW32> { static constexpr int value = SG_MC_TREE32_WAVES; };
 #ifndef SG_PARK
 #define SG_PARK 0
 #endif
@@ -746,7 +760,7 @@ __device__ __forceinline__ void drain(const typename W::View& g, const McArgs& a
 // Persistent transport kernel. Dynamic LDS: the hot-state parking area (hot_words<W>() per lane),
 // then (optionally) the top tree nodes. Cold lane state lives in device memory at cold_r/cold_u.
 template <class W, bool ABS, bool MULTI = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_MC_WAVES_PER_EU, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(McWaves<W>::value, 8)))
 k_mc(const typename W::View g, const McArgs a, int nlds_req, typename W::Real* cold_r, uint32_t* cold_u,
      int nslots, const typename W::View* gdev, const McArgs* adev)
 {
@@ -775,7 +789,7 @@ k_mc(const typename W::View g, const McArgs a, int nlds_req, typename W::Real* c
 // counter has reached its count is finished for good, and every visit to an unfinished range
 // advances its counter, so the scan ends.
 template <class W, bool ABS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_MC_WAVES_PER_EU, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(McWaves<W>::value, 8)))
 k_mc_batch(const typename W::View* gtab, const McArgs* atab, int nranges, typename W::Real* cold_r,
            uint32_t* cold_u, int nslots)
 {
