@@ -1424,8 +1424,8 @@ static void run_split(sg_sim* s, const typename W::View& v, const McArgs& a, uin
     blocks = std::max(1, per) * g_cus;
     // event workgroups stage up to blockDim * (ni + 1) requests in LDS
     int ebs = 256;
-    while (ebs > 64 && staged_bytes<R>(ebs * (a.ni + 1)) > 64 * 1024) ebs /= 2;
-    const size_t eshm = staged_bytes<R>(ebs * (a.ni + 1));
+    while (ebs > 64 && staged_bytes<R>(ebs, a.ni) > 64 * 1024) ebs /= 2;
+    const size_t eshm = staged_bytes<R>(ebs, a.ni);
     const uint64_t batch = std::min<uint64_t>(count, g_emit_batch ? g_emit_batch : kSplitEmit);
     const size_t rsz = sizeof(R);
     DevBuf& emitted = pool().emitted;
@@ -1832,8 +1832,8 @@ static void run_static(sg_sim* s, const typename W::View& v, const McArgs& a0, i
     SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walks<W, ABS>, bs, shm));
     blocks = std::max(1, per) * g_cus;
     int ebs = 256;
-    while (ebs > 64 && staged_bytes<R>(ebs * (a.ni + 1)) > 64 * 1024) ebs /= 2;
-    const size_t eshm = staged_bytes<R>(ebs * (a.ni + 1));
+    while (ebs > 64 && staged_bytes<R>(ebs, a.ni) > 64 * 1024) ebs /= 2;
+    const size_t eshm = staged_bytes<R>(ebs, a.ni);
     if (a.count > 0) {
         const int emb = (int)std::min<uint64_t>((a.count + 255) / 256, (uint64_t)g_cus * 16);
         hipLaunchKernelGGL(k_emit<R>, dim3(emb), dim3(256), 0, st, a, static_cast<R*>(const_cast<void*>(a.emitted)),

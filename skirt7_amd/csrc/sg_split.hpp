@@ -139,35 +139,78 @@ __device__ __forceinline__ uint64_t wave_append(unsigned long long* tail)
     return base + (uint64_t)__popcll(below);
 }
 
-// Walk requests of one workgroup staged in LDS (structure of arrays, cap entries); the workgroup
-// appends them to the walk queue as one block with a single global atomic.
+// Walk requests of one workgroup staged in LDS (structure of arrays) in two parts: peel-offs (origin,
+// value, tag, index, hint: a peel-off walks toward its instrument, so it carries no direction) in capp
+// entries, and the own-path and propagation walks (with their direction) in capw entries, each part with
+// its own count. A lane stages at most ni peel-offs and one other walk per pass, so capp = ni and capw = 1
+// entries per lane suffice, 28 + 40 / ni bytes per request slot in FP32 instead of 40, which leaves room
+// for more workgroups per CU. The workgroup appends them to the walk queue as one block with a single
+// global atomic.
+enum StagedPeelR { QP_OX = 0, QP_OY, QP_OZ, QP_VAL, QP_NR };
 template <typename R> struct StagedQueue {
-    R* r;
-    uint32_t* u;
-    int cap;
-    unsigned* cnt;
+    R* pr;                      // peel-offs: QP_NR reals and QU_NU words per entry
+    uint32_t* pu;
+    int capp;
+    unsigned* cntp;
+    R* wr;                      // other walks: QR_NR reals and QU_NU words per entry
+    uint32_t* wu;
+    int capw;
+    unsigned* cntw;
     __device__ __forceinline__ void push(uint32_t kind, uint32_t inst, uint32_t idx, R ox, R oy, R oz, R dx, R dy,
                                          R dz, R val, int32_t hint) const
     {
+        const bool peel = kind == WK_PEEL;   // the same for every lane of a call
         uint64_t mask = __ballot(1);
         int lane = __lane_id();
         int leader = __ffsll((unsigned long long)mask) - 1;
         unsigned base = 0;
-        if (lane == leader) base = atomicAdd(cnt, (unsigned)__popcll(mask));
+        if (lane == leader) base = atomicAdd(peel ? cntp : cntw, (unsigned)__popcll(mask));
         base = __shfl(base, leader);
         uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
         const int i = (int)(base + (unsigned)__popcll(below));
-        r[QR_OX * cap + i] = ox; r[QR_OY * cap + i] = oy; r[QR_OZ * cap + i] = oz;
-        r[QR_DX * cap + i] = dx; r[QR_DY * cap + i] = dy; r[QR_DZ * cap + i] = dz;
-        r[QR_VAL * cap + i] = val;
-        u[QU_TAG * cap + i] = kind | (inst << 2);
-        u[QU_IDX * cap + i] = idx;
-        u[QU_HINT * cap + i] = (uint32_t)hint;
+        if (peel) {
+            pr[QP_OX * capp + i] = ox; pr[QP_OY * capp + i] = oy; pr[QP_OZ * capp + i] = oz;
+            pr[QP_VAL * capp + i] = val;
+            pu[QU_TAG * capp + i] = kind | (inst << 2);
+            pu[QU_IDX * capp + i] = idx;
+            pu[QU_HINT * capp + i] = (uint32_t)hint;
+        } else {
+            wr[QR_OX * capw + i] = ox; wr[QR_OY * capw + i] = oy; wr[QR_OZ * capw + i] = oz;
+            wr[QR_DX * capw + i] = dx; wr[QR_DY * capw + i] = dy; wr[QR_DZ * capw + i] = dz;
+            wr[QR_VAL * capw + i] = val;
+            wu[QU_TAG * capw + i] = kind | (inst << 2);
+            wu[QU_IDX * capw + i] = idx;
+            wu[QU_HINT * capw + i] = (uint32_t)hint;
+        }
+    }
+    // Request j of the workgroup's block: the peel-offs first, then the other walks.
+    __device__ __forceinline__ WalkReq<R> get(unsigned j, unsigned np) const
+    {
+        WalkReq<R> w;
+        if (j < np) {
+            w.o[0] = pr[QP_OX * capp + j]; w.o[1] = pr[QP_OY * capp + j]; w.o[2] = pr[QP_OZ * capp + j];
+            w.d[0] = (R)0; w.d[1] = (R)0; w.d[2] = (R)0;
+            w.val = pr[QP_VAL * capp + j];
+            w.tag = pu[QU_TAG * capp + j]; w.idx = pu[QU_IDX * capp + j]; w.hint = pu[QU_HINT * capp + j];
+        } else {
+            const unsigned k = j - np;
+            w.o[0] = wr[QR_OX * capw + k]; w.o[1] = wr[QR_OY * capw + k]; w.o[2] = wr[QR_OZ * capw + k];
+            w.d[0] = wr[QR_DX * capw + k]; w.d[1] = wr[QR_DY * capw + k]; w.d[2] = wr[QR_DZ * capw + k];
+            w.val = wr[QR_VAL * capw + k];
+            w.tag = wu[QU_TAG * capw + k]; w.idx = wu[QU_IDX * capw + k]; w.hint = wu[QU_HINT * capw + k];
+        }
+        return w;
     }
 };
-template <typename R> __host__ __device__ constexpr size_t staged_bytes(int cap)
+// LDS of a workgroup's staging area for block lanes and ni instruments (the peel-off part first; its
+// size is a multiple of 8 bytes, so that the FP64 reals of the other part stay aligned)
+template <typename R> __host__ __device__ constexpr size_t staged_peel_bytes(int block, int ni)
 {
-    return (size_t)cap * (QR_NR * sizeof(R) + QU_NU * sizeof(uint32_t));
+    return (((size_t)block * ni * (QP_NR * sizeof(R) + QU_NU * sizeof(uint32_t))) + 7) / 8 * 8;
+}
+template <typename R> __host__ __device__ constexpr size_t staged_bytes(int block, int ni)
+{
+    return staged_peel_bytes<R>(block, ni) + (size_t)block * (QR_NR * sizeof(R) + QU_NU * sizeof(uint32_t));
 }
 
 // Peel-offs toward every instrument (MonteCarloSimulation::peeloffemission/peeloffscattering,
@@ -227,19 +270,22 @@ __device__ __forceinline__ void hg_scatter(PacketRng& rng, R gg, R& kx, R& ky, R
 }
 
 // Event pass: every slot advances its packet to its next walk (or takes a new packet). Dynamic LDS:
-// the workgroup's staged requests, blockDim.x * (a.ni + 1) entries (the most one pass can append).
+// the workgroup's staged requests, blockDim.x * a.ni peel-offs and blockDim.x other walks (the most one
+// pass can append: a lane's pass ends with the walk it requests, after at most one set of peel-offs).
 template <typename R>
 __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, WalkQueue<R> q)
 {
     __shared__ InstrDev s_instr[SG_MAX_INSTR];
-    __shared__ unsigned s_cnt;
+    __shared__ unsigned s_cntp, s_cntw;
     __shared__ unsigned long long s_base;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int cap = (int)blockDim.x * (a.ni + 1);
-    StagedQueue<R> sq{reinterpret_cast<R*>(smem), reinterpret_cast<uint32_t*>(smem + (size_t)cap * QR_NR * sizeof(R)),
-                      cap, &s_cnt};
+    const int capp = (int)blockDim.x * a.ni, capw = (int)blockDim.x;
+    unsigned char* wbase = smem + staged_peel_bytes<R>((int)blockDim.x, a.ni);
+    StagedQueue<R> sq{reinterpret_cast<R*>(smem), reinterpret_cast<uint32_t*>(smem + (size_t)capp * QP_NR * sizeof(R)),
+                      capp, &s_cntp, reinterpret_cast<R*>(wbase),
+                      reinterpret_cast<uint32_t*>(wbase + (size_t)capw * QR_NR * sizeof(R)), capw, &s_cntw};
     if (threadIdx.x < SG_MAX_INSTR) s_instr[threadIdx.x] = a.scene->instr[threadIdx.x];
-    if (threadIdx.x == 0) s_cnt = 0;
+    if (threadIdx.x == 0) { s_cntp = 0; s_cntw = 0; }
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // every lane stays to the end: the counts below are summed over the whole wave
@@ -347,16 +393,12 @@ __global__ void __launch_bounds__(256) k_events(const McArgs a, Slots<R> sl, Wal
     if (i < sl.n) sl.fu(SU_STATE, i) = state;
     // append the workgroup's requests to the walk queue
     __syncthreads();
-    if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(q.tail, (unsigned long long)s_cnt) : 0ull;
+    const unsigned np = s_cntp, n = s_cntp + s_cntw;
+    if (threadIdx.x == 0) s_base = n ? atomicAdd(q.tail, (unsigned long long)n) : 0ull;
     __syncthreads();
-    const unsigned n = s_cnt;
     const uint64_t base = s_base;
     for (unsigned j = threadIdx.x; j < n; j += blockDim.x) {
-        WalkReq<R> w;
-        w.o[0] = sq.r[QR_OX * cap + j]; w.o[1] = sq.r[QR_OY * cap + j]; w.o[2] = sq.r[QR_OZ * cap + j];
-        w.d[0] = sq.r[QR_DX * cap + j]; w.d[1] = sq.r[QR_DY * cap + j]; w.d[2] = sq.r[QR_DZ * cap + j];
-        w.val = sq.r[QR_VAL * cap + j];
-        w.tag = sq.u[QU_TAG * cap + j]; w.idx = sq.u[QU_IDX * cap + j]; w.hint = sq.u[QU_HINT * cap + j];
+        const WalkReq<R> w = sq.get(j, np);
         q.req[base + j] = w;
         if (q.sorted) {
             const uint32_t k = walk_key<R>(a, s_instr, w.tag, w.idx, w.o[0], w.o[1], w.o[2]);
